@@ -1,0 +1,275 @@
+"""ctypes binding of the oracle (CPU restatement of ParFlow's Richards
+Newton-Krylov path).  TEST INFRASTRUCTURE ONLY: used by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker,
+never by the product library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+MAXI = 8
+
+
+class BCSpecC(C.Structure):
+    _fields_ = [("type", C.c_int), ("n_intervals", C.c_int), ("repeat", C.c_int),
+                ("lengths", C.c_int * MAXI), ("values", C.c_double * MAXI),
+                ("z_ref", C.c_double)]
+
+
+_dp = C.POINTER(C.c_double)
+
+
+class ProblemC(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
+                ("rho", C.c_double), ("mu", C.c_double), ("gravity", C.c_double),
+                ("kx", _dp), ("ky", _dp), ("kz", _dp), ("poro", _dp), ("ss", _dp),
+                ("alpha", _dp), ("nvg", _dp), ("sres", _dp), ("ssat", _dp),
+                ("src", _dp),
+                ("slope_x", _dp), ("slope_y", _dp), ("mann", _dp),
+                ("ov_eps", C.c_double),
+                ("bc", BCSpecC * 6),
+                ("base_unit", C.c_double), ("start_time", C.c_double),
+                ("bc_val", _dp * 6)]
+
+
+class OptsC(C.Structure):
+    _fields_ = [("residual_tol", C.c_double), ("step_tol", C.c_double),
+                ("max_iter", C.c_int), ("krylov_dim", C.c_int),
+                ("max_restarts", C.c_int), ("eta_choice", C.c_int),
+                ("eta_value", C.c_double), ("eta_alpha", C.c_double),
+                ("eta_gamma", C.c_double), ("use_jacobian", C.c_int),
+                ("deriv_eps", C.c_double), ("globalization", C.c_int),
+                ("precond", C.c_int), ("mg_pre", C.c_int), ("mg_post", C.c_int),
+                ("mg_symmetric", C.c_int), ("mg_max_levels", C.c_int)]
+
+
+class KinStatsC(C.Structure):
+    _fields_ = [("ret", C.c_int), ("nni", C.c_int), ("nli", C.c_int),
+                ("nfe", C.c_int), ("npe", C.c_int), ("nps", C.c_int),
+                ("nbcf", C.c_int), ("nbktrk", C.c_int), ("ncfl", C.c_int),
+                ("fnorm", C.c_double)]
+
+
+class RunLogC(C.Structure):
+    _fields_ = [("nsteps_done", C.c_int), ("newton", C.c_int * 4096),
+                ("gmres", C.c_int * 4096), ("fevals", C.c_int * 4096),
+                ("dt_used", C.c_double * 4096)]
+
+
+def build(force=False):
+    if force or not os.path.exists(_LIB):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "liboracle.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _lib.pfo_kinsol.restype = C.c_int
+        _lib.pfo_advance.restype = C.c_int
+        for f in ("pfo_dot", "pfo_l2norm", "pfo_maxnorm", "pfo_l1norm"):
+            getattr(_lib, f).restype = C.c_double
+        _lib.pfo_mg_create.restype = C.c_void_p
+        _lib.pfo_mg_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, C.c_double, C.c_int, C.c_int,
+                                       C.c_int, C.c_int]
+        _lib.pfo_mg_setup.argtypes = [C.c_void_p, C.POINTER(_dp)]
+        _lib.pfo_mg_apply.argtypes = [C.c_void_p, _dp, _dp]
+        _lib.pfo_mg_destroy.argtypes = [C.c_void_p]
+        _lib.pfo_mg_num_levels.argtypes = [C.c_void_p]
+        _lib.pfo_mg_level_dims.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)]
+    return _lib
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+class OracleProblem:
+    """Holds a ProblemC view of a parflow_amd.problem.Problem."""
+
+    def __init__(self, prob, src=None):
+        self.prob = prob
+        self.keep = []
+        c = ProblemC()
+        c.nx, c.ny, c.nz = prob.nx, prob.ny, prob.nz
+        c.dx, c.dy, c.dz = prob.dx, prob.dy, prob.dz
+        c.x0, c.y0, c.z0 = prob.x0, prob.y0, prob.z0
+        c.rho, c.mu, c.gravity = prob.rho, prob.mu, prob.gravity
+        for name in ("kx", "ky", "kz", "poro", "ss", "alpha", "nvg", "sres",
+                     "ssat", "slope_x", "slope_y", "mann"):
+            a = np.ascontiguousarray(getattr(prob, "g_" + name))
+            self.keep.append(a)
+            setattr(c, name, _ptr(a))
+        if src is not None:
+            s = np.ascontiguousarray(src, dtype=np.float64)
+            self.keep.append(s)
+            c.src = _ptr(s)
+        c.ov_eps = prob.ov_eps
+        for f, b in enumerate(prob.bc):
+            cb = c.bc[f]
+            cb.type = b.type
+            cb.n_intervals = len(b.values)
+            cb.repeat = b.repeat
+            for i, (L, v) in enumerate(zip(b.lengths, b.values)):
+                cb.lengths[i] = int(L)
+                cb.values[i] = float(v)
+            cb.z_ref = b.z_ref
+            buf = np.zeros(prob.face_ncells(f))
+            self.keep.append(buf)
+            c.bc_val[f] = _ptr(buf)
+        c.base_unit = prob.base_unit
+        c.start_time = prob.start_time
+        self.c = c
+        self.bc_bufs = [k for k in self.keep[-6:]]
+
+    @property
+    def ref(self):
+        return C.byref(self.c)
+
+    def set_time(self, t):
+        lib().pfo_bc_pressure(self.ref, C.c_double(t))
+
+
+def opts_c(o):
+    pre, post, sym = o.mg_params()
+    c = OptsC()
+    c.residual_tol = o.residual_tol
+    c.step_tol = o.step_tol
+    c.max_iter = o.max_iter
+    c.krylov_dim = o.krylov_dim
+    c.max_restarts = o.max_restarts
+    c.eta_choice = o.eta_choice
+    c.eta_value = o.eta_value
+    c.eta_alpha = o.eta_alpha
+    c.eta_gamma = o.eta_gamma
+    c.use_jacobian = int(o.use_jacobian)
+    c.deriv_eps = o.deriv_eps
+    c.globalization = o.globalization
+    c.precond = 0 if o.precond == "NoPC" else 1
+    c.mg_pre, c.mg_post, c.mg_symmetric = pre, post, sym
+    c.mg_max_levels = o.max_levels
+    return c
+
+
+# ---------------------------------------------------------------------------
+def saturation(op, p, fcn=0):
+    out = np.zeros_like(p)
+    lib().pfo_saturation(op.ref, _ptr(p), _ptr(out), C.c_int(fcn))
+    return out
+
+
+def relperm(op, p, fcn=0):
+    out = np.zeros_like(p)
+    lib().pfo_relperm(op.ref, _ptr(p), _ptr(out), C.c_int(fcn))
+    return out
+
+
+def nl_function_eval(op, p, p_old, s_old, dt, t):
+    op.set_time(t)
+    F = np.zeros_like(p)
+    lib().pfo_nl_function_eval(op.ref, _ptr(p), _ptr(p_old), _ptr(s_old),
+                               C.c_double(dt), _ptr(F))
+    return F
+
+
+def jacobian_eval(op, p, dt, t, symm):
+    op.set_time(t)
+    J = [np.zeros_like(p) for _ in range(7)]
+    arr = (_dp * 7)(*[_ptr(a) for a in J])
+    lib().pfo_jacobian_eval(op.ref, _ptr(p), C.c_double(dt), C.c_int(symm), arr)
+    return J
+
+
+def matvec(op, J, symm, x):
+    y = np.zeros_like(x)
+    arr = (_dp * 7)(*[_ptr(a) for a in J])
+    lib().pfo_matvec(op.ref, arr, C.c_int(symm), _ptr(x), _ptr(y))
+    return y
+
+
+def overland_kin(op, p):
+    n = (op.prob.nx + 2) * (op.prob.ny + 2)
+    qx = np.zeros(n)
+    qy = np.zeros(n)
+    lib().pfo_overland_kin(op.ref, _ptr(p), _ptr(qx), _ptr(qy))
+    return qx, qy
+
+
+class MG:
+    def __init__(self, prob, pre, post, symmetric, max_levels=100):
+        self.h = lib().pfo_mg_create(prob.nx, prob.ny, prob.nz, prob.dx,
+                                     prob.dy, prob.dz, max_levels, pre, post,
+                                     symmetric)
+        self.shape = prob.shape
+
+    def levels(self):
+        out = []
+        for l in range(lib().pfo_mg_num_levels(self.h)):
+            d = (C.c_int * 3)()
+            dr = C.c_int()
+            lib().pfo_mg_level_dims(self.h, l, d, C.byref(dr))
+            out.append((tuple(d), dr.value))
+        return out
+
+    def setup(self, J):
+        arr = (_dp * 7)(*[_ptr(a) for a in J])
+        lib().pfo_mg_setup(self.h, arr)
+
+    def apply(self, b):
+        x = np.zeros(self.shape)
+        lib().pfo_mg_apply(self.h, _ptr(np.ascontiguousarray(b)), _ptr(x))
+        return x
+
+    def __del__(self):
+        try:
+            lib().pfo_mg_destroy(self.h)
+        except Exception:
+            pass
+
+
+def kinsol(op, opts, p, p_old, s_old, dt, t):
+    st = KinStatsC()
+    oc = opts_c(opts)
+    r = lib().pfo_kinsol(op.ref, C.byref(oc), _ptr(p), _ptr(p_old), _ptr(s_old),
+                         C.c_double(dt), C.c_double(t), C.byref(st))
+    return r, st
+
+
+def advance(op, opts, p0, t0, dt, nsteps, keep_history=True):
+    """Returns (status, p_hist, s_hist, log) with histories of shape
+    (nsteps+1, nz+2, ny+2, nx+2)."""
+    prob = op.prob
+    p = np.ascontiguousarray(p0.copy())
+    s = np.zeros_like(p)
+    ph = np.zeros((nsteps + 1,) + prob.shape) if keep_history else None
+    sh = np.zeros((nsteps + 1,) + prob.shape) if keep_history else None
+    log = RunLogC()
+    oc = opts_c(opts)
+    r = lib().pfo_advance(op.ref, C.byref(oc), _ptr(p), _ptr(s), C.c_double(t0),
+                          C.c_double(dt), C.c_int(nsteps),
+                          C.c_int(opts.max_convergence_failures),
+                          _ptr(ph) if keep_history else None,
+                          _ptr(sh) if keep_history else None, C.byref(log))
+    return r, (ph if keep_history else p), (sh if keep_history else s), log
+
+
+def storage(op, p, s):
+    a = C.c_double()
+    b = C.c_double()
+    lib().pfo_storage(op.ref, _ptr(p), _ptr(s), C.byref(a), C.byref(b))
+    return a.value, b.value

@@ -1,0 +1,157 @@
+/*
+ * pfo.h -- CPU restatement ("oracle") of ParFlow's Richards-equation
+ * Newton-Krylov path.  TEST INFRASTRUCTURE ONLY: this code is the checker
+ * that the HIP implementation in parflow_amd/ is compared against.  It is
+ * never linked into, imported by, or called from the product library.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load liboracle.so.
+ *
+ * Data layout follows ParFlow's Subvector layout (pfsimulator/parflow_lib/
+ * vector.h:133-137): one ghost layer, x fastest,
+ *   ip = (i+1) + (j+1)*sy + (k+1)*sz,  sy = nx+2, sz = (nx+2)*(ny+2)
+ * for interior cell (i,j,k), 0 <= i < nx etc.  2-D surface arrays (slopes,
+ * Manning, overland fluxes) use io = (i+1) + (j+1)*sy.
+ *
+ * Scope (see each function's citation):
+ *   box domains only (every cell active), constant density / viscosity,
+ *   van Genuchten closures, no terrain-following grid, no variable dz,
+ *   no flow barriers.
+ */
+#ifndef PFO_H
+#define PFO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Boundary-condition types, a subset of PL/problem_bc.h:45-50 */
+#define PFO_BC_FLUX       0   /* FluxConst                        */
+#define PFO_BC_DIR_EQUIL  1   /* DirEquilRefPatch (hydrostatic)   */
+#define PFO_BC_OVERLAND   2   /* OverlandKinematic (top face only)*/
+
+/* Face numbering: 0 x-lower (Left), 1 x-upper (Right), 2 y-lower (Down),
+ * 3 y-upper (Up), 4 z-lower (Back), 5 z-upper (Front). */
+
+#define PFO_MAX_INTERVALS 8
+
+typedef struct {
+  int type;
+  int n_intervals;                      /* time-cycle intervals             */
+  int repeat;                           /* -1 = forever                     */
+  int lengths[PFO_MAX_INTERVALS];       /* in base time units               */
+  double values[PFO_MAX_INTERVALS];     /* flux or reference pressure       */
+  double z_ref;                         /* DirEquilRefPatch elevation       */
+} pfo_bc_spec;
+
+typedef struct {
+  int nx, ny, nz;
+  double dx, dy, dz;
+  double x0, y0, z0;                    /* lower corner of the grid         */
+  double rho, mu, gravity;              /* constant density, viscosity, g   */
+  /* per-cell fields, ghosted (nx+2)(ny+2)(nz+2) */
+  const double *kx, *ky, *kz, *poro, *ss;
+  const double *alpha, *nvg, *sres, *ssat;
+  const double *src;                    /* source + evap_trans, may be NULL */
+  /* surface fields, ghosted (nx+2)(ny+2); only used with an overland face */
+  const double *slope_x, *slope_y, *mann;
+  double ov_eps;                        /* Solver.OverlandKinematic.Epsilon */
+  /* boundary conditions */
+  pfo_bc_spec bc[6];
+  double base_unit, start_time;         /* TimingInfo.BaseUnit/StartTime    */
+  /* per-face BC values at the current time, filled by pfo_bc_pressure */
+  double *bc_val[6];
+} pfo_problem;
+
+/* CALCFCN / CALCDER flags (PL/problem_saturation.c) */
+#define PFO_CALCFCN 0
+#define PFO_CALCDER 1
+
+typedef struct {
+  double residual_tol, step_tol;
+  int max_iter;
+  int krylov_dim, max_restarts;
+  int eta_choice;                       /* 0 EtaConstant 1 Walker1 2 Walker2 */
+  double eta_value, eta_alpha, eta_gamma;
+  int use_jacobian;                     /* analytic J.v instead of DQ       */
+  double deriv_eps;                     /* DerivativeEpsilon                */
+  int globalization;                    /* 0 InexactNewton, 1 LineSearch    */
+  int precond;                          /* 0 none, 1 semicoarsening MG      */
+  int mg_pre, mg_post;                  /* smoother iterations per level    */
+  int mg_symmetric;                     /* R-B-R (1) or R-B (0) sweeps      */
+  int mg_max_levels;
+} pfo_solver_opts;
+
+typedef struct {
+  int ret;                              /* KINSOL return code               */
+  int nni, nli, nfe, npe, nps, nbcf, nbktrk, ncfl;
+  double fnorm;
+} pfo_kin_stats;
+
+/* ---- closures ---------------------------------------------------------- */
+void pfo_saturation(const pfo_problem *pb, const double *p, double *out,
+                    int fcn);
+void pfo_relperm(const pfo_problem *pb, const double *p, double *out, int fcn);
+
+/* ---- boundary values --------------------------------------------------- */
+int  pfo_interval_number(const pfo_problem *pb, const pfo_bc_spec *bc,
+                         double time);
+void pfo_bc_pressure(pfo_problem *pb, double time);
+void pfo_hydrostatic_ic(const pfo_problem *pb, double value, double z_ref,
+                        double *p);
+
+/* ---- residual, overland, Jacobian, matvec ------------------------------ */
+void pfo_overland_kin(const pfo_problem *pb, const double *p, double *qx,
+                      double *qy);
+void pfo_overland_kin_der(const pfo_problem *pb, const double *p,
+                          double *ke_der, double *kw_der, double *kn_der,
+                          double *ks_der);
+void pfo_nl_function_eval(const pfo_problem *pb, const double *p,
+                          const double *p_old, const double *s_old,
+                          double dt, double *F);
+void pfo_jacobian_eval(const pfo_problem *pb, const double *p, double dt,
+                       int symm_part, double **J);
+void pfo_matvec(const pfo_problem *pb, double **J, int symm, const double *x,
+                double *y);
+
+/* ---- multigrid preconditioner (MGSemi-structured V-cycle) --------------- */
+typedef struct pfo_mg pfo_mg;
+pfo_mg *pfo_mg_create(int nx, int ny, int nz, double dx, double dy, double dz,
+                      int max_levels, int pre, int post, int symmetric);
+void    pfo_mg_destroy(pfo_mg *mg);
+int     pfo_mg_num_levels(const pfo_mg *mg);
+void    pfo_mg_level_dims(const pfo_mg *mg, int l, int *dims, int *dir);
+void    pfo_mg_setup(pfo_mg *mg, double **Asym);
+void    pfo_mg_apply(pfo_mg *mg, const double *b, double *x);
+
+/* ---- KINSOL inexact Newton + SPGMR ------------------------------------- */
+int pfo_kinsol(pfo_problem *pb, const pfo_solver_opts *opts, double *p,
+               const double *p_old, const double *s_old, double dt,
+               double time, pfo_kin_stats *st);
+
+/* ---- time loop (AdvanceRichards step contract) ------------------------- */
+typedef struct {
+  int nsteps_done;
+  int newton[4096], gmres[4096], fevals[4096];
+  double dt_used[4096];
+} pfo_run_log;
+
+int pfo_advance(pfo_problem *pb, const pfo_solver_opts *opts, double *p,
+                double *s, double t0, double dt, int nsteps,
+                int max_conv_failures, double *p_hist, double *s_hist,
+                pfo_run_log *log);
+
+/* ---- vector utilities (PL/vector_utilities.c restated on interiors) ---- */
+double pfo_dot(const pfo_problem *pb, const double *x, const double *y);
+double pfo_l2norm(const pfo_problem *pb, const double *x);
+double pfo_maxnorm(const pfo_problem *pb, const double *x);
+double pfo_l1norm(const pfo_problem *pb, const double *x);
+
+/* ---- water balance (pftools hydrology.py:136-205 restated) -------------- */
+void pfo_storage(const pfo_problem *pb, const double *p, const double *s,
+                 double *subsurface, double *surface);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

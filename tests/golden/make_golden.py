@@ -1,0 +1,51 @@
+"""Extracts golden vectors from the reference's own regression outputs
+(test/correct_output/*.pfb, data files written by the reference CI) into
+small .npz fixtures.  Run in the development container only, where the
+reference tree is mounted:
+
+    python tests/golden/make_golden.py /root/reference
+
+Fixtures written:
+  default_richards.npz     -- test/python/default_richards.py (18x15x8,
+                              DirEquilRefPatch + FluxConst, 50 steps):
+                              pressure and saturation at selected steps
+  tiltedv_overlandkin.npz  -- test/python/overland_tiltedV_KWE.py run
+                              "TiltedV_OverlandKin" (5x5x1, OverlandKinematic
+                              rain/recession cycle): all 21 dumps (one every 2 steps)
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.abspath(os.path.join(HERE, "..", "..")))
+from parflow_amd.pfb import read_pfb  # noqa: E402
+
+
+def main(ref):
+    co = os.path.join(ref, "test", "correct_output")
+    steps = [0, 1, 2, 3, 5, 10, 20, 30, 40, 50]
+    press, satur = [], []
+    for s in steps:
+        _, p = read_pfb(os.path.join(co, f"default_richards.out.press.{s:05d}.pfb"))
+        _, q = read_pfb(os.path.join(co, f"default_richards.out.satur.{s:05d}.pfb"))
+        press.append(p)
+        satur.append(q)
+    np.savez_compressed(os.path.join(HERE, "default_richards.npz"),
+                        steps=np.array(steps), press=np.array(press),
+                        satur=np.array(satur))
+    d = co
+    press, satur = [], []
+    for s in range(21):
+        _, p = read_pfb(os.path.join(d, f"TiltedV_OverlandKin.out.press.{s:05d}.pfb"))
+        _, q = read_pfb(os.path.join(d, f"TiltedV_OverlandKin.out.satur.{s:05d}.pfb"))
+        press.append(p)
+        satur.append(q)
+    np.savez_compressed(os.path.join(HERE, "tiltedv_overlandkin.npz"),
+                        dumps=np.arange(21), press=np.array(press),
+                        satur=np.array(satur))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
