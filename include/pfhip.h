@@ -1,0 +1,200 @@
+/*
+ * pfhip.h -- C-ABI of the MI355X (gfx950) Richards-equation Newton-Krylov
+ * path.  Plain C types only: device pointers, sizes, an opaque stream
+ * (hipStream_t passed as void*; NULL = the default stream) and int error
+ * codes.  No entry point exits the process; every one returns PFHIP_OK (0)
+ * or a negative error code, with a message from pfhip_last_error().
+ *
+ * Data layout is ParFlow's Subvector layout (pfsimulator/parflow_lib/
+ * vector.h:133-137): cell-centred double arrays with one ghost layer, x
+ * fastest, (nx+2)*(ny+2)*(nz+2) entries; cell (i,j,k) of the interior is at
+ * (i+1) + (j+1)*(nx+2) + (k+1)*(nx+2)*(ny+2).  Surface (2-D) arrays are
+ * (nx+2)*(ny+2).  So a single-subgrid ParFlow Vector's SubvectorData()
+ * pointer, once resident on the device, can be passed as is.
+ *
+ * Stencil matrices are 7 separate ghosted coefficient arrays in ParFlow's
+ * stencil order C, W, E, S, N, L, U (PL/richards_jacobian_eval.c:126-132).
+ * A symmetric matrix (PCMatrixType PFSymmetric) stores only C, E, N, U; its
+ * W/S/L couplings are the west/south/lower neighbour's E/N/U
+ * (PL/matrix.c:258-290, 470-490).
+ */
+#ifndef PFHIP_H
+#define PFHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFHIP_OK             0
+#define PFHIP_ERR_HIP       -1   /* a HIP runtime call failed            */
+#define PFHIP_ERR_ARG       -2   /* invalid argument / shape             */
+#define PFHIP_ERR_NOMEM     -3   /* device allocation failed             */
+#define PFHIP_ERR_SOLVER    -4   /* nonlinear solve did not converge     */
+
+#define PFHIP_CALCFCN 0
+#define PFHIP_CALCDER 1
+
+#define PFHIP_BC_FLUX       0    /* FluxConst                            */
+#define PFHIP_BC_DIR_EQUIL  1    /* DirEquilRefPatch                     */
+#define PFHIP_BC_OVERLAND   2    /* OverlandKinematic (z-upper face)     */
+#define PFHIP_MAX_INTERVALS 8
+
+/* One box face: type, time cycle (PL/time_cycle_data.c:152-197) and, for
+ * DirEquilRefPatch, the reference elevation (PL/problem_bc_pressure.c:143). */
+typedef struct {
+  int type;
+  int n_intervals;
+  int repeat;
+  int lengths[PFHIP_MAX_INTERVALS];
+  double values[PFHIP_MAX_INTERVALS];
+  double z_ref;
+} pfhip_bc_spec;
+
+/* ProblemData for a box domain (every cell active).  Face order: x-lower,
+ * x-upper, y-lower, y-upper, z-lower, z-upper. */
+typedef struct {
+  int nx, ny, nz;
+  double dx, dy, dz;
+  double x0, y0, z0;
+  double rho, mu, gravity;           /* constant density / viscosity        */
+  const double *kx, *ky, *kz;        /* device, ghosted; ghost entries 0    */
+  const double *poro, *ss;           /* device, ghosted                     */
+  const double *alpha, *nvg;         /* device, ghosted van Genuchten       */
+  const double *sres, *ssat;         /* device, ghosted                     */
+  const double *src;                 /* device, ghosted, or NULL            */
+  const double *slope_x, *slope_y;   /* device, surface, or NULL            */
+  const double *mann;                /* device, surface, or NULL            */
+  double ov_eps;                     /* Solver.OverlandKinematic.Epsilon    */
+  pfhip_bc_spec bc[6];
+  double base_unit, start_time;      /* TimingInfo.BaseUnit / StartTime     */
+} pfhip_problem_desc;
+
+typedef struct pfhip_problem pfhip_problem;
+typedef struct pfhip_mg pfhip_mg;
+typedef struct pfhip_solver pfhip_solver;
+
+/* ---- library / memory ------------------------------------------------- */
+const char *pfhip_last_error(void);
+int  pfhip_device_count(int *count);
+int  pfhip_malloc(void **ptr, long bytes);
+int  pfhip_free(void *ptr);
+int  pfhip_memcpy_h2d(void *dst, const void *src, long bytes);
+int  pfhip_memcpy_d2h(void *dst, const void *src, long bytes);
+int  pfhip_memset(void *dst, int value, long bytes);
+int  pfhip_synchronize(void);
+
+/* ---- problem ---------------------------------------------------------- */
+/* Replaces the ProblemData handed to the Richards modules
+ * (PL/problem.c:420-460).  The problem keeps the device pointers of desc
+ * (it does not copy the fields) and owns its scratch and BC arrays. */
+int  pfhip_problem_create(const pfhip_problem_desc *desc, pfhip_problem **out);
+int  pfhip_problem_destroy(pfhip_problem *pb);
+/* BCPressure (PL/problem_bc_pressure.c:60): evaluates the six faces' values
+ * at `time` on the host and uploads them. */
+int  pfhip_bc_pressure(pfhip_problem *pb, double time, void *stream);
+
+/* ---- physics modules -------------------------------------------------- */
+/* SaturationInvoke (PL/parflow_proto.h:1033), van Genuchten curve
+ * (PL/problem_saturation.c:221-416); fcn = PFHIP_CALCFCN or _CALCDER. */
+int  pfhip_saturation(pfhip_problem *pb, const double *p, double *out,
+                      int fcn, void *stream);
+/* PhaseRelPermInvoke (PL/parflow_proto.h:976), van Genuchten
+ * (PL/problem_phase_rel_perm.c:567-1000). */
+int  pfhip_phase_rel_perm(pfhip_problem *pb, const double *p, double *out,
+                          int fcn, void *stream);
+/* NlFunctionEvalInvoke (PL/parflow_proto.h:498; PL/nl_function_eval.c:167).
+ * Uses the BC values set by the last pfhip_bc_pressure call. */
+int  pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double *F,
+                            const double *p_old, const double *s_old,
+                            double dt, void *stream);
+/* RichardsJacobianEvalInvoke (PL/parflow_proto.h:1135;
+ * PL/richards_jacobian_eval.c:217).  J[7] device arrays; with symm_part
+ * only J[0], J[2], J[4], J[6] are written (J[1], J[3], J[5] may be NULL). */
+int  pfhip_richards_jacobian_eval(pfhip_problem *pb, const double *p,
+                                  double dt, int symm_part, double *const *J,
+                                  void *stream);
+/* Matvec (PL/parflow_proto.h:383; PL/matvec.c:40): y = A x. */
+int  pfhip_matvec(pfhip_problem *pb, const double *const *A, int symm,
+                  const double *x, double *y, void *stream);
+/* OverlandFlowEvalKinInvoke (PL/parflow_proto.h:900;
+ * PL/overlandflow_eval_Kin.c:60).  CALCFCN: out0 = qx, out1 = qy (face
+ * fluxes, surface arrays).  CALCDER: out0..3 = KE, KW, KN, KS derivatives. */
+int  pfhip_overland_flow_eval_kin(pfhip_problem *pb, const double *p,
+                                  double *out0, double *out1, double *out2,
+                                  double *out3, int fcn, void *stream);
+
+/* ---- N_Vector operations (PL/n_vector.h -> PL/vector_utilities.c) ---- */
+/* All act on the interior cells of ghosted vectors of problem pb. */
+int  pfhip_vec_linear_sum(pfhip_problem *pb, double a, const double *x,
+                          double b, const double *y, double *z, void *stream);
+int  pfhip_vec_const(pfhip_problem *pb, double c, double *z, void *stream);
+int  pfhip_vec_dot(pfhip_problem *pb, const double *x, const double *y,
+                   double *result, void *stream);
+int  pfhip_vec_max_norm(pfhip_problem *pb, const double *x, double *result,
+                        void *stream);
+int  pfhip_vec_l1_norm(pfhip_problem *pb, const double *x, double *result,
+                       void *stream);
+
+/* ---- multigrid preconditioner (PrecondInvoke, PL/parflow_proto.h:418) - */
+/* Semicoarsening V-cycle with the level structure, operator-induced
+ * transfers and Galerkin-collapsed coarse operators of MGSemi
+ * (PL/mg_semi.c:93-760), red/black Gauss-Seidel smoothing
+ * (PL/rb_GS_point.c:56).  precision = 64 (FP64) or 32 (FP32 levels). */
+int  pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int post,
+                     int symmetric, int precision, pfhip_mg **out);
+int  pfhip_mg_destroy(pfhip_mg *mg);
+int  pfhip_mg_num_levels(pfhip_mg *mg, int *levels);
+/* PrecondInitInstanceXtraInvoke (PL/parflow_proto.h:419): coarse levels
+ * from the symmetric PC matrix (C, E, N, U arrays). */
+int  pfhip_mg_setup(pfhip_mg *mg, const double *const *Asym, void *stream);
+/* One V-cycle, zero initial guess: x = M^{-1} b. */
+int  pfhip_mg_apply(pfhip_mg *mg, const double *b, double *x, void *stream);
+
+/* ---- nonlinear solver (NonlinSolverInvoke, PL/parflow_proto.h:301) ---- */
+typedef struct {
+  double residual_tol, step_tol;     /* Solver.Nonlinear.ResidualTol/StepTol */
+  int max_iter;                      /* Solver.Nonlinear.MaxIter             */
+  int krylov_dim, max_restarts;      /* Solver.Linear.KrylovDimension/...    */
+  int eta_choice;                    /* 0 EtaConstant, 1 Walker1, 2 Walker2  */
+  double eta_value, eta_alpha, eta_gamma;
+  int use_jacobian;                  /* analytic J.v (KINSolMatVec)          */
+  double deriv_eps;                  /* DerivativeEpsilon                    */
+  int globalization;                 /* 0 InexactNewton, 1 LineSearch        */
+  int precond;                       /* 0 none, 1 multigrid                  */
+  int mg_pre, mg_post, mg_symmetric, mg_max_levels;
+  int mg_precision;                  /* 64 or 32                             */
+  int krylov;                        /* 0 SPGMR + modified Gram-Schmidt (the
+                                        reference's), 1 flexible GMRES with
+                                        block classical Gram-Schmidt (CGS2)  */
+} pfhip_solver_opts;
+
+typedef struct {
+  int ret;                           /* KINSOL return code (KS/kinsol.h:469) */
+  int nni, nli, nfe, npe, nps, nbcf, nbktrk, ncfl;
+  double fnorm;
+} pfhip_kin_stats;
+
+int  pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
+                         pfhip_solver **out);
+int  pfhip_solver_destroy(pfhip_solver *s);
+/* KinsolNonlinSolver (PL/kinsol_nonlin_solver.c:297): Newton solve of one
+ * time step in place on p.  Returns PFHIP_OK when the reference would treat
+ * the step as converged (KINSOL_SUCCESS / KINSOL_INITIAL_GUESS_OK),
+ * PFHIP_ERR_SOLVER otherwise; stats are filled either way. */
+int  pfhip_kinsol_nonlin_solver(pfhip_solver *s, double *p,
+                                const double *p_old, const double *s_old,
+                                double dt, double time, pfhip_kin_stats *st);
+
+/* AdvanceRichards per-step contract (PL/solver_richards.c:2970-3400,
+ * 3661-3669): nsteps steps of size dt from t0 with dt halving on failure.
+ * p, s are device arrays (in/out).  newton/gmres (host, may be NULL) get
+ * per-step iteration counts. */
+int  pfhip_advance_richards(pfhip_solver *s, double *p, double *sat,
+                            double t0, double dt, int nsteps,
+                            int max_conv_failures, int *newton, int *gmres,
+                            int *fevals, double *dt_used, int *steps_done);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

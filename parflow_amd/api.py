@@ -1,0 +1,445 @@
+"""Python binding of libpfhip's C-ABI (include/pfhip.h).
+
+Mirrors the reference's module interface names for the Richards path:
+``NlFunctionEval``, ``RichardsJacobianEval``, ``Saturation``,
+``PhaseRelPerm``, ``Matvec``, ``OverlandFlowEvalKin``, the multigrid
+``Precond`` and ``KinsolNonlinSolver`` / ``AdvanceRichards``.  Device
+memory is allocated through the library (hipMalloc); numpy arrays are the
+host side.  There is no CPU fallback: every call runs the HIP library and
+raises if it is missing or fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .problem import Problem, SolverOptions
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpfhip.so")
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+MAXI = 8
+
+
+class PFHIPError(RuntimeError):
+    pass
+
+
+class BCSpecC(C.Structure):
+    _fields_ = [("type", C.c_int), ("n_intervals", C.c_int), ("repeat", C.c_int),
+                ("lengths", C.c_int * MAXI), ("values", C.c_double * MAXI),
+                ("z_ref", C.c_double)]
+
+
+class ProblemDescC(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
+                ("rho", C.c_double), ("mu", C.c_double), ("gravity", C.c_double),
+                ("kx", C.c_void_p), ("ky", C.c_void_p), ("kz", C.c_void_p),
+                ("poro", C.c_void_p), ("ss", C.c_void_p),
+                ("alpha", C.c_void_p), ("nvg", C.c_void_p),
+                ("sres", C.c_void_p), ("ssat", C.c_void_p),
+                ("src", C.c_void_p),
+                ("slope_x", C.c_void_p), ("slope_y", C.c_void_p), ("mann", C.c_void_p),
+                ("ov_eps", C.c_double),
+                ("bc", BCSpecC * 6),
+                ("base_unit", C.c_double), ("start_time", C.c_double)]
+
+
+class SolverOptsC(C.Structure):
+    _fields_ = [("residual_tol", C.c_double), ("step_tol", C.c_double),
+                ("max_iter", C.c_int), ("krylov_dim", C.c_int),
+                ("max_restarts", C.c_int), ("eta_choice", C.c_int),
+                ("eta_value", C.c_double), ("eta_alpha", C.c_double),
+                ("eta_gamma", C.c_double), ("use_jacobian", C.c_int),
+                ("deriv_eps", C.c_double), ("globalization", C.c_int),
+                ("precond", C.c_int), ("mg_pre", C.c_int), ("mg_post", C.c_int),
+                ("mg_symmetric", C.c_int), ("mg_max_levels", C.c_int),
+                ("mg_precision", C.c_int), ("krylov", C.c_int)]
+
+
+class KinStatsC(C.Structure):
+    _fields_ = [("ret", C.c_int), ("nni", C.c_int), ("nli", C.c_int),
+                ("nfe", C.c_int), ("npe", C.c_int), ("nps", C.c_int),
+                ("nbcf", C.c_int), ("nbktrk", C.c_int), ("ncfl", C.c_int),
+                ("fnorm", C.c_double)]
+
+
+# exported symbols of include/pfhip.h
+EXPORTS = [
+    "pfhip_last_error", "pfhip_device_count", "pfhip_malloc", "pfhip_free",
+    "pfhip_memcpy_h2d", "pfhip_memcpy_d2h", "pfhip_memset", "pfhip_synchronize",
+    "pfhip_problem_create", "pfhip_problem_destroy", "pfhip_bc_pressure",
+    "pfhip_saturation", "pfhip_phase_rel_perm", "pfhip_nl_function_eval",
+    "pfhip_richards_jacobian_eval", "pfhip_matvec", "pfhip_overland_flow_eval_kin",
+    "pfhip_vec_linear_sum", "pfhip_vec_const", "pfhip_vec_dot", "pfhip_vec_max_norm",
+    "pfhip_vec_l1_norm", "pfhip_mg_create", "pfhip_mg_destroy", "pfhip_mg_num_levels",
+    "pfhip_mg_setup", "pfhip_mg_apply", "pfhip_solver_create", "pfhip_solver_destroy",
+    "pfhip_kinsol_nonlin_solver", "pfhip_advance_richards",
+]
+
+
+def lib():
+    """Loads libpfhip.so (built in-tree by parflow_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PFHIPError(f"{LIB_PATH} not built: run python -m parflow_amd.build")
+        L = C.CDLL(LIB_PATH)
+        L.pfhip_last_error.restype = C.c_char_p
+        vp = C.c_void_p
+        L.pfhip_malloc.argtypes = [C.POINTER(vp), C.c_long]
+        L.pfhip_free.argtypes = [vp]
+        L.pfhip_memcpy_h2d.argtypes = [vp, vp, C.c_long]
+        L.pfhip_memcpy_d2h.argtypes = [vp, vp, C.c_long]
+        L.pfhip_memset.argtypes = [vp, C.c_int, C.c_long]
+        L.pfhip_problem_create.argtypes = [C.POINTER(ProblemDescC), C.POINTER(vp)]
+        L.pfhip_problem_destroy.argtypes = [vp]
+        L.pfhip_bc_pressure.argtypes = [vp, C.c_double, vp]
+        L.pfhip_saturation.argtypes = [vp, vp, vp, C.c_int, vp]
+        L.pfhip_phase_rel_perm.argtypes = [vp, vp, vp, C.c_int, vp]
+        L.pfhip_nl_function_eval.argtypes = [vp, vp, vp, vp, vp, C.c_double, vp]
+        L.pfhip_richards_jacobian_eval.argtypes = [vp, vp, C.c_double, C.c_int,
+                                                   C.POINTER(vp), vp]
+        L.pfhip_matvec.argtypes = [vp, C.POINTER(vp), C.c_int, vp, vp, vp]
+        L.pfhip_overland_flow_eval_kin.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, vp]
+        L.pfhip_vec_linear_sum.argtypes = [vp, C.c_double, vp, C.c_double, vp, vp, vp]
+        L.pfhip_vec_const.argtypes = [vp, C.c_double, vp, vp]
+        for f in ("pfhip_vec_dot",):
+            getattr(L, f).argtypes = [vp, vp, vp, C.POINTER(C.c_double), vp]
+        for f in ("pfhip_vec_max_norm", "pfhip_vec_l1_norm"):
+            getattr(L, f).argtypes = [vp, vp, C.POINTER(C.c_double), vp]
+        L.pfhip_mg_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(vp)]
+        L.pfhip_mg_destroy.argtypes = [vp]
+        L.pfhip_mg_num_levels.argtypes = [vp, C.POINTER(C.c_int)]
+        L.pfhip_mg_setup.argtypes = [vp, C.POINTER(vp), vp]
+        L.pfhip_mg_apply.argtypes = [vp, vp, vp, vp]
+        L.pfhip_solver_create.argtypes = [vp, C.POINTER(SolverOptsC), C.POINTER(vp)]
+        L.pfhip_solver_destroy.argtypes = [vp]
+        L.pfhip_kinsol_nonlin_solver.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double,
+                                                 C.POINTER(KinStatsC)]
+        L.pfhip_advance_richards.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_int,
+                                             C.c_int, vp, vp, vp, vp, C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().pfhip_last_error().decode(errors="replace")
+        raise PFHIPError(f"{what}: error {rc}: {msg}")
+
+
+# ---------------------------------------------------------------------------
+class DeviceArray:
+    """A device allocation of float64 (or float32) values."""
+
+    def __init__(self, n, dtype=np.float64, zero=True):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().pfhip_malloc(C.byref(p), self.nbytes), "pfhip_malloc")
+        self.ptr = p
+        if zero:
+            check(lib().pfhip_memset(self.ptr, 0, self.nbytes), "pfhip_memset")
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype, zero=False)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n, (a.size, self.n)
+        check(lib().pfhip_memcpy_h2d(self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes),
+              "h2d")
+
+    def download(self, shape=None):
+        out = np.empty(self.n, dtype=self.dtype)
+        check(lib().pfhip_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes),
+              "d2h")
+        return out.reshape(shape) if shape is not None else out
+
+    def free(self):
+        if self.ptr:
+            lib().pfhip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+
+def solver_opts_c(o: SolverOptions, precision=64, krylov=0):
+    pre, post, sym = o.mg_params()
+    c = SolverOptsC()
+    c.residual_tol = o.residual_tol
+    c.step_tol = o.step_tol
+    c.max_iter = o.max_iter
+    c.krylov_dim = o.krylov_dim
+    c.max_restarts = o.max_restarts
+    c.eta_choice = o.eta_choice
+    c.eta_value = o.eta_value
+    c.eta_alpha = o.eta_alpha
+    c.eta_gamma = o.eta_gamma
+    c.use_jacobian = int(o.use_jacobian)
+    c.deriv_eps = o.deriv_eps
+    c.globalization = o.globalization
+    c.precond = 0 if o.precond == "NoPC" else 1
+    c.mg_pre, c.mg_post, c.mg_symmetric = pre, post, sym
+    c.mg_max_levels = o.max_levels
+    c.mg_precision = precision
+    c.krylov = krylov
+    return c
+
+
+class GPUProblem:
+    """Device-resident ProblemData for one Problem (box domain)."""
+
+    def __init__(self, prob: Problem, src=None):
+        self.prob = prob
+        self._keep = {}
+        d = ProblemDescC()
+        d.nx, d.ny, d.nz = prob.nx, prob.ny, prob.nz
+        d.dx, d.dy, d.dz = prob.dx, prob.dy, prob.dz
+        d.x0, d.y0, d.z0 = prob.x0, prob.y0, prob.z0
+        d.rho, d.mu, d.gravity = prob.rho, prob.mu, prob.gravity
+        fields = ("kx", "ky", "kz", "poro", "ss", "alpha", "nvg", "sres", "ssat")
+        # isotropic permeability shares one device array
+        cache = {}
+        for name in fields:
+            a = np.ascontiguousarray(getattr(prob, "g_" + name))
+            key = None
+            for kname, (arr, darr) in cache.items():
+                if arr.shape == a.shape and np.array_equal(arr, a):
+                    key = kname
+                    break
+            if key is None:
+                darr = DeviceArray.from_host(a)
+                cache[name] = (a, darr)
+                self._keep[name] = darr
+            else:
+                darr = cache[key][1]
+            setattr(d, name, darr.value)
+        if src is not None:
+            self._keep["src"] = DeviceArray.from_host(np.ascontiguousarray(src))
+            d.src = self._keep["src"].value
+        if prob.overland:
+            for name in ("slope_x", "slope_y", "mann"):
+                darr = DeviceArray.from_host(np.ascontiguousarray(getattr(prob, "g_" + name)))
+                self._keep[name] = darr
+                setattr(d, name, darr.value)
+        d.ov_eps = prob.ov_eps
+        for f, b in enumerate(prob.bc):
+            cb = d.bc[f]
+            cb.type = b.type
+            cb.n_intervals = len(b.values)
+            cb.repeat = b.repeat
+            for i, (L_, v) in enumerate(zip(b.lengths, b.values)):
+                cb.lengths[i] = int(L_)
+                cb.values[i] = float(v)
+            cb.z_ref = b.z_ref
+        d.base_unit = prob.base_unit
+        d.start_time = prob.start_time
+        self.desc = d
+        h = C.c_void_p()
+        check(lib().pfhip_problem_create(C.byref(d), C.byref(h)), "pfhip_problem_create")
+        self.h = h
+        self.ng = int(np.prod(prob.shape))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pfhip_problem_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    # -- helpers -------------------------------------------------------------
+    def vec(self, a=None):
+        if a is None:
+            return DeviceArray(self.ng)
+        return DeviceArray.from_host(np.ascontiguousarray(a, dtype=np.float64).ravel())
+
+    def get(self, d):
+        return d.download(self.prob.shape)
+
+    def bc_pressure(self, t):
+        check(lib().pfhip_bc_pressure(self.h, C.c_double(t), None), "bc_pressure")
+
+    # -- module entry points (host arrays in, host arrays out) ---------------
+    def saturation(self, p, fcn=0):
+        dp, out = self.vec(p), self.vec()
+        check(lib().pfhip_saturation(self.h, dp.ptr, out.ptr, fcn, None), "saturation")
+        return self.get(out)
+
+    def phase_rel_perm(self, p, fcn=0):
+        dp, out = self.vec(p), self.vec()
+        check(lib().pfhip_phase_rel_perm(self.h, dp.ptr, out.ptr, fcn, None), "relperm")
+        return self.get(out)
+
+    def nl_function_eval(self, p, p_old, s_old, dt, t):
+        self.bc_pressure(t)
+        dp, dpo, dso, F = self.vec(p), self.vec(p_old), self.vec(s_old), self.vec()
+        check(lib().pfhip_nl_function_eval(self.h, dp.ptr, F.ptr, dpo.ptr, dso.ptr,
+                                           C.c_double(dt), None), "nl_function_eval")
+        return self.get(F)
+
+    def richards_jacobian_eval(self, p, dt, t, symm):
+        self.bc_pressure(t)
+        dp = self.vec(p)
+        J = [self.vec() for _ in range(7)]
+        arr = (C.c_void_p * 7)(*[j.value for j in J])
+        check(lib().pfhip_richards_jacobian_eval(self.h, dp.ptr, C.c_double(dt), int(symm),
+                                                 arr, None), "jacobian_eval")
+        return [self.get(j) for j in J]
+
+    def matvec(self, J, symm, x):
+        dJ = [self.vec(j) for j in J]
+        arr = (C.c_void_p * 7)(*[j.value for j in dJ])
+        dx_, y = self.vec(x), self.vec()
+        check(lib().pfhip_matvec(self.h, arr, int(symm), dx_.ptr, y.ptr, None), "matvec")
+        return self.get(y)
+
+    def overland_flow_eval_kin(self, p, fcn=0):
+        n2 = (self.prob.nx + 2) * (self.prob.ny + 2)
+        dp = self.vec(p)
+        outs = [DeviceArray(n2) for _ in range(4)]
+        check(lib().pfhip_overland_flow_eval_kin(self.h, dp.ptr, *[o.ptr for o in outs],
+                                                 fcn, None), "overland")
+        return [o.download() for o in outs]
+
+    def vec_dot(self, x, y):
+        dx_, dy_ = self.vec(x), self.vec(y)
+        r = C.c_double()
+        check(lib().pfhip_vec_dot(self.h, dx_.ptr, dy_.ptr, C.byref(r), None), "dot")
+        return r.value
+
+    def vec_max_norm(self, x):
+        dx_ = self.vec(x)
+        r = C.c_double()
+        check(lib().pfhip_vec_max_norm(self.h, dx_.ptr, C.byref(r), None), "maxnorm")
+        return r.value
+
+    def vec_l1_norm(self, x):
+        dx_ = self.vec(x)
+        r = C.c_double()
+        check(lib().pfhip_vec_l1_norm(self.h, dx_.ptr, C.byref(r), None), "l1norm")
+        return r.value
+
+
+class GPUMultigrid:
+    """The multigrid preconditioner module (PrecondInvoke)."""
+
+    def __init__(self, gp: GPUProblem, pre, post, symmetric, precision=64, max_levels=100):
+        self.gp = gp
+        h = C.c_void_p()
+        check(lib().pfhip_mg_create(gp.h, max_levels, pre, post, symmetric, precision,
+                                    C.byref(h)), "mg_create")
+        self.h = h
+
+    def levels(self):
+        n = C.c_int()
+        check(lib().pfhip_mg_num_levels(self.h, C.byref(n)), "mg_levels")
+        return n.value
+
+    def setup(self, Jsym):
+        self._J = [self.gp.vec(j) for j in Jsym]
+        arr = (C.c_void_p * 7)(*[j.value for j in self._J])
+        check(lib().pfhip_mg_setup(self.h, arr, None), "mg_setup")
+
+    def apply(self, b):
+        db, x = self.gp.vec(b), self.gp.vec()
+        check(lib().pfhip_mg_apply(self.h, db.ptr, x.ptr, None), "mg_apply")
+        return self.gp.get(x)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pfhip_mg_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class GPUSolver:
+    """KinsolNonlinSolver + AdvanceRichards on the device."""
+
+    def __init__(self, gp: GPUProblem, opts: SolverOptions, precision=64, krylov=0):
+        self.gp = gp
+        self.opts = opts
+        self.c_opts = solver_opts_c(opts, precision, krylov)
+        h = C.c_void_p()
+        check(lib().pfhip_solver_create(gp.h, C.byref(self.c_opts), C.byref(h)),
+              "solver_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pfhip_solver_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def kinsol(self, p, p_old, s_old, dt, t):
+        dp, dpo, dso = self.gp.vec(p), self.gp.vec(p_old), self.gp.vec(s_old)
+        st = KinStatsC()
+        rc = lib().pfhip_kinsol_nonlin_solver(self.h, dp.ptr, dpo.ptr, dso.ptr,
+                                              C.c_double(dt), C.c_double(t), C.byref(st))
+        if rc not in (0, -4):
+            check(rc, "kinsol")
+        return rc, self.gp.get(dp), st
+
+    def advance_device(self, dp, ds, t0, dt, nsteps):
+        """Runs nsteps on device arrays dp, ds; returns per-step counts."""
+        newton = np.zeros(nsteps, dtype=np.int32)
+        gmres = np.zeros(nsteps, dtype=np.int32)
+        fevals = np.zeros(nsteps, dtype=np.int32)
+        dtu = np.zeros(nsteps, dtype=np.float64)
+        done = C.c_int()
+        rc = lib().pfhip_advance_richards(
+            self.h, dp.ptr, ds.ptr, C.c_double(t0), C.c_double(dt), int(nsteps),
+            int(self.opts.max_convergence_failures),
+            newton.ctypes.data_as(C.c_void_p), gmres.ctypes.data_as(C.c_void_p),
+            fevals.ctypes.data_as(C.c_void_p), dtu.ctypes.data_as(C.c_void_p),
+            C.byref(done))
+        check(rc, "advance_richards")
+        return dict(newton=newton, gmres=gmres, fevals=fevals, dt=dtu, steps=done.value)
+
+    def advance(self, p0, t0, dt, nsteps, history=True):
+        """Host-array convenience: returns (p_hist, s_hist, log)."""
+        dp, ds = self.gp.vec(p0), self.gp.vec()
+        ph, sh = [], []
+        logs = []
+        if history:
+            check(lib().pfhip_saturation(self.gp.h, dp.ptr, ds.ptr, 0, None), "saturation")
+            ph.append(self.gp.get(dp))
+            sh.append(self.gp.get(ds))
+            t = t0
+            for n in range(nsteps):
+                lg = self.advance_device(dp, ds, t, dt, 1)
+                t += lg["dt"][0]
+                logs.append(lg)
+                ph.append(self.gp.get(dp))
+                sh.append(self.gp.get(ds))
+            log = {k: np.concatenate([l_[k] for l_ in logs]) for k in ("newton", "gmres",
+                                                                        "fevals", "dt")}
+            return np.array(ph), np.array(sh), log
+        lg = self.advance_device(dp, ds, t0, dt, nsteps)
+        return self.gp.get(dp), self.gp.get(ds), lg

@@ -1,0 +1,1420 @@
+// core.hip -- libpfhip: C-ABI plumbing, N_Vector operations with
+// deterministic wavefront-shuffle reductions, and the Richards physics
+// kernels (van Genuchten closures, residual, Jacobian, 7-point matvec,
+// kinematic overland flux) for gfx950.
+//
+// Layout: ParFlow Subvector layout, one ghost layer, x fastest.  Interior
+// kernels launch 64x4 thread blocks (one wavefront per x-row segment, so
+// every load along x is coalesced) with one z-plane per gridDim.z slice.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "pfhip_internal.h"
+
+// --------------------------------------------------------------------------
+// errors
+// --------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+
+extern "C" void pfhip_set_error(const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char *pfhip_last_error(void) { return g_err; }
+
+#define HIPCHK(x)                                                          \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess) {                                                \
+      pfhip_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x,             \
+                      hipGetErrorString(e_));                              \
+      return PFHIP_ERR_HIP;                                                \
+    }                                                                      \
+  } while (0)
+
+#define LAUNCHCHK()                                                        \
+  do {                                                                     \
+    hipError_t e_ = hipGetLastError();                                     \
+    if (e_ != hipSuccess) {                                                \
+      pfhip_set_error("%s:%d kernel launch: %s", __FILE__, __LINE__,      \
+                      hipGetErrorString(e_));                              \
+      return PFHIP_ERR_HIP;                                                \
+    }                                                                      \
+  } while (0)
+
+#define STREAM(s) ((hipStream_t)(s))
+
+// --------------------------------------------------------------------------
+// memory
+// --------------------------------------------------------------------------
+extern "C" int pfhip_device_count(int *count)
+{
+  HIPCHK(hipGetDeviceCount(count));
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_malloc(void **ptr, long bytes)
+{
+  if (bytes <= 0) { *ptr = nullptr; return PFHIP_OK; }
+  hipError_t e = hipMalloc(ptr, (size_t)bytes);
+  if (e != hipSuccess) {
+    pfhip_set_error("hipMalloc(%ld): %s", bytes, hipGetErrorString(e));
+    return PFHIP_ERR_NOMEM;
+  }
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_free(void *ptr)
+{
+  if (ptr) HIPCHK(hipFree(ptr));
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_memcpy_h2d(void *dst, const void *src, long bytes)
+{
+  HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_memcpy_d2h(void *dst, const void *src, long bytes)
+{
+  HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_memset(void *dst, int value, long bytes)
+{
+  HIPCHK(hipMemset(dst, value, (size_t)bytes));
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_synchronize(void)
+{
+  HIPCHK(hipDeviceSynchronize());
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_dev_alloc(double **p, long n)
+{
+  return pfhip_malloc((void **)p, n * (long)sizeof(double));
+}
+
+__global__ void k_fill(double *__restrict__ p, long n, double v)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+extern "C" int pfhip_zero(double *p, long n, void *stream)
+{
+  if (n <= 0) return PFHIP_OK;
+  long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0,
+                     STREAM(stream), p, n, 0.0);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// --------------------------------------------------------------------------
+// geometry helpers
+// --------------------------------------------------------------------------
+struct Geom {
+  int nx, ny, nz, sy, sz;
+};
+
+static inline Geom geom_of(const pfhip_problem *pb)
+{
+  Geom g;
+  g.nx = pb->d.nx; g.ny = pb->d.ny; g.nz = pb->d.nz;
+  g.sy = pb->sy; g.sz = pb->sz;
+  return g;
+}
+
+static inline dim3 grid3(const Geom &g)
+{
+  return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+}
+#define BLOCK3 dim3(64, 4, 1)
+
+// --------------------------------------------------------------------------
+// problem
+// --------------------------------------------------------------------------
+static int face_ncells(const pfhip_problem_desc *d, int f)
+{
+  if (f < 2) return d->ny * d->nz;
+  if (f < 4) return d->nx * d->nz;
+  return d->nx * d->ny;
+}
+
+extern "C" int pfhip_problem_create(const pfhip_problem_desc *desc,
+                                    pfhip_problem **out)
+{
+  if (!desc || !out) { pfhip_set_error("null argument"); return PFHIP_ERR_ARG; }
+  const pfhip_problem_desc *d = desc;
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1 || d->dx <= 0 || d->dy <= 0 ||
+      d->dz <= 0) {
+    pfhip_set_error("bad grid %dx%dx%d", d->nx, d->ny, d->nz);
+    return PFHIP_ERR_ARG;
+  }
+  if (!d->kx || !d->ky || !d->kz || !d->poro || !d->ss || !d->alpha ||
+      !d->nvg || !d->sres || !d->ssat) {
+    pfhip_set_error("problem fields must be device arrays");
+    return PFHIP_ERR_ARG;
+  }
+  for (int f = 0; f < 6; f++) {
+    const pfhip_bc_spec *b = &d->bc[f];
+    if (b->type < 0 || b->type > 2 || (b->type == PFHIP_BC_OVERLAND && f != 5)) {
+      pfhip_set_error("bad BC type %d on face %d", b->type, f);
+      return PFHIP_ERR_ARG;
+    }
+    if (b->n_intervals < 1 || b->n_intervals > PFHIP_MAX_INTERVALS) {
+      pfhip_set_error("bad BC cycle on face %d", f);
+      return PFHIP_ERR_ARG;
+    }
+  }
+  if (d->bc[5].type == PFHIP_BC_OVERLAND &&
+      (!d->slope_x || !d->slope_y || !d->mann)) {
+    pfhip_set_error("OverlandKinematic needs slope_x, slope_y, mann");
+    return PFHIP_ERR_ARG;
+  }
+  long ng = (long)(d->nx + 2) * (d->ny + 2) * (d->nz + 2);
+  if (ng > 2147483647L) {
+    pfhip_set_error("grid too large for 32-bit cell indexing");
+    return PFHIP_ERR_ARG;
+  }
+  pfhip_problem *pb = (pfhip_problem *)calloc(1, sizeof(pfhip_problem));
+  pb->d = *d;
+  pb->sy = d->nx + 2;
+  pb->sz = (d->nx + 2) * (d->ny + 2);
+  pb->ng = ng;
+  pb->n2 = (long)(d->nx + 2) * (d->ny + 2);
+  pb->ncell = (long)d->nx * d->ny * d->nz;
+  pb->overland = (d->bc[5].type == PFHIP_BC_OVERLAND);
+  int rc = PFHIP_OK;
+  for (int f = 0; f < 6 && rc == PFHIP_OK; f++) {
+    int n = face_ncells(d, f);
+    rc = pfhip_dev_alloc(&pb->bcv[f], n);
+    if (rc == PFHIP_OK) {
+      hipError_t e = hipHostMalloc((void **)&pb->h_bcv[f], sizeof(double) * n);
+      if (e != hipSuccess) { pfhip_set_error("hipHostMalloc"); rc = PFHIP_ERR_NOMEM; }
+    }
+  }
+  if (rc == PFHIP_OK) rc = pfhip_dev_alloc(&pb->krho, ng);
+  if (rc == PFHIP_OK) rc = pfhip_dev_alloc(&pb->dkrho, ng);
+  for (int q = 0; q < 4 && rc == PFHIP_OK; q++) rc = pfhip_dev_alloc(&pb->q2[q], pb->n2);
+  if (rc == PFHIP_OK) rc = pfhip_dev_alloc(&pb->red_partial, PFHIP_RED_MAX_BLOCKS * PFHIP_MAX_MULTI);
+  if (rc == PFHIP_OK) rc = pfhip_dev_alloc(&pb->red_dev, PFHIP_MAX_MULTI);
+  if (rc == PFHIP_OK) {
+    hipError_t e = hipHostMalloc((void **)&pb->h_red, sizeof(double) * PFHIP_MAX_MULTI);
+    if (e != hipSuccess) { pfhip_set_error("hipHostMalloc"); rc = PFHIP_ERR_NOMEM; }
+  }
+  if (rc == PFHIP_OK) rc = pfhip_zero(pb->krho, ng, nullptr);
+  if (rc == PFHIP_OK) rc = pfhip_zero(pb->dkrho, ng, nullptr);
+  if (rc != PFHIP_OK) { pfhip_problem_destroy(pb); return rc; }
+  HIPCHK(hipDeviceSynchronize());
+  *out = pb;
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_problem_destroy(pfhip_problem *pb)
+{
+  if (!pb) return PFHIP_OK;
+  for (int f = 0; f < 6; f++) {
+    if (pb->bcv[f]) hipFree(pb->bcv[f]);
+    if (pb->h_bcv[f]) hipHostFree(pb->h_bcv[f]);
+  }
+  if (pb->krho) hipFree(pb->krho);
+  if (pb->dkrho) hipFree(pb->dkrho);
+  for (int q = 0; q < 4; q++) if (pb->q2[q]) hipFree(pb->q2[q]);
+  if (pb->red_partial) hipFree(pb->red_partial);
+  if (pb->red_dev) hipFree(pb->red_dev);
+  if (pb->h_red) hipHostFree(pb->h_red);
+  free(pb);
+  return PFHIP_OK;
+}
+
+// TimeCycleDataComputeIntervalNumber, PL/time_cycle_data.c:152-197
+static int interval_number(const pfhip_problem_desc *d, const pfhip_bc_spec *b,
+                           double time)
+{
+  int cycle = 0;
+  for (int i = 0; i < b->n_intervals; i++) cycle += b->lengths[i];
+  double bu = d->base_unit;
+  if (b->repeat < 0 || time < (double)(b->repeat * cycle * bu)) {
+    int dt_ = (int)lround(time / (bu / 1000)) / 1000;
+    int ds_ = (int)lround(d->start_time / (bu / 1000)) / 1000;
+    int completed = (dt_ - ds_ - 1) % cycle;
+    int iv = 0, total = b->lengths[0];
+    while (completed >= total) { iv++; total += b->lengths[iv]; }
+    return iv;
+  }
+  return b->n_intervals - 1;
+}
+
+// BCPressure: PL/problem_bc_pressure.c:143-290 (DirEquilRefPatch with a
+// constant density reduces to p = p_ref - rho g (z - z_ref), z = centre for
+// side faces, face elevation for z faces), :582-612, :1070-1098.
+extern "C" int pfhip_bc_pressure(pfhip_problem *pb, double time, void *stream)
+{
+  const pfhip_problem_desc *d = &pb->d;
+  for (int f = 0; f < 6; f++) {
+    const pfhip_bc_spec *b = &d->bc[f];
+    int n = face_ncells(d, f);
+    double val = b->values[interval_number(d, b, time)];
+    double *h = pb->h_bcv[f];
+    if (b->type != PFHIP_BC_DIR_EQUIL) {
+      for (int c = 0; c < n; c++) h[c] = val;
+    } else {
+      for (int c = 0; c < n; c++) {
+        int k;
+        if (f < 2) k = c / d->ny;
+        else if (f < 4) k = c / d->nx;
+        else k = (f == 4) ? 0 : d->nz - 1;
+        double z = d->z0 + (k + 0.5) * d->dz;
+        if (f == 4) z = z - 0.5 * d->dz;
+        if (f == 5) z = z + 0.5 * d->dz;
+        double gneg = -d->gravity;
+        double fcn_val = val - val - 0.5 * (d->rho + d->rho) * gneg * (z - b->z_ref);
+        double pv = val - fcn_val / 1.0;
+        h[c] = pv;
+      }
+    }
+    HIPCHK(hipMemcpyAsync(pb->bcv[f], h, sizeof(double) * n,
+                          hipMemcpyHostToDevice, STREAM(stream)));
+  }
+  HIPCHK(hipStreamSynchronize(STREAM(stream)));
+  return PFHIP_OK;
+}
+
+// --------------------------------------------------------------------------
+// deterministic reductions over interior rows
+// --------------------------------------------------------------------------
+// A block owns whole (j,k) rows, threads stride along x: one division per
+// row, coalesced loads.  Block partials -> one final block, fixed order.
+
+static inline int red_blocks(const Geom &g)
+{
+  long rows = (long)g.ny * g.nz;
+  return (int)(rows < PFHIP_RED_MAX_BLOCKS ? rows : PFHIP_RED_MAX_BLOCKS);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// 256-thread block reduction; result valid in thread 0
+template <bool MAX>
+__device__ __forceinline__ double block_reduce(double v)
+{
+  __shared__ double sh[4];
+  v = MAX ? wave_max(v) : wave_sum(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    r = sh[0];
+    for (int i = 1; i < 4; i++) r = MAX ? fmax(r, sh[i]) : r + sh[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+enum RedOp { RED_DOT = 0, RED_MAXABS = 1, RED_L1 = 2, RED_SCSTEP = 3, RED_DQ = 4 };
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_reduce(Geom g, const double *__restrict__ x,
+                                                const double *__restrict__ y,
+                                                double *__restrict__ partial, int nb)
+{
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  int rows = g.ny * g.nz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    int j = r % g.ny, k = r / g.ny;
+    int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+      int ip = base + i;
+      if (OP == RED_DOT) a0 += x[ip] * y[ip];
+      else if (OP == RED_MAXABS) a0 = fmax(a0, fabs(x[ip]));
+      else if (OP == RED_L1) a0 += fabs(x[ip]);
+      else if (OP == RED_SCSTEP) a0 = fmax(a0, fabs(y[ip] / (1.0 + fabs(x[ip]))));
+      else {  // RED_DQ: u.v, v.v, |v|_1
+        double u = x[ip], v = y[ip];
+        a0 += u * v;
+        a1 += v * v;
+        a2 += fabs(v);
+      }
+    }
+  }
+  const bool MX = (OP == RED_MAXABS || OP == RED_SCSTEP);
+  a0 = block_reduce<MX>(a0);
+  if (OP == RED_DQ) { a1 = block_reduce<false>(a1); a2 = block_reduce<false>(a2); }
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = a0;
+    if (OP == RED_DQ) { partial[nb + blockIdx.x] = a1; partial[2 * nb + blockIdx.x] = a2; }
+  }
+}
+
+// final fixed-order reduction of nv vectors of nb partials each
+template <bool MAX>
+__global__ __launch_bounds__(256) void k_reduce_final(const double *__restrict__ partial,
+                                                      int nb, int nv,
+                                                      double *__restrict__ out)
+{
+  for (int v = 0; v < nv; v++) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      double t = partial[v * nb + i];
+      a = MAX ? fmax(a, t) : a + t;
+    }
+    a = block_reduce<MAX>(a);
+    if (threadIdx.x == 0) out[v] = a;
+  }
+}
+
+static int fetch_results(pfhip_problem *pb, int nv, void *stream)
+{
+  HIPCHK(hipMemcpyAsync(pb->h_red, pb->red_dev, sizeof(double) * nv,
+                        hipMemcpyDeviceToHost, STREAM(stream)));
+  HIPCHK(hipStreamSynchronize(STREAM(stream)));
+  return PFHIP_OK;
+}
+
+template <int OP>
+static int run_reduce(pfhip_problem *pb, const double *x, const double *y,
+                      int nv, void *stream)
+{
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  hipLaunchKernelGGL(k_reduce<OP>, dim3(nb), dim3(256), 0, STREAM(stream), g, x,
+                     y, pb->red_partial, nb);
+  LAUNCHCHK();
+  const bool MX = (OP == RED_MAXABS || OP == RED_SCSTEP);
+  hipLaunchKernelGGL(k_reduce_final<MX>, dim3(1), dim3(256), 0, STREAM(stream),
+                     pb->red_partial, nb, nv, pb->red_dev);
+  LAUNCHCHK();
+  return fetch_results(pb, nv, stream);
+}
+
+extern "C" int pfhip_vec_dot(pfhip_problem *pb, const double *x, const double *y,
+                             double *result, void *stream)
+{
+  int rc = run_reduce<RED_DOT>(pb, x, y, 1, stream);
+  if (rc) return rc;
+  *result = pb->h_red[0];
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_max_norm(pfhip_problem *pb, const double *x,
+                                  double *result, void *stream)
+{
+  int rc = run_reduce<RED_MAXABS>(pb, x, x, 1, stream);
+  if (rc) return rc;
+  *result = pb->h_red[0];
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_l1_norm(pfhip_problem *pb, const double *x,
+                                 double *result, void *stream)
+{
+  int rc = run_reduce<RED_L1>(pb, x, x, 1, stream);
+  if (rc) return rc;
+  *result = pb->h_red[0];
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_scsteplength(pfhip_problem *pb, const double *u,
+                                      const double *s, double *result, void *stream)
+{
+  int rc = run_reduce<RED_SCSTEP>(pb, u, s, 1, stream);
+  if (rc) return rc;
+  *result = pb->h_red[0];
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_dq_sums(pfhip_problem *pb, const double *u,
+                                 const double *v, double *sutsv, double *vtv,
+                                 double *l1, void *stream)
+{
+  int rc = run_reduce<RED_DQ>(pb, u, v, 3, stream);
+  if (rc) return rc;
+  *sutsv = pb->h_red[0];
+  *vtv = pb->h_red[1];
+  *l1 = pb->h_red[2];
+  return PFHIP_OK;
+}
+
+// ---- block Gram-Schmidt kernels ------------------------------------------
+#define MD_CHUNK 16
+
+struct VecPtrs {
+  const double *v[MD_CHUNK];
+};
+
+// h[j] = V[j].w for j < m (m <= MD_CHUNK), one pass over w
+__global__ __launch_bounds__(256) void k_multi_dot(Geom g, VecPtrs V, int m,
+                                                   const double *__restrict__ w,
+                                                   double *__restrict__ partial, int nb)
+{
+  double acc[MD_CHUNK];
+#pragma unroll
+  for (int q = 0; q < MD_CHUNK; q++) acc[q] = 0.0;
+  int rows = g.ny * g.nz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    int j = r % g.ny, k = r / g.ny;
+    int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+      int ip = base + i;
+      double wv = w[ip];
+#pragma unroll
+      for (int q = 0; q < MD_CHUNK; q++)
+        if (q < m) acc[q] += V.v[q][ip] * wv;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MD_CHUNK; q++) {
+    if (q < m) {
+      double s = block_reduce<false>(acc[q]);
+      if (threadIdx.x == 0) partial[q * nb + blockIdx.x] = s;
+    }
+  }
+}
+
+extern "C" int pfhip_vec_multi_dot(pfhip_problem *pb, const double *const *V,
+                                   int m, const double *w, double *h, void *stream)
+{
+  if (m < 0 || m > PFHIP_MAX_MULTI) { pfhip_set_error("multi_dot m=%d", m); return PFHIP_ERR_ARG; }
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  for (int c0 = 0; c0 < m; c0 += MD_CHUNK) {
+    int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
+    VecPtrs P;
+    for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < mc) ? V[c0 + q] : w;
+    hipLaunchKernelGGL(k_multi_dot, dim3(nb), dim3(256), 0, STREAM(stream), g, P,
+                       mc, w, pb->red_partial + (long)c0 * nb, nb);
+    LAUNCHCHK();
+  }
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, STREAM(stream),
+                     pb->red_partial, nb, m, pb->red_dev);
+  LAUNCHCHK();
+  int rc = fetch_results(pb, m, stream);
+  if (rc) return rc;
+  for (int q = 0; q < m; q++) h[q] = pb->h_red[q];
+  return PFHIP_OK;
+}
+
+struct Coefs {
+  double c[MD_CHUNK];
+};
+
+// w += sum_q c[q] V[q]; optional ||w||^2 partials
+template <bool NORM>
+__global__ __launch_bounds__(256) void k_multi_axpy(Geom g, VecPtrs V, Coefs C, int m,
+                                                    double *__restrict__ w,
+                                                    double *__restrict__ partial, int nb)
+{
+  double acc = 0.0;
+  int rows = g.ny * g.nz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    int j = r % g.ny, k = r / g.ny;
+    int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+      int ip = base + i;
+      double s = w[ip];
+#pragma unroll
+      for (int q = 0; q < MD_CHUNK; q++)
+        if (q < m) s += C.c[q] * V.v[q][ip];
+      w[ip] = s;
+      if (NORM) acc += s * s;
+    }
+  }
+  if (NORM) {
+    double s = block_reduce<false>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+static int multi_axpy_impl(pfhip_problem *pb, const double *const *V, int m,
+                           const double *coef, double sign, double *w,
+                           double *norm2_out, void *stream)
+{
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  for (int c0 = 0; c0 < m; c0 += MD_CHUNK) {
+    int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
+    VecPtrs P;
+    Coefs C;
+    for (int q = 0; q < MD_CHUNK; q++) {
+      P.v[q] = (q < mc) ? V[c0 + q] : w;
+      C.c[q] = (q < mc) ? sign * coef[c0 + q] : 0.0;
+    }
+    bool last = (c0 + MD_CHUNK >= m);
+    if (last && norm2_out) {
+      hipLaunchKernelGGL(k_multi_axpy<true>, dim3(nb), dim3(256), 0, STREAM(stream),
+                         g, P, C, mc, w, pb->red_partial, nb);
+    } else {
+      hipLaunchKernelGGL(k_multi_axpy<false>, dim3(nb), dim3(256), 0, STREAM(stream),
+                         g, P, C, mc, w, pb->red_partial, nb);
+    }
+    LAUNCHCHK();
+  }
+  if (norm2_out) {
+    if (m == 0) return pfhip_vec_dot(pb, w, w, norm2_out, stream);
+    hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, STREAM(stream),
+                       pb->red_partial, nb, 1, pb->red_dev);
+    LAUNCHCHK();
+    int rc = fetch_results(pb, 1, stream);
+    if (rc) return rc;
+    *norm2_out = pb->h_red[0];
+  }
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_multi_axpy(pfhip_problem *pb, const double *const *V,
+                                    int m, const double *h, double *w,
+                                    double *norm2_out, void *stream)
+{
+  return multi_axpy_impl(pb, V, m, h, -1.0, w, norm2_out, stream);
+}
+
+extern "C" int pfhip_vec_combine(pfhip_problem *pb, const double *const *V, int m,
+                                 const double *y, double *x, void *stream)
+{
+  return multi_axpy_impl(pb, V, m, y, 1.0, x, nullptr, stream);
+}
+
+// MGS step: w -= hprev*vprev (if vprev) and dot = vnext.w (if vnext)
+__global__ __launch_bounds__(256) void k_axpy_dot(Geom g, double hprev,
+                                                  const double *__restrict__ vprev,
+                                                  double *__restrict__ w,
+                                                  const double *__restrict__ vnext,
+                                                  double *__restrict__ partial)
+{
+  double acc = 0.0;
+  int rows = g.ny * g.nz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    int j = r % g.ny, k = r / g.ny;
+    int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+      int ip = base + i;
+      double s = w[ip];
+      if (vprev) { s = 1.0 * s + (-hprev) * vprev[ip]; w[ip] = s; }
+      if (vnext) acc += vnext[ip] * s;
+    }
+  }
+  double s = block_reduce<false>(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+extern "C" int pfhip_vec_axpy_dot(pfhip_problem *pb, double hprev,
+                                  const double *vprev, double *w,
+                                  const double *vnext, double *dot, void *stream)
+{
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  hipLaunchKernelGGL(k_axpy_dot, dim3(nb), dim3(256), 0, STREAM(stream), g, hprev,
+                     vprev, w, vnext, pb->red_partial);
+  LAUNCHCHK();
+  if (!vnext) return PFHIP_OK;
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, STREAM(stream),
+                     pb->red_partial, nb, 1, pb->red_dev);
+  LAUNCHCHK();
+  int rc = fetch_results(pb, 1, stream);
+  if (rc) return rc;
+  *dot = pb->h_red[0];
+  return PFHIP_OK;
+}
+
+// ---- elementwise ----------------------------------------------------------
+__global__ __launch_bounds__(256) void k_linear_sum(Geom g, double a,
+                                                    const double *__restrict__ x,
+                                                    double b,
+                                                    const double *__restrict__ y,
+                                                    double *__restrict__ z)
+{
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  int ip = (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
+  z[ip] = a * x[ip] + b * y[ip];
+}
+
+__global__ __launch_bounds__(256) void k_scale(Geom g, double c,
+                                               const double *__restrict__ x,
+                                               double *__restrict__ z)
+{
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  int ip = (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
+  z[ip] = c * x[ip];
+}
+
+__global__ __launch_bounds__(256) void k_const(Geom g, double c, double *__restrict__ z)
+{
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  z[(i + 1) + (j + 1) * g.sy + (k + 1) * g.sz] = c;
+}
+
+extern "C" int pfhip_vec_linear_sum(pfhip_problem *pb, double a, const double *x,
+                                    double b, const double *y, double *z, void *stream)
+{
+  Geom g = geom_of(pb);
+  hipLaunchKernelGGL(k_linear_sum, grid3(g), BLOCK3, 0, STREAM(stream), g, a, x, b, y, z);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_scale(pfhip_problem *pb, double c, const double *x,
+                               double *z, void *stream)
+{
+  Geom g = geom_of(pb);
+  hipLaunchKernelGGL(k_scale, grid3(g), BLOCK3, 0, STREAM(stream), g, c, x, z);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_const(pfhip_problem *pb, double c, double *z, void *stream)
+{
+  Geom g = geom_of(pb);
+  hipLaunchKernelGGL(k_const, grid3(g), BLOCK3, 0, STREAM(stream), g, c, z);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// --------------------------------------------------------------------------
+// van Genuchten closures (PL/problem_saturation.c:265-310,
+// PL/problem_phase_rel_perm.c:767-792, 895-912), same operation order
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double vg_sat(double p, double rho, double g,
+                                         double alpha, double n, double sres,
+                                         double ssat)
+{
+  double m = 1.0 - (1.0 / n);
+  double s_dif = ssat - sres;
+  if (p >= 0.0) return s_dif + sres;
+  double head = fabs(p) / (rho * g);
+  return s_dif / pow(1.0 + pow(alpha * head, n), m) + sres;
+}
+
+__device__ __forceinline__ double vg_dsat(double p, double rho, double g,
+                                          double alpha, double n, double sres,
+                                          double ssat)
+{
+  double m = 1.0 - (1.0 / n);
+  double s_dif = ssat - sres;
+  if (p >= 0.0) return 0.0;
+  double head = fabs(p) / (rho * g);
+  return (m * n * alpha * pow(alpha * head, (n - 1))) * s_dif /
+         (pow(1.0 + pow(alpha * head, n), m + 1));
+}
+
+__device__ __forceinline__ double vg_kr(double p, double rho, double g,
+                                        double alpha, double n)
+{
+  if (p >= 0.0) return 1.0;
+  double m = 1.0 - (1.0 / n);
+  double head = fabs(p) / (rho * g);
+  double opahn = 1.0 + pow(alpha * head, n);
+  double ahnm1 = pow(alpha * head, n - 1);
+  double t = 1.0 - ahnm1 / pow(opahn, m);
+  return (t * t) / pow(opahn, m / 2);
+}
+
+__device__ __forceinline__ double vg_dkr(double p, double rho, double g,
+                                         double alpha, double n)
+{
+  if (p >= 0.0) return 0.0;
+  double m = 1.0 - (1.0 / n);
+  double head = fabs(p) / (rho * g);
+  double opahn = 1.0 + pow(alpha * head, n);
+  double ahnm1 = pow(alpha * head, n - 1);
+  double coeff = 1.0 - ahnm1 * pow(opahn, -m);
+  return 2.0 * (coeff / (pow(opahn, (m / 2)))) *
+             ((n - 1) * pow(alpha * head, n - 2) * alpha * pow(opahn, -m) -
+              ahnm1 * m * pow(opahn, -(m + 1)) * n * alpha * ahnm1) +
+         (coeff * coeff) * (m / 2) * pow(opahn, (-(m + 2) / 2)) * n * alpha * ahnm1;
+}
+
+struct Phys {
+  double rho, mu, g;
+  const double *__restrict__ alpha;
+  const double *__restrict__ nvg;
+  const double *__restrict__ sres;
+  const double *__restrict__ ssat;
+};
+
+static inline Phys phys_of(const pfhip_problem *pb)
+{
+  Phys f;
+  f.rho = pb->d.rho; f.mu = pb->d.mu; f.g = pb->d.gravity;
+  f.alpha = pb->d.alpha; f.nvg = pb->d.nvg; f.sres = pb->d.sres; f.ssat = pb->d.ssat;
+  return f;
+}
+
+// out = S (fcn 0) / dS (1) / kr (2) / dkr (3) / kr*rho (4), or both
+// kr*rho and dkr*rho into out, out2 (5); interior cells
+__global__ __launch_bounds__(256) void k_closure(Geom g, Phys f, int what,
+                                                 const double *__restrict__ p,
+                                                 double *__restrict__ out,
+                                                 double *__restrict__ out2)
+{
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  int ip = (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
+  double pv = p[ip], a = f.alpha[ip], n = f.nvg[ip];
+  switch (what) {
+    case 0: out[ip] = vg_sat(pv, f.rho, f.g, a, n, f.sres[ip], f.ssat[ip]); break;
+    case 1: out[ip] = vg_dsat(pv, f.rho, f.g, a, n, f.sres[ip], f.ssat[ip]); break;
+    case 2: out[ip] = vg_kr(pv, f.rho, f.g, a, n); break;
+    case 3: out[ip] = vg_dkr(pv, f.rho, f.g, a, n); break;
+    case 4: out[ip] = vg_kr(pv, f.rho, f.g, a, n) * f.rho; break;
+    default: {
+      double kr = vg_kr(pv, f.rho, f.g, a, n);
+      out[ip] = kr * f.rho;
+      out2[ip] = vg_dkr(pv, f.rho, f.g, a, n) * f.rho + kr * 0.0;
+    }
+  }
+}
+
+static int launch_closure(pfhip_problem *pb, int what, const double *p, double *out,
+                          double *out2, void *stream)
+{
+  Geom g = geom_of(pb);
+  hipLaunchKernelGGL(k_closure, grid3(g), BLOCK3, 0, STREAM(stream), g, phys_of(pb),
+                     what, p, out, out2);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_saturation(pfhip_problem *pb, const double *p, double *out,
+                                int fcn, void *stream)
+{
+  return launch_closure(pb, fcn == PFHIP_CALCDER ? 1 : 0, p, out, nullptr, stream);
+}
+
+extern "C" int pfhip_phase_rel_perm(pfhip_problem *pb, const double *p, double *out,
+                                    int fcn, void *stream)
+{
+  return launch_closure(pb, fcn == PFHIP_CALCDER ? 3 : 2, p, out, nullptr, stream);
+}
+
+// --------------------------------------------------------------------------
+// kinematic overland flux (PL/overlandflow_eval_Kin.c:120-460), one thread
+// per surface column of a box whose whole top is the overland patch
+// --------------------------------------------------------------------------
+struct Surf {
+  const double *__restrict__ sx;
+  const double *__restrict__ sy;
+  const double *__restrict__ mann;
+  double eps;
+};
+
+__device__ __forceinline__ double rpowerr(double b, double e)
+{
+  return (b <= 0.0) ? 0.0 : pow(b, e);
+}
+
+__global__ __launch_bounds__(256) void k_overland_q(Geom g, Surf s,
+                                                    const double *__restrict__ p,
+                                                    double *__restrict__ qx,
+                                                    double *__restrict__ qy)
+{
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  int io = (i + 1) + (j + 1) * g.sy;
+  int kt = g.nz - 1;
+  int ip = io + (kt + 1) * g.sz;
+  double Sf_x = s.sx[io], Sf_y = s.sy[io];
+  double Sf_mag = rpowerr(Sf_x * Sf_x + Sf_y * Sf_y, 0.5);
+  if (Sf_mag < s.eps) Sf_mag = s.eps;
+  double pc = p[ip];
+  double pe = (i + 1 < g.nx) ? p[ip + 1] : 0.0;
+  double pn = (j + 1 < g.ny) ? p[ip + g.sy] : 0.0;
+  double Px = ((-Sf_x - 0.0) >= 0) ? fmax(pc, 0.0) : fmax(pe, 0.0);
+  double Py = ((-Sf_y - 0.0) >= 0) ? fmax(pc, 0.0) : fmax(pn, 0.0);
+  double den = rpowerr(fabs(Sf_mag), 0.5) * s.mann[io];
+  qx[io] = -(Sf_x / den) * rpowerr(Px, (5.0 / 3.0));
+  qy[io] = -(Sf_y / den) * rpowerr(Py, (5.0 / 3.0));
+  if (i == 0)
+    qx[io - 1] = (Sf_x > 0.0) ? -(Sf_x / den) * rpowerr(fmax(pc, 0.0), (5.0 / 3.0)) : 0.0;
+  if (j == 0)
+    qy[io - g.sy] = (Sf_y > 0.0) ? -(Sf_y / den) * rpowerr(fmax(pc, 0.0), (5.0 / 3.0)) : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_overland_der(Geom g, Surf s,
+                                                      const double *__restrict__ p,
+                                                      double *__restrict__ ke,
+                                                      double *__restrict__ kw,
+                                                      double *__restrict__ kn,
+                                                      double *__restrict__ ks)
+{
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  int io = (i + 1) + (j + 1) * g.sy;
+  int kt = g.nz - 1;
+  int ip = io + (kt + 1) * g.sz;
+  double Sf_x = s.sx[io], Sf_y = s.sy[io];
+  double Sf_mag = rpowerr(Sf_x * Sf_x + Sf_y * Sf_y, 0.5);
+  if (Sf_mag < s.eps) Sf_mag = s.eps;
+  double pc = p[ip];
+  double pe = (i + 1 < g.nx) ? p[ip + 1] : 0.0;
+  double pn = (j + 1 < g.ny) ? p[ip + g.sy] : 0.0;
+  double Px = ((-Sf_x - 0.0) >= 0) ? fmax(pc, 0.0) : fmax(pe, 0.0);
+  double Py = ((-Sf_y - 0.0) >= 0) ? fmax(pc, 0.0) : fmax(pn, 0.0);
+  double den = rpowerr(fabs(Sf_mag), 0.5) * s.mann[io];
+  double qxt = -(5.0 / 3.0) * (Sf_x / den) * rpowerr(Px, (2.0 / 3.0));
+  double qyt = -(5.0 / 3.0) * (Sf_y / den) * rpowerr(Py, (2.0 / 3.0));
+  ke[io] = fmax(qxt, 0.0);
+  kw[io + 1] = -fmax(-qxt, 0.0);
+  kn[io] = fmax(qyt, 0.0);
+  ks[io + g.sy] = -fmax(-qyt, 0.0);
+  if (i == 0 && Sf_x > 0.0) {
+    double q = -(5.0 / 3.0) * (Sf_x / den) * rpowerr(fmax(pc, 0.0), (2.0 / 3.0));
+    kw[io] = -fmax(-q, 0.0);
+    ke[io - 1] = fmax(q, 0.0);
+  }
+  if (j == 0 && Sf_y > 0.0) {
+    double q = -(5.0 / 3.0) * (Sf_y / den) * rpowerr(fmax(pc, 0.0), (2.0 / 3.0));
+    ks[io] = -fmax(-q, 0.0);
+    kn[io - g.sy] = fmax(q, 0.0);
+  }
+}
+
+static inline Surf surf_of(const pfhip_problem *pb)
+{
+  Surf s;
+  s.sx = pb->d.slope_x; s.sy = pb->d.slope_y; s.mann = pb->d.mann; s.eps = pb->d.ov_eps;
+  return s;
+}
+
+static inline dim3 grid2(const Geom &g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, 1); }
+
+extern "C" int pfhip_overland_flow_eval_kin(pfhip_problem *pb, const double *p,
+                                            double *o0, double *o1, double *o2,
+                                            double *o3, int fcn, void *stream)
+{
+  if (!pb->d.slope_x) { pfhip_set_error("problem has no surface fields"); return PFHIP_ERR_ARG; }
+  Geom g = geom_of(pb);
+  if (fcn == PFHIP_CALCFCN) {
+    hipLaunchKernelGGL(k_overland_q, grid2(g), BLOCK3, 0, STREAM(stream), g,
+                       surf_of(pb), p, o0, o1);
+  } else {
+    for (double *a : {o0, o1, o2, o3}) {
+      int rc = pfhip_zero(a, pb->n2, stream);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_overland_der, grid2(g), BLOCK3, 0, STREAM(stream), g,
+                       surf_of(pb), p, o0, o1, o2, o3);
+  }
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// --------------------------------------------------------------------------
+// residual F(p) -- NlFunctionEval, PL/nl_function_eval.c:167-2323, gather
+// form: each cell sums the flux through its six faces.  A face flux is
+// computed from both sides with identical operands and operation order, so
+// both cells see the bit-identical value the reference scatters with
+// PlusEquals; no atomics.  Ghost permeability is zero in the reference
+// (PL/vector.c:321), so a domain face contributes only its BC "u_new" term
+// (:975-2260), which is what the boundary branch computes.
+// --------------------------------------------------------------------------
+struct ResArgs {
+  Geom g;
+  Phys f;
+  double dx, dy, dz, dt;
+  const double *__restrict__ p;
+  const double *__restrict__ p_old;
+  const double *__restrict__ s_old;
+  const double *__restrict__ kx;
+  const double *__restrict__ ky;
+  const double *__restrict__ kz;
+  const double *__restrict__ poro;
+  const double *__restrict__ ss;
+  const double *__restrict__ src;
+  const double *__restrict__ krho;
+  const double *__restrict__ qx;
+  const double *__restrict__ qy;
+  const double *bcv[6];
+  int bct[6];
+  double *__restrict__ F;
+};
+
+__device__ __forceinline__ double hmean(double a, double b)
+{
+  return (a + b) ? (2.0 * a * b) / (a + b) : 0;
+}
+__device__ __forceinline__ double hmean_dz1(double a, double b)
+{
+  // HarmonicMeanDZ(a, b, 1, 1), PL/solver.h:45
+  return ((1.0 * b) + (a * 1.0)) ? (((1.0 + 1.0) * a * b) / ((b * 1.0) + (a * 1.0))) : 0;
+}
+__device__ __forceinline__ double upw(double a, double b, double c, double d)
+{
+  return ((a - b) >= 0) ? c : d;
+}
+
+__global__ __launch_bounds__(256) void k_residual(ResArgs A)
+{
+  const Geom &g = A.g;
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const int sy = g.sy, sz = g.sz;
+  int ip = (i + 1) + (j + 1) * sy + (k + 1) * sz;
+  const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
+  const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
+  const double vol = dx * dy * dz;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
+  const double gc = grav;
+  const double *__restrict__ p = A.p;
+  const double *__restrict__ kr = A.krho;
+  double pc = p[ip];
+  double krc = kr[ip];
+  double S = vg_sat(pc, rho, grav, A.f.alpha[ip], A.f.nvg[ip], A.f.sres[ip], A.f.ssat[ip]);
+  double so = A.s_old[ip];
+  double fv = (S * rho - so * rho) * A.poro[ip] * vol;
+  fv += A.ss[ip] * vol * (pc * S * rho - A.p_old[ip] * so * rho);
+  if (A.src) fv -= vol * dt * (A.src[ip]);
+
+  double ur = 0.0, uf = 0.0, uu = 0.0;
+  double kxc = A.kx[ip], kyc = A.ky[ip], kzc = A.kz[ip];
+  if (i + 1 < g.nx) {
+    int b = ip + 1;
+    double diff = pc - p[b];
+    double updir = (diff / dx) * gc;
+    ur = ffx * hmean(kxc, A.kx[b]) * (diff / dx) * gc * upw(updir, 0.0, krc, kr[b]) / mu;
+  }
+  if (j + 1 < g.ny) {
+    int b = ip + sy;
+    double diff = pc - p[b];
+    double updir = (diff / dy) * gc;
+    uf = ffy * hmean(kyc, A.ky[b]) * (diff / dy) * gc * upw(updir, 0.0, krc, kr[b]) / mu;
+  }
+  if (k + 1 < g.nz) {
+    int b = ip + sz;
+    double lc = pc / dz - 0.5 * rho * grav * 1.0;
+    double uc = p[b] / dz + 0.5 * rho * grav * 1.0;
+    double diff = lc - uc;
+    uu = ffz * hmean_dz1(kzc, A.kz[b]) * diff * upw(lc, uc, krc, kr[b]) / mu;
+  }
+  fv += dt * (ur + uf + uu);
+  if (i > 0) {
+    int a = ip - 1;
+    double pa = p[a];
+    double diff = pa - pc;
+    double updir = (diff / dx) * gc;
+    double u = ffx * hmean(A.kx[a], kxc) * (diff / dx) * gc * upw(updir, 0.0, kr[a], krc) / mu;
+    fv += -dt * u;
+  }
+  if (j > 0) {
+    int a = ip - sy;
+    double pa = p[a];
+    double diff = pa - pc;
+    double updir = (diff / dy) * gc;
+    double u = ffy * hmean(A.ky[a], kyc) * (diff / dy) * gc * upw(updir, 0.0, kr[a], krc) / mu;
+    fv += -dt * u;
+  }
+  if (k > 0) {
+    int a = ip - sz;
+    double lc = p[a] / dz - 0.5 * rho * grav * 1.0;
+    double uc = pc / dz + 0.5 * rho * grav * 1.0;
+    double diff = lc - uc;
+    double u = ffz * hmean_dz1(A.kz[a], kzc) * diff * upw(lc, uc, kr[a], krc) / mu;
+    fv += -dt * u;
+  }
+
+  bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
+             (k == g.nz - 1);
+  if (bnd) {
+#pragma unroll 1
+    for (int fc = 0; fc < 6; fc++) {
+      bool on = (fc == 0 && i == 0) || (fc == 1 && i == g.nx - 1) ||
+                (fc == 2 && j == 0) || (fc == 3 && j == g.ny - 1) ||
+                (fc == 4 && k == 0) || (fc == 5 && k == g.nz - 1);
+      if (!on) continue;
+      int fi = (fc < 2) ? j + k * g.ny : (fc < 4) ? i + k * g.nx : i + j * g.nx;
+      double value = A.bcv[fc][fi];
+      int type = A.bct[fc];
+      double dir = (fc & 1) ? 1.0 : -1.0;
+      double h = (fc < 2) ? ffx : (fc < 4) ? ffy : ffz;
+      if (type == PFHIP_BC_FLUX) {
+        double u_new = h * value;
+        fv += dt * dir * u_new;
+      } else if (type == PFHIP_BC_DIR_EQUIL) {
+        double kg = vg_kr(value, rho, grav, A.f.alpha[ip], A.f.nvg[ip]) * rho;
+        double u_new;
+        if (fc == 0) {
+          double diff = value - pc;
+          u_new = upw(value, pc, kg, krc) * h * (kxc / mu) * 2.0 * (diff / dx);
+        } else if (fc == 1) {
+          double diff = pc - value;
+          u_new = upw(pc, value, krc, kg) * h * (kxc / mu) * 2.0 * (diff / dx);
+        } else if (fc == 2) {
+          double diff = value - pc;
+          u_new = upw(value, pc, kg, krc) * h * (kyc / mu) * 2.0 * (diff / dy);
+        } else if (fc == 3) {
+          double diff = pc - value;
+          u_new = upw(pc, value, krc, kg) * h * (kyc / mu) * 2.0 * (diff / dy);
+        } else if (fc == 4) {
+          double sep = dz * 1.0;
+          double lc = value / sep - 0.25 * rho * grav;
+          double uc = pc / sep + 0.25 * rho * grav;
+          double diff = lc - uc;
+          u_new = upw(lc, uc, kg, krc) * h * (kzc / mu) * 2.0 * diff;
+        } else {
+          double sep = dz * 1.0;
+          double lc = (pc / sep) - 0.25 * rho * grav * 1.0;
+          double uc = (value / sep) + 0.25 * rho * grav * 1.0;
+          double diff = lc - uc;
+          u_new = upw(lc, uc, krc, kg) * h * (kzc / mu) * 2.0 * diff;
+        }
+        fv += dt * dir * u_new;
+      } else if (fc == 5) {
+        // OverlandKinematicBC, PL/nl_function_eval.c:1786-2010
+        int io = (i + 1) + (j + 1) * sy;
+        double ke = A.qx[io], kw = A.qx[io - 1];
+        double kn = A.qy[io], ks = A.qy[io - sy];
+        double q_overlnd = vol * (fmax(pc, 0.0) - fmax(A.p_old[ip], 0.0)) / dz +
+                           dt * vol * ((ke - kw) / dx + (kn - ks) / dy) / dz;
+        fv += q_overlnd;
+        double u_new = h * value;
+        fv += dt * dir * u_new;
+      }
+    }
+  }
+  A.F[ip] = fv;
+}
+
+extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double *F,
+                                      const double *p_old, const double *s_old,
+                                      double dt, void *stream)
+{
+  Geom g = geom_of(pb);
+  int rc = launch_closure(pb, 4, p, pb->krho, nullptr, stream);
+  if (rc) return rc;
+  if (pb->overland) {
+    hipLaunchKernelGGL(k_overland_q, grid2(g), BLOCK3, 0, STREAM(stream), g,
+                       surf_of(pb), p, pb->q2[0], pb->q2[1]);
+    LAUNCHCHK();
+  }
+  ResArgs A;
+  A.g = g;
+  A.f = phys_of(pb);
+  A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
+  A.p = p; A.p_old = p_old; A.s_old = s_old;
+  A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
+  A.poro = pb->d.poro; A.ss = pb->d.ss; A.src = pb->d.src;
+  A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
+  for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
+  A.F = F;
+  hipLaunchKernelGGL(k_residual, grid3(g), BLOCK3, 0, STREAM(stream), A);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// --------------------------------------------------------------------------
+// Jacobian -- RichardsJacobianEval, PL/richards_jacobian_eval.c:217-2200,
+// gather form: each cell writes its own row.  PFSymmetric (symm_part)
+// keeps the full derivative on the diagonal and the symmetric part off the
+// diagonal (:594-776); Dirichlet faces :1049-1226; kinematic overland top
+// rows :1547-1644/:1938-1987 with the PFSymmetric top-row assembly of
+// PL/pf_hypre.c:360-420.
+// --------------------------------------------------------------------------
+struct JacArgs {
+  Geom g;
+  Phys f;
+  double dx, dy, dz, dt;
+  const double *__restrict__ p;
+  const double *__restrict__ kx;
+  const double *__restrict__ ky;
+  const double *__restrict__ kz;
+  const double *__restrict__ poro;
+  const double *__restrict__ ss;
+  const double *__restrict__ krho;
+  const double *__restrict__ dkrho;
+  const double *__restrict__ ke;
+  const double *__restrict__ kw;
+  const double *__restrict__ kn;
+  const double *__restrict__ ks;
+  const double *bcv[6];
+  int bct[6];
+  int symm;
+  int overland;
+  double *J[7];
+};
+
+__global__ __launch_bounds__(256) void k_jacobian(JacArgs A)
+{
+  const Geom &g = A.g;
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const int sy = g.sy, sz = g.sz;
+  int ip = (i + 1) + (j + 1) * sy + (k + 1) * sz;
+  const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
+  const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
+  const double vol = dx * dy * dz;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
+  const double gc = grav;
+  const double ddp = 0.0;
+  const double *__restrict__ p = A.p;
+  const double *__restrict__ kr = A.krho;
+  const double *__restrict__ dk = A.dkrho;
+  const bool symm = A.symm != 0;
+  double pc = p[ip];
+  double a = A.f.alpha[ip], nn = A.f.nvg[ip];
+  double S = vg_sat(pc, rho, grav, a, nn, A.f.sres[ip], A.f.ssat[ip]);
+  double dS = vg_dsat(pc, rho, grav, a, nn, A.f.sres[ip], A.f.ssat[ip]);
+  double vol2 = vol * 1.0;
+  double c = (dS * rho + S * ddp) * A.poro[ip] * vol2 +
+             A.ss[ip] * vol2 * (dS * rho * pc + S * ddp * pc + S * rho);
+  double offW = 0, offE = 0, offS = 0, offN = 0, offL = 0, offU = 0;
+  double krc = kr[ip], dkc = dk[ip];
+
+  if (i + 1 < g.nx) {
+    int b = ip + 1;
+    double diff = pc - p[b];
+    double updir = (diff / dx) * gc;
+    double xc = 1.0 * dt * ffx * (1.0 / dx) * 1.0 * hmean(A.kx[ip], A.kx[b]) / mu;
+    double sym = (-xc * upw(updir, 0.0, krc, kr[b])) * gc;
+    double west_temp = (-xc * diff * upw(updir, 0.0, dkc, 0.0)) * gc + sym;
+    c -= west_temp;
+    offE = symm ? sym : (xc * diff * upw(updir, 0.0, 0.0, dk[b])) * gc + sym;
+  }
+  if (i > 0) {
+    int aa = ip - 1;
+    double diff = p[aa] - pc;
+    double updir = (diff / dx) * gc;
+    double xc = 1.0 * dt * ffx * (1.0 / dx) * 1.0 * hmean(A.kx[aa], A.kx[ip]) / mu;
+    double sym = (-xc * upw(updir, 0.0, kr[aa], krc)) * gc;
+    double east_temp = (xc * diff * upw(updir, 0.0, 0.0, dkc)) * gc + sym;
+    c -= east_temp;
+    offW = symm ? sym : (-xc * diff * upw(updir, 0.0, dk[aa], 0.0)) * gc + sym;
+  }
+  if (j + 1 < g.ny) {
+    int b = ip + sy;
+    double diff = pc - p[b];
+    double updir = (diff / dy) * gc;
+    double yc = 1.0 * dt * ffy * (1.0 / dy) * 1.0 * hmean(A.ky[ip], A.ky[b]) / mu;
+    double sym_s = -yc * upw(updir, 0.0, krc, kr[b]) * gc;
+    double south_temp = -yc * diff * upw(updir, 0.0, dkc, 0.0) * gc + sym_s;
+    double sym_n = yc * -upw(updir, 0.0, krc, kr[b]) * gc;
+    c -= south_temp;
+    offN = symm ? sym_n : yc * diff * upw(updir, 0.0, 0.0, dk[b]) * gc + sym_n;
+  }
+  if (j > 0) {
+    int aa = ip - sy;
+    double diff = p[aa] - pc;
+    double updir = (diff / dy) * gc;
+    double yc = 1.0 * dt * ffy * (1.0 / dy) * 1.0 * hmean(A.ky[aa], A.ky[ip]) / mu;
+    double sym_s = -yc * upw(updir, 0.0, kr[aa], krc) * gc;
+    double sym_n = yc * -upw(updir, 0.0, kr[aa], krc) * gc;
+    double north_temp = yc * diff * upw(updir, 0.0, 0.0, dkc) * gc + sym_n;
+    c -= north_temp;
+    offS = symm ? sym_s : -yc * diff * upw(updir, 0.0, dk[aa], 0.0) * gc + sym_s;
+  }
+  const double sep = dz * 0.5 * (1.0 + 1.0);
+  if (k + 1 < g.nz) {
+    int b = ip + sz;
+    double lc = pc / sep - (1.0 / (1.0 + 1.0)) * rho * grav;
+    double uc = p[b] / sep + (1.0 / (1.0 + 1.0)) * rho * grav;
+    double diff = lc - uc;
+    double zc = 1.0 * dt * ffz * hmean_dz1(A.kz[ip], A.kz[b]) / mu;
+    double up = upw(lc, uc, krc, kr[b]);
+    double sym_l = -zc * (1.0 / (dz * 0.5 * (1.0 + 1.0))) * up;
+    double lower_temp = -zc * (diff * upw(lc, uc, dkc, 0.0) +
+                               (-grav * 0.5 * dz * (0.5 * (1.0 + 1.0)) * ddp * up)) +
+                        sym_l;
+    double sym_u = zc * (1.0 / (dz * 0.5 * (1.0 + 1.0))) * -up;
+    c -= lower_temp;
+    offU = symm ? sym_u
+                : zc * (diff * upw(lc, uc, 0.0, dk[b]) +
+                        (-grav * 0.5 * dz * (0.5 * (1.0 + 1.0)) * ddp * up)) +
+                      sym_u;
+  }
+  if (k > 0) {
+    int aa = ip - sz;
+    double lc = p[aa] / sep - (1.0 / (1.0 + 1.0)) * rho * grav;
+    double uc = pc / sep + (1.0 / (1.0 + 1.0)) * rho * grav;
+    double diff = lc - uc;
+    double zc = 1.0 * dt * ffz * hmean_dz1(A.kz[aa], A.kz[ip]) / mu;
+    double up = upw(lc, uc, kr[aa], krc);
+    double sym_l = -zc * (1.0 / (dz * 0.5 * (1.0 + 1.0))) * up;
+    double lower_temp = -zc * (diff * upw(lc, uc, dk[aa], 0.0) +
+                               (-grav * 0.5 * dz * (0.5 * (1.0 + 1.0)) * ddp * up)) +
+                        sym_l;
+    double sym_u = zc * (1.0 / (dz * 0.5 * (1.0 + 1.0))) * -up;
+    double upper_temp = zc * (diff * upw(lc, uc, 0.0, dkc) +
+                              (-grav * 0.5 * dz * (0.5 * (1.0 + 1.0)) * ddp * up)) +
+                        sym_u;
+    c -= upper_temp;
+    offL = symm ? sym_l : lower_temp;
+  }
+
+  bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
+             (k == g.nz - 1);
+  if (bnd) {
+#pragma unroll 1
+    for (int fc = 0; fc < 6; fc++) {
+      bool on = (fc == 0 && i == 0) || (fc == 1 && i == g.nx - 1) ||
+                (fc == 2 && j == 0) || (fc == 3 && j == g.ny - 1) ||
+                (fc == 4 && k == 0) || (fc == 5 && k == g.nz - 1);
+      if (!on || A.bct[fc] != PFHIP_BC_DIR_EQUIL) continue;
+      int fi = (fc < 2) ? j + k * g.ny : (fc < 4) ? i + k * g.nx : i + j * g.nx;
+      double value = A.bcv[fc][fi];
+      double den_d = rho;
+      double prod = krc, prod_der = dkc;
+      double prod_val = vg_kr(value, rho, grav, a, nn) * den_d;
+      double coeff, diff, o_temp;
+      if (fc == 0 || fc == 2) {
+        double ff = (fc == 0) ? ffx : ffy, dd = (fc == 0) ? dx : dy;
+        double kk = (fc == 0) ? A.kx[ip] : A.ky[ip];
+        coeff = dt * ff * 1.0 * (2.0 / dd) * kk / mu;
+        diff = value - pc;
+        o_temp = coeff * (diff * upw(value, pc, 0.0, prod_der) - upw(value, pc, prod_val, prod));
+      } else if (fc == 1 || fc == 3) {
+        double ff = (fc == 1) ? ffx : ffy, dd = (fc == 1) ? dx : dy;
+        double kk = (fc == 1) ? A.kx[ip] : A.ky[ip];
+        coeff = dt * ff * 1.0 * (2.0 / dd) * kk / mu;
+        diff = pc - value;
+        o_temp = -coeff * (diff * upw(pc, value, prod_der, 0.0) + upw(pc, value, prod, prod_val));
+      } else if (fc == 4) {
+        coeff = dt * ffz * (2.0 / (dz * 0.5 * (1.0 + 1.0))) * A.kz[ip] / mu;
+        double lc = (value) - 0.5 * dz * 1.0 * den_d * grav;
+        double uc = (pc) + 0.5 * dz * 1.0 * rho * grav;
+        diff = lc - uc;
+        o_temp = coeff * (diff * upw(lc, uc, 0.0, prod_der) +
+                          ((-1.0 - grav * 0.5 * dz * 1.0 * ddp) * upw(lc, uc, prod_val, prod)));
+      } else {
+        coeff = dt * ffz * (2.0 / (dz * 0.5 * (1.0 + 1.0))) * A.kz[ip] / mu;
+        double lc = (pc) - 0.5 * dz * 1.0 * rho * grav;
+        double uc = (value) + 0.5 * dz * 1.0 * den_d * grav;
+        diff = lc - uc;
+        o_temp = -coeff * (diff * upw(lc, uc, prod_der, 0.0) +
+                           ((1.0 - grav * 0.5 * dz * 1.0 * ddp) * upw(lc, uc, prod, prod_val)));
+      }
+      c -= o_temp;
+    }
+  }
+
+  if (A.overland && k == g.nz - 1) {
+    int io = (i + 1) + (j + 1) * sy;
+    double dvol = 0.0;
+    if (pc > 0.0)
+      dvol = (vol / dz) + (vol / ffy) * dt * (A.ke[io] - A.kw[io]) +
+             (vol / ffx) * dt * (A.kn[io] - A.ks[io]);
+    c += dvol;
+    if (symm) {
+      if (i + 1 < g.nx) offE = 0.0;
+      if (j + 1 < g.ny) offN = 0.0;
+    } else {
+      offW -= (vol / ffy) * dt * (A.ke[io - 1]);
+      offE += (vol / ffy) * dt * (A.kw[io + 1]);
+      offS -= (vol / ffx) * dt * (A.kn[io - sy]);
+      offN += (vol / ffx) * dt * (A.ks[io + sy]);
+      if (i == 0) offW = 0.0;
+      if (i == g.nx - 1) offE = 0.0;
+      if (j == 0) offS = 0.0;
+      if (j == g.ny - 1) offN = 0.0;
+    }
+  }
+  A.J[0][ip] = c;
+  A.J[2][ip] = offE;
+  A.J[4][ip] = offN;
+  A.J[6][ip] = offU;
+  if (!symm) {
+    A.J[1][ip] = offW;
+    A.J[3][ip] = offS;
+    A.J[5][ip] = offL;
+  }
+}
+
+extern "C" int pfhip_richards_jacobian_eval(pfhip_problem *pb, const double *p,
+                                            double dt, int symm_part,
+                                            double *const *J, void *stream)
+{
+  if (!J || !J[0] || !J[2] || !J[4] || !J[6] ||
+      (!symm_part && (!J[1] || !J[3] || !J[5]))) {
+    pfhip_set_error("jacobian_eval: missing coefficient arrays");
+    return PFHIP_ERR_ARG;
+  }
+  Geom g = geom_of(pb);
+  int rc = launch_closure(pb, 5, p, pb->krho, pb->dkrho, stream);
+  if (rc) return rc;
+  if (pb->overland) {
+    rc = pfhip_overland_flow_eval_kin(pb, p, pb->q2[0], pb->q2[1], pb->q2[2],
+                                      pb->q2[3], PFHIP_CALCDER, stream);
+    if (rc) return rc;
+  }
+  JacArgs A;
+  A.g = g;
+  A.f = phys_of(pb);
+  A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
+  A.p = p;
+  A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
+  A.poro = pb->d.poro; A.ss = pb->d.ss;
+  A.krho = pb->krho; A.dkrho = pb->dkrho;
+  A.ke = pb->q2[0]; A.kw = pb->q2[1]; A.kn = pb->q2[2]; A.ks = pb->q2[3];
+  for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
+  A.symm = symm_part;
+  A.overland = pb->overland;
+  for (int s = 0; s < 7; s++) A.J[s] = J[s];
+  hipLaunchKernelGGL(k_jacobian, grid3(g), BLOCK3, 0, STREAM(stream), A);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// --------------------------------------------------------------------------
+// 7-point matvec y = A x (PL/matvec.c:40-330), one pass instead of seven
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_matvec(Geom g, const double *__restrict__ C,
+                                                const double *__restrict__ W,
+                                                const double *__restrict__ E,
+                                                const double *__restrict__ S,
+                                                const double *__restrict__ N,
+                                                const double *__restrict__ L,
+                                                const double *__restrict__ U, int symm,
+                                                const double *__restrict__ x,
+                                                double *__restrict__ y)
+{
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const int sy = g.sy, sz = g.sz;
+  int ip = (i + 1) + (j + 1) * sy + (k + 1) * sz;
+  double v = C[ip] * x[ip];
+  if (i > 0) v += (symm ? E[ip - 1] : W[ip]) * x[ip - 1];
+  if (i + 1 < g.nx) v += E[ip] * x[ip + 1];
+  if (j > 0) v += (symm ? N[ip - sy] : S[ip]) * x[ip - sy];
+  if (j + 1 < g.ny) v += N[ip] * x[ip + sy];
+  if (k > 0) v += (symm ? U[ip - sz] : L[ip]) * x[ip - sz];
+  if (k + 1 < g.nz) v += U[ip] * x[ip + sz];
+  y[ip] = v;
+}
+
+extern "C" int pfhip_matvec(pfhip_problem *pb, const double *const *A, int symm,
+                            const double *x, double *y, void *stream)
+{
+  Geom g = geom_of(pb);
+  hipLaunchKernelGGL(k_matvec, grid3(g), BLOCK3, 0, STREAM(stream), g, A[0],
+                     symm ? A[2] : A[1], A[2], symm ? A[4] : A[3], A[4],
+                     symm ? A[6] : A[5], A[6], symm, x, y);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}

@@ -1,0 +1,705 @@
+/*
+ * kinsol_host.c -- host driver of the Newton-Krylov solve on device vectors.
+ *
+ * Restates the embedded KINSOL that ParFlow builds when SUNDIALS is absent:
+ *   KINSolInit KS/kinsol.c:363-650, KINSol loop :690-860,
+ *   KINForcingTerm :1054-1090, KINInitialStop :1128-1137,
+ *   KINInexactNewton :1152-1220, KINLineSearch :1228-1405,
+ *   KINLinSolDrv :1424-1470, KINStop :1507-1580,
+ *   KINSpgmrSolve / KINSpgmrAtimesDQ KS/kinspgmr.c:344-531,
+ *   SpgmrSolve KS/spgmr.c:168-450, ModifiedGS/QRfact/QRsol KS/iterativ.c,
+ * with ParFlow's callbacks (PL/kinsol_nonlin_solver.c:144-236,
+ * PL/kinsol_pc.c:58-178, PL/richards_jacobian_eval.c:165-211) and the
+ * AdvanceRichards step contract (PL/solver_richards.c:2970-3400).
+ *
+ * Every vector lives on the device; this file only sees scalars.  Two
+ * Krylov variants:
+ *   krylov = 0  right-preconditioned GMRES with modified Gram-Schmidt, the
+ *               reference's algorithm (each MGS step is one fused
+ *               axpy+dot pass, same arithmetic order);
+ *   krylov = 1  flexible GMRES with block classical Gram-Schmidt applied
+ *               twice (CGS2): two multi-dot passes per iteration instead of
+ *               l+1 dependent dots, and the preconditioned vectors kept so
+ *               an FP32 V-cycle can change between applications.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "pfhip_internal.h"
+
+struct pfhip_solver {
+  pfhip_problem *pb;
+  pfhip_solver_opts o;
+  long ng;
+  double *unew, *fval, *pp, *vtemp1, *vtemp2, *xcor, *vtemp;
+  double **V, **Z;
+  double *J[7], *P[7];
+  pfhip_mg *mg;
+  double *p_old, *s_old;        /* advance_richards state                 */
+  /* host Krylov state */
+  double **Hes, *givens, *yg, *hbuf;
+  int l_max, max_restarts;
+  /* KINSOL scalars */
+  double *uu;
+  const double *cur_p_old, *cur_s_old;
+  double dt;
+  double fnorm, f1norm, eta, ealpha, egamma, sJpnorm, sfdotJp, stepl;
+  double mxnewtstep, sqrt_relfunc, uround, fnormtol, scsteptol, eps, pthrsh;
+  int etaflag, callForcingTerm, nni, nnilpre, nfe, nbcf, nbktrk, ncscmx;
+  int precondcurrent, mxiter, nli, nps, npe, ncfl, new_uu;
+};
+
+#define CHK(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
+
+static int alloc_vec(pfhip_solver *s, double **p)
+{
+  CHK(pfhip_dev_alloc(p, s->ng));
+  return pfhip_zero(*p, s->ng, NULL);
+}
+
+int pfhip_solver_destroy(pfhip_solver *s)
+{
+  if (!s) return PFHIP_OK;
+  double *v[9] = { s->unew, s->fval, s->pp, s->vtemp1, s->vtemp2, s->xcor,
+                   s->vtemp, s->p_old, s->s_old };
+  for (int i = 0; i < 9; i++) pfhip_free(v[i]);
+  if (s->V) for (int i = 0; i <= s->l_max; i++) pfhip_free(s->V[i]);
+  if (s->Z) for (int i = 0; i < s->l_max; i++) pfhip_free(s->Z[i]);
+  for (int q = 0; q < 7; q++) { pfhip_free(s->J[q]); pfhip_free(s->P[q]); }
+  if (s->mg) pfhip_mg_destroy(s->mg);
+  if (s->Hes) for (int i = 0; i <= s->l_max; i++) free(s->Hes[i]);
+  free(s->Hes); free(s->givens); free(s->yg); free(s->hbuf);
+  free(s->V); free(s->Z);
+  free(s);
+  return PFHIP_OK;
+}
+
+int pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
+                        pfhip_solver **out)
+{
+  if (!pb || !opts || !out) { pfhip_set_error("null argument"); return PFHIP_ERR_ARG; }
+  if (opts->krylov_dim < 1 || opts->krylov_dim > PFHIP_MAX_MULTI - 2) {
+    pfhip_set_error("krylov_dim must be 1..%d", PFHIP_MAX_MULTI - 2);
+    return PFHIP_ERR_ARG;
+  }
+  if (opts->krylov == 1 && !opts->precond) {
+    /* FGMRES without a preconditioner is plain GMRES: allowed */
+  }
+  pfhip_solver *s = (pfhip_solver *)calloc(1, sizeof(pfhip_solver));
+  s->pb = pb;
+  s->o = *opts;
+  s->ng = pb->ng;
+  s->l_max = opts->krylov_dim;
+  s->max_restarts = opts->max_restarts < 0 ? 0 : opts->max_restarts;
+  int rc = PFHIP_OK;
+  double **vv[9] = { &s->unew, &s->fval, &s->pp, &s->vtemp1, &s->vtemp2,
+                     &s->xcor, &s->vtemp, &s->p_old, &s->s_old };
+  for (int i = 0; i < 9 && !rc; i++) rc = alloc_vec(s, vv[i]);
+  s->V = (double **)calloc(s->l_max + 1, sizeof(double *));
+  for (int i = 0; i <= s->l_max && !rc; i++) rc = alloc_vec(s, &s->V[i]);
+  if (opts->krylov == 1) {
+    s->Z = (double **)calloc(s->l_max, sizeof(double *));
+    for (int i = 0; i < s->l_max && !rc; i++) rc = alloc_vec(s, &s->Z[i]);
+  }
+  for (int q = 0; q < 7 && !rc; q++) {
+    if (opts->use_jacobian) rc = alloc_vec(s, &s->J[q]);
+    if (!rc && opts->precond && (q == 0 || q == 2 || q == 4 || q == 6))
+      rc = alloc_vec(s, &s->P[q]);
+  }
+  if (!rc && opts->precond)
+    rc = pfhip_mg_create(pb, opts->mg_max_levels, opts->mg_pre, opts->mg_post,
+                         opts->mg_symmetric, opts->mg_precision ? opts->mg_precision : 64,
+                         &s->mg);
+  s->Hes = (double **)calloc(s->l_max + 1, sizeof(double *));
+  for (int i = 0; i <= s->l_max; i++) s->Hes[i] = (double *)calloc(s->l_max, sizeof(double));
+  s->givens = (double *)calloc(2 * s->l_max, sizeof(double));
+  s->yg = (double *)calloc(s->l_max + 1, sizeof(double));
+  s->hbuf = (double *)calloc(2 * (s->l_max + 2), sizeof(double));
+  if (!rc) rc = pfhip_synchronize();
+  if (rc) { pfhip_solver_destroy(s); return rc; }
+  *out = s;
+  return PFHIP_OK;
+}
+
+/* ---- callbacks -------------------------------------------------------- */
+static int func(pfhip_solver *s, const double *u, double *f)
+{
+  s->nfe++;
+  return pfhip_nl_function_eval(s->pb, u, f, s->cur_p_old, s->cur_s_old, s->dt, NULL);
+}
+
+static int l2norm(pfhip_solver *s, const double *x, double *r)
+{
+  double d;
+  CHK(pfhip_vec_dot(s->pb, x, x, &d, NULL));
+  *r = sqrt(d);
+  return PFHIP_OK;
+}
+
+/* KINSolInitPC -> KinsolPCInitInstanceXtra: PC matrix + multigrid setup */
+static int pc_setup(pfhip_solver *s)
+{
+  CHK(pfhip_richards_jacobian_eval(s->pb, s->uu, s->dt, 1, s->P, NULL));
+  CHK(pfhip_mg_setup(s->mg, (const double *const *)s->P, NULL));
+  s->npe++;
+  s->nnilpre = s->nni;
+  s->precondcurrent = 1;
+  return PFHIP_OK;
+}
+
+/* KinsolPC: zero-guess V-cycle; z = M^{-1} r (r, z may alias) */
+static int psolve(pfhip_solver *s, const double *r, double *z)
+{
+  s->nps++;
+  if (r == z) {
+    CHK(pfhip_vec_scale(s->pb, 1.0, r, s->vtemp2, NULL));
+    return pfhip_mg_apply(s->mg, s->vtemp2, z, NULL);
+  }
+  return pfhip_mg_apply(s->mg, r, z, NULL);
+}
+
+/* KINSpgmrAtimes: KINSolMatVec or difference quotient KS/kinspgmr.c:481-531 */
+static int atimes(pfhip_solver *s, const double *v, double *z)
+{
+  pfhip_problem *pb = s->pb;
+  if (s->o.use_jacobian) {
+    if (s->new_uu) {
+      CHK(pfhip_richards_jacobian_eval(pb, s->uu, s->dt, 0, s->J, NULL));
+      s->new_uu = 0;
+    }
+    return pfhip_matvec(pb, (const double *const *)s->J, 0, v, z, NULL);
+  }
+  double sutsv, vtv, sq1norm;
+  CHK(pfhip_vec_dq_sums(pb, s->uu, v, &sutsv, &vtv, &sq1norm, NULL));
+  double sign = (sutsv >= 0.0) ? 1.0 : -1.0;
+  double a = fabs(sutsv);
+  double sigma = sign * s->sqrt_relfunc * ((a > sq1norm) ? a : sq1norm) / vtv;
+  double sigma_inv = 1.0 / sigma;
+  CHK(pfhip_vec_linear_sum(pb, 1.0, s->uu, sigma, v, s->vtemp1, NULL));
+  CHK(func(s, s->vtemp1, s->vtemp2));
+  return pfhip_vec_linear_sum(pb, sigma_inv, s->vtemp2, -sigma_inv, s->fval, z, NULL);
+}
+
+/* ModifiedGS (KS/iterativ.c:30-73), each step one fused axpy+dot pass */
+static int modified_gs(pfhip_solver *s, int k, double *new_vk_norm)
+{
+  pfhip_problem *pb = s->pb;
+  double **v = s->V, **h = s->Hes;
+  double vk2;
+  CHK(pfhip_vec_dot(pb, v[k], v[k], &vk2, NULL));
+  double vk_norm = sqrt(vk2);
+  int km1 = k - 1;
+  double d;
+  CHK(pfhip_vec_axpy_dot(pb, 0.0, NULL, v[k], v[0], &d, NULL));
+  h[0][km1] = d;
+  for (int i = 1; i < k; i++) {
+    CHK(pfhip_vec_axpy_dot(pb, h[i - 1][km1], v[i - 1], v[k], v[i], &d, NULL));
+    h[i][km1] = d;
+  }
+  double n2;
+  CHK(pfhip_vec_axpy_dot(pb, h[k - 1][km1], v[k - 1], v[k], v[k], &n2, NULL));
+  *new_vk_norm = sqrt(n2);
+  double temp = 1000.0 * vk_norm;
+  if ((temp + (*new_vk_norm)) != temp) return PFHIP_OK;
+  double new_norm_2 = 0.0;
+  for (int i = 0; i < k; i++) {
+    double np_;
+    CHK(pfhip_vec_dot(pb, v[i], v[k], &np_, NULL));
+    temp = 1000.0 * h[i][km1];
+    if ((temp + np_) == temp) continue;
+    h[i][km1] += np_;
+    CHK(pfhip_vec_linear_sum(pb, 1.0, v[k], -np_, v[i], v[k], NULL));
+    new_norm_2 += np_ * np_;
+  }
+  if (new_norm_2 != 0.0) {
+    double t = (*new_vk_norm) * (*new_vk_norm) - new_norm_2;
+    *new_vk_norm = (t > 0.0) ? sqrt(t) : 0.0;
+  }
+  return PFHIP_OK;
+}
+
+/* CGS2: two passes of block classical Gram-Schmidt against V[0..k-1] */
+static int cgs2(pfhip_solver *s, int k, double *new_vk_norm)
+{
+  pfhip_problem *pb = s->pb;
+  double *h1 = s->hbuf, *h2 = s->hbuf + (s->l_max + 2);
+  int km1 = k - 1;
+  double n2;
+  CHK(pfhip_vec_multi_dot(pb, (const double *const *)s->V, k, s->V[k], h1, NULL));
+  CHK(pfhip_vec_multi_axpy(pb, (const double *const *)s->V, k, h1, s->V[k], NULL, NULL));
+  CHK(pfhip_vec_multi_dot(pb, (const double *const *)s->V, k, s->V[k], h2, NULL));
+  CHK(pfhip_vec_multi_axpy(pb, (const double *const *)s->V, k, h2, s->V[k], &n2, NULL));
+  for (int i = 0; i < k; i++) s->Hes[i][km1] = h1[i] + h2[i];
+  *new_vk_norm = sqrt(n2 > 0.0 ? n2 : 0.0);
+  return PFHIP_OK;
+}
+
+static int qrfact(int n, double **h, double *q)
+{
+  int nm1 = n - 1, code = 0;
+  for (int k = 0; k < nm1; k++) {
+    int i = 2 * k;
+    double t1 = h[k][nm1], t2 = h[k + 1][nm1], c = q[i], sn = q[i + 1];
+    h[k][nm1] = c * t1 - sn * t2;
+    h[k + 1][nm1] = sn * t1 + c * t2;
+  }
+  double t1 = h[nm1][nm1], t2 = h[n][nm1], c, sn;
+  if (t2 == 0.0) { c = 1.0; sn = 0.0; }
+  else if (fabs(t2) >= fabs(t1)) {
+    double t3 = t1 / t2;
+    sn = -1.0 / sqrt(1.0 + t3 * t3);
+    c = -sn * t3;
+  } else {
+    double t3 = t2 / t1;
+    c = 1.0 / sqrt(1.0 + t3 * t3);
+    sn = -c * t3;
+  }
+  q[2 * nm1] = c;
+  q[2 * nm1 + 1] = sn;
+  if ((h[nm1][nm1] = c * t1 - sn * t2) == 0.0) code = n;
+  return code;
+}
+
+static int qrsol(int n, double **h, double *q, double *b)
+{
+  for (int k = 0; k < n; k++) {
+    double c = q[2 * k], sn = q[2 * k + 1], t1 = b[k], t2 = b[k + 1];
+    b[k] = c * t1 - sn * t2;
+    b[k + 1] = sn * t1 + c * t2;
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    if (h[k][k] == 0.0) return k + 1;
+    b[k] /= h[k][k];
+    for (int i = 0; i < k; i++) b[i] -= b[k] * h[i][k];
+  }
+  return 0;
+}
+
+#define SPGMR_SUCCESS 0
+#define SPGMR_RES_REDUCED 1
+#define SPGMR_CONV_FAIL 2
+#define SPGMR_QRFACT_FAIL 5
+
+/* SpgmrSolve (KS/spgmr.c:168-450), right preconditioning, s1 = s2 = 1;
+ * with krylov = 1 the flexible variant keeps Z[l] = M^{-1} V[l]. */
+static int spgmr_solve(pfhip_solver *s, double *x, const double *b, double delta,
+                       double *res_norm, int *nli, int *ret)
+{
+  pfhip_problem *pb = s->pb;
+  double **V = s->V, **Hes = s->Hes;
+  int l_max = s->l_max, flex = (s->o.krylov == 1);
+  int preOnRight = s->o.precond != 0;
+  double beta, rotation_product, r_norm, s_product, rho = 0;
+  int krydim = 0, converged = 0;
+  *nli = 0;
+  CHK(pfhip_vec_scale(pb, 1.0, b, V[0], NULL));
+  CHK(l2norm(s, V[0], &r_norm));
+  *res_norm = beta = r_norm;
+  if (r_norm <= delta) { *ret = SPGMR_SUCCESS; return PFHIP_OK; }
+  CHK(pfhip_vec_const(pb, 0.0, s->xcor, NULL));
+
+  for (int ntries = 0; ntries <= s->max_restarts; ntries++) {
+    for (int i = 0; i <= l_max; i++)
+      for (int j = 0; j < l_max; j++) Hes[i][j] = 0.0;
+    rotation_product = 1.0;
+    CHK(pfhip_vec_scale(pb, 1.0 / r_norm, V[0], V[0], NULL));
+    int l;
+    for (l = 0; l < l_max; l++) {
+      (*nli)++;
+      krydim = l + 1;
+      const double *av = V[l];
+      if (preOnRight) {
+        double *z = flex ? s->Z[l] : s->vtemp;
+        CHK(psolve(s, V[l], z));
+        av = z;
+      }
+      CHK(atimes(s, av, V[l + 1]));
+      if (flex) CHK(cgs2(s, l + 1, &Hes[l + 1][l]));
+      else CHK(modified_gs(s, l + 1, &Hes[l + 1][l]));
+      if (qrfact(krydim, Hes, s->givens) != 0) { *ret = SPGMR_QRFACT_FAIL; return PFHIP_OK; }
+      rotation_product *= s->givens[2 * l + 1];
+      *res_norm = rho = fabs(rotation_product * r_norm);
+      if (rho <= delta) { converged = 1; break; }
+      CHK(pfhip_vec_scale(pb, 1.0 / Hes[l + 1][l], V[l + 1], V[l + 1], NULL));
+    }
+    s->yg[0] = r_norm;
+    for (int i = 1; i <= krydim; i++) s->yg[i] = 0.0;
+    if (qrsol(krydim, Hes, s->givens, s->yg) != 0) { *ret = SPGMR_QRFACT_FAIL; return PFHIP_OK; }
+    if (flex) {
+      /* x += Z y directly (flexible GMRES) */
+      CHK(pfhip_vec_combine(pb, (const double *const *)s->Z, krydim, s->yg, x, NULL));
+    } else {
+      for (int k = 0; k < krydim; k++)
+        CHK(pfhip_vec_linear_sum(pb, s->yg[k], V[k], 1.0, s->xcor, s->xcor, NULL));
+    }
+    if (converged) {
+      if (!flex) {
+        if (preOnRight) CHK(psolve(s, s->xcor, s->vtemp));
+        else CHK(pfhip_vec_scale(pb, 1.0, s->xcor, s->vtemp, NULL));
+        CHK(pfhip_vec_linear_sum(pb, 1.0, x, 1.0, s->vtemp, x, NULL));
+      }
+      *ret = SPGMR_SUCCESS;
+      return PFHIP_OK;
+    }
+    if (ntries == s->max_restarts) break;
+    s_product = 1.0;
+    for (int i = krydim; i > 0; i--) {
+      s->yg[i] = s_product * s->givens[2 * i - 2];
+      s_product *= s->givens[2 * i - 1];
+    }
+    s->yg[0] = s_product;
+    r_norm *= s_product;
+    for (int i = 0; i <= krydim; i++) s->yg[i] *= r_norm;
+    r_norm = fabs(r_norm);
+    CHK(pfhip_vec_scale(pb, s->yg[0], V[0], V[0], NULL));
+    for (int k = 1; k <= krydim; k++)
+      CHK(pfhip_vec_linear_sum(pb, s->yg[k], V[k], 1.0, V[0], V[0], NULL));
+  }
+  if (rho < beta) {
+    if (!flex) {
+      if (preOnRight) CHK(psolve(s, s->xcor, s->vtemp));
+      else CHK(pfhip_vec_scale(pb, 1.0, s->xcor, s->vtemp, NULL));
+      CHK(pfhip_vec_linear_sum(pb, 1.0, x, 1.0, s->vtemp, x, NULL));
+    }
+    *ret = SPGMR_RES_REDUCED;
+    return PFHIP_OK;
+  }
+  *ret = SPGMR_CONV_FAIL;
+  return PFHIP_OK;
+}
+
+/* KINSpgmrSolve KS/kinspgmr.c:344-420 ; *lret = KINSOL-level return */
+static int lsolve(pfhip_solver *s, double *xx, double *bb, int *lret)
+{
+  pfhip_problem *pb = s->pb;
+  int nli_inc, sret;
+  double res_norm;
+  CHK(pfhip_vec_const(pb, 0.0, xx, NULL));
+  s->new_uu = 1;
+  CHK(spgmr_solve(s, xx, bb, s->eps, &res_norm, &nli_inc, &sret));
+  s->nli += nli_inc;
+  if (sret != 0) s->ncfl++;
+  CHK(atimes(s, xx, bb));
+  CHK(l2norm(s, bb, &s->sJpnorm));
+  CHK(pfhip_vec_dot(pb, s->fval, bb, &s->sfdotJp, NULL));
+  *lret = (sret == SPGMR_SUCCESS || sret == SPGMR_RES_REDUCED) ? 0 : 8;
+  return PFHIP_OK;
+}
+
+/* KINLinSolDrv KS/kinsol.c:1424-1470 */
+static int linsoldrv(pfhip_solver *s, double *bb, double *xx, int *ret)
+{
+  int setupNonNull = s->o.precond != 0;
+  if (s->nni - s->nnilpre >= 1) s->pthrsh = 2.0;
+  for (;;) {
+    s->precondcurrent = 0;
+    if ((s->pthrsh > 1.5) && setupNonNull) CHK(pc_setup(s));
+    CHK(pfhip_vec_scale(s->pb, -1.0, s->fval, bb, NULL));
+    int lret;
+    CHK(lsolve(s, xx, bb, &lret));
+    if (lret != 1) { *ret = lret; return PFHIP_OK; }
+    if (!setupNonNull || s->precondcurrent) { *ret = 8; return PFHIP_OK; }
+    s->pthrsh = 2.0;
+  }
+}
+
+static int eval_unew(pfhip_solver *s, double rl, double *fnormp, double *f1normp)
+{
+  CHK(pfhip_vec_linear_sum(s->pb, 1.0, s->uu, rl, s->pp, s->unew, NULL));
+  CHK(func(s, s->unew, s->fval));
+  CHK(l2norm(s, s->fval, fnormp));
+  *f1normp = 0.5 * (*fnormp) * (*fnormp);
+  return PFHIP_OK;
+}
+
+/* KINLineSearch KS/kinsol.c:1228-1405 (no constraints) */
+static int linesearch(pfhip_solver *s, double *fnormp, double *f1normp, int *maxStep,
+                      int *gret)
+{
+  pfhip_problem *pb = s->pb;
+  double pnorm, ratio = 1.0, alpha = 0.0001, beta = 0.9;
+  double slpi, rlmin, rlength, rl, rlmax, rldiff, rltmp, rlprev = 0.0;
+  double pt1trl, rllo, rlincr, alpha_cond, beta_cond = 0.0;
+  *maxStep = 0;
+  *gret = 0;
+  CHK(l2norm(s, s->pp, &pnorm));
+  if (pnorm > s->mxnewtstep) {
+    ratio = s->mxnewtstep / pnorm;
+    CHK(pfhip_vec_scale(pb, ratio, s->pp, s->pp, NULL));
+    pnorm = s->mxnewtstep;
+  }
+  s->stepl = pnorm;
+  slpi = s->sfdotJp * ratio;
+  CHK(pfhip_vec_scsteplength(pb, s->uu, s->pp, &rlength, NULL));
+  rlmin = s->scsteptol / rlength;
+  rl = 1.0;
+  int nfesav = s->nfe;
+  for (;;) {
+    CHK(eval_unew(s, rl, fnormp, f1normp));
+    alpha_cond = s->f1norm + alpha * slpi * rl;
+    if ((*f1normp) <= alpha_cond) break;
+    if (rl < rlmin) {
+      CHK(pfhip_vec_scale(pb, 1.0, s->uu, s->unew, NULL));
+      CHK(func(s, s->unew, s->fval));
+      CHK(l2norm(s, s->fval, fnormp));
+      *f1normp = 0.5 * (*fnormp) * (*fnormp);
+      s->nbktrk = s->nfe - nfesav - 1;
+      *gret = 1;
+      return PFHIP_OK;
+    }
+    rltmp = -slpi / (2.0 * ((*f1normp) - s->f1norm - slpi));
+    if (rltmp > 0.5 * rl) rltmp = 0.5 * rl;
+    rlprev = rl;
+    pt1trl = 0.1 * rl;
+    rl = (pt1trl > rltmp) ? pt1trl : rltmp;
+  }
+  beta_cond = s->f1norm + beta * slpi * rl;
+  if (*f1normp < beta_cond) {
+    if (rl == 1.0 && pnorm < s->stepl) {
+      rlmax = s->stepl / pnorm;
+      do {
+        rlprev = rl;
+        rl = (2.0 * rl < rlmax) ? 2.0 * rl : rlmax;
+        CHK(eval_unew(s, rl, fnormp, f1normp));
+        alpha_cond = s->f1norm + alpha * slpi * rl;
+        beta_cond = s->f1norm + beta * slpi * rl;
+      } while ((*f1normp) <= alpha_cond && (*f1normp) < beta_cond && (rl < rlmax));
+    }
+    if (rl < 1.0 || ((rl > 1.0) && (*f1normp > alpha_cond))) {
+      rllo = (rl < rlprev) ? rl : rlprev;
+      rldiff = fabs(rlprev - rl);
+      do {
+        rlincr = 0.5 * rldiff;
+        rl = rllo + rlincr;
+        CHK(eval_unew(s, rl, fnormp, f1normp));
+        alpha_cond = s->f1norm + alpha * slpi * rl;
+        beta_cond = s->f1norm + beta * slpi * rl;
+        if ((*f1normp) > alpha_cond) rldiff = rlincr;
+        else if (*f1normp < beta_cond) {
+          rllo = rl;
+          rldiff = rldiff - rlincr;
+        }
+      } while (*f1normp > alpha_cond || ((*f1normp < beta_cond) && (rldiff > rlmin)));
+      if ((*f1normp) < beta_cond) {
+        CHK(eval_unew(s, rllo, fnormp, f1normp));
+        s->nbcf++;
+      }
+    }
+  }
+  s->nbktrk = s->nfe - nfesav - 1;
+  s->sfdotJp = s->sfdotJp * rl * ratio;
+  s->sJpnorm = s->sJpnorm * rl * ratio;
+  if (rl * pnorm > 0.99 * s->mxnewtstep) *maxStep = 1;
+  return PFHIP_OK;
+}
+
+/* KINInexactNewton KS/kinsol.c:1152-1220 */
+static int inexact_newton(pfhip_solver *s, double *fnormp, double *f1normp, int *maxStep)
+{
+  double pnorm, ratio = 1.0;
+  *maxStep = 0;
+  CHK(l2norm(s, s->pp, &pnorm));
+  if (pnorm > s->mxnewtstep) {
+    ratio = s->mxnewtstep / pnorm;
+    CHK(pfhip_vec_scale(s->pb, ratio, s->pp, s->pp, NULL));
+    pnorm = s->mxnewtstep;
+  }
+  s->stepl = pnorm;
+  s->sfdotJp *= ratio;
+  s->sJpnorm *= ratio;
+  CHK(eval_unew(s, 1.0, fnormp, f1normp));
+  if (pnorm > 0.99 * s->mxnewtstep) *maxStep = 1;
+  return PFHIP_OK;
+}
+
+/* KINForcingTerm KS/kinsol.c:1054-1090; etaflag 1 = Walker1, 2 = Walker2 */
+static void forcing_term(pfhip_solver *s, double fnormp)
+{
+  double eta_max = 0.9, eta_min = 0.0001, eta_safe = 0.5;
+  if (s->etaflag == 1) {
+    double lm = sqrt(s->fnorm * s->fnorm + 2.0 * s->sfdotJp + s->sJpnorm * s->sJpnorm);
+    eta_safe = (s->eta <= 0.0) ? 0.0 : pow(s->eta, s->ealpha);
+    s->eta = fabs(fnormp - lm) / s->fnorm;
+  }
+  if (s->etaflag == 2) {
+    eta_safe = s->egamma * ((s->eta <= 0.0) ? 0.0 : pow(s->eta, s->ealpha));
+    double r = fnormp / s->fnorm;
+    s->eta = s->egamma * ((r <= 0.0) ? 0.0 : pow(r, s->ealpha));
+  }
+  if (eta_safe < 0.1) eta_safe = 0.0;
+  if (s->eta < eta_safe) s->eta = eta_safe;
+  if (s->eta < eta_min) s->eta = eta_min;
+  if (s->eta > eta_max) s->eta = eta_max;
+}
+
+/* KINStop KS/kinsol.c:1507-1580 */
+static int kin_stop(pfhip_solver *s, int maxStepTaken, int gret, int *ret)
+{
+  int setupNonNull = s->o.precond != 0;
+  *ret = 0;
+  if (gret == 1) {
+    if (setupNonNull && !s->precondcurrent) { s->pthrsh = 2.0; return PFHIP_OK; }
+    *ret = 3;
+    return PFHIP_OK;
+  }
+  double fmax, rlength;
+  CHK(pfhip_vec_max_norm(s->pb, s->fval, &fmax, NULL));
+  if (fmax <= s->fnormtol) { *ret = 1; return PFHIP_OK; }
+  CHK(pfhip_vec_linear_sum(s->pb, 1.0, s->unew, -1.0, s->uu, s->vtemp1, NULL));
+  CHK(pfhip_vec_scsteplength(s->pb, s->unew, s->vtemp1, &rlength, NULL));
+  if (rlength <= s->scsteptol) {
+    if (!s->precondcurrent) { s->pthrsh = 2.0; return PFHIP_OK; }
+    *ret = 2;
+    return PFHIP_OK;
+  }
+  if (s->nni >= s->mxiter) { *ret = 4; return PFHIP_OK; }
+  if (maxStepTaken) s->ncscmx++;
+  else s->ncscmx = 0;
+  if (s->ncscmx == 5) { *ret = 5; return PFHIP_OK; }
+  s->pthrsh = rlength;
+  return PFHIP_OK;
+}
+
+static double unit_roundoff(void)
+{
+  double u = 1.0;
+  volatile double one_plus_u = 1.0 + u;
+  while (one_plus_u != 1.0) { u /= 2.0; one_plus_u = 1.0 + u; }
+  return u * 2.0;
+}
+
+int pfhip_kinsol_nonlin_solver(pfhip_solver *s, double *p, const double *p_old,
+                               const double *s_old, double dt, double time,
+                               pfhip_kin_stats *st)
+{
+  pfhip_problem *pb = s->pb;
+  const pfhip_solver_opts *o = &s->o;
+  CHK(pfhip_bc_pressure(pb, time, NULL));
+  s->uu = p;
+  s->cur_p_old = p_old;
+  s->cur_s_old = s_old;
+  s->dt = dt;
+  s->nni = s->nnilpre = s->nfe = s->nbcf = s->nbktrk = s->ncscmx = 0;
+  s->nli = s->nps = s->npe = s->ncfl = 0;
+
+  /* KINSolInit */
+  s->uround = unit_roundoff();
+  s->fnormtol = o->residual_tol > 0.0 ? o->residual_tol : pow(s->uround, 1.0 / 3.0);
+  s->scsteptol = o->step_tol > 0.0 ? o->step_tol : pow(s->uround, 2.0 / 3.0);
+  s->mxiter = o->max_iter > 0 ? o->max_iter : 200;
+  s->sqrt_relfunc = (o->deriv_eps > 0.0) ? sqrt(o->deriv_eps) : sqrt(s->uround);
+  double un;
+  CHK(l2norm(s, p, &un));
+  s->mxnewtstep = 1000.0 * un;
+  if (s->mxnewtstep < 1.0) s->mxnewtstep = 1.0;
+  /* eta setup, KS/kinsol.c:526-575 */
+  if (o->eta_choice == 0) {
+    s->etaflag = 0;
+    s->callForcingTerm = 0;
+    s->eta = o->eta_value;
+  } else if (o->eta_choice == 1) {
+    s->etaflag = 1;
+    s->callForcingTerm = 1;
+    s->ealpha = (1.0 + sqrt(5.0)) / 2.0;
+    s->eta = 0.5;
+  } else {
+    s->etaflag = 2;
+    s->callForcingTerm = 1;
+    s->ealpha = o->eta_alpha;
+    s->egamma = o->eta_gamma;
+    if (s->ealpha <= 1.0 || s->ealpha > 2.0) s->ealpha = 2.0;
+    if (s->egamma <= 0.0 || s->egamma > 1.0) s->egamma = 0.9;
+    s->eta = 0.5;
+  }
+  if (s->eta <= 0.0 || s->eta > 1.0) s->eta = 0.1;
+
+  int ret = 0;
+  /* KINInitialStop */
+  CHK(pfhip_vec_scale(pb, 1.0, p, s->unew, NULL));
+  CHK(func(s, p, s->fval));
+  double fmax;
+  CHK(pfhip_vec_max_norm(pb, s->fval, &fmax, NULL));
+  if (fmax <= 0.01 * s->fnormtol) {
+    ret = 2;
+  } else {
+    CHK(l2norm(s, s->fval, &s->fnorm));
+    s->f1norm = 0.5 * s->fnorm * s->fnorm;
+    s->pthrsh = 2.0;
+    double epsmin = 0.01 * s->fnormtol;
+    for (;;) {
+      s->nni++;
+      s->eps = (s->eta + s->uround) * s->fnorm;
+      if (s->eps < epsmin) s->eps = epsmin;
+      CHK(linsoldrv(s, s->unew, s->pp, &ret));
+      if (ret != 0) break;
+      double fnormp = 0, f1normp = 0;
+      int maxStep = 0, gret = 0;
+      if (o->globalization == 0) CHK(inexact_newton(s, &fnormp, &f1normp, &maxStep));
+      else CHK(linesearch(s, &fnormp, &f1normp, &maxStep, &gret));
+      if (s->nbcf > 10) { ret = 7; break; }
+      if (s->callForcingTerm) forcing_term(s, fnormp);
+      s->fnorm = fnormp;
+      CHK(kin_stop(s, maxStep, gret, &ret));
+      CHK(pfhip_vec_scale(pb, 1.0, s->unew, s->uu, NULL));
+      s->f1norm = f1normp;
+      if (ret != 0) break;
+    }
+  }
+  CHK(pfhip_synchronize());
+  if (st) {
+    st->ret = ret;
+    st->nni = s->nni; st->nli = s->nli; st->nfe = s->nfe; st->npe = s->npe;
+    st->nps = s->nps; st->nbcf = s->nbcf; st->nbktrk = s->nbktrk;
+    st->ncfl = s->ncfl; st->fnorm = s->fnorm;
+  }
+  if (ret == 1 || ret == 2) return PFHIP_OK;
+  pfhip_set_error("KINSOL returned %d", ret);
+  return PFHIP_ERR_SOLVER;
+}
+
+int pfhip_advance_richards(pfhip_solver *s, double *p, double *sat, double t0,
+                           double dt, int nsteps, int max_conv_failures, int *newton,
+                           int *gmres, int *fevals, double *dt_used, int *steps_done)
+{
+  pfhip_problem *pb = s->pb;
+  double t = t0;
+  long bytes = s->ng * (long)sizeof(double);
+  if (steps_done) *steps_done = 0;
+  CHK(pfhip_saturation(pb, p, sat, PFHIP_CALCFCN, NULL));
+  for (int n = 0; n < nsteps; n++) {
+    int converged = 1, failures = 0, tn = 0, tl = 0, tf = 0;
+    double dtc = dt;
+    do {
+      if (converged) {
+        CHK(pfhip_vec_scale(pb, 1.0, sat, s->s_old, NULL));
+        CHK(pfhip_vec_scale(pb, 1.0, p, s->p_old, NULL));
+      } else {
+        t = t - dtc;
+        dtc = 0.5 * dtc;
+        CHK(pfhip_vec_scale(pb, 1.0, s->s_old, sat, NULL));
+        CHK(pfhip_vec_scale(pb, 1.0, s->p_old, p, NULL));
+      }
+      t += dtc;
+      pfhip_kin_stats st;
+      int rc = pfhip_kinsol_nonlin_solver(s, p, s->p_old, s->s_old, dtc, t, &st);
+      if (rc != PFHIP_OK && rc != PFHIP_ERR_SOLVER) return rc;
+      tn += st.nni; tl += st.nli; tf += st.nfe;
+      if (rc == PFHIP_ERR_SOLVER) { converged = 0; failures++; }
+      else converged = 1;
+    } while (!converged && failures < max_conv_failures);
+    if (!converged) {
+      pfhip_set_error("time step %d failed after %d convergence failures", n, failures);
+      return PFHIP_ERR_SOLVER;
+    }
+    CHK(pfhip_saturation(pb, p, sat, PFHIP_CALCFCN, NULL));
+    if (newton) newton[n] = tn;
+    if (gmres) gmres[n] = tl;
+    if (fevals) fevals[n] = tf;
+    if (dt_used) dt_used[n] = dtc;
+    if (steps_done) *steps_done = n + 1;
+  }
+  (void)bytes;
+  return pfhip_synchronize();
+}

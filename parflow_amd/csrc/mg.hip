@@ -1,0 +1,430 @@
+// mg.hip -- semicoarsening multigrid preconditioner for gfx950.
+//
+// Structure of ParFlow's in-repo MGSemi (PL/mg_semi.c): each level halves
+// the direction of smallest spacing that still has >= 2 points
+// (:832-900), coarse points are the even indices, prolongation weights are
+// operator induced at the odd (fine) points and coarse operators are the
+// collapsed Galerkin product (SetupCoarseOps :437-760); transfers follow
+// PL/mg_semi_restrict.c / PL/mg_semi_prolong.c and the smoother is point
+// red/black Gauss-Seidel (PL/rb_GS_point.c, red = (i+j+k) odd).
+//
+// One template serves FP64 and FP32 level storage (the mixed-precision
+// preconditioner keeps every level, vector and operator in FP32; only the
+// interface vectors are FP64).  Every kernel is a one-thread-per-point
+// x-coalesced pass over one level; matrices are symmetric-stored C,E,N,U.
+#include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "pfhip_internal.h"
+
+#define HIPCHK(x)                                                          \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess) {                                                \
+      pfhip_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x,             \
+                      hipGetErrorString(e_));                              \
+      return PFHIP_ERR_HIP;                                                \
+    }                                                                      \
+  } while (0)
+#define LAUNCHCHK() HIPCHK(hipGetLastError())
+#define STREAM(s) ((hipStream_t)(s))
+#define MAXL 128
+
+struct LGeom {
+  int n[3];
+  int s[3];   // strides: 1, n0+2, (n0+2)(n1+2)
+};
+
+static LGeom lgeom(const int *n)
+{
+  LGeom g;
+  for (int q = 0; q < 3; q++) g.n[q] = n[q];
+  g.s[0] = 1; g.s[1] = n[0] + 2; g.s[2] = (n[0] + 2) * (n[1] + 2);
+  return g;
+}
+static long lsize(const int *n) { return (long)(n[0] + 2) * (n[1] + 2) * (n[2] + 2); }
+static dim3 lgrid(const LGeom &g) { return dim3((g.n[0] + 63) / 64, (g.n[1] + 3) / 4, g.n[2]); }
+#define LBLOCK dim3(64, 4, 1)
+
+#define LIDX(g, i, j, k) ((i) + 1 + ((j) + 1) * (g).s[1] + ((k) + 1) * (g).s[2])
+#define THREAD_IJK(g)                                                            \
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y,      \
+      k = blockIdx.z;                                                            \
+  if (i >= (g).n[0] || j >= (g).n[1]) return;
+
+template <typename T> struct Mat { T *c, *u[3]; };   // C and upper E, N, U
+
+// coupling of point ip in direction q on side -1/+1 (symmetric storage)
+template <typename T>
+__device__ __forceinline__ T cpl(const Mat<T> &A, const LGeom &g, long ip, int q, int side)
+{
+  return (side > 0) ? A.u[q][ip] : A.u[q][ip - g.s[q]];
+}
+
+// ---- setup ------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_pweights(LGeom g, Mat<T> A, int d, int o1, int o2,
+                                                  T *__restrict__ p1, T *__restrict__ p2)
+{
+  THREAD_IJK(g);
+  int id[3] = { i, j, k };
+  if ((id[d] & 1) == 0) return;
+  long ip = LIDX(g, i, j, k);
+  T a0 = A.c[ip];
+  T a1 = cpl(A, g, ip, d, -1), a2 = cpl(A, g, ip, d, +1);
+  T a3 = cpl(A, g, ip, o1, -1), a4 = cpl(A, g, ip, o1, +1);
+  T a5 = cpl(A, g, ip, o2, -1), a6 = cpl(A, g, ip, o2, +1);
+  T ap0 = a0 + a3 + a4 + a5 + a6;
+  if (ap0 != T(0)) { p1[ip] = -a1 / ap0; p2[ip] = -a2 / ap0; }
+  else { p1[ip] = T(0); p2[ip] = T(0); }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_op(LGeom gf, LGeom gc, Mat<T> A, Mat<T> Ac,
+                                                   int d, int o1, int o2,
+                                                   const T *__restrict__ p1,
+                                                   const T *__restrict__ p2)
+{
+  THREAD_IJK(gc);
+  int cid[3] = { i, j, k };
+  int fi[3] = { i, j, k };
+  fi[d] *= 2;
+  long iA = LIDX(gf, fi[0], fi[1], fi[2]);
+  long iAc = LIDX(gc, i, j, k);
+  long sd = gf.s[d];
+  long iA1 = iA - sd, iA2 = iA + sd;
+  T up1 = A.u[o1][iA] + T(0.5) * A.u[o1][iA1] + T(0.5) * A.u[o1][iA2];
+  T up2 = A.u[o2][iA] + T(0.5) * A.u[o2][iA1] + T(0.5) * A.u[o2][iA2];
+  Ac.u[o1][iAc] = (cid[o1] == gc.n[o1] - 1) ? T(0) : up1;
+  Ac.u[o2][iAc] = (cid[o2] == gc.n[o2] - 1) ? T(0) : up2;
+  T upd;
+  if (cid[d] + 1 < gc.n[d]) upd = cpl(A, gf, iA + 2 * sd, d, -1) * p1[iA + sd];
+  else upd = T(0);
+  Ac.u[d][iAc] = upd;
+  T a0 = A.c[iA];
+  T a1 = cpl(A, gf, iA, d, -1), a2 = cpl(A, gf, iA, d, +1);
+  T a3 = cpl(A, gf, iA, o1, -1), a4 = cpl(A, gf, iA, o1, +1);
+  T a5 = cpl(A, gf, iA, o2, -1), a6 = cpl(A, gf, iA, o2, +1);
+  Ac.c[iAc] = a0 + a3 + a4 + a5 + a6 + a1 * p2[iA1] + a2 * p1[iA2];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_diag(LGeom gc, Mat<T> Ac, int o1, int o2)
+{
+  THREAD_IJK(gc);
+  int cid[3] = { i, j, k };
+  long iAc = LIDX(gc, i, j, k);
+  T c3 = (cid[o1] > 0) ? Ac.u[o1][iAc - gc.s[o1]] : T(0);
+  T c4 = Ac.u[o1][iAc];
+  T c5 = (cid[o2] > 0) ? Ac.u[o2][iAc - gc.s[o2]] : T(0);
+  T c6 = Ac.u[o2][iAc];
+  Ac.c[iAc] -= (c3 + c4 + c5 + c6);
+}
+
+// ---- cycle kernels ----------------------------------------------------------
+// red/black sweep of parity `par` ((i+j+k)&1 == par); init: x = b/a0
+template <typename T>
+__global__ __launch_bounds__(256) void k_rbgs(LGeom g, Mat<T> A, T *__restrict__ x,
+                                              const T *__restrict__ b, int par, int init)
+{
+  THREAD_IJK(g);
+  if (((i + j + k) & 1) != par) return;
+  long ip = LIDX(g, i, j, k);
+  if (init) { x[ip] = b[ip] / A.c[ip]; return; }
+  const int sy = g.s[1], sz = g.s[2];
+  T w = (i > 0) ? A.u[0][ip - 1] * x[ip - 1] : T(0);
+  T e = (i + 1 < g.n[0]) ? A.u[0][ip] * x[ip + 1] : T(0);
+  T so = (j > 0) ? A.u[1][ip - sy] * x[ip - sy] : T(0);
+  T no = (j + 1 < g.n[1]) ? A.u[1][ip] * x[ip + sy] : T(0);
+  T lo = (k > 0) ? A.u[2][ip - sz] * x[ip - sz] : T(0);
+  T up = (k + 1 < g.n[2]) ? A.u[2][ip] * x[ip + sz] : T(0);
+  T sum = w + e + so + no + lo + up;
+  x[ip] = (b[ip] - sum) / A.c[ip];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_residual_l(LGeom g, Mat<T> A, const T *__restrict__ x,
+                                                    const T *__restrict__ b, T *__restrict__ t)
+{
+  THREAD_IJK(g);
+  long ip = LIDX(g, i, j, k);
+  const int sy = g.s[1], sz = g.s[2];
+  T v = A.c[ip] * x[ip];
+  if (i > 0) v += A.u[0][ip - 1] * x[ip - 1];
+  if (i + 1 < g.n[0]) v += A.u[0][ip] * x[ip + 1];
+  if (j > 0) v += A.u[1][ip - sy] * x[ip - sy];
+  if (j + 1 < g.n[1]) v += A.u[1][ip] * x[ip + sy];
+  if (k > 0) v += A.u[2][ip - sz] * x[ip - sz];
+  if (k + 1 < g.n[2]) v += A.u[2][ip] * x[ip + sz];
+  t[ip] = b[ip] - v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict(LGeom gf, LGeom gc, int d,
+                                                  const T *__restrict__ rf, T *__restrict__ rc,
+                                                  const T *__restrict__ p1,
+                                                  const T *__restrict__ p2)
+{
+  THREAD_IJK(gc);
+  int fi[3] = { i, j, k };
+  fi[d] *= 2;
+  long iF = LIDX(gf, fi[0], fi[1], fi[2]);
+  long sd = gf.s[d];
+  T lo = (fi[d] > 0) ? p2[iF - sd] * rf[iF - sd] : T(0);
+  T hi = (fi[d] + 1 < gf.n[d]) ? p1[iF + sd] * rf[iF + sd] : T(0);
+  rc[LIDX(gc, i, j, k)] = rf[iF] + (lo + hi);
+}
+
+// x_f += P e_c (injection at coarse points, interpolation at fine points)
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong_add(LGeom gf, LGeom gc, int d,
+                                                     const T *__restrict__ ec,
+                                                     T *__restrict__ xf,
+                                                     const T *__restrict__ p1,
+                                                     const T *__restrict__ p2)
+{
+  THREAD_IJK(gf);
+  int id[3] = { i, j, k };
+  long ip = LIDX(gf, i, j, k);
+  int c[3] = { i, j, k };
+  if ((id[d] & 1) == 0) {
+    c[d] = id[d] >> 1;
+    xf[ip] += ec[LIDX(gc, c[0], c[1], c[2])];
+    return;
+  }
+  c[d] = id[d] >> 1;
+  T elo = ec[LIDX(gc, c[0], c[1], c[2])];
+  T lo = p1[ip] * elo;
+  T hi = T(0);
+  if (id[d] + 1 < gf.n[d]) {
+    c[d] = (id[d] + 1) >> 1;
+    hi = p2[ip] * ec[LIDX(gc, c[0], c[1], c[2])];
+  }
+  xf[ip] += lo + hi;
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void k_convert(LGeom g, const S *__restrict__ src,
+                                                 D *__restrict__ dst)
+{
+  THREAD_IJK(g);
+  long ip = LIDX(g, i, j, k);
+  dst[ip] = (D)src[ip];
+}
+
+template <typename T>
+__global__ void k_fill_t(T *p, long n)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < n; i += (long)gridDim.x * blockDim.x) p[i] = T(0);
+}
+
+// ---- host side ----------------------------------------------------------------
+struct pfhip_mg {
+  pfhip_problem *pb;
+  int L, prec, pre, post, symmetric;
+  int n[MAXL][3], dir[MAXL];
+  void *A[MAXL][4];
+  void *x[MAXL], *b[MAXL], *t[MAXL], *p1[MAXL], *p2[MAXL];
+};
+
+template <typename T> static Mat<T> matof(pfhip_mg *mg, int l)
+{
+  Mat<T> M;
+  M.c = (T *)mg->A[l][0];
+  for (int q = 0; q < 3; q++) M.u[q] = (T *)mg->A[l][1 + q];
+  return M;
+}
+
+static void offdirs(int d, int *o1, int *o2)
+{
+  // a3..a6 order of PL/mg_semi.c:470-492
+  if (d == 0) { *o1 = 1; *o2 = 2; }
+  else if (d == 1) { *o1 = 0; *o2 = 2; }
+  else { *o1 = 1; *o2 = 0; }
+}
+
+extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int post,
+                               int symmetric, int precision, pfhip_mg **out)
+{
+  if (precision != 64 && precision != 32) {
+    pfhip_set_error("mg precision must be 64 or 32");
+    return PFHIP_ERR_ARG;
+  }
+  if (pre < 0 || post < 0) { pfhip_set_error("negative sweep count"); return PFHIP_ERR_ARG; }
+  pfhip_mg *mg = (pfhip_mg *)calloc(1, sizeof(pfhip_mg));
+  mg->pb = pb;
+  mg->prec = precision;
+  mg->pre = pre; mg->post = post; mg->symmetric = symmetric;
+  int N[3] = { pb->d.nx, pb->d.ny, pb->d.nz };
+  double D[3] = { pb->d.dx, pb->d.dy, pb->d.dz };
+  if (max_levels <= 0 || max_levels > MAXL) max_levels = MAXL;
+  for (int q = 0; q < 3; q++) mg->n[0][q] = N[q];
+  int l;
+  for (l = 0; l < max_levels - 1; l++) {
+    double min_spacing = D[0] + D[1] + D[2];
+    int d = -1;
+    for (int q = 0; q < 3; q++)
+      if (N[q] >= 2 && D[q] < min_spacing) { min_spacing = D[q]; d = q; }
+    if (d == -1) break;
+    D[d] *= 2;
+    N[d] = (N[d] + 1) / 2;
+    mg->dir[l] = d;
+    for (int q = 0; q < 3; q++) mg->n[l + 1][q] = N[q];
+  }
+  mg->L = l + 1;
+  if (mg->L < 2) {
+    free(mg);
+    pfhip_set_error("grid too small for a V-cycle");
+    return PFHIP_ERR_ARG;
+  }
+  size_t es = (precision == 64) ? sizeof(double) : sizeof(float);
+  int rc = PFHIP_OK;
+  for (l = 0; l < mg->L && rc == PFHIP_OK; l++) {
+    long s = lsize(mg->n[l]);
+    void **ptrs[9] = { &mg->A[l][0], &mg->A[l][1], &mg->A[l][2], &mg->A[l][3],
+                       &mg->x[l], &mg->b[l], &mg->t[l], &mg->p1[l], &mg->p2[l] };
+    for (int q = 0; q < 9 && rc == PFHIP_OK; q++) {
+      rc = pfhip_malloc(ptrs[q], (long)(s * es));
+      if (rc == PFHIP_OK) {
+        hipError_t e = hipMemset(*ptrs[q], 0, s * es);
+        if (e != hipSuccess) { pfhip_set_error("hipMemset"); rc = PFHIP_ERR_HIP; }
+      }
+    }
+  }
+  if (rc != PFHIP_OK) { pfhip_mg_destroy(mg); return rc; }
+  HIPCHK(hipDeviceSynchronize());
+  *out = mg;
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_mg_destroy(pfhip_mg *mg)
+{
+  if (!mg) return PFHIP_OK;
+  for (int l = 0; l < mg->L; l++) {
+    for (int q = 0; q < 4; q++) if (mg->A[l][q]) hipFree(mg->A[l][q]);
+    void *o[5] = { mg->x[l], mg->b[l], mg->t[l], mg->p1[l], mg->p2[l] };
+    for (int q = 0; q < 5; q++) if (o[q]) hipFree(o[q]);
+  }
+  free(mg);
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_mg_num_levels(pfhip_mg *mg, int *levels)
+{
+  *levels = mg->L;
+  return PFHIP_OK;
+}
+
+template <typename T>
+static int mg_setup_t(pfhip_mg *mg, const double *const *Asym, hipStream_t st)
+{
+  LGeom g0 = lgeom(mg->n[0]);
+  const double *src[4] = { Asym[0], Asym[2], Asym[4], Asym[6] };
+  for (int q = 0; q < 4; q++) {
+    hipLaunchKernelGGL((k_convert<double, T>), lgrid(g0), LBLOCK, 0, st, g0, src[q],
+                       (T *)mg->A[0][q]);
+    LAUNCHCHK();
+  }
+  for (int l = 0; l < mg->L - 1; l++) {
+    LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
+    int d = mg->dir[l], o1, o2;
+    offdirs(d, &o1, &o2);
+    hipLaunchKernelGGL(k_pweights<T>, lgrid(gf), LBLOCK, 0, st, gf, matof<T>(mg, l), d, o1,
+                       o2, (T *)mg->p1[l], (T *)mg->p2[l]);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_coarse_op<T>, lgrid(gc), LBLOCK, 0, st, gf, gc, matof<T>(mg, l),
+                       matof<T>(mg, l + 1), d, o1, o2, (const T *)mg->p1[l],
+                       (const T *)mg->p2[l]);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_coarse_diag<T>, lgrid(gc), LBLOCK, 0, st, gc, matof<T>(mg, l + 1),
+                       o1, o2);
+    LAUNCHCHK();
+  }
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_mg_setup(pfhip_mg *mg, const double *const *Asym, void *stream)
+{
+  if (!Asym || !Asym[0] || !Asym[2] || !Asym[4] || !Asym[6]) {
+    pfhip_set_error("mg_setup: need C, E, N, U arrays");
+    return PFHIP_ERR_ARG;
+  }
+  return (mg->prec == 64) ? mg_setup_t<double>(mg, Asym, STREAM(stream))
+                          : mg_setup_t<float>(mg, Asym, STREAM(stream));
+}
+
+// RedBlackGSPoint call sequence (PL/rb_GS_point.c:100-300)
+template <typename T>
+static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
+{
+  LGeom g = lgeom(mg->n[l]);
+  Mat<T> A = matof<T>(mg, l);
+  T *x = (T *)mg->x[l];
+  const T *b = (const T *)mg->b[l];
+  int count = mg->symmetric ? 1 : 2;
+  int rb = 0;   // red first; red = odd parity
+  if (zero) {
+    count++;
+    if ((count / 2) > max_iter) {
+      long n = lsize(mg->n[l]);
+      hipLaunchKernelGGL(k_fill_t<T>, dim3(256), dim3(256), 0, st, x, n);
+      LAUNCHCHK();
+      if (!mg->symmetric) return PFHIP_OK;
+    }
+    hipLaunchKernelGGL(k_rbgs<T>, lgrid(g), LBLOCK, 0, st, g, A, x, b, 1, 1);
+    LAUNCHCHK();
+    rb = !rb;
+  }
+  for (; (count / 2) <= max_iter; count++) {
+    int par = rb ? 0 : 1;
+    hipLaunchKernelGGL(k_rbgs<T>, lgrid(g), LBLOCK, 0, st, g, A, x, b, par, 0);
+    LAUNCHCHK();
+    rb = !rb;
+  }
+  return PFHIP_OK;
+}
+
+// V-cycle of PL/mg_semi.c:205-330 (MaxIter 1, zero initial guess)
+template <typename T>
+static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t st)
+{
+  int L = mg->L, rc;
+  LGeom g0 = lgeom(mg->n[0]);
+  hipLaunchKernelGGL((k_convert<double, T>), lgrid(g0), LBLOCK, 0, st, g0, bin, (T *)mg->b[0]);
+  LAUNCHCHK();
+  if ((rc = rbgs<T>(mg, 0, 1, mg->pre, st))) return rc;
+  int l;
+  for (l = 0; l <= L - 2; l++) {
+    if (l > 0 && (rc = rbgs<T>(mg, l, 1, mg->pre, st))) return rc;
+    LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
+    hipLaunchKernelGGL(k_residual_l<T>, lgrid(gf), LBLOCK, 0, st, gf, matof<T>(mg, l),
+                       (const T *)mg->x[l], (const T *)mg->b[l], (T *)mg->t[l]);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_restrict<T>, lgrid(gc), LBLOCK, 0, st, gf, gc, mg->dir[l],
+                       (const T *)mg->t[l], (T *)mg->b[l + 1], (const T *)mg->p1[l],
+                       (const T *)mg->p2[l]);
+    LAUNCHCHK();
+  }
+  // coarse solve: RedBlackGSPoint with MaxIter 1
+  if ((rc = rbgs<T>(mg, L - 1, 1, 1, st))) return rc;
+  for (l = L - 2; l >= 0; l--) {
+    LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
+    hipLaunchKernelGGL(k_prolong_add<T>, lgrid(gf), LBLOCK, 0, st, gf, gc, mg->dir[l],
+                       (const T *)mg->x[l + 1], (T *)mg->x[l], (const T *)mg->p1[l],
+                       (const T *)mg->p2[l]);
+    LAUNCHCHK();
+    if ((rc = rbgs<T>(mg, l, 0, mg->post, st))) return rc;
+  }
+  hipLaunchKernelGGL((k_convert<T, double>), lgrid(g0), LBLOCK, 0, st, g0,
+                     (const T *)mg->x[0], xout);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_mg_apply(pfhip_mg *mg, const double *b, double *x, void *stream)
+{
+  return (mg->prec == 64) ? mg_apply_t<double>(mg, b, x, STREAM(stream))
+                          : mg_apply_t<float>(mg, b, x, STREAM(stream));
+}

@@ -1,0 +1,66 @@
+/*
+ * pfhip_internal.h -- internal declarations shared by the HIP kernels
+ * (*.hip) and the C host driver (kinsol_host.c) of libpfhip.
+ */
+#ifndef PFHIP_INTERNAL_H
+#define PFHIP_INTERNAL_H
+
+#include "../../include/pfhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct pfhip_problem {
+  pfhip_problem_desc d;
+  int sy, sz;                 /* strides of the ghosted layout            */
+  long ng;                    /* (nx+2)(ny+2)(nz+2)                       */
+  long n2;                    /* (nx+2)(ny+2)                             */
+  long ncell;                 /* nx*ny*nz                                 */
+  int overland;               /* top face is OverlandKinematic            */
+  double *bcv[6];             /* device per-face BC values                */
+  double *h_bcv[6];           /* pinned host staging                      */
+  double *krho, *dkrho;       /* device scratch, ghosted                  */
+  double *q2[4];              /* device surface scratch                   */
+  double *red_partial;        /* reduction partials (RED_MAX_BLOCKS*8)    */
+  double *red_dev;            /* device result slots (64)                 */
+  double *h_red;              /* pinned host result slots (64)            */
+};
+
+#define PFHIP_RED_MAX_BLOCKS 1024
+#define PFHIP_MAX_MULTI 64
+
+void pfhip_set_error(const char *fmt, ...);
+
+/* ---- extra vector ops used by the nonlinear driver ------------------ */
+int pfhip_vec_scale(pfhip_problem *pb, double c, const double *x, double *z,
+                    void *stream);
+/* max_i |s_i| / (1 + |u_i|)   (KINScSteplength with uscale = 1) */
+int pfhip_vec_scsteplength(pfhip_problem *pb, const double *u,
+                           const double *s, double *result, void *stream);
+/* dot products (u.v, v.v) and ||v||_1 in one pass (KINSpgmrAtimesDQ) */
+int pfhip_vec_dq_sums(pfhip_problem *pb, const double *u, const double *v,
+                      double *sutsv, double *vtv, double *l1, void *stream);
+/* h[j] = V[j] . w for j < m, one pass over w (block Gram-Schmidt) */
+int pfhip_vec_multi_dot(pfhip_problem *pb, const double *const *V, int m,
+                        const double *w, double *h, void *stream);
+/* w -= sum_j h[j] V[j]; optionally returns ||w||^2 after the update */
+int pfhip_vec_multi_axpy(pfhip_problem *pb, const double *const *V, int m,
+                         const double *h, double *w, double *norm2_out,
+                         void *stream);
+/* x += sum_j y[j] V[j] */
+int pfhip_vec_combine(pfhip_problem *pb, const double *const *V, int m,
+                      const double *y, double *x, void *stream);
+/* MGS step fused: w -= hprev * vprev (if vprev), then returns vnext . w */
+int pfhip_vec_axpy_dot(pfhip_problem *pb, double hprev, const double *vprev,
+                       double *w, const double *vnext, double *dot,
+                       void *stream);
+
+/* ---- device scratch helpers ---------------------------------------- */
+int pfhip_dev_alloc(double **p, long n);
+int pfhip_zero(double *p, long n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,51 @@
+"""Seeded small problems shared by the CPU and GPU tests."""
+import numpy as np
+
+from parflow_amd.problem import (BC_DIR_EQUIL, BC_FLUX, BC_OVERLAND, BCSpec, Problem,
+                                 SolverOptions)
+
+
+def hetero(nx=12, ny=10, nz=8, seed=0, overland=False, dir_sides=False):
+    """Heterogeneous box: lognormal K, alpha; uniform n; mixed BCs."""
+    rng = np.random.default_rng(seed)
+    shp = (nz, ny, nx)
+    K = np.exp(rng.normal(np.log(0.1), 1.0, shp))
+    alpha = np.exp(rng.normal(np.log(1.0), 0.3, shp))
+    n = rng.uniform(1.5, 3.0, shp)
+    bc = [BCSpec(BC_FLUX, (0.0,)) for _ in range(6)]
+    if dir_sides:
+        bc[0] = BCSpec(BC_DIR_EQUIL, (3.0,), (1,), -1, z_ref=0.0)
+        bc[3] = BCSpec(BC_DIR_EQUIL, (2.0,), (1,), -1, z_ref=0.0)
+    bc[4] = BCSpec(BC_DIR_EQUIL, (0.0,), (1,), -1, z_ref=0.0)
+    if overland:
+        bc[5] = BCSpec(BC_OVERLAND, (-0.05, 0.0), (2, 8), -1)
+        xc = (np.arange(nx) + 0.5)
+        sx = np.where(xc < nx / 2, -0.02, 0.02)
+        slope_x = np.tile(sx, (ny, 1)) + rng.normal(0, 0.002, (ny, nx))
+        slope_y = np.full((ny, nx), 0.005)
+    else:
+        bc[5] = BCSpec(BC_FLUX, (-0.01,))
+        slope_x = slope_y = 0.0
+    prob = Problem(nx=nx, ny=ny, nz=nz, dx=2.0, dy=2.0, dz=0.5, kx=K, poro=0.4,
+                   ss=1e-4, alpha=alpha, nvg=n, sres=0.2, ssat=1.0,
+                   slope_x=slope_x, slope_y=slope_y, mann=5.52e-6, bc=bc,
+                   base_unit=0.1, name=f"hetero{seed}")
+    return prob
+
+
+def random_pressure(prob, seed=1, wet_fraction=0.3):
+    """Pressure field mixing saturated (p >= 0) and unsaturated cells."""
+    rng = np.random.default_rng(seed)
+    p = prob.hydrostatic(1.0, 0.0)
+    p += rng.normal(0.0, 0.3, p.shape)
+    if prob.overland:
+        top = p[-2]
+        top[...] = rng.uniform(-0.05, 0.02, top.shape)
+    return np.ascontiguousarray(p)
+
+
+def solver(**kw):
+    base = dict(residual_tol=1e-9, step_tol=1e-30, max_iter=30, krylov_dim=25,
+                precond="MGSemi", eta_choice=2)
+    base.update(kw)
+    return SolverOptions(**base)

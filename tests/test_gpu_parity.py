@@ -1,0 +1,236 @@
+"""GPU parity tests: the HIP library, called through its C-ABI, against the
+oracle on identical seeded inputs.
+
+Bars: FP64 kernels agree with the oracle to round-off (1e-12 relative to
+the magnitude of the terms involved; device pow and FMA contraction differ
+from glibc by a few ulp); the plain (reference-algorithm) Newton solve
+agrees per cell within 1e-9 relative after every step with Newton counts
+within +-1.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+import cases
+import ref_decks
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _api():
+    from parflow_amd import api
+    return api
+
+
+def _I(prob, a):
+    return prob.interior(a)
+
+
+def _relmax(a, b, scale=None):
+    d = np.max(np.abs(a - b))
+    s = scale if scale is not None else max(np.max(np.abs(a)), np.max(np.abs(b)), 1e-300)
+    return d / s
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_closures(gpu, seed):
+    api = _api()
+    prob = cases.hetero(13, 7, 5, seed=seed)
+    print(f"closures seed={seed} shape={prob.shape}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p = cases.random_pressure(prob, seed=seed + 10)
+    for fcn in (0, 1):
+        a = gp.saturation(p, fcn)
+        b = O.saturation(op, p, fcn)
+        assert _relmax(_I(prob, a), _I(prob, b)) < 1e-13
+        a = gp.phase_rel_perm(p, fcn)
+        b = O.relperm(op, p, fcn)
+        assert _relmax(_I(prob, a), _I(prob, b)) < 1e-12
+
+
+@pytest.mark.parametrize("overland,dirs", [(False, False), (False, True), (True, True)])
+def test_residual(gpu, overland, dirs):
+    api = _api()
+    prob = cases.hetero(11, 9, 6, seed=3, overland=overland, dir_sides=dirs)
+    print(f"residual overland={overland} dir={dirs}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p_old = cases.random_pressure(prob, seed=4)
+    s_old = O.saturation(op, p_old)
+    p = p_old + np.random.default_rng(5).normal(0, 0.01, p_old.shape)
+    for t in (0.1, 0.35):
+        Fg = gp.nl_function_eval(p, p_old, s_old, 0.1, t)
+        Fo = O.nl_function_eval(op, p, p_old, s_old, 0.1, t)
+        # scale: magnitude of the accumulation terms
+        scale = np.max(np.abs(_I(prob, Fo))) + 1e-30
+        assert _relmax(_I(prob, Fg), _I(prob, Fo), scale) < 1e-12
+
+
+@pytest.mark.parametrize("symm", [0, 1])
+@pytest.mark.parametrize("overland", [False, True])
+def test_jacobian_and_matvec(gpu, symm, overland):
+    api = _api()
+    prob = cases.hetero(9, 8, 7, seed=6, overland=overland, dir_sides=True)
+    print(f"jacobian symm={symm} overland={overland}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p = cases.random_pressure(prob, seed=7)
+    Jg = gp.richards_jacobian_eval(p, 0.2, 0.2, symm)
+    Jo = O.jacobian_eval(op, p, 0.2, 0.2, symm)
+    idx = [0, 2, 4, 6] if symm else range(7)
+    scale = max(np.max(np.abs(Jo[s])) for s in idx)
+    for s in idx:
+        assert _relmax(_I(prob, Jg[s]), _I(prob, Jo[s]), scale) < 1e-12, s
+    x = np.zeros(prob.shape)
+    x[1:-1, 1:-1, 1:-1] = np.random.default_rng(8).normal(size=(prob.nz, prob.ny, prob.nx))
+    yg = gp.matvec(Jo, symm, x)
+    yo = O.matvec(op, Jo, symm, x)
+    assert _relmax(_I(prob, yg), _I(prob, yo)) < 1e-13
+
+
+def test_overland_fluxes(gpu):
+    api = _api()
+    prob = cases.hetero(10, 7, 3, seed=9, overland=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p = cases.random_pressure(prob, seed=10)
+    qx, qy, _, _ = gp.overland_flow_eval_kin(p, 0)
+    qxo, qyo = O.overland_kin(op, p)
+    n2 = (prob.nx + 2) * (prob.ny + 2)
+    sl = np.zeros((prob.ny + 2, prob.nx + 2), bool)
+    sl[1:-1, 0:-1] = True        # faces read by the residual (incl. west outflow)
+    m = sl.ravel()
+    assert np.allclose(qx[m], qxo[m], rtol=1e-13, atol=1e-300)
+    sl = np.zeros((prob.ny + 2, prob.nx + 2), bool)
+    sl[0:-1, 1:-1] = True
+    m = sl.ravel()
+    assert np.allclose(qy[m], qyo[m], rtol=1e-13, atol=1e-300)
+
+
+def test_vector_reductions(gpu):
+    api = _api()
+    prob = cases.hetero(37, 21, 9, seed=11)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    rng = np.random.default_rng(12)
+    x = np.zeros(prob.shape); y = np.zeros(prob.shape)
+    x[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+    y[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+    x[0] = 1e9   # ghost values must not be reduced
+    lib = O.lib()
+    ref_dot = float(np.sum(_I(prob, x) * _I(prob, y)))
+    assert abs(gp.vec_dot(x, y) - ref_dot) < 1e-12 * np.sum(np.abs(_I(prob, x) * _I(prob, y)))
+    assert gp.vec_max_norm(x) == np.max(np.abs(_I(prob, x)))
+    assert abs(gp.vec_l1_norm(x) - np.sum(np.abs(_I(prob, x)))) < 1e-12 * np.sum(np.abs(_I(prob, x)))
+    # determinism: identical bits on repeat
+    assert gp.vec_dot(x, y) == gp.vec_dot(x, y)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_multigrid(gpu, precision):
+    api = _api()
+    prob = cases.hetero(16, 12, 10, seed=13, dir_sides=True)
+    print(f"mg precision={precision}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p = cases.random_pressure(prob, seed=14)
+    J = O.jacobian_eval(op, p, 0.1, 0.1, 1)
+    for pre, post, sym in ((1, 1, 1), (2, 2, 0)):
+        mgo = O.MG(prob, pre, post, sym)
+        mgo.setup(J)
+        mgg = api.GPUMultigrid(gp, pre, post, sym, precision)
+        assert mgg.levels() == len(mgo.levels())
+        mgg.setup(J)
+        b = np.zeros(prob.shape)
+        b[1:-1, 1:-1, 1:-1] = np.random.default_rng(15).normal(size=(prob.nz, prob.ny, prob.nx))
+        xo = mgo.apply(b)
+        xg = mgg.apply(b)
+        tol = 1e-11 if precision == 64 else 2e-5
+        assert _relmax(_I(prob, xg), _I(prob, xo)) < tol
+
+
+def _compare_runs(prob, opts, p0, dt, nsteps, precision=64, krylov=0, ptol=1e-9,
+                  newton_tol=1):
+    api = _api()
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    r, pho, sho, logo = O.advance(op, opts, p0, 0.0, dt, nsteps)
+    assert r == 0
+    sv = api.GPUSolver(gp, opts, precision, krylov)
+    phg, shg, logg = sv.advance(p0, 0.0, dt, nsteps)
+    worst = 0.0
+    for s in range(nsteps + 1):
+        a, b = _I(prob, phg[s]), _I(prob, pho[s])
+        e = np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3))
+        es = _relmax(_I(prob, shg[s]), _I(prob, sho[s]))
+        worst = max(worst, e, es)
+        assert e < ptol, (s, e)
+        assert es < ptol, (s, es)
+    newton_o = np.array(logo.newton[:nsteps])
+    assert np.all(np.abs(logg["newton"] - newton_o) <= newton_tol), (logg["newton"], newton_o)
+    return worst, logg, logo
+
+
+def test_kinsol_plain_matches_oracle_default_richards(gpu):
+    """Reference deck default_richards (UseJacobian True) on the plain path."""
+    prob, opts, p0, dt, n = ref_decks.default_richards()
+    print("advance default_richards", flush=True)
+    worst, lg, lo = _compare_runs(prob, opts, p0, dt, 10)
+    print("worst rel diff", worst)
+
+
+def test_kinsol_plain_matches_oracle_tiltedv(gpu):
+    """Reference deck TiltedV_OverlandKin (DQ J.v, overland BC)."""
+    prob, opts, p0, dt, n = ref_decks.tiltedv_kin()
+    print("advance tiltedv_kin", flush=True)
+    _compare_runs(prob, opts, p0, dt, 12)
+
+
+@pytest.mark.parametrize("use_jac", [False, True])
+def test_kinsol_plain_matches_oracle_hetero(gpu, use_jac):
+    prob = cases.hetero(14, 12, 10, seed=21, dir_sides=True)
+    opts = cases.solver(use_jacobian=use_jac, residual_tol=1e-8, krylov_dim=25,
+                        precond="PFMG")
+    p0 = prob.hydrostatic(0.5, 0.0)
+    print(f"advance hetero use_jac={use_jac}", flush=True)
+    _compare_runs(prob, opts, p0, 0.2, 4)
+
+
+def test_golden_default_richards_on_gpu(gpu):
+    """The GPU solve against the reference's own outputs."""
+    api = _api()
+    prob, opts, p0, dt, n = ref_decks.default_richards()
+    gp = api.GPUProblem(prob)
+    sv = api.GPUSolver(gp, opts)
+    ph, sh, log = sv.advance(p0, 0.0, dt, 20)
+    g = np.load(os.path.join(HERE, "golden", "default_richards.npz"))
+    for i, s in enumerate(g["steps"]):
+        if s > 20:
+            break
+        a = prob.interior(ph[s])
+        e = np.max(np.abs(a - g["press"][i]) / np.maximum(np.abs(a), np.abs(g["press"][i])))
+        assert e < 6e-7, (s, e)
+    assert log["newton"][0] == 3
+
+
+def test_mixed_precision_close_to_plain(gpu):
+    """FGMRES + FP32 V-cycle vs the plain FP64 path on a tight tolerance."""
+    api = _api()
+    prob = cases.hetero(16, 14, 12, seed=31, dir_sides=True)
+    opts = cases.solver(residual_tol=1e-12, krylov_dim=30, precond="PFMG",
+                        num_pre_relax=2, num_post_relax=2)
+    p0 = prob.hydrostatic(0.5, 0.0)
+    gp = api.GPUProblem(prob)
+    plain = api.GPUSolver(gp, opts, 64, 0)
+    mixed = api.GPUSolver(gp, opts, 32, 1)
+    pp, sp_, lp = plain.advance(p0, 0.0, 0.2, 3)
+    pm, sm, lm = mixed.advance(p0, 0.0, 0.2, 3)
+    for s in range(4):
+        a, b = _I(prob, pm[s]), _I(prob, pp[s])
+        e = np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3))
+        assert e < 1e-8, (s, e)
