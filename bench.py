@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark: Richards Newton-Krylov time stepping on one MI355X.
+
+Metric: cell-timesteps/s on the 16.8M-cell heterogeneous workload
+(512x512x64, dt = 0.5 h), solved with the mixed-precision preconditioner
+(flexible GMRES(30), FP32 semicoarsening V-cycle with 2+2 red/black sweeps;
+residual, J.v, Krylov basis and Gram-Schmidt in FP64).  The FP64 plain path
+(the reference's algorithm: GMRES with modified Gram-Schmidt and an FP64
+V-cycle) runs the same steps beside it; the JSON line reports its
+throughput, the per-step pressure error of the mixed solve against it, and
+the Newton/GMRES iteration counts.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
+
+Warmup runs W steps on a scratch copy of the state; the timed region is
+then exactly the workload's first K steps from its initial condition, with
+inputs resident in HBM and a device synchronisation on both sides.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def residual_bytes_per_cell(prob, gp):
+    """Compulsory HBM bytes per cell of one residual-kernel launch: the
+    distinct input arrays it reads (p, kr*rho, K (aliased arrays count
+    once), porosity, Ss, alpha, n, s_res, s_sat, p_old, S_old) plus F written;
+    neighbour values come from cache."""
+    d = gp.desc
+    ptrs = {d.kx, d.ky, d.kz}
+    n_k = len(ptrs)
+    reads = 1 + 1 + n_k + 2 + 4 + 2
+    return 8 * (reads + 1)
+
+
+def run_gpu(args):
+    from parflow_amd import api
+    from parflow_amd.decks import deck
+
+    L = api.lib()
+    t0 = time.time()
+    prob, opts, p0, dt, nsteps_cfg, extras = deck(args.config)
+    _log(f"deck {args.config}: {prob.nx}x{prob.ny}x{prob.nz} "
+         f"({prob.ncells} cells) generated in {time.time() - t0:.1f}s")
+    gp = api.GPUProblem(prob)
+    ncell = prob.ncells
+    K, W = args.steps, args.warmup
+
+    def make(mode):
+        if mode == "mixed":
+            return api.GPUSolver(gp, opts, precision=32, krylov=1)
+        return api.GPUSolver(gp, opts, precision=64, krylov=0)
+
+    results = {}
+    snaps = {}
+    for mode in ("mixed", "plain"):
+        if mode == "plain" and args.no_plain:
+            continue
+        sv = make(mode)
+        # warmup on a scratch state
+        if W > 0:
+            dp, ds = gp.vec(p0), gp.vec()
+            sv.advance_device(dp, ds, 0.0, dt, W)
+            del dp, ds
+        dp, ds = gp.vec(p0), gp.vec()
+        snap = [gp.vec() for _ in range(K)]
+        if mode == "mixed":
+            api.check(L.pfhip_prof_enable(api.PROF_RESIDUAL, 200000), "prof")
+        newton, gmres, fevals = [], [], []
+        t = 0.0
+        api.check(L.pfhip_synchronize(), "sync")
+        tic = time.perf_counter()
+        for s in range(K):
+            lg = sv.advance_device(dp, ds, t, dt, 1)
+            t += float(lg["dt"][0])
+            newton.append(int(lg["newton"][0]))
+            gmres.append(int(lg["gmres"][0]))
+            fevals.append(int(lg["fevals"][0]))
+            api.check(L.pfhip_vec_linear_sum(gp.h, 1.0, dp.ptr, 0.0, dp.ptr,
+                                             snap[s].ptr, None), "snapshot")
+        api.check(L.pfhip_synchronize(), "sync")
+        elapsed = time.perf_counter() - tic
+        prof = None
+        if mode == "mixed":
+            ms = api.C.c_double()
+            n = api.C.c_long()
+            api.check(L.pfhip_prof_read(api.PROF_RESIDUAL, api.C.byref(ms), api.C.byref(n)),
+                      "prof_read")
+            api.check(L.pfhip_prof_disable(api.PROF_RESIDUAL), "prof_off")
+            prof = (ms.value, n.value)
+        results[mode] = dict(elapsed=elapsed, newton=newton, gmres=gmres, fevals=fevals,
+                             value=ncell * K / elapsed, prof=prof)
+        snaps[mode] = snap
+        _log(f"{mode}: {K} steps in {elapsed:.3f}s = {ncell * K / elapsed:.4e} cell-steps/s "
+             f"newton={newton} gmres={gmres}")
+        del sv
+
+    errs = []
+    if "plain" in snaps:
+        for s in range(K):
+            a = prob.interior(gp.get(snaps["mixed"][s]))
+            b = prob.interior(gp.get(snaps["plain"][s]))
+            errs.append(float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3))))
+    return prob, gp, results, errs
+
+
+def run_cpu_baseline(args, cores):
+    """The oracle (C restatement, OpenMP over cells) on a bounded sample of
+    the same workload: the deck's fields cropped to the first
+    (nx_s x ny_s x nz) columns, one time step."""
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    from oracle import oracle as O
+    from parflow_amd.decks import deck
+    from parflow_amd.problem import Problem
+
+    prob, opts, p0, dt, nsteps_cfg, extras = deck(args.config)
+    n = args.cpu_sample
+    I = lambda a: prob.interior(a)[:, :n, :n]
+    sub = Problem(nx=n, ny=n, nz=prob.nz, dx=prob.dx, dy=prob.dy, dz=prob.dz,
+                  x0=prob.x0, y0=prob.y0, z0=prob.z0, kx=I(prob.g_kx), poro=I(prob.g_poro),
+                  ss=I(prob.g_ss), alpha=I(prob.g_alpha), nvg=I(prob.g_nvg),
+                  sres=I(prob.g_sres), ssat=I(prob.g_ssat), bc=prob.bc)
+    op = O.OracleProblem(sub)
+    ps = sub.hydrostatic(0.0, prob.bc[4].z_ref)
+    tic = time.perf_counter()
+    r, ph, sh, log = O.advance(op, opts, ps, 0.0, dt, 1, keep_history=False)
+    el = time.perf_counter() - tic
+    return dict(value=sub.ncells / el, unit="cell-timesteps/s", cores=int(cores),
+                kind="port",
+                sample=f"oracle (C restatement, OpenMP) on the {n}x{n}x{prob.nz} "
+                       f"corner of the {args.config} fields, 1 step of dt={dt} h, "
+                       f"{el:.1f}s, newton={log.newton[0]}, status={r}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU domain decomposition is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    prob, gp, res, errs = run_gpu(args)
+    mixed = res["mixed"]
+    K = args.steps
+    ms_tot, nlaunch = mixed["prof"]
+    avg_s = (ms_tot / 1e3) / max(nlaunch, 1)
+    bpc = residual_bytes_per_cell(prob, gp)
+    bytes_per_launch = bpc * prob.ncells
+    achieved = bytes_per_launch / avg_s / 1e9
+    line = {
+        "metric": "cell-timesteps/s (16.8M cells), mixed-precond vs FP64 plain at 1e-8 rel err",
+        "value": mixed["value"],
+        "unit": "cell-timesteps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * mixed["elapsed"] / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (FP32 preconditioner)",
+        "data": "synthetic (seeded FFT Gaussian random field, seeded van Genuchten fields)",
+        "config": {
+            "workload": f"{args.config}: {prob.nx}x{prob.ny}x{prob.nz} heterogeneous Richards, "
+                        "FGMRES(30) + FP32 V-cycle (2+2 RB sweeps), dt=0.5 h",
+            "cells": prob.ncells,
+            "newton_per_step": mixed["newton"],
+            "gmres_per_step": mixed["gmres"],
+            "fevals_per_step": mixed["fevals"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_residual (NlFunctionEval)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_cell": bpc,
+            "avg_launch_us": avg_s * 1e6,
+            "launches": nlaunch,
+        },
+    }
+    if "plain" in res:
+        plain = res["plain"]
+        line["plain_fp64"] = {
+            "value": plain["value"],
+            "ms_per_step": 1e3 * plain["elapsed"] / K,
+            "newton_per_step": plain["newton"],
+            "gmres_per_step": plain["gmres"],
+            "mixed_over_plain": mixed["value"] / plain["value"],
+            "max_rel_err_vs_plain_per_step": errs,
+        }
+    if not args.no_cpu:
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        line["cpu_baseline"] = run_cpu_baseline(args, cores)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

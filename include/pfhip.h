@@ -194,6 +194,19 @@ int  pfhip_advance_richards(pfhip_solver *s, double *p, double *sat,
                             int max_conv_failures, int *newton, int *gmres,
                             int *fevals, double *dt_used, int *steps_done);
 
+/* ---- live kernel timing (HIP events on the launching stream) --------- */
+/* No counterpart in the reference (its timers are host wall clocks,
+ * PL/timing.c:75).  kernel_id: */
+#define PFHIP_PROF_RESIDUAL  0    /* the residual kernel of NlFunctionEval  */
+#define PFHIP_PROF_CLOSURE   1    /* van Genuchten closure kernel           */
+#define PFHIP_PROF_JACOBIAN  2    /* RichardsJacobianEval assembly kernel   */
+#define PFHIP_PROF_RBGS      3    /* finest-level red/black sweep           */
+#define PFHIP_PROF_NKERNELS  4
+int  pfhip_prof_enable(int kernel_id, int max_launches);
+int  pfhip_prof_reset(int kernel_id);
+int  pfhip_prof_read(int kernel_id, double *total_ms, long *launches);
+int  pfhip_prof_disable(int kernel_id);
+
 #ifdef __cplusplus
 }
 #endif

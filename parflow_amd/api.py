@@ -24,6 +24,9 @@ _lib = None
 _dp = C.POINTER(C.c_double)
 MAXI = 8
 
+# kernel ids of pfhip_prof_* (include/pfhip.h)
+PROF_RESIDUAL, PROF_CLOSURE, PROF_JACOBIAN, PROF_RBGS = 0, 1, 2, 3
+
 
 class PFHIPError(RuntimeError):
     pass
@@ -126,6 +129,10 @@ def lib():
                                                  C.POINTER(KinStatsC)]
         L.pfhip_advance_richards.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_int,
                                              C.c_int, vp, vp, vp, vp, C.POINTER(C.c_int)]
+        L.pfhip_prof_enable.argtypes = [C.c_int, C.c_int]
+        L.pfhip_prof_reset.argtypes = [C.c_int]
+        L.pfhip_prof_read.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
+        L.pfhip_prof_disable.argtypes = [C.c_int]
         _lib = L
     return _lib
 

@@ -52,6 +52,81 @@ extern "C" const char *pfhip_last_error(void) { return g_err; }
 #define STREAM(s) ((hipStream_t)(s))
 
 // --------------------------------------------------------------------------
+// live per-kernel timing with HIP events (pfhip_prof_*): when a kernel id is
+// enabled, every launch of that kernel is bracketed by an event pair on the
+// stream it is launched on.
+// --------------------------------------------------------------------------
+struct ProfState {
+  int enabled[PFHIP_PROF_NKERNELS];
+  int cap;
+  int count[PFHIP_PROF_NKERNELS];
+  hipEvent_t *ev[PFHIP_PROF_NKERNELS];
+};
+static ProfState g_prof;
+
+extern "C" int pfhip_prof_enable(int kernel_id, int max_launches)
+{
+  if (kernel_id < 0 || kernel_id >= PFHIP_PROF_NKERNELS || max_launches < 1) {
+    pfhip_set_error("prof_enable: bad kernel id %d", kernel_id);
+    return PFHIP_ERR_ARG;
+  }
+  if (g_prof.ev[kernel_id]) {
+    for (int i = 0; i < 2 * g_prof.cap; i++) hipEventDestroy(g_prof.ev[kernel_id][i]);
+    free(g_prof.ev[kernel_id]);
+  }
+  g_prof.cap = max_launches;
+  g_prof.ev[kernel_id] = (hipEvent_t *)calloc(2 * max_launches, sizeof(hipEvent_t));
+  for (int i = 0; i < 2 * max_launches; i++)
+    HIPCHK(hipEventCreate(&g_prof.ev[kernel_id][i]));
+  g_prof.count[kernel_id] = 0;
+  g_prof.enabled[kernel_id] = 1;
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_prof_reset(int kernel_id)
+{
+  if (kernel_id < 0 || kernel_id >= PFHIP_PROF_NKERNELS) return PFHIP_ERR_ARG;
+  g_prof.count[kernel_id] = 0;
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_prof_read(int kernel_id, double *total_ms, long *launches)
+{
+  if (kernel_id < 0 || kernel_id >= PFHIP_PROF_NKERNELS) return PFHIP_ERR_ARG;
+  double tot = 0.0;
+  int n = g_prof.count[kernel_id];
+  if (n > g_prof.cap) n = g_prof.cap;
+  for (int i = 0; i < n; i++) {
+    float ms = 0.f;
+    HIPCHK(hipEventSynchronize(g_prof.ev[kernel_id][2 * i + 1]));
+    HIPCHK(hipEventElapsedTime(&ms, g_prof.ev[kernel_id][2 * i],
+                               g_prof.ev[kernel_id][2 * i + 1]));
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = n;
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_prof_disable(int kernel_id)
+{
+  if (kernel_id < 0 || kernel_id >= PFHIP_PROF_NKERNELS) return PFHIP_ERR_ARG;
+  g_prof.enabled[kernel_id] = 0;
+  return PFHIP_OK;
+}
+
+extern "C" void pfhip_prof_mark(int kernel_id, int end, void *stream)
+{
+  if (!g_prof.enabled[kernel_id]) return;
+  int c = g_prof.count[kernel_id];
+  if (c >= g_prof.cap) return;
+  hipEventRecord(g_prof.ev[kernel_id][2 * c + end], STREAM(stream));
+  if (end) g_prof.count[kernel_id] = c + 1;
+}
+#define PROF_BEGIN(id, st) pfhip_prof_mark((id), 0, (st))
+#define PROF_END(id, st) pfhip_prof_mark((id), 1, (st))
+
+// --------------------------------------------------------------------------
 // memory
 // --------------------------------------------------------------------------
 extern "C" int pfhip_device_count(int *count)
@@ -798,8 +873,10 @@ static int launch_closure(pfhip_problem *pb, int what, const double *p, double *
                           double *out2, void *stream)
 {
   Geom g = geom_of(pb);
+  PROF_BEGIN(PFHIP_PROF_CLOSURE, stream);
   hipLaunchKernelGGL(k_closure, grid3(g), BLOCK3, 0, STREAM(stream), g, phys_of(pb),
                      what, p, out, out2);
+  PROF_END(PFHIP_PROF_CLOSURE, stream);
   LAUNCHCHK();
   return PFHIP_OK;
 }
@@ -1125,7 +1202,9 @@ extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double
   A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
   for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
   A.F = F;
+  PROF_BEGIN(PFHIP_PROF_RESIDUAL, stream);
   hipLaunchKernelGGL(k_residual, grid3(g), BLOCK3, 0, STREAM(stream), A);
+  PROF_END(PFHIP_PROF_RESIDUAL, stream);
   LAUNCHCHK();
   return PFHIP_OK;
 }
@@ -1376,7 +1455,9 @@ extern "C" int pfhip_richards_jacobian_eval(pfhip_problem *pb, const double *p,
   A.symm = symm_part;
   A.overland = pb->overland;
   for (int s = 0; s < 7; s++) A.J[s] = J[s];
+  PROF_BEGIN(PFHIP_PROF_JACOBIAN, stream);
   hipLaunchKernelGGL(k_jacobian, grid3(g), BLOCK3, 0, STREAM(stream), A);
+  PROF_END(PFHIP_PROF_JACOBIAN, stream);
   LAUNCHCHK();
   return PFHIP_OK;
 }

@@ -379,7 +379,9 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
   }
   for (; (count / 2) <= max_iter; count++) {
     int par = rb ? 0 : 1;
+    if (l == 0) pfhip_prof_mark(PFHIP_PROF_RBGS, 0, st);
     hipLaunchKernelGGL(k_rbgs<T>, lgrid(g), LBLOCK, 0, st, g, A, x, b, par, 0);
+    if (l == 0) pfhip_prof_mark(PFHIP_PROF_RBGS, 1, st);
     LAUNCHCHK();
     rb = !rb;
   }

@@ -31,6 +31,7 @@ struct pfhip_problem {
 #define PFHIP_MAX_MULTI 64
 
 void pfhip_set_error(const char *fmt, ...);
+void pfhip_prof_mark(int kernel_id, int end, void *stream);
 
 /* ---- extra vector ops used by the nonlinear driver ------------------ */
 int pfhip_vec_scale(pfhip_problem *pb, double c, const double *x, double *z,

@@ -106,10 +106,14 @@ def _ghost3(a, nx, ny, nz, fill_edge=True):
         return out
     if a.shape != (nz, ny, nx):
         raise ValueError(f"field shape {a.shape} != {(nz, ny, nx)}")
+    out[1:-1, 1:-1, 1:-1] = a
     if fill_edge:
-        out[...] = np.pad(a, 1, mode="edge")
-    else:
-        out[1:-1, 1:-1, 1:-1] = a
+        out[1:-1, 1:-1, 0] = a[:, :, 0]
+        out[1:-1, 1:-1, -1] = a[:, :, -1]
+        out[1:-1, 0, :] = out[1:-1, 1, :]
+        out[1:-1, -1, :] = out[1:-1, -2, :]
+        out[0] = out[1]
+        out[-1] = out[-2]
     return out
 
 
