@@ -37,14 +37,15 @@ def _log(*a):
 
 
 def residual_bytes_per_cell(prob, gp):
-    """Compulsory HBM bytes per cell of one residual-kernel launch: the
-    distinct input arrays it reads (p, kr*rho, K (aliased arrays count
-    once), porosity, Ss, alpha, n, s_res, s_sat, p_old, S_old) plus F written;
-    neighbour values come from cache."""
+    """Compulsory HBM bytes per cell of one fused residual launch: the
+    distinct input arrays it reads -- p, alpha, n, K (aliased Kx/Ky/Kz count
+    once), porosity, Ss, p_old, S_old, plus s_res/s_sat when they are
+    per-cell fields -- and F written.  Neighbour values come from LDS or
+    cache, so each array is counted once."""
     d = gp.desc
-    ptrs = {d.kx, d.ky, d.kz}
-    n_k = len(ptrs)
-    reads = 1 + 1 + n_k + 2 + 4 + 2
+    n_k = len({d.kx, d.ky, d.kz})
+    n_vg = sum(1 for v in (d.sres, d.ssat) if v)
+    reads = 1 + 2 + n_k + 2 + 2 + n_vg
     return 8 * (reads + 1)
 
 
@@ -193,7 +194,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_residual (NlFunctionEval)",
+            "kernel": "k_residual_fused (NlFunctionEval)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -60,7 +60,8 @@ typedef struct {
   const double *kx, *ky, *kz;        /* device, ghosted; ghost entries 0    */
   const double *poro, *ss;           /* device, ghosted                     */
   const double *alpha, *nvg;         /* device, ghosted van Genuchten       */
-  const double *sres, *ssat;         /* device, ghosted                     */
+  const double *sres, *ssat;         /* device, ghosted, or NULL for ...    */
+  double sres_value, ssat_value;     /* ... these region-constant values    */
   const double *src;                 /* device, ghosted, or NULL            */
   const double *slope_x, *slope_y;   /* device, surface, or NULL            */
   const double *mann;                /* device, surface, or NULL            */

@@ -47,6 +47,7 @@ class ProblemDescC(C.Structure):
                 ("poro", C.c_void_p), ("ss", C.c_void_p),
                 ("alpha", C.c_void_p), ("nvg", C.c_void_p),
                 ("sres", C.c_void_p), ("ssat", C.c_void_p),
+                ("sres_value", C.c_double), ("ssat_value", C.c_double),
                 ("src", C.c_void_p),
                 ("slope_x", C.c_void_p), ("slope_y", C.c_void_p), ("mann", C.c_void_p),
                 ("ov_eps", C.c_double),
@@ -226,7 +227,14 @@ class GPUProblem:
         d.dx, d.dy, d.dz = prob.dx, prob.dy, prob.dz
         d.x0, d.y0, d.z0 = prob.x0, prob.y0, prob.z0
         d.rho, d.mu, d.gravity = prob.rho, prob.mu, prob.gravity
-        fields = ("kx", "ky", "kz", "poro", "ss", "alpha", "nvg", "sres", "ssat")
+        fields = ["kx", "ky", "kz", "poro", "ss", "alpha", "nvg"]
+        # region-constant residual/saturated saturations travel as scalars
+        for name in ("sres", "ssat"):
+            if getattr(prob, "uniform_" + name):
+                setattr(d, name, None)
+                setattr(d, name + "_value", float(getattr(prob, name)))
+            else:
+                fields.append(name)
         # isotropic permeability shares one device array
         cache = {}
         for name in fields:

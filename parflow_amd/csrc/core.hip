@@ -10,6 +10,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "pfhip_internal.h"
@@ -241,7 +242,7 @@ extern "C" int pfhip_problem_create(const pfhip_problem_desc *desc,
     return PFHIP_ERR_ARG;
   }
   if (!d->kx || !d->ky || !d->kz || !d->poro || !d->ss || !d->alpha ||
-      !d->nvg || !d->sres || !d->ssat) {
+      !d->nvg) {
     pfhip_set_error("problem fields must be device arrays");
     return PFHIP_ERR_ARG;
   }
@@ -775,65 +776,107 @@ extern "C" int pfhip_vec_const(pfhip_problem *pb, double c, double *z, void *str
 }
 
 // --------------------------------------------------------------------------
-// van Genuchten closures (PL/problem_saturation.c:265-310,
-// PL/problem_phase_rel_perm.c:767-792, 895-912), same operation order
+// van Genuchten closures: S and dS/dp (PL/problem_saturation.c:265-310),
+// kr and dkr/dp (PL/problem_phase_rel_perm.c:767-792, 895-912).
+//
+// The reference spends 3-8 pow() calls per cell.  Here every power of
+// (alpha h) and of (1 + (alpha h)^n) is derived from one shared chain,
+//   L = log(alpha h), t = (alpha h)^n = exp(n L), a = (1+t)^-m = exp(-m log1p(t)),
+// so S and kr together cost two logs, two exps and a square root; the
+// derivative terms reuse t, a and sqrt(a).  Values agree with the pow form
+// to a few ulp (the 1 - (alpha h)^(n-1) a cancellation of dry cells is
+// common to both forms).
 // --------------------------------------------------------------------------
+struct VGTerms {
+  double ah, t, a, sa, ahnm1, m;
+};
+
+__device__ __forceinline__ VGTerms vg_terms(double p, double rho, double g, double alpha,
+                                            double n)
+{
+  VGTerms v;
+  v.m = 1.0 - (1.0 / n);
+  double head = fabs(p) / (rho * g);
+  v.ah = alpha * head;
+  double L = log(v.ah);
+  v.t = exp(n * L);                 // (alpha h)^n
+  v.a = exp(-v.m * log1p(v.t));     // (1 + (alpha h)^n)^-m
+  v.sa = sqrt(v.a);                 // (1 + (alpha h)^n)^(-m/2)
+  v.ahnm1 = v.t / v.ah;             // (alpha h)^(n-1)
+  return v;
+}
+
 __device__ __forceinline__ double vg_sat(double p, double rho, double g,
                                          double alpha, double n, double sres,
                                          double ssat)
 {
-  double m = 1.0 - (1.0 / n);
   double s_dif = ssat - sres;
   if (p >= 0.0) return s_dif + sres;
-  double head = fabs(p) / (rho * g);
-  return s_dif / pow(1.0 + pow(alpha * head, n), m) + sres;
+  VGTerms v = vg_terms(p, rho, g, alpha, n);
+  return s_dif * v.a + sres;
 }
 
 __device__ __forceinline__ double vg_dsat(double p, double rho, double g,
                                           double alpha, double n, double sres,
                                           double ssat)
 {
-  double m = 1.0 - (1.0 / n);
   double s_dif = ssat - sres;
   if (p >= 0.0) return 0.0;
-  double head = fabs(p) / (rho * g);
-  return (m * n * alpha * pow(alpha * head, (n - 1))) * s_dif /
-         (pow(1.0 + pow(alpha * head, n), m + 1));
+  VGTerms v = vg_terms(p, rho, g, alpha, n);
+  return (v.m * n * alpha * v.ahnm1) * s_dif * (v.a / (1.0 + v.t));
+}
+
+__device__ __forceinline__ double vg_kr_t(const VGTerms &v)
+{
+  double coeff = 1.0 - v.ahnm1 * v.a;
+  return (coeff * coeff) * v.sa;
+}
+
+__device__ __forceinline__ double vg_dkr_t(const VGTerms &v, double alpha, double n)
+{
+  double m = v.m, opahn = 1.0 + v.t;
+  double coeff = 1.0 - v.ahnm1 * v.a;
+  return 2.0 * (coeff * v.sa) *
+             ((n - 1) * (v.ahnm1 / v.ah) * alpha * v.a -
+              v.ahnm1 * m * (v.a / opahn) * n * alpha * v.ahnm1) +
+         (coeff * coeff) * (m / 2) * (v.sa / opahn) * n * alpha * v.ahnm1;
 }
 
 __device__ __forceinline__ double vg_kr(double p, double rho, double g,
                                         double alpha, double n)
 {
   if (p >= 0.0) return 1.0;
-  double m = 1.0 - (1.0 / n);
-  double head = fabs(p) / (rho * g);
-  double opahn = 1.0 + pow(alpha * head, n);
-  double ahnm1 = pow(alpha * head, n - 1);
-  double t = 1.0 - ahnm1 / pow(opahn, m);
-  return (t * t) / pow(opahn, m / 2);
+  return vg_kr_t(vg_terms(p, rho, g, alpha, n));
 }
 
 __device__ __forceinline__ double vg_dkr(double p, double rho, double g,
                                          double alpha, double n)
 {
   if (p >= 0.0) return 0.0;
-  double m = 1.0 - (1.0 / n);
-  double head = fabs(p) / (rho * g);
-  double opahn = 1.0 + pow(alpha * head, n);
-  double ahnm1 = pow(alpha * head, n - 1);
-  double coeff = 1.0 - ahnm1 * pow(opahn, -m);
-  return 2.0 * (coeff / (pow(opahn, (m / 2)))) *
-             ((n - 1) * pow(alpha * head, n - 2) * alpha * pow(opahn, -m) -
-              ahnm1 * m * pow(opahn, -(m + 1)) * n * alpha * ahnm1) +
-         (coeff * coeff) * (m / 2) * pow(opahn, (-(m + 2) / 2)) * n * alpha * ahnm1;
+  return vg_dkr_t(vg_terms(p, rho, g, alpha, n), alpha, n);
+}
+
+// S and kr of one cell from one chain
+__device__ __forceinline__ void vg_s_kr(double p, double rho, double g, double alpha,
+                                        double n, double sres, double ssat, double &S,
+                                        double &kr)
+{
+  double s_dif = ssat - sres;
+  if (p >= 0.0) { S = s_dif + sres; kr = 1.0; return; }
+  VGTerms v = vg_terms(p, rho, g, alpha, n);
+  S = s_dif * v.a + sres;
+  kr = vg_kr_t(v);
 }
 
 struct Phys {
   double rho, mu, g;
   const double *__restrict__ alpha;
   const double *__restrict__ nvg;
-  const double *__restrict__ sres;
-  const double *__restrict__ ssat;
+  const double *__restrict__ sres;   // NULL: uniform sres_v
+  const double *__restrict__ ssat;   // NULL: uniform ssat_v
+  double sres_v, ssat_v;
+  __device__ __forceinline__ double Sres(int ip) const { return sres ? sres[ip] : sres_v; }
+  __device__ __forceinline__ double Ssat(int ip) const { return ssat ? ssat[ip] : ssat_v; }
 };
 
 static inline Phys phys_of(const pfhip_problem *pb)
@@ -841,6 +884,7 @@ static inline Phys phys_of(const pfhip_problem *pb)
   Phys f;
   f.rho = pb->d.rho; f.mu = pb->d.mu; f.g = pb->d.gravity;
   f.alpha = pb->d.alpha; f.nvg = pb->d.nvg; f.sres = pb->d.sres; f.ssat = pb->d.ssat;
+  f.sres_v = pb->d.sres_value; f.ssat_v = pb->d.ssat_value;
   return f;
 }
 
@@ -856,8 +900,8 @@ __global__ __launch_bounds__(256) void k_closure(Geom g, Phys f, int what,
   int ip = (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
   double pv = p[ip], a = f.alpha[ip], n = f.nvg[ip];
   switch (what) {
-    case 0: out[ip] = vg_sat(pv, f.rho, f.g, a, n, f.sres[ip], f.ssat[ip]); break;
-    case 1: out[ip] = vg_dsat(pv, f.rho, f.g, a, n, f.sres[ip], f.ssat[ip]); break;
+    case 0: out[ip] = vg_sat(pv, f.rho, f.g, a, n, f.Sres(ip), f.Ssat(ip)); break;
+    case 1: out[ip] = vg_dsat(pv, f.rho, f.g, a, n, f.Sres(ip), f.Ssat(ip)); break;
     case 2: out[ip] = vg_kr(pv, f.rho, f.g, a, n); break;
     case 3: out[ip] = vg_dkr(pv, f.rho, f.g, a, n); break;
     case 4: out[ip] = vg_kr(pv, f.rho, f.g, a, n) * f.rho; break;
@@ -1049,74 +1093,50 @@ __device__ __forceinline__ double upw(double a, double b, double c, double d)
   return ((a - b) >= 0) ? c : d;
 }
 
-__global__ __launch_bounds__(256) void k_residual(ResArgs A)
+// Face fluxes with the operands and operation order of
+// PL/nl_function_eval.c:696-854 (zero terrain slopes: x_dir_g = 0,
+// x_dir_g_c = gravity, z_mult = FB = 1).  a is the lower-index cell.
+__device__ __forceinline__ double flux_xy(double pa, double pb, double ka, double kb,
+                                          double kra, double krb, double ff, double d,
+                                          double gc, double mu)
+{
+  double diff = pa - pb;
+  double updir = (diff / d) * gc;
+  return ff * hmean(ka, kb) * (diff / d) * gc * upw(updir, 0.0, kra, krb) / mu;
+}
+
+__device__ __forceinline__ double flux_z(double pa, double pb, double ka, double kb,
+                                         double kra, double krb, double ffz, double dz,
+                                         double rho, double grav, double mu)
+{
+  double lc = pa / dz - 0.5 * rho * grav * 1.0;
+  double uc = pb / dz + 0.5 * rho * grav * 1.0;
+  double diff = lc - uc;
+  return ffz * hmean_dz1(ka, kb) * diff * upw(lc, uc, kra, krb) / mu;
+}
+
+// F of one interior cell: accumulation + storage + sources (:394-540), the
+// interior face fluxes (own faces out, incoming faces in, summed in the
+// oracle's order) and the boundary-face terms (:975-2260).
+__device__ __forceinline__ double residual_cell(const ResArgs &A, int i, int j, int k, int ip,
+                                                double pc, double krc, double S,
+                                                double kxc, double kyc, double kzc,
+                                                double ur, double uf, double uu,
+                                                double ul, double ud, double ub)
 {
   const Geom &g = A.g;
-  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= g.nx || j >= g.ny) return;
-  const int sy = g.sy, sz = g.sz;
-  int ip = (i + 1) + (j + 1) * sy + (k + 1) * sz;
   const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
   const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
   const double vol = dx * dy * dz;
   const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
-  const double gc = grav;
-  const double *__restrict__ p = A.p;
-  const double *__restrict__ kr = A.krho;
-  double pc = p[ip];
-  double krc = kr[ip];
-  double S = vg_sat(pc, rho, grav, A.f.alpha[ip], A.f.nvg[ip], A.f.sres[ip], A.f.ssat[ip]);
   double so = A.s_old[ip];
   double fv = (S * rho - so * rho) * A.poro[ip] * vol;
   fv += A.ss[ip] * vol * (pc * S * rho - A.p_old[ip] * so * rho);
   if (A.src) fv -= vol * dt * (A.src[ip]);
-
-  double ur = 0.0, uf = 0.0, uu = 0.0;
-  double kxc = A.kx[ip], kyc = A.ky[ip], kzc = A.kz[ip];
-  if (i + 1 < g.nx) {
-    int b = ip + 1;
-    double diff = pc - p[b];
-    double updir = (diff / dx) * gc;
-    ur = ffx * hmean(kxc, A.kx[b]) * (diff / dx) * gc * upw(updir, 0.0, krc, kr[b]) / mu;
-  }
-  if (j + 1 < g.ny) {
-    int b = ip + sy;
-    double diff = pc - p[b];
-    double updir = (diff / dy) * gc;
-    uf = ffy * hmean(kyc, A.ky[b]) * (diff / dy) * gc * upw(updir, 0.0, krc, kr[b]) / mu;
-  }
-  if (k + 1 < g.nz) {
-    int b = ip + sz;
-    double lc = pc / dz - 0.5 * rho * grav * 1.0;
-    double uc = p[b] / dz + 0.5 * rho * grav * 1.0;
-    double diff = lc - uc;
-    uu = ffz * hmean_dz1(kzc, A.kz[b]) * diff * upw(lc, uc, krc, kr[b]) / mu;
-  }
   fv += dt * (ur + uf + uu);
-  if (i > 0) {
-    int a = ip - 1;
-    double pa = p[a];
-    double diff = pa - pc;
-    double updir = (diff / dx) * gc;
-    double u = ffx * hmean(A.kx[a], kxc) * (diff / dx) * gc * upw(updir, 0.0, kr[a], krc) / mu;
-    fv += -dt * u;
-  }
-  if (j > 0) {
-    int a = ip - sy;
-    double pa = p[a];
-    double diff = pa - pc;
-    double updir = (diff / dy) * gc;
-    double u = ffy * hmean(A.ky[a], kyc) * (diff / dy) * gc * upw(updir, 0.0, kr[a], krc) / mu;
-    fv += -dt * u;
-  }
-  if (k > 0) {
-    int a = ip - sz;
-    double lc = p[a] / dz - 0.5 * rho * grav * 1.0;
-    double uc = pc / dz + 0.5 * rho * grav * 1.0;
-    double diff = lc - uc;
-    double u = ffz * hmean_dz1(A.kz[a], kzc) * diff * upw(lc, uc, kr[a], krc) / mu;
-    fv += -dt * u;
-  }
+  if (i > 0) fv += -dt * ul;
+  if (j > 0) fv += -dt * ud;
+  if (k > 0) fv += -dt * ub;
 
   bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
              (k == g.nz - 1);
@@ -1136,6 +1156,7 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs A)
         double u_new = h * value;
         fv += dt * dir * u_new;
       } else if (type == PFHIP_BC_DIR_EQUIL) {
+        // ghost kr from the Dirichlet value inserted at :574-610
         double kg = vg_kr(value, rho, grav, A.f.alpha[ip], A.f.nvg[ip]) * rho;
         double u_new;
         if (fc == 0) {
@@ -1166,9 +1187,9 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs A)
         fv += dt * dir * u_new;
       } else if (fc == 5) {
         // OverlandKinematicBC, PL/nl_function_eval.c:1786-2010
-        int io = (i + 1) + (j + 1) * sy;
+        int io = (i + 1) + (j + 1) * g.sy;
         double ke = A.qx[io], kw = A.qx[io - 1];
-        double kn = A.qy[io], ks = A.qy[io - sy];
+        double kn = A.qy[io], ks = A.qy[io - g.sy];
         double q_overlnd = vol * (fmax(pc, 0.0) - fmax(A.p_old[ip], 0.0)) / dz +
                            dt * vol * ((ke - kw) / dx + (kn - ks) / dy) / dz;
         fv += q_overlnd;
@@ -1177,7 +1198,166 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs A)
       }
     }
   }
-  A.F[ip] = fv;
+  return fv;
+}
+
+// Fused residual: one launch per evaluation.  A 64 x TY block owns a tile
+// of columns and marches up a chunk of ZC z-planes.  Per plane:
+//  A. every thread computes S and kr*rho of its own cell one plane ahead
+//     (registers, for the z faces); the 2*(64+TY) edge-halo cells' pressure
+//     and kr*rho go to LDS with the tile's own;
+//  B. every cell computes the fluxes through its own x+, y+, z+ faces once;
+//     the west/south halo cells compute the face into the tile; x/y faces
+//     go to LDS, the z face stays in a register for the plane above;
+//  C. each cell sums outgoing minus incoming face fluxes.
+// So each face flux and each kr is evaluated once (plus the halo share),
+// and p, alpha, n, K stream from HBM once.
+#define RTX 64
+
+template <int TY, int ZC>
+__global__ __launch_bounds__(RTX * TY) void k_residual_fused(ResArgs A)
+{
+  constexpr int HALO = 2 * RTX + 2 * TY;
+  constexpr int LDSN = (RTX + 2) * (TY + 2);
+  __shared__ double s_p[LDSN];
+  __shared__ double s_kr[LDSN];
+  __shared__ double s_ux[TY * (RTX + 1)];   // x faces: west face of tile column tx at tx
+  __shared__ double s_uy[(TY + 1) * RTX];   // y faces: south face of tile row ty at ty
+  const Geom g = A.g;
+  const Phys &f = A.f;
+  const double rho = f.rho, grav = f.g, mu = f.mu;
+  const double dx = A.dx, dy = A.dy, dz = A.dz;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * RTX + tx;
+  const int i = blockIdx.x * RTX + tx, j = blockIdx.y * TY + ty;
+  const bool act = (i < g.nx) && (j < g.ny);
+  const int ls = (ty + 1) * (RTX + 2) + (tx + 1);
+  // edge-halo cell of this thread (corners are not needed by 7 points)
+  int hx = 0, hy = 0;
+  bool hact = false;
+  if (tid < HALO) {
+    if (tid < RTX) { hx = tid + 1; hy = 0; }
+    else if (tid < 2 * RTX) { hx = tid - RTX + 1; hy = TY + 1; }
+    else if (tid < 2 * RTX + TY) { hx = 0; hy = tid - 2 * RTX + 1; }
+    else { hx = RTX + 1; hy = tid - 2 * RTX - TY + 1; }
+    int hi = blockIdx.x * RTX + hx - 1, hj = blockIdx.y * TY + hy - 1;
+    hact = (hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny);
+  }
+  const int hls = hy * (RTX + 2) + hx;
+  const int hcol = (blockIdx.x * RTX + hx) + (blockIdx.y * TY + hy) * g.sy;
+  // west halo (hx == 0) supplies the x face into the tile, south halo
+  // (hy == 0) the y face into the tile
+  const bool hwest = hact && hx == 0;
+  const bool hsouth = hact && hy == 0;
+  const int col = (i + 1) + (j + 1) * g.sy;
+  const int k0 = blockIdx.z * ZC;
+  const int k1 = min(k0 + ZC, g.nz);
+
+  double p_m = 0, kr_m = 0, kz_m = 0;
+  double p_c = 0, kr_c = 0, S_c = 0, kz_c = 0;
+  double p_n = 0, kr_n = 0, S_n = 0, kz_n = 0;
+  if (act) {
+    if (k0 > 0) {
+      int ipm = col + k0 * g.sz;
+      p_m = A.p[ipm];
+      kr_m = vg_kr(p_m, rho, grav, f.alpha[ipm], f.nvg[ipm]) * rho;
+      kz_m = A.kz[ipm];
+    }
+    int ip0 = col + (k0 + 1) * g.sz;
+    p_c = A.p[ip0];
+    vg_s_kr(p_c, rho, grav, f.alpha[ip0], f.nvg[ip0], f.Sres(ip0), f.Ssat(ip0), S_c, kr_c);
+    kr_c *= rho;
+    kz_c = A.kz[ip0];
+  }
+  // flux through the lower z face of the own cell at plane k0
+  double ub = 0.0;
+  if (act && k0 > 0) ub = flux_z(p_m, p_c, kz_m, kz_c, kr_m, kr_c, ffz, dz, rho, grav, mu);
+
+  for (int k = k0; k < k1; k++) {
+    const int ip = col + (k + 1) * g.sz;
+    // ---- A
+    if (act && k + 1 < g.nz) {
+      int ipn = ip + g.sz;
+      p_n = A.p[ipn];
+      vg_s_kr(p_n, rho, grav, f.alpha[ipn], f.nvg[ipn], f.Sres(ipn), f.Ssat(ipn), S_n, kr_n);
+      kr_n *= rho;
+      kz_n = A.kz[ipn];
+    }
+    int hip = hcol + (k + 1) * g.sz;
+    if (hact) {
+      double ph = A.p[hip];
+      s_p[hls] = ph;
+      s_kr[hls] = vg_kr(ph, rho, grav, f.alpha[hip], f.nvg[hip]) * rho;
+    }
+    if (act) { s_p[ls] = p_c; s_kr[ls] = kr_c; }
+    __syncthreads();
+    // ---- B
+    double ur = 0.0, uf = 0.0, uu = 0.0, kxc = 0.0, kyc = 0.0;
+    if (act) {
+      kxc = A.kx[ip];
+      kyc = A.ky[ip];
+      if (i + 1 < g.nx)
+        ur = flux_xy(p_c, s_p[ls + 1], kxc, A.kx[ip + 1], kr_c, s_kr[ls + 1], ffx, dx,
+                     grav, mu);
+      if (j + 1 < g.ny)
+        uf = flux_xy(p_c, s_p[ls + RTX + 2], kyc, A.ky[ip + g.sy], kr_c,
+                     s_kr[ls + RTX + 2], ffy, dy, grav, mu);
+      if (k + 1 < g.nz)
+        uu = flux_z(p_c, p_n, kz_c, kz_n, kr_c, kr_n, ffz, dz, rho, grav, mu);
+      s_ux[ty * (RTX + 1) + tx + 1] = ur;
+      s_uy[(ty + 1) * RTX + tx] = uf;
+    }
+    if (hwest) {
+      // west halo cell -> face into tile column 0 (that tile cell is in the
+      // domain: the block's first column always is, and hj < ny)
+      s_ux[(hy - 1) * (RTX + 1)] =
+          flux_xy(s_p[hls], s_p[hls + 1], A.kx[hip], A.kx[hip + 1], s_kr[hls],
+                  s_kr[hls + 1], ffx, dx, grav, mu);
+    }
+    if (hsouth) {
+      s_uy[hx - 1] = flux_xy(s_p[hls], s_p[hls + RTX + 2], A.ky[hip], A.ky[hip + g.sy],
+                             s_kr[hls], s_kr[hls + RTX + 2], ffy, dy, grav, mu);
+    }
+    __syncthreads();
+    // ---- C
+    if (act) {
+      double ul = s_ux[ty * (RTX + 1) + tx];
+      double ud = s_uy[ty * RTX + tx];
+      A.F[ip] = residual_cell(A, i, j, k, ip, p_c, kr_c, S_c, kxc, kyc, kz_c, ur, uf, uu,
+                              ul, ud, ub);
+    }
+    ub = uu;
+    p_m = p_c; kr_m = kr_c; kz_m = kz_c;
+    p_c = p_n; kr_c = kr_n; S_c = S_n; kz_c = kz_n;
+  }
+}
+
+// tile height / z-chunk variants; PFHIP_RES_VARIANT selects one (default 0)
+static int residual_variant()
+{
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("PFHIP_RES_VARIANT");
+    v = e ? atoi(e) : 0;
+  }
+  return v;
+}
+
+static void launch_residual(const ResArgs &A, hipStream_t st)
+{
+  const Geom &g = A.g;
+  int v = residual_variant();
+  int TY = (v == 0 || v == 1 || v == 4) ? 8 : 4;
+  int ZC = (v == 0) ? 16 : (v == 1) ? 64 : (v == 2) ? 16 : (v == 3) ? 8 : 4;
+  dim3 grid((g.nx + RTX - 1) / RTX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC);
+  dim3 block(RTX, TY, 1);
+  switch (v) {
+    case 1: hipLaunchKernelGGL((k_residual_fused<8, 64>), grid, block, 0, st, A); break;
+    case 2: hipLaunchKernelGGL((k_residual_fused<4, 16>), grid, block, 0, st, A); break;
+    case 3: hipLaunchKernelGGL((k_residual_fused<4, 8>), grid, block, 0, st, A); break;
+    case 4: hipLaunchKernelGGL((k_residual_fused<8, 4>), grid, block, 0, st, A); break;
+    default: hipLaunchKernelGGL((k_residual_fused<8, 16>), grid, block, 0, st, A); break;
+  }
 }
 
 extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double *F,
@@ -1185,8 +1365,6 @@ extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double
                                       double dt, void *stream)
 {
   Geom g = geom_of(pb);
-  int rc = launch_closure(pb, 4, p, pb->krho, nullptr, stream);
-  if (rc) return rc;
   if (pb->overland) {
     hipLaunchKernelGGL(k_overland_q, grid2(g), BLOCK3, 0, STREAM(stream), g,
                        surf_of(pb), p, pb->q2[0], pb->q2[1]);
@@ -1203,7 +1381,7 @@ extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double
   for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
   A.F = F;
   PROF_BEGIN(PFHIP_PROF_RESIDUAL, stream);
-  hipLaunchKernelGGL(k_residual, grid3(g), BLOCK3, 0, STREAM(stream), A);
+  launch_residual(A, STREAM(stream));
   PROF_END(PFHIP_PROF_RESIDUAL, stream);
   LAUNCHCHK();
   return PFHIP_OK;
@@ -1259,8 +1437,8 @@ __global__ __launch_bounds__(256) void k_jacobian(JacArgs A)
   const bool symm = A.symm != 0;
   double pc = p[ip];
   double a = A.f.alpha[ip], nn = A.f.nvg[ip];
-  double S = vg_sat(pc, rho, grav, a, nn, A.f.sres[ip], A.f.ssat[ip]);
-  double dS = vg_dsat(pc, rho, grav, a, nn, A.f.sres[ip], A.f.ssat[ip]);
+  double S = vg_sat(pc, rho, grav, a, nn, A.f.Sres(ip), A.f.Ssat(ip));
+  double dS = vg_dsat(pc, rho, grav, a, nn, A.f.Sres(ip), A.f.Ssat(ip));
   double vol2 = vol * 1.0;
   double c = (dS * rho + S * ddp) * A.poro[ip] * vol2 +
              A.ss[ip] * vol2 * (dS * rho * pc + S * ddp * pc + S * rho);

@@ -10,8 +10,19 @@
 //
 // One template serves FP64 and FP32 level storage (the mixed-precision
 // preconditioner keeps every level, vector and operator in FP32; only the
-// interface vectors are FP64).  Every kernel is a one-thread-per-point
-// x-coalesced pass over one level; matrices are symmetric-stored C,E,N,U.
+// interface vectors are FP64).  Matrices are symmetric-stored C,E,N,U.
+//
+// MI355X mapping:
+//  * every per-point operation is a device function; large levels run them
+//    as one-thread-per-point x-coalesced grid kernels whose tiles are
+//    renumbered so each of the 8 XCDs owns a contiguous z-range (the
+//    hardware deals workgroups to XCDs round-robin; without the renumbering
+//    the z-neighbour planes of a tile sit in other XCDs' L2);
+//  * every level with <= MG_SMALL_DEFAULT points (the bottom of the V-cycle,
+//    a dozen levels of a semicoarsened hierarchy; PFHIP_MG_SMALL overrides
+//    the bound) runs inside ONE workgroup of 1024 threads, phase by phase
+//    with __syncthreads, instead of ~100 launches that are pure launch
+//    latency.
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <string.h>
@@ -30,6 +41,8 @@
 #define LAUNCHCHK() HIPCHK(hipGetLastError())
 #define STREAM(s) ((hipStream_t)(s))
 #define MAXL 128
+#define MG_SMALL_DEFAULT 4096
+#define CWG 1024
 
 struct LGeom {
   int n[3];
@@ -44,16 +57,58 @@ static LGeom lgeom(const int *n)
   return g;
 }
 static long lsize(const int *n) { return (long)(n[0] + 2) * (n[1] + 2) * (n[2] + 2); }
-static dim3 lgrid(const LGeom &g) { return dim3((g.n[0] + 63) / 64, (g.n[1] + 3) / 4, g.n[2]); }
-#define LBLOCK dim3(64, 4, 1)
+static long npts(const int *n) { return (long)n[0] * n[1] * n[2]; }
 
 #define LIDX(g, i, j, k) ((i) + 1 + ((j) + 1) * (g).s[1] + ((k) + 1) * (g).s[2])
-#define THREAD_IJK(g)                                                            \
-  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y,      \
-      k = blockIdx.z;                                                            \
-  if (i >= (g).n[0] || j >= (g).n[1]) return;
+
+// ---- XCD-aware tile numbering ---------------------------------------------
+// Tiles are 64 x 4 points of one plane; nb tiles in x-fastest order.  The
+// launch has nb rounded up to a multiple of 8 workgroups; hardware block b
+// runs on XCD b % 8, so logical tile (b % 8) * (nb8 / 8) + b / 8 gives XCD x
+// the contiguous tile range [x * nb8/8, (x+1) * nb8/8).
+static inline unsigned ntiles(const LGeom &g)
+{
+  unsigned nb = (unsigned)((g.n[0] + 63) / 64) * ((g.n[1] + 3) / 4) * g.n[2];
+  return (nb + 7u) & ~7u;
+}
+#define LBLOCK dim3(64, 4, 1)
+
+__device__ __forceinline__ bool xcd_point(const LGeom &g, int &i, int &j, int &k)
+{
+  int gx = (g.n[0] + 63) >> 6, gy = (g.n[1] + 3) >> 2;
+  int nb = gx * gy * g.n[2];
+  int nb8 = (nb + 7) & ~7;
+  int b = blockIdx.x;
+  int L = (b & 7) * (nb8 >> 3) + (b >> 3);
+  if (L >= nb) return false;
+  int bx = L % gx, r = L / gx;
+  int by = r % gy, bz = r / gy;
+  i = bx * 64 + threadIdx.x;
+  j = by * 4 + threadIdx.y;
+  k = bz;
+  return i < g.n[0] && j < g.n[1];
+}
+
+#define XCD_IJK(g)                    \
+  int i, j, k;                        \
+  if (!xcd_point((g), i, j, k)) return;
+
+__device__ __forceinline__ void unflat(const LGeom &g, int idx, int &i, int &j, int &k)
+{
+  i = idx % g.n[0];
+  int r = idx / g.n[0];
+  j = r % g.n[1];
+  k = r / g.n[1];
+}
 
 template <typename T> struct Mat { T *c, *u[3]; };   // C and upper E, N, U
+
+template <typename T> struct LevelDev {
+  LGeom g;
+  int dir, o1, o2;
+  Mat<T> A;
+  T *x, *b, *t, *p1, *p2;
+};
 
 // coupling of point ip in direction q on side -1/+1 (symmetric storage)
 template <typename T>
@@ -62,12 +117,12 @@ __device__ __forceinline__ T cpl(const Mat<T> &A, const LGeom &g, long ip, int q
   return (side > 0) ? A.u[q][ip] : A.u[q][ip - g.s[q]];
 }
 
-// ---- setup ------------------------------------------------------------------
+// ---- per-point operations -------------------------------------------------
+// prolongation weights at fine (odd) points, PL/mg_semi.c:505-545
 template <typename T>
-__global__ __launch_bounds__(256) void k_pweights(LGeom g, Mat<T> A, int d, int o1, int o2,
-                                                  T *__restrict__ p1, T *__restrict__ p2)
+__device__ __forceinline__ void pw_point(const LGeom &g, const Mat<T> &A, int d, int o1, int o2,
+                                         T *p1, T *p2, int i, int j, int k)
 {
-  THREAD_IJK(g);
   int id[3] = { i, j, k };
   if ((id[d] & 1) == 0) return;
   long ip = LIDX(g, i, j, k);
@@ -80,13 +135,12 @@ __global__ __launch_bounds__(256) void k_pweights(LGeom g, Mat<T> A, int d, int 
   else { p1[ip] = T(0); p2[ip] = T(0); }
 }
 
+// coarse operator at coarse point (i,j,k), PL/mg_semi.c:560-700
 template <typename T>
-__global__ __launch_bounds__(256) void k_coarse_op(LGeom gf, LGeom gc, Mat<T> A, Mat<T> Ac,
-                                                   int d, int o1, int o2,
-                                                   const T *__restrict__ p1,
-                                                   const T *__restrict__ p2)
+__device__ __forceinline__ void cop_point(const LGeom &gf, const LGeom &gc, const Mat<T> &A,
+                                          const Mat<T> &Ac, int d, int o1, int o2,
+                                          const T *p1, const T *p2, int i, int j, int k)
 {
-  THREAD_IJK(gc);
   int cid[3] = { i, j, k };
   int fi[3] = { i, j, k };
   fi[d] *= 2;
@@ -109,10 +163,11 @@ __global__ __launch_bounds__(256) void k_coarse_op(LGeom gf, LGeom gc, Mat<T> A,
   Ac.c[iAc] = a0 + a3 + a4 + a5 + a6 + a1 * p2[iA1] + a2 * p1[iA2];
 }
 
+// centre completion, PL/mg_semi.c:720-750
 template <typename T>
-__global__ __launch_bounds__(256) void k_coarse_diag(LGeom gc, Mat<T> Ac, int o1, int o2)
+__device__ __forceinline__ void cdiag_point(const LGeom &gc, const Mat<T> &Ac, int o1, int o2,
+                                            int i, int j, int k)
 {
-  THREAD_IJK(gc);
   int cid[3] = { i, j, k };
   long iAc = LIDX(gc, i, j, k);
   T c3 = (cid[o1] > 0) ? Ac.u[o1][iAc - gc.s[o1]] : T(0);
@@ -122,13 +177,11 @@ __global__ __launch_bounds__(256) void k_coarse_diag(LGeom gc, Mat<T> Ac, int o1
   Ac.c[iAc] -= (c3 + c4 + c5 + c6);
 }
 
-// ---- cycle kernels ----------------------------------------------------------
-// red/black sweep of parity `par` ((i+j+k)&1 == par); init: x = b/a0
+// red/black update of one point of parity `par`; init: x = b / a0
 template <typename T>
-__global__ __launch_bounds__(256) void k_rbgs(LGeom g, Mat<T> A, T *__restrict__ x,
-                                              const T *__restrict__ b, int par, int init)
+__device__ __forceinline__ void rbgs_point(const LGeom &g, const Mat<T> &A, T *x, const T *b,
+                                           int par, int init, int i, int j, int k)
 {
-  THREAD_IJK(g);
   if (((i + j + k) & 1) != par) return;
   long ip = LIDX(g, i, j, k);
   if (init) { x[ip] = b[ip] / A.c[ip]; return; }
@@ -144,10 +197,9 @@ __global__ __launch_bounds__(256) void k_rbgs(LGeom g, Mat<T> A, T *__restrict__
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_residual_l(LGeom g, Mat<T> A, const T *__restrict__ x,
-                                                    const T *__restrict__ b, T *__restrict__ t)
+__device__ __forceinline__ void resid_point(const LGeom &g, const Mat<T> &A, const T *x,
+                                            const T *b, T *t, int i, int j, int k)
 {
-  THREAD_IJK(g);
   long ip = LIDX(g, i, j, k);
   const int sy = g.s[1], sz = g.s[2];
   T v = A.c[ip] * x[ip];
@@ -160,13 +212,12 @@ __global__ __launch_bounds__(256) void k_residual_l(LGeom g, Mat<T> A, const T *
   t[ip] = b[ip] - v;
 }
 
+// r_c = r_f[C] + p2[C-1] r_f[C-1] + p1[C+1] r_f[C+1]  (mg_semi_restrict.c:120-150)
 template <typename T>
-__global__ __launch_bounds__(256) void k_restrict(LGeom gf, LGeom gc, int d,
-                                                  const T *__restrict__ rf, T *__restrict__ rc,
-                                                  const T *__restrict__ p1,
-                                                  const T *__restrict__ p2)
+__device__ __forceinline__ void restrict_point(const LGeom &gf, const LGeom &gc, int d,
+                                               const T *rf, T *rc, const T *p1, const T *p2,
+                                               int i, int j, int k)
 {
-  THREAD_IJK(gc);
   int fi[3] = { i, j, k };
   fi[d] *= 2;
   long iF = LIDX(gf, fi[0], fi[1], fi[2]);
@@ -176,26 +227,22 @@ __global__ __launch_bounds__(256) void k_restrict(LGeom gf, LGeom gc, int d,
   rc[LIDX(gc, i, j, k)] = rf[iF] + (lo + hi);
 }
 
-// x_f += P e_c (injection at coarse points, interpolation at fine points)
+// x_f += P e_c (injection at coarse points, interpolation at fine points;
+// mg_semi_prolong.c:85-160)
 template <typename T>
-__global__ __launch_bounds__(256) void k_prolong_add(LGeom gf, LGeom gc, int d,
-                                                     const T *__restrict__ ec,
-                                                     T *__restrict__ xf,
-                                                     const T *__restrict__ p1,
-                                                     const T *__restrict__ p2)
+__device__ __forceinline__ void prolong_point(const LGeom &gf, const LGeom &gc, int d,
+                                              const T *ec, T *xf, const T *p1, const T *p2,
+                                              int i, int j, int k)
 {
-  THREAD_IJK(gf);
   int id[3] = { i, j, k };
   long ip = LIDX(gf, i, j, k);
   int c[3] = { i, j, k };
+  c[d] = id[d] >> 1;
   if ((id[d] & 1) == 0) {
-    c[d] = id[d] >> 1;
     xf[ip] += ec[LIDX(gc, c[0], c[1], c[2])];
     return;
   }
-  c[d] = id[d] >> 1;
-  T elo = ec[LIDX(gc, c[0], c[1], c[2])];
-  T lo = p1[ip] * elo;
+  T lo = p1[ip] * ec[LIDX(gc, c[0], c[1], c[2])];
   T hi = T(0);
   if (id[d] + 1 < gf.n[d]) {
     c[d] = (id[d] + 1) >> 1;
@@ -204,11 +251,73 @@ __global__ __launch_bounds__(256) void k_prolong_add(LGeom gf, LGeom gc, int d,
   xf[ip] += lo + hi;
 }
 
+// ---- grid kernels (large levels) --------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_pweights(LGeom g, Mat<T> A, int d, int o1, int o2,
+                                                  T *__restrict__ p1, T *__restrict__ p2)
+{
+  XCD_IJK(g);
+  pw_point(g, A, d, o1, o2, p1, p2, i, j, k);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_op(LGeom gf, LGeom gc, Mat<T> A, Mat<T> Ac,
+                                                   int d, int o1, int o2, const T *p1,
+                                                   const T *p2)
+{
+  XCD_IJK(gc);
+  cop_point(gf, gc, A, Ac, d, o1, o2, p1, p2, i, j, k);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_diag(LGeom gc, Mat<T> Ac, int o1, int o2)
+{
+  XCD_IJK(gc);
+  cdiag_point(gc, Ac, o1, o2, i, j, k);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rbgs(LGeom g, Mat<T> A, T *__restrict__ x,
+                                              const T *__restrict__ b, int par, int init)
+{
+  XCD_IJK(g);
+  rbgs_point(g, A, x, b, par, init, i, j, k);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_residual_l(LGeom g, Mat<T> A, const T *__restrict__ x,
+                                                    const T *__restrict__ b, T *__restrict__ t)
+{
+  XCD_IJK(g);
+  resid_point(g, A, x, b, t, i, j, k);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict(LGeom gf, LGeom gc, int d,
+                                                  const T *__restrict__ rf, T *__restrict__ rc,
+                                                  const T *__restrict__ p1,
+                                                  const T *__restrict__ p2)
+{
+  XCD_IJK(gc);
+  restrict_point(gf, gc, d, rf, rc, p1, p2, i, j, k);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong_add(LGeom gf, LGeom gc, int d,
+                                                     const T *__restrict__ ec,
+                                                     T *__restrict__ xf,
+                                                     const T *__restrict__ p1,
+                                                     const T *__restrict__ p2)
+{
+  XCD_IJK(gf);
+  prolong_point(gf, gc, d, ec, xf, p1, p2, i, j, k);
+}
+
 template <typename S, typename D>
 __global__ __launch_bounds__(256) void k_convert(LGeom g, const S *__restrict__ src,
                                                  D *__restrict__ dst)
 {
-  THREAD_IJK(g);
+  XCD_IJK(g);
   long ip = LIDX(g, i, j, k);
   dst[ip] = (D)src[ip];
 }
@@ -220,13 +329,85 @@ __global__ void k_fill_t(T *p, long n)
   for (; i < n; i += (long)gridDim.x * blockDim.x) p[i] = T(0);
 }
 
+// ---- single-workgroup kernels for the small levels ------------------------
+// Phases over a level's points; every phase ends with a block barrier.
+#define FOR_POINTS(g, body)                                              \
+  {                                                                      \
+    int np_ = (g).n[0] * (g).n[1] * (g).n[2];                            \
+    for (int idx_ = threadIdx.x; idx_ < np_; idx_ += blockDim.x) {       \
+      int i, j, k;                                                       \
+      unflat((g), idx_, i, j, k);                                        \
+      body;                                                              \
+    }                                                                    \
+    __syncthreads();                                                     \
+  }
+
+// RedBlackGSPoint call sequence (PL/rb_GS_point.c:100-300) inside a block
+template <typename T>
+__device__ void rbgs_block(const LevelDev<T> &L, int zero, int max_iter, int symmetric)
+{
+  const LGeom &g = L.g;
+  int count = symmetric ? 1 : 2;
+  int rb = 0;
+  if (zero) {
+    count++;
+    if ((count / 2) > max_iter) {
+      FOR_POINTS(g, L.x[LIDX(g, i, j, k)] = T(0));
+      if (!symmetric) return;
+    }
+    FOR_POINTS(g, rbgs_point(g, L.A, L.x, L.b, 1, 1, i, j, k));
+    rb = !rb;
+  }
+  for (; (count / 2) <= max_iter; count++) {
+    int par = rb ? 0 : 1;
+    FOR_POINTS(g, rbgs_point(g, L.A, L.x, L.b, par, 0, i, j, k));
+    rb = !rb;
+  }
+}
+
+// bottom of the V-cycle: levels lc .. L-1 (b[lc] given, x[lc] returned)
+template <typename T>
+__global__ __launch_bounds__(CWG) void k_coarse_vcycle(const LevelDev<T> *__restrict__ lev,
+                                                       int lc, int L, int pre, int post,
+                                                       int symmetric)
+{
+  int l;
+  for (l = lc; l <= L - 2; l++) {
+    const LevelDev<T> &F = lev[l], &Cc = lev[l + 1];
+    rbgs_block(F, 1, pre, symmetric);
+    FOR_POINTS(F.g, resid_point(F.g, F.A, F.x, F.b, F.t, i, j, k));
+    FOR_POINTS(Cc.g, restrict_point(F.g, Cc.g, F.dir, F.t, Cc.b, F.p1, F.p2, i, j, k));
+  }
+  rbgs_block(lev[L - 1], 1, 1, symmetric);
+  for (l = L - 2; l >= lc; l--) {
+    const LevelDev<T> &F = lev[l], &Cc = lev[l + 1];
+    FOR_POINTS(F.g, prolong_point(F.g, Cc.g, F.dir, Cc.x, F.x, F.p1, F.p2, i, j, k));
+    rbgs_block(F, 0, post, symmetric);
+  }
+}
+
+// coarse operators of levels ls .. L-2 -> ls+1 .. L-1
+template <typename T>
+__global__ __launch_bounds__(CWG) void k_coarse_setup(const LevelDev<T> *__restrict__ lev,
+                                                      int ls, int L)
+{
+  for (int l = ls; l <= L - 2; l++) {
+    const LevelDev<T> &F = lev[l], &Cc = lev[l + 1];
+    FOR_POINTS(F.g, pw_point(F.g, F.A, F.dir, F.o1, F.o2, F.p1, F.p2, i, j, k));
+    FOR_POINTS(Cc.g, cop_point(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1, F.p2, i, j, k));
+    FOR_POINTS(Cc.g, cdiag_point(Cc.g, Cc.A, F.o1, F.o2, i, j, k));
+  }
+}
+
 // ---- host side ----------------------------------------------------------------
 struct pfhip_mg {
   pfhip_problem *pb;
   int L, prec, pre, post, symmetric;
+  int lc;                      // first level run by the single-workgroup kernels
   int n[MAXL][3], dir[MAXL];
   void *A[MAXL][4];
   void *x[MAXL], *b[MAXL], *t[MAXL], *p1[MAXL], *p2[MAXL];
+  void *levdev;                // device LevelDev<T>[L]
 };
 
 template <typename T> static Mat<T> matof(pfhip_mg *mg, int l)
@@ -243,6 +424,24 @@ static void offdirs(int d, int *o1, int *o2)
   if (d == 0) { *o1 = 1; *o2 = 2; }
   else if (d == 1) { *o1 = 0; *o2 = 2; }
   else { *o1 = 1; *o2 = 0; }
+}
+
+template <typename T> static int upload_levels(pfhip_mg *mg)
+{
+  LevelDev<T> h[MAXL];
+  memset(h, 0, sizeof(h));
+  for (int l = 0; l < mg->L; l++) {
+    h[l].g = lgeom(mg->n[l]);
+    h[l].dir = (l < mg->L - 1) ? mg->dir[l] : 0;
+    offdirs(h[l].dir, &h[l].o1, &h[l].o2);
+    h[l].A = matof<T>(mg, l);
+    h[l].x = (T *)mg->x[l]; h[l].b = (T *)mg->b[l]; h[l].t = (T *)mg->t[l];
+    h[l].p1 = (T *)mg->p1[l]; h[l].p2 = (T *)mg->p2[l];
+  }
+  int rc = pfhip_malloc(&mg->levdev, (long)sizeof(LevelDev<T>) * mg->L);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(mg->levdev, h, sizeof(LevelDev<T>) * mg->L, hipMemcpyHostToDevice));
+  return PFHIP_OK;
 }
 
 extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int post,
@@ -279,6 +478,13 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
     pfhip_set_error("grid too small for a V-cycle");
     return PFHIP_ERR_ARG;
   }
+  // first level small enough for the single-workgroup bottom (never level 0)
+  long small = MG_SMALL_DEFAULT;
+  const char *env = getenv("PFHIP_MG_SMALL");
+  if (env && *env) small = atol(env);
+  mg->lc = mg->L - 1;
+  for (l = 1; l < mg->L; l++)
+    if (npts(mg->n[l]) <= small) { mg->lc = l; break; }
   size_t es = (precision == 64) ? sizeof(double) : sizeof(float);
   int rc = PFHIP_OK;
   for (l = 0; l < mg->L && rc == PFHIP_OK; l++) {
@@ -293,6 +499,8 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
       }
     }
   }
+  if (rc == PFHIP_OK)
+    rc = (precision == 64) ? upload_levels<double>(mg) : upload_levels<float>(mg);
   if (rc != PFHIP_OK) { pfhip_mg_destroy(mg); return rc; }
   HIPCHK(hipDeviceSynchronize());
   *out = mg;
@@ -307,6 +515,7 @@ extern "C" int pfhip_mg_destroy(pfhip_mg *mg)
     void *o[5] = { mg->x[l], mg->b[l], mg->t[l], mg->p1[l], mg->p2[l] };
     for (int q = 0; q < 5; q++) if (o[q]) hipFree(o[q]);
   }
+  if (mg->levdev) hipFree(mg->levdev);
   free(mg);
   return PFHIP_OK;
 }
@@ -323,23 +532,30 @@ static int mg_setup_t(pfhip_mg *mg, const double *const *Asym, hipStream_t st)
   LGeom g0 = lgeom(mg->n[0]);
   const double *src[4] = { Asym[0], Asym[2], Asym[4], Asym[6] };
   for (int q = 0; q < 4; q++) {
-    hipLaunchKernelGGL((k_convert<double, T>), lgrid(g0), LBLOCK, 0, st, g0, src[q],
+    hipLaunchKernelGGL((k_convert<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, src[q],
                        (T *)mg->A[0][q]);
     LAUNCHCHK();
   }
-  for (int l = 0; l < mg->L - 1; l++) {
+  // large levels: grid kernels down to (and into) level lc
+  for (int l = 0; l < mg->lc; l++) {
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
     int d = mg->dir[l], o1, o2;
     offdirs(d, &o1, &o2);
-    hipLaunchKernelGGL(k_pweights<T>, lgrid(gf), LBLOCK, 0, st, gf, matof<T>(mg, l), d, o1,
-                       o2, (T *)mg->p1[l], (T *)mg->p2[l]);
+    hipLaunchKernelGGL(k_pweights<T>, dim3(ntiles(gf)), LBLOCK, 0, st, gf, matof<T>(mg, l), d,
+                       o1, o2, (T *)mg->p1[l], (T *)mg->p2[l]);
     LAUNCHCHK();
-    hipLaunchKernelGGL(k_coarse_op<T>, lgrid(gc), LBLOCK, 0, st, gf, gc, matof<T>(mg, l),
-                       matof<T>(mg, l + 1), d, o1, o2, (const T *)mg->p1[l],
-                       (const T *)mg->p2[l]);
+    hipLaunchKernelGGL(k_coarse_op<T>, dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc,
+                       matof<T>(mg, l), matof<T>(mg, l + 1), d, o1, o2,
+                       (const T *)mg->p1[l], (const T *)mg->p2[l]);
     LAUNCHCHK();
-    hipLaunchKernelGGL(k_coarse_diag<T>, lgrid(gc), LBLOCK, 0, st, gc, matof<T>(mg, l + 1),
-                       o1, o2);
+    hipLaunchKernelGGL(k_coarse_diag<T>, dim3(ntiles(gc)), LBLOCK, 0, st, gc,
+                       matof<T>(mg, l + 1), o1, o2);
+    LAUNCHCHK();
+  }
+  // small levels: one workgroup
+  if (mg->lc <= mg->L - 2) {
+    hipLaunchKernelGGL(k_coarse_setup<T>, dim3(1), dim3(CWG), 0, st,
+                       (const LevelDev<T> *)mg->levdev, mg->lc, mg->L);
     LAUNCHCHK();
   }
   return PFHIP_OK;
@@ -355,11 +571,12 @@ extern "C" int pfhip_mg_setup(pfhip_mg *mg, const double *const *Asym, void *str
                           : mg_setup_t<float>(mg, Asym, STREAM(stream));
 }
 
-// RedBlackGSPoint call sequence (PL/rb_GS_point.c:100-300)
+// RedBlackGSPoint call sequence (PL/rb_GS_point.c:100-300) with grid kernels
 template <typename T>
 static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
 {
   LGeom g = lgeom(mg->n[l]);
+  dim3 grid(ntiles(g));
   Mat<T> A = matof<T>(mg, l);
   T *x = (T *)mg->x[l];
   const T *b = (const T *)mg->b[l];
@@ -373,14 +590,14 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
       LAUNCHCHK();
       if (!mg->symmetric) return PFHIP_OK;
     }
-    hipLaunchKernelGGL(k_rbgs<T>, lgrid(g), LBLOCK, 0, st, g, A, x, b, 1, 1);
+    hipLaunchKernelGGL(k_rbgs<T>, grid, LBLOCK, 0, st, g, A, x, b, 1, 1);
     LAUNCHCHK();
     rb = !rb;
   }
   for (; (count / 2) <= max_iter; count++) {
     int par = rb ? 0 : 1;
     if (l == 0) pfhip_prof_mark(PFHIP_PROF_RBGS, 0, st);
-    hipLaunchKernelGGL(k_rbgs<T>, lgrid(g), LBLOCK, 0, st, g, A, x, b, par, 0);
+    hipLaunchKernelGGL(k_rbgs<T>, grid, LBLOCK, 0, st, g, A, x, b, par, 0);
     if (l == 0) pfhip_prof_mark(PFHIP_PROF_RBGS, 1, st);
     LAUNCHCHK();
     rb = !rb;
@@ -392,34 +609,42 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
 template <typename T>
 static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t st)
 {
-  int L = mg->L, rc;
+  int L = mg->L, lc = mg->lc, rc;
   LGeom g0 = lgeom(mg->n[0]);
-  hipLaunchKernelGGL((k_convert<double, T>), lgrid(g0), LBLOCK, 0, st, g0, bin, (T *)mg->b[0]);
+  hipLaunchKernelGGL((k_convert<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, bin,
+                     (T *)mg->b[0]);
   LAUNCHCHK();
-  if ((rc = rbgs<T>(mg, 0, 1, mg->pre, st))) return rc;
   int l;
-  for (l = 0; l <= L - 2; l++) {
-    if (l > 0 && (rc = rbgs<T>(mg, l, 1, mg->pre, st))) return rc;
+  for (l = 0; l < lc; l++) {
+    if ((rc = rbgs<T>(mg, l, 1, mg->pre, st))) return rc;
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
-    hipLaunchKernelGGL(k_residual_l<T>, lgrid(gf), LBLOCK, 0, st, gf, matof<T>(mg, l),
+    hipLaunchKernelGGL(k_residual_l<T>, dim3(ntiles(gf)), LBLOCK, 0, st, gf, matof<T>(mg, l),
                        (const T *)mg->x[l], (const T *)mg->b[l], (T *)mg->t[l]);
     LAUNCHCHK();
-    hipLaunchKernelGGL(k_restrict<T>, lgrid(gc), LBLOCK, 0, st, gf, gc, mg->dir[l],
+    hipLaunchKernelGGL(k_restrict<T>, dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc, mg->dir[l],
                        (const T *)mg->t[l], (T *)mg->b[l + 1], (const T *)mg->p1[l],
                        (const T *)mg->p2[l]);
     LAUNCHCHK();
   }
-  // coarse solve: RedBlackGSPoint with MaxIter 1
-  if ((rc = rbgs<T>(mg, L - 1, 1, 1, st))) return rc;
-  for (l = L - 2; l >= 0; l--) {
+  // bottom of the cycle: levels lc .. L-1 in one workgroup; the coarsest
+  // level's solve is RedBlackGSPoint with MaxIter 1
+  if (lc <= L - 2) {
+    hipLaunchKernelGGL(k_coarse_vcycle<T>, dim3(1), dim3(CWG), 0, st,
+                       (const LevelDev<T> *)mg->levdev, lc, L, mg->pre, mg->post,
+                       mg->symmetric);
+    LAUNCHCHK();
+  } else {
+    if ((rc = rbgs<T>(mg, L - 1, 1, 1, st))) return rc;
+  }
+  for (l = lc - 1; l >= 0; l--) {
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
-    hipLaunchKernelGGL(k_prolong_add<T>, lgrid(gf), LBLOCK, 0, st, gf, gc, mg->dir[l],
+    hipLaunchKernelGGL(k_prolong_add<T>, dim3(ntiles(gf)), LBLOCK, 0, st, gf, gc, mg->dir[l],
                        (const T *)mg->x[l + 1], (T *)mg->x[l], (const T *)mg->p1[l],
                        (const T *)mg->p2[l]);
     LAUNCHCHK();
     if ((rc = rbgs<T>(mg, l, 0, mg->post, st))) return rc;
   }
-  hipLaunchKernelGGL((k_convert<T, double>), lgrid(g0), LBLOCK, 0, st, g0,
+  hipLaunchKernelGGL((k_convert<T, double>), dim3(ntiles(g0)), LBLOCK, 0, st, g0,
                      (const T *)mg->x[0], xout);
   LAUNCHCHK();
   return PFHIP_OK;
