@@ -12,6 +12,15 @@
  * Matrices are symmetric-stored (C, E, N, U; the lower couplings are the
  * neighbour's upper ones, PL/matrix.c:258-290).  All arrays are ghosted
  * (n_x+2)(n_y+2)(n_z+2) and their ghost entries stay zero.
+ *
+ * Parity status: parity unpinned for the V-cycle output itself.  The
+ * reference's default PFMG preconditioner lives in hypre (not vendored) and
+ * no reference test dumps an MGSemi iterate.  What pins this file: the level
+ * hierarchy against the coarsening rule of PL/mg_semi.c:832-900, the cycle
+ * as a convergent solver on a 7-point Poisson operator (error contraction
+ * per cycle well below 1, exact on a 1-point coarsest level), and the
+ * end-to-end Newton solves reproducing the reference's goldens
+ * (test/correct_output/default_richards.out.*, overland_tiltedV_KWE).
  */
 #include <stdlib.h>
 #include <string.h>
