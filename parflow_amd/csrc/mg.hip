@@ -43,6 +43,7 @@
 #define MAXL 128
 #define MG_SMALL_DEFAULT 4096
 #define CWG 1024
+#define MAX_SWEEPS 64
 
 struct LGeom {
   int n[3];
@@ -177,6 +178,24 @@ __device__ __forceinline__ void cdiag_point(const LGeom &gc, const Mat<T> &Ac, i
   Ac.c[iAc] -= (c3 + c4 + c5 + c6);
 }
 
+// Gauss-Seidel point update x = (b - sum of present neighbour terms) / c,
+// terms summed west, east, south, north, lower, upper (PL/rb_GS_point.c:
+// 200-290); absent neighbours contribute exactly zero
+template <typename T>
+__device__ __forceinline__ T gs_update(T bb, T c, T aw, T xw, bool hw, T ae, T xe, bool he,
+                                       T as, T xs, bool hs, T an, T xn, bool hn, T al, T xl,
+                                       bool hl, T au, T xu, bool hu)
+{
+  T w = hw ? aw * xw : T(0);
+  T e = he ? ae * xe : T(0);
+  T so = hs ? as * xs : T(0);
+  T no = hn ? an * xn : T(0);
+  T lo = hl ? al * xl : T(0);
+  T up = hu ? au * xu : T(0);
+  T sum = w + e + so + no + lo + up;
+  return (bb - sum) / c;
+}
+
 // red/black update of one point of parity `par`; init: x = b / a0
 template <typename T>
 __device__ __forceinline__ void rbgs_point(const LGeom &g, const Mat<T> &A, T *x, const T *b,
@@ -186,14 +205,15 @@ __device__ __forceinline__ void rbgs_point(const LGeom &g, const Mat<T> &A, T *x
   long ip = LIDX(g, i, j, k);
   if (init) { x[ip] = b[ip] / A.c[ip]; return; }
   const int sy = g.s[1], sz = g.s[2];
-  T w = (i > 0) ? A.u[0][ip - 1] * x[ip - 1] : T(0);
-  T e = (i + 1 < g.n[0]) ? A.u[0][ip] * x[ip + 1] : T(0);
-  T so = (j > 0) ? A.u[1][ip - sy] * x[ip - sy] : T(0);
-  T no = (j + 1 < g.n[1]) ? A.u[1][ip] * x[ip + sy] : T(0);
-  T lo = (k > 0) ? A.u[2][ip - sz] * x[ip - sz] : T(0);
-  T up = (k + 1 < g.n[2]) ? A.u[2][ip] * x[ip + sz] : T(0);
-  T sum = w + e + so + no + lo + up;
-  x[ip] = (b[ip] - sum) / A.c[ip];
+  bool hw = i > 0, he = i + 1 < g.n[0], hs = j > 0, hn = j + 1 < g.n[1];
+  bool hl = k > 0, hu = k + 1 < g.n[2];
+  x[ip] = gs_update(b[ip], A.c[ip],
+                    hw ? A.u[0][ip - 1] : T(0), hw ? x[ip - 1] : T(0), hw,
+                    A.u[0][ip], he ? x[ip + 1] : T(0), he,
+                    hs ? A.u[1][ip - sy] : T(0), hs ? x[ip - sy] : T(0), hs,
+                    A.u[1][ip], hn ? x[ip + sy] : T(0), hn,
+                    hl ? A.u[2][ip - sz] : T(0), hl ? x[ip - sz] : T(0), hl,
+                    A.u[2][ip], hu ? x[ip + sz] : T(0), hu);
 }
 
 template <typename T>
@@ -282,6 +302,152 @@ __global__ __launch_bounds__(256) void k_rbgs(LGeom g, Mat<T> A, T *__restrict__
 {
   XCD_IJK(g);
   rbgs_point(g, A, x, b, par, init, i, j, k);
+}
+
+// ---- fused red -> black sweep pair ------------------------------------------
+// One launch performs a red half-sweep followed by a black half-sweep with
+// the arithmetic of two k_rbgs launches.  A workgroup owns PX x TY columns
+// and marches a chunk of KC planes; its threads cover one extra ring column
+// on each side, whose red values it recomputes so that the black update at
+// the tile edge sees the new red neighbours.  Plane k: red at k (into an LDS
+// ring of 3 planes), barrier, black at k-1 from the red planes k-2, k-1, k.
+// Reads x_in (old values) and writes x_out (every interior point exactly
+// once), so workgroups never read what another one writes.  Traffic per
+// point: C, E, N, U, b and x_in read, x_out written (7 words instead of 13).
+#define PX 62
+
+// everything one point's update reads: centre, b, the six couplings and the
+// six neighbour values of x_in.  Every load is issued unconditionally from a
+// clamped in-bounds point (ghost entries of every level array are 0 and
+// absent neighbours are masked by gs_update), so the loop body is straight-
+// line code and the prefetch of the next plane stays in flight while the
+// current one is computed.
+template <typename T> struct PointSet { T c, bb, aw, ae, as, an, al, au, xw, xe, xs, xn, xl, xu; };
+
+template <typename T, bool INIT>
+__device__ __forceinline__ void load_point(PointSet<T> &S, const Mat<T> &A,
+                                           const T *__restrict__ xin,
+                                           const T *__restrict__ b, unsigned ip, unsigned sy,
+                                           unsigned sz)
+{
+  S.c = A.c[ip]; S.bb = b[ip];
+  S.aw = A.u[0][ip - 1]; S.ae = A.u[0][ip];
+  S.as = A.u[1][ip - sy]; S.an = A.u[1][ip];
+  S.al = A.u[2][ip - sz]; S.au = A.u[2][ip];
+  if (INIT) {
+    S.xw = S.xe = S.xs = S.xn = S.xl = S.xu = T(0);
+  } else {
+    S.xw = xin[ip - 1]; S.xe = xin[ip + 1];
+    S.xs = xin[ip - sy]; S.xn = xin[ip + sy];
+    S.xl = xin[ip - sz]; S.xu = xin[ip + sz];
+  }
+}
+
+// Workgroup barrier that orders LDS only: waits for this wave's LDS
+// operations, not for its outstanding global loads (prefetches stay in
+// flight across it).  The "memory" clobber keeps the compiler from moving
+// memory accesses across it.
+__device__ __forceinline__ void lds_barrier()
+{
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// state a column carries from plane to plane
+template <typename T> struct ColState {
+  T sc, sb, sw, se, ss, sn, sl, su;   // coefficients of the last black point
+  T rm1, rm2;                         // red values at k-1, k-2
+};
+
+// one plane of the march: red at k from `cur` (loaded one step earlier),
+// black at k-1, one store per point of plane k-1 (its red value computed
+// last step or its black value computed now).  Branch-free apart from the
+// store.
+template <typename T, int TY, bool INIT>
+__device__ __forceinline__ void pair_step(T (*red)[TY + 2][64], int slot, int pslot,
+                                          const PointSet<T> &cur, ColState<T> &cs, int k,
+                                          int kb, int ke, int nz, bool valid, bool owned,
+                                          bool hw, bool he, bool hs, bool hn, int i, int j,
+                                          T *__restrict__ xout, unsigned ip, unsigned sz)
+{
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const bool isred = ((i + j + k) & 1) == 1;
+  const bool here = valid && k >= 0 && k < nz;
+  T rr = INIT ? cur.bb / cur.c
+              : gs_update(cur.bb, cur.c, cur.aw, cur.xw, hw, cur.ae, cur.xe, he, cur.as,
+                          cur.xs, hs, cur.an, cur.xn, hn, cur.al, cur.xl, k > 0, cur.au,
+                          cur.xu, k + 1 < nz);
+  const T rk = (here && isred) ? rr : T(0);
+  const bool keep = here && !isred;
+  cs.sc = keep ? cur.c : cs.sc;   cs.sb = keep ? cur.bb : cs.sb;
+  cs.sw = keep ? cur.aw : cs.sw;  cs.se = keep ? cur.ae : cs.se;
+  cs.ss = keep ? cur.as : cs.ss;  cs.sn = keep ? cur.an : cs.sn;
+  cs.sl = keep ? cur.al : cs.sl;  cs.su = keep ? cur.au : cs.su;
+  red[slot][ty][tx] = rk;
+  // black at k-1 (black exactly when (i,j,k) is red); in-plane red
+  // neighbours from the plane stored last step (ring lanes read clamped
+  // slots and never store)
+  const T(*R)[64] = red[pslot];
+  const int txm = tx > 0 ? tx - 1 : 0, txp = tx < 63 ? tx + 1 : 63;
+  const int tym = ty > 0 ? ty - 1 : 0, typ = ty < TY + 1 ? ty + 1 : TY + 1;
+  const int kk = k - 1;
+  T vb = gs_update(cs.sb, cs.sc, cs.sw, R[ty][txm], hw, cs.se, R[ty][txp], he,
+                   cs.ss, R[tym][tx], hs, cs.sn, R[typ][tx], hn,
+                   cs.sl, cs.rm2, kk > 0, cs.su, rk, kk + 1 < nz);
+  T v = isred ? vb : cs.rm1;
+  if (owned && kk >= kb && kk < ke) xout[ip - sz] = v;
+  cs.rm2 = cs.rm1;
+  cs.rm1 = rk;
+}
+
+template <typename T, int TY, int KC, bool INIT>
+__global__ __launch_bounds__(64 * (TY + 2)) void k_rbgs_pair(LGeom g, Mat<T> A,
+                                                             const T *__restrict__ xin,
+                                                             T *__restrict__ xout,
+                                                             const T *__restrict__ b)
+{
+  __shared__ T red[3][TY + 2][64];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int gx = (g.n[0] + PX - 1) / PX, gy = (g.n[1] + TY - 1) / TY;
+  const int gz = (g.n[2] + KC - 1) / KC;
+  const int nb = gx * gy * gz, nb8 = (nb + 7) & ~7;
+  const int bid = blockIdx.x;
+  const int Lt = (bid & 7) * (nb8 >> 3) + (bid >> 3);
+  if (Lt >= nb) return;                       // uniform over the workgroup
+  const int bx = Lt % gx, rr = Lt / gx;
+  const int by = rr % gy, bz = rr / gy;
+  const int i = bx * PX - 1 + tx, j = by * TY - 1 + ty;
+  const bool valid = i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1];
+  const bool owned = valid && tx >= 1 && tx <= PX && ty >= 1 && ty <= TY;
+  const int nz = g.n[2];
+  const int kb = bz * KC, ke = min(kb + KC, nz);
+  // 32-bit unsigned offsets: every level array has < 2^32 entries
+  const unsigned sy = g.s[1], sz = g.s[2];
+  const bool hw = i > 0, he = i + 1 < g.n[0], hs = j > 0, hn = j + 1 < g.n[1];
+  // loads come from the clamped column (ic, jc) and planes in [0, nz-1]
+  const int ic = min(max(i, 0), g.n[0] - 1), jc = min(max(j, 0), g.n[1] - 1);
+  const unsigned col = (unsigned)LIDX(g, ic, jc, 0) - sz;   // plane k at col + (k+1) sz
+  unsigned ip = valid ? (unsigned)LIDX(g, i, j, kb - 1) : 0u; // store address, plane kb-1
+  ColState<T> cs;
+  cs.sc = T(1); cs.sb = cs.sw = cs.se = cs.ss = cs.sn = cs.sl = cs.su = T(0);
+  cs.rm1 = cs.rm2 = T(0);
+  PointSet<T> P0, P1;                         // two register sets, roles alternate
+  load_point<T, INIT>(P0, A, xin, b, col + (unsigned)(max(kb - 1, 0) + 1) * sz, sy, sz);
+  int it = 0;
+  for (int k = kb - 1; k <= ke; k += 2) {
+    // plane k from P0, prefetch k+1 into P1
+    load_point<T, INIT>(P1, A, xin, b, col + (unsigned)(min(k + 1, nz - 1) + 1) * sz, sy, sz);
+    pair_step<T, TY, INIT>(red, it % 3, (it + 2) % 3, P0, cs, k, kb, ke, nz, valid, owned, hw,
+                           he, hs, hn, i, j, xout, ip, sz);
+    lds_barrier();
+    it++; ip += sz;
+    if (k + 1 > ke) break;
+    // plane k+1 from P1, prefetch k+2 into P0
+    load_point<T, INIT>(P0, A, xin, b, col + (unsigned)(min(k + 2, nz - 1) + 1) * sz, sy, sz);
+    pair_step<T, TY, INIT>(red, it % 3, (it + 2) % 3, P1, cs, k + 1, kb, ke, nz, valid, owned,
+                           hw, he, hs, hn, i, j, xout, ip, sz);
+    lds_barrier();
+    it++; ip += sz;
+  }
 }
 
 template <typename T>
@@ -407,6 +573,8 @@ struct pfhip_mg {
   int n[MAXL][3], dir[MAXL];
   void *A[MAXL][4];
   void *x[MAXL], *b[MAXL], *t[MAXL], *p1[MAXL], *p2[MAXL];
+  void *xa[MAXL];              // second iterate buffer of the grid levels (l < lc)
+  int fuse;                    // red -> black pairs as one launch (k_rbgs_pair)
   void *levdev;                // device LevelDev<T>[L]
 };
 
@@ -451,7 +619,10 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
     pfhip_set_error("mg precision must be 64 or 32");
     return PFHIP_ERR_ARG;
   }
-  if (pre < 0 || post < 0) { pfhip_set_error("negative sweep count"); return PFHIP_ERR_ARG; }
+  if (pre < 0 || post < 0 || pre > MAX_SWEEPS || post > MAX_SWEEPS) {
+    pfhip_set_error("sweep counts must lie in [0, %d]", MAX_SWEEPS);
+    return PFHIP_ERR_ARG;
+  }
   pfhip_mg *mg = (pfhip_mg *)calloc(1, sizeof(pfhip_mg));
   mg->pb = pb;
   mg->prec = precision;
@@ -479,6 +650,8 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
     return PFHIP_ERR_ARG;
   }
   // first level small enough for the single-workgroup bottom (never level 0)
+  const char *fe = getenv("PFHIP_MG_FUSE");
+  mg->fuse = (fe && *fe) ? atoi(fe) : 1;
   long small = MG_SMALL_DEFAULT;
   const char *env = getenv("PFHIP_MG_SMALL");
   if (env && *env) small = atol(env);
@@ -489,9 +662,11 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
   int rc = PFHIP_OK;
   for (l = 0; l < mg->L && rc == PFHIP_OK; l++) {
     long s = lsize(mg->n[l]);
-    void **ptrs[9] = { &mg->A[l][0], &mg->A[l][1], &mg->A[l][2], &mg->A[l][3],
-                       &mg->x[l], &mg->b[l], &mg->t[l], &mg->p1[l], &mg->p2[l] };
-    for (int q = 0; q < 9 && rc == PFHIP_OK; q++) {
+    void **ptrs[10] = { &mg->A[l][0], &mg->A[l][1], &mg->A[l][2], &mg->A[l][3],
+                        &mg->x[l], &mg->b[l], &mg->t[l], &mg->p1[l], &mg->p2[l],
+                        &mg->xa[l] };
+    int nalloc = (l < mg->lc) ? 10 : 9;
+    for (int q = 0; q < nalloc && rc == PFHIP_OK; q++) {
       rc = pfhip_malloc(ptrs[q], (long)(s * es));
       if (rc == PFHIP_OK) {
         hipError_t e = hipMemset(*ptrs[q], 0, s * es);
@@ -512,8 +687,8 @@ extern "C" int pfhip_mg_destroy(pfhip_mg *mg)
   if (!mg) return PFHIP_OK;
   for (int l = 0; l < mg->L; l++) {
     for (int q = 0; q < 4; q++) if (mg->A[l][q]) hipFree(mg->A[l][q]);
-    void *o[5] = { mg->x[l], mg->b[l], mg->t[l], mg->p1[l], mg->p2[l] };
-    for (int q = 0; q < 5; q++) if (o[q]) hipFree(o[q]);
+    void *o[6] = { mg->x[l], mg->b[l], mg->t[l], mg->p1[l], mg->p2[l], mg->xa[l] };
+    for (int q = 0; q < 6; q++) if (o[q]) hipFree(o[q]);
   }
   if (mg->levdev) hipFree(mg->levdev);
   free(mg);
@@ -571,36 +746,96 @@ extern "C" int pfhip_mg_setup(pfhip_mg *mg, const double *const *Asym, void *str
                           : mg_setup_t<float>(mg, Asym, STREAM(stream));
 }
 
-// RedBlackGSPoint call sequence (PL/rb_GS_point.c:100-300) with grid kernels
+// RedBlackGSPoint call sequence (PL/rb_GS_point.c:100-300): the list of
+// half-sweeps (parity, initial) it performs, and whether it first zeroes x
+struct HalfSweep { int par, init; };
+static int sweep_list(int symmetric, int zero, int max_iter, HalfSweep *seq, int *fill)
+{
+  int n = 0, count = symmetric ? 1 : 2, rb = 0;   // red first; red = odd parity
+  *fill = 0;
+  if (zero) {
+    count++;
+    if ((count / 2) > max_iter) {
+      *fill = 1;
+      if (!symmetric) return 0;
+    }
+    seq[n].par = 1; seq[n].init = 1; n++;
+    rb = !rb;
+  }
+  for (; (count / 2) <= max_iter; count++) {
+    seq[n].par = rb ? 0 : 1; seq[n].init = 0; n++;
+    rb = !rb;
+  }
+  return n;
+}
+
+// fused pair tile shapes; PFHIP_RBGS_VARIANT selects one (default 0)
+static int rbgs_variant()
+{
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("PFHIP_RBGS_VARIANT");
+    v = (e && *e) ? atoi(e) : 0;
+  }
+  return v;
+}
+
+template <typename T, int TY, int KC>
+static void launch_pair_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
+                          int init, hipStream_t st)
+{
+  unsigned gx = (g.n[0] + PX - 1) / PX, gy = (g.n[1] + TY - 1) / TY, gz = (g.n[2] + KC - 1) / KC;
+  unsigned nb = gx * gy * gz, nb8 = (nb + 7u) & ~7u;
+  if (init)
+    hipLaunchKernelGGL((k_rbgs_pair<T, TY, KC, true>), dim3(nb8), dim3(64, TY + 2, 1), 0, st, g,
+                       A, xin, xout, b);
+  else
+    hipLaunchKernelGGL((k_rbgs_pair<T, TY, KC, false>), dim3(nb8), dim3(64, TY + 2, 1), 0, st, g,
+                       A, xin, xout, b);
+}
+
+template <typename T>
+static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
+                        int init, hipStream_t st)
+{
+  switch (rbgs_variant()) {
+    case 1: launch_pair_t<T, 6, 16>(g, A, xin, xout, b, init, st); break;
+    case 2: launch_pair_t<T, 6, 32>(g, A, xin, xout, b, init, st); break;
+    case 3: launch_pair_t<T, 14, 8>(g, A, xin, xout, b, init, st); break;
+    case 4: launch_pair_t<T, 2, 16>(g, A, xin, xout, b, init, st); break;
+    default: launch_pair_t<T, 14, 16>(g, A, xin, xout, b, init, st); break;
+  }
+}
+
+// smoothing of grid level l: red -> black pairs run fused (x and its second
+// buffer swap roles after each pair), any other half-sweep runs alone
 template <typename T>
 static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
 {
   LGeom g = lgeom(mg->n[l]);
   dim3 grid(ntiles(g));
   Mat<T> A = matof<T>(mg, l);
-  T *x = (T *)mg->x[l];
   const T *b = (const T *)mg->b[l];
-  int count = mg->symmetric ? 1 : 2;
-  int rb = 0;   // red first; red = odd parity
-  if (zero) {
-    count++;
-    if ((count / 2) > max_iter) {
-      long n = lsize(mg->n[l]);
-      hipLaunchKernelGGL(k_fill_t<T>, dim3(256), dim3(256), 0, st, x, n);
-      LAUNCHCHK();
-      if (!mg->symmetric) return PFHIP_OK;
-    }
-    hipLaunchKernelGGL(k_rbgs<T>, grid, LBLOCK, 0, st, g, A, x, b, 1, 1);
+  HalfSweep seq[2 * MAX_SWEEPS + 4];
+  int fill;
+  int n = sweep_list(mg->symmetric, zero, max_iter, seq, &fill);
+  if (fill) {
+    hipLaunchKernelGGL(k_fill_t<T>, dim3(256), dim3(256), 0, st, (T *)mg->x[l], lsize(mg->n[l]));
     LAUNCHCHK();
-    rb = !rb;
   }
-  for (; (count / 2) <= max_iter; count++) {
-    int par = rb ? 0 : 1;
+  for (int q = 0; q < n;) {
     if (l == 0) pfhip_prof_mark(PFHIP_PROF_RBGS, 0, st);
-    hipLaunchKernelGGL(k_rbgs<T>, grid, LBLOCK, 0, st, g, A, x, b, par, 0);
+    if (mg->fuse && mg->xa[l] && q + 1 < n && seq[q].par == 1 && seq[q + 1].par == 0) {
+      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, st);
+      void *tmp = mg->x[l]; mg->x[l] = mg->xa[l]; mg->xa[l] = tmp;
+      q += 2;
+    } else {
+      hipLaunchKernelGGL(k_rbgs<T>, grid, LBLOCK, 0, st, g, A, (T *)mg->x[l], b, seq[q].par,
+                         seq[q].init);
+      q += 1;
+    }
     if (l == 0) pfhip_prof_mark(PFHIP_PROF_RBGS, 1, st);
     LAUNCHCHK();
-    rb = !rb;
   }
   return PFHIP_OK;
 }

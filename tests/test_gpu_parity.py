@@ -154,6 +154,56 @@ def test_multigrid(gpu, precision):
         assert _relmax(_I(prob, xg), _I(prob, xo)) < tol
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("fuse", [0, 1])
+def test_multigrid_grid_levels(gpu, precision, fuse, monkeypatch):
+    """A hierarchy whose upper levels run as grid kernels (ragged 62 x 14 x 16
+    pair tiles and 64 x 4 tiles) and whose bottom runs in one workgroup,
+    with the red -> black pairs fused or not."""
+    api = _api()
+    monkeypatch.setenv("PFHIP_MG_SMALL", "300")
+    monkeypatch.setenv("PFHIP_MG_FUSE", str(fuse))
+    prob = cases.hetero(70, 40, 24, seed=17, dir_sides=True)
+    print(f"mg grid levels precision={precision} fuse={fuse}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p = cases.random_pressure(prob, seed=18)
+    J = O.jacobian_eval(op, p, 0.1, 0.1, 1)
+    rng = np.random.default_rng(19)
+    for pre, post, sym in ((1, 1, 1), (2, 2, 0), (3, 1, 0)):
+        mgo = O.MG(prob, pre, post, sym)
+        mgo.setup(J)
+        mgg = api.GPUMultigrid(gp, pre, post, sym, precision)
+        assert mgg.levels() == len(mgo.levels())
+        mgg.setup(J)
+        for rep in range(2):
+            b = np.zeros(prob.shape)
+            b[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+            xo = mgo.apply(b)
+            xg = mgg.apply(b)
+            tol = 1e-11 if precision == 64 else 2e-5
+            err = _relmax(_I(prob, xg), _I(prob, xo))
+            assert err < tol, (pre, post, sym, rep, err)
+
+
+@pytest.mark.parametrize("overland,dirs", [(False, True), (True, False)])
+def test_residual_multi_tile(gpu, overland, dirs):
+    """Residual on a grid spanning several 64-wide tiles, ragged tile rows
+    and z-chunks."""
+    api = _api()
+    prob = cases.hetero(150, 37, 40, seed=23, overland=overland, dir_sides=dirs)
+    print(f"residual multi-tile overland={overland} dir={dirs}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p_old = cases.random_pressure(prob, seed=24)
+    s_old = O.saturation(op, p_old)
+    p = p_old + np.random.default_rng(25).normal(0, 0.01, p_old.shape)
+    Fg = gp.nl_function_eval(p, p_old, s_old, 0.1, 0.2)
+    Fo = O.nl_function_eval(op, p, p_old, s_old, 0.1, 0.2)
+    scale = np.max(np.abs(_I(prob, Fo))) + 1e-30
+    assert _relmax(_I(prob, Fg), _I(prob, Fo), scale) < 1e-12
+
+
 def _compare_runs(prob, opts, p0, dt, nsteps, precision=64, krylov=0, ptol=1e-9,
                   newton_tol=1):
     api = _api()
