@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include "pfhip_internal.h"
+#include "device_util.h"
 
 // --------------------------------------------------------------------------
 // errors
@@ -791,6 +792,41 @@ struct VGTerms {
   double ah, t, a, sa, ahnm1, m;
 };
 
+// Reciprocal and quotient by the hardware reciprocal estimate plus two
+// Newton steps and one residual correction (the steps of the compiler's own
+// FP64 division, without its range scaling and special-value fix-up: the
+// operands here are finite, normal and far from overflow).  Within 1 ulp.
+__device__ __forceinline__ double fast_rcp(double b)
+{
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ double fast_div(double a, double b)
+{
+  double r = fast_rcp(b);
+  double q = a * r;
+  double e = fma(-b, q, a);
+  return fma(e, r, q);
+}
+
+// the chain with 1/(rho g) supplied by the caller
+__device__ __forceinline__ VGTerms vg_terms_i(double p, double irg, double alpha, double n)
+{
+  VGTerms v;
+  v.m = 1.0 - fast_rcp(n);
+  double head = fabs(p) * irg;
+  v.ah = alpha * head;
+  double L = log(v.ah);
+  v.t = exp(n * L);
+  v.a = exp(-v.m * log1p(v.t));
+  v.sa = sqrt(v.a);
+  v.ahnm1 = fast_div(v.t, v.ah);
+  return v;
+}
+
 __device__ __forceinline__ VGTerms vg_terms(double p, double rho, double g, double alpha,
                                             double n)
 {
@@ -856,6 +892,22 @@ __device__ __forceinline__ double vg_dkr(double p, double rho, double g,
   return vg_dkr_t(vg_terms(p, rho, g, alpha, n), alpha, n);
 }
 
+__device__ __forceinline__ double vg_kr_i(double p, double irg, double alpha, double n)
+{
+  if (p >= 0.0) return 1.0;
+  return vg_kr_t(vg_terms_i(p, irg, alpha, n));
+}
+
+__device__ __forceinline__ void vg_s_kr_i(double p, double irg, double alpha, double n,
+                                          double sres, double ssat, double &S, double &kr)
+{
+  double s_dif = ssat - sres;
+  if (p >= 0.0) { S = s_dif + sres; kr = 1.0; return; }
+  VGTerms v = vg_terms_i(p, irg, alpha, n);
+  S = s_dif * v.a + sres;
+  kr = vg_kr_t(v);
+}
+
 // S and kr of one cell from one chain
 __device__ __forceinline__ void vg_s_kr(double p, double rho, double g, double alpha,
                                         double n, double sres, double ssat, double &S,
@@ -875,6 +927,7 @@ struct Phys {
   const double *__restrict__ sres;   // NULL: uniform sres_v
   const double *__restrict__ ssat;   // NULL: uniform ssat_v
   double sres_v, ssat_v;
+  double irg;                        // 1 / (rho g)
   __device__ __forceinline__ double Sres(int ip) const { return sres ? sres[ip] : sres_v; }
   __device__ __forceinline__ double Ssat(int ip) const { return ssat ? ssat[ip] : ssat_v; }
 };
@@ -885,6 +938,7 @@ static inline Phys phys_of(const pfhip_problem *pb)
   f.rho = pb->d.rho; f.mu = pb->d.mu; f.g = pb->d.gravity;
   f.alpha = pb->d.alpha; f.nvg = pb->d.nvg; f.sres = pb->d.sres; f.ssat = pb->d.ssat;
   f.sres_v = pb->d.sres_value; f.ssat_v = pb->d.ssat_value;
+  f.irg = 1.0 / (f.rho * f.g);
   return f;
 }
 
@@ -1062,6 +1116,7 @@ struct ResArgs {
   Geom g;
   Phys f;
   double dx, dy, dz, dt;
+  double idx, idy, idz, imu;         // reciprocals of dx, dy, dz, mu
   const double *__restrict__ p;
   const double *__restrict__ p_old;
   const double *__restrict__ s_old;
@@ -1122,22 +1177,36 @@ __device__ __forceinline__ double residual_cell(const ResArgs &A, int i, int j, 
                                                 double pc, double krc, double S,
                                                 double kxc, double kyc, double kzc,
                                                 double ur, double uf, double uu,
-                                                double ul, double ud, double ub)
+                                                double ul, double ud, double ub, double so,
+                                                double po, double ssv, double pold)
 {
   const Geom &g = A.g;
   const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
   const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
   const double vol = dx * dy * dz;
   const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
-  double so = A.s_old[ip];
-  double fv = (S * rho - so * rho) * A.poro[ip] * vol;
-  fv += A.ss[ip] * vol * (pc * S * rho - A.p_old[ip] * so * rho);
+  double fv = (S * rho - so * rho) * po * vol;
+  fv += ssv * vol * (pc * S * rho - pold * so * rho);
   if (A.src) fv -= vol * dt * (A.src[ip]);
   fv += dt * (ur + uf + uu);
   if (i > 0) fv += -dt * ul;
   if (j > 0) fv += -dt * ud;
   if (k > 0) fv += -dt * ub;
+  return fv;
+}
 
+// The boundary-face terms of one cell (:975-2260), added to fv face by face
+// in the order x-lower, x-upper, y-lower, y-upper, z-lower, z-upper, after
+// all interior terms -- the order of the reference's patch loops.
+__device__ __forceinline__ double residual_bc(const ResArgs &A, int i, int j, int k, int ip,
+                                              double fv, double pc, double krc, double kxc,
+                                              double kyc, double kzc)
+{
+  const Geom &g = A.g;
+  const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
+  const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
+  const double vol = dx * dy * dz;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
   bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
              (k == g.nz - 1);
   if (bnd) {
@@ -1323,8 +1392,201 @@ __global__ __launch_bounds__(RTX * TY) void k_residual_fused(ResArgs A)
     if (act) {
       double ul = s_ux[ty * (RTX + 1) + tx];
       double ud = s_uy[ty * RTX + tx];
-      A.F[ip] = residual_cell(A, i, j, k, ip, p_c, kr_c, S_c, kxc, kyc, kz_c, ur, uf, uu,
-                              ul, ud, ub);
+      double fv = residual_cell(A, i, j, k, ip, p_c, kr_c, S_c, kxc, kyc, kz_c, ur, uf, uu,
+                                ul, ud, ub, A.s_old[ip], A.poro[ip], A.ss[ip], A.p_old[ip]);
+      A.F[ip] = residual_bc(A, i, j, k, ip, fv, p_c, kr_c, kxc, kyc, kz_c);
+    }
+    ub = uu;
+    p_m = p_c; kr_m = kr_c; kz_m = kz_c;
+    p_c = p_n; kr_c = kr_n; S_c = S_n; kz_c = kz_n;
+  }
+}
+
+// Face fluxes with reciprocal constants (1/d, 1/mu multiply instead of
+// divide: within an ulp of the quotients) and the fast harmonic mean.  Same
+// operand order as flux_xy / flux_z.
+__device__ __forceinline__ double hmean_f(double a, double b)
+{
+  return (a + b) ? fast_div(2.0 * a * b, a + b) : 0;
+}
+__device__ __forceinline__ double flux_xy_i(double pa, double pb, double ka, double kb,
+                                            double kra, double krb, double ff, double id,
+                                            double gc, double imu)
+{
+  double dd = (pa - pb) * id;
+  double updir = dd * gc;
+  return ff * hmean_f(ka, kb) * dd * gc * upw(updir, 0.0, kra, krb) * imu;
+}
+__device__ __forceinline__ double flux_z_i(double pa, double pb, double ka, double kb,
+                                           double kra, double krb, double ffz, double idz,
+                                           double hrg, double imu)
+{
+  double lc = pa * idz - hrg;
+  double uc = pb * idz + hrg;
+  double diff = lc - uc;
+  return ffz * hmean_f(ka, kb) * diff * upw(lc, uc, kra, krb) * imu;
+}
+
+// Software-pipelined fused residual.  Same tiling and phases as
+// k_residual_fused (TX x TY columns, a z-chunk of ZC planes, own x+/y+/z+
+// faces once, edge halo recomputed), but every global load of plane k+1
+// (k+2 for the closure inputs) is issued at the top of plane k from a
+// clamped in-bounds address, and the two per-plane barriers order LDS only,
+// so the loads stay in flight while plane k is computed.  PCS: s_res/s_sat
+// are per-cell fields.
+template <bool PCS> struct VGIn { double p, al, n, kz, sr, st; };
+template <> struct VGIn<false> { double p, al, n, kz; };
+struct OthIn { double kx, kxe, ky, kyn, so, po, ss, pold, hk0, hk1; };
+struct HalIn { double p, al, n; };
+
+template <bool PCS>
+__device__ __forceinline__ void load_vg(VGIn<PCS> &v, const ResArgs &A, int ip)
+{
+  v.p = A.p[ip]; v.al = A.f.alpha[ip]; v.n = A.f.nvg[ip]; v.kz = A.kz[ip];
+  if constexpr (PCS) { v.sr = A.f.sres[ip]; v.st = A.f.ssat[ip]; }
+}
+template <bool PCS> __device__ __forceinline__ double vg_sr(const VGIn<PCS> &v, const Phys &f)
+{
+  if constexpr (PCS) return v.sr; else return f.sres_v;
+}
+template <bool PCS> __device__ __forceinline__ double vg_st(const VGIn<PCS> &v, const Phys &f)
+{
+  if constexpr (PCS) return v.st; else return f.ssat_v;
+}
+__device__ __forceinline__ void load_oth(OthIn &o, const ResArgs &A, int ip, int sy, int hip,
+                                         const double *__restrict__ Kh, int hoff)
+{
+  o.kx = A.kx[ip]; o.kxe = A.kx[ip + 1]; o.ky = A.ky[ip]; o.kyn = A.ky[ip + sy];
+  o.hk0 = Kh[hip]; o.hk1 = Kh[hip + hoff];
+  o.so = A.s_old[ip]; o.po = A.poro[ip]; o.ss = A.ss[ip]; o.pold = A.p_old[ip];
+}
+__device__ __forceinline__ void load_hal(HalIn &h, const ResArgs &A, int hip)
+{
+  h.p = A.p[hip]; h.al = A.f.alpha[hip]; h.n = A.f.nvg[hip];
+}
+
+template <int TX, int TY, int ZC, bool PCS>
+__global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
+{
+  constexpr int HALO = 2 * TX + 2 * TY;
+  constexpr int W2 = TX + 2;
+  constexpr int LDSN = (TX + 2) * (TY + 2);
+  __shared__ double s_p[LDSN];
+  __shared__ double s_kr[LDSN];
+  __shared__ double s_ux[TY * (TX + 1)];   // x faces: west face of tile column tx at tx
+  __shared__ double s_uy[(TY + 1) * TX];   // y faces: south face of tile row ty at ty
+  const Geom g = A.g;
+  const Phys &f = A.f;
+  const double rho = f.rho, grav = f.g, irg = f.irg, imu = A.imu;
+  const double hrg = 0.5 * rho * grav * 1.0;
+  const double dx = A.dx, dy = A.dy, dz = A.dz;
+  const double idx = A.idx, idy = A.idy, idz = A.idz;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int i = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty;
+  const bool act = (i < g.nx) && (j < g.ny);
+  const int nz = g.nz, sy = g.sy, sz = g.sz;
+  const int ls = (ty + 1) * W2 + (tx + 1);
+  // loads use the clamped column; stores only happen for act cells
+  const int col = (min(i, g.nx - 1) + 1) + (min(j, g.ny - 1) + 1) * sy;
+  // edge-halo cell of this thread (corners are not needed by 7 points)
+  int hx = 0, hy = 0;
+  bool hact = false;
+  if (tid < HALO) {
+    if (tid < TX) { hx = tid + 1; hy = 0; }
+    else if (tid < 2 * TX) { hx = tid - TX + 1; hy = TY + 1; }
+    else if (tid < 2 * TX + TY) { hx = 0; hy = tid - 2 * TX + 1; }
+    else { hx = TX + 1; hy = tid - 2 * TX - TY + 1; }
+    int hi = blockIdx.x * TX + hx - 1, hj = blockIdx.y * TY + hy - 1;
+    hact = (hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny);
+  }
+  const int hls = hy * W2 + hx;
+  // halo loads come from the halo cell when it exists, else the own column
+  const int hcol = hact ? (blockIdx.x * TX + hx) + (blockIdx.y * TY + hy) * sy : col;
+  // west halo (hx == 0) supplies the x face into the tile, south halo
+  // (hy == 0) the y face into the tile
+  const bool hwest = hact && hx == 0;
+  const bool hsouth = hact && hy == 0;
+  const double *__restrict__ Kh = hsouth ? A.ky : A.kx;
+  const int hoff = hsouth ? sy : 1;
+  const int k0 = blockIdx.z * ZC;
+  const int k1 = min(k0 + ZC, nz);
+  auto plane = [&](int q) { return (min(q, nz - 1) + 1) * sz; };
+
+  double p_m = 0, kr_m = 0, kz_m = 0;
+  double p_c = 0, kr_c = 0, S_c = 0, kz_c = 0;
+  double p_n = 0, kr_n = 0, S_n = 0, kz_n = 0;
+  {
+    VGIn<PCS> v0;
+    load_vg<PCS>(v0, A, col + plane(k0));
+    if (k0 > 0) {
+      VGIn<PCS> vm;
+      load_vg<PCS>(vm, A, col + plane(k0 - 1));
+      p_m = vm.p;
+      kr_m = vg_kr_i(p_m, irg, vm.al, vm.n) * rho;
+      kz_m = vm.kz;
+    }
+    p_c = v0.p;
+    vg_s_kr_i(p_c, irg, v0.al, v0.n, vg_sr<PCS>(v0, f), vg_st<PCS>(v0, f), S_c, kr_c);
+    kr_c *= rho;
+    kz_c = v0.kz;
+  }
+  // flux through the lower z face of the own cell at plane k0
+  double ub = 0.0;
+  if (k0 > 0) ub = flux_z_i(p_m, p_c, kz_m, kz_c, kr_m, kr_c, ffz, idz, hrg, imu);
+
+  VGIn<PCS> vgN;
+  HalIn hal;
+  load_vg<PCS>(vgN, A, col + plane(k0 + 1));
+  load_hal(hal, A, hcol + plane(k0));
+
+  for (int k = k0; k < k1; k++) {
+    const int ip = col + (k + 1) * sz;
+    // this plane's flux and storage inputs (used after the closures), then
+    // the prefetch of the next plane's closure inputs
+    OthIn o;
+    load_oth(o, A, col + plane(k), sy, hcol + plane(k), Kh, hoff);
+    const VGIn<PCS> vgU = vgN;
+    const HalIn h = hal;
+    load_vg<PCS>(vgN, A, col + plane(k + 2));
+    load_hal(hal, A, hcol + plane(k + 1));
+    // ---- A: closures of the own cell one plane ahead and of the halo cell
+    if (k + 1 < nz) {
+      p_n = vgU.p;
+      vg_s_kr_i(p_n, irg, vgU.al, vgU.n, vg_sr<PCS>(vgU, f), vg_st<PCS>(vgU, f), S_n, kr_n);
+      kr_n *= rho;
+      kz_n = vgU.kz;
+    }
+    if (hact) {
+      s_p[hls] = h.p;
+      s_kr[hls] = vg_kr_i(h.p, irg, h.al, h.n) * rho;
+    }
+    s_p[ls] = p_c;
+    s_kr[ls] = kr_c;
+    lds_barrier();
+    // ---- B: own x+, y+, z+ faces; west / south halo faces into the tile
+    double ur = 0.0, uf = 0.0, uu = 0.0;
+    if (i + 1 < g.nx)
+      ur = flux_xy_i(p_c, s_p[ls + 1], o.kx, o.kxe, kr_c, s_kr[ls + 1], ffx, idx, grav, imu);
+    if (j + 1 < g.ny)
+      uf = flux_xy_i(p_c, s_p[ls + W2], o.ky, o.kyn, kr_c, s_kr[ls + W2], ffy, idy, grav,
+                     imu);
+    if (k + 1 < nz) uu = flux_z_i(p_c, p_n, kz_c, kz_n, kr_c, kr_n, ffz, idz, hrg, imu);
+    s_ux[ty * (TX + 1) + tx + 1] = ur;
+    s_uy[(ty + 1) * TX + tx] = uf;
+    if (hwest)
+      s_ux[(hy - 1) * (TX + 1)] = flux_xy_i(s_p[hls], s_p[hls + 1], o.hk0, o.hk1, s_kr[hls],
+                                            s_kr[hls + 1], ffx, idx, grav, imu);
+    if (hsouth)
+      s_uy[hx - 1] = flux_xy_i(s_p[hls], s_p[hls + W2], o.hk0, o.hk1, s_kr[hls], s_kr[hls + W2],
+                               ffy, idy, grav, imu);
+    lds_barrier();
+    // ---- C
+    if (act) {
+      double ul = s_ux[ty * (TX + 1) + tx];
+      double ud = s_uy[ty * TX + tx];
+      A.F[ip] = residual_cell(A, i, j, k, ip, p_c, kr_c, S_c, o.kx, o.ky, kz_c, ur, uf, uu,
+                              ul, ud, ub, o.so, o.po, o.ss, o.pold);   // + k_residual_bc
     }
     ub = uu;
     p_m = p_c; kr_m = kr_c; kz_m = kz_c;
@@ -1343,10 +1605,90 @@ static int residual_variant()
   return v;
 }
 
+// Boundary cells of the domain, each exactly once: both z planes first,
+// then the y-lower/upper rows of the inner planes, then the x-lower/upper
+// columns of the inner planes and rows.
+__device__ __forceinline__ bool shell_cell(const Geom &g, long idx, int &i, int &j, int &k)
+{
+  const int nzb = (g.nz > 1) ? 2 : 1, nyb = (g.ny > 1) ? 2 : 1, nxb = (g.nx > 1) ? 2 : 1;
+  const int nzi = max(g.nz - 2, 0), nyi = max(g.ny - 2, 0);
+  const long nplane = (long)g.nx * g.ny;
+  const long n0 = nplane * nzb, n1 = (long)g.nx * nyb * nzi, n2 = (long)nxb * nyi * nzi;
+  if (idx < n0) {
+    k = (idx >= nplane) ? g.nz - 1 : 0;
+    long r = idx % nplane;
+    i = (int)(r % g.nx); j = (int)(r / g.nx);
+    return true;
+  }
+  idx -= n0;
+  if (idx < n1) {
+    i = (int)(idx % g.nx);
+    long r = idx / g.nx;
+    j = (r % nyb) ? g.ny - 1 : 0;
+    k = 1 + (int)(r / nyb);
+    return true;
+  }
+  idx -= n1;
+  if (idx < n2) {
+    i = (idx % nxb) ? g.nx - 1 : 0;
+    long r = idx / nxb;
+    j = 1 + (int)(r % nyi);
+    k = 1 + (int)(r / nyi);
+    return true;
+  }
+  return false;
+}
+
+static long shell_count(const Geom &g)
+{
+  const int nzb = (g.nz > 1) ? 2 : 1, nyb = (g.ny > 1) ? 2 : 1, nxb = (g.nx > 1) ? 2 : 1;
+  const long nzi = max(g.nz - 2, 0), nyi = max(g.ny - 2, 0);
+  return (long)g.nx * g.ny * nzb + (long)g.nx * nyb * nzi + (long)nxb * nyi * nzi;
+}
+
+// F += boundary-face terms (k_residual_pf leaves them out)
+__global__ __launch_bounds__(256) void k_residual_bc(ResArgs A, long nshell)
+{
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nshell) return;
+  int i, j, k;
+  if (!shell_cell(A.g, idx, i, j, k)) return;
+  const int ip = (i + 1) + (j + 1) * A.g.sy + (k + 1) * A.g.sz;
+  double pc = A.p[ip];
+  double krc = vg_kr(pc, A.f.rho, A.f.g, A.f.alpha[ip], A.f.nvg[ip]) * A.f.rho;
+  A.F[ip] = residual_bc(A, i, j, k, ip, A.F[ip], pc, krc, A.kx[ip], A.ky[ip], A.kz[ip]);
+}
+
+template <int TX, int TY, int ZC>
+static void launch_pf(const ResArgs &A, hipStream_t st)
+{
+  const Geom &g = A.g;
+  dim3 grid((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC);
+  dim3 block(TX, TY, 1);
+  PROF_BEGIN(PFHIP_PROF_RESIDUAL, st);
+  if (A.f.sres || A.f.ssat)
+    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true>), grid, block, 0, st, A);
+  else
+    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false>), grid, block, 0, st, A);
+  PROF_END(PFHIP_PROF_RESIDUAL, st);
+  long ns = shell_count(g);
+  hipLaunchKernelGGL(k_residual_bc, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, A, ns);
+}
+
 static void launch_residual(const ResArgs &A, hipStream_t st)
 {
   const Geom &g = A.g;
   int v = residual_variant();
+  switch (v) {
+    case 10: launch_pf<64, 16, 16>(A, st); return;
+    case 11: launch_pf<32, 16, 16>(A, st); return;
+    case 12: launch_pf<64, 8, 32>(A, st); return;
+    case 13: launch_pf<32, 32, 16>(A, st); return;
+    default: break;
+  }
+  if (v == 0) { launch_pf<64, 8, 16>(A, st); return; }
+  if (v == 5) v = 0;   // the unpipelined kernel, 64 x 8 x 16
+  PROF_BEGIN(PFHIP_PROF_RESIDUAL, st);
   int TY = (v == 0 || v == 1 || v == 4) ? 8 : 4;
   int ZC = (v == 0) ? 16 : (v == 1) ? 64 : (v == 2) ? 16 : (v == 3) ? 8 : 4;
   dim3 grid((g.nx + RTX - 1) / RTX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC);
@@ -1358,6 +1700,7 @@ static void launch_residual(const ResArgs &A, hipStream_t st)
     case 4: hipLaunchKernelGGL((k_residual_fused<8, 4>), grid, block, 0, st, A); break;
     default: hipLaunchKernelGGL((k_residual_fused<8, 16>), grid, block, 0, st, A); break;
   }
+  PROF_END(PFHIP_PROF_RESIDUAL, st);
 }
 
 extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double *F,
@@ -1374,15 +1717,14 @@ extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double
   A.g = g;
   A.f = phys_of(pb);
   A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
+  A.idx = 1.0 / A.dx; A.idy = 1.0 / A.dy; A.idz = 1.0 / A.dz; A.imu = 1.0 / A.f.mu;
   A.p = p; A.p_old = p_old; A.s_old = s_old;
   A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
   A.poro = pb->d.poro; A.ss = pb->d.ss; A.src = pb->d.src;
   A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
   for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
   A.F = F;
-  PROF_BEGIN(PFHIP_PROF_RESIDUAL, stream);
   launch_residual(A, STREAM(stream));
-  PROF_END(PFHIP_PROF_RESIDUAL, stream);
   LAUNCHCHK();
   return PFHIP_OK;
 }
@@ -1528,7 +1870,7 @@ __global__ __launch_bounds__(256) void k_jacobian(JacArgs A)
   bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
              (k == g.nz - 1);
   if (bnd) {
-#pragma unroll 1
+#pragma unroll
     for (int fc = 0; fc < 6; fc++) {
       bool on = (fc == 0 && i == 0) || (fc == 1 && i == g.nx - 1) ||
                 (fc == 2 && j == 0) || (fc == 3 && j == g.ny - 1) ||

@@ -28,6 +28,7 @@
 #include <string.h>
 
 #include "pfhip_internal.h"
+#include "device_util.h"
 
 #define HIPCHK(x)                                                          \
   do {                                                                     \
@@ -341,15 +342,6 @@ __device__ __forceinline__ void load_point(PointSet<T> &S, const Mat<T> &A,
     S.xs = xin[ip - sy]; S.xn = xin[ip + sy];
     S.xl = xin[ip - sz]; S.xu = xin[ip + sz];
   }
-}
-
-// Workgroup barrier that orders LDS only: waits for this wave's LDS
-// operations, not for its outstanding global loads (prefetches stay in
-// flight across it).  The "memory" clobber keeps the compiler from moving
-// memory accesses across it.
-__device__ __forceinline__ void lds_barrier()
-{
-  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 // state a column carries from plane to plane
