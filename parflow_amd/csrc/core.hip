@@ -15,6 +15,7 @@
 
 #include "pfhip_internal.h"
 #include "device_util.h"
+#include "pf_math.h"
 
 // --------------------------------------------------------------------------
 // errors
@@ -782,84 +783,47 @@ extern "C" int pfhip_vec_const(pfhip_problem *pb, double c, double *z, void *str
 //
 // The reference spends 3-8 pow() calls per cell.  Here every power of
 // (alpha h) and of (1 + (alpha h)^n) is derived from one shared chain,
-//   L = log(alpha h), t = (alpha h)^n = exp(n L), a = (1+t)^-m = exp(-m log1p(t)),
-// so S and kr together cost two logs, two exps and a square root; the
-// derivative terms reuse t, a and sqrt(a).  Values agree with the pow form
-// to a few ulp (the 1 - (alpha h)^(n-1) a cancellation of dry cells is
-// common to both forms).
+//   L = log(alpha h), t = (alpha h)^n = exp(n L),
+//   sa = (1+t)^(-m/2) = exp(-m/2 log1p(t)), a = (1+t)^-m = sa^2,
+// so S and kr together cost two logs and two exps (pf_math.h); the
+// derivative terms reuse t, a and sa.  h = |p| / (rho g) takes 1/(rho g)
+// from the caller.  Values agree with the pow form to a few ulp (the
+// 1 - (alpha h)^(n-1) a cancellation of dry cells is common to both forms).
 // --------------------------------------------------------------------------
 struct VGTerms {
   double ah, t, a, sa, ahnm1, m;
 };
 
-// Reciprocal and quotient by the hardware reciprocal estimate plus two
-// Newton steps and one residual correction (the steps of the compiler's own
-// FP64 division, without its range scaling and special-value fix-up: the
-// operands here are finite, normal and far from overflow).  Within 1 ulp.
-__device__ __forceinline__ double fast_rcp(double b)
-{
-  double r = __builtin_amdgcn_rcp(b);
-  double e = fma(-b, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-b, r, 1.0);
-  return fma(r, e, r);
-}
-__device__ __forceinline__ double fast_div(double a, double b)
-{
-  double r = fast_rcp(b);
-  double q = a * r;
-  double e = fma(-b, q, a);
-  return fma(e, r, q);
-}
-
-// the chain with 1/(rho g) supplied by the caller
-__device__ __forceinline__ VGTerms vg_terms_i(double p, double irg, double alpha, double n)
+__device__ __forceinline__ VGTerms vg_terms(double p, double irg, double alpha, double n)
 {
   VGTerms v;
-  v.m = 1.0 - fast_rcp(n);
+  v.m = 1.0 - pf_rcp(n);
   double head = fabs(p) * irg;
   v.ah = alpha * head;
-  double L = log(v.ah);
-  v.t = exp(n * L);
-  v.a = exp(-v.m * log1p(v.t));
-  v.sa = sqrt(v.a);
-  v.ahnm1 = fast_div(v.t, v.ah);
+  double L = pf_log(v.ah);
+  v.t = pf_exp(n * L);                            // (alpha h)^n
+  v.sa = pf_exp(-0.5 * v.m * pf_log1p(v.t));      // (1 + (alpha h)^n)^(-m/2)
+  v.a = v.sa * v.sa;                              // (1 + (alpha h)^n)^-m
+  v.ahnm1 = pf_div(v.t, v.ah);                    // (alpha h)^(n-1)
   return v;
 }
 
-__device__ __forceinline__ VGTerms vg_terms(double p, double rho, double g, double alpha,
-                                            double n)
-{
-  VGTerms v;
-  v.m = 1.0 - (1.0 / n);
-  double head = fabs(p) / (rho * g);
-  v.ah = alpha * head;
-  double L = log(v.ah);
-  v.t = exp(n * L);                 // (alpha h)^n
-  v.a = exp(-v.m * log1p(v.t));     // (1 + (alpha h)^n)^-m
-  v.sa = sqrt(v.a);                 // (1 + (alpha h)^n)^(-m/2)
-  v.ahnm1 = v.t / v.ah;             // (alpha h)^(n-1)
-  return v;
-}
-
-__device__ __forceinline__ double vg_sat(double p, double rho, double g,
-                                         double alpha, double n, double sres,
-                                         double ssat)
+__device__ __forceinline__ double vg_sat(double p, double irg, double alpha, double n,
+                                         double sres, double ssat)
 {
   double s_dif = ssat - sres;
   if (p >= 0.0) return s_dif + sres;
-  VGTerms v = vg_terms(p, rho, g, alpha, n);
+  VGTerms v = vg_terms(p, irg, alpha, n);
   return s_dif * v.a + sres;
 }
 
-__device__ __forceinline__ double vg_dsat(double p, double rho, double g,
-                                          double alpha, double n, double sres,
-                                          double ssat)
+__device__ __forceinline__ double vg_dsat(double p, double irg, double alpha, double n,
+                                          double sres, double ssat)
 {
   double s_dif = ssat - sres;
   if (p >= 0.0) return 0.0;
-  VGTerms v = vg_terms(p, rho, g, alpha, n);
-  return (v.m * n * alpha * v.ahnm1) * s_dif * (v.a / (1.0 + v.t));
+  VGTerms v = vg_terms(p, irg, alpha, n);
+  return (v.m * n * alpha * v.ahnm1) * s_dif * pf_div(v.a, 1.0 + v.t);
 }
 
 __device__ __forceinline__ double vg_kr_t(const VGTerms &v)
@@ -870,52 +834,33 @@ __device__ __forceinline__ double vg_kr_t(const VGTerms &v)
 
 __device__ __forceinline__ double vg_dkr_t(const VGTerms &v, double alpha, double n)
 {
-  double m = v.m, opahn = 1.0 + v.t;
+  double m = v.m, ropahn = pf_rcp(1.0 + v.t);
   double coeff = 1.0 - v.ahnm1 * v.a;
   return 2.0 * (coeff * v.sa) *
-             ((n - 1) * (v.ahnm1 / v.ah) * alpha * v.a -
-              v.ahnm1 * m * (v.a / opahn) * n * alpha * v.ahnm1) +
-         (coeff * coeff) * (m / 2) * (v.sa / opahn) * n * alpha * v.ahnm1;
+             ((n - 1) * pf_div(v.ahnm1, v.ah) * alpha * v.a -
+              v.ahnm1 * m * (v.a * ropahn) * n * alpha * v.ahnm1) +
+         (coeff * coeff) * (m / 2) * (v.sa * ropahn) * n * alpha * v.ahnm1;
 }
 
-__device__ __forceinline__ double vg_kr(double p, double rho, double g,
-                                        double alpha, double n)
+__device__ __forceinline__ double vg_kr(double p, double irg, double alpha, double n)
 {
   if (p >= 0.0) return 1.0;
-  return vg_kr_t(vg_terms(p, rho, g, alpha, n));
+  return vg_kr_t(vg_terms(p, irg, alpha, n));
 }
 
-__device__ __forceinline__ double vg_dkr(double p, double rho, double g,
-                                         double alpha, double n)
+__device__ __forceinline__ double vg_dkr(double p, double irg, double alpha, double n)
 {
   if (p >= 0.0) return 0.0;
-  return vg_dkr_t(vg_terms(p, rho, g, alpha, n), alpha, n);
-}
-
-__device__ __forceinline__ double vg_kr_i(double p, double irg, double alpha, double n)
-{
-  if (p >= 0.0) return 1.0;
-  return vg_kr_t(vg_terms_i(p, irg, alpha, n));
-}
-
-__device__ __forceinline__ void vg_s_kr_i(double p, double irg, double alpha, double n,
-                                          double sres, double ssat, double &S, double &kr)
-{
-  double s_dif = ssat - sres;
-  if (p >= 0.0) { S = s_dif + sres; kr = 1.0; return; }
-  VGTerms v = vg_terms_i(p, irg, alpha, n);
-  S = s_dif * v.a + sres;
-  kr = vg_kr_t(v);
+  return vg_dkr_t(vg_terms(p, irg, alpha, n), alpha, n);
 }
 
 // S and kr of one cell from one chain
-__device__ __forceinline__ void vg_s_kr(double p, double rho, double g, double alpha,
-                                        double n, double sres, double ssat, double &S,
-                                        double &kr)
+__device__ __forceinline__ void vg_s_kr(double p, double irg, double alpha, double n,
+                                        double sres, double ssat, double &S, double &kr)
 {
   double s_dif = ssat - sres;
   if (p >= 0.0) { S = s_dif + sres; kr = 1.0; return; }
-  VGTerms v = vg_terms(p, rho, g, alpha, n);
+  VGTerms v = vg_terms(p, irg, alpha, n);
   S = s_dif * v.a + sres;
   kr = vg_kr_t(v);
 }
@@ -954,15 +899,15 @@ __global__ __launch_bounds__(256) void k_closure(Geom g, Phys f, int what,
   int ip = (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
   double pv = p[ip], a = f.alpha[ip], n = f.nvg[ip];
   switch (what) {
-    case 0: out[ip] = vg_sat(pv, f.rho, f.g, a, n, f.Sres(ip), f.Ssat(ip)); break;
-    case 1: out[ip] = vg_dsat(pv, f.rho, f.g, a, n, f.Sres(ip), f.Ssat(ip)); break;
-    case 2: out[ip] = vg_kr(pv, f.rho, f.g, a, n); break;
-    case 3: out[ip] = vg_dkr(pv, f.rho, f.g, a, n); break;
-    case 4: out[ip] = vg_kr(pv, f.rho, f.g, a, n) * f.rho; break;
+    case 0: out[ip] = vg_sat(pv, f.irg, a, n, f.Sres(ip), f.Ssat(ip)); break;
+    case 1: out[ip] = vg_dsat(pv, f.irg, a, n, f.Sres(ip), f.Ssat(ip)); break;
+    case 2: out[ip] = vg_kr(pv, f.irg, a, n); break;
+    case 3: out[ip] = vg_dkr(pv, f.irg, a, n); break;
+    case 4: out[ip] = vg_kr(pv, f.irg, a, n) * f.rho; break;
     default: {
-      double kr = vg_kr(pv, f.rho, f.g, a, n);
+      double kr = vg_kr(pv, f.irg, a, n);
       out[ip] = kr * f.rho;
-      out2[ip] = vg_dkr(pv, f.rho, f.g, a, n) * f.rho + kr * 0.0;
+      out2[ip] = vg_dkr(pv, f.irg, a, n) * f.rho + kr * 0.0;
     }
   }
 }
@@ -1226,7 +1171,7 @@ __device__ __forceinline__ double residual_bc(const ResArgs &A, int i, int j, in
         fv += dt * dir * u_new;
       } else if (type == PFHIP_BC_DIR_EQUIL) {
         // ghost kr from the Dirichlet value inserted at :574-610
-        double kg = vg_kr(value, rho, grav, A.f.alpha[ip], A.f.nvg[ip]) * rho;
+        double kg = vg_kr(value, A.f.irg, A.f.alpha[ip], A.f.nvg[ip]) * rho;
         double u_new;
         if (fc == 0) {
           double diff = value - pc;
@@ -1329,12 +1274,12 @@ __global__ __launch_bounds__(RTX * TY) void k_residual_fused(ResArgs A)
     if (k0 > 0) {
       int ipm = col + k0 * g.sz;
       p_m = A.p[ipm];
-      kr_m = vg_kr(p_m, rho, grav, f.alpha[ipm], f.nvg[ipm]) * rho;
+      kr_m = vg_kr(p_m, f.irg, f.alpha[ipm], f.nvg[ipm]) * rho;
       kz_m = A.kz[ipm];
     }
     int ip0 = col + (k0 + 1) * g.sz;
     p_c = A.p[ip0];
-    vg_s_kr(p_c, rho, grav, f.alpha[ip0], f.nvg[ip0], f.Sres(ip0), f.Ssat(ip0), S_c, kr_c);
+    vg_s_kr(p_c, f.irg, f.alpha[ip0], f.nvg[ip0], f.Sres(ip0), f.Ssat(ip0), S_c, kr_c);
     kr_c *= rho;
     kz_c = A.kz[ip0];
   }
@@ -1348,7 +1293,7 @@ __global__ __launch_bounds__(RTX * TY) void k_residual_fused(ResArgs A)
     if (act && k + 1 < g.nz) {
       int ipn = ip + g.sz;
       p_n = A.p[ipn];
-      vg_s_kr(p_n, rho, grav, f.alpha[ipn], f.nvg[ipn], f.Sres(ipn), f.Ssat(ipn), S_n, kr_n);
+      vg_s_kr(p_n, f.irg, f.alpha[ipn], f.nvg[ipn], f.Sres(ipn), f.Ssat(ipn), S_n, kr_n);
       kr_n *= rho;
       kz_n = A.kz[ipn];
     }
@@ -1356,7 +1301,7 @@ __global__ __launch_bounds__(RTX * TY) void k_residual_fused(ResArgs A)
     if (hact) {
       double ph = A.p[hip];
       s_p[hls] = ph;
-      s_kr[hls] = vg_kr(ph, rho, grav, f.alpha[hip], f.nvg[hip]) * rho;
+      s_kr[hls] = vg_kr(ph, f.irg, f.alpha[hip], f.nvg[hip]) * rho;
     }
     if (act) { s_p[ls] = p_c; s_kr[ls] = kr_c; }
     __syncthreads();
@@ -1407,7 +1352,7 @@ __global__ __launch_bounds__(RTX * TY) void k_residual_fused(ResArgs A)
 // operand order as flux_xy / flux_z.
 __device__ __forceinline__ double hmean_f(double a, double b)
 {
-  return (a + b) ? fast_div(2.0 * a * b, a + b) : 0;
+  return (a + b) ? pf_div(2.0 * a * b, a + b) : 0;
 }
 __device__ __forceinline__ double flux_xy_i(double pa, double pb, double ka, double kb,
                                             double kra, double krb, double ff, double id,
@@ -1523,11 +1468,11 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
       VGIn<PCS> vm;
       load_vg<PCS>(vm, A, col + plane(k0 - 1));
       p_m = vm.p;
-      kr_m = vg_kr_i(p_m, irg, vm.al, vm.n) * rho;
+      kr_m = vg_kr(p_m, irg, vm.al, vm.n) * rho;
       kz_m = vm.kz;
     }
     p_c = v0.p;
-    vg_s_kr_i(p_c, irg, v0.al, v0.n, vg_sr<PCS>(v0, f), vg_st<PCS>(v0, f), S_c, kr_c);
+    vg_s_kr(p_c, irg, v0.al, v0.n, vg_sr<PCS>(v0, f), vg_st<PCS>(v0, f), S_c, kr_c);
     kr_c *= rho;
     kz_c = v0.kz;
   }
@@ -1553,13 +1498,13 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
     // ---- A: closures of the own cell one plane ahead and of the halo cell
     if (k + 1 < nz) {
       p_n = vgU.p;
-      vg_s_kr_i(p_n, irg, vgU.al, vgU.n, vg_sr<PCS>(vgU, f), vg_st<PCS>(vgU, f), S_n, kr_n);
+      vg_s_kr(p_n, irg, vgU.al, vgU.n, vg_sr<PCS>(vgU, f), vg_st<PCS>(vgU, f), S_n, kr_n);
       kr_n *= rho;
       kz_n = vgU.kz;
     }
     if (hact) {
       s_p[hls] = h.p;
-      s_kr[hls] = vg_kr_i(h.p, irg, h.al, h.n) * rho;
+      s_kr[hls] = vg_kr(h.p, irg, h.al, h.n) * rho;
     }
     s_p[ls] = p_c;
     s_kr[ls] = kr_c;
@@ -1655,7 +1600,7 @@ __global__ __launch_bounds__(256) void k_residual_bc(ResArgs A, long nshell)
   if (!shell_cell(A.g, idx, i, j, k)) return;
   const int ip = (i + 1) + (j + 1) * A.g.sy + (k + 1) * A.g.sz;
   double pc = A.p[ip];
-  double krc = vg_kr(pc, A.f.rho, A.f.g, A.f.alpha[ip], A.f.nvg[ip]) * A.f.rho;
+  double krc = vg_kr(pc, A.f.irg, A.f.alpha[ip], A.f.nvg[ip]) * A.f.rho;
   A.F[ip] = residual_bc(A, i, j, k, ip, A.F[ip], pc, krc, A.kx[ip], A.ky[ip], A.kz[ip]);
 }
 
@@ -1779,8 +1724,8 @@ __global__ __launch_bounds__(256) void k_jacobian(JacArgs A)
   const bool symm = A.symm != 0;
   double pc = p[ip];
   double a = A.f.alpha[ip], nn = A.f.nvg[ip];
-  double S = vg_sat(pc, rho, grav, a, nn, A.f.Sres(ip), A.f.Ssat(ip));
-  double dS = vg_dsat(pc, rho, grav, a, nn, A.f.Sres(ip), A.f.Ssat(ip));
+  double S = vg_sat(pc, A.f.irg, a, nn, A.f.Sres(ip), A.f.Ssat(ip));
+  double dS = vg_dsat(pc, A.f.irg, a, nn, A.f.Sres(ip), A.f.Ssat(ip));
   double vol2 = vol * 1.0;
   double c = (dS * rho + S * ddp) * A.poro[ip] * vol2 +
              A.ss[ip] * vol2 * (dS * rho * pc + S * ddp * pc + S * rho);
@@ -1880,7 +1825,7 @@ __global__ __launch_bounds__(256) void k_jacobian(JacArgs A)
       double value = A.bcv[fc][fi];
       double den_d = rho;
       double prod = krc, prod_der = dkc;
-      double prod_val = vg_kr(value, rho, grav, a, nn) * den_d;
+      double prod_val = vg_kr(value, A.f.irg, a, nn) * den_d;
       double coeff, diff, o_temp;
       if (fc == 0 || fc == 2) {
         double ff = (fc == 0) ? ffx : ffy, dd = (fc == 0) ? dx : dy;
