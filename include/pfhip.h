@@ -136,6 +136,15 @@ int  pfhip_vec_max_norm(pfhip_problem *pb, const double *x, double *result,
 int  pfhip_vec_l1_norm(pfhip_problem *pb, const double *x, double *result,
                        void *stream);
 
+/* ---- water balance ------------------------------------------------------ */
+/* calculate_subsurface_storage / calculate_surface_storage
+ * (pftools/python/parflow/tools/hydrology.py:136-205) summed over the
+ * domain: subsurface = sum(phi S dz dx dy + [p >= 0] p S Ss dz dx dy),
+ * surface = sum over the top layer of max(p dx dy, 0).  p, s device. */
+int  pfhip_water_balance_storage(pfhip_problem *pb, const double *p,
+                                 const double *s, double *subsurface,
+                                 double *surface, void *stream);
+
 /* ---- multigrid preconditioner (PrecondInvoke, PL/parflow_proto.h:418) - */
 /* Semicoarsening V-cycle with the level structure, operator-induced
  * transfers and Galerkin-collapsed coarse operators of MGSemi

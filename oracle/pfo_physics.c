@@ -948,28 +948,37 @@ double pfo_l1norm(const pfo_problem *pb, const double *x)
   return s;
 }
 
-/* Subsurface storage sum(phi*S*V + p*S*Ss*V) and surface storage
- * sum(max(p_top,0)*dx*dy): pftools/python/parflow/tools/hydrology.py
+/* Subsurface storage sum(phi*S*V + [p >= 0] p*S*Ss*V) and surface storage
+ * sum(max(p_top*dx*dy, 0)): pftools/python/parflow/tools/hydrology.py
  * :136-205 restated for a box. */
 void pfo_storage(const pfo_problem *pb, const double *p, const double *s,
                  double *subsurface, double *surface)
 {
+  /* calculate_subsurface_storage / calculate_surface_storage,
+   * pftools/python/parflow/tools/hydrology.py:136-205: per cell
+   * incompressible = porosity*saturation*dz*dx*dy, compressible =
+   * pressure*saturation*specific_storage*dz*dx*dy set to 0 where pressure < 0;
+   * per top cell pressure*dx*dy set to 0 where negative; summed x fastest. */
   const int nx = pb->nx, ny = pb->ny, nz = pb->nz;
   const int sy = nx + 2, sz = (nx + 2) * (ny + 2);
-  const double vol = pb->dx * pb->dy * pb->dz;
+  const double dx = pb->dx, dy = pb->dy, dz = pb->dz;
   double sub = 0.0, sur = 0.0;
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++)
       for (int i = 0; i < nx; i++)
       {
         int ip = IDX(i, j, k);
-        sub += pb->poro[ip] * s[ip] * vol + p[ip] * s[ip] * pb->ss[ip] * vol;
+        double inc = pb->poro[ip] * s[ip] * dz * dx * dy;
+        double comp = p[ip] * s[ip] * pb->ss[ip] * dz * dx * dy;
+        if (p[ip] < 0.0) comp = 0.0;
+        sub += inc + comp;
       }
   for (int j = 0; j < ny; j++)
     for (int i = 0; i < nx; i++)
     {
-      double pt = p[IDX(i, j, nz - 1)];
-      if (pt > 0.0) sur += pt * pb->dx * pb->dy;
+      double t = p[IDX(i, j, nz - 1)] * dx * dy;
+      if (t < 0.0) t = 0.0;
+      sur += t;
     }
   *subsurface = sub;
   *surface = sur;

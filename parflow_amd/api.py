@@ -84,7 +84,7 @@ EXPORTS = [
     "pfhip_vec_linear_sum", "pfhip_vec_const", "pfhip_vec_dot", "pfhip_vec_max_norm",
     "pfhip_vec_l1_norm", "pfhip_mg_create", "pfhip_mg_destroy", "pfhip_mg_num_levels",
     "pfhip_mg_setup", "pfhip_mg_apply", "pfhip_solver_create", "pfhip_solver_destroy",
-    "pfhip_kinsol_nonlin_solver", "pfhip_advance_richards",
+    "pfhip_kinsol_nonlin_solver", "pfhip_advance_richards", "pfhip_water_balance_storage",
 ]
 
 
@@ -118,6 +118,8 @@ def lib():
             getattr(L, f).argtypes = [vp, vp, vp, C.POINTER(C.c_double), vp]
         for f in ("pfhip_vec_max_norm", "pfhip_vec_l1_norm"):
             getattr(L, f).argtypes = [vp, vp, C.POINTER(C.c_double), vp]
+        L.pfhip_water_balance_storage.argtypes = [vp, vp, vp, C.POINTER(C.c_double),
+                                                  C.POINTER(C.c_double), vp]
         L.pfhip_mg_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.POINTER(vp)]
         L.pfhip_mg_destroy.argtypes = [vp]
@@ -356,6 +358,16 @@ class GPUProblem:
         r = C.c_double()
         check(lib().pfhip_vec_l1_norm(self.h, dx_.ptr, C.byref(r), None), "l1norm")
         return r.value
+
+    def water_balance_storage(self, p, s):
+        """(subsurface, surface) storage of pressure p and saturation s
+        (host arrays or DeviceArrays)."""
+        dp = p if isinstance(p, DeviceArray) else self.vec(p)
+        ds = s if isinstance(s, DeviceArray) else self.vec(s)
+        a, b = C.c_double(), C.c_double()
+        check(lib().pfhip_water_balance_storage(self.h, dp.ptr, ds.ptr, C.byref(a),
+                                                C.byref(b), None), "storage")
+        return a.value, b.value
 
 
 class GPUMultigrid:

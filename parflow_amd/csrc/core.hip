@@ -540,6 +540,64 @@ extern "C" int pfhip_vec_dq_sums(pfhip_problem *pb, const double *u,
   return PFHIP_OK;
 }
 
+// ---- water-balance storage ---------------------------------------------
+// calculate_subsurface_storage / calculate_surface_storage
+// (pftools/python/parflow/tools/hydrology.py:136-205) summed over the box,
+// with their per-cell operation order; row-owning blocks and the fixed-order
+// final pass make the sums reproducible bit for bit.
+__global__ __launch_bounds__(256) void k_storage(Geom g, double dx, double dy, double dz,
+                                                 const double *__restrict__ p,
+                                                 const double *__restrict__ s,
+                                                 const double *__restrict__ poro,
+                                                 const double *__restrict__ ss,
+                                                 double *__restrict__ partial, int nb)
+{
+  double sub = 0.0, sur = 0.0;
+  int rows = g.ny * g.nz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    int j = r % g.ny, k = r / g.ny;
+    int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+      int ip = base + i;
+      double pv = p[ip], sv = s[ip];
+      double inc = poro[ip] * sv * dz * dx * dy;
+      double comp = pv * sv * ss[ip] * dz * dx * dy;
+      if (pv < 0.0) comp = 0.0;
+      sub += inc + comp;
+      if (k == g.nz - 1) {
+        double t = pv * dx * dy;
+        sur += (t < 0.0) ? 0.0 : t;
+      }
+    }
+  }
+  sub = block_reduce<false>(sub);
+  sur = block_reduce<false>(sur);
+  if (threadIdx.x == 0) { partial[blockIdx.x] = sub; partial[nb + blockIdx.x] = sur; }
+}
+
+extern "C" int pfhip_water_balance_storage(pfhip_problem *pb, const double *p,
+                                           const double *s, double *subsurface,
+                                           double *surface, void *stream)
+{
+  if (!p || !s || !subsurface || !surface) {
+    pfhip_set_error("water_balance_storage: NULL argument");
+    return PFHIP_ERR_ARG;
+  }
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  hipLaunchKernelGGL(k_storage, dim3(nb), dim3(256), 0, STREAM(stream), g, pb->d.dx,
+                     pb->d.dy, pb->d.dz, p, s, pb->d.poro, pb->d.ss, pb->red_partial, nb);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, STREAM(stream),
+                     pb->red_partial, nb, 2, pb->red_dev);
+  LAUNCHCHK();
+  int rc = fetch_results(pb, 2, stream);
+  if (rc) return rc;
+  *subsurface = pb->h_red[0];
+  *surface = pb->h_red[1];
+  return PFHIP_OK;
+}
+
 // ---- block Gram-Schmidt kernels ------------------------------------------
 #define MD_CHUNK 16
 
@@ -1063,6 +1121,11 @@ struct ResArgs {
   double dx, dy, dz, dt;
   double idx, idy, idz, imu;         // reciprocals of dx, dy, dz, mu
   const double *__restrict__ p;
+  // difference-quotient J.v (k_residual_pf JV form): the pressure is
+  // p + psig pv and the cell output fsc F - fsc fsub
+  const double *__restrict__ pv;
+  const double *__restrict__ fsub;
+  double psig, fsc;
   const double *__restrict__ p_old;
   const double *__restrict__ s_old;
   const double *__restrict__ kx;
@@ -1381,13 +1444,20 @@ __device__ __forceinline__ double flux_z_i(double pa, double pb, double ka, doub
 // are per-cell fields.
 template <bool PCS> struct VGIn { double p, al, n, kz, sr, st; };
 template <> struct VGIn<false> { double p, al, n, kz; };
-struct OthIn { double kx, kxe, ky, kyn, so, po, ss, pold, hk0, hk1; };
+struct OthIn { double kx, kxe, ky, kyn, so, po, ss, pold, hk0, hk1, fs; };
 struct HalIn { double p, al, n; };
 
-template <bool PCS>
+// the pressure the residual is evaluated at: p, or p + psig pv (J.v form)
+template <bool JV> __device__ __forceinline__ double p_at(const ResArgs &A, int ip)
+{
+  if constexpr (JV) return A.p[ip] + A.psig * A.pv[ip];
+  else return A.p[ip];
+}
+
+template <bool PCS, bool JV>
 __device__ __forceinline__ void load_vg(VGIn<PCS> &v, const ResArgs &A, int ip)
 {
-  v.p = A.p[ip]; v.al = A.f.alpha[ip]; v.n = A.f.nvg[ip]; v.kz = A.kz[ip];
+  v.p = p_at<JV>(A, ip); v.al = A.f.alpha[ip]; v.n = A.f.nvg[ip]; v.kz = A.kz[ip];
   if constexpr (PCS) { v.sr = A.f.sres[ip]; v.st = A.f.ssat[ip]; }
 }
 template <bool PCS> __device__ __forceinline__ double vg_sr(const VGIn<PCS> &v, const Phys &f)
@@ -1398,19 +1468,22 @@ template <bool PCS> __device__ __forceinline__ double vg_st(const VGIn<PCS> &v, 
 {
   if constexpr (PCS) return v.st; else return f.ssat_v;
 }
+template <bool JV>
 __device__ __forceinline__ void load_oth(OthIn &o, const ResArgs &A, int ip, int sy, int hip,
                                          const double *__restrict__ Kh, int hoff)
 {
   o.kx = A.kx[ip]; o.kxe = A.kx[ip + 1]; o.ky = A.ky[ip]; o.kyn = A.ky[ip + sy];
   o.hk0 = Kh[hip]; o.hk1 = Kh[hip + hoff];
   o.so = A.s_old[ip]; o.po = A.poro[ip]; o.ss = A.ss[ip]; o.pold = A.p_old[ip];
+  if constexpr (JV) o.fs = A.fsub[ip];
 }
+template <bool JV>
 __device__ __forceinline__ void load_hal(HalIn &h, const ResArgs &A, int hip)
 {
-  h.p = A.p[hip]; h.al = A.f.alpha[hip]; h.n = A.f.nvg[hip];
+  h.p = p_at<JV>(A, hip); h.al = A.f.alpha[hip]; h.n = A.f.nvg[hip];
 }
 
-template <int TX, int TY, int ZC, bool PCS>
+template <int TX, int TY, int ZC, bool PCS, bool JV>
 __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
 {
   constexpr int HALO = 2 * TX + 2 * TY;
@@ -1463,10 +1536,10 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
   double p_n = 0, kr_n = 0, S_n = 0, kz_n = 0;
   {
     VGIn<PCS> v0;
-    load_vg<PCS>(v0, A, col + plane(k0));
+    load_vg<PCS, JV>(v0, A, col + plane(k0));
     if (k0 > 0) {
       VGIn<PCS> vm;
-      load_vg<PCS>(vm, A, col + plane(k0 - 1));
+      load_vg<PCS, JV>(vm, A, col + plane(k0 - 1));
       p_m = vm.p;
       kr_m = vg_kr(p_m, irg, vm.al, vm.n) * rho;
       kz_m = vm.kz;
@@ -1482,19 +1555,19 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
 
   VGIn<PCS> vgN;
   HalIn hal;
-  load_vg<PCS>(vgN, A, col + plane(k0 + 1));
-  load_hal(hal, A, hcol + plane(k0));
+  load_vg<PCS, JV>(vgN, A, col + plane(k0 + 1));
+  load_hal<JV>(hal, A, hcol + plane(k0));
 
   for (int k = k0; k < k1; k++) {
     const int ip = col + (k + 1) * sz;
     // this plane's flux and storage inputs (used after the closures), then
     // the prefetch of the next plane's closure inputs
     OthIn o;
-    load_oth(o, A, col + plane(k), sy, hcol + plane(k), Kh, hoff);
+    load_oth<JV>(o, A, col + plane(k), sy, hcol + plane(k), Kh, hoff);
     const VGIn<PCS> vgU = vgN;
     const HalIn h = hal;
-    load_vg<PCS>(vgN, A, col + plane(k + 2));
-    load_hal(hal, A, hcol + plane(k + 1));
+    load_vg<PCS, JV>(vgN, A, col + plane(k + 2));
+    load_hal<JV>(hal, A, hcol + plane(k + 1));
     // ---- A: closures of the own cell one plane ahead and of the halo cell
     if (k + 1 < nz) {
       p_n = vgU.p;
@@ -1530,8 +1603,15 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
     if (act) {
       double ul = s_ux[ty * (TX + 1) + tx];
       double ud = s_uy[ty * TX + tx];
-      A.F[ip] = residual_cell(A, i, j, k, ip, p_c, kr_c, S_c, o.kx, o.ky, kz_c, ur, uf, uu,
-                              ul, ud, ub, o.so, o.po, o.ss, o.pold);   // + k_residual_bc
+      double fv = residual_cell(A, i, j, k, ip, p_c, kr_c, S_c, o.kx, o.ky, kz_c, ur, uf, uu,
+                                ul, ud, ub, o.so, o.po, o.ss, o.pold);   // + k_residual_bc
+      if constexpr (JV) {
+        // boundary cells get the transform after their face terms
+        const bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
+                         (k == nz - 1);
+        fv = bnd ? fv : A.fsc * fv - A.fsc * o.fs;
+      }
+      A.F[ip] = fv;
     }
     ub = uu;
     p_m = p_c; kr_m = kr_c; kz_m = kz_c;
@@ -1591,7 +1671,9 @@ static long shell_count(const Geom &g)
   return (long)g.nx * g.ny * nzb + (long)g.nx * nyb * nzi + (long)nxb * nyi * nzi;
 }
 
-// F += boundary-face terms (k_residual_pf leaves them out)
+// F += boundary-face terms (k_residual_pf leaves them out); in the J.v
+// form the output transform follows them
+template <bool JV>
 __global__ __launch_bounds__(256) void k_residual_bc(ResArgs A, long nshell)
 {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1599,12 +1681,14 @@ __global__ __launch_bounds__(256) void k_residual_bc(ResArgs A, long nshell)
   int i, j, k;
   if (!shell_cell(A.g, idx, i, j, k)) return;
   const int ip = (i + 1) + (j + 1) * A.g.sy + (k + 1) * A.g.sz;
-  double pc = A.p[ip];
+  double pc = p_at<JV>(A, ip);
   double krc = vg_kr(pc, A.f.irg, A.f.alpha[ip], A.f.nvg[ip]) * A.f.rho;
-  A.F[ip] = residual_bc(A, i, j, k, ip, A.F[ip], pc, krc, A.kx[ip], A.ky[ip], A.kz[ip]);
+  double fv = residual_bc(A, i, j, k, ip, A.F[ip], pc, krc, A.kx[ip], A.ky[ip], A.kz[ip]);
+  if constexpr (JV) fv = A.fsc * fv - A.fsc * A.fsub[ip];
+  A.F[ip] = fv;
 }
 
-template <int TX, int TY, int ZC>
+template <int TX, int TY, int ZC, bool JV>
 static void launch_pf(const ResArgs &A, hipStream_t st)
 {
   const Geom &g = A.g;
@@ -1612,27 +1696,29 @@ static void launch_pf(const ResArgs &A, hipStream_t st)
   dim3 block(TX, TY, 1);
   PROF_BEGIN(PFHIP_PROF_RESIDUAL, st);
   if (A.f.sres || A.f.ssat)
-    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true>), grid, block, 0, st, A);
+    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, JV>), grid, block, 0, st, A);
   else
-    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false>), grid, block, 0, st, A);
+    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, JV>), grid, block, 0, st, A);
   PROF_END(PFHIP_PROF_RESIDUAL, st);
   long ns = shell_count(g);
-  hipLaunchKernelGGL(k_residual_bc, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, A, ns);
+  hipLaunchKernelGGL(k_residual_bc<JV>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
+                     A, ns);
 }
 
 static void launch_residual(const ResArgs &A, hipStream_t st)
 {
   const Geom &g = A.g;
   int v = residual_variant();
-  switch (v) {
-    case 10: launch_pf<64, 16, 16>(A, st); return;
-    case 11: launch_pf<32, 16, 16>(A, st); return;
-    case 12: launch_pf<64, 8, 32>(A, st); return;
-    case 13: launch_pf<32, 32, 16>(A, st); return;
-    default: break;
+  const bool jv = A.pv != nullptr;
+  if (v == 12) {
+    if (jv) launch_pf<64, 8, 32, true>(A, st); else launch_pf<64, 8, 32, false>(A, st);
+    return;
   }
-  if (v == 0) { launch_pf<64, 8, 16>(A, st); return; }
-  if (v == 5) v = 0;   // the unpipelined kernel, 64 x 8 x 16
+  if (v != 5 || jv) {
+    if (jv) launch_pf<64, 8, 16, true>(A, st); else launch_pf<64, 8, 16, false>(A, st);
+    return;
+  }
+  v = 0;   // 5: the unpipelined kernel, 64 x 8 x 16
   PROF_BEGIN(PFHIP_PROF_RESIDUAL, st);
   int TY = (v == 0 || v == 1 || v == 4) ? 8 : 4;
   int ZC = (v == 0) ? 16 : (v == 1) ? 64 : (v == 2) ? 16 : (v == 3) ? 8 : 4;
@@ -1669,6 +1755,37 @@ extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double
   A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
   for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
   A.F = F;
+  A.pv = nullptr; A.fsub = nullptr; A.psig = 0.0; A.fsc = 0.0;
+  launch_residual(A, STREAM(stream));
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// Difference-quotient J.v of KINSpgmrAtimesDQ (KS/kinspgmr.c:481-531) in one
+// residual launch: z = sinv F(u + sigma v) - sinv fval, without the two
+// separate vector passes.  Not for problems with overland flow (their
+// surface fluxes are evaluated from the pressure beforehand).
+int pfhip_residual_jv(pfhip_problem *pb, const double *u, const double *v, double sigma,
+                      const double *fval, double sinv, double *z, const double *p_old,
+                      const double *s_old, double dt, void *stream)
+{
+  if (pb->overland) {
+    pfhip_set_error("residual_jv: overland problems take the two-pass form");
+    return PFHIP_ERR_ARG;
+  }
+  Geom g = geom_of(pb);
+  ResArgs A;
+  A.g = g;
+  A.f = phys_of(pb);
+  A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
+  A.idx = 1.0 / A.dx; A.idy = 1.0 / A.dy; A.idz = 1.0 / A.dz; A.imu = 1.0 / A.f.mu;
+  A.p = u; A.p_old = p_old; A.s_old = s_old;
+  A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
+  A.poro = pb->d.poro; A.ss = pb->d.ss; A.src = pb->d.src;
+  A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
+  for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
+  A.F = z;
+  A.pv = v; A.fsub = fval; A.psig = sigma; A.fsc = sinv;
   launch_residual(A, STREAM(stream));
   LAUNCHCHK();
   return PFHIP_OK;
@@ -1705,117 +1822,310 @@ struct JacArgs {
   double *J[7];
 };
 
-__global__ __launch_bounds__(256) void k_jacobian(JacArgs A)
+// ---- fused, tiled Jacobian -------------------------------------------------
+// One launch per assembly: a TX x TY tile of columns marches a z-chunk like
+// k_residual_pf; every cell evaluates ONE van Genuchten chain (S, dS/dp, kr,
+// dkr/dp), every face is evaluated once by its lower cell, which keeps its
+// own terms and passes the upper cell's through LDS (x, y faces) or a
+// register (z face).  Per face (a = lower cell, b = upper) the reference's
+// two branches (PL/richards_jacobian_eval.c:594-776) reduce to
+//   T1 = -(a's derivative term) + sym   ->  c_a -= T1, full-J W/S/L of b
+//   T2 =  (b's derivative term) + sym   ->  c_b -= T2, full-J E/N/U of a
+// with the symmetric part sym stored off the diagonal by PFSymmetric.  The
+// diagonal subtracts its faces in the reference's order E, W, N, S, U, L.
+// Dirichlet faces and the overland top rows are applied afterwards by
+// k_jacobian_bc.  OT is the coefficient type (float: the FP32 multigrid's
+// finest-level matrix is written directly).
+template <typename OT> struct JOut { OT *c[7]; };
+
+struct JacPFArgs {
+  JacArgs a;
+  double idx, idy, idz, imu;
+};
+
+__device__ __forceinline__ void vg_all(double p, double irg, double alpha, double n,
+                                       double sres, double ssat, double &S, double &dS,
+                                       double &kr, double &dkr)
 {
+  double s_dif = ssat - sres;
+  if (p >= 0.0) { S = s_dif + sres; dS = 0.0; kr = 1.0; dkr = 0.0; return; }
+  VGTerms v = vg_terms(p, irg, alpha, n);
+  S = s_dif * v.a + sres;
+  dS = (v.m * n * alpha * v.ahnm1) * s_dif * pf_div(v.a, 1.0 + v.t);
+  kr = vg_kr_t(v);
+  dkr = vg_dkr_t(v, alpha, n);
+}
+
+template <bool PCS> struct JVG { double p, al, n, kz, sr, st; };
+template <> struct JVG<false> { double p, al, n, kz; };
+struct JOth { double kx, kxe, ky, kyn, po, ss, hk0, hk1; };
+struct JHal { double p, al, n; };
+
+template <bool PCS>
+__device__ __forceinline__ void jload_vg(JVG<PCS> &v, const JacArgs &A, int ip)
+{
+  v.p = A.p[ip]; v.al = A.f.alpha[ip]; v.n = A.f.nvg[ip]; v.kz = A.kz[ip];
+  if constexpr (PCS) { v.sr = A.f.sres[ip]; v.st = A.f.ssat[ip]; }
+}
+template <bool PCS> __device__ __forceinline__ double jsr(const JVG<PCS> &v, const Phys &f)
+{
+  if constexpr (PCS) return v.sr; else return f.sres_v;
+}
+template <bool PCS> __device__ __forceinline__ double jst(const JVG<PCS> &v, const Phys &f)
+{
+  if constexpr (PCS) return v.st; else return f.ssat_v;
+}
+
+// x or y face: T1 (for c_a, full-J W/S of b), T2 (for c_b, full-J E/N of a)
+// and the symmetric couplings; ysign selects the y-face spelling whose
+// symmetric part is written yc * -upw (same value)
+__device__ __forceinline__ void jface_xy(double pa, double pb, double ka, double kb,
+                                         double kra, double krb, double dka, double dkb,
+                                         double dt, double ff, double id, double gc,
+                                         double imu, double &T1, double &T2, double &sym)
+{
+  double diff = pa - pb;
+  double updir = (diff * id) * gc;
+  double xc = 1.0 * dt * ff * id * 1.0 * hmean_f(ka, kb) * imu;
+  sym = (-xc * upw(updir, 0.0, kra, krb)) * gc;
+  T1 = (-xc * diff * upw(updir, 0.0, dka, 0.0)) * gc + sym;
+  T2 = (xc * diff * upw(updir, 0.0, 0.0, dkb)) * gc + sym;
+}
+
+// z face (constant density: the density-derivative terms vanish)
+__device__ __forceinline__ void jface_z(double pa, double pb, double ka, double kb,
+                                        double kra, double krb, double dka, double dkb,
+                                        double dt, double ffz, double isep, double hrg,
+                                        double imu, double &T1, double &T2, double &sym_l,
+                                        double &sym_u)
+{
+  double lc = pa * isep - hrg;
+  double uc = pb * isep + hrg;
+  double diff = lc - uc;
+  double zc = 1.0 * dt * ffz * hmean_f(ka, kb) * imu;
+  double up = upw(lc, uc, kra, krb);
+  sym_l = -zc * isep * up;
+  sym_u = zc * isep * -up;
+  T1 = -zc * (diff * upw(lc, uc, dka, 0.0)) + sym_l;
+  T2 = zc * (diff * upw(lc, uc, 0.0, dkb)) + sym_u;
+}
+
+template <int TX, int TY, int ZC, bool PCS, bool SYMM, typename OT>
+__global__ __launch_bounds__(TX * TY) void k_jacobian_pf(JacPFArgs P, JOut<OT> J)
+{
+  constexpr int HALO = 2 * TX + 2 * TY;
+  constexpr int W2 = TX + 2;
+  constexpr int LDSN = (TX + 2) * (TY + 2);
+  __shared__ double s_p[LDSN];
+  __shared__ double s_kr[LDSN];
+  __shared__ double s_dk[LDSN];
+  __shared__ double s_xc[TY * (TX + 1)];   // x faces: T2 for the east cell
+  __shared__ double s_yc[(TY + 1) * TX];   // y faces: T2 for the north cell
+  __shared__ double s_xw[SYMM ? 1 : TY * (TX + 1)];   // full J: W of the east cell
+  __shared__ double s_ys[SYMM ? 1 : (TY + 1) * TX];   // full J: S of the north cell
+  const JacArgs &A = P.a;
+  const Geom g = A.g;
+  const Phys &f = A.f;
+  const double rho = f.rho, gc = f.g, irg = f.irg, imu = P.imu;
+  const double dt = A.dt, dx = A.dx, dy = A.dy, dz = A.dz;
+  const double idx = P.idx, idy = P.idy;
+  const double isep = 1.0 / (dz * 0.5 * (1.0 + 1.0));
+  const double hrg = (1.0 / (1.0 + 1.0)) * rho * gc;
+  const double vol2 = dx * dy * dz * 1.0;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
+  const double ddp = 0.0;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int i = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty;
+  const bool act = (i < g.nx) && (j < g.ny);
+  const int nz = g.nz, sy = g.sy, sz = g.sz;
+  const int ls = (ty + 1) * W2 + (tx + 1);
+  const int col = (min(i, g.nx - 1) + 1) + (min(j, g.ny - 1) + 1) * sy;
+  int hx = 0, hy = 0;
+  bool hact = false;
+  if (tid < HALO) {
+    if (tid < TX) { hx = tid + 1; hy = 0; }
+    else if (tid < 2 * TX) { hx = tid - TX + 1; hy = TY + 1; }
+    else if (tid < 2 * TX + TY) { hx = 0; hy = tid - 2 * TX + 1; }
+    else { hx = TX + 1; hy = tid - 2 * TX - TY + 1; }
+    int hi = blockIdx.x * TX + hx - 1, hj = blockIdx.y * TY + hy - 1;
+    hact = (hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny);
+  }
+  const int hls = hy * W2 + hx;
+  const int hcol = hact ? (blockIdx.x * TX + hx) + (blockIdx.y * TY + hy) * sy : col;
+  const bool hwest = hact && hx == 0;
+  const bool hsouth = hact && hy == 0;
+  const double *__restrict__ Kh = hsouth ? A.ky : A.kx;
+  const int hoff = hsouth ? sy : 1;
+  const int k0 = blockIdx.z * ZC;
+  const int k1 = min(k0 + ZC, nz);
+  auto plane = [&](int q) { return (min(q, nz - 1) + 1) * sz; };
+
+  double p_c = 0, S_c = 0, dS_c = 0, kr_c = 0, dk_c = 0, kz_c = 0;
+  double p_n = 0, S_n = 0, dS_n = 0, kr_n = 0, dk_n = 0, kz_n = 0;
+  double T2b = 0.0, T1b = 0.0, syml_b = 0.0;   // z face below the own cell
+  {
+    JVG<PCS> v0;
+    jload_vg<PCS>(v0, A, col + plane(k0));
+    p_c = v0.p;
+    vg_all(p_c, irg, v0.al, v0.n, jsr<PCS>(v0, f), jst<PCS>(v0, f), S_c, dS_c, kr_c, dk_c);
+    kr_c *= rho;
+    dk_c = dk_c * rho + kr_c * 0.0;
+    kz_c = v0.kz;
+    if (k0 > 0) {
+      JVG<PCS> vm;
+      jload_vg<PCS>(vm, A, col + plane(k0 - 1));
+      double Sm, dSm, krm, dkm;
+      vg_all(vm.p, irg, vm.al, vm.n, jsr<PCS>(vm, f), jst<PCS>(vm, f), Sm, dSm, krm, dkm);
+      krm *= rho;
+      dkm = dkm * rho + krm * 0.0;
+      double symu;
+      jface_z(vm.p, p_c, vm.kz, kz_c, krm, kr_c, dkm, dk_c, dt, ffz, isep, hrg, imu, T1b,
+              T2b, syml_b, symu);
+    }
+  }
+  JVG<PCS> vgN;
+  JHal hal;
+  jload_vg<PCS>(vgN, A, col + plane(k0 + 1));
+  hal.p = A.p[hcol + plane(k0)]; hal.al = f.alpha[hcol + plane(k0)];
+  hal.n = f.nvg[hcol + plane(k0)];
+
+  for (int k = k0; k < k1; k++) {
+    const int ip = col + (k + 1) * sz;
+    JOth o;
+    {
+      const int q = col + plane(k), hq = hcol + plane(k);
+      o.kx = A.kx[q]; o.kxe = A.kx[q + 1]; o.ky = A.ky[q]; o.kyn = A.ky[q + sy];
+      o.po = A.poro[q]; o.ss = A.ss[q];
+      o.hk0 = Kh[hq]; o.hk1 = Kh[hq + hoff];
+    }
+    const JVG<PCS> vgU = vgN;
+    const JHal h = hal;
+    jload_vg<PCS>(vgN, A, col + plane(k + 2));
+    {
+      const int hq = hcol + plane(k + 1);
+      hal.p = A.p[hq]; hal.al = f.alpha[hq]; hal.n = f.nvg[hq];
+    }
+    // ---- A: closures one plane ahead (own) and of the halo cell
+    if (k + 1 < nz) {
+      p_n = vgU.p;
+      vg_all(p_n, irg, vgU.al, vgU.n, jsr<PCS>(vgU, f), jst<PCS>(vgU, f), S_n, dS_n, kr_n,
+             dk_n);
+      kr_n *= rho;
+      dk_n = dk_n * rho + kr_n * 0.0;
+      kz_n = vgU.kz;
+    }
+    if (hact) {
+      double Sh, dSh, krh, dkh;
+      vg_all(h.p, irg, h.al, h.n, f.sres_v, f.ssat_v, Sh, dSh, krh, dkh);
+      krh *= rho;
+      s_p[hls] = h.p;
+      s_kr[hls] = krh;
+      s_dk[hls] = dkh * rho + krh * 0.0;
+    }
+    s_p[ls] = p_c;
+    s_kr[ls] = kr_c;
+    s_dk[ls] = dk_c;
+    lds_barrier();
+    // ---- B: own E, N, U faces; west / south halo faces into the tile
+    double T1e = 0, T1n = 0, T1u = 0, T2u = 0, offE = 0, offN = 0, offU = 0, symlu = 0;
+    if (i + 1 < g.nx) {
+      double T2, sym;
+      jface_xy(p_c, s_p[ls + 1], o.kx, o.kxe, kr_c, s_kr[ls + 1], dk_c, s_dk[ls + 1], dt, ffx,
+               idx, gc, imu, T1e, T2, sym);
+      offE = SYMM ? sym : T2;
+      s_xc[ty * (TX + 1) + tx + 1] = T2;
+      if constexpr (!SYMM) s_xw[ty * (TX + 1) + tx + 1] = T1e;
+    }
+    if (j + 1 < g.ny) {
+      double T2, sym;
+      jface_xy(p_c, s_p[ls + W2], o.ky, o.kyn, kr_c, s_kr[ls + W2], dk_c, s_dk[ls + W2], dt,
+               ffy, idy, gc, imu, T1n, T2, sym);
+      offN = SYMM ? sym : T2;
+      s_yc[(ty + 1) * TX + tx] = T2;
+      if constexpr (!SYMM) s_ys[(ty + 1) * TX + tx] = T1n;
+    }
+    if (k + 1 < nz) {
+      double syml, symu;
+      jface_z(p_c, p_n, kz_c, kz_n, kr_c, kr_n, dk_c, dk_n, dt, ffz, isep, hrg, imu, T1u, T2u,
+              syml, symu);
+      offU = SYMM ? symu : T2u;
+      symlu = syml;
+    }
+    if (hwest) {
+      double T1, T2, sym;
+      jface_xy(s_p[hls], s_p[hls + 1], o.hk0, o.hk1, s_kr[hls], s_kr[hls + 1], s_dk[hls],
+               s_dk[hls + 1], dt, ffx, idx, gc, imu, T1, T2, sym);
+      s_xc[(hy - 1) * (TX + 1)] = T2;
+      if constexpr (!SYMM) s_xw[(hy - 1) * (TX + 1)] = T1;
+    }
+    if (hsouth) {
+      double T1, T2, sym;
+      jface_xy(s_p[hls], s_p[hls + W2], o.hk0, o.hk1, s_kr[hls], s_kr[hls + W2], s_dk[hls],
+               s_dk[hls + W2], dt, ffy, idy, gc, imu, T1, T2, sym);
+      s_yc[hx - 1] = T2;
+      if constexpr (!SYMM) s_ys[hx - 1] = T1;
+    }
+    lds_barrier();
+    // ---- C: the row of the own cell
+    if (act) {
+      double c = (dS_c * rho + S_c * ddp) * o.po * vol2 +
+                 o.ss * vol2 * (dS_c * rho * p_c + S_c * ddp * p_c + S_c * rho);
+      if (i + 1 < g.nx) c -= T1e;
+      if (i > 0) c -= s_xc[ty * (TX + 1) + tx];
+      if (j + 1 < g.ny) c -= T1n;
+      if (j > 0) c -= s_yc[ty * TX + tx];
+      if (k + 1 < nz) c -= T1u;
+      if (k > 0) c -= T2b;
+      J.c[0][ip] = (OT)c;
+      J.c[2][ip] = (OT)offE;
+      J.c[4][ip] = (OT)offN;
+      J.c[6][ip] = (OT)offU;
+      if constexpr (!SYMM) {
+        J.c[1][ip] = (OT)((i > 0) ? s_xw[ty * (TX + 1) + tx] : 0.0);
+        J.c[3][ip] = (OT)((j > 0) ? s_ys[ty * TX + tx] : 0.0);
+        J.c[5][ip] = (OT)((k > 0) ? T1b : 0.0);
+      }
+    }
+    // the z face above becomes the face below
+    T2b = T2u; T1b = T1u; syml_b = symlu;
+    p_c = p_n; S_c = S_n; dS_c = dS_n; kr_c = kr_n; dk_c = dk_n; kz_c = kz_n;
+  }
+}
+
+// Dirichlet faces (:1049-1226) and kinematic overland top rows
+// (:1547-1644/:1938-1987, PFSymmetric top rows of PL/pf_hypre.c:360-420) of
+// the shell cells, applied to the assembled rows in the reference's order.
+template <bool SYMM, typename OT>
+__global__ __launch_bounds__(256) void k_jacobian_bc(JacArgs A, JOut<OT> J, long nshell)
+{
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nshell) return;
+  int i, j, k;
   const Geom &g = A.g;
-  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= g.nx || j >= g.ny) return;
-  const int sy = g.sy, sz = g.sz;
-  int ip = (i + 1) + (j + 1) * sy + (k + 1) * sz;
+  if (!shell_cell(g, idx, i, j, k)) return;
+  const bool dir = (i == 0 && A.bct[0] == PFHIP_BC_DIR_EQUIL) ||
+                   (i == g.nx - 1 && A.bct[1] == PFHIP_BC_DIR_EQUIL) ||
+                   (j == 0 && A.bct[2] == PFHIP_BC_DIR_EQUIL) ||
+                   (j == g.ny - 1 && A.bct[3] == PFHIP_BC_DIR_EQUIL) ||
+                   (k == 0 && A.bct[4] == PFHIP_BC_DIR_EQUIL) ||
+                   (k == g.nz - 1 && A.bct[5] == PFHIP_BC_DIR_EQUIL);
+  const bool ovl = A.overland && k == g.nz - 1;
+  if (!dir && !ovl) return;
+  const int sy = g.sy;
+  const int ip = (i + 1) + (j + 1) * sy + (k + 1) * g.sz;
   const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
   const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
   const double vol = dx * dy * dz;
   const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
-  const double gc = grav;
   const double ddp = 0.0;
-  const double *__restrict__ p = A.p;
-  const double *__restrict__ kr = A.krho;
-  const double *__restrict__ dk = A.dkrho;
-  const bool symm = A.symm != 0;
-  double pc = p[ip];
-  double a = A.f.alpha[ip], nn = A.f.nvg[ip];
-  double S = vg_sat(pc, A.f.irg, a, nn, A.f.Sres(ip), A.f.Ssat(ip));
-  double dS = vg_dsat(pc, A.f.irg, a, nn, A.f.Sres(ip), A.f.Ssat(ip));
-  double vol2 = vol * 1.0;
-  double c = (dS * rho + S * ddp) * A.poro[ip] * vol2 +
-             A.ss[ip] * vol2 * (dS * rho * pc + S * ddp * pc + S * rho);
-  double offW = 0, offE = 0, offS = 0, offN = 0, offL = 0, offU = 0;
-  double krc = kr[ip], dkc = dk[ip];
-
-  if (i + 1 < g.nx) {
-    int b = ip + 1;
-    double diff = pc - p[b];
-    double updir = (diff / dx) * gc;
-    double xc = 1.0 * dt * ffx * (1.0 / dx) * 1.0 * hmean(A.kx[ip], A.kx[b]) / mu;
-    double sym = (-xc * upw(updir, 0.0, krc, kr[b])) * gc;
-    double west_temp = (-xc * diff * upw(updir, 0.0, dkc, 0.0)) * gc + sym;
-    c -= west_temp;
-    offE = symm ? sym : (xc * diff * upw(updir, 0.0, 0.0, dk[b])) * gc + sym;
-  }
-  if (i > 0) {
-    int aa = ip - 1;
-    double diff = p[aa] - pc;
-    double updir = (diff / dx) * gc;
-    double xc = 1.0 * dt * ffx * (1.0 / dx) * 1.0 * hmean(A.kx[aa], A.kx[ip]) / mu;
-    double sym = (-xc * upw(updir, 0.0, kr[aa], krc)) * gc;
-    double east_temp = (xc * diff * upw(updir, 0.0, 0.0, dkc)) * gc + sym;
-    c -= east_temp;
-    offW = symm ? sym : (-xc * diff * upw(updir, 0.0, dk[aa], 0.0)) * gc + sym;
-  }
-  if (j + 1 < g.ny) {
-    int b = ip + sy;
-    double diff = pc - p[b];
-    double updir = (diff / dy) * gc;
-    double yc = 1.0 * dt * ffy * (1.0 / dy) * 1.0 * hmean(A.ky[ip], A.ky[b]) / mu;
-    double sym_s = -yc * upw(updir, 0.0, krc, kr[b]) * gc;
-    double south_temp = -yc * diff * upw(updir, 0.0, dkc, 0.0) * gc + sym_s;
-    double sym_n = yc * -upw(updir, 0.0, krc, kr[b]) * gc;
-    c -= south_temp;
-    offN = symm ? sym_n : yc * diff * upw(updir, 0.0, 0.0, dk[b]) * gc + sym_n;
-  }
-  if (j > 0) {
-    int aa = ip - sy;
-    double diff = p[aa] - pc;
-    double updir = (diff / dy) * gc;
-    double yc = 1.0 * dt * ffy * (1.0 / dy) * 1.0 * hmean(A.ky[aa], A.ky[ip]) / mu;
-    double sym_s = -yc * upw(updir, 0.0, kr[aa], krc) * gc;
-    double sym_n = yc * -upw(updir, 0.0, kr[aa], krc) * gc;
-    double north_temp = yc * diff * upw(updir, 0.0, 0.0, dkc) * gc + sym_n;
-    c -= north_temp;
-    offS = symm ? sym_s : -yc * diff * upw(updir, 0.0, dk[aa], 0.0) * gc + sym_s;
-  }
-  const double sep = dz * 0.5 * (1.0 + 1.0);
-  if (k + 1 < g.nz) {
-    int b = ip + sz;
-    double lc = pc / sep - (1.0 / (1.0 + 1.0)) * rho * grav;
-    double uc = p[b] / sep + (1.0 / (1.0 + 1.0)) * rho * grav;
-    double diff = lc - uc;
-    double zc = 1.0 * dt * ffz * hmean_dz1(A.kz[ip], A.kz[b]) / mu;
-    double up = upw(lc, uc, krc, kr[b]);
-    double sym_l = -zc * (1.0 / (dz * 0.5 * (1.0 + 1.0))) * up;
-    double lower_temp = -zc * (diff * upw(lc, uc, dkc, 0.0) +
-                               (-grav * 0.5 * dz * (0.5 * (1.0 + 1.0)) * ddp * up)) +
-                        sym_l;
-    double sym_u = zc * (1.0 / (dz * 0.5 * (1.0 + 1.0))) * -up;
-    c -= lower_temp;
-    offU = symm ? sym_u
-                : zc * (diff * upw(lc, uc, 0.0, dk[b]) +
-                        (-grav * 0.5 * dz * (0.5 * (1.0 + 1.0)) * ddp * up)) +
-                      sym_u;
-  }
-  if (k > 0) {
-    int aa = ip - sz;
-    double lc = p[aa] / sep - (1.0 / (1.0 + 1.0)) * rho * grav;
-    double uc = pc / sep + (1.0 / (1.0 + 1.0)) * rho * grav;
-    double diff = lc - uc;
-    double zc = 1.0 * dt * ffz * hmean_dz1(A.kz[aa], A.kz[ip]) / mu;
-    double up = upw(lc, uc, kr[aa], krc);
-    double sym_l = -zc * (1.0 / (dz * 0.5 * (1.0 + 1.0))) * up;
-    double lower_temp = -zc * (diff * upw(lc, uc, dk[aa], 0.0) +
-                               (-grav * 0.5 * dz * (0.5 * (1.0 + 1.0)) * ddp * up)) +
-                        sym_l;
-    double sym_u = zc * (1.0 / (dz * 0.5 * (1.0 + 1.0))) * -up;
-    double upper_temp = zc * (diff * upw(lc, uc, 0.0, dkc) +
-                              (-grav * 0.5 * dz * (0.5 * (1.0 + 1.0)) * ddp * up)) +
-                        sym_u;
-    c -= upper_temp;
-    offL = symm ? sym_l : lower_temp;
-  }
-
-  bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
-             (k == g.nz - 1);
-  if (bnd) {
-#pragma unroll
+  double c = (double)J.c[0][ip];
+  const double pc = A.p[ip];
+  const double a = A.f.alpha[ip], nn = A.f.nvg[ip];
+  if (dir) {
+    double S, dS, kr, dkr;
+    vg_all(pc, A.f.irg, a, nn, A.f.Sres(ip), A.f.Ssat(ip), S, dS, kr, dkr);
+    const double krc = kr * rho, dkc = dkr * rho + krc * 0.0;
+#pragma unroll 1
     for (int fc = 0; fc < 6; fc++) {
       bool on = (fc == 0 && i == 0) || (fc == 1 && i == g.nx - 1) ||
                 (fc == 2 && j == 0) || (fc == 3 && j == g.ny - 1) ||
@@ -1857,18 +2167,19 @@ __global__ __launch_bounds__(256) void k_jacobian(JacArgs A)
       c -= o_temp;
     }
   }
-
-  if (A.overland && k == g.nz - 1) {
+  if (ovl) {
     int io = (i + 1) + (j + 1) * sy;
     double dvol = 0.0;
     if (pc > 0.0)
       dvol = (vol / dz) + (vol / ffy) * dt * (A.ke[io] - A.kw[io]) +
              (vol / ffx) * dt * (A.kn[io] - A.ks[io]);
     c += dvol;
-    if (symm) {
-      if (i + 1 < g.nx) offE = 0.0;
-      if (j + 1 < g.ny) offN = 0.0;
+    if (SYMM) {
+      if (i + 1 < g.nx) J.c[2][ip] = (OT)0.0;
+      if (j + 1 < g.ny) J.c[4][ip] = (OT)0.0;
     } else {
+      double offW = (double)J.c[1][ip], offE = (double)J.c[2][ip];
+      double offS = (double)J.c[3][ip], offN = (double)J.c[4][ip];
       offW -= (vol / ffy) * dt * (A.ke[io - 1]);
       offE += (vol / ffy) * dt * (A.kw[io + 1]);
       offS -= (vol / ffx) * dt * (A.kn[io - sy]);
@@ -1877,17 +2188,72 @@ __global__ __launch_bounds__(256) void k_jacobian(JacArgs A)
       if (i == g.nx - 1) offE = 0.0;
       if (j == 0) offS = 0.0;
       if (j == g.ny - 1) offN = 0.0;
+      J.c[1][ip] = (OT)offW; J.c[2][ip] = (OT)offE;
+      J.c[3][ip] = (OT)offS; J.c[4][ip] = (OT)offN;
     }
   }
-  A.J[0][ip] = c;
-  A.J[2][ip] = offE;
-  A.J[4][ip] = offN;
-  A.J[6][ip] = offU;
-  if (!symm) {
-    A.J[1][ip] = offW;
-    A.J[3][ip] = offS;
-    A.J[5][ip] = offL;
+  J.c[0][ip] = (OT)c;
+}
+
+template <bool SYMM, typename OT>
+static void launch_jacobian_pf(const JacPFArgs &P, const JOut<OT> &J, hipStream_t st)
+{
+  const Geom &g = P.a.g;
+  constexpr int TX = 64, TY = 8, ZC = 16;
+  dim3 grid((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC);
+  dim3 block(TX, TY, 1);
+  PROF_BEGIN(PFHIP_PROF_JACOBIAN, st);
+  if (P.a.f.sres || P.a.f.ssat)
+    hipLaunchKernelGGL((k_jacobian_pf<TX, TY, ZC, true, SYMM, OT>), grid, block, 0, st, P, J);
+  else
+    hipLaunchKernelGGL((k_jacobian_pf<TX, TY, ZC, false, SYMM, OT>), grid, block, 0, st, P, J);
+  PROF_END(PFHIP_PROF_JACOBIAN, st);
+  long ns = shell_count(g);
+  hipLaunchKernelGGL((k_jacobian_bc<SYMM, OT>), dim3((unsigned)((ns + 255) / 256)), dim3(256),
+                     0, st, P.a, J, ns);
+}
+
+// RichardsJacobianEval into coefficient arrays of type double (fp32 = 0) or
+// float (fp32 = 1); J[7] in stencil order C, W, E, S, N, L, U, with symm
+// only C, E, N, U used.
+int pfhip_jacobian_into(pfhip_problem *pb, const double *p, double dt, int symm,
+                        void *const *J, int fp32, void *stream)
+{
+  Geom g = geom_of(pb);
+  if (pb->overland) {
+    int rc = pfhip_overland_flow_eval_kin(pb, p, pb->q2[0], pb->q2[1], pb->q2[2],
+                                          pb->q2[3], PFHIP_CALCDER, stream);
+    if (rc) return rc;
   }
+  JacPFArgs P;
+  JacArgs &A = P.a;
+  A.g = g;
+  A.f = phys_of(pb);
+  A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
+  A.p = p;
+  A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
+  A.poro = pb->d.poro; A.ss = pb->d.ss;
+  A.krho = nullptr; A.dkrho = nullptr;
+  A.ke = pb->q2[0]; A.kw = pb->q2[1]; A.kn = pb->q2[2]; A.ks = pb->q2[3];
+  for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
+  A.symm = symm;
+  A.overland = pb->overland;
+  for (int s = 0; s < 7; s++) A.J[s] = nullptr;
+  P.idx = 1.0 / A.dx; P.idy = 1.0 / A.dy; P.idz = 1.0 / A.dz; P.imu = 1.0 / A.f.mu;
+  hipStream_t st = STREAM(stream);
+  if (fp32) {
+    JOut<float> O;
+    for (int s = 0; s < 7; s++) O.c[s] = (float *)J[s];
+    if (symm) launch_jacobian_pf<true, float>(P, O, st);
+    else launch_jacobian_pf<false, float>(P, O, st);
+  } else {
+    JOut<double> O;
+    for (int s = 0; s < 7; s++) O.c[s] = (double *)J[s];
+    if (symm) launch_jacobian_pf<true, double>(P, O, st);
+    else launch_jacobian_pf<false, double>(P, O, st);
+  }
+  LAUNCHCHK();
+  return PFHIP_OK;
 }
 
 extern "C" int pfhip_richards_jacobian_eval(pfhip_problem *pb, const double *p,
@@ -1899,32 +2265,9 @@ extern "C" int pfhip_richards_jacobian_eval(pfhip_problem *pb, const double *p,
     pfhip_set_error("jacobian_eval: missing coefficient arrays");
     return PFHIP_ERR_ARG;
   }
-  Geom g = geom_of(pb);
-  int rc = launch_closure(pb, 5, p, pb->krho, pb->dkrho, stream);
-  if (rc) return rc;
-  if (pb->overland) {
-    rc = pfhip_overland_flow_eval_kin(pb, p, pb->q2[0], pb->q2[1], pb->q2[2],
-                                      pb->q2[3], PFHIP_CALCDER, stream);
-    if (rc) return rc;
-  }
-  JacArgs A;
-  A.g = g;
-  A.f = phys_of(pb);
-  A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
-  A.p = p;
-  A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
-  A.poro = pb->d.poro; A.ss = pb->d.ss;
-  A.krho = pb->krho; A.dkrho = pb->dkrho;
-  A.ke = pb->q2[0]; A.kw = pb->q2[1]; A.kn = pb->q2[2]; A.ks = pb->q2[3];
-  for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
-  A.symm = symm_part;
-  A.overland = pb->overland;
-  for (int s = 0; s < 7; s++) A.J[s] = J[s];
-  PROF_BEGIN(PFHIP_PROF_JACOBIAN, stream);
-  hipLaunchKernelGGL(k_jacobian, grid3(g), BLOCK3, 0, STREAM(stream), A);
-  PROF_END(PFHIP_PROF_JACOBIAN, stream);
-  LAUNCHCHK();
-  return PFHIP_OK;
+  void *Jv[7];
+  for (int s = 0; s < 7; s++) Jv[s] = J[s];
+  return pfhip_jacobian_into(pb, p, dt, symm_part, Jv, 0, stream);
 }
 
 // --------------------------------------------------------------------------

@@ -35,7 +35,7 @@ struct pfhip_solver {
   long ng;
   double *unew, *fval, *pp, *vtemp1, *vtemp2, *xcor, *vtemp;
   double **V, **Z;
-  double *J[7], *P[7];
+  double *J[7];
   pfhip_mg *mg;
   double *p_old, *s_old;        /* advance_richards state                 */
   /* host Krylov state */
@@ -67,7 +67,7 @@ int pfhip_solver_destroy(pfhip_solver *s)
   for (int i = 0; i < 9; i++) pfhip_free(v[i]);
   if (s->V) for (int i = 0; i <= s->l_max; i++) pfhip_free(s->V[i]);
   if (s->Z) for (int i = 0; i < s->l_max; i++) pfhip_free(s->Z[i]);
-  for (int q = 0; q < 7; q++) { pfhip_free(s->J[q]); pfhip_free(s->P[q]); }
+  for (int q = 0; q < 7; q++) pfhip_free(s->J[q]);
   if (s->mg) pfhip_mg_destroy(s->mg);
   if (s->Hes) for (int i = 0; i <= s->l_max; i++) free(s->Hes[i]);
   free(s->Hes); free(s->givens); free(s->yg); free(s->hbuf);
@@ -103,11 +103,9 @@ int pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
     s->Z = (double **)calloc(s->l_max, sizeof(double *));
     for (int i = 0; i < s->l_max && !rc; i++) rc = alloc_vec(s, &s->Z[i]);
   }
-  for (int q = 0; q < 7 && !rc; q++) {
+  /* the PC matrix is assembled straight into the multigrid's finest level */
+  for (int q = 0; q < 7 && !rc; q++)
     if (opts->use_jacobian) rc = alloc_vec(s, &s->J[q]);
-    if (!rc && opts->precond && (q == 0 || q == 2 || q == 4 || q == 6))
-      rc = alloc_vec(s, &s->P[q]);
-  }
   if (!rc && opts->precond)
     rc = pfhip_mg_create(pb, opts->mg_max_levels, opts->mg_pre, opts->mg_post,
                          opts->mg_symmetric, opts->mg_precision ? opts->mg_precision : 64,
@@ -141,8 +139,7 @@ static int l2norm(pfhip_solver *s, const double *x, double *r)
 /* KINSolInitPC -> KinsolPCInitInstanceXtra: PC matrix + multigrid setup */
 static int pc_setup(pfhip_solver *s)
 {
-  CHK(pfhip_richards_jacobian_eval(s->pb, s->uu, s->dt, 1, s->P, NULL));
-  CHK(pfhip_mg_setup(s->mg, (const double *const *)s->P, NULL));
+  CHK(pfhip_mg_setup_jacobian(s->mg, s->uu, s->dt, NULL));
   s->npe++;
   s->nnilpre = s->nni;
   s->precondcurrent = 1;
@@ -177,6 +174,12 @@ static int atimes(pfhip_solver *s, const double *v, double *z)
   double a = fabs(sutsv);
   double sigma = sign * s->sqrt_relfunc * ((a > sq1norm) ? a : sq1norm) / vtv;
   double sigma_inv = 1.0 / sigma;
+  if (!pb->overland) {
+    /* u + sigma v, F and the quotient in one residual launch */
+    s->nfe++;
+    return pfhip_residual_jv(pb, s->uu, v, sigma, s->fval, sigma_inv, z, s->cur_p_old,
+                             s->cur_s_old, s->dt, NULL);
+  }
   CHK(pfhip_vec_linear_sum(pb, 1.0, s->uu, sigma, v, s->vtemp1, NULL));
   CHK(func(s, s->vtemp1, s->vtemp2));
   return pfhip_vec_linear_sum(pb, sigma_inv, s->vtemp2, -sigma_inv, s->fval, z, NULL);
@@ -283,8 +286,10 @@ static int qrsol(int n, double **h, double *q, double *b)
 #define SPGMR_QRFACT_FAIL 5
 
 /* SpgmrSolve (KS/spgmr.c:168-450), right preconditioning, s1 = s2 = 1;
- * with krylov = 1 the flexible variant keeps Z[l] = M^{-1} V[l]. */
-static int spgmr_solve(pfhip_solver *s, double *x, const double *b, double delta,
+ * with krylov = 1 the flexible variant keeps Z[l] = M^{-1} V[l].  The
+ * right-hand side is b = -fneg (the Newton system J p = -F): ||b|| is
+ * ||fneg|| and V[0] = b / ||b|| is formed in one pass. */
+static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double delta,
                        double *res_norm, int *nli, int *ret)
 {
   pfhip_problem *pb = s->pb;
@@ -294,17 +299,17 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *b, double delta
   double beta, rotation_product, r_norm, s_product, rho = 0;
   int krydim = 0, converged = 0;
   *nli = 0;
-  CHK(pfhip_vec_scale(pb, 1.0, b, V[0], NULL));
-  CHK(l2norm(s, V[0], &r_norm));
+  CHK(l2norm(s, fneg, &r_norm));
   *res_norm = beta = r_norm;
   if (r_norm <= delta) { *ret = SPGMR_SUCCESS; return PFHIP_OK; }
-  CHK(pfhip_vec_const(pb, 0.0, s->xcor, NULL));
+  if (!flex) CHK(pfhip_vec_const(pb, 0.0, s->xcor, NULL));
 
   for (int ntries = 0; ntries <= s->max_restarts; ntries++) {
     for (int i = 0; i <= l_max; i++)
       for (int j = 0; j < l_max; j++) Hes[i][j] = 0.0;
     rotation_product = 1.0;
-    CHK(pfhip_vec_scale(pb, 1.0 / r_norm, V[0], V[0], NULL));
+    if (ntries == 0) CHK(pfhip_vec_scale(pb, -1.0 / r_norm, fneg, V[0], NULL));
+    else CHK(pfhip_vec_scale(pb, 1.0 / r_norm, V[0], V[0], NULL));
     int l;
     for (l = 0; l < l_max; l++) {
       (*nli)++;
@@ -370,7 +375,8 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *b, double delta
   return PFHIP_OK;
 }
 
-/* KINSpgmrSolve KS/kinspgmr.c:344-420 ; *lret = KINSOL-level return */
+/* KINSpgmrSolve KS/kinspgmr.c:344-420 for J xx = -fval; bb receives J xx;
+ * *lret = KINSOL-level return */
 static int lsolve(pfhip_solver *s, double *xx, double *bb, int *lret)
 {
   pfhip_problem *pb = s->pb;
@@ -378,7 +384,7 @@ static int lsolve(pfhip_solver *s, double *xx, double *bb, int *lret)
   double res_norm;
   CHK(pfhip_vec_const(pb, 0.0, xx, NULL));
   s->new_uu = 1;
-  CHK(spgmr_solve(s, xx, bb, s->eps, &res_norm, &nli_inc, &sret));
+  CHK(spgmr_solve(s, xx, s->fval, s->eps, &res_norm, &nli_inc, &sret));
   s->nli += nli_inc;
   if (sret != 0) s->ncfl++;
   CHK(atimes(s, xx, bb));
@@ -396,7 +402,6 @@ static int linsoldrv(pfhip_solver *s, double *bb, double *xx, int *ret)
   for (;;) {
     s->precondcurrent = 0;
     if ((s->pthrsh > 1.5) && setupNonNull) CHK(pc_setup(s));
-    CHK(pfhip_vec_scale(s->pb, -1.0, s->fval, bb, NULL));
     int lret;
     CHK(lsolve(s, xx, bb, &lret));
     if (lret != 1) { *ret = lret; return PFHIP_OK; }

@@ -693,6 +693,8 @@ extern "C" int pfhip_mg_num_levels(pfhip_mg *mg, int *levels)
   return PFHIP_OK;
 }
 
+template <typename T> static int mg_coarse_setup_t(pfhip_mg *mg, hipStream_t st);
+
 template <typename T>
 static int mg_setup_t(pfhip_mg *mg, const double *const *Asym, hipStream_t st)
 {
@@ -703,6 +705,12 @@ static int mg_setup_t(pfhip_mg *mg, const double *const *Asym, hipStream_t st)
                        (T *)mg->A[0][q]);
     LAUNCHCHK();
   }
+  return mg_coarse_setup_t<T>(mg, st);
+}
+
+// coarse operators of every level from the finest-level matrix
+template <typename T> static int mg_coarse_setup_t(pfhip_mg *mg, hipStream_t st)
+{
   // large levels: grid kernels down to (and into) level lc
   for (int l = 0; l < mg->lc; l++) {
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
@@ -726,6 +734,19 @@ static int mg_setup_t(pfhip_mg *mg, const double *const *Asym, hipStream_t st)
     LAUNCHCHK();
   }
   return PFHIP_OK;
+}
+
+// PC setup of KinsolPC (PL/kinsol_pc.c:147-154) without an intermediate
+// matrix: the symmetric Jacobian of pressure p is assembled straight into
+// the finest level (in the level precision), then the coarse operators.
+int pfhip_mg_setup_jacobian(pfhip_mg *mg, const double *p, double dt, void *stream)
+{
+  void *J[7] = { mg->A[0][0], nullptr, mg->A[0][1], nullptr, mg->A[0][2], nullptr,
+                 mg->A[0][3] };
+  int rc = pfhip_jacobian_into(mg->pb, p, dt, 1, J, mg->prec == 32, stream);
+  if (rc) return rc;
+  return (mg->prec == 64) ? mg_coarse_setup_t<double>(mg, STREAM(stream))
+                          : mg_coarse_setup_t<float>(mg, STREAM(stream));
 }
 
 extern "C" int pfhip_mg_setup(pfhip_mg *mg, const double *const *Asym, void *stream)

@@ -57,6 +57,20 @@ int pfhip_vec_axpy_dot(pfhip_problem *pb, double hprev, const double *vprev,
                        double *w, const double *vnext, double *dot,
                        void *stream);
 
+/* difference-quotient J.v in one residual launch (no overland problems):
+ * z = sinv F(u + sigma v) - sinv fval */
+int pfhip_residual_jv(pfhip_problem *pb, const double *u, const double *v, double sigma,
+                      const double *fval, double sinv, double *z, const double *p_old,
+                      const double *s_old, double dt, void *stream);
+
+/* RichardsJacobianEval into double (fp32 = 0) or float (fp32 = 1)
+ * coefficient arrays J[7] (C, W, E, S, N, L, U; symm: C, E, N, U) */
+int pfhip_jacobian_into(pfhip_problem *pb, const double *p, double dt, int symm,
+                        void *const *J, int fp32, void *stream);
+/* multigrid PC setup from the pressure: symmetric Jacobian straight into the
+ * finest level, then the coarse operators */
+int pfhip_mg_setup_jacobian(pfhip_mg *mg, const double *p, double dt, void *stream);
+
 /* ---- device scratch helpers ---------------------------------------- */
 int pfhip_dev_alloc(double **p, long n);
 int pfhip_zero(double *p, long n, void *stream);

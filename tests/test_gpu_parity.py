@@ -131,6 +131,23 @@ def test_vector_reductions(gpu):
     assert gp.vec_dot(x, y) == gp.vec_dot(x, y)
 
 
+def test_water_balance_storage(gpu):
+    api = _api()
+    prob = cases.hetero(37, 21, 9, seed=27)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p = cases.random_pressure(prob, seed=28)
+    # pond part of the top layer
+    p[prob.nz][1:-1, 1:-1] = np.random.default_rng(29).uniform(-0.2, 0.3, (prob.ny, prob.nx))
+    s = O.saturation(op, p)
+    sub_o, sur_o = O.storage(op, p, s)
+    sub_g, sur_g = gp.water_balance_storage(p, s)
+    assert abs(sub_g - sub_o) < 1e-13 * abs(sub_o)
+    assert abs(sur_g - sur_o) < 1e-13 * max(abs(sur_o), 1e-300)
+    assert sur_o > 0.0
+    assert gp.water_balance_storage(p, s) == (sub_g, sur_g)
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 def test_multigrid(gpu, precision):
     api = _api()

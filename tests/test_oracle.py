@@ -271,3 +271,30 @@ def test_time_cycle_interval_numbers():
     for t in (0.05, 0.1, 0.15, 15.1, 15.15):
         assert O.lib().pfo_interval_number(op.ref, C.byref(op.c.bc[5]), C.c_double(t)) == \
             prob.interval_number(b, t)
+
+
+def test_storage_closed_form():
+    """Water-balance storage (hydrology.py:136-205) on fields whose sums are
+    known from counts: constant porosity 0.4 and Ss 1e-4, every cell dry
+    (p = -2, S = 0.3) except the top layer, where the first half of each row
+    is ponded (p = 0.5, S = 1) and the rest dry.  Dry cells add no
+    compressible storage and negative top pressure no surface storage."""
+    prob = cases.hetero(8, 6, 5, seed=3)
+    op = O.OracleProblem(prob)
+    p = np.full(prob.shape, -2.0)
+    s = np.full(prob.shape, 0.3)
+    top = p[prob.nz]                       # ghosted plane of the top layer
+    top[1:-1, 1:1 + prob.nx // 2] = 0.5
+    s[prob.nz][1:-1, 1:1 + prob.nx // 2] = 1.0
+    n_pond = prob.ny * (prob.nx // 2)
+    n_all = prob.nx * prob.ny * prob.nz
+    vol = prob.dx * prob.dy * prob.dz
+    sub, sur = O.storage(op, p, s)
+    sum_s = 0.3 * (n_all - n_pond) + 1.0 * n_pond
+    expect_sub = 0.4 * vol * sum_s + 0.5 * 1.0 * 1e-4 * vol * n_pond
+    assert abs(sub - expect_sub) < 1e-12 * expect_sub
+    assert abs(sur - 0.5 * prob.dx * prob.dy * n_pond) < 1e-12 * sur
+    # all dry: only the incompressible part remains, no surface water
+    sub2, sur2 = O.storage(op, np.full(prob.shape, -2.0), np.full(prob.shape, 0.3))
+    assert abs(sub2 - 0.4 * vol * 0.3 * n_all) < 1e-12 * sub2
+    assert sur2 == 0.0
