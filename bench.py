@@ -36,17 +36,61 @@ def _log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def residual_bytes_per_cell(prob, gp):
-    """Compulsory HBM bytes per cell of one fused residual launch: the
-    distinct input arrays it reads -- p, alpha, n, K (aliased Kx/Ky/Kz count
-    once), porosity, Ss, p_old, S_old, plus s_res/s_sat when they are
-    per-cell fields -- and F written.  Neighbour values come from LDS or
-    cache, so each array is counted once."""
+def _field_reads(gp):
+    """Distinct FP64 input fields of the closures: alpha, n, K (aliased
+    Kx/Ky/Kz count once), plus s_res/s_sat when they are per-cell fields."""
     d = gp.desc
     n_k = len({d.kx, d.ky, d.kz})
     n_vg = sum(1 for v in (d.sres, d.ssat) if v)
-    reads = 1 + 2 + n_k + 2 + 2 + n_vg
-    return 8 * (reads + 1)
+    return 2 + n_k + n_vg
+
+
+def kernel_bytes_per_cell(gp):
+    """Compulsory HBM bytes per interior cell of one launch of each measured
+    kernel: every array it reads or writes counted once (neighbour values
+    come from LDS or cache).
+      residual (F form):   p, alpha, n, K, porosity, Ss, p_old, S_old read,
+                           F written
+      residual (J.v form): u and v instead of p, and F(u) read in addition
+      Jacobian (PC matrix): p, alpha, n, K, porosity, Ss read; C, E, N, U
+                           written in the multigrid's FP32
+      smoother pair (finest level, FP32): C, E, N, U, b, x_in read, x_out
+                           written"""
+    f = _field_reads(gp)
+    res = 8 * (1 + f + 2 + 2 + 1)
+    return {
+        "residual": res,
+        "residual_jv": res + 16,
+        "jacobian": 8 * (1 + f + 2) + 4 * 4,
+        "rbgs_pair": 4 * 7,
+    }
+
+
+PROF_KERNELS = (
+    # (key, api prof id name, kernel as rocprofv3 names it)
+    ("residual", "PROF_RESIDUAL", "k_residual_pf<64, 8, 16, *, false> (NlFunctionEval)"),
+    ("residual_jv", "PROF_RESIDUAL_JV",
+     "k_residual_pf<64, 8, 16, *, true> (difference-quotient J.v)"),
+    ("jacobian", "PROF_JACOBIAN", "k_jacobian_pf<64, 8, 16, *, true, float> (PC matrix)"),
+    ("rbgs_pair", "PROF_RBGS", "k_rbgs_pair<float, 14, 16, *> (finest-level red/black pair)"),
+)
+
+
+def load_traffic():
+    """HBM traffic per launch from the committed rocprofv3 --pmc summary:
+    FETCH_SIZE and WRITE_SIZE of each kernel scaled by what the same counters
+    read for a plain copy kernel of known size."""
+    path = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    if not os.path.exists(path):
+        return {}
+    d = json.load(open(path))
+    cp = d["copy"]
+    fscale = cp["bytes_read"] / (cp["fetch_kib"] * 1024.0)
+    wscale = cp["bytes_written"] / (cp["write_kib"] * 1024.0)
+    out = {}
+    for key, v in d["kernels"].items():
+        out[key] = v["fetch_kib"] * 1024.0 * fscale + v["write_kib"] * 1024.0 * wscale
+    return out
 
 
 def run_gpu(args):
@@ -81,7 +125,8 @@ def run_gpu(args):
         dp, ds = gp.vec(p0), gp.vec()
         snap = [gp.vec() for _ in range(K)]
         if mode == "mixed":
-            api.check(L.pfhip_prof_enable(api.PROF_RESIDUAL, 200000), "prof")
+            for _, pid, _ in PROF_KERNELS:
+                api.check(L.pfhip_prof_enable(getattr(api, pid), 200000), "prof")
         newton, gmres, fevals = [], [], []
         t = 0.0
         api.check(L.pfhip_synchronize(), "sync")
@@ -98,12 +143,14 @@ def run_gpu(args):
         elapsed = time.perf_counter() - tic
         prof = None
         if mode == "mixed":
-            ms = api.C.c_double()
-            n = api.C.c_long()
-            api.check(L.pfhip_prof_read(api.PROF_RESIDUAL, api.C.byref(ms), api.C.byref(n)),
-                      "prof_read")
-            api.check(L.pfhip_prof_disable(api.PROF_RESIDUAL), "prof_off")
-            prof = (ms.value, n.value)
+            prof = {}
+            for key, pid, _ in PROF_KERNELS:
+                ms = api.C.c_double()
+                n = api.C.c_long()
+                api.check(L.pfhip_prof_read(getattr(api, pid), api.C.byref(ms),
+                                            api.C.byref(n)), "prof_read")
+                api.check(L.pfhip_prof_disable(getattr(api, pid)), "prof_off")
+                prof[key] = (ms.value, n.value)
         results[mode] = dict(elapsed=elapsed, newton=newton, gmres=gmres, fevals=fevals,
                              value=ncell * K / elapsed, prof=prof)
         snaps[mode] = snap
@@ -148,6 +195,45 @@ def run_cpu_baseline(args, cores):
                        f"{el:.1f}s, newton={log.newton[0]}, status={r}")
 
 
+def kernel_probe(args):
+    """Each measured kernel once on the workload's fields, for hardware
+    counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the residual in
+    both forms, the PC-matrix assembly into the FP32 multigrid, one V-cycle,
+    and a plain copy kernel (z = 1.0 x over the interior cells) that
+    calibrates the counters."""
+    import ctypes as C
+    from parflow_amd import api
+    from parflow_amd.decks import deck
+    L = api.lib()
+    prob, opts, p0, dt, _, _ = deck(args.config)
+    gp = api.GPUProblem(prob)
+    p = gp.vec(p0)
+    pold, sold, F, Z, v = gp.vec(p0), gp.vec(), gp.vec(), gp.vec(), gp.vec()
+    rng = np.random.default_rng(7)
+    vh = np.zeros(prob.shape)
+    vh[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+    v = gp.vec(vh)
+    api.check(L.pfhip_saturation(gp.h, pold.ptr, sold.ptr, 0, None), "S")
+    gp.bc_pressure(dt)
+    mg = api.GPUMultigrid(gp, opts.num_pre_relax, opts.num_post_relax, 0, 32)
+    L.pfhip_residual_jv.argtypes = [C.c_void_p] * 3 + [C.c_double, C.c_void_p, C.c_double] + \
+        [C.c_void_p] * 3 + [C.c_double, C.c_void_p]
+    L.pfhip_mg_setup_jacobian.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
+    L.pfhip_vec_scale.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+    for _ in range(2):   # the first round warms up; a counter run reads both
+        api.check(L.pfhip_nl_function_eval(gp.h, p.ptr, F.ptr, pold.ptr, sold.ptr,
+                                           C.c_double(dt), None), "F")
+        api.check(L.pfhip_residual_jv(gp.h, p.ptr, v.ptr, 1e-6, F.ptr, 1e6, Z.ptr, pold.ptr,
+                                      sold.ptr, dt, None), "Jv")
+        api.check(L.pfhip_mg_setup_jacobian(mg.h, p.ptr, dt, None), "PC")
+        api.check(L.pfhip_mg_apply(mg.h, F.ptr, Z.ptr, None), "V-cycle")
+        api.check(L.pfhip_vec_scale(gp.h, 1.0, F.ptr, Z.ptr, None), "copy")
+    api.check(L.pfhip_synchronize(), "sync")
+    print(json.dumps({"kernel_probe": args.config, "cells": prob.ncells,
+                      "copy_bytes_read": 8 * prob.ncells,
+                      "copy_bytes_written": 8 * prob.ncells}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,7 +243,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="run each measured kernel once (for counter runs) and exit")
     args = ap.parse_args()
+    if args.kernel_probe:
+        kernel_probe(args)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU domain decomposition is design-only).", file=sys.stderr)
@@ -166,11 +257,23 @@ def main():
     prob, gp, res, errs = run_gpu(args)
     mixed = res["mixed"]
     K = args.steps
-    ms_tot, nlaunch = mixed["prof"]
-    avg_s = (ms_tot / 1e3) / max(nlaunch, 1)
-    bpc = residual_bytes_per_cell(prob, gp)
-    bytes_per_launch = bpc * prob.ncells
-    achieved = bytes_per_launch / avg_s / 1e9
+    bpc_all = kernel_bytes_per_cell(gp)
+    traffic = load_traffic()
+    kernels = []
+    for key, _, kname in PROF_KERNELS:
+        ms_tot, nlaunch = mixed["prof"][key]
+        if nlaunch == 0:
+            continue
+        avg_s = (ms_tot / 1e3) / nlaunch
+        bpl = bpc_all[key] * prob.ncells
+        ach = bpl / avg_s / 1e9
+        kernels.append({"kernel": kname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                        "traffic": traffic.get(key), "bytes_per_cell": bpc_all[key],
+                        "avg_launch_us": avg_s * 1e6, "launches": nlaunch,
+                        "total_ms": ms_tot})
+    # the dominant kernel: most total time in the timed region
+    dom = max(kernels, key=lambda r: r["total_ms"])
     line = {
         "metric": "cell-timesteps/s (16.8M cells), mixed-precond vs FP64 plain at 1e-8 rel err",
         "value": mixed["value"],
@@ -192,18 +295,10 @@ def main():
             "gmres_per_step": mixed["gmres"],
             "fevals_per_step": mixed["fevals"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_residual_fused (NlFunctionEval)",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "bytes_per_cell": bpc,
-            "avg_launch_us": avg_s * 1e6,
-            "launches": nlaunch,
-        },
+        "roofline": {k: dom[k] for k in ("bound", "achieved", "peak", "unit", "frac",
+                                          "traffic", "kernel", "bytes_per_cell",
+                                          "avg_launch_us", "launches")},
+        "roofline_kernels": kernels,
     }
     if "plain" in res:
         plain = res["plain"]

@@ -1694,12 +1694,13 @@ static void launch_pf(const ResArgs &A, hipStream_t st)
   const Geom &g = A.g;
   dim3 grid((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC);
   dim3 block(TX, TY, 1);
-  PROF_BEGIN(PFHIP_PROF_RESIDUAL, st);
+  const int pid = JV ? PFHIP_PROF_RESIDUAL_JV : PFHIP_PROF_RESIDUAL;
+  PROF_BEGIN(pid, st);
   if (A.f.sres || A.f.ssat)
     hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, JV>), grid, block, 0, st, A);
   else
     hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, JV>), grid, block, 0, st, A);
-  PROF_END(PFHIP_PROF_RESIDUAL, st);
+  PROF_END(pid, st);
   long ns = shell_count(g);
   hipLaunchKernelGGL(k_residual_bc<JV>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
                      A, ns);
