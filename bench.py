@@ -242,7 +242,7 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=384)
     ap.add_argument("--kernel-probe", action="store_true",
                     help="run each measured kernel once (for counter runs) and exit")
     args = ap.parse_args()
