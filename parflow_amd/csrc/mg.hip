@@ -442,6 +442,166 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_rbgs_pair(LGeom g, Mat<T> A,
   }
 }
 
+// ---- fused pair, two points per lane ----------------------------------------
+// The same red -> black pair for levels with an even x extent, where every
+// lane owns the x-adjacent points (i0, i0+1), i0 even: each load fetches both
+// points as one 8/16-byte vector (the level arrays start one element past
+// an aligned address, so the ghosted index of an even i0 is aligned), the
+// x-neighbours come from the lanes beside through lane exchanges, the z
+// neighbours and the lower coupling are carried from plane to plane, so a
+// plane costs 9 vector loads per lane for 2 points instead of 14 per point
+// (the vector memory pipeline was the limiter).  A row of a tile is one
+// wavefront: lanes 1..62 own 124 points, lane 0's second and lane 63's
+// first point are the ring whose red values the tile recomputes.
+#define PX2 124
+
+template <typename T> struct V2T;
+template <> struct V2T<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct V2T<double> { typedef double type __attribute__((ext_vector_type(2))); };
+
+// lane l receives lane l-1's value (lane 0 its own), lane l+1's (lane 63
+// its own): bpermute lane exchanges within the wavefront
+template <typename T> __device__ __forceinline__ T from_prev(T v) { return __shfl_up(v, 1, 64); }
+template <typename T> __device__ __forceinline__ T from_next(T v) { return __shfl_down(v, 1, 64); }
+
+template <typename T> struct PairSet {
+  typedef typename V2T<T>::type T2;
+  T2 c, bb, ae, an, as, au, xs, xn;
+};
+
+template <typename T, bool INIT>
+__device__ __forceinline__ void load_pair(PairSet<T> &S, const Mat<T> &A,
+                                          const T *__restrict__ xin, const T *__restrict__ b,
+                                          unsigned ip, unsigned sy)
+{
+  typedef typename V2T<T>::type T2;
+  S.c = *(const T2 *)(A.c + ip);
+  S.bb = *(const T2 *)(b + ip);
+  S.ae = *(const T2 *)(A.u[0] + ip);
+  S.an = *(const T2 *)(A.u[1] + ip);
+  S.as = *(const T2 *)(A.u[1] + ip - sy);
+  S.au = *(const T2 *)(A.u[2] + ip);
+  if (INIT) {
+    S.xs = T2{T(0), T(0)};
+    S.xn = T2{T(0), T(0)};
+  } else {
+    S.xs = *(const T2 *)(xin + ip - sy);
+    S.xn = *(const T2 *)(xin + ip + sy);
+  }
+}
+
+template <typename T, bool INIT>
+__device__ __forceinline__ typename V2T<T>::type load_x2(const T *__restrict__ xin, unsigned ip)
+{
+  typedef typename V2T<T>::type T2;
+  if (INIT) return T2{T(0), T(0)};
+  return *(const T2 *)(xin + ip);
+}
+
+template <typename T, int TY, int KC, bool INIT>
+__global__ __launch_bounds__(64 * (TY + 2)) void k_rbgs_pair2(LGeom g, Mat<T> A,
+                                                              const T *__restrict__ xin,
+                                                              T *__restrict__ xout,
+                                                              const T *__restrict__ b)
+{
+  typedef typename V2T<T>::type T2;
+  __shared__ T red[3][TY + 2][64];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int gx = (g.n[0] + PX2 - 1) / PX2, gy = (g.n[1] + TY - 1) / TY;
+  const int gz = (g.n[2] + KC - 1) / KC;
+  const int nb = gx * gy * gz, nb8 = (nb + 7) & ~7;
+  const int bid = blockIdx.x;
+  const int Lt = (bid & 7) * (nb8 >> 3) + (bid >> 3);
+  if (Lt >= nb) return;                       // uniform over the workgroup
+  const int bx = Lt % gx, rr = Lt / gx;
+  const int by = rr % gy, bz = rr / gy;
+  const int i0 = bx * PX2 - 2 + 2 * tx;       // even; the lane's points are i0, i0+1
+  const int j = by * TY - 1 + ty;
+  const bool valid = j >= 0 && j < g.n[1] && i0 >= 0 && i0 < g.n[0];
+  const bool owned = valid && tx >= 1 && tx <= 62 && ty >= 1 && ty <= TY;
+  const int nz = g.n[2];
+  const int kb = bz * KC, ke = min(kb + KC, nz);
+  const unsigned sy = g.s[1], sz = g.s[2];
+  const bool hw0 = i0 > 0, he1 = i0 + 2 < g.n[0], hs = j > 0, hn = j + 1 < g.n[1];
+  const int ic = min(max(i0, 0), g.n[0] - 2), jc = min(max(j, 0), g.n[1] - 1);
+  const unsigned col = (unsigned)LIDX(g, ic, jc, 0) - sz;   // plane q at col + (q+1) sz
+  auto at = [&](int q) { return col + (unsigned)(min(max(q, 0), nz - 1) + 1) * sz; };
+  unsigned ip = valid ? (unsigned)LIDX(g, i0, j, kb - 1) : 0u;   // store address, plane k-1
+
+  // carried state
+  T sc = T(1), sb = T(0), saw = T(0), sae = T(0), sas = T(0), san = T(0), sal = T(0), sau = T(0);
+  T rm1 = T(0), rm2 = T(0);
+  PairSet<T> cur, nxt;
+  load_pair<T, INIT>(cur, A, xin, b, at(kb - 1), sy);
+  T2 al = *(const T2 *)(A.u[2] + at(kb - 2));        // lower coupling of plane kb-1
+  T2 xm = load_x2<T, INIT>(xin, at(kb - 2));
+  T2 x0 = load_x2<T, INIT>(xin, at(kb - 1));
+  T2 xp = load_x2<T, INIT>(xin, at(kb));
+  int it = 0;
+  for (int k = kb - 1; k <= ke; k++, it++) {
+    // prefetch: coefficients and y-neighbour x of plane k+1, x of plane k+2
+    load_pair<T, INIT>(nxt, A, xin, b, at(k + 1), sy);
+    const T2 xpp = load_x2<T, INIT>(xin, at(k + 2));
+    // ---- red at plane k: the lane's red point is 0 when (i0+j+k) is odd
+    const bool p0red = ((i0 + j + k) & 1) == 1;
+    const bool here = valid && k >= 0 && k < nz;
+    const T xw0 = from_prev(x0.y), xe1 = from_next(x0.x);
+    const T aw0 = from_prev(cur.ae.y);
+    T rr;
+    {
+      T c = p0red ? cur.c.x : cur.c.y, bb = p0red ? cur.bb.x : cur.bb.y;
+      if (INIT) {
+        rr = bb / c;
+      } else {
+        T aw = p0red ? aw0 : cur.ae.x, xw = p0red ? xw0 : x0.x;
+        bool hw = p0red ? hw0 : true;
+        T ae = p0red ? cur.ae.x : cur.ae.y, xe = p0red ? x0.y : xe1;
+        bool he = p0red ? true : he1;
+        T as = p0red ? cur.as.x : cur.as.y, xs = p0red ? cur.xs.x : cur.xs.y;
+        T an = p0red ? cur.an.x : cur.an.y, xn = p0red ? cur.xn.x : cur.xn.y;
+        T alv = p0red ? al.x : al.y, xl = p0red ? xm.x : xm.y;
+        T au = p0red ? cur.au.x : cur.au.y, xu = p0red ? xp.x : xp.y;
+        rr = gs_update(bb, c, aw, xw, hw, ae, xe, he, as, xs, hs, an, xn, hn, alv, xl, k > 0,
+                       au, xu, k + 1 < nz);
+      }
+    }
+    const T rk = here ? rr : T(0);
+    red[it % 3][ty][tx] = rk;
+    // ---- black at plane k-1: the lane's black point there is its red
+    // point of plane k; x neighbours are red values of plane k-1 (rm1 is
+    // the lane's own, the lane beside supplies the other)
+    const int kk = k - 1;
+    const T rprev = from_prev(rm1), rnext = from_next(rm1);
+    {
+      const T(*R)[64] = red[(it + 2) % 3];
+      const int tym = ty > 0 ? ty - 1 : 0, typ = ty < TY + 1 ? ty + 1 : TY + 1;
+      T xw = p0red ? rprev : rm1, xe = p0red ? rm1 : rnext;
+      bool hw = p0red ? hw0 : true, he = p0red ? true : he1;
+      T vb = gs_update(sb, sc, saw, xw, hw, sae, xe, he, sas, R[tym][tx], hs, san, R[typ][tx], hn,
+                       sal, rm2, kk > 0, sau, rk, kk + 1 < nz);
+      if (owned && kk >= kb && kk < ke) {
+        T2 out = p0red ? T2{vb, rm1} : T2{rm1, vb};
+        *(T2 *)(xout + ip - sz) = out;
+      }
+    }
+    // the black point of plane k keeps its coefficients for the next step
+    // (after the black update of plane k-1, which still needs the old set)
+    if (here) {
+      sc = p0red ? cur.c.y : cur.c.x;   sb = p0red ? cur.bb.y : cur.bb.x;
+      saw = p0red ? cur.ae.x : aw0;     sae = p0red ? cur.ae.y : cur.ae.x;
+      sas = p0red ? cur.as.y : cur.as.x; san = p0red ? cur.an.y : cur.an.x;
+      sal = p0red ? al.y : al.x;        sau = p0red ? cur.au.y : cur.au.x;
+    }
+    lds_barrier();
+    rm2 = rm1;
+    rm1 = rk;
+    al = cur.au;
+    cur = nxt;
+    xm = x0; x0 = xp; xp = xpp;
+    ip += sz;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_residual_l(LGeom g, Mat<T> A, const T *__restrict__ x,
                                                     const T *__restrict__ b, T *__restrict__ t)
@@ -566,6 +726,8 @@ struct pfhip_mg {
   void *A[MAXL][4];
   void *x[MAXL], *b[MAXL], *t[MAXL], *p1[MAXL], *p2[MAXL];
   void *xa[MAXL];              // second iterate buffer of the grid levels (l < lc)
+  void *raw[MAXL][10];         // allocations; each array starts one element in, so
+                               // the ghosted index of an even x is vector aligned
   int fuse;                    // red -> black pairs as one launch (k_rbgs_pair)
   void *levdev;                // device LevelDev<T>[L]
 };
@@ -659,9 +821,10 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
                         &mg->xa[l] };
     int nalloc = (l < mg->lc) ? 10 : 9;
     for (int q = 0; q < nalloc && rc == PFHIP_OK; q++) {
-      rc = pfhip_malloc(ptrs[q], (long)(s * es));
+      rc = pfhip_malloc(&mg->raw[l][q], (long)((s + 1) * es));
       if (rc == PFHIP_OK) {
-        hipError_t e = hipMemset(*ptrs[q], 0, s * es);
+        *ptrs[q] = (char *)mg->raw[l][q] + es;
+        hipError_t e = hipMemset(mg->raw[l][q], 0, (s + 1) * es);
         if (e != hipSuccess) { pfhip_set_error("hipMemset"); rc = PFHIP_ERR_HIP; }
       }
     }
@@ -678,9 +841,7 @@ extern "C" int pfhip_mg_destroy(pfhip_mg *mg)
 {
   if (!mg) return PFHIP_OK;
   for (int l = 0; l < mg->L; l++) {
-    for (int q = 0; q < 4; q++) if (mg->A[l][q]) hipFree(mg->A[l][q]);
-    void *o[6] = { mg->x[l], mg->b[l], mg->t[l], mg->p1[l], mg->p2[l], mg->xa[l] };
-    for (int q = 0; q < 6; q++) if (o[q]) hipFree(o[q]);
+    for (int q = 0; q < 10; q++) if (mg->raw[l][q]) hipFree(mg->raw[l][q]);
   }
   if (mg->levdev) hipFree(mg->levdev);
   free(mg);
@@ -807,10 +968,32 @@ static void launch_pair_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout
                        A, xin, xout, b);
 }
 
+template <typename T, int TY, int KC>
+static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
+                           int init, hipStream_t st)
+{
+  unsigned gx = (g.n[0] + PX2 - 1) / PX2, gy = (g.n[1] + TY - 1) / TY, gz = (g.n[2] + KC - 1) / KC;
+  unsigned nb = gx * gy * gz, nb8 = (nb + 7u) & ~7u;
+  if (init)
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true>), dim3(nb8), dim3(64, TY + 2, 1), 0, st,
+                       g, A, xin, xout, b);
+  else
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false>), dim3(nb8), dim3(64, TY + 2, 1), 0, st,
+                       g, A, xin, xout, b);
+}
+
 template <typename T>
 static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
                         int init, hipStream_t st)
 {
+  if ((g.n[0] % 2) == 0 && rbgs_variant() < 10) {
+    // two points per lane (aligned vector loads need an even x extent)
+    switch (rbgs_variant()) {
+      case 1: launch_pair2_t<T, 6, 16>(g, A, xin, xout, b, init, st); return;
+      case 2: launch_pair2_t<T, 6, 32>(g, A, xin, xout, b, init, st); return;
+      default: launch_pair2_t<T, 14, 16>(g, A, xin, xout, b, init, st); return;
+    }
+  }
   switch (rbgs_variant()) {
     case 1: launch_pair_t<T, 6, 16>(g, A, xin, xout, b, init, st); break;
     case 2: launch_pair_t<T, 6, 32>(g, A, xin, xout, b, init, st); break;

@@ -171,17 +171,20 @@ def test_multigrid(gpu, precision):
         assert _relmax(_I(prob, xg), _I(prob, xo)) < tol
 
 
+@pytest.mark.parametrize("shape", [(70, 40, 24), (130, 20, 18)])
 @pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("fuse", [0, 1])
-def test_multigrid_grid_levels(gpu, precision, fuse, monkeypatch):
-    """A hierarchy whose upper levels run as grid kernels (ragged 62 x 14 x 16
-    pair tiles and 64 x 4 tiles) and whose bottom runs in one workgroup,
-    with the red -> black pairs fused or not."""
+def test_multigrid_grid_levels(gpu, precision, fuse, shape, monkeypatch):
+    """A hierarchy whose upper levels run as grid kernels and whose bottom
+    runs in one workgroup, with the red -> black pairs fused or not: levels
+    of even x extent take the two-points-per-lane pair (ragged 124 x 14 x 16
+    tiles, one or two tiles across x), odd ones the one-point pair (62 x 14
+    x 16), and the 64 x 4 tiles of the other kernels are ragged too."""
     api = _api()
     monkeypatch.setenv("PFHIP_MG_SMALL", "300")
     monkeypatch.setenv("PFHIP_MG_FUSE", str(fuse))
-    prob = cases.hetero(70, 40, 24, seed=17, dir_sides=True)
-    print(f"mg grid levels precision={precision} fuse={fuse}", flush=True)
+    prob = cases.hetero(*shape, seed=17, dir_sides=True)
+    print(f"mg grid levels precision={precision} fuse={fuse} shape={shape}", flush=True)
     gp = api.GPUProblem(prob)
     op = O.OracleProblem(prob)
     p = cases.random_pressure(prob, seed=18)
