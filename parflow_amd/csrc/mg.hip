@@ -120,28 +120,36 @@ __device__ __forceinline__ T cpl(const Mat<T> &A, const LGeom &g, long ip, int q
 }
 
 // ---- per-point operations -------------------------------------------------
-// prolongation weights at fine (odd) points, PL/mg_semi.c:505-545
+// Coarse operator of one coarse point (SetupCoarseOps, PL/mg_semi.c:437-760)
+// in one pass: the operator-induced prolongation weights of the odd fine
+// points beside it (:505-545; the one above is stored for the transfers,
+// the one below is recomputed), the collapsed Galerkin couplings (:560-700)
+// and the centre completion (:720-750), whose lower-neighbour couplings are
+// recomputed from the fine operator instead of read back, with the
+// reference's operand order.
 template <typename T>
-__device__ __forceinline__ void pw_point(const LGeom &g, const Mat<T> &A, int d, int o1, int o2,
-                                         T *p1, T *p2, int i, int j, int k)
+__device__ __forceinline__ void pw_calc(const LGeom &g, const Mat<T> &A, int d, int o1, int o2,
+                                        long ip, T &w1, T &w2)
 {
-  int id[3] = { i, j, k };
-  if ((id[d] & 1) == 0) return;
-  long ip = LIDX(g, i, j, k);
   T a0 = A.c[ip];
   T a1 = cpl(A, g, ip, d, -1), a2 = cpl(A, g, ip, d, +1);
   T a3 = cpl(A, g, ip, o1, -1), a4 = cpl(A, g, ip, o1, +1);
   T a5 = cpl(A, g, ip, o2, -1), a6 = cpl(A, g, ip, o2, +1);
   T ap0 = a0 + a3 + a4 + a5 + a6;
-  if (ap0 != T(0)) { p1[ip] = -a1 / ap0; p2[ip] = -a2 / ap0; }
-  else { p1[ip] = T(0); p2[ip] = T(0); }
+  if (ap0 != T(0)) { w1 = -a1 / ap0; w2 = -a2 / ap0; }
+  else { w1 = T(0); w2 = T(0); }
 }
 
-// coarse operator at coarse point (i,j,k), PL/mg_semi.c:560-700
 template <typename T>
-__device__ __forceinline__ void cop_point(const LGeom &gf, const LGeom &gc, const Mat<T> &A,
-                                          const Mat<T> &Ac, int d, int o1, int o2,
-                                          const T *p1, const T *p2, int i, int j, int k)
+__device__ __forceinline__ T up_coupling(const Mat<T> &A, int q, long iA, long sd)
+{
+  return A.u[q][iA] + T(0.5) * A.u[q][iA - sd] + T(0.5) * A.u[q][iA + sd];
+}
+
+template <typename T>
+__device__ __forceinline__ void coarse_point(const LGeom &gf, const LGeom &gc, const Mat<T> &A,
+                                             const Mat<T> &Ac, int d, int o1, int o2, T *p1,
+                                             T *p2, int i, int j, int k)
 {
   int cid[3] = { i, j, k };
   int fi[3] = { i, j, k };
@@ -150,36 +158,38 @@ __device__ __forceinline__ void cop_point(const LGeom &gf, const LGeom &gc, cons
   long iAc = LIDX(gc, i, j, k);
   long sd = gf.s[d];
   long iA1 = iA - sd, iA2 = iA + sd;
-  T up1 = A.u[o1][iA] + T(0.5) * A.u[o1][iA1] + T(0.5) * A.u[o1][iA2];
-  T up2 = A.u[o2][iA] + T(0.5) * A.u[o2][iA1] + T(0.5) * A.u[o2][iA2];
-  Ac.u[o1][iAc] = (cid[o1] == gc.n[o1] - 1) ? T(0) : up1;
-  Ac.u[o2][iAc] = (cid[o2] == gc.n[o2] - 1) ? T(0) : up2;
+  // weights of the odd point below (owned by the coarse point below) and
+  // above (owned here); absent points have zero weights
+  T p2lo = T(0), p1hi = T(0);
+  if (fi[d] > 0) { T w1, w2; pw_calc(gf, A, d, o1, o2, iA1, w1, w2); p2lo = w2; }
+  if (fi[d] + 1 < gf.n[d]) {
+    T w1, w2;
+    pw_calc(gf, A, d, o1, o2, iA2, w1, w2);
+    p1[iA2] = w1; p2[iA2] = w2;
+    p1hi = w1;
+  }
+  T up1 = up_coupling(A, o1, iA, sd);
+  T up2 = up_coupling(A, o2, iA, sd);
+  T c4 = (cid[o1] == gc.n[o1] - 1) ? T(0) : up1;
+  T c6 = (cid[o2] == gc.n[o2] - 1) ? T(0) : up2;
+  Ac.u[o1][iAc] = c4;
+  Ac.u[o2][iAc] = c6;
   T upd;
-  if (cid[d] + 1 < gc.n[d]) upd = cpl(A, gf, iA + 2 * sd, d, -1) * p1[iA + sd];
+  if (cid[d] + 1 < gc.n[d]) upd = cpl(A, gf, iA + 2 * sd, d, -1) * p1hi;
   else upd = T(0);
   Ac.u[d][iAc] = upd;
   T a0 = A.c[iA];
   T a1 = cpl(A, gf, iA, d, -1), a2 = cpl(A, gf, iA, d, +1);
   T a3 = cpl(A, gf, iA, o1, -1), a4 = cpl(A, gf, iA, o1, +1);
   T a5 = cpl(A, gf, iA, o2, -1), a6 = cpl(A, gf, iA, o2, +1);
-  Ac.c[iAc] = a0 + a3 + a4 + a5 + a6 + a1 * p2[iA1] + a2 * p1[iA2];
+  T cc = a0 + a3 + a4 + a5 + a6 + a1 * p2lo + a2 * p1hi;
+  // lower neighbours' o1 / o2 couplings (never the last index, so unzeroed)
+  T c3 = (cid[o1] > 0) ? up_coupling(A, o1, iA - gf.s[o1], sd) : T(0);
+  T c5 = (cid[o2] > 0) ? up_coupling(A, o2, iA - gf.s[o2], sd) : T(0);
+  Ac.c[iAc] = cc - (c3 + c4 + c5 + c6);
 }
 
-// centre completion, PL/mg_semi.c:720-750
-template <typename T>
-__device__ __forceinline__ void cdiag_point(const LGeom &gc, const Mat<T> &Ac, int o1, int o2,
-                                            int i, int j, int k)
-{
-  int cid[3] = { i, j, k };
-  long iAc = LIDX(gc, i, j, k);
-  T c3 = (cid[o1] > 0) ? Ac.u[o1][iAc - gc.s[o1]] : T(0);
-  T c4 = Ac.u[o1][iAc];
-  T c5 = (cid[o2] > 0) ? Ac.u[o2][iAc - gc.s[o2]] : T(0);
-  T c6 = Ac.u[o2][iAc];
-  Ac.c[iAc] -= (c3 + c4 + c5 + c6);
-}
-
-// Gauss-Seidel point update x = (b - sum of present neighbour terms) / c,
+// Gauss-Seidel point update x = (b - sum of present neighbour terms) / c,// Gauss-Seidel point update x = (b - sum of present neighbour terms) / c,
 // terms summed west, east, south, north, lower, upper (PL/rb_GS_point.c:
 // 200-290); absent neighbours contribute exactly zero
 template <typename T>
@@ -274,27 +284,12 @@ __device__ __forceinline__ void prolong_point(const LGeom &gf, const LGeom &gc, 
 
 // ---- grid kernels (large levels) --------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_pweights(LGeom g, Mat<T> A, int d, int o1, int o2,
-                                                  T *__restrict__ p1, T *__restrict__ p2)
-{
-  XCD_IJK(g);
-  pw_point(g, A, d, o1, o2, p1, p2, i, j, k);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_coarse_op(LGeom gf, LGeom gc, Mat<T> A, Mat<T> Ac,
-                                                   int d, int o1, int o2, const T *p1,
-                                                   const T *p2)
+__global__ __launch_bounds__(256) void k_coarse_fused(LGeom gf, LGeom gc, Mat<T> A, Mat<T> Ac,
+                                                      int d, int o1, int o2, T *__restrict__ p1,
+                                                      T *__restrict__ p2)
 {
   XCD_IJK(gc);
-  cop_point(gf, gc, A, Ac, d, o1, o2, p1, p2, i, j, k);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_coarse_diag(LGeom gc, Mat<T> Ac, int o1, int o2)
-{
-  XCD_IJK(gc);
-  cdiag_point(gc, Ac, o1, o2, i, j, k);
+  coarse_point(gf, gc, A, Ac, d, o1, o2, p1, p2, i, j, k);
 }
 
 template <typename T>
@@ -711,9 +706,7 @@ __global__ __launch_bounds__(CWG) void k_coarse_setup(const LevelDev<T> *__restr
 {
   for (int l = ls; l <= L - 2; l++) {
     const LevelDev<T> &F = lev[l], &Cc = lev[l + 1];
-    FOR_POINTS(F.g, pw_point(F.g, F.A, F.dir, F.o1, F.o2, F.p1, F.p2, i, j, k));
-    FOR_POINTS(Cc.g, cop_point(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1, F.p2, i, j, k));
-    FOR_POINTS(Cc.g, cdiag_point(Cc.g, Cc.A, F.o1, F.o2, i, j, k));
+    FOR_POINTS(Cc.g, coarse_point(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1, F.p2, i, j, k));
   }
 }
 
@@ -877,15 +870,9 @@ template <typename T> static int mg_coarse_setup_t(pfhip_mg *mg, hipStream_t st)
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
     int d = mg->dir[l], o1, o2;
     offdirs(d, &o1, &o2);
-    hipLaunchKernelGGL(k_pweights<T>, dim3(ntiles(gf)), LBLOCK, 0, st, gf, matof<T>(mg, l), d,
-                       o1, o2, (T *)mg->p1[l], (T *)mg->p2[l]);
-    LAUNCHCHK();
-    hipLaunchKernelGGL(k_coarse_op<T>, dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc,
-                       matof<T>(mg, l), matof<T>(mg, l + 1), d, o1, o2,
-                       (const T *)mg->p1[l], (const T *)mg->p2[l]);
-    LAUNCHCHK();
-    hipLaunchKernelGGL(k_coarse_diag<T>, dim3(ntiles(gc)), LBLOCK, 0, st, gc,
-                       matof<T>(mg, l + 1), o1, o2);
+    hipLaunchKernelGGL(k_coarse_fused<T>, dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc,
+                       matof<T>(mg, l), matof<T>(mg, l + 1), d, o1, o2, (T *)mg->p1[l],
+                       (T *)mg->p2[l]);
     LAUNCHCHK();
   }
   // small levels: one workgroup
