@@ -52,6 +52,8 @@ def kernel_bytes_per_cell(gp):
       residual (F form):   p, alpha, n, K, porosity, Ss, p_old, S_old read,
                            F written
       residual (J.v form): u and v instead of p, and F(u) read in addition
+      residual (line-search form): u and the step instead of p, the trial
+                           point written in addition
       Jacobian (PC matrix): p, alpha, n, K, porosity, Ss read; C, E, N, U
                            written in the multigrid's FP32
       smoother pair (finest level, FP32): C, E, N, U, b, x_in read, x_out
@@ -61,6 +63,7 @@ def kernel_bytes_per_cell(gp):
     return {
         "residual": res,
         "residual_jv": res + 16,
+        "residual_step": res + 16,
         "jacobian": 8 * (1 + f + 2) + 4 * 4,
         "rbgs_pair": 4 * 7,
     }
@@ -68,9 +71,11 @@ def kernel_bytes_per_cell(gp):
 
 PROF_KERNELS = (
     # (key, api prof id name, kernel as rocprofv3 names it)
-    ("residual", "PROF_RESIDUAL", "k_residual_pf<64, 8, 16, *, false> (NlFunctionEval)"),
+    ("residual", "PROF_RESIDUAL", "k_residual_pf<64, 8, 16, *, 0> (NlFunctionEval)"),
     ("residual_jv", "PROF_RESIDUAL_JV",
-     "k_residual_pf<64, 8, 16, *, true> (difference-quotient J.v)"),
+     "k_residual_pf<64, 8, 16, *, 1> (difference-quotient J.v)"),
+    ("residual_step", "PROF_RESIDUAL_STEP",
+     "k_residual_pf<64, 8, 16, *, 2> (line-search trial point)"),
     ("jacobian", "PROF_JACOBIAN", "k_jacobian_pf<64, 8, 16, *, true, float> (PC matrix)"),
     ("rbgs_pair", "PROF_RBGS", "k_rbgs_pair<float, 14, 16, *> (finest-level red/black pair)"),
 )

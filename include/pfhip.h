@@ -212,7 +212,8 @@ int  pfhip_advance_richards(pfhip_solver *s, double *p, double *sat,
 #define PFHIP_PROF_JACOBIAN  2    /* RichardsJacobianEval assembly kernel   */
 #define PFHIP_PROF_RBGS      3    /* finest-level red/black smoothing launch */
 #define PFHIP_PROF_RESIDUAL_JV 4  /* residual kernel in the J.v quotient form */
-#define PFHIP_PROF_NKERNELS  5
+#define PFHIP_PROF_RESIDUAL_STEP 5 /* residual kernel at a line-search point */
+#define PFHIP_PROF_NKERNELS  6
 int  pfhip_prof_enable(int kernel_id, int max_launches);
 int  pfhip_prof_reset(int kernel_id);
 int  pfhip_prof_read(int kernel_id, double *total_ms, long *launches);

@@ -25,7 +25,8 @@ _dp = C.POINTER(C.c_double)
 MAXI = 8
 
 # kernel ids of pfhip_prof_* (include/pfhip.h)
-PROF_RESIDUAL, PROF_CLOSURE, PROF_JACOBIAN, PROF_RBGS, PROF_RESIDUAL_JV = 0, 1, 2, 3, 4
+(PROF_RESIDUAL, PROF_CLOSURE, PROF_JACOBIAN, PROF_RBGS, PROF_RESIDUAL_JV,
+ PROF_RESIDUAL_STEP) = 0, 1, 2, 3, 4, 5
 
 
 class PFHIPError(RuntimeError):

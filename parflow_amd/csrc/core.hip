@@ -540,6 +540,50 @@ extern "C" int pfhip_vec_dq_sums(pfhip_problem *pb, const double *u,
   return PFHIP_OK;
 }
 
+// KINStop's two tests in one pass (KS/kinsol.c:1530-1550 with uscale =
+// fscale = 1): max |fval| and max |unew - uu| / (1 + |unew|), the step being
+// formed on the fly with the rounding of the separate difference pass.
+__global__ __launch_bounds__(256) void k_stop_red(Geom g, const double *__restrict__ fval,
+                                                  const double *__restrict__ unew,
+                                                  const double *__restrict__ uu,
+                                                  double *__restrict__ partial, int nb)
+{
+  double a0 = 0.0, a1 = 0.0;
+  int rows = g.ny * g.nz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    int j = r % g.ny, k = r / g.ny;
+    int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+      int ip = base + i;
+      a0 = fmax(a0, fabs(fval[ip]));
+      double un = unew[ip];
+      double st = 1.0 * un + (-1.0) * uu[ip];
+      a1 = fmax(a1, fabs(st / (1.0 + fabs(un))));
+    }
+  }
+  a0 = block_reduce<true>(a0);
+  a1 = block_reduce<true>(a1);
+  if (threadIdx.x == 0) { partial[blockIdx.x] = a0; partial[nb + blockIdx.x] = a1; }
+}
+
+int pfhip_kin_stop_norms(pfhip_problem *pb, const double *fval, const double *unew,
+                         const double *uu, double *fmax, double *steplength, void *stream)
+{
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  hipLaunchKernelGGL(k_stop_red, dim3(nb), dim3(256), 0, STREAM(stream), g, fval, unew, uu,
+                     pb->red_partial, nb);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_reduce_final<true>, dim3(1), dim3(256), 0, STREAM(stream),
+                     pb->red_partial, nb, 2, pb->red_dev);
+  LAUNCHCHK();
+  int rc = fetch_results(pb, 2, stream);
+  if (rc) return rc;
+  *fmax = pb->h_red[0];
+  *steplength = pb->h_red[1];
+  return PFHIP_OK;
+}
+
 // ---- water-balance storage ---------------------------------------------
 // calculate_subsurface_storage / calculate_surface_storage
 // (pftools/python/parflow/tools/hydrology.py:136-205) summed over the box,
@@ -1121,10 +1165,12 @@ struct ResArgs {
   double dx, dy, dz, dt;
   double idx, idy, idz, imu;         // reciprocals of dx, dy, dz, mu
   const double *__restrict__ p;
-  // difference-quotient J.v (k_residual_pf JV form): the pressure is
-  // p + psig pv and the cell output fsc F - fsc fsub
+  // k_residual_pf forms: MODE 1 (difference-quotient J.v) evaluates at
+  // p + psig pv and writes fsc F - fsc fsub; MODE 2 (line-search step)
+  // evaluates at p + psig pv, writes F and the pressure itself to pout
   const double *__restrict__ pv;
   const double *__restrict__ fsub;
+  double *__restrict__ pout;
   double psig, fsc;
   const double *__restrict__ p_old;
   const double *__restrict__ s_old;
@@ -1447,17 +1493,17 @@ template <> struct VGIn<false> { double p, al, n, kz; };
 struct OthIn { double kx, kxe, ky, kyn, so, po, ss, pold, hk0, hk1, fs; };
 struct HalIn { double p, al, n; };
 
-// the pressure the residual is evaluated at: p, or p + psig pv (J.v form)
-template <bool JV> __device__ __forceinline__ double p_at(const ResArgs &A, int ip)
+// the pressure the residual is evaluated at: p, or p + psig pv
+template <int MODE> __device__ __forceinline__ double p_at(const ResArgs &A, int ip)
 {
-  if constexpr (JV) return A.p[ip] + A.psig * A.pv[ip];
+  if constexpr (MODE != 0) return A.p[ip] + A.psig * A.pv[ip];
   else return A.p[ip];
 }
 
-template <bool PCS, bool JV>
+template <bool PCS, int MODE>
 __device__ __forceinline__ void load_vg(VGIn<PCS> &v, const ResArgs &A, int ip)
 {
-  v.p = p_at<JV>(A, ip); v.al = A.f.alpha[ip]; v.n = A.f.nvg[ip]; v.kz = A.kz[ip];
+  v.p = p_at<MODE>(A, ip); v.al = A.f.alpha[ip]; v.n = A.f.nvg[ip]; v.kz = A.kz[ip];
   if constexpr (PCS) { v.sr = A.f.sres[ip]; v.st = A.f.ssat[ip]; }
 }
 template <bool PCS> __device__ __forceinline__ double vg_sr(const VGIn<PCS> &v, const Phys &f)
@@ -1468,22 +1514,22 @@ template <bool PCS> __device__ __forceinline__ double vg_st(const VGIn<PCS> &v, 
 {
   if constexpr (PCS) return v.st; else return f.ssat_v;
 }
-template <bool JV>
+template <int MODE>
 __device__ __forceinline__ void load_oth(OthIn &o, const ResArgs &A, int ip, int sy, int hip,
                                          const double *__restrict__ Kh, int hoff)
 {
   o.kx = A.kx[ip]; o.kxe = A.kx[ip + 1]; o.ky = A.ky[ip]; o.kyn = A.ky[ip + sy];
   o.hk0 = Kh[hip]; o.hk1 = Kh[hip + hoff];
   o.so = A.s_old[ip]; o.po = A.poro[ip]; o.ss = A.ss[ip]; o.pold = A.p_old[ip];
-  if constexpr (JV) o.fs = A.fsub[ip];
+  if constexpr (MODE == 1) o.fs = A.fsub[ip];
 }
-template <bool JV>
+template <int MODE>
 __device__ __forceinline__ void load_hal(HalIn &h, const ResArgs &A, int hip)
 {
-  h.p = p_at<JV>(A, hip); h.al = A.f.alpha[hip]; h.n = A.f.nvg[hip];
+  h.p = p_at<MODE>(A, hip); h.al = A.f.alpha[hip]; h.n = A.f.nvg[hip];
 }
 
-template <int TX, int TY, int ZC, bool PCS, bool JV>
+template <int TX, int TY, int ZC, bool PCS, int MODE>
 __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
 {
   constexpr int HALO = 2 * TX + 2 * TY;
@@ -1536,10 +1582,10 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
   double p_n = 0, kr_n = 0, S_n = 0, kz_n = 0;
   {
     VGIn<PCS> v0;
-    load_vg<PCS, JV>(v0, A, col + plane(k0));
+    load_vg<PCS, MODE>(v0, A, col + plane(k0));
     if (k0 > 0) {
       VGIn<PCS> vm;
-      load_vg<PCS, JV>(vm, A, col + plane(k0 - 1));
+      load_vg<PCS, MODE>(vm, A, col + plane(k0 - 1));
       p_m = vm.p;
       kr_m = vg_kr(p_m, irg, vm.al, vm.n) * rho;
       kz_m = vm.kz;
@@ -1555,19 +1601,19 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
 
   VGIn<PCS> vgN;
   HalIn hal;
-  load_vg<PCS, JV>(vgN, A, col + plane(k0 + 1));
-  load_hal<JV>(hal, A, hcol + plane(k0));
+  load_vg<PCS, MODE>(vgN, A, col + plane(k0 + 1));
+  load_hal<MODE>(hal, A, hcol + plane(k0));
 
   for (int k = k0; k < k1; k++) {
     const int ip = col + (k + 1) * sz;
     // this plane's flux and storage inputs (used after the closures), then
     // the prefetch of the next plane's closure inputs
     OthIn o;
-    load_oth<JV>(o, A, col + plane(k), sy, hcol + plane(k), Kh, hoff);
+    load_oth<MODE>(o, A, col + plane(k), sy, hcol + plane(k), Kh, hoff);
     const VGIn<PCS> vgU = vgN;
     const HalIn h = hal;
-    load_vg<PCS, JV>(vgN, A, col + plane(k + 2));
-    load_hal<JV>(hal, A, hcol + plane(k + 1));
+    load_vg<PCS, MODE>(vgN, A, col + plane(k + 2));
+    load_hal<MODE>(hal, A, hcol + plane(k + 1));
     // ---- A: closures of the own cell one plane ahead and of the halo cell
     if (k + 1 < nz) {
       p_n = vgU.p;
@@ -1605,12 +1651,13 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
       double ud = s_uy[ty * TX + tx];
       double fv = residual_cell(A, i, j, k, ip, p_c, kr_c, S_c, o.kx, o.ky, kz_c, ur, uf, uu,
                                 ul, ud, ub, o.so, o.po, o.ss, o.pold);   // + k_residual_bc
-      if constexpr (JV) {
+      if constexpr (MODE == 1) {
         // boundary cells get the transform after their face terms
         const bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
                          (k == nz - 1);
         fv = bnd ? fv : A.fsc * fv - A.fsc * o.fs;
       }
+      if constexpr (MODE == 2) A.pout[ip] = p_c;
       A.F[ip] = fv;
     }
     ub = uu;
@@ -1673,7 +1720,7 @@ static long shell_count(const Geom &g)
 
 // F += boundary-face terms (k_residual_pf leaves them out); in the J.v
 // form the output transform follows them
-template <bool JV>
+template <int MODE>
 __global__ __launch_bounds__(256) void k_residual_bc(ResArgs A, long nshell)
 {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1681,42 +1728,41 @@ __global__ __launch_bounds__(256) void k_residual_bc(ResArgs A, long nshell)
   int i, j, k;
   if (!shell_cell(A.g, idx, i, j, k)) return;
   const int ip = (i + 1) + (j + 1) * A.g.sy + (k + 1) * A.g.sz;
-  double pc = p_at<JV>(A, ip);
+  double pc = p_at<MODE>(A, ip);
   double krc = vg_kr(pc, A.f.irg, A.f.alpha[ip], A.f.nvg[ip]) * A.f.rho;
   double fv = residual_bc(A, i, j, k, ip, A.F[ip], pc, krc, A.kx[ip], A.ky[ip], A.kz[ip]);
-  if constexpr (JV) fv = A.fsc * fv - A.fsc * A.fsub[ip];
+  if constexpr (MODE == 1) fv = A.fsc * fv - A.fsc * A.fsub[ip];
   A.F[ip] = fv;
 }
 
-template <int TX, int TY, int ZC, bool JV>
+template <int TX, int TY, int ZC, int MODE>
 static void launch_pf(const ResArgs &A, hipStream_t st)
 {
   const Geom &g = A.g;
   dim3 grid((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC);
   dim3 block(TX, TY, 1);
-  const int pid = JV ? PFHIP_PROF_RESIDUAL_JV : PFHIP_PROF_RESIDUAL;
+  const int pid = (MODE == 1) ? PFHIP_PROF_RESIDUAL_JV
+                  : (MODE == 2) ? PFHIP_PROF_RESIDUAL_STEP : PFHIP_PROF_RESIDUAL;
   PROF_BEGIN(pid, st);
   if (A.f.sres || A.f.ssat)
-    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, JV>), grid, block, 0, st, A);
+    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, MODE>), grid, block, 0, st, A);
   else
-    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, JV>), grid, block, 0, st, A);
+    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, MODE>), grid, block, 0, st, A);
   PROF_END(pid, st);
   long ns = shell_count(g);
-  hipLaunchKernelGGL(k_residual_bc<JV>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
-                     A, ns);
+  hipLaunchKernelGGL(k_residual_bc<MODE>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
+                     st, A, ns);
 }
 
 static void launch_residual(const ResArgs &A, hipStream_t st)
 {
   const Geom &g = A.g;
   int v = residual_variant();
-  const bool jv = A.pv != nullptr;
-  if (v == 12) {
-    if (jv) launch_pf<64, 8, 32, true>(A, st); else launch_pf<64, 8, 32, false>(A, st);
-    return;
-  }
-  if (v != 5 || jv) {
-    if (jv) launch_pf<64, 8, 16, true>(A, st); else launch_pf<64, 8, 16, false>(A, st);
+  const int mode = (A.pv == nullptr) ? 0 : (A.fsub != nullptr) ? 1 : 2;
+  if (v != 5 || mode != 0) {
+    if (mode == 1) launch_pf<64, 8, 16, 1>(A, st);
+    else if (mode == 2) launch_pf<64, 8, 16, 2>(A, st);
+    else launch_pf<64, 8, 16, 0>(A, st);
     return;
   }
   v = 0;   // 5: the unpipelined kernel, 64 x 8 x 16
@@ -1756,7 +1802,35 @@ extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double
   A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
   for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
   A.F = F;
-  A.pv = nullptr; A.fsub = nullptr; A.psig = 0.0; A.fsc = 0.0;
+  A.pv = nullptr; A.fsub = nullptr; A.pout = nullptr; A.psig = 0.0; A.fsc = 0.0;
+  launch_residual(A, STREAM(stream));
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// Line-search trial point of KINLineSearch (KS/kinsol.c:1228-1405): unew =
+// uu + rl pp formed inside the residual launch (written to unew) and F at it.
+int pfhip_residual_step(pfhip_problem *pb, const double *uu, const double *pp, double rl,
+                        double *unew, double *F, const double *p_old, const double *s_old,
+                        double dt, void *stream)
+{
+  if (pb->overland) {
+    pfhip_set_error("residual_step: overland problems take the two-pass form");
+    return PFHIP_ERR_ARG;
+  }
+  Geom g = geom_of(pb);
+  ResArgs A;
+  A.g = g;
+  A.f = phys_of(pb);
+  A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
+  A.idx = 1.0 / A.dx; A.idy = 1.0 / A.dy; A.idz = 1.0 / A.dz; A.imu = 1.0 / A.f.mu;
+  A.p = uu; A.p_old = p_old; A.s_old = s_old;
+  A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
+  A.poro = pb->d.poro; A.ss = pb->d.ss; A.src = pb->d.src;
+  A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
+  for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
+  A.F = F;
+  A.pv = pp; A.fsub = nullptr; A.pout = unew; A.psig = rl; A.fsc = 0.0;
   launch_residual(A, STREAM(stream));
   LAUNCHCHK();
   return PFHIP_OK;
@@ -1786,7 +1860,7 @@ int pfhip_residual_jv(pfhip_problem *pb, const double *u, const double *v, doubl
   A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
   for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
   A.F = z;
-  A.pv = v; A.fsub = fval; A.psig = sigma; A.fsc = sinv;
+  A.pv = v; A.fsub = fval; A.pout = nullptr; A.psig = sigma; A.fsc = sinv;
   launch_residual(A, STREAM(stream));
   LAUNCHCHK();
   return PFHIP_OK;

@@ -412,8 +412,15 @@ static int linsoldrv(pfhip_solver *s, double *bb, double *xx, int *ret)
 
 static int eval_unew(pfhip_solver *s, double rl, double *fnormp, double *f1normp)
 {
-  CHK(pfhip_vec_linear_sum(s->pb, 1.0, s->uu, rl, s->pp, s->unew, NULL));
-  CHK(func(s, s->unew, s->fval));
+  if (!s->pb->overland) {
+    /* unew = uu + rl pp formed inside the residual launch */
+    s->nfe++;
+    CHK(pfhip_residual_step(s->pb, s->uu, s->pp, rl, s->unew, s->fval, s->cur_p_old,
+                            s->cur_s_old, s->dt, NULL));
+  } else {
+    CHK(pfhip_vec_linear_sum(s->pb, 1.0, s->uu, rl, s->pp, s->unew, NULL));
+    CHK(func(s, s->unew, s->fval));
+  }
   CHK(l2norm(s, s->fval, fnormp));
   *f1normp = 0.5 * (*fnormp) * (*fnormp);
   return PFHIP_OK;
@@ -550,10 +557,9 @@ static int kin_stop(pfhip_solver *s, int maxStepTaken, int gret, int *ret)
     return PFHIP_OK;
   }
   double fmax, rlength;
-  CHK(pfhip_vec_max_norm(s->pb, s->fval, &fmax, NULL));
+  /* max|fval| and the scaled step length from one pass */
+  CHK(pfhip_kin_stop_norms(s->pb, s->fval, s->unew, s->uu, &fmax, &rlength, NULL));
   if (fmax <= s->fnormtol) { *ret = 1; return PFHIP_OK; }
-  CHK(pfhip_vec_linear_sum(s->pb, 1.0, s->unew, -1.0, s->uu, s->vtemp1, NULL));
-  CHK(pfhip_vec_scsteplength(s->pb, s->unew, s->vtemp1, &rlength, NULL));
   if (rlength <= s->scsteptol) {
     if (!s->precondcurrent) { s->pthrsh = 2.0; return PFHIP_OK; }
     *ret = 2;

@@ -71,6 +71,15 @@ int pfhip_jacobian_into(pfhip_problem *pb, const double *p, double dt, int symm,
  * finest level, then the coarse operators */
 int pfhip_mg_setup_jacobian(pfhip_mg *mg, const double *p, double dt, void *stream);
 
+/* line-search trial point in one residual launch (no overland problems):
+ * unew = uu + rl pp, F = F(unew) */
+int pfhip_residual_step(pfhip_problem *pb, const double *uu, const double *pp, double rl,
+                        double *unew, double *F, const double *p_old, const double *s_old,
+                        double dt, void *stream);
+/* KINStop norms in one pass: max|fval| and max|unew-uu|/(1+|unew|) */
+int pfhip_kin_stop_norms(pfhip_problem *pb, const double *fval, const double *unew,
+                         const double *uu, double *fmax, double *steplength, void *stream);
+
 /* ---- device scratch helpers ---------------------------------------- */
 int pfhip_dev_alloc(double **p, long n);
 int pfhip_zero(double *p, long n, void *stream);
