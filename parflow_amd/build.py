@@ -21,7 +21,8 @@ C_SOURCES = ["kinsol_host.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
-             # no packed FP32 instructions (see DESIGN.md, MI355X notes)
+             # no packed FP32 instructions: the kernels are written for the
+             # scalar FP32/FP64 pipes and must not be rewritten into them
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wno-unused-function",

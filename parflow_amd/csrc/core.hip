@@ -1493,18 +1493,27 @@ template <> struct VGIn<false> { double p, al, n, kz; };
 struct OthIn { double kx, kxe, ky, kyn, so, po, ss, pold, hk0, hk1, fs; };
 struct HalIn { double p, al, n; };
 
+// load through a 32-bit byte offset from the array base (every ghosted
+// array is far below 4 GiB): the load then takes the base from scalar
+// registers and one shared vector offset instead of a 64-bit address each
+__device__ __forceinline__ double ldb(const double *__restrict__ base, int ip)
+{
+  return *(const double *)((const char *)base + ((unsigned)ip << 3));
+}
+
 // the pressure the residual is evaluated at: p, or p + psig pv
 template <int MODE> __device__ __forceinline__ double p_at(const ResArgs &A, int ip)
 {
-  if constexpr (MODE != 0) return A.p[ip] + A.psig * A.pv[ip];
-  else return A.p[ip];
+  if constexpr (MODE != 0) return ldb(A.p, ip) + A.psig * ldb(A.pv, ip);
+  else return ldb(A.p, ip);
 }
 
 template <bool PCS, int MODE>
 __device__ __forceinline__ void load_vg(VGIn<PCS> &v, const ResArgs &A, int ip)
 {
-  v.p = p_at<MODE>(A, ip); v.al = A.f.alpha[ip]; v.n = A.f.nvg[ip]; v.kz = A.kz[ip];
-  if constexpr (PCS) { v.sr = A.f.sres[ip]; v.st = A.f.ssat[ip]; }
+  v.p = p_at<MODE>(A, ip); v.al = ldb(A.f.alpha, ip); v.n = ldb(A.f.nvg, ip);
+  v.kz = ldb(A.kz, ip);
+  if constexpr (PCS) { v.sr = ldb(A.f.sres, ip); v.st = ldb(A.f.ssat, ip); }
 }
 template <bool PCS> __device__ __forceinline__ double vg_sr(const VGIn<PCS> &v, const Phys &f)
 {
@@ -1514,22 +1523,27 @@ template <bool PCS> __device__ __forceinline__ double vg_st(const VGIn<PCS> &v, 
 {
   if constexpr (PCS) return v.st; else return f.ssat_v;
 }
-template <int MODE>
+// ISO: Kx, Ky and Kz are one array (isotropic permeability); the cell's own
+// value is then the Kz already carried for the z face
+template <int MODE, bool ISO>
 __device__ __forceinline__ void load_oth(OthIn &o, const ResArgs &A, int ip, int sy, int hip,
                                          const double *__restrict__ Kh, int hoff)
 {
-  o.kx = A.kx[ip]; o.kxe = A.kx[ip + 1]; o.ky = A.ky[ip]; o.kyn = A.ky[ip + sy];
-  o.hk0 = Kh[hip]; o.hk1 = Kh[hip + hoff];
-  o.so = A.s_old[ip]; o.po = A.poro[ip]; o.ss = A.ss[ip]; o.pold = A.p_old[ip];
-  if constexpr (MODE == 1) o.fs = A.fsub[ip];
+  if constexpr (!ISO) { o.kx = ldb(A.kx, ip); o.ky = ldb(A.ky, ip); }
+  o.kxe = ldb(A.kx, ip + 1);
+  o.kyn = ldb(A.ky, ip + sy);
+  o.hk0 = ldb(Kh, hip); o.hk1 = ldb(Kh, hip + hoff);
+  o.so = ldb(A.s_old, ip); o.po = ldb(A.poro, ip); o.ss = ldb(A.ss, ip);
+  o.pold = ldb(A.p_old, ip);
+  if constexpr (MODE == 1) o.fs = ldb(A.fsub, ip);
 }
 template <int MODE>
 __device__ __forceinline__ void load_hal(HalIn &h, const ResArgs &A, int hip)
 {
-  h.p = p_at<MODE>(A, hip); h.al = A.f.alpha[hip]; h.n = A.f.nvg[hip];
+  h.p = p_at<MODE>(A, hip); h.al = ldb(A.f.alpha, hip); h.n = ldb(A.f.nvg, hip);
 }
 
-template <int TX, int TY, int ZC, bool PCS, int MODE>
+template <int TX, int TY, int ZC, bool PCS, int MODE, bool ISO>
 __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
 {
   constexpr int HALO = 2 * TX + 2 * TY;
@@ -1609,7 +1623,8 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
     // this plane's flux and storage inputs (used after the closures), then
     // the prefetch of the next plane's closure inputs
     OthIn o;
-    load_oth<MODE>(o, A, col + plane(k), sy, hcol + plane(k), Kh, hoff);
+    load_oth<MODE, ISO>(o, A, col + plane(k), sy, hcol + plane(k), Kh, hoff);
+    if constexpr (ISO) { o.kx = kz_c; o.ky = kz_c; }
     const VGIn<PCS> vgU = vgN;
     const HalIn h = hal;
     load_vg<PCS, MODE>(vgN, A, col + plane(k + 2));
@@ -1744,10 +1759,14 @@ static void launch_pf(const ResArgs &A, hipStream_t st)
   const int pid = (MODE == 1) ? PFHIP_PROF_RESIDUAL_JV
                   : (MODE == 2) ? PFHIP_PROF_RESIDUAL_STEP : PFHIP_PROF_RESIDUAL;
   PROF_BEGIN(pid, st);
-  if (A.f.sres || A.f.ssat)
-    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, MODE>), grid, block, 0, st, A);
-  else
-    hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, MODE>), grid, block, 0, st, A);
+  const bool iso = (A.kx == A.ky) && (A.kx == A.kz);
+  if (A.f.sres || A.f.ssat) {
+    if (iso) hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, MODE, true>), grid, block, 0, st, A);
+    else hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, MODE, false>), grid, block, 0, st, A);
+  } else {
+    if (iso) hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, MODE, true>), grid, block, 0, st, A);
+    else hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, MODE, false>), grid, block, 0, st, A);
+  }
   PROF_END(pid, st);
   long ns = shell_count(g);
   hipLaunchKernelGGL(k_residual_bc<MODE>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
