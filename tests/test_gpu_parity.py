@@ -131,6 +131,50 @@ def test_vector_reductions(gpu):
     assert gp.vec_dot(x, y) == gp.vec_dot(x, y)
 
 
+@pytest.mark.parametrize("m", [1, 7, 16, 31])
+def test_block_gram_schmidt_products(gpu, m):
+    """CGS2 block products: h = V^T w on the FP64 matrix cores and
+    w -= V h with the fused ||w||^2, against numpy on the interior cells
+    (ragged nx, ghosts poisoned)."""
+    import ctypes as C
+    api = _api()
+    L = api.lib()
+    prob = cases.hetero(37, 21, 9, seed=3)
+    gp = api.GPUProblem(prob)
+    rng = np.random.default_rng(100 + m)
+
+    def field():
+        a = np.full(prob.shape, 1e9)
+        a[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+        return a
+    Vh = [field() for _ in range(m)]
+    wh = field()
+    V = [gp.vec(a) for a in Vh]
+    w = gp.vec(wh)
+    ptrs = (C.c_void_p * m)(*[v.value for v in V])
+    L.pfhip_vec_multi_dot.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]
+    L.pfhip_vec_multi_axpy.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p]
+    h = (C.c_double * m)()
+    api.check(L.pfhip_vec_multi_dot(gp.h, ptrs, m, w.ptr, h, None), "multi_dot")
+    Wi = _I(prob, wh)
+    for q in range(m):
+        ref = float(np.sum(_I(prob, Vh[q]) * Wi))
+        scale = float(np.sum(np.abs(_I(prob, Vh[q]) * Wi)))
+        assert abs(h[q] - ref) <= 1e-13 * scale, (q, h[q], ref)
+    h2 = list(h)
+    again = (C.c_double * m)()
+    api.check(L.pfhip_vec_multi_dot(gp.h, ptrs, m, w.ptr, again, None), "multi_dot")
+    assert list(again) == h2          # fixed reduction order: identical bits
+    n2 = C.c_double()
+    api.check(L.pfhip_vec_multi_axpy(gp.h, ptrs, m, h, w.ptr, C.byref(n2), None), "multi_axpy")
+    wref = Wi - sum(h2[q] * _I(prob, Vh[q]) for q in range(m))
+    got = _I(prob, gp.get(w))
+    assert np.max(np.abs(got - wref)) <= 1e-12 * np.max(np.abs(wref))
+    assert abs(n2.value - float(np.sum(wref * wref))) <= 1e-12 * float(np.sum(wref * wref))
+
+
 def test_water_balance_storage(gpu):
     api = _api()
     prob = cases.hetero(37, 21, 9, seed=27)
