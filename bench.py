@@ -11,6 +11,10 @@ throughput, the per-step pressure error of the mixed solve against it, and
 the Newton/GMRES iteration counts.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
+    python bench.py --config c4 [--members 512] [--ensemble-stride 16]
+
+With --config c4 it advances the hillslope ensemble with batched launches
+and a sample of its members one at a time, and reports both rates.
 
 Warmup runs W steps on a scratch copy of the state; the timed region is
 then exactly the workload's first K steps from its initial condition, with
@@ -239,10 +243,104 @@ def kernel_probe(args):
                       "copy_bytes_written": 8 * prob.ncells}), flush=True)
 
 
+def run_ensemble(args):
+    """The hillslope ensemble (deck c4): every member advanced together with
+    batched launches, then a sample of the same members advanced one at a
+    time on the single-problem path; prints one JSON line."""
+    from parflow_amd import api
+    from parflow_amd.decks import c4_members
+
+    L = api.lib()
+    K = args.steps if args.steps is not None else 24
+    W = args.warmup
+    tic = time.perf_counter()
+    members = c4_members(args.members)
+    ncells = sum(p.ncells for p, _, _, _ in members)
+    _log(f"c4: {len(members)} members, {ncells} cells, generated in "
+         f"{time.perf_counter() - tic:.1f}s")
+    tic = time.perf_counter()
+    ens = api.GPUEnsemble([(p, o) for p, o, _, _ in members])
+    _log(f"ensemble created in {time.perf_counter() - tic:.1f}s")
+    dts = [m[3] for m in members]
+    if W > 0:
+        dps = [ens.problems[i].vec(m[2]) for i, m in enumerate(members)]
+        dss = [ens.problems[i].vec() for i in range(len(members))]
+        ens.advance_device(dps, dss, 0.0, dts, W)
+        del dps, dss
+    dps = [ens.problems[i].vec(m[2]) for i, m in enumerate(members)]
+    dss = [ens.problems[i].vec() for i in range(len(members))]
+    api.check(L.pfhip_synchronize(), "sync")
+    tic = time.perf_counter()
+    lg = ens.advance_device(dps, dss, 0.0, dts, K)
+    api.check(L.pfhip_synchronize(), "sync")
+    el = time.perf_counter() - tic
+    rounds, launches = ens.stats()
+    value = ncells * K / el
+    _log(f"ensemble: {K} steps in {el:.3f}s = {value:.4e} cell-steps/s, {rounds} rounds, "
+         f"{launches} launches, newton/step {lg['newton'].mean():.2f}")
+    # one at a time: every stride-th member on the single-problem mixed path
+    sample = members[::args.ensemble_stride]
+    solvers = [(api.GPUProblem(p), o, p0, dt) for p, o, p0, dt in sample]
+    solvers = [(gp, api.GPUSolver(gp, o, precision=32, krylov=1), p0, dt)
+               for gp, o, p0, dt in solvers]
+    scells = sum(gp.prob.ncells for gp, _, _, _ in solvers)
+    seq = 0.0
+    snewton = []
+    for gp, sv, p0, dt in solvers:
+        dp, ds = gp.vec(p0), gp.vec()
+        api.check(L.pfhip_synchronize(), "sync")
+        t1 = time.perf_counter()
+        lg1 = sv.advance_device(dp, ds, 0.0, dt, K)
+        api.check(L.pfhip_synchronize(), "sync")
+        seq += time.perf_counter() - t1
+        snewton.append(float(np.mean(lg1["newton"])))
+    seq_value = scells * K / seq
+    _log(f"one at a time: {len(solvers)} members, {K} steps in {seq:.3f}s = "
+         f"{seq_value:.4e} cell-steps/s")
+    line = {
+        "metric": "cell-timesteps/s (512-member hillslope ensemble, batched)",
+        "value": value,
+        "unit": "cell-timesteps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * el / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (FP32 preconditioner)",
+        "data": "synthetic (seeded per-member hillslopes)",
+        "config": {
+            "workload": f"c4: {len(members)} hillslopes, nx, ny in [8, 64], nz in [20, 50], "
+                        "FGMRES(30) + FP32 V-cycle (2+2 RB sweeps), dt=0.25 h",
+            "members": len(members),
+            "cells": ncells,
+            "newton_per_member_step": float(lg["newton"].mean()),
+            "gmres_per_member_step": float(lg["gmres"].mean()),
+        },
+        "ensemble": {
+            "rounds": rounds,
+            "launches": launches,
+            "one_at_a_time": {
+                "value": seq_value,
+                "sample": f"every {args.ensemble_stride}th member ({len(solvers)} members, "
+                          f"{scells} cells), single-problem mixed path, {K} steps",
+                "newton_per_member_step": float(np.mean(snewton)),
+            },
+            "speedup": value / seq_value,
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 20 for c2, 24 for c4)")
+    ap.add_argument("--members", type=int, default=512, help="c4 ensemble size")
+    ap.add_argument("--ensemble-stride", type=int, default=16,
+                    help="c4: every stride-th member runs one at a time for the comparison")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-plain", action="store_true")
@@ -258,6 +356,11 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU domain decomposition is design-only).", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c4":
+        run_ensemble(args)
+        return
+    if args.steps is None:
+        args.steps = 20
 
     prob, gp, res, errs = run_gpu(args)
     mixed = res["mixed"]

@@ -204,6 +204,32 @@ int  pfhip_advance_richards(pfhip_solver *s, double *p, double *sat,
                             int max_conv_failures, int *newton, int *gmres,
                             int *fevals, double *dt_used, int *steps_done);
 
+/* ---- ensembles of independent problems ------------------------------- */
+/* No single counterpart in the reference, which solves one problem per
+ * run: each member keeps the AdvanceRichards contract of
+ * pfhip_advance_richards (PL/solver_richards.c:2970-3400) and its own
+ * solver, while the device work of all members is batched -- every
+ * vector operation, residual form, PC setup and V-cycle that members have
+ * pending is served by one launch, and a member that has finished its
+ * steps simply stops contributing (a per-member convergence mask).
+ * Members must use the flexible Krylov variant with a preconditioner, the
+ * difference-quotient J.v and no overland flow; krylov_dim <= 31. */
+typedef struct pfhip_ensemble pfhip_ensemble;
+int  pfhip_ensemble_create(int n, pfhip_solver *const *solvers, pfhip_ensemble **out);
+int  pfhip_ensemble_destroy(pfhip_ensemble *e);
+/* nsteps steps of size dt[m] from t0[m] for every member m, on device
+ * arrays p[m], sat[m]; newton, gmres, fevals (host, n * nsteps, may be NULL)
+ * get per-member per-step counts, steps_done (n, may be NULL) the steps each
+ * member completed.  Returns the first member error (its message in
+ * pfhip_last_error), after every member has stopped. */
+int  pfhip_ensemble_advance_richards(pfhip_ensemble *e, double *const *p,
+                                     double *const *sat, const double *t0,
+                                     const double *dt, int nsteps,
+                                     int max_conv_failures, int *newton, int *gmres,
+                                     int *fevals, int *steps_done);
+/* scheduler rounds and batched launches of the last advance */
+int  pfhip_ensemble_stats(pfhip_ensemble *e, long *rounds, long *launches);
+
 /* ---- live kernel timing (HIP events on the launching stream) --------- */
 /* No counterpart in the reference (its timers are host wall clocks,
  * PL/timing.c:75).  kernel_id: */

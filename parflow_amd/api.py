@@ -86,6 +86,8 @@ EXPORTS = [
     "pfhip_vec_l1_norm", "pfhip_mg_create", "pfhip_mg_destroy", "pfhip_mg_num_levels",
     "pfhip_mg_setup", "pfhip_mg_apply", "pfhip_solver_create", "pfhip_solver_destroy",
     "pfhip_kinsol_nonlin_solver", "pfhip_advance_richards", "pfhip_water_balance_storage",
+    "pfhip_ensemble_create", "pfhip_ensemble_destroy", "pfhip_ensemble_advance_richards",
+    "pfhip_ensemble_stats",
 ]
 
 
@@ -133,6 +135,11 @@ def lib():
                                                  C.POINTER(KinStatsC)]
         L.pfhip_advance_richards.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_int,
                                              C.c_int, vp, vp, vp, vp, C.POINTER(C.c_int)]
+        L.pfhip_ensemble_create.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(vp)]
+        L.pfhip_ensemble_destroy.argtypes = [vp]
+        L.pfhip_ensemble_advance_richards.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), vp, vp,
+                                                      C.c_int, C.c_int, vp, vp, vp, vp]
+        L.pfhip_ensemble_stats.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
         L.pfhip_prof_enable.argtypes = [C.c_int, C.c_int]
         L.pfhip_prof_reset.argtypes = [C.c_int]
         L.pfhip_prof_read.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
@@ -471,3 +478,52 @@ class GPUSolver:
             return np.array(ph), np.array(sh), log
         lg = self.advance_device(dp, ds, t0, dt, nsteps)
         return self.gp.get(dp), self.gp.get(ds), lg
+
+
+class GPUEnsemble:
+    """Independent problems advanced together, their device work batched
+    (pfhip_ensemble_*).  members: list of (Problem, SolverOptions); every
+    member uses the mixed form (FP32 V-cycle inside flexible GMRES)."""
+
+    def __init__(self, members, precision=32):
+        self.problems = [GPUProblem(p) for p, _ in members]
+        self.solvers = [GPUSolver(gp, o, precision=precision, krylov=1)
+                        for gp, (_, o) in zip(self.problems, members)]
+        self.n = len(members)
+        arr = (C.c_void_p * self.n)(*[s.h.value for s in self.solvers])
+        h = C.c_void_p()
+        check(lib().pfhip_ensemble_create(self.n, arr, C.byref(h)), "ensemble_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pfhip_ensemble_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def advance_device(self, dps, dss, t0, dts, nsteps, max_conv_failures=3):
+        """nsteps steps for every member on device arrays dps[m], dss[m];
+        returns per-member per-step counts (arrays n x nsteps)."""
+        n = self.n
+        P = (C.c_void_p * n)(*[d.value for d in dps])
+        S = (C.c_void_p * n)(*[d.value for d in dss])
+        t0a = np.ascontiguousarray(np.broadcast_to(np.asarray(t0, dtype=np.float64), (n,)))
+        dta = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, dtype=np.float64), (n,)))
+        newton = np.zeros((n, nsteps), dtype=np.int32)
+        gmres = np.zeros((n, nsteps), dtype=np.int32)
+        fevals = np.zeros((n, nsteps), dtype=np.int32)
+        done = np.zeros(n, dtype=np.int32)
+        rc = lib().pfhip_ensemble_advance_richards(
+            self.h, P, S, t0a.ctypes.data_as(C.c_void_p), dta.ctypes.data_as(C.c_void_p),
+            int(nsteps), int(max_conv_failures), newton.ctypes.data_as(C.c_void_p),
+            gmres.ctypes.data_as(C.c_void_p), fevals.ctypes.data_as(C.c_void_p),
+            done.ctypes.data_as(C.c_void_p))
+        check(rc, "ensemble_advance_richards")
+        return dict(newton=newton, gmres=gmres, fevals=fevals, steps=done)
+
+    def stats(self):
+        r, l_ = C.c_long(), C.c_long()
+        check(lib().pfhip_ensemble_stats(self.h, C.byref(r), C.byref(l_)), "ensemble_stats")
+        return r.value, l_.value

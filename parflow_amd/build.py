@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpfhip.so")
 
 HIP_SOURCES = ["core.hip", "mg.hip"]
-C_SOURCES = ["kinsol_host.c"]
+C_SOURCES = ["kinsol_host.c", "ensemble.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -341,7 +341,7 @@ static int interval_number(const pfhip_problem_desc *d, const pfhip_bc_spec *b,
 // BCPressure: PL/problem_bc_pressure.c:143-290 (DirEquilRefPatch with a
 // constant density reduces to p = p_ref - rho g (z - z_ref), z = centre for
 // side faces, face elevation for z faces), :582-612, :1070-1098.
-extern "C" int pfhip_bc_pressure(pfhip_problem *pb, double time, void *stream)
+extern "C" int pfhip_bc_pressure_async(pfhip_problem *pb, double time, int sync, void *stream)
 {
   const pfhip_problem_desc *d = &pb->d;
   for (int f = 0; f < 6; f++) {
@@ -369,8 +369,13 @@ extern "C" int pfhip_bc_pressure(pfhip_problem *pb, double time, void *stream)
     HIPCHK(hipMemcpyAsync(pb->bcv[f], h, sizeof(double) * n,
                           hipMemcpyHostToDevice, STREAM(stream)));
   }
-  HIPCHK(hipStreamSynchronize(STREAM(stream)));
+  if (sync) HIPCHK(hipStreamSynchronize(STREAM(stream)));
   return PFHIP_OK;
+}
+
+extern "C" int pfhip_bc_pressure(pfhip_problem *pb, double time, void *stream)
+{
+  return pfhip_bc_pressure_async(pb, time, 1, stream);
 }
 
 // --------------------------------------------------------------------------
@@ -1800,6 +1805,26 @@ static void launch_residual(const ResArgs &A, hipStream_t st)
   PROF_END(PFHIP_PROF_RESIDUAL, st);
 }
 
+// the problem-fixed part of the residual arguments; the caller sets the
+// pressure, output and form fields
+static ResArgs res_args(const pfhip_problem *pb, const double *p_old, const double *s_old,
+                        double dt)
+{
+  ResArgs A;
+  A.g = geom_of(pb);
+  A.f = phys_of(pb);
+  A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
+  A.idx = 1.0 / A.dx; A.idy = 1.0 / A.dy; A.idz = 1.0 / A.dz; A.imu = 1.0 / A.f.mu;
+  A.p_old = p_old; A.s_old = s_old;
+  A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
+  A.poro = pb->d.poro; A.ss = pb->d.ss; A.src = pb->d.src;
+  A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
+  for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
+  A.p = nullptr; A.F = nullptr;
+  A.pv = nullptr; A.fsub = nullptr; A.pout = nullptr; A.psig = 0.0; A.fsc = 0.0;
+  return A;
+}
+
 extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double *F,
                                       const double *p_old, const double *s_old,
                                       double dt, void *stream)
@@ -1810,18 +1835,9 @@ extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double
                        surf_of(pb), p, pb->q2[0], pb->q2[1]);
     LAUNCHCHK();
   }
-  ResArgs A;
-  A.g = g;
-  A.f = phys_of(pb);
-  A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
-  A.idx = 1.0 / A.dx; A.idy = 1.0 / A.dy; A.idz = 1.0 / A.dz; A.imu = 1.0 / A.f.mu;
-  A.p = p; A.p_old = p_old; A.s_old = s_old;
-  A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
-  A.poro = pb->d.poro; A.ss = pb->d.ss; A.src = pb->d.src;
-  A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
-  for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
+  ResArgs A = res_args(pb, p_old, s_old, dt);
+  A.p = p;
   A.F = F;
-  A.pv = nullptr; A.fsub = nullptr; A.pout = nullptr; A.psig = 0.0; A.fsc = 0.0;
   launch_residual(A, STREAM(stream));
   LAUNCHCHK();
   return PFHIP_OK;
@@ -1837,19 +1853,10 @@ int pfhip_residual_step(pfhip_problem *pb, const double *uu, const double *pp, d
     pfhip_set_error("residual_step: overland problems take the two-pass form");
     return PFHIP_ERR_ARG;
   }
-  Geom g = geom_of(pb);
-  ResArgs A;
-  A.g = g;
-  A.f = phys_of(pb);
-  A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
-  A.idx = 1.0 / A.dx; A.idy = 1.0 / A.dy; A.idz = 1.0 / A.dz; A.imu = 1.0 / A.f.mu;
-  A.p = uu; A.p_old = p_old; A.s_old = s_old;
-  A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
-  A.poro = pb->d.poro; A.ss = pb->d.ss; A.src = pb->d.src;
-  A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
-  for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
+  ResArgs A = res_args(pb, p_old, s_old, dt);
+  A.p = uu;
   A.F = F;
-  A.pv = pp; A.fsub = nullptr; A.pout = unew; A.psig = rl; A.fsc = 0.0;
+  A.pv = pp; A.pout = unew; A.psig = rl;
   launch_residual(A, STREAM(stream));
   LAUNCHCHK();
   return PFHIP_OK;
@@ -1867,19 +1874,10 @@ int pfhip_residual_jv(pfhip_problem *pb, const double *u, const double *v, doubl
     pfhip_set_error("residual_jv: overland problems take the two-pass form");
     return PFHIP_ERR_ARG;
   }
-  Geom g = geom_of(pb);
-  ResArgs A;
-  A.g = g;
-  A.f = phys_of(pb);
-  A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
-  A.idx = 1.0 / A.dx; A.idy = 1.0 / A.dy; A.idz = 1.0 / A.dz; A.imu = 1.0 / A.f.mu;
-  A.p = u; A.p_old = p_old; A.s_old = s_old;
-  A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
-  A.poro = pb->d.poro; A.ss = pb->d.ss; A.src = pb->d.src;
-  A.krho = pb->krho; A.qx = pb->q2[0]; A.qy = pb->q2[1];
-  for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
+  ResArgs A = res_args(pb, p_old, s_old, dt);
+  A.p = u;
   A.F = z;
-  A.pv = v; A.fsub = fval; A.pout = nullptr; A.psig = sigma; A.fsc = sinv;
+  A.pv = v; A.fsub = fval; A.psig = sigma; A.fsc = sinv;
   launch_residual(A, STREAM(stream));
   LAUNCHCHK();
   return PFHIP_OK;
@@ -2186,6 +2184,65 @@ __global__ __launch_bounds__(TX * TY) void k_jacobian_pf(JacPFArgs P, JOut<OT> J
   }
 }
 
+// Dirichlet faces of one shell cell's row (PL/richards_jacobian_eval.c:
+// 1049-1226), subtracted from the diagonal c face by face in face order
+__device__ __forceinline__ double jac_dir_faces(const JacArgs &A, int i, int j, int k, int ip,
+                                                double c)
+{
+  const Geom &g = A.g;
+  const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
+  const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
+  const double ddp = 0.0;
+  const double pc = A.p[ip];
+  const double a = A.f.alpha[ip], nn = A.f.nvg[ip];
+  double S, dS, kr, dkr;
+  vg_all(pc, A.f.irg, a, nn, A.f.Sres(ip), A.f.Ssat(ip), S, dS, kr, dkr);
+  const double krc = kr * rho, dkc = dkr * rho + krc * 0.0;
+#pragma unroll 1
+  for (int fc = 0; fc < 6; fc++) {
+    bool on = (fc == 0 && i == 0) || (fc == 1 && i == g.nx - 1) ||
+              (fc == 2 && j == 0) || (fc == 3 && j == g.ny - 1) ||
+              (fc == 4 && k == 0) || (fc == 5 && k == g.nz - 1);
+    if (!on || A.bct[fc] != PFHIP_BC_DIR_EQUIL) continue;
+    int fi = (fc < 2) ? j + k * g.ny : (fc < 4) ? i + k * g.nx : i + j * g.nx;
+    double value = A.bcv[fc][fi];
+    double den_d = rho;
+    double prod = krc, prod_der = dkc;
+    double prod_val = vg_kr(value, A.f.irg, a, nn) * den_d;
+    double coeff, diff, o_temp;
+    if (fc == 0 || fc == 2) {
+      double ff = (fc == 0) ? ffx : ffy, dd = (fc == 0) ? dx : dy;
+      double kk = (fc == 0) ? A.kx[ip] : A.ky[ip];
+      coeff = dt * ff * 1.0 * (2.0 / dd) * kk / mu;
+      diff = value - pc;
+      o_temp = coeff * (diff * upw(value, pc, 0.0, prod_der) - upw(value, pc, prod_val, prod));
+    } else if (fc == 1 || fc == 3) {
+      double ff = (fc == 1) ? ffx : ffy, dd = (fc == 1) ? dx : dy;
+      double kk = (fc == 1) ? A.kx[ip] : A.ky[ip];
+      coeff = dt * ff * 1.0 * (2.0 / dd) * kk / mu;
+      diff = pc - value;
+      o_temp = -coeff * (diff * upw(pc, value, prod_der, 0.0) + upw(pc, value, prod, prod_val));
+    } else if (fc == 4) {
+      coeff = dt * ffz * (2.0 / (dz * 0.5 * (1.0 + 1.0))) * A.kz[ip] / mu;
+      double lc = (value) - 0.5 * dz * 1.0 * den_d * grav;
+      double uc = (pc) + 0.5 * dz * 1.0 * rho * grav;
+      diff = lc - uc;
+      o_temp = coeff * (diff * upw(lc, uc, 0.0, prod_der) +
+                        ((-1.0 - grav * 0.5 * dz * 1.0 * ddp) * upw(lc, uc, prod_val, prod)));
+    } else {
+      coeff = dt * ffz * (2.0 / (dz * 0.5 * (1.0 + 1.0))) * A.kz[ip] / mu;
+      double lc = (pc) - 0.5 * dz * 1.0 * rho * grav;
+      double uc = (value) + 0.5 * dz * 1.0 * den_d * grav;
+      diff = lc - uc;
+      o_temp = -coeff * (diff * upw(lc, uc, prod_der, 0.0) +
+                         ((1.0 - grav * 0.5 * dz * 1.0 * ddp) * upw(lc, uc, prod, prod_val)));
+    }
+    c -= o_temp;
+  }
+  return c;
+}
+
 // Dirichlet faces (:1049-1226) and kinematic overland top rows
 // (:1547-1644/:1938-1987, PFSymmetric top rows of PL/pf_hypre.c:360-420) of
 // the shell cells, applied to the assembled rows in the reference's order.
@@ -2207,60 +2264,12 @@ __global__ __launch_bounds__(256) void k_jacobian_bc(JacArgs A, JOut<OT> J, long
   if (!dir && !ovl) return;
   const int sy = g.sy;
   const int ip = (i + 1) + (j + 1) * sy + (k + 1) * g.sz;
-  const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
   const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
   const double vol = dx * dy * dz;
-  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
-  const double ddp = 0.0;
+  const double ffx = dy * dz, ffy = dx * dz;
   double c = (double)J.c[0][ip];
   const double pc = A.p[ip];
-  const double a = A.f.alpha[ip], nn = A.f.nvg[ip];
-  if (dir) {
-    double S, dS, kr, dkr;
-    vg_all(pc, A.f.irg, a, nn, A.f.Sres(ip), A.f.Ssat(ip), S, dS, kr, dkr);
-    const double krc = kr * rho, dkc = dkr * rho + krc * 0.0;
-#pragma unroll 1
-    for (int fc = 0; fc < 6; fc++) {
-      bool on = (fc == 0 && i == 0) || (fc == 1 && i == g.nx - 1) ||
-                (fc == 2 && j == 0) || (fc == 3 && j == g.ny - 1) ||
-                (fc == 4 && k == 0) || (fc == 5 && k == g.nz - 1);
-      if (!on || A.bct[fc] != PFHIP_BC_DIR_EQUIL) continue;
-      int fi = (fc < 2) ? j + k * g.ny : (fc < 4) ? i + k * g.nx : i + j * g.nx;
-      double value = A.bcv[fc][fi];
-      double den_d = rho;
-      double prod = krc, prod_der = dkc;
-      double prod_val = vg_kr(value, A.f.irg, a, nn) * den_d;
-      double coeff, diff, o_temp;
-      if (fc == 0 || fc == 2) {
-        double ff = (fc == 0) ? ffx : ffy, dd = (fc == 0) ? dx : dy;
-        double kk = (fc == 0) ? A.kx[ip] : A.ky[ip];
-        coeff = dt * ff * 1.0 * (2.0 / dd) * kk / mu;
-        diff = value - pc;
-        o_temp = coeff * (diff * upw(value, pc, 0.0, prod_der) - upw(value, pc, prod_val, prod));
-      } else if (fc == 1 || fc == 3) {
-        double ff = (fc == 1) ? ffx : ffy, dd = (fc == 1) ? dx : dy;
-        double kk = (fc == 1) ? A.kx[ip] : A.ky[ip];
-        coeff = dt * ff * 1.0 * (2.0 / dd) * kk / mu;
-        diff = pc - value;
-        o_temp = -coeff * (diff * upw(pc, value, prod_der, 0.0) + upw(pc, value, prod, prod_val));
-      } else if (fc == 4) {
-        coeff = dt * ffz * (2.0 / (dz * 0.5 * (1.0 + 1.0))) * A.kz[ip] / mu;
-        double lc = (value) - 0.5 * dz * 1.0 * den_d * grav;
-        double uc = (pc) + 0.5 * dz * 1.0 * rho * grav;
-        diff = lc - uc;
-        o_temp = coeff * (diff * upw(lc, uc, 0.0, prod_der) +
-                          ((-1.0 - grav * 0.5 * dz * 1.0 * ddp) * upw(lc, uc, prod_val, prod)));
-      } else {
-        coeff = dt * ffz * (2.0 / (dz * 0.5 * (1.0 + 1.0))) * A.kz[ip] / mu;
-        double lc = (pc) - 0.5 * dz * 1.0 * rho * grav;
-        double uc = (value) + 0.5 * dz * 1.0 * den_d * grav;
-        diff = lc - uc;
-        o_temp = -coeff * (diff * upw(lc, uc, prod_der, 0.0) +
-                           ((1.0 - grav * 0.5 * dz * 1.0 * ddp) * upw(lc, uc, prod, prod_val)));
-      }
-      c -= o_temp;
-    }
-  }
+  if (dir) c = jac_dir_faces(A, i, j, k, ip, c);
   if (ovl) {
     int io = (i + 1) + (j + 1) * sy;
     double dvol = 0.0;
@@ -2400,4 +2409,542 @@ extern "C" int pfhip_matvec(pfhip_problem *pb, const double *const *A, int symm,
                      symm ? A[6] : A[5], A[6], symm, x, y);
   LAUNCHCHK();
   return PFHIP_OK;
+}
+
+// ==========================================================================
+// Ensemble: many independent problems per launch.  Each member keeps its
+// own ghosted arrays; a per-launch table carries every member's geometry
+// and pointers.  A member's vector operation (linear combination or
+// reduction) runs in one workgroup, so one launch covers every member with a
+// pending request and each reduction is finished inside its workgroup in a
+// fixed order.  The residual and the Jacobian run as flat grids over all
+// members' cells (a block never straddles two members) in two passes: the
+// closures of every cell into member scratch, then the face terms of every
+// cell from its own and its neighbours' closures.  Members differ in size
+// by two orders of magnitude, which the flat grids balance and the
+// one-workgroup reductions keep deterministic.
+// ==========================================================================
+#define EWG 512
+
+__device__ __forceinline__ int cell_ip(const Geom &g, int c, int &i, int &j, int &k)
+{
+  i = c % g.nx;
+  int r = c / g.nx;
+  j = r % g.ny;
+  k = r / g.ny;
+  return (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
+}
+
+__device__ __forceinline__ double ens_block_sum(double v, double *sh)
+{
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = sh[0];
+#pragma unroll
+  for (int q = 1; q < EWG / 64; q++) r += sh[q];
+  return r;
+}
+
+__device__ __forceinline__ double ens_block_max(double v, double *sh)
+{
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = sh[0];
+#pragma unroll
+  for (int q = 1; q < EWG / 64; q++) r = fmax(r, sh[q]);
+  return r;
+}
+
+// one member's vector operation (PFHIP_EOP_LIN .. PFHIP_EOP_STOP)
+struct EnsVec {
+  Geom g;
+  int op, nq, base, norm;
+  double cst;
+  double c[PFHIP_ENS_MAXQ];
+  const double *xq[PFHIP_ENS_MAXQ];
+  const double *x, *y, *u;
+  double *z;
+};
+
+#define ENS_MDOT 32
+__global__ __launch_bounds__(EWG) void k_ens_vec(const EnsVec *__restrict__ T,
+                                                 double *__restrict__ res)
+{
+  __shared__ double sh[EWG / 64];
+  const EnsVec &E = T[blockIdx.x];
+  const Geom g = E.g;
+  const int op = E.op, nq = E.nq;
+  const int n = g.nx * g.ny * g.nz;
+  double *R = res + (long)blockIdx.x * PFHIP_ENS_NRES;
+  int i, j, k;
+  if (op == PFHIP_EOP_LIN) {
+    // z may alias an input: every cell is read before it is written, by
+    // the same thread
+    double acc = 0.0;
+    for (int c = threadIdx.x; c < n; c += EWG) {
+      const int ip = cell_ip(g, c, i, j, k);
+      double s;
+      int q0;
+      if (E.base) { s = E.z[ip]; q0 = 0; }
+      else if (nq > 0) { s = E.c[0] * E.xq[0][ip]; q0 = 1; }
+      else { s = E.cst; q0 = 0; }
+      for (int q = q0; q < nq; q++) s += E.c[q] * E.xq[q][ip];
+      E.z[ip] = s;
+      acc += s * s;
+    }
+    if (E.norm) {
+      acc = ens_block_sum(acc, sh);
+      if (threadIdx.x == 0) R[0] = acc;
+    }
+    return;
+  }
+  if (op == PFHIP_EOP_MDOT) {
+    double acc[ENS_MDOT];
+#pragma unroll
+    for (int q = 0; q < ENS_MDOT; q++) acc[q] = 0.0;
+    for (int c = threadIdx.x; c < n; c += EWG) {
+      const int ip = cell_ip(g, c, i, j, k);
+      const double w = E.y[ip];
+#pragma unroll
+      for (int q = 0; q < ENS_MDOT; q++)
+        if (q < nq) acc[q] += E.xq[q][ip] * w;
+    }
+#pragma unroll
+    for (int q = 0; q < ENS_MDOT; q++) {
+      if (q < nq) {
+        double s = ens_block_sum(acc[q], sh);
+        if (threadIdx.x == 0) R[q] = s;
+      }
+    }
+    return;
+  }
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int c = threadIdx.x; c < n; c += EWG) {
+    const int ip = cell_ip(g, c, i, j, k);
+    switch (op) {
+      case PFHIP_EOP_DOT: a0 += E.x[ip] * E.y[ip]; break;
+      case PFHIP_EOP_MAXABS: a0 = fmax(a0, fabs(E.x[ip])); break;
+      case PFHIP_EOP_SCSTEP: a0 = fmax(a0, fabs(E.y[ip] / (1.0 + fabs(E.x[ip])))); break;
+      case PFHIP_EOP_DQ: {
+        double u = E.x[ip], v = E.y[ip];
+        a0 += u * v;
+        a1 += v * v;
+        a2 += fabs(v);
+        break;
+      }
+      default: {   // PFHIP_EOP_STOP
+        a0 = fmax(a0, fabs(E.x[ip]));
+        double un = E.y[ip];
+        double st = 1.0 * un + (-1.0) * E.u[ip];
+        a1 = fmax(a1, fabs(st / (1.0 + fabs(un))));
+      }
+    }
+  }
+  if (op == PFHIP_EOP_DOT || op == PFHIP_EOP_DQ) a0 = ens_block_sum(a0, sh);
+  else a0 = ens_block_max(a0, sh);
+  if (op == PFHIP_EOP_DQ) { a1 = ens_block_sum(a1, sh); a2 = ens_block_sum(a2, sh); }
+  if (op == PFHIP_EOP_STOP) a1 = ens_block_max(a1, sh);
+  if (threadIdx.x == 0) { R[0] = a0; R[1] = a1; R[2] = a2; }
+}
+
+extern "C" void *pfhip_ens_upload(pfhip_ens_stage *st, const void *src, long bytes,
+                                  void *stream)
+{
+  long need = (bytes + 255) & ~255L;
+  if (need > st->cap) {
+    pfhip_set_error("ensemble table of %ld bytes exceeds the staging ring", bytes);
+    return nullptr;
+  }
+  if (st->off + need > st->cap) {
+    // wrap: earlier copies and the launches reading them must be done
+    if (hipStreamSynchronize(STREAM(stream)) != hipSuccess) {
+      pfhip_set_error("ensemble staging: stream synchronisation failed");
+      return nullptr;
+    }
+    st->off = 0;
+  }
+  memcpy(st->host + st->off, src, bytes);
+  if (hipMemcpyAsync(st->dev + st->off, st->host + st->off, bytes, hipMemcpyHostToDevice,
+                     STREAM(stream)) != hipSuccess) {
+    pfhip_set_error("ensemble staging: host-to-device copy failed");
+    return nullptr;
+  }
+  void *d = st->dev + st->off;
+  st->off += need;
+  return d;
+}
+
+extern "C" int pfhip_ens_exec_vec(pfhip_ens_req *const *rq, int n, double *res,
+                                  pfhip_ens_stage *st, void *stream)
+{
+  if (n <= 0) return PFHIP_OK;
+  EnsVec *T = (EnsVec *)calloc(n, sizeof(EnsVec));
+  if (!T) { pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
+  for (int r = 0; r < n; r++) {
+    const pfhip_ens_req *q = rq[r];
+    EnsVec &E = T[r];
+    E.g = geom_of(q->pb);
+    E.op = q->op; E.nq = q->nq; E.base = q->base; E.norm = q->norm; E.cst = q->cst;
+    for (int t = 0; t < q->nq && t < PFHIP_ENS_MAXQ; t++) { E.c[t] = q->c[t]; E.xq[t] = q->xq[t]; }
+    E.x = q->x; E.y = q->y; E.u = q->u; E.z = q->z;
+  }
+  void *dT = pfhip_ens_upload(st, T, (long)n * sizeof(EnsVec), stream);
+  free(T);
+  if (!dT) return PFHIP_ERR_HIP;
+  hipLaunchKernelGGL(k_ens_vec, dim3(n), dim3(EWG), 0, STREAM(stream), (const EnsVec *)dT,
+                     res);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// ---- flat two-pass residual ------------------------------------------------
+#define ERB 256   // cells per block of the flat member grids
+
+struct EnsRes {
+  ResArgs A;
+  double *pe, *kr, *sat;   // member scratch: evaluation pressure, kr rho, S
+  int mode, ncell;
+};
+
+// the member owning flat block b: last e with starts[e] <= b
+__device__ __forceinline__ int ens_find(const int *__restrict__ starts, int n, int b)
+{
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// pass 1: pressure of the form (p, or p + sig pv), S and kr rho of every cell
+__global__ __launch_bounds__(ERB) void k_ens_res_closure(const EnsRes *__restrict__ T,
+                                                         const int *__restrict__ starts,
+                                                         int n)
+{
+  const int e = ens_find(starts, n, blockIdx.x);
+  const EnsRes &E = T[e];
+  const int c = (blockIdx.x - starts[e]) * ERB + threadIdx.x;
+  if (c >= E.ncell) return;
+  const ResArgs &A = E.A;
+  const Phys &f = A.f;
+  int i, j, k;
+  const int ip = cell_ip(A.g, c, i, j, k);
+  const double p = (E.mode != 0) ? A.p[ip] + A.psig * A.pv[ip] : A.p[ip];
+  double S, kr;
+  vg_s_kr(p, f.irg, f.alpha[ip], f.nvg[ip], f.Sres(ip), f.Ssat(ip), S, kr);
+  E.pe[ip] = p;
+  E.kr[ip] = kr * f.rho;
+  E.sat[ip] = S;
+  if (E.mode == 2) A.pout[ip] = p;
+}
+
+// pass 2: the cell's six faces (own upper faces and the lower faces, each
+// with the lower cell first, as the tiled kernel evaluates them), storage,
+// the boundary-face terms, and the output form
+__global__ __launch_bounds__(ERB) void k_ens_res_faces(const EnsRes *__restrict__ T,
+                                                       const int *__restrict__ starts,
+                                                       int n)
+{
+  const int e = ens_find(starts, n, blockIdx.x);
+  const EnsRes &E = T[e];
+  const int c = (blockIdx.x - starts[e]) * ERB + threadIdx.x;
+  if (c >= E.ncell) return;
+  const ResArgs &A = E.A;
+  const Geom &g = A.g;
+  const Phys &f = A.f;
+  int i, j, k;
+  const int ip = cell_ip(g, c, i, j, k);
+  const int sy = g.sy, sz = g.sz;
+  const double grav = f.g, imu = A.imu;
+  const double hrg = 0.5 * f.rho * grav * 1.0;
+  const double ffx = A.dy * A.dz, ffy = A.dx * A.dz, ffz = A.dx * A.dy;
+  const double *__restrict__ P = E.pe;
+  const double *__restrict__ KR = E.kr;
+  const double pc = P[ip], krc = KR[ip];
+  const double kxc = A.kx[ip], kyc = A.ky[ip], kzc = A.kz[ip];
+  double ur = 0.0, uf = 0.0, uu = 0.0, ul = 0.0, ud = 0.0, ub = 0.0;
+  if (i + 1 < g.nx)
+    ur = flux_xy_i(pc, P[ip + 1], kxc, A.kx[ip + 1], krc, KR[ip + 1], ffx, A.idx, grav, imu);
+  if (j + 1 < g.ny)
+    uf = flux_xy_i(pc, P[ip + sy], kyc, A.ky[ip + sy], krc, KR[ip + sy], ffy, A.idy, grav, imu);
+  if (k + 1 < g.nz)
+    uu = flux_z_i(pc, P[ip + sz], kzc, A.kz[ip + sz], krc, KR[ip + sz], ffz, A.idz, hrg, imu);
+  if (i > 0)
+    ul = flux_xy_i(P[ip - 1], pc, A.kx[ip - 1], kxc, KR[ip - 1], krc, ffx, A.idx, grav, imu);
+  if (j > 0)
+    ud = flux_xy_i(P[ip - sy], pc, A.ky[ip - sy], kyc, KR[ip - sy], krc, ffy, A.idy, grav, imu);
+  if (k > 0)
+    ub = flux_z_i(P[ip - sz], pc, A.kz[ip - sz], kzc, KR[ip - sz], krc, ffz, A.idz, hrg, imu);
+  double fv = residual_cell(A, i, j, k, ip, pc, krc, E.sat[ip], kxc, kyc, kzc, ur, uf, uu, ul,
+                            ud, ub, A.s_old[ip], A.poro[ip], A.ss[ip], A.p_old[ip]);
+  const bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
+                   (k == g.nz - 1);
+  if (bnd) {
+    const double krb = vg_kr(pc, f.irg, f.alpha[ip], f.nvg[ip]) * f.rho;
+    fv = residual_bc(A, i, j, k, ip, fv, pc, krb, kxc, kyc, kzc);
+  }
+  if (E.mode == 1) fv = A.fsc * fv - A.fsc * A.fsub[ip];
+  A.F[ip] = fv;
+}
+
+static int *ens_starts(int n, const long *ncell, int per_block, int *total)
+{
+  int *s = (int *)malloc(sizeof(int) * (n + 1));
+  if (!s) return nullptr;
+  long b = 0;
+  for (int e = 0; e < n; e++) {
+    s[e] = (int)b;
+    b += (ncell[e] + per_block - 1) / per_block;
+  }
+  s[n] = (int)b;
+  *total = (int)b;
+  return s;
+}
+
+extern "C" int pfhip_ens_exec_residual(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
+                                       void *stream)
+{
+  if (n <= 0) return PFHIP_OK;
+  EnsRes *T = (EnsRes *)calloc(n, sizeof(EnsRes));
+  long *nc = (long *)malloc(sizeof(long) * n);
+  if (!T || !nc) { free(T); free(nc); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
+  for (int r = 0; r < n; r++) {
+    const pfhip_ens_req *q = rq[r];
+    EnsRes &E = T[r];
+    E.A = res_args(q->pb, q->xq[0], q->xq[1], q->dt);
+    E.A.p = q->x; E.A.F = q->z;
+    if (q->mode == 1) { E.A.pv = q->y; E.A.fsub = q->u; E.A.psig = q->sig; E.A.fsc = q->fsc; }
+    if (q->mode == 2) { E.A.pv = q->y; E.A.pout = q->z2; E.A.psig = q->sig; }
+    E.pe = q->scratch[0]; E.kr = q->scratch[1]; E.sat = q->scratch[2];
+    E.mode = q->mode;
+    E.ncell = (int)q->pb->ncell;
+    nc[r] = q->pb->ncell;
+  }
+  int total = 0;
+  int *starts = ens_starts(n, nc, ERB, &total);
+  free(nc);
+  if (!starts) { free(T); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
+  void *dT = pfhip_ens_upload(st, T, (long)n * sizeof(EnsRes), stream);
+  void *dS = dT ? pfhip_ens_upload(st, starts, (long)(n + 1) * sizeof(int), stream) : nullptr;
+  free(T);
+  free(starts);
+  if (!dS) return PFHIP_ERR_HIP;
+  hipLaunchKernelGGL(k_ens_res_closure, dim3(total), dim3(ERB), 0, STREAM(stream),
+                     (const EnsRes *)dT, (const int *)dS, n);
+  hipLaunchKernelGGL(k_ens_res_faces, dim3(total), dim3(ERB), 0, STREAM(stream),
+                     (const EnsRes *)dT, (const int *)dS, n);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// ---- flat two-pass PFSymmetric Jacobian into the members' level-0 matrices --
+struct EnsJac {
+  JacArgs A;
+  double *kr, *dk, *sat, *dsat;   // member scratch
+  void *J[4];                     // C, E, N, U of the multigrid's finest level
+  int ncell;
+};
+
+__global__ __launch_bounds__(ERB) void k_ens_jac_closure(const EnsJac *__restrict__ T,
+                                                         const int *__restrict__ starts,
+                                                         int n)
+{
+  const int e = ens_find(starts, n, blockIdx.x);
+  const EnsJac &E = T[e];
+  const int c = (blockIdx.x - starts[e]) * ERB + threadIdx.x;
+  if (c >= E.ncell) return;
+  const JacArgs &A = E.A;
+  const Phys &f = A.f;
+  int i, j, k;
+  const int ip = cell_ip(A.g, c, i, j, k);
+  double S, dS, kr, dk;
+  vg_all(A.p[ip], f.irg, f.alpha[ip], f.nvg[ip], f.Sres(ip), f.Ssat(ip), S, dS, kr, dk);
+  kr *= f.rho;
+  E.kr[ip] = kr;
+  E.dk[ip] = dk * f.rho + kr * 0.0;
+  E.sat[ip] = S;
+  E.dsat[ip] = dS;
+}
+
+template <typename OT>
+__global__ __launch_bounds__(ERB) void k_ens_jac_rows(const EnsJac *__restrict__ T,
+                                                      const int *__restrict__ starts, int n)
+{
+  const int e = ens_find(starts, n, blockIdx.x);
+  const EnsJac &E = T[e];
+  const int c = (blockIdx.x - starts[e]) * ERB + threadIdx.x;
+  if (c >= E.ncell) return;
+  const JacArgs &A = E.A;
+  const Geom &g = A.g;
+  const Phys &f = A.f;
+  int i, j, k;
+  const int ip = cell_ip(g, c, i, j, k);
+  const int sy = g.sy, sz = g.sz;
+  const double rho = f.rho, gc = f.g, imu = 1.0 / f.mu;
+  const double dt = A.dt, dx = A.dx, dy = A.dy, dz = A.dz;
+  const double idx = 1.0 / dx, idy = 1.0 / dy;
+  const double isep = 1.0 / (dz * 0.5 * (1.0 + 1.0));
+  const double hrg = (1.0 / (1.0 + 1.0)) * rho * gc;
+  const double vol2 = dx * dy * dz * 1.0;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
+  const double ddp = 0.0;
+  const double *__restrict__ P = A.p;
+  const double *__restrict__ KR = E.kr;
+  const double *__restrict__ DK = E.dk;
+  const double pc = P[ip], krc = KR[ip], dkc = DK[ip], S = E.sat[ip], dS = E.dsat[ip];
+  double cc = (dS * rho + S * ddp) * A.poro[ip] * vol2 +
+              A.ss[ip] * vol2 * (dS * rho * pc + S * ddp * pc + S * rho);
+  double offE = 0.0, offN = 0.0, offU = 0.0;
+  double T1, T2, sym, syml, symu;
+  if (i + 1 < g.nx) {
+    jface_xy(pc, P[ip + 1], A.kx[ip], A.kx[ip + 1], krc, KR[ip + 1], dkc, DK[ip + 1], dt, ffx,
+             idx, gc, imu, T1, T2, sym);
+    cc -= T1;
+    offE = sym;
+  }
+  if (i > 0) {
+    jface_xy(P[ip - 1], pc, A.kx[ip - 1], A.kx[ip], KR[ip - 1], krc, DK[ip - 1], dkc, dt, ffx,
+             idx, gc, imu, T1, T2, sym);
+    cc -= T2;
+  }
+  if (j + 1 < g.ny) {
+    jface_xy(pc, P[ip + sy], A.ky[ip], A.ky[ip + sy], krc, KR[ip + sy], dkc, DK[ip + sy], dt,
+             ffy, idy, gc, imu, T1, T2, sym);
+    cc -= T1;
+    offN = sym;
+  }
+  if (j > 0) {
+    jface_xy(P[ip - sy], pc, A.ky[ip - sy], A.ky[ip], KR[ip - sy], krc, DK[ip - sy], dkc, dt,
+             ffy, idy, gc, imu, T1, T2, sym);
+    cc -= T2;
+  }
+  if (k + 1 < g.nz) {
+    jface_z(pc, P[ip + sz], A.kz[ip], A.kz[ip + sz], krc, KR[ip + sz], dkc, DK[ip + sz], dt,
+            ffz, isep, hrg, imu, T1, T2, syml, symu);
+    cc -= T1;
+    offU = symu;
+  }
+  if (k > 0) {
+    jface_z(P[ip - sz], pc, A.kz[ip - sz], A.kz[ip], KR[ip - sz], krc, DK[ip - sz], dkc, dt,
+            ffz, isep, hrg, imu, T1, T2, syml, symu);
+    cc -= T2;
+  }
+  const bool dir = (i == 0 && A.bct[0] == PFHIP_BC_DIR_EQUIL) ||
+                   (i == g.nx - 1 && A.bct[1] == PFHIP_BC_DIR_EQUIL) ||
+                   (j == 0 && A.bct[2] == PFHIP_BC_DIR_EQUIL) ||
+                   (j == g.ny - 1 && A.bct[3] == PFHIP_BC_DIR_EQUIL) ||
+                   (k == 0 && A.bct[4] == PFHIP_BC_DIR_EQUIL) ||
+                   (k == g.nz - 1 && A.bct[5] == PFHIP_BC_DIR_EQUIL);
+  // the single-problem path stores the interior row, then applies the
+  // Dirichlet faces to the stored (rounded) diagonal
+  if (dir) cc = jac_dir_faces(A, i, j, k, ip, (double)(OT)cc);
+  ((OT *)E.J[0])[ip] = (OT)cc;
+  ((OT *)E.J[1])[ip] = (OT)offE;
+  ((OT *)E.J[2])[ip] = (OT)offN;
+  ((OT *)E.J[3])[ip] = (OT)offU;
+}
+
+extern "C" int pfhip_ens_exec_jacobian(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
+                                       void *stream)
+{
+  if (n <= 0) return PFHIP_OK;
+  EnsJac *T = (EnsJac *)calloc(n, sizeof(EnsJac));
+  long *nc = (long *)malloc(sizeof(long) * n);
+  if (!T || !nc) { free(T); free(nc); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
+  int prec0 = 0;
+  for (int r = 0; r < n; r++) {
+    const pfhip_ens_req *q = rq[r];
+    pfhip_problem *pb = q->pb;
+    EnsJac &E = T[r];
+    JacArgs &A = E.A;
+    A.g = geom_of(pb);
+    A.f = phys_of(pb);
+    A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = q->dt;
+    A.p = q->x;
+    A.kx = pb->d.kx; A.ky = pb->d.ky; A.kz = pb->d.kz;
+    A.poro = pb->d.poro; A.ss = pb->d.ss;
+    A.krho = nullptr; A.dkrho = nullptr;
+    A.ke = A.kw = A.kn = A.ks = nullptr;
+    for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
+    A.symm = 1;
+    A.overland = 0;
+    for (int s = 0; s < 7; s++) A.J[s] = nullptr;
+    E.kr = q->scratch[0]; E.dk = q->scratch[1]; E.sat = q->scratch[2]; E.dsat = q->scratch[3];
+    int prec;
+    pfhip_mg_level0(q->mg, E.J, &prec);
+    if (r == 0) prec0 = prec;
+    if (prec != prec0 || pb->overland) {
+      free(T); free(nc);
+      pfhip_set_error("ensemble members need one multigrid precision and no overland flow");
+      return PFHIP_ERR_ARG;
+    }
+    E.ncell = (int)pb->ncell;
+    nc[r] = pb->ncell;
+  }
+  int total = 0;
+  int *starts = ens_starts(n, nc, ERB, &total);
+  free(nc);
+  if (!starts) { free(T); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
+  void *dT = pfhip_ens_upload(st, T, (long)n * sizeof(EnsJac), stream);
+  void *dS = dT ? pfhip_ens_upload(st, starts, (long)(n + 1) * sizeof(int), stream) : nullptr;
+  free(T);
+  free(starts);
+  if (!dS) return PFHIP_ERR_HIP;
+  hipLaunchKernelGGL(k_ens_jac_closure, dim3(total), dim3(ERB), 0, STREAM(stream),
+                     (const EnsJac *)dT, (const int *)dS, n);
+  if (prec0 == 32)
+    hipLaunchKernelGGL(k_ens_jac_rows<float>, dim3(total), dim3(ERB), 0, STREAM(stream),
+                       (const EnsJac *)dT, (const int *)dS, n);
+  else
+    hipLaunchKernelGGL(k_ens_jac_rows<double>, dim3(total), dim3(ERB), 0, STREAM(stream),
+                       (const EnsJac *)dT, (const int *)dS, n);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// pinned staging ring, device result slots
+extern "C" int pfhip_ens_stage_create(pfhip_ens_stage *st, long cap)
+{
+  st->off = 0;
+  st->cap = cap;
+  st->host = nullptr;
+  st->dev = nullptr;
+  HIPCHK(hipHostMalloc((void **)&st->host, cap, hipHostMallocDefault));
+  HIPCHK(hipMalloc((void **)&st->dev, cap));
+  return PFHIP_OK;
+}
+
+extern "C" void pfhip_ens_stage_destroy(pfhip_ens_stage *st)
+{
+  if (st->host) hipHostFree(st->host);
+  if (st->dev) hipFree(st->dev);
+  st->host = st->dev = nullptr;
+}
+
+extern "C" int pfhip_ens_fetch(double *host, const double *dev, long n, void *stream)
+{
+  if (n > 0)
+    HIPCHK(hipMemcpyAsync(host, dev, sizeof(double) * n, hipMemcpyDeviceToHost,
+                          STREAM(stream)));
+  HIPCHK(hipStreamSynchronize(STREAM(stream)));
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_host_alloc(double **p, long n)
+{
+  *p = nullptr;
+  HIPCHK(hipHostMalloc((void **)p, sizeof(double) * (n > 0 ? n : 1), hipHostMallocDefault));
+  return PFHIP_OK;
+}
+
+extern "C" void pfhip_host_free(double *p)
+{
+  if (p) hipHostFree(p);
 }

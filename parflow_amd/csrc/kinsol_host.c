@@ -31,6 +31,7 @@
 
 struct pfhip_solver {
   pfhip_problem *pb;
+  const pfhip_ops *ops;         /* device operations (direct or batched)  */
   pfhip_solver_opts o;
   long ng;
   double *unew, *fval, *pp, *vtemp1, *vtemp2, *xcor, *vtemp;
@@ -52,6 +53,19 @@ struct pfhip_solver {
 };
 
 #define CHK(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
+
+const pfhip_ops pfhip_direct_ops = {
+  pfhip_vec_scale, pfhip_vec_linear_sum, pfhip_vec_const, pfhip_vec_dot,
+  pfhip_vec_max_norm, pfhip_vec_scsteplength, pfhip_vec_dq_sums, pfhip_vec_multi_dot,
+  pfhip_vec_multi_axpy, pfhip_vec_combine, pfhip_kin_stop_norms, pfhip_nl_function_eval,
+  pfhip_residual_jv, pfhip_residual_step, pfhip_mg_setup_jacobian, pfhip_mg_apply,
+  pfhip_bc_pressure, pfhip_saturation, pfhip_synchronize,
+};
+
+void pfhip_solver_set_ops(pfhip_solver *s, const pfhip_ops *ops) { s->ops = ops; }
+pfhip_problem *pfhip_solver_problem(pfhip_solver *s) { return s->pb; }
+pfhip_mg *pfhip_solver_mg(pfhip_solver *s) { return s->mg; }
+const pfhip_solver_opts *pfhip_solver_options(pfhip_solver *s) { return &s->o; }
 
 static int alloc_vec(pfhip_solver *s, double **p)
 {
@@ -89,6 +103,7 @@ int pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
   }
   pfhip_solver *s = (pfhip_solver *)calloc(1, sizeof(pfhip_solver));
   s->pb = pb;
+  s->ops = &pfhip_direct_ops;
   s->o = *opts;
   s->ng = pb->ng;
   s->l_max = opts->krylov_dim;
@@ -115,7 +130,7 @@ int pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
   s->givens = (double *)calloc(2 * s->l_max, sizeof(double));
   s->yg = (double *)calloc(s->l_max + 1, sizeof(double));
   s->hbuf = (double *)calloc(2 * (s->l_max + 2), sizeof(double));
-  if (!rc) rc = pfhip_synchronize();
+  if (!rc) rc = s->ops->synchronize();
   if (rc) { pfhip_solver_destroy(s); return rc; }
   *out = s;
   return PFHIP_OK;
@@ -125,13 +140,13 @@ int pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
 static int func(pfhip_solver *s, const double *u, double *f)
 {
   s->nfe++;
-  return pfhip_nl_function_eval(s->pb, u, f, s->cur_p_old, s->cur_s_old, s->dt, NULL);
+  return s->ops->nl_function_eval(s->pb, u, f, s->cur_p_old, s->cur_s_old, s->dt, NULL);
 }
 
 static int l2norm(pfhip_solver *s, const double *x, double *r)
 {
   double d;
-  CHK(pfhip_vec_dot(s->pb, x, x, &d, NULL));
+  CHK(s->ops->vec_dot(s->pb, x, x, &d, NULL));
   *r = sqrt(d);
   return PFHIP_OK;
 }
@@ -139,7 +154,7 @@ static int l2norm(pfhip_solver *s, const double *x, double *r)
 /* KINSolInitPC -> KinsolPCInitInstanceXtra: PC matrix + multigrid setup */
 static int pc_setup(pfhip_solver *s)
 {
-  CHK(pfhip_mg_setup_jacobian(s->mg, s->uu, s->dt, NULL));
+  CHK(s->ops->mg_setup_jacobian(s->mg, s->uu, s->dt, NULL));
   s->npe++;
   s->nnilpre = s->nni;
   s->precondcurrent = 1;
@@ -151,10 +166,10 @@ static int psolve(pfhip_solver *s, const double *r, double *z)
 {
   s->nps++;
   if (r == z) {
-    CHK(pfhip_vec_scale(s->pb, 1.0, r, s->vtemp2, NULL));
-    return pfhip_mg_apply(s->mg, s->vtemp2, z, NULL);
+    CHK(s->ops->vec_scale(s->pb, 1.0, r, s->vtemp2, NULL));
+    return s->ops->mg_apply(s->mg, s->vtemp2, z, NULL);
   }
-  return pfhip_mg_apply(s->mg, r, z, NULL);
+  return s->ops->mg_apply(s->mg, r, z, NULL);
 }
 
 /* KINSpgmrAtimes: KINSolMatVec or difference quotient KS/kinspgmr.c:481-531 */
@@ -169,7 +184,7 @@ static int atimes(pfhip_solver *s, const double *v, double *z)
     return pfhip_matvec(pb, (const double *const *)s->J, 0, v, z, NULL);
   }
   double sutsv, vtv, sq1norm;
-  CHK(pfhip_vec_dq_sums(pb, s->uu, v, &sutsv, &vtv, &sq1norm, NULL));
+  CHK(s->ops->vec_dq_sums(pb, s->uu, v, &sutsv, &vtv, &sq1norm, NULL));
   double sign = (sutsv >= 0.0) ? 1.0 : -1.0;
   double a = fabs(sutsv);
   double sigma = sign * s->sqrt_relfunc * ((a > sq1norm) ? a : sq1norm) / vtv;
@@ -177,12 +192,12 @@ static int atimes(pfhip_solver *s, const double *v, double *z)
   if (!pb->overland) {
     /* u + sigma v, F and the quotient in one residual launch */
     s->nfe++;
-    return pfhip_residual_jv(pb, s->uu, v, sigma, s->fval, sigma_inv, z, s->cur_p_old,
+    return s->ops->residual_jv(pb, s->uu, v, sigma, s->fval, sigma_inv, z, s->cur_p_old,
                              s->cur_s_old, s->dt, NULL);
   }
-  CHK(pfhip_vec_linear_sum(pb, 1.0, s->uu, sigma, v, s->vtemp1, NULL));
+  CHK(s->ops->vec_linear_sum(pb, 1.0, s->uu, sigma, v, s->vtemp1, NULL));
   CHK(func(s, s->vtemp1, s->vtemp2));
-  return pfhip_vec_linear_sum(pb, sigma_inv, s->vtemp2, -sigma_inv, s->fval, z, NULL);
+  return s->ops->vec_linear_sum(pb, sigma_inv, s->vtemp2, -sigma_inv, s->fval, z, NULL);
 }
 
 /* ModifiedGS (KS/iterativ.c:30-73), each step one fused axpy+dot pass */
@@ -191,7 +206,7 @@ static int modified_gs(pfhip_solver *s, int k, double *new_vk_norm)
   pfhip_problem *pb = s->pb;
   double **v = s->V, **h = s->Hes;
   double vk2;
-  CHK(pfhip_vec_dot(pb, v[k], v[k], &vk2, NULL));
+  CHK(s->ops->vec_dot(pb, v[k], v[k], &vk2, NULL));
   double vk_norm = sqrt(vk2);
   int km1 = k - 1;
   double d;
@@ -209,11 +224,11 @@ static int modified_gs(pfhip_solver *s, int k, double *new_vk_norm)
   double new_norm_2 = 0.0;
   for (int i = 0; i < k; i++) {
     double np_;
-    CHK(pfhip_vec_dot(pb, v[i], v[k], &np_, NULL));
+    CHK(s->ops->vec_dot(pb, v[i], v[k], &np_, NULL));
     temp = 1000.0 * h[i][km1];
     if ((temp + np_) == temp) continue;
     h[i][km1] += np_;
-    CHK(pfhip_vec_linear_sum(pb, 1.0, v[k], -np_, v[i], v[k], NULL));
+    CHK(s->ops->vec_linear_sum(pb, 1.0, v[k], -np_, v[i], v[k], NULL));
     new_norm_2 += np_ * np_;
   }
   if (new_norm_2 != 0.0) {
@@ -230,10 +245,10 @@ static int cgs2(pfhip_solver *s, int k, double *new_vk_norm)
   double *h1 = s->hbuf, *h2 = s->hbuf + (s->l_max + 2);
   int km1 = k - 1;
   double n2;
-  CHK(pfhip_vec_multi_dot(pb, (const double *const *)s->V, k, s->V[k], h1, NULL));
-  CHK(pfhip_vec_multi_axpy(pb, (const double *const *)s->V, k, h1, s->V[k], NULL, NULL));
-  CHK(pfhip_vec_multi_dot(pb, (const double *const *)s->V, k, s->V[k], h2, NULL));
-  CHK(pfhip_vec_multi_axpy(pb, (const double *const *)s->V, k, h2, s->V[k], &n2, NULL));
+  CHK(s->ops->vec_multi_dot(pb, (const double *const *)s->V, k, s->V[k], h1, NULL));
+  CHK(s->ops->vec_multi_axpy(pb, (const double *const *)s->V, k, h1, s->V[k], NULL, NULL));
+  CHK(s->ops->vec_multi_dot(pb, (const double *const *)s->V, k, s->V[k], h2, NULL));
+  CHK(s->ops->vec_multi_axpy(pb, (const double *const *)s->V, k, h2, s->V[k], &n2, NULL));
   for (int i = 0; i < k; i++) s->Hes[i][km1] = h1[i] + h2[i];
   *new_vk_norm = sqrt(n2 > 0.0 ? n2 : 0.0);
   return PFHIP_OK;
@@ -302,14 +317,14 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double de
   CHK(l2norm(s, fneg, &r_norm));
   *res_norm = beta = r_norm;
   if (r_norm <= delta) { *ret = SPGMR_SUCCESS; return PFHIP_OK; }
-  if (!flex) CHK(pfhip_vec_const(pb, 0.0, s->xcor, NULL));
+  if (!flex) CHK(s->ops->vec_const(pb, 0.0, s->xcor, NULL));
 
   for (int ntries = 0; ntries <= s->max_restarts; ntries++) {
     for (int i = 0; i <= l_max; i++)
       for (int j = 0; j < l_max; j++) Hes[i][j] = 0.0;
     rotation_product = 1.0;
-    if (ntries == 0) CHK(pfhip_vec_scale(pb, -1.0 / r_norm, fneg, V[0], NULL));
-    else CHK(pfhip_vec_scale(pb, 1.0 / r_norm, V[0], V[0], NULL));
+    if (ntries == 0) CHK(s->ops->vec_scale(pb, -1.0 / r_norm, fneg, V[0], NULL));
+    else CHK(s->ops->vec_scale(pb, 1.0 / r_norm, V[0], V[0], NULL));
     int l;
     for (l = 0; l < l_max; l++) {
       (*nli)++;
@@ -327,23 +342,23 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double de
       rotation_product *= s->givens[2 * l + 1];
       *res_norm = rho = fabs(rotation_product * r_norm);
       if (rho <= delta) { converged = 1; break; }
-      CHK(pfhip_vec_scale(pb, 1.0 / Hes[l + 1][l], V[l + 1], V[l + 1], NULL));
+      CHK(s->ops->vec_scale(pb, 1.0 / Hes[l + 1][l], V[l + 1], V[l + 1], NULL));
     }
     s->yg[0] = r_norm;
     for (int i = 1; i <= krydim; i++) s->yg[i] = 0.0;
     if (qrsol(krydim, Hes, s->givens, s->yg) != 0) { *ret = SPGMR_QRFACT_FAIL; return PFHIP_OK; }
     if (flex) {
       /* x += Z y directly (flexible GMRES) */
-      CHK(pfhip_vec_combine(pb, (const double *const *)s->Z, krydim, s->yg, x, NULL));
+      CHK(s->ops->vec_combine(pb, (const double *const *)s->Z, krydim, s->yg, x, NULL));
     } else {
       for (int k = 0; k < krydim; k++)
-        CHK(pfhip_vec_linear_sum(pb, s->yg[k], V[k], 1.0, s->xcor, s->xcor, NULL));
+        CHK(s->ops->vec_linear_sum(pb, s->yg[k], V[k], 1.0, s->xcor, s->xcor, NULL));
     }
     if (converged) {
       if (!flex) {
         if (preOnRight) CHK(psolve(s, s->xcor, s->vtemp));
-        else CHK(pfhip_vec_scale(pb, 1.0, s->xcor, s->vtemp, NULL));
-        CHK(pfhip_vec_linear_sum(pb, 1.0, x, 1.0, s->vtemp, x, NULL));
+        else CHK(s->ops->vec_scale(pb, 1.0, s->xcor, s->vtemp, NULL));
+        CHK(s->ops->vec_linear_sum(pb, 1.0, x, 1.0, s->vtemp, x, NULL));
       }
       *ret = SPGMR_SUCCESS;
       return PFHIP_OK;
@@ -358,15 +373,15 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double de
     r_norm *= s_product;
     for (int i = 0; i <= krydim; i++) s->yg[i] *= r_norm;
     r_norm = fabs(r_norm);
-    CHK(pfhip_vec_scale(pb, s->yg[0], V[0], V[0], NULL));
+    CHK(s->ops->vec_scale(pb, s->yg[0], V[0], V[0], NULL));
     for (int k = 1; k <= krydim; k++)
-      CHK(pfhip_vec_linear_sum(pb, s->yg[k], V[k], 1.0, V[0], V[0], NULL));
+      CHK(s->ops->vec_linear_sum(pb, s->yg[k], V[k], 1.0, V[0], V[0], NULL));
   }
   if (rho < beta) {
     if (!flex) {
       if (preOnRight) CHK(psolve(s, s->xcor, s->vtemp));
-      else CHK(pfhip_vec_scale(pb, 1.0, s->xcor, s->vtemp, NULL));
-      CHK(pfhip_vec_linear_sum(pb, 1.0, x, 1.0, s->vtemp, x, NULL));
+      else CHK(s->ops->vec_scale(pb, 1.0, s->xcor, s->vtemp, NULL));
+      CHK(s->ops->vec_linear_sum(pb, 1.0, x, 1.0, s->vtemp, x, NULL));
     }
     *ret = SPGMR_RES_REDUCED;
     return PFHIP_OK;
@@ -382,14 +397,14 @@ static int lsolve(pfhip_solver *s, double *xx, double *bb, int *lret)
   pfhip_problem *pb = s->pb;
   int nli_inc, sret;
   double res_norm;
-  CHK(pfhip_vec_const(pb, 0.0, xx, NULL));
+  CHK(s->ops->vec_const(pb, 0.0, xx, NULL));
   s->new_uu = 1;
   CHK(spgmr_solve(s, xx, s->fval, s->eps, &res_norm, &nli_inc, &sret));
   s->nli += nli_inc;
   if (sret != 0) s->ncfl++;
   CHK(atimes(s, xx, bb));
   CHK(l2norm(s, bb, &s->sJpnorm));
-  CHK(pfhip_vec_dot(pb, s->fval, bb, &s->sfdotJp, NULL));
+  CHK(s->ops->vec_dot(pb, s->fval, bb, &s->sfdotJp, NULL));
   *lret = (sret == SPGMR_SUCCESS || sret == SPGMR_RES_REDUCED) ? 0 : 8;
   return PFHIP_OK;
 }
@@ -415,10 +430,10 @@ static int eval_unew(pfhip_solver *s, double rl, double *fnormp, double *f1normp
   if (!s->pb->overland) {
     /* unew = uu + rl pp formed inside the residual launch */
     s->nfe++;
-    CHK(pfhip_residual_step(s->pb, s->uu, s->pp, rl, s->unew, s->fval, s->cur_p_old,
+    CHK(s->ops->residual_step(s->pb, s->uu, s->pp, rl, s->unew, s->fval, s->cur_p_old,
                             s->cur_s_old, s->dt, NULL));
   } else {
-    CHK(pfhip_vec_linear_sum(s->pb, 1.0, s->uu, rl, s->pp, s->unew, NULL));
+    CHK(s->ops->vec_linear_sum(s->pb, 1.0, s->uu, rl, s->pp, s->unew, NULL));
     CHK(func(s, s->unew, s->fval));
   }
   CHK(l2norm(s, s->fval, fnormp));
@@ -439,12 +454,12 @@ static int linesearch(pfhip_solver *s, double *fnormp, double *f1normp, int *max
   CHK(l2norm(s, s->pp, &pnorm));
   if (pnorm > s->mxnewtstep) {
     ratio = s->mxnewtstep / pnorm;
-    CHK(pfhip_vec_scale(pb, ratio, s->pp, s->pp, NULL));
+    CHK(s->ops->vec_scale(pb, ratio, s->pp, s->pp, NULL));
     pnorm = s->mxnewtstep;
   }
   s->stepl = pnorm;
   slpi = s->sfdotJp * ratio;
-  CHK(pfhip_vec_scsteplength(pb, s->uu, s->pp, &rlength, NULL));
+  CHK(s->ops->vec_scsteplength(pb, s->uu, s->pp, &rlength, NULL));
   rlmin = s->scsteptol / rlength;
   rl = 1.0;
   int nfesav = s->nfe;
@@ -453,7 +468,7 @@ static int linesearch(pfhip_solver *s, double *fnormp, double *f1normp, int *max
     alpha_cond = s->f1norm + alpha * slpi * rl;
     if ((*f1normp) <= alpha_cond) break;
     if (rl < rlmin) {
-      CHK(pfhip_vec_scale(pb, 1.0, s->uu, s->unew, NULL));
+      CHK(s->ops->vec_scale(pb, 1.0, s->uu, s->unew, NULL));
       CHK(func(s, s->unew, s->fval));
       CHK(l2norm(s, s->fval, fnormp));
       *f1normp = 0.5 * (*fnormp) * (*fnormp);
@@ -515,7 +530,7 @@ static int inexact_newton(pfhip_solver *s, double *fnormp, double *f1normp, int 
   CHK(l2norm(s, s->pp, &pnorm));
   if (pnorm > s->mxnewtstep) {
     ratio = s->mxnewtstep / pnorm;
-    CHK(pfhip_vec_scale(s->pb, ratio, s->pp, s->pp, NULL));
+    CHK(s->ops->vec_scale(s->pb, ratio, s->pp, s->pp, NULL));
     pnorm = s->mxnewtstep;
   }
   s->stepl = pnorm;
@@ -558,7 +573,7 @@ static int kin_stop(pfhip_solver *s, int maxStepTaken, int gret, int *ret)
   }
   double fmax, rlength;
   /* max|fval| and the scaled step length from one pass */
-  CHK(pfhip_kin_stop_norms(s->pb, s->fval, s->unew, s->uu, &fmax, &rlength, NULL));
+  CHK(s->ops->kin_stop_norms(s->pb, s->fval, s->unew, s->uu, &fmax, &rlength, NULL));
   if (fmax <= s->fnormtol) { *ret = 1; return PFHIP_OK; }
   if (rlength <= s->scsteptol) {
     if (!s->precondcurrent) { s->pthrsh = 2.0; return PFHIP_OK; }
@@ -587,7 +602,7 @@ int pfhip_kinsol_nonlin_solver(pfhip_solver *s, double *p, const double *p_old,
 {
   pfhip_problem *pb = s->pb;
   const pfhip_solver_opts *o = &s->o;
-  CHK(pfhip_bc_pressure(pb, time, NULL));
+  CHK(s->ops->bc_pressure(pb, time, NULL));
   s->uu = p;
   s->cur_p_old = p_old;
   s->cur_s_old = s_old;
@@ -628,10 +643,10 @@ int pfhip_kinsol_nonlin_solver(pfhip_solver *s, double *p, const double *p_old,
 
   int ret = 0;
   /* KINInitialStop */
-  CHK(pfhip_vec_scale(pb, 1.0, p, s->unew, NULL));
+  CHK(s->ops->vec_scale(pb, 1.0, p, s->unew, NULL));
   CHK(func(s, p, s->fval));
   double fmax;
-  CHK(pfhip_vec_max_norm(pb, s->fval, &fmax, NULL));
+  CHK(s->ops->vec_max_norm(pb, s->fval, &fmax, NULL));
   if (fmax <= 0.01 * s->fnormtol) {
     ret = 2;
   } else {
@@ -653,12 +668,12 @@ int pfhip_kinsol_nonlin_solver(pfhip_solver *s, double *p, const double *p_old,
       if (s->callForcingTerm) forcing_term(s, fnormp);
       s->fnorm = fnormp;
       CHK(kin_stop(s, maxStep, gret, &ret));
-      CHK(pfhip_vec_scale(pb, 1.0, s->unew, s->uu, NULL));
+      CHK(s->ops->vec_scale(pb, 1.0, s->unew, s->uu, NULL));
       s->f1norm = f1normp;
       if (ret != 0) break;
     }
   }
-  CHK(pfhip_synchronize());
+  CHK(s->ops->synchronize());
   if (st) {
     st->ret = ret;
     st->nni = s->nni; st->nli = s->nli; st->nfe = s->nfe; st->npe = s->npe;
@@ -678,19 +693,19 @@ int pfhip_advance_richards(pfhip_solver *s, double *p, double *sat, double t0,
   double t = t0;
   long bytes = s->ng * (long)sizeof(double);
   if (steps_done) *steps_done = 0;
-  CHK(pfhip_saturation(pb, p, sat, PFHIP_CALCFCN, NULL));
+  CHK(s->ops->saturation(pb, p, sat, PFHIP_CALCFCN, NULL));
   for (int n = 0; n < nsteps; n++) {
     int converged = 1, failures = 0, tn = 0, tl = 0, tf = 0;
     double dtc = dt;
     do {
       if (converged) {
-        CHK(pfhip_vec_scale(pb, 1.0, sat, s->s_old, NULL));
-        CHK(pfhip_vec_scale(pb, 1.0, p, s->p_old, NULL));
+        CHK(s->ops->vec_scale(pb, 1.0, sat, s->s_old, NULL));
+        CHK(s->ops->vec_scale(pb, 1.0, p, s->p_old, NULL));
       } else {
         t = t - dtc;
         dtc = 0.5 * dtc;
-        CHK(pfhip_vec_scale(pb, 1.0, s->s_old, sat, NULL));
-        CHK(pfhip_vec_scale(pb, 1.0, s->p_old, p, NULL));
+        CHK(s->ops->vec_scale(pb, 1.0, s->s_old, sat, NULL));
+        CHK(s->ops->vec_scale(pb, 1.0, s->p_old, p, NULL));
       }
       t += dtc;
       pfhip_kin_stats st;
@@ -704,7 +719,7 @@ int pfhip_advance_richards(pfhip_solver *s, double *p, double *sat, double t0,
       pfhip_set_error("time step %d failed after %d convergence failures", n, failures);
       return PFHIP_ERR_SOLVER;
     }
-    CHK(pfhip_saturation(pb, p, sat, PFHIP_CALCFCN, NULL));
+    CHK(s->ops->saturation(pb, p, sat, PFHIP_CALCFCN, NULL));
     if (newton) newton[n] = tn;
     if (gmres) gmres[n] = tl;
     if (fevals) fevals[n] = tf;
@@ -712,5 +727,5 @@ int pfhip_advance_richards(pfhip_solver *s, double *p, double *sat, double t0,
     if (steps_done) *steps_done = n + 1;
   }
   (void)bytes;
-  return pfhip_synchronize();
+  return s->ops->synchronize();
 }

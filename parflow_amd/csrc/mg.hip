@@ -678,11 +678,11 @@ __device__ void rbgs_block(const LevelDev<T> &L, int zero, int max_iter, int sym
   }
 }
 
-// bottom of the V-cycle: levels lc .. L-1 (b[lc] given, x[lc] returned)
+// V-cycle over levels lc .. L-1 inside one workgroup (b[lc] given, x[lc]
+// returned)
 template <typename T>
-__global__ __launch_bounds__(CWG) void k_coarse_vcycle(const LevelDev<T> *__restrict__ lev,
-                                                       int lc, int L, int pre, int post,
-                                                       int symmetric)
+__device__ void vcycle_block(const LevelDev<T> *__restrict__ lev, int lc, int L, int pre,
+                             int post, int symmetric)
 {
   int l;
   for (l = lc; l <= L - 2; l++) {
@@ -697,6 +697,15 @@ __global__ __launch_bounds__(CWG) void k_coarse_vcycle(const LevelDev<T> *__rest
     FOR_POINTS(F.g, prolong_point(F.g, Cc.g, F.dir, Cc.x, F.x, F.p1, F.p2, i, j, k));
     rbgs_block(F, 0, post, symmetric);
   }
+}
+
+// bottom of the V-cycle: levels lc .. L-1 (b[lc] given, x[lc] returned)
+template <typename T>
+__global__ __launch_bounds__(CWG) void k_coarse_vcycle(const LevelDev<T> *__restrict__ lev,
+                                                       int lc, int L, int pre, int post,
+                                                       int symmetric)
+{
+  vcycle_block(lev, lc, L, pre, post, symmetric);
 }
 
 // coarse operators of levels ls .. L-2 -> ls+1 .. L-1
@@ -1072,4 +1081,92 @@ extern "C" int pfhip_mg_apply(pfhip_mg *mg, const double *b, double *x, void *st
 {
   return (mg->prec == 64) ? mg_apply_t<double>(mg, b, x, STREAM(stream))
                           : mg_apply_t<float>(mg, b, x, STREAM(stream));
+}
+
+// ---- ensemble: one workgroup per member ---------------------------------------
+// A member's whole hierarchy (every level, not only the small ones) runs in
+// its workgroup: ensemble members are small, and one launch then covers the
+// coarse-operator setup or the V-cycle of every member with a pending request.
+template <typename T> struct EnsMG {
+  const LevelDev<T> *lev;
+  int L, pre, post, symmetric;
+  const double *bin;
+  double *xout;
+};
+
+template <typename T>
+__global__ __launch_bounds__(CWG) void k_ens_mg_coarse(const EnsMG<T> *__restrict__ E)
+{
+  const EnsMG<T> &M = E[blockIdx.x];
+  for (int l = 0; l <= M.L - 2; l++) {
+    const LevelDev<T> &F = M.lev[l], &Cc = M.lev[l + 1];
+    FOR_POINTS(Cc.g, coarse_point(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1, F.p2, i, j, k));
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(CWG) void k_ens_vcycle(const EnsMG<T> *__restrict__ E)
+{
+  const EnsMG<T> &M = E[blockIdx.x];
+  const LevelDev<T> &L0 = M.lev[0];
+  FOR_POINTS(L0.g, L0.b[LIDX(L0.g, i, j, k)] = (T)M.bin[LIDX(L0.g, i, j, k)]);
+  vcycle_block(M.lev, 0, M.L, M.pre, M.post, M.symmetric);
+  FOR_POINTS(L0.g, M.xout[LIDX(L0.g, i, j, k)] = (double)L0.x[LIDX(L0.g, i, j, k)]);
+}
+
+extern "C" void pfhip_mg_level0(pfhip_mg *mg, void **A, int *prec)
+{
+  for (int q = 0; q < 4; q++) A[q] = mg->A[0][q];
+  *prec = mg->prec;
+}
+
+template <typename T>
+static int ens_mg_launch(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st, int apply,
+                         hipStream_t s)
+{
+  EnsMG<T> *T_ = (EnsMG<T> *)calloc(n, sizeof(EnsMG<T>));
+  if (!T_) { pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
+  for (int r = 0; r < n; r++) {
+    pfhip_mg *mg = rq[r]->mg;
+    T_[r].lev = (const LevelDev<T> *)mg->levdev;
+    T_[r].L = mg->L;
+    T_[r].pre = mg->pre; T_[r].post = mg->post; T_[r].symmetric = mg->symmetric;
+    T_[r].bin = rq[r]->x;
+    T_[r].xout = rq[r]->z;
+  }
+  void *dT = pfhip_ens_upload(st, T_, (long)n * sizeof(EnsMG<T>), s);
+  free(T_);
+  if (!dT) return PFHIP_ERR_HIP;
+  if (apply)
+    hipLaunchKernelGGL(k_ens_vcycle<T>, dim3(n), dim3(CWG), 0, s, (const EnsMG<T> *)dT);
+  else
+    hipLaunchKernelGGL(k_ens_mg_coarse<T>, dim3(n), dim3(CWG), 0, s, (const EnsMG<T> *)dT);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+static int ens_mg_exec(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st, int apply,
+                       void *stream)
+{
+  if (n <= 0) return PFHIP_OK;
+  int prec = rq[0]->mg->prec;
+  for (int r = 1; r < n; r++)
+    if (rq[r]->mg->prec != prec) {
+      pfhip_set_error("ensemble members need one multigrid precision");
+      return PFHIP_ERR_ARG;
+    }
+  return (prec == 32) ? ens_mg_launch<float>(rq, n, st, apply, STREAM(stream))
+                      : ens_mg_launch<double>(rq, n, st, apply, STREAM(stream));
+}
+
+extern "C" int pfhip_ens_exec_mg_coarse(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
+                                        void *stream)
+{
+  return ens_mg_exec(rq, n, st, 0, stream);
+}
+
+extern "C" int pfhip_ens_exec_mg_apply(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
+                                       void *stream)
+{
+  return ens_mg_exec(rq, n, st, 1, stream);
 }

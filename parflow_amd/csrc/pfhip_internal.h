@@ -84,6 +84,121 @@ int pfhip_kin_stop_norms(pfhip_problem *pb, const double *fval, const double *un
 int pfhip_dev_alloc(double **p, long n);
 int pfhip_zero(double *p, long n, void *stream);
 
+/* BCPressure values of time `time` into the device tables; sync = 0 leaves
+ * the host-to-device copies in flight (the caller guarantees a stream
+ * synchronisation before the next call on the same problem) */
+int pfhip_bc_pressure_async(pfhip_problem *pb, double time, int sync, void *stream);
+
+/* ---- device operations of the nonlinear driver ------------------------
+ * The driver (kinsol_host.c) reaches the device only through this table:
+ * the direct table calls the entry points above; the ensemble's table
+ * turns each call into a request of a batched launch. */
+typedef struct pfhip_ops {
+  int (*vec_scale)(pfhip_problem *, double, const double *, double *, void *);
+  int (*vec_linear_sum)(pfhip_problem *, double, const double *, double, const double *,
+                        double *, void *);
+  int (*vec_const)(pfhip_problem *, double, double *, void *);
+  int (*vec_dot)(pfhip_problem *, const double *, const double *, double *, void *);
+  int (*vec_max_norm)(pfhip_problem *, const double *, double *, void *);
+  int (*vec_scsteplength)(pfhip_problem *, const double *, const double *, double *, void *);
+  int (*vec_dq_sums)(pfhip_problem *, const double *, const double *, double *, double *,
+                     double *, void *);
+  int (*vec_multi_dot)(pfhip_problem *, const double *const *, int, const double *,
+                       double *, void *);
+  int (*vec_multi_axpy)(pfhip_problem *, const double *const *, int, const double *,
+                        double *, double *, void *);
+  int (*vec_combine)(pfhip_problem *, const double *const *, int, const double *, double *,
+                     void *);
+  int (*kin_stop_norms)(pfhip_problem *, const double *, const double *, const double *,
+                        double *, double *, void *);
+  int (*nl_function_eval)(pfhip_problem *, const double *, double *, const double *,
+                          const double *, double, void *);
+  int (*residual_jv)(pfhip_problem *, const double *, const double *, double, const double *,
+                     double, double *, const double *, const double *, double, void *);
+  int (*residual_step)(pfhip_problem *, const double *, const double *, double, double *,
+                       double *, const double *, const double *, double, void *);
+  int (*mg_setup_jacobian)(pfhip_mg *, const double *, double, void *);
+  int (*mg_apply)(pfhip_mg *, const double *, double *, void *);
+  int (*bc_pressure)(pfhip_problem *, double, void *);
+  int (*saturation)(pfhip_problem *, const double *, double *, int, void *);
+  int (*synchronize)(void);
+} pfhip_ops;
+
+extern const pfhip_ops pfhip_direct_ops;
+void pfhip_solver_set_ops(pfhip_solver *s, const pfhip_ops *ops);
+pfhip_problem *pfhip_solver_problem(pfhip_solver *s);
+pfhip_mg *pfhip_solver_mg(pfhip_solver *s);
+const pfhip_solver_opts *pfhip_solver_options(pfhip_solver *s);
+
+/* ---- ensemble (many independent problems per launch) ------------------ */
+enum {
+  PFHIP_EOP_NONE = 0,
+  /* vector operations: one workgroup per member, one launch for all */
+  PFHIP_EOP_LIN,        /* z = [z] + sum_q c_q x_q (optionally ||z||^2) */
+  PFHIP_EOP_DOT,        /* x . y                                      */
+  PFHIP_EOP_MAXABS,     /* max |x|                                    */
+  PFHIP_EOP_SCSTEP,     /* max |y| / (1 + |x|)                         */
+  PFHIP_EOP_DQ,         /* x . y, y . y, |y|_1                         */
+  PFHIP_EOP_MDOT,       /* V_q . y, q < m                              */
+  PFHIP_EOP_STOP,       /* max |x|, max |y - u| / (1 + |y|)            */
+  /* residual forms: mode 0 F(x), 1 J.v quotient, 2 line-search point */
+  PFHIP_EOP_RESIDUAL,
+  PFHIP_EOP_MG_SETUP,   /* symmetric Jacobian into level 0 + coarse ops */
+  PFHIP_EOP_MG_APPLY,   /* one V-cycle                                 */
+  PFHIP_EOP_COUNT
+};
+
+#define PFHIP_ENS_MAXQ 33       /* terms of one EOP_LIN request          */
+#define PFHIP_ENS_NRES 32       /* result slots per request              */
+
+typedef struct pfhip_ens_req {
+  int op;
+  int mode;                     /* residual form                         */
+  int base;                     /* EOP_LIN: z itself is the first term   */
+  int norm;                     /* EOP_LIN: also ||z||^2                 */
+  int nq;                       /* EOP_LIN terms / EOP_MDOT vectors      */
+  pfhip_problem *pb;
+  pfhip_mg *mg;
+  double cst, sig, fsc, dt;
+  double c[PFHIP_ENS_MAXQ];
+  const double *xq[PFHIP_ENS_MAXQ];
+  const double *x, *y, *u;
+  double *z, *z2;
+  double *scratch[4];           /* member scratch (ghosted, ng doubles)  */
+  double *out[PFHIP_ENS_NRES];  /* host destinations of the results      */
+  int nout;
+  int rc;
+} pfhip_ens_req;
+
+/* staging ring for the per-launch member tables: pinned host memory copied
+ * asynchronously to its device twin; off returns to 0 after a stream
+ * synchronisation */
+typedef struct pfhip_ens_stage {
+  char *host, *dev;
+  long off, cap;
+} pfhip_ens_stage;
+
+void *pfhip_ens_upload(pfhip_ens_stage *st, const void *src, long bytes, void *stream);
+/* batched launches; results of request r land in res[r * PFHIP_ENS_NRES + v] */
+int pfhip_ens_exec_vec(pfhip_ens_req *const *rq, int n, double *res, pfhip_ens_stage *st,
+                       void *stream);
+int pfhip_ens_exec_residual(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
+                            void *stream);
+int pfhip_ens_exec_jacobian(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
+                            void *stream);
+int pfhip_ens_exec_mg_coarse(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
+                             void *stream);
+int pfhip_ens_exec_mg_apply(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
+                            void *stream);
+/* level-0 matrix arrays (C, E, N, U) and precision of a multigrid */
+void pfhip_mg_level0(pfhip_mg *mg, void **A, int *prec);
+int pfhip_ens_stage_create(pfhip_ens_stage *st, long cap);
+void pfhip_ens_stage_destroy(pfhip_ens_stage *st);
+/* n doubles device -> pinned host, then a stream synchronisation */
+int pfhip_ens_fetch(double *host, const double *dev, long n, void *stream);
+int pfhip_host_alloc(double **p, long n);
+void pfhip_host_free(double *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,13 @@ c2  512x512x64 (16.8M cells), the c1 recipe with a new seed; flexible
 c3  512x512x32, dx = dy = 10 m, dz = 0.5 m, tilted-V catchment with
     kinematic overland flow, rain 0.05 m/h for 1 h then 4 h recession,
     dt = 0.05 h.
+c4  an ensemble of independent hillslopes (``c4_members``): nx, ny
+    log-uniform in [8, 64], nz log-uniform in [20, 50], dx = dy = 1 m,
+    dz = 0.25 m; per member Ksat lognormal (median 0.05 m/h, sigma 1),
+    alpha lognormal (median 1.5 1/m, sigma 0.4), n uniform in [1.5, 3],
+    surface slope uniform in [0.005, 0.1] (the initial water table is tilted
+    along x by it), top flux uniform in [-0.02, 0] m/h; 24 steps of 0.25 h
+    with the mixed solver of c2.
 
 Every field is generated from a fixed seed so a CPU run and a GPU run of
 the same deck see bit-identical inputs.
@@ -129,3 +136,43 @@ def deck(name: str, seed: int = None):
         p0 = prob.hydrostatic(0.0, WATER_TABLE)
         return prob, opts, p0, 0.05, 100, {"seed": seed}
     raise ValueError(f"unknown deck {name!r}")
+
+
+def c4_member(m: int, seed: int = 404):
+    """Member m of the hillslope ensemble: (Problem, SolverOptions, p0, dt).
+
+    Each member draws from its own generator (seed, m).  The water table
+    starts at mid-depth at the centre of the slope and tilts along x with
+    the member's surface slope (higher upslope, clipped to 10-90 % of the
+    depth); the bottom holds the hydrostatic head of the mid-depth table."""
+    rng = np.random.default_rng([seed, m])
+    nx = int(round(np.exp(rng.uniform(np.log(8), np.log(64)))))
+    ny = int(round(np.exp(rng.uniform(np.log(8), np.log(64)))))
+    nz = int(round(np.exp(rng.uniform(np.log(20), np.log(50)))))
+    ksat = float(np.exp(rng.normal(np.log(0.05), 1.0)))
+    alpha = float(np.exp(rng.normal(np.log(1.5), 0.4)))
+    nvg = float(rng.uniform(1.5, 3.0))
+    slope = float(rng.uniform(0.005, 0.1))
+    qtop = float(rng.uniform(-0.02, 0.0))
+    dx = dy = 1.0
+    dz = 0.25
+    depth = nz * dz
+    wt0 = 0.5 * depth
+    bc = [BCSpec(BC_FLUX, (0.0,)) for _ in range(4)]
+    bc.append(BCSpec(BC_DIR_EQUIL, (0.0,), (1,), -1, z_ref=wt0))
+    bc.append(BCSpec(BC_FLUX, (qtop,)))
+    prob = Problem(nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, kx=ksat, poro=0.4, ss=1e-4,
+                   alpha=alpha, nvg=nvg, sres=0.2, ssat=1.0, bc=bc, name=f"c4m{m}")
+    opts = SolverOptions(krylov_dim=30, precond="PFMG", num_pre_relax=2, num_post_relax=2)
+    xc = (np.arange(nx) + 0.5) * dx
+    zwt = np.clip(wt0 + slope * (0.5 * nx * dx - xc), 0.1 * depth, 0.9 * depth)
+    zc = prob.z0 + (np.arange(nz) + 0.5) * dz
+    p0 = prob.zeros()
+    p0[1:-1, 1:-1, 1:-1] = (-prob.rho * prob.gravity *
+                            (zc[:, None, None] - zwt[None, None, :]))
+    return prob, opts, p0, 0.25
+
+
+def c4_members(n: int = 512, seed: int = 404):
+    """The first n members of the hillslope ensemble."""
+    return [c4_member(m, seed) for m in range(n)]

@@ -348,3 +348,46 @@ def test_mixed_precision_close_to_plain(gpu):
         a, b = _I(prob, pm[s]), _I(prob, pp[s])
         e = np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3))
         assert e < 1e-8, (s, e)
+
+
+def test_ensemble_matches_one_at_a_time(gpu):
+    """Hillslope ensemble members advanced together (batched launches, one
+    coroutine per member) against the same members advanced one at a time
+    on the single-problem mixed path, and the smallest member against the
+    oracle's plain FP64 solve (the mixed-vs-plain bar)."""
+    api = _api()
+    from parflow_amd.decks import c4_member
+    nm, nsteps = 6, 3
+    members = [c4_member(m) for m in range(nm)]
+    for m, (prob, _, _, _) in enumerate(members):
+        print(f"member {m}: {prob.nx}x{prob.ny}x{prob.nz}", flush=True)
+    ens = api.GPUEnsemble([(p, o) for p, o, _, _ in members])
+    dps = [ens.problems[i].vec(members[i][2]) for i in range(nm)]
+    dss = [ens.problems[i].vec() for i in range(nm)]
+    print("ensemble advance", flush=True)
+    lg = ens.advance_device(dps, dss, 0.0, [m[3] for m in members], nsteps)
+    rounds, launches = ens.stats()
+    print("rounds", rounds, "launches", launches, "newton", lg["newton"].tolist(), flush=True)
+    assert np.all(lg["steps"] == nsteps)
+    for i, (prob, opts, p0, dt) in enumerate(members):
+        gp = api.GPUProblem(prob)
+        sv = api.GPUSolver(gp, opts, 32, 1)
+        pg, sg, l1 = sv.advance(p0, 0.0, dt, nsteps, history=False)
+        pe = _I(prob, ens.problems[i].get(dps[i]))
+        se = _I(prob, ens.problems[i].get(dss[i]))
+        e = np.max(np.abs(pe - _I(prob, pg)) / np.maximum(np.abs(_I(prob, pg)), 1e-3))
+        es = _relmax(se, _I(prob, sg))
+        print(f"member {i}: rel diff p {e:.3e} S {es:.3e}", flush=True)
+        assert e < 1e-10 and es < 1e-10, (i, e, es)
+        assert np.all(np.abs(lg["newton"][i] - l1["newton"]) <= 1), (i, lg["newton"][i],
+                                                                      l1["newton"])
+    small = min(range(nm), key=lambda i: members[i][0].ncells)
+    prob, opts, p0, dt = members[small]
+    op = O.OracleProblem(prob)
+    r, pho, sho, logo = O.advance(op, opts, p0, 0.0, dt, nsteps)
+    assert r == 0
+    pe = _I(prob, ens.problems[small].get(dps[small]))
+    b = _I(prob, pho[nsteps])
+    e = np.max(np.abs(pe - b) / np.maximum(np.abs(b), 1e-3))
+    print(f"member {small} vs oracle plain: {e:.3e}", flush=True)
+    assert e < 1e-8
