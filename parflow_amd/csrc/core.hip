@@ -2424,7 +2424,6 @@ extern "C" int pfhip_matvec(pfhip_problem *pb, const double *const *A, int symm,
 // by two orders of magnitude, which the flat grids balance and the
 // one-workgroup reductions keep deterministic.
 // ==========================================================================
-#define EWG 512
 
 __device__ __forceinline__ int cell_ip(const Geom &g, int c, int &i, int &j, int &k)
 {
@@ -2435,30 +2434,15 @@ __device__ __forceinline__ int cell_ip(const Geom &g, int c, int &i, int &j, int
   return (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
 }
 
-__device__ __forceinline__ double ens_block_sum(double v, double *sh)
+// the member owning flat block b: last e with starts[e] <= b
+__device__ __forceinline__ int ens_find(const int *__restrict__ starts, int n, int b)
 {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double r = sh[0];
-#pragma unroll
-  for (int q = 1; q < EWG / 64; q++) r += sh[q];
-  return r;
-}
-
-__device__ __forceinline__ double ens_block_max(double v, double *sh)
-{
-  v = wave_max(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double r = sh[0];
-#pragma unroll
-  for (int q = 1; q < EWG / 64; q++) r = fmax(r, sh[q]);
-  return r;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
 }
 
 // one member's vector operation (PFHIP_EOP_LIN .. PFHIP_EOP_STOP)
@@ -2470,24 +2454,51 @@ struct EnsVec {
   const double *xq[PFHIP_ENS_MAXQ];
   const double *x, *y, *u;
   double *z;
+  int ncell;
 };
 
-#define ENS_MDOT 32
-__global__ __launch_bounds__(EWG) void k_ens_vec(const EnsVec *__restrict__ T,
-                                                 double *__restrict__ res)
+// Vector operations as a flat grid: block b takes EVC consecutive interior
+// cells of one member (EVB threads, EVC / EVB cells each, a fixed mapping)
+// and writes its partial results to part[b * PFHIP_ENS_NRES + v]; the
+// finishing kernel then adds (or maxes) each member's block partials in
+// block order, so the results are reproducible bit for bit.
+#define EVB 256
+#define EVC 2048
+
+__device__ __forceinline__ bool ens_is_max(int op)
 {
-  __shared__ double sh[EWG / 64];
-  const EnsVec &E = T[blockIdx.x];
+  return op == PFHIP_EOP_MAXABS || op == PFHIP_EOP_SCSTEP || op == PFHIP_EOP_STOP;
+}
+
+__device__ __forceinline__ int ens_nres(const EnsVec &E)
+{
+  switch (E.op) {
+    case PFHIP_EOP_LIN: return E.norm ? 1 : 0;
+    case PFHIP_EOP_DQ: return 3;
+    case PFHIP_EOP_STOP: return 2;
+    case PFHIP_EOP_MDOT: return E.nq;
+    default: return 1;
+  }
+}
+
+#define ENS_MDOT 32
+__global__ __launch_bounds__(EVB) void k_ens_vec(const EnsVec *__restrict__ T,
+                                                 const int *__restrict__ starts, int n,
+                                                 double *__restrict__ part)
+{
+  const int e = ens_find(starts, n, blockIdx.x);
+  const EnsVec &E = T[e];
   const Geom g = E.g;
   const int op = E.op, nq = E.nq;
-  const int n = g.nx * g.ny * g.nz;
-  double *R = res + (long)blockIdx.x * PFHIP_ENS_NRES;
+  const int c0 = (blockIdx.x - starts[e]) * EVC;
+  const int c1 = min(c0 + EVC, E.ncell);
+  double *P = part + (long)blockIdx.x * PFHIP_ENS_NRES;
   int i, j, k;
   if (op == PFHIP_EOP_LIN) {
     // z may alias an input: every cell is read before it is written, by
     // the same thread
     double acc = 0.0;
-    for (int c = threadIdx.x; c < n; c += EWG) {
+    for (int c = c0 + threadIdx.x; c < c1; c += EVB) {
       const int ip = cell_ip(g, c, i, j, k);
       double s;
       int q0;
@@ -2499,8 +2510,8 @@ __global__ __launch_bounds__(EWG) void k_ens_vec(const EnsVec *__restrict__ T,
       acc += s * s;
     }
     if (E.norm) {
-      acc = ens_block_sum(acc, sh);
-      if (threadIdx.x == 0) R[0] = acc;
+      acc = block_reduce<false>(acc);
+      if (threadIdx.x == 0) P[0] = acc;
     }
     return;
   }
@@ -2508,7 +2519,7 @@ __global__ __launch_bounds__(EWG) void k_ens_vec(const EnsVec *__restrict__ T,
     double acc[ENS_MDOT];
 #pragma unroll
     for (int q = 0; q < ENS_MDOT; q++) acc[q] = 0.0;
-    for (int c = threadIdx.x; c < n; c += EWG) {
+    for (int c = c0 + threadIdx.x; c < c1; c += EVB) {
       const int ip = cell_ip(g, c, i, j, k);
       const double w = E.y[ip];
 #pragma unroll
@@ -2518,14 +2529,14 @@ __global__ __launch_bounds__(EWG) void k_ens_vec(const EnsVec *__restrict__ T,
 #pragma unroll
     for (int q = 0; q < ENS_MDOT; q++) {
       if (q < nq) {
-        double s = ens_block_sum(acc[q], sh);
-        if (threadIdx.x == 0) R[q] = s;
+        double s = block_reduce<false>(acc[q]);
+        if (threadIdx.x == 0) P[q] = s;
       }
     }
     return;
   }
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  for (int c = threadIdx.x; c < n; c += EWG) {
+  for (int c = c0 + threadIdx.x; c < c1; c += EVB) {
     const int ip = cell_ip(g, c, i, j, k);
     switch (op) {
       case PFHIP_EOP_DOT: a0 += E.x[ip] * E.y[ip]; break;
@@ -2546,11 +2557,33 @@ __global__ __launch_bounds__(EWG) void k_ens_vec(const EnsVec *__restrict__ T,
       }
     }
   }
-  if (op == PFHIP_EOP_DOT || op == PFHIP_EOP_DQ) a0 = ens_block_sum(a0, sh);
-  else a0 = ens_block_max(a0, sh);
-  if (op == PFHIP_EOP_DQ) { a1 = ens_block_sum(a1, sh); a2 = ens_block_sum(a2, sh); }
-  if (op == PFHIP_EOP_STOP) a1 = ens_block_max(a1, sh);
-  if (threadIdx.x == 0) { R[0] = a0; R[1] = a1; R[2] = a2; }
+  const bool mx = ens_is_max(op);
+  a0 = mx ? block_reduce<true>(a0) : block_reduce<false>(a0);
+  if (op == PFHIP_EOP_DQ) { a1 = block_reduce<false>(a1); a2 = block_reduce<false>(a2); }
+  if (op == PFHIP_EOP_STOP) a1 = block_reduce<true>(a1);
+  if (threadIdx.x == 0) { P[0] = a0; P[1] = a1; P[2] = a2; }
+}
+
+// one block per request: its member's block partials in block order
+__global__ __launch_bounds__(EVB) void k_ens_vec_final(const EnsVec *__restrict__ T,
+                                                       const int *__restrict__ starts,
+                                                       const double *__restrict__ part,
+                                                       double *__restrict__ res)
+{
+  const int e = blockIdx.x;
+  const EnsVec &E = T[e];
+  const int nv = ens_nres(E);
+  const bool mx = ens_is_max(E.op);
+  const int b0 = starts[e], b1 = starts[e + 1];
+  for (int v = 0; v < nv; v++) {
+    double a = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += EVB) {
+      const double t = part[(long)b * PFHIP_ENS_NRES + v];
+      a = mx ? fmax(a, t) : a + t;
+    }
+    a = mx ? block_reduce<true>(a) : block_reduce<false>(a);
+    if (threadIdx.x == 0) res[(long)e * PFHIP_ENS_NRES + v] = a;
+  }
 }
 
 extern "C" void *pfhip_ens_upload(pfhip_ens_stage *st, const void *src, long bytes,
@@ -2580,12 +2613,28 @@ extern "C" void *pfhip_ens_upload(pfhip_ens_stage *st, const void *src, long byt
   return d;
 }
 
+static int *ens_starts(int n, const long *ncell, int per_block, int *total)
+{
+  int *s = (int *)malloc(sizeof(int) * (n + 1));
+  if (!s) return nullptr;
+  long b = 0;
+  for (int e = 0; e < n; e++) {
+    s[e] = (int)b;
+    b += (ncell[e] + per_block - 1) / per_block;
+  }
+  s[n] = (int)b;
+  *total = (int)b;
+  return s;
+}
+
 extern "C" int pfhip_ens_exec_vec(pfhip_ens_req *const *rq, int n, double *res,
-                                  pfhip_ens_stage *st, void *stream)
+                                  double *part, pfhip_ens_stage *st, void *stream)
 {
   if (n <= 0) return PFHIP_OK;
   EnsVec *T = (EnsVec *)calloc(n, sizeof(EnsVec));
-  if (!T) { pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
+  long *nc = (long *)malloc(sizeof(long) * n);
+  if (!T || !nc) { free(T); free(nc); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
+  int results = 0;
   for (int r = 0; r < n; r++) {
     const pfhip_ens_req *q = rq[r];
     EnsVec &E = T[r];
@@ -2593,15 +2642,30 @@ extern "C" int pfhip_ens_exec_vec(pfhip_ens_req *const *rq, int n, double *res,
     E.op = q->op; E.nq = q->nq; E.base = q->base; E.norm = q->norm; E.cst = q->cst;
     for (int t = 0; t < q->nq && t < PFHIP_ENS_MAXQ; t++) { E.c[t] = q->c[t]; E.xq[t] = q->xq[t]; }
     E.x = q->x; E.y = q->y; E.u = q->u; E.z = q->z;
+    E.ncell = (int)q->pb->ncell;
+    nc[r] = q->pb->ncell;
+    if (q->nout) results = 1;
   }
+  int total = 0;
+  int *starts = ens_starts(n, nc, EVC, &total);
+  free(nc);
+  if (!starts) { free(T); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
   void *dT = pfhip_ens_upload(st, T, (long)n * sizeof(EnsVec), stream);
+  void *dS = dT ? pfhip_ens_upload(st, starts, (long)(n + 1) * sizeof(int), stream) : nullptr;
   free(T);
-  if (!dT) return PFHIP_ERR_HIP;
-  hipLaunchKernelGGL(k_ens_vec, dim3(n), dim3(EWG), 0, STREAM(stream), (const EnsVec *)dT,
-                     res);
+  free(starts);
+  if (!dS) return PFHIP_ERR_HIP;
+  hipLaunchKernelGGL(k_ens_vec, dim3(total), dim3(EVB), 0, STREAM(stream), (const EnsVec *)dT,
+                     (const int *)dS, n, part);
+  if (results)
+    hipLaunchKernelGGL(k_ens_vec_final, dim3(n), dim3(EVB), 0, STREAM(stream),
+                       (const EnsVec *)dT, (const int *)dS, (const double *)part, res);
   LAUNCHCHK();
   return PFHIP_OK;
 }
+
+// blocks of the flat vector grid of a member of ncell cells
+extern "C" long pfhip_ens_vec_blocks(long ncell) { return (ncell + EVC - 1) / EVC; }
 
 // ---- flat two-pass residual ------------------------------------------------
 #define ERB 256   // cells per block of the flat member grids
@@ -2611,17 +2675,6 @@ struct EnsRes {
   double *pe, *kr, *sat;   // member scratch: evaluation pressure, kr rho, S
   int mode, ncell;
 };
-
-// the member owning flat block b: last e with starts[e] <= b
-__device__ __forceinline__ int ens_find(const int *__restrict__ starts, int n, int b)
-{
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (starts[mid] <= b) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
 
 // pass 1: pressure of the form (p, or p + sig pv), S and kr rho of every cell
 __global__ __launch_bounds__(ERB) void k_ens_res_closure(const EnsRes *__restrict__ T,
@@ -2692,20 +2745,6 @@ __global__ __launch_bounds__(ERB) void k_ens_res_faces(const EnsRes *__restrict_
   }
   if (E.mode == 1) fv = A.fsc * fv - A.fsc * A.fsub[ip];
   A.F[ip] = fv;
-}
-
-static int *ens_starts(int n, const long *ncell, int per_block, int *total)
-{
-  int *s = (int *)malloc(sizeof(int) * (n + 1));
-  if (!s) return nullptr;
-  long b = 0;
-  for (int e = 0; e < n; e++) {
-    s[e] = (int)b;
-    b += (ncell[e] + per_block - 1) / per_block;
-  }
-  s[n] = (int)b;
-  *total = (int)b;
-  return s;
 }
 
 extern "C" int pfhip_ens_exec_residual(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,

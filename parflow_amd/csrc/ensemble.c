@@ -10,7 +10,8 @@
  * batched launch each (all vector operations and reductions together, all
  * residual forms, all PC setups, all V-cycles), copies the reduction
  * results back with one transfer and one synchronisation, and resumes the
- * members.  A member whose steps are done leaves the rotation: the set of
+ * members it served.  PC setups and V-cycles are held back while lighter
+ * requests are pending, which keeps the members in step.  A member whose steps are done leaves the rotation: the set of
  * live members is the convergence mask of every launch.
  *
  * Members diverge freely (different Newton and GMRES counts, line-search
@@ -68,6 +69,7 @@ typedef struct ens_member {
   void *sp;                             /* saved stack pointer             */
   void *stack;
   int live, rc;
+  int waiting;                          /* request pending, not yet served */
   pfhip_ens_req req;
   /* advance arguments and outputs */
   double *p, *sat, t0, dt;
@@ -82,6 +84,7 @@ struct pfhip_ensemble {
   void *sched_sp;
   int cur;
   double *d_res, *h_res;
+  double *d_part;                       /* block partials of vector ops    */
   pfhip_ens_stage st;
   pfhip_ens_req **lists[PFHIP_EOP_COUNT];
   int nlist[PFHIP_EOP_COUNT];
@@ -291,6 +294,7 @@ int pfhip_ensemble_destroy(pfhip_ensemble *e)
     if (m->s) pfhip_solver_set_ops(m->s, &pfhip_direct_ops);
   }
   pfhip_free(e->d_res);
+  pfhip_free(e->d_part);
   pfhip_host_free(e->h_res);
   pfhip_ens_stage_destroy(&e->st);
   for (int k = 0; k < PFHIP_EOP_COUNT; k++) free(e->lists[k]);
@@ -335,6 +339,9 @@ int pfhip_ensemble_create(int n, pfhip_solver *const *solvers, pfhip_ensemble **
       if (!rc) rc = pfhip_zero(m->scratch[q], m->pb->ng, NULL);
     }
   }
+  long blocks = 0;
+  for (int i = 0; i < n && !rc; i++) blocks += pfhip_ens_vec_blocks(e->m[i].pb->ncell);
+  if (!rc) rc = pfhip_dev_alloc(&e->d_part, blocks * PFHIP_ENS_NRES);
   if (!rc) rc = pfhip_dev_alloc(&e->d_res, (long)n * PFHIP_ENS_NRES);
   if (!rc) rc = pfhip_host_alloc(&e->h_res, (long)n * PFHIP_ENS_NRES);
   if (!rc) rc = pfhip_ens_stage_create(&e->st, ENS_STAGE);
@@ -382,15 +389,25 @@ static void set_rc(pfhip_ens_req **l, int n, int rc)
   for (int i = 0; i < n; i++) l[i]->rc = rc;
 }
 
-/* serve every pending request of this round */
+static int is_heavy(int op) { return op == PFHIP_EOP_MG_SETUP || op == PFHIP_EOP_MG_APPLY; }
+
+/* Serve the pending requests of this round.  The heavy kinds (PC setup,
+ * V-cycle: their cost is set by the largest member, not by how many members
+ * take part) wait while any light request (vector operation, residual) is
+ * pending: the members that are behind catch up through light rounds, and
+ * each heavy launch then serves nearly every member at once. */
 static int serve_round(pfhip_ensemble *e)
 {
-  int nv = 0, need = 0;
+  int nv = 0, need = 0, light = 0;
+  for (int i = 0; i < e->n; i++)
+    if (e->m[i].live && !is_heavy(e->m[i].req.op)) light = 1;
   for (int k = 0; k < PFHIP_EOP_COUNT; k++) e->nlist[k] = 0;
   for (int i = 0; i < e->n; i++) {
     ens_member *m = &e->m[i];
     if (!m->live) continue;
     pfhip_ens_req *r = &m->req;
+    if (is_heavy(r->op) == light) continue;   /* deferred: stays waiting */
+    m->waiting = 0;
     if (r->op >= PFHIP_EOP_LIN && r->op <= PFHIP_EOP_STOP) {
       e->vec[nv++] = r;
       if (r->nout) need = 1;
@@ -400,7 +417,7 @@ static int serve_round(pfhip_ensemble *e)
   }
   int rc;
   if (nv) {
-    rc = pfhip_ens_exec_vec(e->vec, nv, e->d_res, &e->st, NULL);
+    rc = pfhip_ens_exec_vec(e->vec, nv, e->d_res, e->d_part, &e->st, NULL);
     set_rc(e->vec, nv, rc);
     e->launches++;
   }
@@ -462,6 +479,7 @@ int pfhip_ensemble_advance_richards(pfhip_ensemble *e, double *const *p, double 
     if (!m->stack) m->stack = malloc(ENS_STACK);
     if (!m->stack) { rc = PFHIP_ERR_NOMEM; break; }
     m->sp = coroutine_start(m->stack, ENS_STACK);
+    m->waiting = 0;
     pfhip_solver_set_ops(m->s, &ens_ops);
     m->live = 1;
   }
@@ -476,11 +494,12 @@ int pfhip_ensemble_advance_richards(pfhip_ensemble *e, double *const *p, double 
     /* every live member runs until its next device request (or its end) */
     for (int i = 0; i < e->n; i++) {
       ens_member *m = &e->m[i];
-      if (!m->live) continue;
+      if (!m->live || m->waiting) continue;
       e->cur = i;
       m->req.op = PFHIP_EOP_NONE;
       pfhip_ens_switch(&e->sched_sp, m->sp);
       if (!m->live) live--;
+      else m->waiting = 1;
     }
     if (live == 0) break;
     int src = serve_round(e);

@@ -180,8 +180,11 @@ typedef struct pfhip_ens_stage {
 
 void *pfhip_ens_upload(pfhip_ens_stage *st, const void *src, long bytes, void *stream);
 /* batched launches; results of request r land in res[r * PFHIP_ENS_NRES + v] */
-int pfhip_ens_exec_vec(pfhip_ens_req *const *rq, int n, double *res, pfhip_ens_stage *st,
-                       void *stream);
+int pfhip_ens_exec_vec(pfhip_ens_req *const *rq, int n, double *res, double *part,
+                       pfhip_ens_stage *st, void *stream);
+/* blocks of the flat vector grid for a member of ncell cells; part needs
+ * PFHIP_ENS_NRES doubles per block */
+long pfhip_ens_vec_blocks(long ncell);
 int pfhip_ens_exec_residual(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
                             void *stream);
 int pfhip_ens_exec_jacobian(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
