@@ -2448,13 +2448,10 @@ __device__ __forceinline__ int ens_find(const int *__restrict__ starts, int n, i
 // one member's vector operation (PFHIP_EOP_LIN .. PFHIP_EOP_STOP)
 struct EnsVec {
   Geom g;
-  int op, nq, base, norm;
+  int op, nq, base, norm, ncell, qoff;   // terms at QC/QX[qoff .. qoff + nq)
   double cst;
-  double c[PFHIP_ENS_MAXQ];
-  const double *xq[PFHIP_ENS_MAXQ];
   const double *x, *y, *u;
   double *z;
-  int ncell;
 };
 
 // Vector operations as a flat grid: block b takes EVC consecutive interior
@@ -2484,10 +2481,14 @@ __device__ __forceinline__ int ens_nres(const EnsVec &E)
 #define ENS_MDOT 32
 __global__ __launch_bounds__(EVB) void k_ens_vec(const EnsVec *__restrict__ T,
                                                  const int *__restrict__ starts, int n,
+                                                 const double *__restrict__ QC,
+                                                 const double *const *__restrict__ QX,
                                                  double *__restrict__ part)
 {
   const int e = ens_find(starts, n, blockIdx.x);
   const EnsVec &E = T[e];
+  const double *__restrict__ Ec = QC + E.qoff;
+  const double *const *__restrict__ Ex = QX + E.qoff;
   const Geom g = E.g;
   const int op = E.op, nq = E.nq;
   const int c0 = (blockIdx.x - starts[e]) * EVC;
@@ -2503,9 +2504,9 @@ __global__ __launch_bounds__(EVB) void k_ens_vec(const EnsVec *__restrict__ T,
       double s;
       int q0;
       if (E.base) { s = E.z[ip]; q0 = 0; }
-      else if (nq > 0) { s = E.c[0] * E.xq[0][ip]; q0 = 1; }
+      else if (nq > 0) { s = Ec[0] * Ex[0][ip]; q0 = 1; }
       else { s = E.cst; q0 = 0; }
-      for (int q = q0; q < nq; q++) s += E.c[q] * E.xq[q][ip];
+      for (int q = q0; q < nq; q++) s += Ec[q] * Ex[q][ip];
       E.z[ip] = s;
       acc += s * s;
     }
@@ -2524,7 +2525,7 @@ __global__ __launch_bounds__(EVB) void k_ens_vec(const EnsVec *__restrict__ T,
       const double w = E.y[ip];
 #pragma unroll
       for (int q = 0; q < ENS_MDOT; q++)
-        if (q < nq) acc[q] += E.xq[q][ip] * w;
+        if (q < nq) acc[q] += Ex[q][ip] * w;
     }
 #pragma unroll
     for (int q = 0; q < ENS_MDOT; q++) {
@@ -2586,6 +2587,38 @@ __global__ __launch_bounds__(EVB) void k_ens_vec_final(const EnsVec *__restrict_
   }
 }
 
+// room for `bytes` of table at the ring's current offset (host address);
+// pfhip_ens_commit then copies it to the device twin
+extern "C" void *pfhip_ens_reserve(pfhip_ens_stage *st, long bytes, void *stream)
+{
+  long need = (bytes + 255) & ~255L;
+  if (need > st->cap) {
+    pfhip_set_error("ensemble table of %ld bytes exceeds the staging ring", bytes);
+    return nullptr;
+  }
+  if (st->off + need > st->cap) {
+    if (hipStreamSynchronize(STREAM(stream)) != hipSuccess) {
+      pfhip_set_error("ensemble staging: stream synchronisation failed");
+      return nullptr;
+    }
+    st->off = 0;
+  }
+  return st->host + st->off;
+}
+
+extern "C" void *pfhip_ens_commit(pfhip_ens_stage *st, long bytes, void *stream)
+{
+  long need = (bytes + 255) & ~255L;
+  if (hipMemcpyAsync(st->dev + st->off, st->host + st->off, bytes, hipMemcpyHostToDevice,
+                     STREAM(stream)) != hipSuccess) {
+    pfhip_set_error("ensemble staging: host-to-device copy failed");
+    return nullptr;
+  }
+  void *d = st->dev + st->off;
+  st->off += need;
+  return d;
+}
+
 extern "C" void *pfhip_ens_upload(pfhip_ens_stage *st, const void *src, long bytes,
                                   void *stream)
 {
@@ -2613,53 +2646,51 @@ extern "C" void *pfhip_ens_upload(pfhip_ens_stage *st, const void *src, long byt
   return d;
 }
 
-static int *ens_starts(int n, const long *ncell, int per_block, int *total)
-{
-  int *s = (int *)malloc(sizeof(int) * (n + 1));
-  if (!s) return nullptr;
-  long b = 0;
-  for (int e = 0; e < n; e++) {
-    s[e] = (int)b;
-    b += (ncell[e] + per_block - 1) / per_block;
-  }
-  s[n] = (int)b;
-  *total = (int)b;
-  return s;
-}
+static inline long al256(long b) { return (b + 255) & ~255L; }
 
 extern "C" int pfhip_ens_exec_vec(pfhip_ens_req *const *rq, int n, double *res,
                                   double *part, pfhip_ens_stage *st, void *stream)
 {
   if (n <= 0) return PFHIP_OK;
-  EnsVec *T = (EnsVec *)calloc(n, sizeof(EnsVec));
-  long *nc = (long *)malloc(sizeof(long) * n);
-  if (!T || !nc) { free(T); free(nc); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
-  int results = 0;
+  long nqt = 0;
+  for (int r = 0; r < n; r++) nqt += rq[r]->nq;
+  // one table upload: entries | block starts | term coefficients | term pointers
+  const long bT = al256((long)n * sizeof(EnsVec)), bS = al256((long)(n + 1) * sizeof(int));
+  const long bC = al256(nqt * (long)sizeof(double));
+  const long bytes = bT + bS + bC + al256(nqt * (long)sizeof(void *));
+  char *h = (char *)pfhip_ens_reserve(st, bytes, stream);
+  if (!h) return PFHIP_ERR_HIP;
+  EnsVec *T = (EnsVec *)h;
+  int *starts = (int *)(h + bT);
+  double *QC = (double *)(h + bT + bS);
+  const double **QX = (const double **)(h + bT + bS + bC);
+  int results = 0, blk = 0;
+  long qo = 0;
   for (int r = 0; r < n; r++) {
     const pfhip_ens_req *q = rq[r];
     EnsVec &E = T[r];
     E.g = geom_of(q->pb);
     E.op = q->op; E.nq = q->nq; E.base = q->base; E.norm = q->norm; E.cst = q->cst;
-    for (int t = 0; t < q->nq && t < PFHIP_ENS_MAXQ; t++) { E.c[t] = q->c[t]; E.xq[t] = q->xq[t]; }
     E.x = q->x; E.y = q->y; E.u = q->u; E.z = q->z;
     E.ncell = (int)q->pb->ncell;
-    nc[r] = q->pb->ncell;
+    E.qoff = (int)qo;
+    for (int t = 0; t < q->nq; t++) { QC[qo + t] = q->c[t]; QX[qo + t] = q->xq[t]; }
+    qo += q->nq;
+    starts[r] = blk;
+    blk += (int)((q->pb->ncell + EVC - 1) / EVC);
     if (q->nout) results = 1;
   }
-  int total = 0;
-  int *starts = ens_starts(n, nc, EVC, &total);
-  free(nc);
-  if (!starts) { free(T); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
-  void *dT = pfhip_ens_upload(st, T, (long)n * sizeof(EnsVec), stream);
-  void *dS = dT ? pfhip_ens_upload(st, starts, (long)(n + 1) * sizeof(int), stream) : nullptr;
-  free(T);
-  free(starts);
-  if (!dS) return PFHIP_ERR_HIP;
-  hipLaunchKernelGGL(k_ens_vec, dim3(total), dim3(EVB), 0, STREAM(stream), (const EnsVec *)dT,
-                     (const int *)dS, n, part);
+  starts[n] = blk;
+  char *d = (char *)pfhip_ens_commit(st, bytes, stream);
+  if (!d) return PFHIP_ERR_HIP;
+  const EnsVec *dT = (const EnsVec *)d;
+  const int *dS = (const int *)(d + bT);
+  hipLaunchKernelGGL(k_ens_vec, dim3(blk), dim3(EVB), 0, STREAM(stream), dT, dS, n,
+                     (const double *)(d + bT + bS), (const double *const *)(d + bT + bS + bC),
+                     part);
   if (results)
-    hipLaunchKernelGGL(k_ens_vec_final, dim3(n), dim3(EVB), 0, STREAM(stream),
-                       (const EnsVec *)dT, (const int *)dS, (const double *)part, res);
+    hipLaunchKernelGGL(k_ens_vec_final, dim3(n), dim3(EVB), 0, STREAM(stream), dT, dS,
+                       (const double *)part, res);
   LAUNCHCHK();
   return PFHIP_OK;
 }
@@ -2751,9 +2782,13 @@ extern "C" int pfhip_ens_exec_residual(pfhip_ens_req *const *rq, int n, pfhip_en
                                        void *stream)
 {
   if (n <= 0) return PFHIP_OK;
-  EnsRes *T = (EnsRes *)calloc(n, sizeof(EnsRes));
-  long *nc = (long *)malloc(sizeof(long) * n);
-  if (!T || !nc) { free(T); free(nc); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
+  const long bT = al256((long)n * sizeof(EnsRes));
+  const long bytes = bT + al256((long)(n + 1) * sizeof(int));
+  char *h = (char *)pfhip_ens_reserve(st, bytes, stream);
+  if (!h) return PFHIP_ERR_HIP;
+  EnsRes *T = (EnsRes *)h;
+  int *starts = (int *)(h + bT);
+  int blk = 0;
   for (int r = 0; r < n; r++) {
     const pfhip_ens_req *q = rq[r];
     EnsRes &E = T[r];
@@ -2764,21 +2799,16 @@ extern "C" int pfhip_ens_exec_residual(pfhip_ens_req *const *rq, int n, pfhip_en
     E.pe = q->scratch[0]; E.kr = q->scratch[1]; E.sat = q->scratch[2];
     E.mode = q->mode;
     E.ncell = (int)q->pb->ncell;
-    nc[r] = q->pb->ncell;
+    starts[r] = blk;
+    blk += (int)((q->pb->ncell + ERB - 1) / ERB);
   }
-  int total = 0;
-  int *starts = ens_starts(n, nc, ERB, &total);
-  free(nc);
-  if (!starts) { free(T); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
-  void *dT = pfhip_ens_upload(st, T, (long)n * sizeof(EnsRes), stream);
-  void *dS = dT ? pfhip_ens_upload(st, starts, (long)(n + 1) * sizeof(int), stream) : nullptr;
-  free(T);
-  free(starts);
-  if (!dS) return PFHIP_ERR_HIP;
-  hipLaunchKernelGGL(k_ens_res_closure, dim3(total), dim3(ERB), 0, STREAM(stream),
-                     (const EnsRes *)dT, (const int *)dS, n);
-  hipLaunchKernelGGL(k_ens_res_faces, dim3(total), dim3(ERB), 0, STREAM(stream),
-                     (const EnsRes *)dT, (const int *)dS, n);
+  starts[n] = blk;
+  char *d = (char *)pfhip_ens_commit(st, bytes, stream);
+  if (!d) return PFHIP_ERR_HIP;
+  hipLaunchKernelGGL(k_ens_res_closure, dim3(blk), dim3(ERB), 0, STREAM(stream),
+                     (const EnsRes *)d, (const int *)(d + bT), n);
+  hipLaunchKernelGGL(k_ens_res_faces, dim3(blk), dim3(ERB), 0, STREAM(stream),
+                     (const EnsRes *)d, (const int *)(d + bT), n);
   LAUNCHCHK();
   return PFHIP_OK;
 }
@@ -2894,10 +2924,13 @@ extern "C" int pfhip_ens_exec_jacobian(pfhip_ens_req *const *rq, int n, pfhip_en
                                        void *stream)
 {
   if (n <= 0) return PFHIP_OK;
-  EnsJac *T = (EnsJac *)calloc(n, sizeof(EnsJac));
-  long *nc = (long *)malloc(sizeof(long) * n);
-  if (!T || !nc) { free(T); free(nc); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
-  int prec0 = 0;
+  const long bT = al256((long)n * sizeof(EnsJac));
+  const long bytes = bT + al256((long)(n + 1) * sizeof(int));
+  char *h = (char *)pfhip_ens_reserve(st, bytes, stream);
+  if (!h) return PFHIP_ERR_HIP;
+  EnsJac *T = (EnsJac *)h;
+  int *starts = (int *)(h + bT);
+  int prec0 = 0, blk = 0;
   for (int r = 0; r < n; r++) {
     const pfhip_ens_req *q = rq[r];
     pfhip_problem *pb = q->pb;
@@ -2914,36 +2947,30 @@ extern "C" int pfhip_ens_exec_jacobian(pfhip_ens_req *const *rq, int n, pfhip_en
     for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
     A.symm = 1;
     A.overland = 0;
-    for (int s = 0; s < 7; s++) A.J[s] = nullptr;
+    for (int k = 0; k < 7; k++) A.J[k] = nullptr;
     E.kr = q->scratch[0]; E.dk = q->scratch[1]; E.sat = q->scratch[2]; E.dsat = q->scratch[3];
     int prec;
     pfhip_mg_level0(q->mg, E.J, &prec);
     if (r == 0) prec0 = prec;
     if (prec != prec0 || pb->overland) {
-      free(T); free(nc);
       pfhip_set_error("ensemble members need one multigrid precision and no overland flow");
       return PFHIP_ERR_ARG;
     }
     E.ncell = (int)pb->ncell;
-    nc[r] = pb->ncell;
+    starts[r] = blk;
+    blk += (int)((pb->ncell + ERB - 1) / ERB);
   }
-  int total = 0;
-  int *starts = ens_starts(n, nc, ERB, &total);
-  free(nc);
-  if (!starts) { free(T); pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
-  void *dT = pfhip_ens_upload(st, T, (long)n * sizeof(EnsJac), stream);
-  void *dS = dT ? pfhip_ens_upload(st, starts, (long)(n + 1) * sizeof(int), stream) : nullptr;
-  free(T);
-  free(starts);
-  if (!dS) return PFHIP_ERR_HIP;
-  hipLaunchKernelGGL(k_ens_jac_closure, dim3(total), dim3(ERB), 0, STREAM(stream),
-                     (const EnsJac *)dT, (const int *)dS, n);
+  starts[n] = blk;
+  char *d = (char *)pfhip_ens_commit(st, bytes, stream);
+  if (!d) return PFHIP_ERR_HIP;
+  const EnsJac *dT = (const EnsJac *)d;
+  const int *dS = (const int *)(d + bT);
+  hipLaunchKernelGGL(k_ens_jac_closure, dim3(blk), dim3(ERB), 0, STREAM(stream), dT, dS, n);
   if (prec0 == 32)
-    hipLaunchKernelGGL(k_ens_jac_rows<float>, dim3(total), dim3(ERB), 0, STREAM(stream),
-                       (const EnsJac *)dT, (const int *)dS, n);
+    hipLaunchKernelGGL(k_ens_jac_rows<float>, dim3(blk), dim3(ERB), 0, STREAM(stream), dT, dS, n);
   else
-    hipLaunchKernelGGL(k_ens_jac_rows<double>, dim3(total), dim3(ERB), 0, STREAM(stream),
-                       (const EnsJac *)dT, (const int *)dS, n);
+    hipLaunchKernelGGL(k_ens_jac_rows<double>, dim3(blk), dim3(ERB), 0, STREAM(stream), dT, dS,
+                       n);
   LAUNCHCHK();
   return PFHIP_OK;
 }

@@ -29,6 +29,7 @@
 
 #define ENS_STACK (1L << 20)            /* coroutine stack (lazily touched) */
 #define ENS_STAGE (64L << 20)           /* staging ring for member tables   */
+#define ENS_QMAX 16                     /* requests a member queues a round */
 
 /* Coroutine switch (System V x86-64): push the callee-saved registers,
  * store the stack pointer through the first argument, load the second and
@@ -69,8 +70,9 @@ typedef struct ens_member {
   void *sp;                             /* saved stack pointer             */
   void *stack;
   int live, rc;
-  int waiting;                          /* request pending, not yet served */
-  pfhip_ens_req req;
+  int waiting;                          /* queue awaits the scheduler      */
+  int qlen, nexec, rc_round;            /* queued / run this round / error */
+  pfhip_ens_req q[ENS_QMAX];
   /* advance arguments and outputs */
   double *p, *sat, t0, dt;
   int nsteps, max_fail, steps_done;
@@ -88,7 +90,7 @@ struct pfhip_ensemble {
   pfhip_ens_stage st;
   pfhip_ens_req **lists[PFHIP_EOP_COUNT];
   int nlist[PFHIP_EOP_COUNT];
-  pfhip_ens_req **vec;
+  pfhip_ens_req **vec, **outs;
   long rounds, launches;
 };
 
@@ -98,11 +100,19 @@ static __thread pfhip_ensemble *g_ens;
 #define CHK(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
 
 /* ---- the operations table of a member -------------------------------- */
+static int is_heavy(int op) { return op == PFHIP_EOP_MG_SETUP || op == PFHIP_EOP_MG_APPLY; }
+
 static pfhip_ens_req *new_req(int op)
 {
   ens_member *m = &g_ens->m[g_ens->cur];
-  pfhip_ens_req *r = &m->req;
-  memset(r, 0, sizeof(*r));
+  pfhip_ens_req *r = &m->q[m->qlen];
+  /* the fields every kind reads; term and result arrays are filled up to
+   * nq / nout by the caller */
+  r->mode = r->base = r->norm = r->nq = r->nout = r->rc = 0;
+  r->cst = r->sig = r->fsc = r->dt = 0.0;
+  r->x = r->y = r->u = NULL;
+  r->z = r->z2 = NULL;
+  r->xq[0] = r->xq[1] = NULL;
   r->op = op;
   r->pb = m->pb;
   r->mg = m->mg;
@@ -110,11 +120,28 @@ static pfhip_ens_req *new_req(int op)
   return r;
 }
 
+/* A request that returns nothing to the host and is light (a linear
+ * combination without its norm, a residual form) only joins the member's
+ * queue, and the member runs on; a request with results, a heavy one or a
+ * full queue hands the queue to the scheduler.  The return code is the
+ * first error of the queue's execution. */
 static int submit(void)
 {
   ens_member *m = &g_ens->m[g_ens->cur];
+  pfhip_ens_req *r = &m->q[m->qlen++];
+  if (r->nout == 0 && !is_heavy(r->op) && m->qlen < ENS_QMAX) return PFHIP_OK;
   pfhip_ens_switch(&m->sp, g_ens->sched_sp);
-  return m->req.rc;
+  return m->rc_round;
+}
+
+/* queued requests are launched before the member touches the device
+ * itself (its own launches would otherwise overtake them) */
+static int flush(void)
+{
+  ens_member *m = &g_ens->m[g_ens->cur];
+  if (m->qlen == 0) return PFHIP_OK;
+  pfhip_ens_switch(&m->sp, g_ens->sched_sp);
+  return m->rc_round;
 }
 
 static int e_vec_scale(pfhip_problem *pb, double c, const double *x, double *z, void *st)
@@ -265,16 +292,19 @@ static int e_mg_apply(pfhip_mg *mg, const double *b, double *x, void *st)
  * reductions of that solve, which synchronise the stream. */
 static int e_bc_pressure(pfhip_problem *pb, double t, void *st)
 {
+  CHK(flush());
   return pfhip_bc_pressure_async(pb, t, 0, NULL);
 }
 
 static int e_saturation(pfhip_problem *pb, const double *p, double *out, int fcn, void *st)
 {
+  CHK(flush());
   return pfhip_saturation(pb, p, out, fcn, NULL);
 }
 
-/* results reach a member only through the scheduler's synchronised copy */
-static int e_synchronize(void) { return PFHIP_OK; }
+/* results reach a member only through the scheduler's synchronised copy;
+ * a synchronisation point only has to launch what is queued */
+static int e_synchronize(void) { return flush(); }
 
 static const pfhip_ops ens_ops = {
   e_vec_scale, e_vec_linear_sum, e_vec_const, e_vec_dot, e_vec_max_norm,
@@ -299,6 +329,7 @@ int pfhip_ensemble_destroy(pfhip_ensemble *e)
   pfhip_ens_stage_destroy(&e->st);
   for (int k = 0; k < PFHIP_EOP_COUNT; k++) free(e->lists[k]);
   free(e->vec);
+  free(e->outs);
   free(e->m);
   free(e);
   return PFHIP_OK;
@@ -323,7 +354,8 @@ int pfhip_ensemble_create(int n, pfhip_solver *const *solvers, pfhip_ensemble **
   e->n = n;
   e->m = (ens_member *)calloc(n, sizeof(ens_member));
   e->vec = (pfhip_ens_req **)calloc(n, sizeof(pfhip_ens_req *));
-  int rc = (e->m && e->vec) ? PFHIP_OK : PFHIP_ERR_NOMEM;
+  e->outs = (pfhip_ens_req **)calloc((long)n * ENS_QMAX, sizeof(pfhip_ens_req *));
+  int rc = (e->m && e->vec && e->outs) ? PFHIP_OK : PFHIP_ERR_NOMEM;
   for (int k = 0; k < PFHIP_EOP_COUNT && !rc; k++) {
     e->lists[k] = (pfhip_ens_req **)calloc(n, sizeof(pfhip_ens_req *));
     if (!e->lists[k]) rc = PFHIP_ERR_NOMEM;
@@ -342,8 +374,8 @@ int pfhip_ensemble_create(int n, pfhip_solver *const *solvers, pfhip_ensemble **
   long blocks = 0;
   for (int i = 0; i < n && !rc; i++) blocks += pfhip_ens_vec_blocks(e->m[i].pb->ncell);
   if (!rc) rc = pfhip_dev_alloc(&e->d_part, blocks * PFHIP_ENS_NRES);
-  if (!rc) rc = pfhip_dev_alloc(&e->d_res, (long)n * PFHIP_ENS_NRES);
-  if (!rc) rc = pfhip_host_alloc(&e->h_res, (long)n * PFHIP_ENS_NRES);
+  if (!rc) rc = pfhip_dev_alloc(&e->d_res, (long)n * ENS_QMAX * PFHIP_ENS_NRES);
+  if (!rc) rc = pfhip_host_alloc(&e->h_res, (long)n * ENS_QMAX * PFHIP_ENS_NRES);
   if (!rc) rc = pfhip_ens_stage_create(&e->st, ENS_STAGE);
   if (!rc) rc = pfhip_synchronize();
   if (rc) { pfhip_ensemble_destroy(e); return rc; }
@@ -367,6 +399,8 @@ static void member_main(void)
   m->rc = pfhip_advance_richards(m->s, m->p, m->sat, m->t0, m->dt, m->nsteps, m->max_fail,
                                  m->newton, m->gmres, m->fevals, m->dt_used,
                                  &m->steps_done);
+  int frc = flush();
+  if (m->rc == PFHIP_OK) m->rc = frc;
   m->live = 0;
   pfhip_ens_switch(&m->sp, e->sched_sp);   /* never resumed */
   abort();
@@ -389,68 +423,98 @@ static void set_rc(pfhip_ens_req **l, int n, int rc)
   for (int i = 0; i < n; i++) l[i]->rc = rc;
 }
 
-static int is_heavy(int op) { return op == PFHIP_EOP_MG_SETUP || op == PFHIP_EOP_MG_APPLY; }
-
-/* Serve the pending requests of this round.  The heavy kinds (PC setup,
- * V-cycle: their cost is set by the largest member, not by how many members
- * take part) wait while any light request (vector operation, residual) is
- * pending: the members that are behind catch up through light rounds, and
- * each heavy launch then serves nearly every member at once. */
+/* Serve the queued requests of this round, position by position (a
+ * member's queue runs in its order; every position is one launch per kind
+ * across the members), then fetch all results with one copy.  The heavy
+ * kinds (PC setup, V-cycle: their cost is set by the largest member, not by
+ * how many members take part) wait while any light request is queued: the
+ * members that are behind catch up through light rounds, and each heavy
+ * launch then serves nearly every member at once. */
 static int serve_round(pfhip_ensemble *e)
 {
-  int nv = 0, need = 0, light = 0;
-  for (int i = 0; i < e->n; i++)
-    if (e->m[i].live && !is_heavy(e->m[i].req.op)) light = 1;
-  for (int k = 0; k < PFHIP_EOP_COUNT; k++) e->nlist[k] = 0;
+  int light = 0, maxp = 0, rc;
   for (int i = 0; i < e->n; i++) {
     ens_member *m = &e->m[i];
-    if (!m->live) continue;
-    pfhip_ens_req *r = &m->req;
-    if (is_heavy(r->op) == light) continue;   /* deferred: stays waiting */
-    m->waiting = 0;
-    if (r->op >= PFHIP_EOP_LIN && r->op <= PFHIP_EOP_STOP) {
-      e->vec[nv++] = r;
-      if (r->nout) need = 1;
-    } else {
-      e->lists[r->op][e->nlist[r->op]++] = r;
+    if (m->live && m->waiting && (m->qlen > 1 || !is_heavy(m->q[m->qlen - 1].op))) light = 1;
+  }
+  for (int i = 0; i < e->n; i++) {
+    ens_member *m = &e->m[i];
+    m->nexec = 0;
+    if (!m->live || !m->waiting) continue;
+    m->nexec = m->qlen;
+    if (light && is_heavy(m->q[m->qlen - 1].op)) m->nexec--;   /* deferred */
+    if (m->nexec > maxp) maxp = m->nexec;
+  }
+  int slots = 0, nouts = 0;
+  for (int pos = 0; pos < maxp; pos++) {
+    int nv = 0;
+    for (int k = 0; k < PFHIP_EOP_COUNT; k++) e->nlist[k] = 0;
+    for (int i = 0; i < e->n; i++) {
+      ens_member *m = &e->m[i];
+      if (pos >= m->nexec) continue;
+      pfhip_ens_req *r = &m->q[pos];
+      if (r->op >= PFHIP_EOP_LIN && r->op <= PFHIP_EOP_STOP) {
+        r->slot = slots + nv;
+        e->vec[nv++] = r;
+        if (r->nout) e->outs[nouts++] = r;
+      } else {
+        e->lists[r->op][e->nlist[r->op]++] = r;
+      }
+    }
+    if (nv) {
+      rc = pfhip_ens_exec_vec(e->vec, nv, e->d_res + (long)slots * PFHIP_ENS_NRES, e->d_part,
+                              &e->st, NULL);
+      set_rc(e->vec, nv, rc);
+      slots += nv;
+      e->launches++;
+    }
+    int nr = e->nlist[PFHIP_EOP_RESIDUAL];
+    if (nr) {
+      rc = pfhip_ens_exec_residual(e->lists[PFHIP_EOP_RESIDUAL], nr, &e->st, NULL);
+      set_rc(e->lists[PFHIP_EOP_RESIDUAL], nr, rc);
+      e->launches += 2;
+    }
+    int ns = e->nlist[PFHIP_EOP_MG_SETUP];
+    if (ns) {
+      rc = pfhip_ens_exec_jacobian(e->lists[PFHIP_EOP_MG_SETUP], ns, &e->st, NULL);
+      if (!rc) rc = pfhip_ens_exec_mg_coarse(e->lists[PFHIP_EOP_MG_SETUP], ns, &e->st, NULL);
+      set_rc(e->lists[PFHIP_EOP_MG_SETUP], ns, rc);
+      e->launches += 3;
+    }
+    int na = e->nlist[PFHIP_EOP_MG_APPLY];
+    if (na) {
+      rc = pfhip_ens_exec_mg_apply(e->lists[PFHIP_EOP_MG_APPLY], na, &e->st, NULL);
+      set_rc(e->lists[PFHIP_EOP_MG_APPLY], na, rc);
+      e->launches++;
     }
   }
-  int rc;
-  if (nv) {
-    rc = pfhip_ens_exec_vec(e->vec, nv, e->d_res, e->d_part, &e->st, NULL);
-    set_rc(e->vec, nv, rc);
-    e->launches++;
+  int frc = PFHIP_OK;
+  if (nouts) {
+    frc = pfhip_ens_fetch(e->h_res, e->d_res, (long)slots * PFHIP_ENS_NRES, NULL);
+    if (frc == PFHIP_OK) {
+      e->st.off = 0;
+      for (int v = 0; v < nouts; v++) {
+        pfhip_ens_req *r = e->outs[v];
+        for (int k = 0; k < r->nout; k++)
+          *r->out[k] = e->h_res[(long)r->slot * PFHIP_ENS_NRES + k];
+      }
+    }
   }
-  int nr = e->nlist[PFHIP_EOP_RESIDUAL];
-  if (nr) {
-    rc = pfhip_ens_exec_residual(e->lists[PFHIP_EOP_RESIDUAL], nr, &e->st, NULL);
-    set_rc(e->lists[PFHIP_EOP_RESIDUAL], nr, rc);
-    e->launches += 2;
-  }
-  int ns = e->nlist[PFHIP_EOP_MG_SETUP];
-  if (ns) {
-    rc = pfhip_ens_exec_jacobian(e->lists[PFHIP_EOP_MG_SETUP], ns, &e->st, NULL);
-    if (!rc) rc = pfhip_ens_exec_mg_coarse(e->lists[PFHIP_EOP_MG_SETUP], ns, &e->st, NULL);
-    set_rc(e->lists[PFHIP_EOP_MG_SETUP], ns, rc);
-    e->launches += 3;
-  }
-  int na = e->nlist[PFHIP_EOP_MG_APPLY];
-  if (na) {
-    rc = pfhip_ens_exec_mg_apply(e->lists[PFHIP_EOP_MG_APPLY], na, &e->st, NULL);
-    set_rc(e->lists[PFHIP_EOP_MG_APPLY], na, rc);
-    e->launches++;
-  }
-  if (need) {
-    rc = pfhip_ens_fetch(e->h_res, e->d_res, (long)nv * PFHIP_ENS_NRES, NULL);
-    if (rc) { set_rc(e->vec, nv, rc); return rc; }
-    e->st.off = 0;
-    for (int v = 0; v < nv; v++) {
-      pfhip_ens_req *r = e->vec[v];
-      for (int k = 0; k < r->nout; k++) *r->out[k] = e->h_res[(long)v * PFHIP_ENS_NRES + k];
+  for (int i = 0; i < e->n; i++) {
+    ens_member *m = &e->m[i];
+    if (!m->live || !m->waiting) continue;
+    m->rc_round = frc;
+    for (int p = 0; p < m->nexec && m->rc_round == PFHIP_OK; p++) m->rc_round = m->q[p].rc;
+    if (m->nexec == m->qlen) {
+      m->qlen = 0;
+      m->waiting = 0;
+    } else if (m->nexec > 0) {
+      m->q[0] = m->q[m->qlen - 1];   /* the deferred heavy request */
+      m->qlen = 1;
     }
   }
   e->rounds++;
-  return PFHIP_OK;
+  return frc;
 }
 
 int pfhip_ensemble_advance_richards(pfhip_ensemble *e, double *const *p, double *const *sat,
@@ -480,6 +544,8 @@ int pfhip_ensemble_advance_richards(pfhip_ensemble *e, double *const *p, double 
     if (!m->stack) { rc = PFHIP_ERR_NOMEM; break; }
     m->sp = coroutine_start(m->stack, ENS_STACK);
     m->waiting = 0;
+    m->qlen = 0;
+    m->rc_round = PFHIP_OK;
     pfhip_solver_set_ops(m->s, &ens_ops);
     m->live = 1;
   }
@@ -496,7 +562,6 @@ int pfhip_ensemble_advance_richards(pfhip_ensemble *e, double *const *p, double 
       ens_member *m = &e->m[i];
       if (!m->live || m->waiting) continue;
       e->cur = i;
-      m->req.op = PFHIP_EOP_NONE;
       pfhip_ens_switch(&e->sched_sp, m->sp);
       if (!m->live) live--;
       else m->waiting = 1;

@@ -655,6 +655,123 @@ __global__ void k_fill_t(T *p, long n)
     __syncthreads();                                                     \
   }
 
+// One half-sweep of parity par over a level inside a workgroup.  Threads
+// enumerate the points of that parity only ((i + j + k) & 1 == par), so no
+// lane idles on the parity test; WUNR points per thread per pass have their
+// loads issued before their updates (1 measured fastest at 1024 threads:
+// 2 and 4 raise register use past the 1024-thread budget).  The arithmetic
+// per point is rbgs_point's.
+#define WUNR 1
+template <typename T>
+__device__ void rbgs_phase(const LGeom &g, const Mat<T> &A, T *__restrict__ x,
+                           const T *__restrict__ b, int par, int init)
+{
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int ncol = (nx + 1) >> 1;
+  const int np = ncol * ny * nz;
+  const int sy = g.s[1], sz = g.s[2];
+  for (int base = threadIdx.x; base < np; base += WUNR * blockDim.x) {
+    T bb[WUNR], c[WUNR], aw[WUNR], xw[WUNR], ae[WUNR], xe[WUNR], as[WUNR], xs[WUNR];
+    T an[WUNR], xn[WUNR], al[WUNR], xl[WUNR], au[WUNR], xu[WUNR];
+    bool hw[WUNR], he[WUNR], hs[WUNR], hn[WUNR], hl[WUNR], hu[WUNR], act[WUNR];
+    int ip[WUNR];
+#pragma unroll
+    for (int u = 0; u < WUNR; u++) {
+      const int idx = base + u * blockDim.x;
+      const int t = idx % ncol, r = idx / ncol;
+      const int j = r % ny, k = r / ny;
+      const int i = ((par - j - k) & 1) + 2 * t;
+      const bool a = (idx < np) && (i < nx);
+      act[u] = a;
+      ip[u] = a ? LIDX(g, i, j, k) : 0;
+      const int q = ip[u];
+      hw[u] = a && i > 0; he[u] = a && i + 1 < nx; hs[u] = a && j > 0;
+      hn[u] = a && j + 1 < ny; hl[u] = a && k > 0; hu[u] = a && k + 1 < nz;
+      bb[u] = a ? b[q] : T(0);
+      c[u] = a ? A.c[q] : T(1);
+      const bool nb = a && !init;
+      aw[u] = (nb && hw[u]) ? A.u[0][q - 1] : T(0);
+      xw[u] = (nb && hw[u]) ? x[q - 1] : T(0);
+      ae[u] = nb ? A.u[0][q] : T(0);
+      xe[u] = (nb && he[u]) ? x[q + 1] : T(0);
+      as[u] = (nb && hs[u]) ? A.u[1][q - sy] : T(0);
+      xs[u] = (nb && hs[u]) ? x[q - sy] : T(0);
+      an[u] = nb ? A.u[1][q] : T(0);
+      xn[u] = (nb && hn[u]) ? x[q + sy] : T(0);
+      al[u] = (nb && hl[u]) ? A.u[2][q - sz] : T(0);
+      xl[u] = (nb && hl[u]) ? x[q - sz] : T(0);
+      au[u] = nb ? A.u[2][q] : T(0);
+      xu[u] = (nb && hu[u]) ? x[q + sz] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < WUNR; u++) {
+      if (!act[u]) continue;
+      x[ip[u]] = init ? bb[u] / c[u]
+                      : gs_update(bb[u], c[u], aw[u], xw[u], hw[u], ae[u], xe[u], he[u], as[u],
+                                  xs[u], hs[u], an[u], xn[u], hn[u], al[u], xl[u], hl[u], au[u],
+                                  xu[u], hu[u]);
+    }
+  }
+  __syncthreads();
+}
+
+// t = b - A x over a level inside a workgroup, WUNR points per thread per
+// pass with the loads first; resid_point's expressions and order
+template <typename T>
+__device__ void resid_phase(const LGeom &g, const Mat<T> &A, const T *__restrict__ x,
+                            const T *__restrict__ b, T *__restrict__ tt)
+{
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int np = nx * ny * nz;
+  const int sy = g.s[1], sz = g.s[2];
+  for (int base = threadIdx.x; base < np; base += WUNR * blockDim.x) {
+    T bb[WUNR], c[WUNR], x0[WUNR], aw[WUNR], xw[WUNR], ae[WUNR], xe[WUNR], as[WUNR], xs[WUNR];
+    T an[WUNR], xn[WUNR], al[WUNR], xl[WUNR], au[WUNR], xu[WUNR];
+    bool hw[WUNR], he[WUNR], hs[WUNR], hn[WUNR], hl[WUNR], hu[WUNR], act[WUNR];
+    int ip[WUNR];
+#pragma unroll
+    for (int u = 0; u < WUNR; u++) {
+      const int idx = base + u * blockDim.x;
+      const bool a = idx < np;
+      int i = 0, j = 0, k = 0;
+      if (a) unflat(g, idx, i, j, k);
+      act[u] = a;
+      const int q = a ? LIDX(g, i, j, k) : 0;
+      ip[u] = q;
+      hw[u] = a && i > 0; he[u] = a && i + 1 < nx; hs[u] = a && j > 0;
+      hn[u] = a && j + 1 < ny; hl[u] = a && k > 0; hu[u] = a && k + 1 < nz;
+      bb[u] = a ? b[q] : T(0);
+      c[u] = a ? A.c[q] : T(0);
+      x0[u] = a ? x[q] : T(0);
+      aw[u] = hw[u] ? A.u[0][q - 1] : T(0);
+      xw[u] = hw[u] ? x[q - 1] : T(0);
+      ae[u] = he[u] ? A.u[0][q] : T(0);
+      xe[u] = he[u] ? x[q + 1] : T(0);
+      as[u] = hs[u] ? A.u[1][q - sy] : T(0);
+      xs[u] = hs[u] ? x[q - sy] : T(0);
+      an[u] = hn[u] ? A.u[1][q] : T(0);
+      xn[u] = hn[u] ? x[q + sy] : T(0);
+      al[u] = hl[u] ? A.u[2][q - sz] : T(0);
+      xl[u] = hl[u] ? x[q - sz] : T(0);
+      au[u] = hu[u] ? A.u[2][q] : T(0);
+      xu[u] = hu[u] ? x[q + sz] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < WUNR; u++) {
+      if (!act[u]) continue;
+      T v = c[u] * x0[u];
+      if (hw[u]) v += aw[u] * xw[u];
+      if (he[u]) v += ae[u] * xe[u];
+      if (hs[u]) v += as[u] * xs[u];
+      if (hn[u]) v += an[u] * xn[u];
+      if (hl[u]) v += al[u] * xl[u];
+      if (hu[u]) v += au[u] * xu[u];
+      tt[ip[u]] = bb[u] - v;
+    }
+  }
+  __syncthreads();
+}
+
 // RedBlackGSPoint call sequence (PL/rb_GS_point.c:100-300) inside a block
 template <typename T>
 __device__ void rbgs_block(const LevelDev<T> &L, int zero, int max_iter, int symmetric)
@@ -668,12 +785,12 @@ __device__ void rbgs_block(const LevelDev<T> &L, int zero, int max_iter, int sym
       FOR_POINTS(g, L.x[LIDX(g, i, j, k)] = T(0));
       if (!symmetric) return;
     }
-    FOR_POINTS(g, rbgs_point(g, L.A, L.x, L.b, 1, 1, i, j, k));
+    rbgs_phase(g, L.A, L.x, L.b, 1, 1);
     rb = !rb;
   }
   for (; (count / 2) <= max_iter; count++) {
     int par = rb ? 0 : 1;
-    FOR_POINTS(g, rbgs_point(g, L.A, L.x, L.b, par, 0, i, j, k));
+    rbgs_phase(g, L.A, L.x, L.b, par, 0);
     rb = !rb;
   }
 }
@@ -688,7 +805,7 @@ __device__ void vcycle_block(const LevelDev<T> *__restrict__ lev, int lc, int L,
   for (l = lc; l <= L - 2; l++) {
     const LevelDev<T> &F = lev[l], &Cc = lev[l + 1];
     rbgs_block(F, 1, pre, symmetric);
-    FOR_POINTS(F.g, resid_point(F.g, F.A, F.x, F.b, F.t, i, j, k));
+    resid_phase(F.g, F.A, F.x, F.b, F.t);
     FOR_POINTS(Cc.g, restrict_point(F.g, Cc.g, F.dir, F.t, Cc.b, F.p1, F.p2, i, j, k));
   }
   rbgs_block(lev[L - 1], 1, 1, symmetric);

@@ -167,6 +167,7 @@ typedef struct pfhip_ens_req {
   double *scratch[4];           /* member scratch (ghosted, ng doubles)  */
   double *out[PFHIP_ENS_NRES];  /* host destinations of the results      */
   int nout;
+  int slot;                     /* result slot within its round          */
   int rc;
 } pfhip_ens_req;
 
@@ -179,6 +180,10 @@ typedef struct pfhip_ens_stage {
 } pfhip_ens_stage;
 
 void *pfhip_ens_upload(pfhip_ens_stage *st, const void *src, long bytes, void *stream);
+/* in-place form: reserve returns the host address to fill, commit copies it
+ * and returns the device address */
+void *pfhip_ens_reserve(pfhip_ens_stage *st, long bytes, void *stream);
+void *pfhip_ens_commit(pfhip_ens_stage *st, long bytes, void *stream);
 /* batched launches; results of request r land in res[r * PFHIP_ENS_NRES + v] */
 int pfhip_ens_exec_vec(pfhip_ens_req *const *rq, int n, double *res, double *part,
                        pfhip_ens_stage *st, void *stream);
