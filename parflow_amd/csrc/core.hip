@@ -341,14 +341,14 @@ static int interval_number(const pfhip_problem_desc *d, const pfhip_bc_spec *b,
 // BCPressure: PL/problem_bc_pressure.c:143-290 (DirEquilRefPatch with a
 // constant density reduces to p = p_ref - rho g (z - z_ref), z = centre for
 // side faces, face elevation for z faces), :582-612, :1070-1098.
-extern "C" int pfhip_bc_pressure_async(pfhip_problem *pb, double time, int sync, void *stream)
+// the BC values of face f at `time` into h (face_ncells values)
+static void bc_face_values(const pfhip_problem *pb, int f, double time, double *h)
 {
   const pfhip_problem_desc *d = &pb->d;
-  for (int f = 0; f < 6; f++) {
-    const pfhip_bc_spec *b = &d->bc[f];
-    int n = face_ncells(d, f);
-    double val = b->values[interval_number(d, b, time)];
-    double *h = pb->h_bcv[f];
+  const pfhip_bc_spec *b = &d->bc[f];
+  int n = face_ncells(d, f);
+  double val = b->values[interval_number(d, b, time)];
+  {
     if (b->type != PFHIP_BC_DIR_EQUIL) {
       for (int c = 0; c < n; c++) h[c] = val;
     } else {
@@ -366,7 +366,14 @@ extern "C" int pfhip_bc_pressure_async(pfhip_problem *pb, double time, int sync,
         h[c] = pv;
       }
     }
-    HIPCHK(hipMemcpyAsync(pb->bcv[f], h, sizeof(double) * n,
+  }
+}
+
+extern "C" int pfhip_bc_pressure_async(pfhip_problem *pb, double time, int sync, void *stream)
+{
+  for (int f = 0; f < 6; f++) {
+    bc_face_values(pb, f, time, pb->h_bcv[f]);
+    HIPCHK(hipMemcpyAsync(pb->bcv[f], pb->h_bcv[f], sizeof(double) * face_ncells(&pb->d, f),
                           hipMemcpyHostToDevice, STREAM(stream)));
   }
   if (sync) HIPCHK(hipStreamSynchronize(STREAM(stream)));
@@ -3013,4 +3020,109 @@ extern "C" int pfhip_host_alloc(double **p, long n)
 extern "C" void pfhip_host_free(double *p)
 {
   if (p) hipHostFree(p);
+}
+
+// ---- BCPressure and Saturation of many members -----------------------------
+// BC values of every requesting member are computed on the host into one
+// staged block and scattered to the members' face tables by one kernel
+// (one segment of one face per block).
+struct EnsSeg {
+  double *dst;
+  long src;      // offset in the staged values
+  int n;
+};
+
+__global__ __launch_bounds__(256) void k_ens_scatter(const EnsSeg *__restrict__ S,
+                                                     const double *__restrict__ vals)
+{
+  const EnsSeg &G = S[blockIdx.x];
+  for (int c = threadIdx.x; c < G.n; c += blockDim.x) G.dst[c] = vals[G.src + c];
+}
+
+extern "C" int pfhip_ens_exec_bc(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
+                                 void *stream)
+{
+  if (n <= 0) return PFHIP_OK;
+  long nvals = 0;
+  for (int r = 0; r < n; r++)
+    for (int f = 0; f < 6; f++) nvals += face_ncells(&rq[r]->pb->d, f);
+  const long bS = al256((long)6 * n * sizeof(EnsSeg));
+  const long bytes = bS + al256(nvals * (long)sizeof(double));
+  char *h = (char *)pfhip_ens_reserve(st, bytes, stream);
+  if (!h) return PFHIP_ERR_HIP;
+  EnsSeg *S = (EnsSeg *)h;
+  double *vals = (double *)(h + bS);
+  long off = 0;
+  for (int r = 0; r < n; r++) {
+    pfhip_problem *pb = rq[r]->pb;
+    for (int f = 0; f < 6; f++) {
+      EnsSeg &G = S[6 * r + f];
+      G.n = face_ncells(&pb->d, f);
+      G.dst = pb->bcv[f];
+      G.src = off;
+      bc_face_values(pb, f, rq[r]->dt, vals + off);
+      off += G.n;
+    }
+  }
+  char *d = (char *)pfhip_ens_commit(st, bytes, stream);
+  if (!d) return PFHIP_ERR_HIP;
+  hipLaunchKernelGGL(k_ens_scatter, dim3(6 * n), dim3(256), 0, STREAM(stream),
+                     (const EnsSeg *)d, (const double *)(d + bS));
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+struct EnsSat {
+  Geom g;
+  Phys f;
+  const double *p;
+  double *out;
+  int what, ncell;   // what: 0 S, 1 dS/dp (the k_closure codes)
+};
+
+__global__ __launch_bounds__(ERB) void k_ens_sat(const EnsSat *__restrict__ T,
+                                                 const int *__restrict__ starts, int n)
+{
+  const int e = ens_find(starts, n, blockIdx.x);
+  const EnsSat &E = T[e];
+  const int c = (blockIdx.x - starts[e]) * ERB + threadIdx.x;
+  if (c >= E.ncell) return;
+  const Phys &f = E.f;
+  int i, j, k;
+  const int ip = cell_ip(E.g, c, i, j, k);
+  const double pv = E.p[ip], a = f.alpha[ip], nn = f.nvg[ip];
+  E.out[ip] = (E.what == 1) ? vg_dsat(pv, f.irg, a, nn, f.Sres(ip), f.Ssat(ip))
+                            : vg_sat(pv, f.irg, a, nn, f.Sres(ip), f.Ssat(ip));
+}
+
+extern "C" int pfhip_ens_exec_saturation(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
+                                         void *stream)
+{
+  if (n <= 0) return PFHIP_OK;
+  const long bT = al256((long)n * sizeof(EnsSat));
+  const long bytes = bT + al256((long)(n + 1) * sizeof(int));
+  char *h = (char *)pfhip_ens_reserve(st, bytes, stream);
+  if (!h) return PFHIP_ERR_HIP;
+  EnsSat *T = (EnsSat *)h;
+  int *starts = (int *)(h + bT);
+  int blk = 0;
+  for (int r = 0; r < n; r++) {
+    const pfhip_ens_req *q = rq[r];
+    EnsSat &E = T[r];
+    E.g = geom_of(q->pb);
+    E.f = phys_of(q->pb);
+    E.p = q->x;
+    E.out = q->z;
+    E.what = q->mode == PFHIP_CALCDER ? 1 : 0;
+    E.ncell = (int)q->pb->ncell;
+    starts[r] = blk;
+    blk += (int)((q->pb->ncell + ERB - 1) / ERB);
+  }
+  starts[n] = blk;
+  char *d = (char *)pfhip_ens_commit(st, bytes, stream);
+  if (!d) return PFHIP_ERR_HIP;
+  hipLaunchKernelGGL(k_ens_sat, dim3(blk), dim3(ERB), 0, STREAM(stream), (const EnsSat *)d,
+                     (const int *)(d + bT), n);
+  LAUNCHCHK();
+  return PFHIP_OK;
 }

@@ -134,8 +134,8 @@ static int submit(void)
   return m->rc_round;
 }
 
-/* queued requests are launched before the member touches the device
- * itself (its own launches would otherwise overtake them) */
+/* hands a non-empty queue to the scheduler (at a synchronisation point
+ * and when the member ends) */
 static int flush(void)
 {
   ens_member *m = &g_ens->m[g_ens->cur];
@@ -292,14 +292,16 @@ static int e_mg_apply(pfhip_mg *mg, const double *b, double *x, void *st)
  * reductions of that solve, which synchronise the stream. */
 static int e_bc_pressure(pfhip_problem *pb, double t, void *st)
 {
-  CHK(flush());
-  return pfhip_bc_pressure_async(pb, t, 0, NULL);
+  pfhip_ens_req *r = new_req(PFHIP_EOP_BC);
+  r->dt = t;
+  return submit();
 }
 
 static int e_saturation(pfhip_problem *pb, const double *p, double *out, int fcn, void *st)
 {
-  CHK(flush());
-  return pfhip_saturation(pb, p, out, fcn, NULL);
+  pfhip_ens_req *r = new_req(PFHIP_EOP_SAT);
+  r->x = p; r->z = out; r->mode = fcn;
+  return submit();
 }
 
 /* results reach a member only through the scheduler's synchronised copy;
@@ -485,6 +487,18 @@ static int serve_round(pfhip_ensemble *e)
     if (na) {
       rc = pfhip_ens_exec_mg_apply(e->lists[PFHIP_EOP_MG_APPLY], na, &e->st, NULL);
       set_rc(e->lists[PFHIP_EOP_MG_APPLY], na, rc);
+      e->launches++;
+    }
+    int nb = e->nlist[PFHIP_EOP_BC];
+    if (nb) {
+      rc = pfhip_ens_exec_bc(e->lists[PFHIP_EOP_BC], nb, &e->st, NULL);
+      set_rc(e->lists[PFHIP_EOP_BC], nb, rc);
+      e->launches++;
+    }
+    int nsat = e->nlist[PFHIP_EOP_SAT];
+    if (nsat) {
+      rc = pfhip_ens_exec_saturation(e->lists[PFHIP_EOP_SAT], nsat, &e->st, NULL);
+      set_rc(e->lists[PFHIP_EOP_SAT], nsat, rc);
       e->launches++;
     }
   }

@@ -145,6 +145,8 @@ enum {
   PFHIP_EOP_RESIDUAL,
   PFHIP_EOP_MG_SETUP,   /* symmetric Jacobian into level 0 + coarse ops */
   PFHIP_EOP_MG_APPLY,   /* one V-cycle                                 */
+  PFHIP_EOP_BC,         /* BCPressure values at time dt                */
+  PFHIP_EOP_SAT,        /* Saturation of x into z (mode: CALCFCN/DER)  */
   PFHIP_EOP_COUNT
 };
 
@@ -198,6 +200,9 @@ int pfhip_ens_exec_mg_coarse(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *s
                              void *stream);
 int pfhip_ens_exec_mg_apply(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
                             void *stream);
+int pfhip_ens_exec_bc(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st, void *stream);
+int pfhip_ens_exec_saturation(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
+                              void *stream);
 /* level-0 matrix arrays (C, E, N, U) and precision of a multigrid */
 void pfhip_mg_level0(pfhip_mg *mg, void **A, int *prec);
 int pfhip_ens_stage_create(pfhip_ens_stage *st, long cap);
