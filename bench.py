@@ -122,10 +122,20 @@ def run_gpu(args):
 
     results = {}
     snaps = {}
+    shapes = []
     for mode in ("mixed", "plain"):
         if mode == "plain" and args.no_plain:
             continue
         sv = make(mode)
+        if mode == "mixed":
+            # smoother tile shape the multigrid's autotuner chose per grid level
+            L.pfhip_solver_mg.restype = api.C.c_void_p
+            L.pfhip_solver_mg.argtypes = [api.C.c_void_p]
+            L.pfhip_mg_pair_shapes.argtypes = [api.C.c_void_p, api.C.c_void_p, api.C.c_int]
+            arr = (api.C.c_int * 64)()
+            nl = L.pfhip_mg_pair_shapes(L.pfhip_solver_mg(sv.h), arr, 64)
+            shapes = list(arr[:nl])
+            _log(f"smoother tile shape per grid level (autotuned): {shapes}")
         # warmup on a scratch state
         if W > 0:
             dp, ds = gp.vec(p0), gp.vec()
@@ -173,6 +183,7 @@ def run_gpu(args):
             a = prob.interior(gp.get(snaps["mixed"][s]))
             b = prob.interior(gp.get(snaps["plain"][s]))
             errs.append(float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3))))
+    results["mixed"]["pair_shapes"] = shapes
     return prob, gp, results, errs
 
 
@@ -402,6 +413,7 @@ def main():
             "newton_per_step": mixed["newton"],
             "gmres_per_step": mixed["gmres"],
             "fevals_per_step": mixed["fevals"],
+            "mg_pair_shape_per_level": mixed["pair_shapes"],
         },
         "roofline": {k: dom[k] for k in ("bound", "achieved", "peak", "unit", "frac",
                                           "traffic", "kernel", "bytes_per_cell",

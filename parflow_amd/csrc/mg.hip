@@ -848,8 +848,11 @@ struct pfhip_mg {
   void *raw[MAXL][10];         // allocations; each array starts one element in, so
                                // the ghosted index of an even x is vector aligned
   int fuse;                    // red -> black pairs as one launch (k_rbgs_pair)
+  int pv[MAXL];                // pair tile shape of each grid level (tune_pairs)
   void *levdev;                // device LevelDev<T>[L]
 };
+
+template <typename T> static int tune_pairs(pfhip_mg *mg);
 
 template <typename T> static Mat<T> matof(pfhip_mg *mg, int l)
 {
@@ -950,6 +953,7 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
   }
   if (rc == PFHIP_OK)
     rc = (precision == 64) ? upload_levels<double>(mg) : upload_levels<float>(mg);
+  if (rc == PFHIP_OK) rc = (precision == 64) ? tune_pairs<double>(mg) : tune_pairs<float>(mg);
   if (rc != PFHIP_OK) { pfhip_mg_destroy(mg); return rc; }
   HIPCHK(hipDeviceSynchronize());
   *out = mg;
@@ -1056,15 +1060,15 @@ static int sweep_list(int symmetric, int zero, int max_iter, HalfSweep *seq, int
   return n;
 }
 
-// fused pair tile shapes; PFHIP_RBGS_VARIANT selects one (default 0)
-static int rbgs_variant()
+// Fused pair tile shapes (rows TY of a tile, z-chunk KC).  Each multigrid
+// times them per grid level when it is created (tune_pairs) and keeps the
+// fastest; PFHIP_RBGS_VARIANT=v forces shape v on every level.
+#define NPAIRV 5
+static int rbgs_forced()
 {
-  static int v = -1;
-  if (v < 0) {
-    const char *e = getenv("PFHIP_RBGS_VARIANT");
-    v = (e && *e) ? atoi(e) : 0;
-  }
-  return v;
+  const char *e = getenv("PFHIP_RBGS_VARIANT");
+  int v = (e && *e) ? atoi(e) : -1;
+  return (v >= 0 && v < NPAIRV) ? v : -1;
 }
 
 template <typename T, int TY, int KC>
@@ -1095,25 +1099,74 @@ static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xou
                        g, A, xin, xout, b);
 }
 
+// shape v: (TY, KC) = (14, 16), (6, 16), (6, 32), (14, 8), (14, 4)
 template <typename T>
 static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
-                        int init, hipStream_t st)
+                        int init, int v, hipStream_t st)
 {
-  if ((g.n[0] % 2) == 0 && rbgs_variant() < 10) {
+  if ((g.n[0] % 2) == 0) {
     // two points per lane (aligned vector loads need an even x extent)
-    switch (rbgs_variant()) {
+    switch (v) {
       case 1: launch_pair2_t<T, 6, 16>(g, A, xin, xout, b, init, st); return;
       case 2: launch_pair2_t<T, 6, 32>(g, A, xin, xout, b, init, st); return;
+      case 3: launch_pair2_t<T, 14, 8>(g, A, xin, xout, b, init, st); return;
+      case 4: launch_pair2_t<T, 14, 4>(g, A, xin, xout, b, init, st); return;
       default: launch_pair2_t<T, 14, 16>(g, A, xin, xout, b, init, st); return;
     }
   }
-  switch (rbgs_variant()) {
+  switch (v) {
     case 1: launch_pair_t<T, 6, 16>(g, A, xin, xout, b, init, st); break;
     case 2: launch_pair_t<T, 6, 32>(g, A, xin, xout, b, init, st); break;
     case 3: launch_pair_t<T, 14, 8>(g, A, xin, xout, b, init, st); break;
-    case 4: launch_pair_t<T, 2, 16>(g, A, xin, xout, b, init, st); break;
+    case 4: launch_pair_t<T, 14, 4>(g, A, xin, xout, b, init, st); break;
     default: launch_pair_t<T, 14, 16>(g, A, xin, xout, b, init, st); break;
   }
+}
+
+// The autotuner: every pair shape on every grid level of this hierarchy,
+// one untimed launch and then TUNE_REPS timed ones (HIP events), keeping the
+// fastest per level.  The level arrays are re-zeroed afterwards.
+#define TUNE_REPS 3
+template <typename T>
+static int tune_pairs(pfhip_mg *mg)
+{
+  int forced = rbgs_forced();
+  for (int l = 0; l < MAXL; l++) mg->pv[l] = (forced >= 0) ? forced : 0;
+  if (forced >= 0 || !mg->fuse) return PFHIP_OK;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (int l = 0; l < mg->lc; l++) {
+    if (!mg->xa[l]) continue;
+    LGeom g = lgeom(mg->n[l]);
+    Mat<T> A = matof<T>(mg, l);
+    float best = 0.f;
+    for (int v = 0; v < NPAIRV; v++) {
+      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], 0, v, 0);
+      HIPCHK(hipEventRecord(e0, 0));
+      for (int r = 0; r < TUNE_REPS; r++)
+        launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], 0, v, 0);
+      HIPCHK(hipEventRecord(e1, 0));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (v == 0 || ms < best) { best = ms; mg->pv[l] = v; }
+    }
+    size_t es = sizeof(T);
+    long sz = lsize(mg->n[l]);
+    HIPCHK(hipMemset(mg->raw[l][4], 0, (sz + 1) * es));
+    HIPCHK(hipMemset(mg->raw[l][9], 0, (sz + 1) * es));
+  }
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_mg_pair_shapes(pfhip_mg *mg, int *shape, int max)
+{
+  int n = mg->lc < max ? mg->lc : max;
+  for (int l = 0; l < n; l++) shape[l] = mg->pv[l];
+  return n;
 }
 
 // smoothing of grid level l: red -> black pairs run fused (x and its second
@@ -1135,7 +1188,7 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
   for (int q = 0; q < n;) {
     if (l == 0) pfhip_prof_mark(PFHIP_PROF_RBGS, 0, st);
     if (mg->fuse && mg->xa[l] && q + 1 < n && seq[q].par == 1 && seq[q + 1].par == 0) {
-      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, st);
+      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, mg->pv[l], st);
       void *tmp = mg->x[l]; mg->x[l] = mg->xa[l]; mg->xa[l] = tmp;
       q += 2;
     } else {

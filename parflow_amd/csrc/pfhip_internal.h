@@ -203,6 +203,9 @@ int pfhip_ens_exec_mg_apply(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st
 int pfhip_ens_exec_bc(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st, void *stream);
 int pfhip_ens_exec_saturation(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
                               void *stream);
+/* the smoother tile shape chosen for each grid level (its autotuner);
+ * returns the number of grid levels written */
+int pfhip_mg_pair_shapes(pfhip_mg *mg, int *shape, int max);
 /* level-0 matrix arrays (C, E, N, U) and precision of a multigrid */
 void pfhip_mg_level0(pfhip_mg *mg, void **A, int *prec);
 int pfhip_ens_stage_create(pfhip_ens_stage *st, long cap);

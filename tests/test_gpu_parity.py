@@ -250,6 +250,31 @@ def test_multigrid_grid_levels(gpu, precision, fuse, shape, monkeypatch):
             assert err < tol, (pre, post, sym, rep, err)
 
 
+@pytest.mark.parametrize("shape", [(70, 40, 24), (131, 20, 18)])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+def test_multigrid_pair_shapes(gpu, variant, shape, monkeypatch):
+    """Every smoother tile shape the autotuner chooses from, forced on all
+    grid levels (even and odd x extents), against the oracle V-cycle."""
+    api = _api()
+    monkeypatch.setenv("PFHIP_MG_SMALL", "300")
+    monkeypatch.setenv("PFHIP_RBGS_VARIANT", str(variant))
+    prob = cases.hetero(*shape, seed=23, dir_sides=True)
+    print(f"mg pair shape {variant} shape={shape}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p = cases.random_pressure(prob, seed=24)
+    J = O.jacobian_eval(op, p, 0.1, 0.1, 1)
+    rng = np.random.default_rng(25)
+    mgo = O.MG(prob, 2, 2, 1)
+    mgo.setup(J)
+    mgg = api.GPUMultigrid(gp, 2, 2, 1, 64)
+    mgg.setup(J)
+    b = np.zeros(prob.shape)
+    b[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+    err = _relmax(_I(prob, mgg.apply(b)), _I(prob, mgo.apply(b)))
+    assert err < 1e-11, err
+
+
 @pytest.mark.parametrize("overland,dirs", [(False, True), (True, False)])
 def test_residual_multi_tile(gpu, overland, dirs):
     """Residual on a grid spanning several 64-wide tiles, ragged tile rows
