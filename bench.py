@@ -127,15 +127,6 @@ def run_gpu(args):
         if mode == "plain" and args.no_plain:
             continue
         sv = make(mode)
-        if mode == "mixed":
-            # smoother tile shape the multigrid's autotuner chose per grid level
-            L.pfhip_solver_mg.restype = api.C.c_void_p
-            L.pfhip_solver_mg.argtypes = [api.C.c_void_p]
-            L.pfhip_mg_pair_shapes.argtypes = [api.C.c_void_p, api.C.c_void_p, api.C.c_int]
-            arr = (api.C.c_int * 64)()
-            nl = L.pfhip_mg_pair_shapes(L.pfhip_solver_mg(sv.h), arr, 64)
-            shapes = list(arr[:nl])
-            _log(f"smoother tile shape per grid level (autotuned): {shapes}")
         # warmup on a scratch state
         if W > 0:
             dp, ds = gp.vec(p0), gp.vec()
@@ -170,6 +161,16 @@ def run_gpu(args):
                                             api.C.byref(n)), "prof_read")
                 api.check(L.pfhip_prof_disable(getattr(api, pid)), "prof_off")
                 prof[key] = (ms.value, n.value)
+        if mode == "mixed":
+            # smoother tile shape the multigrid's autotuner chose per grid level
+            # (at its first V-cycle)
+            L.pfhip_solver_mg.restype = api.C.c_void_p
+            L.pfhip_solver_mg.argtypes = [api.C.c_void_p]
+            L.pfhip_mg_pair_shapes.argtypes = [api.C.c_void_p, api.C.c_void_p, api.C.c_int]
+            arr = (api.C.c_int * 64)()
+            nl = L.pfhip_mg_pair_shapes(L.pfhip_solver_mg(sv.h), arr, 64)
+            shapes = list(arr[:nl])
+            _log(f"smoother tile shape per grid level (autotuned): {shapes}")
         results[mode] = dict(elapsed=elapsed, newton=newton, gmres=gmres, fevals=fevals,
                              value=ncell * K / elapsed, prof=prof)
         snaps[mode] = snap

@@ -849,10 +849,12 @@ struct pfhip_mg {
                                // the ghosted index of an even x is vector aligned
   int fuse;                    // red -> black pairs as one launch (k_rbgs_pair)
   int pv[MAXL];                // pair tile shape of each grid level (tune_pairs)
+  int tuned;                   // pv chosen (at the first V-cycle)
   void *levdev;                // device LevelDev<T>[L]
 };
 
-template <typename T> static int tune_pairs(pfhip_mg *mg);
+template <typename T> static int tune_pairs(pfhip_mg *mg, hipStream_t st);
+static int rbgs_forced();
 
 template <typename T> static Mat<T> matof(pfhip_mg *mg, int l)
 {
@@ -953,8 +955,10 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
   }
   if (rc == PFHIP_OK)
     rc = (precision == 64) ? upload_levels<double>(mg) : upload_levels<float>(mg);
-  if (rc == PFHIP_OK) rc = (precision == 64) ? tune_pairs<double>(mg) : tune_pairs<float>(mg);
   if (rc != PFHIP_OK) { pfhip_mg_destroy(mg); return rc; }
+  int forced = rbgs_forced();
+  for (l = 0; l < MAXL; l++) mg->pv[l] = (forced >= 0) ? forced : 0;
+  mg->tuned = (forced >= 0) || !mg->fuse;
   HIPCHK(hipDeviceSynchronize());
   *out = mg;
   return PFHIP_OK;
@@ -1123,16 +1127,16 @@ static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, 
   }
 }
 
-// The autotuner: every pair shape on every grid level of this hierarchy,
-// one untimed launch and then TUNE_REPS timed ones (HIP events), keeping the
-// fastest per level.  The level arrays are re-zeroed afterwards.
+// The autotuner, run at the first V-cycle of a hierarchy (so on its real
+// operator, and never for hierarchies that the ensemble drives a workgroup
+// at a time): every pair shape on every grid level, one untimed launch and
+// then TUNE_REPS timed ones (HIP events), keeping the fastest per level.
+// The iterate buffers are re-zeroed afterwards.
 #define TUNE_REPS 3
 template <typename T>
-static int tune_pairs(pfhip_mg *mg)
+static int tune_pairs(pfhip_mg *mg, hipStream_t st)
 {
-  int forced = rbgs_forced();
-  for (int l = 0; l < MAXL; l++) mg->pv[l] = (forced >= 0) ? forced : 0;
-  if (forced >= 0 || !mg->fuse) return PFHIP_OK;
+  mg->tuned = 1;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -1142,20 +1146,20 @@ static int tune_pairs(pfhip_mg *mg)
     Mat<T> A = matof<T>(mg, l);
     float best = 0.f;
     for (int v = 0; v < NPAIRV; v++) {
-      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], 0, v, 0);
-      HIPCHK(hipEventRecord(e0, 0));
+      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], 0, v, st);
+      HIPCHK(hipEventRecord(e0, st));
       for (int r = 0; r < TUNE_REPS; r++)
-        launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], 0, v, 0);
-      HIPCHK(hipEventRecord(e1, 0));
+        launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], 0, v, st);
+      HIPCHK(hipEventRecord(e1, st));
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, e0, e1));
       if (v == 0 || ms < best) { best = ms; mg->pv[l] = v; }
     }
-    size_t es = sizeof(T);
-    long sz = lsize(mg->n[l]);
-    HIPCHK(hipMemset(mg->raw[l][4], 0, (sz + 1) * es));
-    HIPCHK(hipMemset(mg->raw[l][9], 0, (sz + 1) * es));
+    const long sz = lsize(mg->n[l]) + 1;
+    hipLaunchKernelGGL(k_fill_t<T>, dim3(256), dim3(256), 0, st, (T *)mg->raw[l][4], sz);
+    hipLaunchKernelGGL(k_fill_t<T>, dim3(256), dim3(256), 0, st, (T *)mg->raw[l][9], sz);
+    LAUNCHCHK();
   }
   HIPCHK(hipEventDestroy(e0));
   HIPCHK(hipEventDestroy(e1));
@@ -1207,6 +1211,7 @@ template <typename T>
 static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t st)
 {
   int L = mg->L, lc = mg->lc, rc;
+  if (!mg->tuned && (rc = tune_pairs<T>(mg, st))) return rc;
   LGeom g0 = lgeom(mg->n[0]);
   hipLaunchKernelGGL((k_convert<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, bin,
                      (T *)mg->b[0]);
