@@ -357,6 +357,30 @@ def test_golden_default_richards_on_gpu(gpu):
     assert log["newton"][0] == 3
 
 
+def test_pfidb_deck_to_pfb_outputs(gpu, tmp_path):
+    """A .pfidb deck in, ParFlow-named PFB outputs out (advance_deck),
+    compared with the reference's correct_output for default_richards."""
+    from parflow_amd import pfidb
+    from parflow_amd.pfb import read_pfb
+    import test_pfidb
+    keys = test_pfidb.default_richards_keys()
+    deck = tmp_path / "default_richards.pfidb"
+    pfidb.write_pfidb(keys, str(deck))
+    print("advance_deck default_richards.pfidb 10 steps", flush=True)
+    log = pfidb.advance_deck(str(deck), str(tmp_path / "out"), nsteps=10)
+    g = np.load(os.path.join(HERE, "golden", "default_richards.npz"))
+    for i, s in enumerate(g["steps"]):
+        if s > 10:
+            break
+        hdr, a = read_pfb(str(tmp_path / "out" / f"default_richards.out.press.{s:05d}.pfb"))
+        assert (hdr["nx"], hdr["ny"], hdr["nz"]) == (18, 15, 8)
+        e = np.max(np.abs(a - g["press"][i]) / np.maximum(np.abs(a), np.abs(g["press"][i])))
+        assert e < 6e-7, (s, e)
+        _, sat = read_pfb(str(tmp_path / "out" / f"default_richards.out.satur.{s:05d}.pfb"))
+        assert np.all((sat > 0.2) & (sat <= 0.99))
+    assert log["newton"][0] == 3
+
+
 def test_mixed_precision_close_to_plain(gpu):
     """FGMRES + FP32 V-cycle vs the plain FP64 path on a tight tolerance."""
     api = _api()

@@ -1,0 +1,243 @@
+"""CPU tests of the .pfidb deck input: the file format round trip, and the
+reference's default_richards and TiltedV_OverlandKin decks rebuilt from
+their input keys (the values of test/python/default_richards.py and
+test/python/overland_tiltedV_KWE.py, written here as key/value data) equal
+the restatements the solver parity tests run against the reference's
+correct_output."""
+import os
+
+import numpy as np
+import pytest
+
+import ref_decks
+from parflow_amd import pfidb
+from parflow_amd.pfb import write_pfb
+from parflow_amd.problem import BC_DIR_EQUIL
+
+
+def _box(d, inp, name, lo, up, patches=None):
+    d[f"GeomInput.{inp}.InputType"] = "Box"
+    d[f"GeomInput.{inp}.GeomName"] = name
+    for a, v in zip("XYZ", lo):
+        d[f"Geom.{name}.Lower.{a}"] = v
+    for a, v in zip("XYZ", up):
+        d[f"Geom.{name}.Upper.{a}"] = v
+    if patches:
+        d[f"Geom.{name}.Patches"] = patches
+
+
+def default_richards_keys():
+    d = {"Process.Topology.P": 1, "Process.Topology.Q": 1, "Process.Topology.R": 1,
+         "ComputationalGrid.Lower.X": -10.0, "ComputationalGrid.Lower.Y": 10.0,
+         "ComputationalGrid.Lower.Z": 1.0, "ComputationalGrid.DX": 8.8888888888888893,
+         "ComputationalGrid.DY": 10.666666666666666, "ComputationalGrid.DZ": 1.0,
+         "ComputationalGrid.NX": 18, "ComputationalGrid.NY": 15, "ComputationalGrid.NZ": 8,
+         "GeomInput.Names": "domain_input background_input source_region_input "
+                            "concen_region_input"}
+    _box(d, "domain_input", "domain", (-10.0, 10.0, 1.0), (150.0, 170.0, 9.0),
+         "left right front back bottom top")
+    _box(d, "background_input", "background", (-99999999.0,) * 3, (99999999.0,) * 3)
+    _box(d, "source_region_input", "source_region", (65.56, 79.34, 4.5), (74.44, 89.99, 5.5))
+    _box(d, "concen_region_input", "concen_region", (60.0, 80.0, 4.0), (80.0, 100.0, 6.0))
+    d.update({
+        "Geom.Perm.Names": "background", "Geom.background.Perm.Type": "Constant",
+        "Geom.background.Perm.Value": 4.0, "Perm.TensorType": "TensorByGeom",
+        "Geom.Perm.TensorByGeom.Names": "background",
+        "Geom.background.Perm.TensorValX": 1.0, "Geom.background.Perm.TensorValY": 1.0,
+        "Geom.background.Perm.TensorValZ": 1.0,
+        "SpecificStorage.Type": "Constant", "SpecificStorage.GeomNames": "domain",
+        "Geom.domain.SpecificStorage.Value": 1.0e-4,
+        "Phase.Names": "water", "Phase.water.Density.Type": "Constant",
+        "Phase.water.Density.Value": 1.0, "Phase.water.Viscosity.Type": "Constant",
+        "Phase.water.Viscosity.Value": 1.0, "Contaminants.Names": "",
+        "Geom.Retardation.GeomNames": "", "Gravity": 1.0,
+        "TimingInfo.BaseUnit": 1.0, "TimingInfo.StartCount": 0,
+        "TimingInfo.StartTime": 0.0, "TimingInfo.StopTime": 1.0,
+        "TimingInfo.DumpInterval": -1, "TimeStep.Type": "Constant", "TimeStep.Value": 0.001,
+        "Geom.Porosity.GeomNames": "background", "Geom.background.Porosity.Type": "Constant",
+        "Geom.background.Porosity.Value": 1.0, "Domain.GeomName": "domain",
+        "Phase.RelPerm.Type": "VanGenuchten", "Phase.RelPerm.GeomNames": "domain",
+        "Geom.domain.RelPerm.Alpha": 0.005, "Geom.domain.RelPerm.N": 2.0,
+        "Phase.Saturation.Type": "VanGenuchten", "Phase.Saturation.GeomNames": "domain",
+        "Geom.domain.Saturation.Alpha": 0.005, "Geom.domain.Saturation.N": 2.0,
+        "Geom.domain.Saturation.SRes": 0.2, "Geom.domain.Saturation.SSat": 0.99,
+        "Wells.Names": "", "Cycle.Names": "constant", "Cycle.constant.Names": "alltime",
+        "Cycle.constant.alltime.Length": 1, "Cycle.constant.Repeat": -1,
+        "BCPressure.PatchNames": "left right front back bottom top"})
+    for p, v in (("left", 5.0), ("right", 3.0)):
+        d.update({f"Patch.{p}.BCPressure.Type": "DirEquilRefPatch",
+                  f"Patch.{p}.BCPressure.Cycle": "constant",
+                  f"Patch.{p}.BCPressure.RefGeom": "domain",
+                  f"Patch.{p}.BCPressure.RefPatch": "bottom",
+                  f"Patch.{p}.BCPressure.alltime.Value": v})
+    for p in ("front", "back", "bottom", "top"):
+        d.update({f"Patch.{p}.BCPressure.Type": "FluxConst",
+                  f"Patch.{p}.BCPressure.Cycle": "constant",
+                  f"Patch.{p}.BCPressure.alltime.Value": 0.0})
+    d.update({
+        "TopoSlopesX.Type": "Constant", "TopoSlopesX.GeomNames": "domain",
+        "TopoSlopesX.Geom.domain.Value": 0.0, "TopoSlopesY.Type": "Constant",
+        "TopoSlopesY.GeomNames": "domain", "TopoSlopesY.Geom.domain.Value": 0.0,
+        "Mannings.Type": "Constant", "Mannings.GeomNames": "domain",
+        "Mannings.Geom.domain.Value": 0.0,
+        "ICPressure.Type": "HydroStaticPatch", "ICPressure.GeomNames": "domain",
+        "Geom.domain.ICPressure.Value": 3.0, "Geom.domain.ICPressure.RefGeom": "domain",
+        "Geom.domain.ICPressure.RefPatch": "bottom",
+        "PhaseSources.water.Type": "Constant", "PhaseSources.water.GeomNames": "background",
+        "PhaseSources.water.Geom.background.Value": 0.0,
+        "KnownSolution": "NoKnownSolution", "Solver": "Richards", "Solver.MaxIter": 50,
+        "Solver.Nonlinear.MaxIter": 10, "Solver.Nonlinear.ResidualTol": 1e-9,
+        "Solver.Nonlinear.EtaChoice": "EtaConstant", "Solver.Nonlinear.EtaValue": 1e-5,
+        "Solver.Nonlinear.UseJacobian": True, "Solver.Nonlinear.DerivativeEpsilon": 1e-2,
+        "Solver.Linear.KrylovDimension": 10, "Solver.Linear.Preconditioner": "PFMG",
+        "Solver.PrintVelocities": True})
+    return d
+
+
+def tiltedv_kin_keys():
+    d = {"FileVersion": 4,
+         "ComputationalGrid.Lower.X": 0.0, "ComputationalGrid.Lower.Y": 0.0,
+         "ComputationalGrid.Lower.Z": 0.0, "ComputationalGrid.NX": 5,
+         "ComputationalGrid.NY": 5, "ComputationalGrid.NZ": 1,
+         "ComputationalGrid.DX": 10.0, "ComputationalGrid.DY": 10.0,
+         "ComputationalGrid.DZ": 0.05,
+         "GeomInput.Names": "domaininput leftinput rightinput channelinput"}
+    _box(d, "domaininput", "domain", (0.0, 0.0, 0.0), (50.0, 50.0, 0.05),
+         "x_lower x_upper y_lower y_upper z_lower z_upper")
+    _box(d, "leftinput", "left", (0.0, 0.0, 0.0), (20.0, 50.0, 0.05))
+    _box(d, "rightinput", "right", (30.0, 0.0, 0.0), (50.0, 50.0, 0.05))
+    _box(d, "channelinput", "channel", (20.0, 0.0, 0.0), (30.0, 50.0, 0.05))
+    d.update({
+        "Geom.Perm.Names": "domain", "Geom.domain.Perm.Type": "Constant",
+        "Geom.domain.Perm.Value": 0.0000001, "Perm.TensorType": "TensorByGeom",
+        "Geom.Perm.TensorByGeom.Names": "domain", "Geom.domain.Perm.TensorValX": 1.0,
+        "Geom.domain.Perm.TensorValY": 1.0, "Geom.domain.Perm.TensorValZ": 1.0,
+        "SpecificStorage.Type": "Constant", "SpecificStorage.GeomNames": "domain",
+        "Geom.domain.SpecificStorage.Value": 1.0e-4, "Phase.Names": "water",
+        "Phase.water.Density.Type": "Constant", "Phase.water.Density.Value": 1.0,
+        "Phase.water.Viscosity.Type": "Constant", "Phase.water.Viscosity.Value": 1.0,
+        "Gravity": 1.0, "TimingInfo.BaseUnit": 0.05, "TimingInfo.StartCount": 0,
+        "TimingInfo.StartTime": 0.0, "TimingInfo.StopTime": 2.0,
+        "TimingInfo.DumpInterval": -2, "TimeStep.Type": "Constant", "TimeStep.Value": 0.05,
+        "Geom.Porosity.GeomNames": "domain", "Geom.domain.Porosity.Type": "Constant",
+        "Geom.domain.Porosity.Value": 0.01, "Domain.GeomName": "domain",
+        "Phase.RelPerm.Type": "VanGenuchten", "Phase.RelPerm.GeomNames": "domain",
+        "Geom.domain.RelPerm.Alpha": 6.0, "Geom.domain.RelPerm.N": 2.0,
+        "Phase.Saturation.Type": "VanGenuchten", "Phase.Saturation.GeomNames": "domain",
+        "Geom.domain.Saturation.Alpha": 6.0, "Geom.domain.Saturation.N": 2.0,
+        "Geom.domain.Saturation.SRes": 0.2, "Geom.domain.Saturation.SSat": 1.0,
+        "Wells.Names": "", "Cycle.Names": "constant rainrec",
+        "Cycle.constant.Names": "alltime", "Cycle.constant.alltime.Length": 1,
+        "Cycle.constant.Repeat": -1, "Cycle.rainrec.Names": "rain rec",
+        "Cycle.rainrec.rain.Length": 2, "Cycle.rainrec.rec.Length": 300,
+        "Cycle.rainrec.Repeat": -1,
+        "BCPressure.PatchNames": "x_lower x_upper y_lower y_upper z_lower z_upper"})
+    for p in ("x_lower", "y_lower", "z_lower", "x_upper", "y_upper"):
+        d.update({f"Patch.{p}.BCPressure.Type": "FluxConst",
+                  f"Patch.{p}.BCPressure.Cycle": "constant",
+                  f"Patch.{p}.BCPressure.alltime.Value": 0.0})
+    d.update({
+        "Patch.z_upper.BCPressure.Type": "OverlandKinematic",
+        "Patch.z_upper.BCPressure.Cycle": "rainrec",
+        "Patch.z_upper.BCPressure.rain.Value": -0.01,
+        "Patch.z_upper.BCPressure.rec.Value": 0.0000,
+        "Mannings.Type": "Constant", "Mannings.GeomNames": "domain",
+        "Mannings.Geom.domain.Value": 3.0e-6,
+        "PhaseSources.water.Type": "Constant", "PhaseSources.water.GeomNames": "domain",
+        "PhaseSources.water.Geom.domain.Value": 0.0, "Solver": "Richards",
+        "Solver.MaxIter": 2500, "Solver.Nonlinear.MaxIter": 100,
+        "Solver.Nonlinear.ResidualTol": 1e-9, "Solver.Nonlinear.EtaChoice": "EtaConstant",
+        "Solver.Nonlinear.EtaValue": 0.01, "Solver.Nonlinear.UseJacobian": False,
+        "Solver.Nonlinear.DerivativeEpsilon": 1e-15, "Solver.Nonlinear.StepTol": 1e-20,
+        "Solver.Nonlinear.Globalization": "LineSearch",
+        "Solver.Linear.KrylovDimension": 50, "Solver.Linear.MaxRestart": 2,
+        "Solver.OverlandKinematic.Epsilon": 1e-5, "Solver.Linear.Preconditioner": "PFMG",
+        "Solver.Linear.Preconditioner.PCMatrixType": "PFSymmetric",
+        "ICPressure.Type": "HydroStaticPatch", "ICPressure.GeomNames": "domain",
+        "Geom.domain.ICPressure.Value": -3.0, "Geom.domain.ICPressure.RefGeom": "domain",
+        "Geom.domain.ICPressure.RefPatch": "z_upper",
+        "TopoSlopesX.Type": "Constant", "TopoSlopesX.GeomNames": "left right channel",
+        "TopoSlopesX.Geom.left.Value": -0.01, "TopoSlopesX.Geom.right.Value": 0.01,
+        "TopoSlopesX.Geom.channel.Value": 0.01,
+        "TopoSlopesY.Type": "Constant", "TopoSlopesY.GeomNames": "domain",
+        "TopoSlopesY.Geom.domain.Value": 0.01})
+    return d
+
+
+def _same_problem(a, b):
+    for f in ("nx", "ny", "nz", "dx", "dy", "dz", "x0", "y0", "z0", "rho", "mu", "gravity",
+              "base_unit", "ov_eps"):
+        assert getattr(a, f) == getattr(b, f), f
+    for f in ("g_kx", "g_ky", "g_kz", "g_poro", "g_ss", "g_alpha", "g_nvg", "g_sres",
+              "g_ssat"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    for f in ("g_slope_x", "g_slope_y", "g_mann"):
+        if hasattr(b, f) and b.overland:
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    assert a.overland == b.overland
+    for fa, fb in zip(a.bc, b.bc):
+        assert fa.type == fb.type and tuple(fa.values) == tuple(fb.values)
+        assert tuple(fa.lengths) == tuple(fb.lengths) and fa.repeat == fb.repeat
+        if fa.type == BC_DIR_EQUIL:
+            assert fa.z_ref == fb.z_ref
+
+
+def test_pfidb_round_trip(tmp_path):
+    d = {"A.B": "1.5", "Names": "left right  front", "Empty": "", "Flag": True,
+         "Text.With.Spaces": "a b c"}
+    path = tmp_path / "x.pfidb"
+    pfidb.write_pfidb(d, str(path))
+    back = pfidb.read_pfidb(str(path))
+    assert back == {k: pfidb._str(v) for k, v in d.items()}
+    lines = path.read_text().splitlines()
+    assert lines[0] == str(len(d)) and lines[1] == "3" and lines[2] == "A.B"
+
+
+@pytest.mark.parametrize("which", ["default_richards", "tiltedv_kin"])
+def test_reference_decks_from_keys(tmp_path, which):
+    keys = default_richards_keys() if which == "default_richards" else tiltedv_kin_keys()
+    ref = getattr(ref_decks, which)()
+    path = tmp_path / f"{which}.pfidb"
+    pfidb.write_pfidb(keys, str(path))
+    prob, opts, p0, dt, n = pfidb.problem_from_pfidb(str(path))
+    rprob, ropts, rp0, rdt, rn = ref
+    _same_problem(prob, rprob)
+    assert opts == ropts
+    assert np.array_equal(p0, rp0)
+    assert dt == rdt
+    assert n == rn
+
+
+def test_pfb_file_fields(tmp_path):
+    """Perm and the initial pressure read from PFB files beside the deck."""
+    keys = default_richards_keys()
+    rng = np.random.default_rng(5)
+    K = np.exp(rng.normal(size=(8, 15, 18)))
+    P = rng.normal(size=(8, 15, 18))
+    write_pfb(str(tmp_path / "perm.pfb"), K)
+    write_pfb(str(tmp_path / "press.pfb"), P)
+    keys["Geom.background.Perm.Type"] = "PFBFile"
+    keys["Geom.background.Perm.FileName"] = "perm.pfb"
+    keys["ICPressure.Type"] = "PFBFile"
+    keys["Geom.domain.ICPressure.FileName"] = "press.pfb"
+    path = tmp_path / "deck.pfidb"
+    pfidb.write_pfidb(keys, str(path))
+    prob, opts, p0, dt, n = pfidb.problem_from_pfidb(str(path))
+    assert np.array_equal(prob.interior(prob.g_kx), K)
+    assert np.array_equal(prob.interior(p0), P)
+
+
+@pytest.mark.parametrize("key,value", [
+    ("GeomInput.domain_input.InputType", "SolidFile"),
+    ("Wells.Names", "w1"),
+    ("Solver.TerrainFollowingGrid", True),
+    ("Patch.top.BCPressure.Type", "SeepageFace"),
+    ("Solver.Linear.Preconditioner.PCMatrixType", "FullJacobian"),
+    ("Geom.domain.RelPerm.N", 3.0),
+    ("PhaseSources.water.Geom.background.Value", 1.0),
+])
+def test_unsupported_keys_raise(key, value):
+    keys = default_richards_keys()
+    keys[key] = value
+    with pytest.raises(ValueError):
+        pfidb.problem_from_pfidb({k: pfidb._str(v) for k, v in keys.items()})
