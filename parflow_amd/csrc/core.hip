@@ -290,7 +290,7 @@ extern "C" int pfhip_problem_create(const pfhip_problem_desc *desc,
   if (rc == PFHIP_OK) rc = pfhip_dev_alloc(&pb->dkrho, ng);
   for (int q = 0; q < 4 && rc == PFHIP_OK; q++) rc = pfhip_dev_alloc(&pb->q2[q], pb->n2);
   if (rc == PFHIP_OK) rc = pfhip_dev_alloc(&pb->red_partial, PFHIP_RED_MAX_BLOCKS * PFHIP_MAX_MULTI);
-  if (rc == PFHIP_OK) rc = pfhip_dev_alloc(&pb->red_dev, PFHIP_MAX_MULTI);
+  if (rc == PFHIP_OK) rc = pfhip_dev_alloc(&pb->red_dev, PFHIP_RED_SLOTS);
   if (rc == PFHIP_OK) {
     hipError_t e = hipHostMalloc((void **)&pb->h_red, sizeof(double) * PFHIP_MAX_MULTI);
     if (e != hipSuccess) { pfhip_set_error("hipHostMalloc"); rc = PFHIP_ERR_NOMEM; }
@@ -780,6 +780,103 @@ static int multi_axpy_impl(pfhip_problem *pb, const double *const *V, int m,
   return PFHIP_OK;
 }
 
+// w -= sum_q h[q] V[q] with h read from device memory (the coefficients the
+// previous multi-dot left there); optional ||w||^2 partials
+template <bool NORM>
+__global__ __launch_bounds__(256) void k_multi_axpy_d(Geom g, VecPtrs V,
+                                                      const double *__restrict__ h, int m,
+                                                      double *__restrict__ w,
+                                                      double *__restrict__ partial, int nb)
+{
+  double c[MD_CHUNK];
+#pragma unroll
+  for (int q = 0; q < MD_CHUNK; q++) c[q] = (q < m) ? -1.0 * h[q] : 0.0;
+  double acc = 0.0;
+  int rows = g.ny * g.nz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    int j = r % g.ny, k = r / g.ny;
+    int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+      int ip = base + i;
+      double s = w[ip];
+#pragma unroll
+      for (int q = 0; q < MD_CHUNK; q++)
+        if (q < m) s += c[q] * V.v[q][ip];
+      w[ip] = s;
+      if (NORM) acc += s * s;
+    }
+  }
+  if (NORM) {
+    double s = block_reduce<false>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+// one device pass of h = V^T w (m values into dst)
+static void multi_dot_dev(pfhip_problem *pb, const Geom &g, int nb, const double *const *V,
+                          int m, const double *w, double *dst, hipStream_t st)
+{
+  for (int c0 = 0; c0 < m; c0 += MD_CHUNK) {
+    int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
+    VecPtrs P;
+    for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < mc) ? V[c0 + q] : w;
+    hipLaunchKernelGGL(k_multi_dot, dim3(nb), dim3(256), 0, st, g, P, mc, w,
+                       pb->red_partial + (long)c0 * nb, nb);
+  }
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, st, pb->red_partial, nb, m,
+                     dst);
+}
+
+// w -= V h with h on the device; the last chunk optionally adds ||w||^2
+// partials, reduced into *n2dst
+static void multi_axpy_dev(pfhip_problem *pb, const Geom &g, int nb, const double *const *V,
+                           int m, const double *h, double *w, double *n2dst, hipStream_t st)
+{
+  for (int c0 = 0; c0 < m; c0 += MD_CHUNK) {
+    int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
+    VecPtrs P;
+    for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < mc) ? V[c0 + q] : w;
+    bool last = (c0 + MD_CHUNK >= m);
+    if (last && n2dst)
+      hipLaunchKernelGGL(k_multi_axpy_d<true>, dim3(nb), dim3(256), 0, st, g, P, h + c0, mc,
+                         w, pb->red_partial, nb);
+    else
+      hipLaunchKernelGGL(k_multi_axpy_d<false>, dim3(nb), dim3(256), 0, st, g, P, h + c0, mc,
+                         w, pb->red_partial, nb);
+  }
+  if (n2dst)
+    hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, st, pb->red_partial, nb, 1,
+                       n2dst);
+}
+
+// CGS2 against V[0..k-1] with one host synchronisation: h1 = V^T w,
+// w -= V h1, h2 = V^T w, w -= V h2 and ||w||^2, the coefficients passing
+// from each reduction to the next update on the device.  The arithmetic is
+// that of the separate multi-dot / multi-axpy calls.
+int pfhip_vec_cgs2(pfhip_problem *pb, const double *const *V, int k, double *w, double *h1,
+                   double *h2, double *norm2, void *stream)
+{
+  if (k < 1 || k > PFHIP_SLOT_H2) {
+    pfhip_set_error("cgs2: k=%d outside 1..%d", k, PFHIP_SLOT_H2);
+    return PFHIP_ERR_ARG;
+  }
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  hipStream_t st = STREAM(stream);
+  double *d1 = pb->red_dev, *d2 = pb->red_dev + PFHIP_SLOT_H2;
+  double *dn = pb->red_dev + PFHIP_SLOT_N2;
+  multi_dot_dev(pb, g, nb, V, k, w, d1, st);
+  multi_axpy_dev(pb, g, nb, V, k, d1, w, nullptr, st);
+  multi_dot_dev(pb, g, nb, V, k, w, d2, st);
+  multi_axpy_dev(pb, g, nb, V, k, d2, w, dn, st);
+  LAUNCHCHK();
+  int rc = fetch_results(pb, PFHIP_MAX_MULTI, stream);
+  if (rc) return rc;
+  for (int q = 0; q < k; q++) { h1[q] = pb->h_red[q]; h2[q] = pb->h_red[PFHIP_SLOT_H2 + q]; }
+  *norm2 = pb->h_red[PFHIP_SLOT_N2];
+  return PFHIP_OK;
+}
+
 extern "C" int pfhip_vec_multi_axpy(pfhip_problem *pb, const double *const *V,
                                     int m, const double *h, double *w,
                                     double *norm2_out, void *stream)
@@ -1184,6 +1281,10 @@ struct ResArgs {
   const double *__restrict__ fsub;
   double *__restrict__ pout;
   double psig, fsc;
+  // MODE 1 with dq set: sigma and 1/sigma of KINSpgmrAtimesDQ from the
+  // device sums dq = (u.v, v.v, |v|_1) and sqrt(relfunc) sqrf, in the kernel
+  const double *__restrict__ dq;
+  double sqrf;
   const double *__restrict__ p_old;
   const double *__restrict__ s_old;
   const double *__restrict__ kx;
@@ -1514,16 +1615,33 @@ __device__ __forceinline__ double ldb(const double *__restrict__ base, int ip)
 }
 
 // the pressure the residual is evaluated at: p, or p + psig pv
-template <int MODE> __device__ __forceinline__ double p_at(const ResArgs &A, int ip)
+template <int MODE>
+__device__ __forceinline__ double p_at(const ResArgs &A, int ip, double psig)
 {
-  if constexpr (MODE != 0) return ldb(A.p, ip) + A.psig * ldb(A.pv, ip);
+  if constexpr (MODE != 0) return ldb(A.p, ip) + psig * ldb(A.pv, ip);
   else return ldb(A.p, ip);
 }
 
-template <bool PCS, int MODE>
-__device__ __forceinline__ void load_vg(VGIn<PCS> &v, const ResArgs &A, int ip)
+// the form's scalars: sigma and the output scale, from the device sums of
+// the difference quotient when given (the expression of atimes in
+// kinsol_host.c, KS/kinspgmr.c:500-515)
+__device__ __forceinline__ void form_scalars(const ResArgs &A, double &psig, double &fsc)
 {
-  v.p = p_at<MODE>(A, ip); v.al = ldb(A.f.alpha, ip); v.n = ldb(A.f.nvg, ip);
+  psig = A.psig;
+  fsc = A.fsc;
+  if (A.dq) {
+    const double sutsv = A.dq[0], vtv = A.dq[1], sq1norm = A.dq[2];
+    const double sign = (sutsv >= 0.0) ? 1.0 : -1.0;
+    const double a = fabs(sutsv);
+    psig = sign * A.sqrf * ((a > sq1norm) ? a : sq1norm) / vtv;
+    fsc = 1.0 / psig;
+  }
+}
+
+template <bool PCS, int MODE>
+__device__ __forceinline__ void load_vg(VGIn<PCS> &v, const ResArgs &A, int ip, double psig)
+{
+  v.p = p_at<MODE>(A, ip, psig); v.al = ldb(A.f.alpha, ip); v.n = ldb(A.f.nvg, ip);
   v.kz = ldb(A.kz, ip);
   if constexpr (PCS) { v.sr = ldb(A.f.sres, ip); v.st = ldb(A.f.ssat, ip); }
 }
@@ -1550,9 +1668,9 @@ __device__ __forceinline__ void load_oth(OthIn &o, const ResArgs &A, int ip, int
   if constexpr (MODE == 1) o.fs = ldb(A.fsub, ip);
 }
 template <int MODE>
-__device__ __forceinline__ void load_hal(HalIn &h, const ResArgs &A, int hip)
+__device__ __forceinline__ void load_hal(HalIn &h, const ResArgs &A, int hip, double psig)
 {
-  h.p = p_at<MODE>(A, hip); h.al = ldb(A.f.alpha, hip); h.n = ldb(A.f.nvg, hip);
+  h.p = p_at<MODE>(A, hip, psig); h.al = ldb(A.f.alpha, hip); h.n = ldb(A.f.nvg, hip);
 }
 
 template <int TX, int TY, int ZC, bool PCS, int MODE, bool ISO>
@@ -1567,6 +1685,8 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
   __shared__ double s_uy[(TY + 1) * TX];   // y faces: south face of tile row ty at ty
   const Geom g = A.g;
   const Phys &f = A.f;
+  double psig = 0.0, fsc = 0.0;
+  if constexpr (MODE != 0) form_scalars(A, psig, fsc);
   const double rho = f.rho, grav = f.g, irg = f.irg, imu = A.imu;
   const double hrg = 0.5 * rho * grav * 1.0;
   const double dx = A.dx, dy = A.dy, dz = A.dz;
@@ -1608,10 +1728,10 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
   double p_n = 0, kr_n = 0, S_n = 0, kz_n = 0;
   {
     VGIn<PCS> v0;
-    load_vg<PCS, MODE>(v0, A, col + plane(k0));
+    load_vg<PCS, MODE>(v0, A, col + plane(k0), psig);
     if (k0 > 0) {
       VGIn<PCS> vm;
-      load_vg<PCS, MODE>(vm, A, col + plane(k0 - 1));
+      load_vg<PCS, MODE>(vm, A, col + plane(k0 - 1), psig);
       p_m = vm.p;
       kr_m = vg_kr(p_m, irg, vm.al, vm.n) * rho;
       kz_m = vm.kz;
@@ -1627,8 +1747,8 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
 
   VGIn<PCS> vgN;
   HalIn hal;
-  load_vg<PCS, MODE>(vgN, A, col + plane(k0 + 1));
-  load_hal<MODE>(hal, A, hcol + plane(k0));
+  load_vg<PCS, MODE>(vgN, A, col + plane(k0 + 1), psig);
+  load_hal<MODE>(hal, A, hcol + plane(k0), psig);
 
   for (int k = k0; k < k1; k++) {
     const int ip = col + (k + 1) * sz;
@@ -1639,8 +1759,8 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
     if constexpr (ISO) { o.kx = kz_c; o.ky = kz_c; }
     const VGIn<PCS> vgU = vgN;
     const HalIn h = hal;
-    load_vg<PCS, MODE>(vgN, A, col + plane(k + 2));
-    load_hal<MODE>(hal, A, hcol + plane(k + 1));
+    load_vg<PCS, MODE>(vgN, A, col + plane(k + 2), psig);
+    load_hal<MODE>(hal, A, hcol + plane(k + 1), psig);
     // ---- A: closures of the own cell one plane ahead and of the halo cell
     if (k + 1 < nz) {
       p_n = vgU.p;
@@ -1682,7 +1802,7 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
         // boundary cells get the transform after their face terms
         const bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
                          (k == nz - 1);
-        fv = bnd ? fv : A.fsc * fv - A.fsc * o.fs;
+        fv = bnd ? fv : fsc * fv - fsc * o.fs;
       }
       if constexpr (MODE == 2) A.pout[ip] = p_c;
       A.F[ip] = fv;
@@ -1755,10 +1875,12 @@ __global__ __launch_bounds__(256) void k_residual_bc(ResArgs A, long nshell)
   int i, j, k;
   if (!shell_cell(A.g, idx, i, j, k)) return;
   const int ip = (i + 1) + (j + 1) * A.g.sy + (k + 1) * A.g.sz;
-  double pc = p_at<MODE>(A, ip);
+  double psig = 0.0, fsc = 0.0;
+  if constexpr (MODE != 0) form_scalars(A, psig, fsc);
+  double pc = p_at<MODE>(A, ip, psig);
   double krc = vg_kr(pc, A.f.irg, A.f.alpha[ip], A.f.nvg[ip]) * A.f.rho;
   double fv = residual_bc(A, i, j, k, ip, A.F[ip], pc, krc, A.kx[ip], A.ky[ip], A.kz[ip]);
-  if constexpr (MODE == 1) fv = A.fsc * fv - A.fsc * A.fsub[ip];
+  if constexpr (MODE == 1) fv = fsc * fv - fsc * A.fsub[ip];
   A.F[ip] = fv;
 }
 
@@ -1829,6 +1951,7 @@ static ResArgs res_args(const pfhip_problem *pb, const double *p_old, const doub
   for (int f = 0; f < 6; f++) { A.bcv[f] = pb->bcv[f]; A.bct[f] = pb->d.bc[f].type; }
   A.p = nullptr; A.F = nullptr;
   A.pv = nullptr; A.fsub = nullptr; A.pout = nullptr; A.psig = 0.0; A.fsc = 0.0;
+  A.dq = nullptr; A.sqrf = 0.0;
   return A;
 }
 
@@ -1885,6 +2008,35 @@ int pfhip_residual_jv(pfhip_problem *pb, const double *u, const double *v, doubl
   A.p = u;
   A.F = z;
   A.pv = v; A.fsub = fval; A.psig = sigma; A.fsc = sinv;
+  launch_residual(A, STREAM(stream));
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// The same J.v with sigma taken on the device: the DQ sums (u.v, v.v,
+// |v|_1) stay in the problem's result slots and the residual kernel forms
+// sigma and 1/sigma from them, so no host synchronisation separates the
+// reduction from the residual.
+int pfhip_residual_jv_dq(pfhip_problem *pb, const double *u, const double *v, double sqrf,
+                         const double *fval, double *z, const double *p_old,
+                         const double *s_old, double dt, void *stream)
+{
+  if (pb->overland) {
+    pfhip_set_error("residual_jv: overland problems take the two-pass form");
+    return PFHIP_ERR_ARG;
+  }
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  hipLaunchKernelGGL(k_reduce<RED_DQ>, dim3(nb), dim3(256), 0, STREAM(stream), g, u, v,
+                     pb->red_partial, nb);
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, STREAM(stream),
+                     pb->red_partial, nb, 3, pb->red_dev + PFHIP_SLOT_DQ);
+  LAUNCHCHK();
+  ResArgs A = res_args(pb, p_old, s_old, dt);
+  A.p = u;
+  A.F = z;
+  A.pv = v; A.fsub = fval;
+  A.dq = pb->red_dev + PFHIP_SLOT_DQ; A.sqrf = sqrf;
   launch_residual(A, STREAM(stream));
   LAUNCHCHK();
   return PFHIP_OK;

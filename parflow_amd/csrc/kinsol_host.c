@@ -59,7 +59,8 @@ const pfhip_ops pfhip_direct_ops = {
   pfhip_vec_max_norm, pfhip_vec_scsteplength, pfhip_vec_dq_sums, pfhip_vec_multi_dot,
   pfhip_vec_multi_axpy, pfhip_vec_combine, pfhip_kin_stop_norms, pfhip_nl_function_eval,
   pfhip_residual_jv, pfhip_residual_step, pfhip_mg_setup_jacobian, pfhip_mg_apply,
-  pfhip_bc_pressure, pfhip_saturation, pfhip_synchronize,
+  pfhip_bc_pressure, pfhip_saturation, pfhip_synchronize, pfhip_residual_jv_dq,
+  pfhip_vec_cgs2,
 };
 
 void pfhip_solver_set_ops(pfhip_solver *s, const pfhip_ops *ops) { s->ops = ops; }
@@ -183,6 +184,12 @@ static int atimes(pfhip_solver *s, const double *v, double *z)
     }
     return pfhip_matvec(pb, (const double *const *)s->J, 0, v, z, NULL);
   }
+  if (!pb->overland && s->ops->residual_jv_dq) {
+    /* the DQ sums, sigma and the quotient without leaving the device */
+    s->nfe++;
+    return s->ops->residual_jv_dq(pb, s->uu, v, s->sqrt_relfunc, s->fval, z, s->cur_p_old,
+                                  s->cur_s_old, s->dt, NULL);
+  }
   double sutsv, vtv, sq1norm;
   CHK(s->ops->vec_dq_sums(pb, s->uu, v, &sutsv, &vtv, &sq1norm, NULL));
   double sign = (sutsv >= 0.0) ? 1.0 : -1.0;
@@ -245,10 +252,14 @@ static int cgs2(pfhip_solver *s, int k, double *new_vk_norm)
   double *h1 = s->hbuf, *h2 = s->hbuf + (s->l_max + 2);
   int km1 = k - 1;
   double n2;
-  CHK(s->ops->vec_multi_dot(pb, (const double *const *)s->V, k, s->V[k], h1, NULL));
-  CHK(s->ops->vec_multi_axpy(pb, (const double *const *)s->V, k, h1, s->V[k], NULL, NULL));
-  CHK(s->ops->vec_multi_dot(pb, (const double *const *)s->V, k, s->V[k], h2, NULL));
-  CHK(s->ops->vec_multi_axpy(pb, (const double *const *)s->V, k, h2, s->V[k], &n2, NULL));
+  if (s->ops->cgs2 && k <= PFHIP_SLOT_H2) {
+    CHK(s->ops->cgs2(pb, (const double *const *)s->V, k, s->V[k], h1, h2, &n2, NULL));
+  } else {
+    CHK(s->ops->vec_multi_dot(pb, (const double *const *)s->V, k, s->V[k], h1, NULL));
+    CHK(s->ops->vec_multi_axpy(pb, (const double *const *)s->V, k, h1, s->V[k], NULL, NULL));
+    CHK(s->ops->vec_multi_dot(pb, (const double *const *)s->V, k, s->V[k], h2, NULL));
+    CHK(s->ops->vec_multi_axpy(pb, (const double *const *)s->V, k, h2, s->V[k], &n2, NULL));
+  }
   for (int i = 0; i < k; i++) s->Hes[i][km1] = h1[i] + h2[i];
   *new_vk_norm = sqrt(n2 > 0.0 ? n2 : 0.0);
   return PFHIP_OK;

@@ -23,12 +23,17 @@ struct pfhip_problem {
   double *krho, *dkrho;       /* device scratch, ghosted                  */
   double *q2[4];              /* device surface scratch                   */
   double *red_partial;        /* reduction partials (RED_MAX_BLOCKS*8)    */
-  double *red_dev;            /* device result slots (64)                 */
+  double *red_dev;            /* device result slots (PFHIP_RED_SLOTS)    */
   double *h_red;              /* pinned host result slots (64)            */
 };
 
 #define PFHIP_RED_MAX_BLOCKS 1024
 #define PFHIP_MAX_MULTI 64
+/* device result slots (red_dev) of the fused GMRES steps */
+#define PFHIP_SLOT_H2 31          /* CGS2 second-pass coefficients 31..61 */
+#define PFHIP_SLOT_N2 62          /* CGS2 ||w||^2                         */
+#define PFHIP_SLOT_DQ 64          /* DQ sums of the J.v quotient (3)      */
+#define PFHIP_RED_SLOTS 72        /* size of red_dev                      */
 
 void pfhip_set_error(const char *fmt, ...);
 void pfhip_prof_mark(int kernel_id, int end, void *stream);
@@ -62,6 +67,16 @@ int pfhip_vec_axpy_dot(pfhip_problem *pb, double hprev, const double *vprev,
 int pfhip_residual_jv(pfhip_problem *pb, const double *u, const double *v, double sigma,
                       const double *fval, double sinv, double *z, const double *p_old,
                       const double *s_old, double dt, void *stream);
+
+/* the same with sigma and 1/sigma formed on the device from the DQ sums
+ * (u.v, v.v, |v|_1) with sqrt(relfunc) sqrf: no host synchronisation */
+int pfhip_residual_jv_dq(pfhip_problem *pb, const double *u, const double *v, double sqrf,
+                         const double *fval, double *z, const double *p_old,
+                         const double *s_old, double dt, void *stream);
+/* CGS2 of w against V[0..k-1] (k <= PFHIP_SLOT_H2) with one host
+ * synchronisation: h1, h2 and ||w||^2 after the second pass */
+int pfhip_vec_cgs2(pfhip_problem *pb, const double *const *V, int k, double *w, double *h1,
+                   double *h2, double *norm2, void *stream);
 
 /* RichardsJacobianEval into double (fp32 = 0) or float (fp32 = 1)
  * coefficient arrays J[7] (C, W, E, S, N, L, U; symm: C, E, N, U) */
@@ -122,6 +137,13 @@ typedef struct pfhip_ops {
   int (*bc_pressure)(pfhip_problem *, double, void *);
   int (*saturation)(pfhip_problem *, const double *, double *, int, void *);
   int (*synchronize)(void);
+  /* fused forms without host synchronisation; NULL: the driver composes
+   * them from the operations above */
+  int (*residual_jv_dq)(pfhip_problem *, const double *, const double *, double,
+                        const double *, double *, const double *, const double *, double,
+                        void *);
+  int (*cgs2)(pfhip_problem *, const double *const *, int, double *, double *, double *,
+              double *, void *);
 } pfhip_ops;
 
 extern const pfhip_ops pfhip_direct_ops;
