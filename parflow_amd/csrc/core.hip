@@ -2162,7 +2162,10 @@ __device__ __forceinline__ void jface_z(double pa, double pb, double ka, double 
 }
 
 template <int TX, int TY, int ZC, bool PCS, bool SYMM, typename OT>
-__global__ __launch_bounds__(TX * TY) void k_jacobian_pf(JacPFArgs P, JOut<OT> J)
+// four waves per SIMD: the register budget of 128 VGPRs (without the hint
+// the compiler took 131 and three waves; 401 -> 328 us on 16.8M cells)
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4)))
+void k_jacobian_pf(JacPFArgs P, JOut<OT> J)
 {
   constexpr int HALO = 2 * TX + 2 * TY;
   constexpr int W2 = TX + 2;
