@@ -108,6 +108,22 @@ int  pfhip_phase_rel_perm(pfhip_problem *pb, const double *p, double *out,
 int  pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double *F,
                             const double *p_old, const double *s_old,
                             double dt, void *stream);
+/* The face velocities NlFunctionEval writes into x/y/z_velocity
+ * (PL/nl_function_eval.c:846-848 interior, :1230 DirichletBC, :1432
+ * FluxBC, :2023 OverlandKinematicBC; vectors created at
+ * PL/solver_richards.c:1034-1044, written out as velx/vely/velz PFBs at
+ * :1671-1700),
+ * evaluated at pressure p with the BC values of the last
+ * pfhip_bc_pressure call.  Face-centred device arrays without ghosts:
+ *   vx[i + (nx+1)*(j + ny*k)]    (nx+1)*ny*nz
+ *   vy[i + nx*(j + (ny+1)*k)]    nx*(ny+1)*nz
+ *   vz[i + nx*(j + ny*k)]        nx*ny*(nz+1)
+ * p_old and dt enter only an OverlandKinematic top (q_overlnd / dt);
+ * they may be NULL / 0 otherwise.  Overwrites the problem's internal
+ * kr*rho buffer. */
+int  pfhip_face_velocity(pfhip_problem *pb, const double *p,
+                         const double *p_old, double dt, double *vx,
+                         double *vy, double *vz, void *stream);
 /* RichardsJacobianEvalInvoke (PL/parflow_proto.h:1135;
  * PL/richards_jacobian_eval.c:217).  J[7] device arrays; with symm_part
  * only J[0], J[2], J[4], J[6] are written (J[1], J[3], J[5] may be NULL). */

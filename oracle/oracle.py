@@ -202,6 +202,20 @@ def matvec(op, J, symm, x):
     return y
 
 
+def face_velocity(op, p, p_old, dt, t):
+    """vx (nz, ny, nx+1), vy (nz, ny+1, nx), vz (nz+1, ny, nx): the face
+    velocities NlFunctionEval writes (PL/nl_function_eval.c:846-848, 1230,
+    1432, 2023) at pressure p, BC values at time t."""
+    op.set_time(t)
+    nx, ny, nz = op.prob.nx, op.prob.ny, op.prob.nz
+    vx = np.zeros((nz, ny, nx + 1))
+    vy = np.zeros((nz, ny + 1, nx))
+    vz = np.zeros((nz + 1, ny, nx))
+    lib().pfo_face_velocity(op.ref, _ptr(p), _ptr(p_old), C.c_double(dt),
+                            _ptr(vx), _ptr(vy), _ptr(vz))
+    return vx, vy, vz
+
+
 def overland_kin(op, p):
     n = (op.prob.nx + 2) * (op.prob.ny + 2)
     qx = np.zeros(n)

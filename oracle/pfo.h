@@ -109,6 +109,9 @@ void pfo_overland_kin_der(const pfo_problem *pb, const double *p,
 void pfo_nl_function_eval(const pfo_problem *pb, const double *p,
                           const double *p_old, const double *s_old,
                           double dt, double *F);
+void pfo_face_velocity(const pfo_problem *pb, const double *p,
+                       const double *p_old, double dt, double *vx,
+                       double *vy, double *vz);
 void pfo_jacobian_eval(const pfo_problem *pb, const double *p, double dt,
                        int symm_part, double **J);
 void pfo_matvec(const pfo_problem *pb, double **J, int symm, const double *x,

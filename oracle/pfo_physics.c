@@ -379,6 +379,65 @@ static inline int fidx(const pfo_problem *pb, int f, int i, int j, int k)
   return i + j * pb->nx;
 }
 
+/* DirichletBC "u_new" of face fc of boundary cell ip with BC value `value`
+ * (PL/nl_function_eval.c:1018-1232); the ghost kr*rho is kr at the
+ * Dirichlet value inserted at :574-610.  Used for F (:1225) and for the
+ * face velocity u_new / h (:1230). */
+static double dir_u_new(const pfo_problem *pb, int fc, int ip, double value,
+                        double pc, double krc)
+{
+  const double dx = pb->dx, dy = pb->dy, dz = pb->dz;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
+  const double rho = pb->rho, g = pb->gravity, mu = pb->mu;
+  const double z_dir_g = 1.0;
+  double h = (fc < 2) ? ffx : (fc < 4) ? ffy : ffz;
+  double kg = vg_kr(value, rho, g, pb->alpha[ip], pb->nvg[ip]) * rho;
+  double u_new;
+  if (fc == 0)
+  {
+    double diff = value - pc;
+    u_new = upstream(value, pc, kg, krc);
+    u_new = u_new * h * (pb->kx[ip] / mu) * 2.0 * (diff / dx);
+  }
+  else if (fc == 1)
+  {
+    double diff = pc - value;
+    u_new = upstream(pc, value, krc, kg);
+    u_new = u_new * h * (pb->kx[ip] / mu) * 2.0 * (diff / dx);
+  }
+  else if (fc == 2)
+  {
+    double diff = value - pc;
+    u_new = upstream(value, pc, kg, krc);
+    u_new = u_new * h * (pb->ky[ip] / mu) * 2.0 * (diff / dy);
+  }
+  else if (fc == 3)
+  {
+    double diff = pc - value;
+    u_new = upstream(pc, value, krc, kg);
+    u_new = u_new * h * (pb->ky[ip] / mu) * 2.0 * (diff / dy);
+  }
+  else if (fc == 4)
+  {
+    double sep = dz * 1.0;
+    double lc = value / sep - 0.25 * rho * g;
+    double uc = pc / sep + 0.25 * rho * g;
+    double diff = lc - uc;
+    u_new = upstream(lc, uc, kg, krc);
+    u_new = u_new * h * (pb->kz[ip] / mu) * 2.0 * diff;
+  }
+  else
+  {
+    double sep = dz * 1.0;
+    double lc = (pc / sep) - 0.25 * rho * g * z_dir_g;
+    double uc = (value / sep) + 0.25 * rho * g * z_dir_g;
+    double diff = lc - uc;
+    u_new = upstream(lc, uc, krc, kg);
+    u_new = u_new * h * (pb->kz[ip] / mu) * 2.0 * diff;
+  }
+  return u_new;
+}
+
 /* NlFunctionEval: PL/nl_function_eval.c:167-2323, gather form. */
 void pfo_nl_function_eval(const pfo_problem *pb, const double *p,
                           const double *p_old, const double *s_old,
@@ -510,52 +569,7 @@ void pfo_nl_function_eval(const pfo_problem *pb, const double *p,
           }
           else if (type == PFO_BC_DIR_EQUIL)
           {
-            /* DirichletBC :1018-1232; ghost kr from the Dirichlet value
-             * inserted at :574-610 */
-            double kg = vg_kr(value, rho, g, pb->alpha[ip], pb->nvg[ip]) * rho;
-            double u_new = 0.0;
-            if (fc == 0)
-            {
-              double diff = value - p[ip];
-              u_new = upstream(value, p[ip], kg, krho[ip]);
-              u_new = u_new * h * (pb->kx[ip] / mu) * 2.0 * (diff / dx);
-            }
-            else if (fc == 1)
-            {
-              double diff = p[ip] - value;
-              u_new = upstream(p[ip], value, krho[ip], kg);
-              u_new = u_new * h * (pb->kx[ip] / mu) * 2.0 * (diff / dx);
-            }
-            else if (fc == 2)
-            {
-              double diff = value - p[ip];
-              u_new = upstream(value, p[ip], kg, krho[ip]);
-              u_new = u_new * h * (pb->ky[ip] / mu) * 2.0 * (diff / dy);
-            }
-            else if (fc == 3)
-            {
-              double diff = p[ip] - value;
-              u_new = upstream(p[ip], value, krho[ip], kg);
-              u_new = u_new * h * (pb->ky[ip] / mu) * 2.0 * (diff / dy);
-            }
-            else if (fc == 4)
-            {
-              double sep = dz * 1.0;
-              double lc = value / sep - 0.25 * rho * g;
-              double uc = p[ip] / sep + 0.25 * rho * g;
-              double diff = lc - uc;
-              u_new = upstream(lc, uc, kg, krho[ip]);
-              u_new = u_new * h * (pb->kz[ip] / mu) * 2.0 * diff;
-            }
-            else
-            {
-              double sep = dz * 1.0;
-              double lc = (p[ip] / sep) - 0.25 * rho * g * z_dir_g;
-              double uc = (value / sep) + 0.25 * rho * g * z_dir_g;
-              double diff = lc - uc;
-              u_new = upstream(lc, uc, krho[ip], kg);
-              u_new = u_new * h * (pb->kz[ip] / mu) * 2.0 * diff;
-            }
+            double u_new = dir_u_new(pb, fc, ip, value, p[ip], krho[ip]);
             f += dt * dir * u_new;
           }
           else if (type == PFO_BC_OVERLAND && fc == 5)
@@ -576,6 +590,135 @@ void pfo_nl_function_eval(const pfo_problem *pb, const double *p,
         F[ip] = f;
       }
 
+  free(krho);
+  if (qx) free(qx);
+  if (qy) free(qy);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Face velocities                                                         */
+/* ---------------------------------------------------------------------- */
+
+/* The x/y/z face velocities NlFunctionEval writes beside F
+ * (PL/nl_function_eval.c:710-712, :846-848 interior; :1230 DirichletBC,
+ * :1432 FluxBC, :2023 OverlandKinematicBC), stored face-centred without
+ * ghosts, x fastest:
+ *   vx[i + (nx+1)*(j + ny*k)], face i between cells i-1 and i (0..nx)
+ *   vy[i + nx*(j + (ny+1)*k)], face j (0..ny)
+ *   vz[i + nx*(j + ny*k)],     face k (0..nz)
+ * the layout of the reference's velx/vely/velz PFB outputs.  Interior
+ * faces: the face flux u of the residual divided by the face area
+ * (ffx * z_mult * del_y_slope = ffx with z_mult = 1 and zero slopes).
+ * Domain faces: the value the BC patch loop leaves (u_new / h for
+ * Dirichlet, the flux value for FluxConst, (h value + q_overlnd / dt) / h
+ * on a kinematic overland top).  p_old and dt enter only through q_overlnd. */
+void pfo_face_velocity(const pfo_problem *pb, const double *p,
+                       const double *p_old, double dt, double *vx,
+                       double *vy, double *vz)
+{
+  const int nx = pb->nx, ny = pb->ny, nz = pb->nz;
+  const int sy = nx + 2, sz = (nx + 2) * (ny + 2);
+  const long ng = (long)sz * (nz + 2);
+  const double dx = pb->dx, dy = pb->dy, dz = pb->dz;
+  const double vol = dx * dy * dz;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
+  const double rho = pb->rho, g = pb->gravity, mu = pb->mu;
+  const double gc = 0.5 * (g * cos(atan(0.0)) + g * cos(atan(0.0)));
+  const double z_dir_g = 1.0;
+  double *krho = (double*)calloc(ng, sizeof(double));
+  double *qx = NULL, *qy = NULL;
+  if (pb->bc[5].type == PFO_BC_OVERLAND)
+  {
+    qx = (double*)malloc(sizeof(double) * sz);
+    qy = (double*)malloc(sizeof(double) * sz);
+    pfo_overland_kin(pb, p, qx, qy);
+  }
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+      {
+        int ip = IDX(i, j, k);
+        krho[ip] = vg_kr(p[ip], rho, g, pb->alpha[ip], pb->nvg[ip]) * rho;
+      }
+
+  /* value the BC patch loop of face fc leaves for boundary cell (i,j,k) */
+  #define BC_VEL(fc, i, j, k, out)                                          \
+  do {                                                                      \
+    int ip_ = IDX(i, j, k);                                                 \
+    double value_ = pb->bc_val[fc][fidx(pb, fc, i, j, k)];                  \
+    double h_ = ((fc) < 2) ? ffx : ((fc) < 4) ? ffy : ffz;                  \
+    int type_ = pb->bc[fc].type;                                            \
+    if (type_ == PFO_BC_FLUX) (out) = value_;                               \
+    else if (type_ == PFO_BC_DIR_EQUIL)                                     \
+      (out) = dir_u_new(pb, fc, ip_, value_, p[ip_], krho[ip_]) / h_;       \
+    else {                                                                  \
+      int io_ = IO2(i, j);                                                  \
+      double q_ = vol * (pfmaxd(p[ip_], 0.0) - pfmaxd(p_old[ip_], 0.0)) / dz \
+                  + dt * vol * ((qx[io_] - qx[io_ - 1]) / dx                \
+                                + (qy[io_] - qy[io_ - sy]) / dy) / dz;      \
+      double u_ = h_ * value_;                                              \
+      u_ += q_ / dt;                                                        \
+      (out) = u_ / h_;                                                      \
+    }                                                                       \
+  } while (0)
+
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i <= nx; i++)
+      {
+        double v;
+        if (i == 0) BC_VEL(0, 0, j, k, v);
+        else if (i == nx) BC_VEL(1, nx - 1, j, k, v);
+        else
+        {
+          int a = IDX(i - 1, j, k), b = IDX(i, j, k);
+          double diff = p[a] - p[b];
+          double updir = (diff / dx) * gc;
+          double up = upstream(updir, 0.0, krho[a], krho[b]);
+          double u = ffx * hmean(pb->kx[a], pb->kx[b]) * (diff / dx) * gc * up / mu;
+          v = u / (ffx * 1.0 * 1.0);
+        }
+        vx[i + (long)(nx + 1) * (j + (long)ny * k)] = v;
+      }
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j <= ny; j++)
+      for (int i = 0; i < nx; i++)
+      {
+        double v;
+        if (j == 0) BC_VEL(2, i, 0, k, v);
+        else if (j == ny) BC_VEL(3, i, ny - 1, k, v);
+        else
+        {
+          int a = IDX(i, j - 1, k), b = IDX(i, j, k);
+          double diff = p[a] - p[b];
+          double updir = (diff / dy) * gc;
+          double up = upstream(updir, 0.0, krho[a], krho[b]);
+          double u = ffy * hmean(pb->ky[a], pb->ky[b]) * (diff / dy) * gc * up / mu;
+          v = u / (ffy * 1.0 * 1.0);
+        }
+        vy[i + (long)nx * (j + (long)(ny + 1) * k)] = v;
+      }
+  for (int k = 0; k <= nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+      {
+        double v;
+        if (k == 0) BC_VEL(4, i, j, 0, v);
+        else if (k == nz) BC_VEL(5, i, j, nz - 1, v);
+        else
+        {
+          int a = IDX(i, j, k - 1), b = IDX(i, j, k);
+          double sep = dz;
+          double lc = p[a] / sep - 0.5 * rho * g * z_dir_g;
+          double uc = p[b] / sep + 0.5 * rho * g * z_dir_g;
+          double diff = lc - uc;
+          double u = ffz * hmean_dz(pb->kz[a], pb->kz[b], 1.0, 1.0) * diff
+                     * upstream(lc, uc, krho[a], krho[b]) / mu;
+          v = u / (ffz * 1.0 * 1.0);
+        }
+        vz[i + (long)nx * (j + (long)ny * k)] = v;
+      }
+  #undef BC_VEL
   free(krho);
   if (qx) free(qx);
   if (qy) free(qy);

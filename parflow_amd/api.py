@@ -81,6 +81,7 @@ EXPORTS = [
     "pfhip_memcpy_h2d", "pfhip_memcpy_d2h", "pfhip_memset", "pfhip_synchronize",
     "pfhip_problem_create", "pfhip_problem_destroy", "pfhip_bc_pressure",
     "pfhip_saturation", "pfhip_phase_rel_perm", "pfhip_nl_function_eval",
+    "pfhip_face_velocity",
     "pfhip_richards_jacobian_eval", "pfhip_matvec", "pfhip_overland_flow_eval_kin",
     "pfhip_vec_linear_sum", "pfhip_vec_const", "pfhip_vec_dot", "pfhip_vec_max_norm",
     "pfhip_vec_l1_norm", "pfhip_mg_create", "pfhip_mg_destroy", "pfhip_mg_num_levels",
@@ -111,6 +112,7 @@ def lib():
         L.pfhip_saturation.argtypes = [vp, vp, vp, C.c_int, vp]
         L.pfhip_phase_rel_perm.argtypes = [vp, vp, vp, C.c_int, vp]
         L.pfhip_nl_function_eval.argtypes = [vp, vp, vp, vp, vp, C.c_double, vp]
+        L.pfhip_face_velocity.argtypes = [vp, vp, vp, C.c_double, vp, vp, vp, vp]
         L.pfhip_richards_jacobian_eval.argtypes = [vp, vp, C.c_double, C.c_int,
                                                    C.POINTER(vp), vp]
         L.pfhip_matvec.argtypes = [vp, C.POINTER(vp), C.c_int, vp, vp, vp]
@@ -324,6 +326,18 @@ class GPUProblem:
         check(lib().pfhip_nl_function_eval(self.h, dp.ptr, F.ptr, dpo.ptr, dso.ptr,
                                            C.c_double(dt), None), "nl_function_eval")
         return self.get(F)
+
+    def face_velocity(self, p, p_old, dt, t):
+        """(vx, vy, vz) face velocities at pressure p, shaped (nz, ny, nx+1),
+        (nz, ny+1, nx), (nz+1, ny, nx) like the reference's velx/vely/velz."""
+        self.bc_pressure(t)
+        nx, ny, nz = self.prob.nx, self.prob.ny, self.prob.nz
+        dp, dpo = self.vec(p), self.vec(p_old)
+        shp = [(nz, ny, nx + 1), (nz, ny + 1, nx), (nz + 1, ny, nx)]
+        outs = [DeviceArray(int(np.prod(s))) for s in shp]
+        check(lib().pfhip_face_velocity(self.h, dp.ptr, dpo.ptr, C.c_double(dt),
+                                        *[o.ptr for o in outs], None), "face_velocity")
+        return [o.download(s) for o, s in zip(outs, shp)]
 
     def richards_jacobian_eval(self, p, dt, t, symm):
         self.bc_pressure(t)

@@ -1362,6 +1362,47 @@ __device__ __forceinline__ double residual_cell(const ResArgs &A, int i, int j, 
   return fv;
 }
 
+// DirichletBC "u_new" of face fc of a boundary cell (:1018-1232): the flux
+// through the face from the Dirichlet value at half a cell's distance, with
+// the ghost kr*rho taken at that value (inserted at :574-610).  F adds
+// dt dir u_new (:1225); the face velocity is u_new / h (:1230).
+__device__ __forceinline__ double dir_u_new(const ResArgs &A, int fc, int ip, double value,
+                                            double pc, double krc, double kxc, double kyc,
+                                            double kzc)
+{
+  const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
+  const double dx = A.dx, dy = A.dy, dz = A.dz;
+  const double h = (fc < 2) ? dy * dz : (fc < 4) ? dx * dz : dx * dy;
+  double kg = vg_kr(value, A.f.irg, A.f.alpha[ip], A.f.nvg[ip]) * rho;
+  double u_new;
+  if (fc == 0) {
+    double diff = value - pc;
+    u_new = upw(value, pc, kg, krc) * h * (kxc / mu) * 2.0 * (diff / dx);
+  } else if (fc == 1) {
+    double diff = pc - value;
+    u_new = upw(pc, value, krc, kg) * h * (kxc / mu) * 2.0 * (diff / dx);
+  } else if (fc == 2) {
+    double diff = value - pc;
+    u_new = upw(value, pc, kg, krc) * h * (kyc / mu) * 2.0 * (diff / dy);
+  } else if (fc == 3) {
+    double diff = pc - value;
+    u_new = upw(pc, value, krc, kg) * h * (kyc / mu) * 2.0 * (diff / dy);
+  } else if (fc == 4) {
+    double sep = dz * 1.0;
+    double lc = value / sep - 0.25 * rho * grav;
+    double uc = pc / sep + 0.25 * rho * grav;
+    double diff = lc - uc;
+    u_new = upw(lc, uc, kg, krc) * h * (kzc / mu) * 2.0 * diff;
+  } else {
+    double sep = dz * 1.0;
+    double lc = (pc / sep) - 0.25 * rho * grav * 1.0;
+    double uc = (value / sep) + 0.25 * rho * grav * 1.0;
+    double diff = lc - uc;
+    u_new = upw(lc, uc, krc, kg) * h * (kzc / mu) * 2.0 * diff;
+  }
+  return u_new;
+}
+
 // The boundary-face terms of one cell (:975-2260), added to fv face by face
 // in the order x-lower, x-upper, y-lower, y-upper, z-lower, z-upper, after
 // all interior terms -- the order of the reference's patch loops.
@@ -1370,7 +1411,6 @@ __device__ __forceinline__ double residual_bc(const ResArgs &A, int i, int j, in
                                               double kyc, double kzc)
 {
   const Geom &g = A.g;
-  const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
   const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
   const double vol = dx * dy * dz;
   const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
@@ -1392,34 +1432,7 @@ __device__ __forceinline__ double residual_bc(const ResArgs &A, int i, int j, in
         double u_new = h * value;
         fv += dt * dir * u_new;
       } else if (type == PFHIP_BC_DIR_EQUIL) {
-        // ghost kr from the Dirichlet value inserted at :574-610
-        double kg = vg_kr(value, A.f.irg, A.f.alpha[ip], A.f.nvg[ip]) * rho;
-        double u_new;
-        if (fc == 0) {
-          double diff = value - pc;
-          u_new = upw(value, pc, kg, krc) * h * (kxc / mu) * 2.0 * (diff / dx);
-        } else if (fc == 1) {
-          double diff = pc - value;
-          u_new = upw(pc, value, krc, kg) * h * (kxc / mu) * 2.0 * (diff / dx);
-        } else if (fc == 2) {
-          double diff = value - pc;
-          u_new = upw(value, pc, kg, krc) * h * (kyc / mu) * 2.0 * (diff / dy);
-        } else if (fc == 3) {
-          double diff = pc - value;
-          u_new = upw(pc, value, krc, kg) * h * (kyc / mu) * 2.0 * (diff / dy);
-        } else if (fc == 4) {
-          double sep = dz * 1.0;
-          double lc = value / sep - 0.25 * rho * grav;
-          double uc = pc / sep + 0.25 * rho * grav;
-          double diff = lc - uc;
-          u_new = upw(lc, uc, kg, krc) * h * (kzc / mu) * 2.0 * diff;
-        } else {
-          double sep = dz * 1.0;
-          double lc = (pc / sep) - 0.25 * rho * grav * 1.0;
-          double uc = (value / sep) + 0.25 * rho * grav * 1.0;
-          double diff = lc - uc;
-          u_new = upw(lc, uc, krc, kg) * h * (kzc / mu) * 2.0 * diff;
-        }
+        double u_new = dir_u_new(A, fc, ip, value, pc, krc, kxc, kyc, kzc);
         fv += dt * dir * u_new;
       } else if (fc == 5) {
         // OverlandKinematicBC, PL/nl_function_eval.c:1786-2010
@@ -1969,6 +1982,133 @@ extern "C" int pfhip_nl_function_eval(pfhip_problem *pb, const double *p, double
   A.p = p;
   A.F = F;
   launch_residual(A, STREAM(stream));
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// --------------------------------------------------------------------------
+// face velocities -- the vx, vy, vz NlFunctionEval writes beside F
+// (PL/nl_function_eval.c:846-848 interior faces; :1230 DirichletBC, :1432
+// FluxBC, :2023 OverlandKinematicBC on domain faces), face-centred without
+// ghosts, x fastest (the layout of the reference's velx/vely/velz output):
+//   vx[i + (nx+1)(j + ny k)]  face i between cells i-1 and i, 0 <= i <= nx
+//   vy[i + nx (j + (ny+1) k)] face j, 0 <= j <= ny
+//   vz[i + nx (j + ny k)]     face k, 0 <= k <= nz
+// Interior faces: the residual's face flux (same operands and order)
+// divided by the face area ffx * z_mult * del_y_slope (= ffx here).
+// One thread per point (i, j, k) of the (nx+1)(ny+1)(nz+1) lattice writes
+// the x, y and z faces on the low side of cell (i, j, k); kr*rho comes from
+// a closure pass into the problem's krho buffer.
+// Algorithmic traffic per cell: p, kr*rho, Kx, Ky, Kz read (40 B) and three
+// velocities written (24 B), plus the closure pass (p, alpha, n read, kr*rho
+// written: 32 B).
+// --------------------------------------------------------------------------
+struct VelOut {
+  double *__restrict__ vx;
+  double *__restrict__ vy;
+  double *__restrict__ vz;
+};
+
+// velocity the BC patch loop of face fc leaves at boundary cell (i, j, k)
+__device__ __forceinline__ double bc_face_velocity(const ResArgs &A, int fc, int i, int j, int k)
+{
+  const Geom &g = A.g;
+  const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
+  const int ip = (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
+  const int fi = (fc < 2) ? j + k * g.ny : (fc < 4) ? i + k * g.nx : i + j * g.nx;
+  const double value = A.bcv[fc][fi];
+  const double h = (fc < 2) ? dy * dz : (fc < 4) ? dx * dz : dx * dy;
+  const int type = A.bct[fc];
+  if (type == PFHIP_BC_FLUX) return value;
+  const double pc = A.p[ip];
+  if (type == PFHIP_BC_DIR_EQUIL)
+    return dir_u_new(A, fc, ip, value, pc, A.krho[ip], A.kx[ip], A.ky[ip], A.kz[ip]) / h;
+  // OverlandKinematicBC (top face): u_new = h value + q_overlnd / dt
+  const double vol = dx * dy * dz;
+  const int io = (i + 1) + (j + 1) * g.sy;
+  double q_overlnd = vol * (fmax(pc, 0.0) - fmax(A.p_old[ip], 0.0)) / dz +
+                     dt * vol * ((A.qx[io] - A.qx[io - 1]) / dx +
+                                 (A.qy[io] - A.qy[io - g.sy]) / dy) / dz;
+  double u_new = h * value;
+  u_new += q_overlnd / dt;
+  return u_new / h;
+}
+
+__global__ __launch_bounds__(256) void k_face_velocity(ResArgs A, VelOut V)
+{
+  const Geom &g = A.g;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > g.nx || j > g.ny) return;
+  const double *__restrict__ p = A.p;
+  const double *__restrict__ kr = A.krho;
+  const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
+  const double dx = A.dx, dy = A.dy, dz = A.dz;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
+  // (i, j, k) may lie one past the last cell in any direction: ip is then
+  // a ghost index, read only by faces that are interior
+  const int ip = (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
+  if (j < g.ny && k < g.nz) {
+    double v;
+    if (i == 0) v = bc_face_velocity(A, 0, 0, j, k);
+    else if (i == g.nx) v = bc_face_velocity(A, 1, g.nx - 1, j, k);
+    else {
+      const int a = ip - 1;
+      v = flux_xy(p[a], p[ip], A.kx[a], A.kx[ip], kr[a], kr[ip], ffx, dx, grav, mu) /
+          (ffx * 1.0 * 1.0);
+    }
+    V.vx[i + (long)(g.nx + 1) * (j + (long)g.ny * k)] = v;
+  }
+  if (i < g.nx && k < g.nz) {
+    double v;
+    if (j == 0) v = bc_face_velocity(A, 2, i, 0, k);
+    else if (j == g.ny) v = bc_face_velocity(A, 3, i, g.ny - 1, k);
+    else {
+      const int a = ip - g.sy;
+      v = flux_xy(p[a], p[ip], A.ky[a], A.ky[ip], kr[a], kr[ip], ffy, dy, grav, mu) /
+          (ffy * 1.0 * 1.0);
+    }
+    V.vy[i + (long)g.nx * (j + (long)(g.ny + 1) * k)] = v;
+  }
+  if (i < g.nx && j < g.ny) {
+    double v;
+    if (k == 0) v = bc_face_velocity(A, 4, i, j, 0);
+    else if (k == g.nz) v = bc_face_velocity(A, 5, i, j, g.nz - 1);
+    else {
+      const int a = ip - g.sz;
+      v = flux_z(p[a], p[ip], A.kz[a], A.kz[ip], kr[a], kr[ip], ffz, dz, rho, grav, mu) /
+          (ffz * 1.0 * 1.0);
+    }
+    V.vz[i + (long)g.nx * (j + (long)g.ny * k)] = v;
+  }
+}
+
+extern "C" int pfhip_face_velocity(pfhip_problem *pb, const double *p, const double *p_old,
+                                   double dt, double *vx, double *vy, double *vz, void *stream)
+{
+  if (!pb || !p || !vx || !vy || !vz) {
+    pfhip_set_error("face_velocity: NULL argument");
+    return PFHIP_ERR_ARG;
+  }
+  if (pb->overland && (!p_old || !(dt > 0.0))) {
+    pfhip_set_error("face_velocity: an overland top needs p_old and dt > 0");
+    return PFHIP_ERR_ARG;
+  }
+  Geom g = geom_of(pb);
+  hipStream_t st = STREAM(stream);
+  if (pb->overland) {
+    hipLaunchKernelGGL(k_overland_q, grid2(g), BLOCK3, 0, st, g, surf_of(pb), p, pb->q2[0],
+                       pb->q2[1]);
+    LAUNCHCHK();
+  }
+  int rc = launch_closure(pb, 4, p, pb->krho, nullptr, stream);
+  if (rc) return rc;
+  ResArgs A = res_args(pb, p_old, nullptr, dt);
+  A.p = p;
+  VelOut V;
+  V.vx = vx; V.vy = vy; V.vz = vz;
+  dim3 grid((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
+  hipLaunchKernelGGL(k_face_velocity, grid, dim3(64, 4, 1), 0, st, A, V);
   LAUNCHCHK();
   return PFHIP_OK;
 }

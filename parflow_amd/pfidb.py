@@ -32,7 +32,8 @@ grids, variable dz, other BC or curve types, a full-Jacobian
 preconditioner matrix) raise ``ValueError``.
 
 ``advance_deck`` runs a deck on the device and writes ParFlow-named PFB
-outputs (``<run>.out.press.NNNNN.pfb``, ``<run>.out.satur.NNNNN.pfb``).
+outputs (``<run>.out.press.NNNNN.pfb``, ``<run>.out.satur.NNNNN.pfb`` and,
+with ``Solver.PrintVelocities``, ``<run>.out.velx/vely/velz.NNNNN.pfb``).
 """
 from __future__ import annotations
 
@@ -420,9 +421,15 @@ def problem_from_pfidb(db, base_dir=".", name=None):
 def advance_deck(path, out_dir, run_name=None, nsteps=None, precision=64, krylov=0):
     """Runs a .pfidb deck on the device (plain form by default) and writes
     <run>.out.press.NNNNN.pfb / .satur.NNNNN.pfb after every step, step 0
-    being the initial state.  Returns the per-step iteration log."""
+    being the initial state, and with ``Solver.PrintVelocities`` True also
+    <run>.out.velx/vely/velz.NNNNN.pfb: the face velocities of the step's
+    last residual evaluation, i.e. at the step's pressure
+    (pfsimulator/parflow_lib/solver_richards.c:1669-1700; zero at step 0,
+    :1034-1044).  Returns the per-step iteration log."""
     from . import api
-    prob, opts, p0, dt, n = problem_from_pfidb(path)
+    db = read_pfidb(path)
+    prob, opts, p0, dt, n = problem_from_pfidb(db, os.path.dirname(os.path.abspath(path)))
+    print_vel = _Keys(db).b("Solver.PrintVelocities")
     run_name = run_name or os.path.splitext(os.path.basename(path))[0]
     n = n if nsteps is None else nsteps
     os.makedirs(out_dir, exist_ok=True)
@@ -430,9 +437,22 @@ def advance_deck(path, out_dir, run_name=None, nsteps=None, precision=64, krylov
     sv = api.GPUSolver(gp, opts, precision, krylov)
     ph, sh, log = sv.advance(p0, prob.start_time, dt, n)
     kw = dict(x=prob.x0, y=prob.y0, z=prob.z0, dx=prob.dx, dy=prob.dy, dz=prob.dz)
+    t = prob.start_time
     for s in range(n + 1):
         write_pfb(os.path.join(out_dir, f"{run_name}.out.press.{s:05d}.pfb"),
                   prob.interior(ph[s]), **kw)
         write_pfb(os.path.join(out_dir, f"{run_name}.out.satur.{s:05d}.pfb"),
                   prob.interior(sh[s]), **kw)
+        if not print_vel:
+            continue
+        if s == 0:
+            nx, ny, nz = prob.nx, prob.ny, prob.nz
+            vel = [np.zeros((nz, ny, nx + 1)), np.zeros((nz, ny + 1, nx)),
+                   np.zeros((nz + 1, ny, nx))]
+        else:
+            dts = float(log["dt"][s - 1])
+            t += dts
+            vel = gp.face_velocity(ph[s], ph[s - 1], dts, t)
+        for c, v in zip("xyz", vel):
+            write_pfb(os.path.join(out_dir, f"{run_name}.out.vel{c}.{s:05d}.pfb"), v, **kw)
     return log

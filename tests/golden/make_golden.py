@@ -9,6 +9,9 @@ Fixtures written:
   default_richards.npz     -- test/python/default_richards.py (18x15x8,
                               DirEquilRefPatch + FluxConst, 50 steps):
                               pressure and saturation at selected steps
+  default_richards_vel.npz -- the same run's face velocities (velx/vely/velz,
+                              (nx+1)ny nz, nx(ny+1)nz, nx ny(nz+1)) at steps 1,
+                              10, 50
   tiltedv_overlandkin.npz  -- test/python/overland_tiltedV_KWE.py run
                               "TiltedV_OverlandKin" (5x5x1, OverlandKinematic
                               rain/recession cycle): all 21 dumps (one every 2 steps)
@@ -35,6 +38,16 @@ def main(ref):
     np.savez_compressed(os.path.join(HERE, "default_richards.npz"),
                         steps=np.array(steps), press=np.array(press),
                         satur=np.array(satur))
+    # face velocities at steps whose pressure default_richards.npz holds
+    vsteps = [1, 10, 50]
+    vel = {c: [] for c in "xyz"}
+    for s in vsteps:
+        for c in "xyz":
+            _, v = read_pfb(os.path.join(co, f"default_richards.out.vel{c}.{s:05d}.pfb"))
+            vel[c].append(v)
+    np.savez_compressed(os.path.join(HERE, "default_richards_vel.npz"),
+                        steps=np.array(vsteps), velx=np.array(vel["x"]),
+                        vely=np.array(vel["y"]), velz=np.array(vel["z"]))
     d = co
     press, satur = [], []
     for s in range(21):

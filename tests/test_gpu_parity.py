@@ -71,6 +71,47 @@ def test_residual(gpu, overland, dirs):
         assert _relmax(_I(prob, Fg), _I(prob, Fo), scale) < 1e-12
 
 
+@pytest.mark.parametrize("shape", [(11, 9, 6), (70, 13, 5)])
+@pytest.mark.parametrize("overland,dirs", [(False, False), (False, True), (True, True)])
+def test_face_velocity(gpu, overland, dirs, shape):
+    """pfhip_face_velocity against the oracle; the second shape spans two
+    workgroups in x and four in y."""
+    api = _api()
+    prob = cases.hetero(*shape, seed=3, overland=overland, dir_sides=dirs)
+    print(f"face_velocity overland={overland} dir={dirs} shape={shape}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p_old = cases.random_pressure(prob, seed=4)
+    p = p_old + np.random.default_rng(5).normal(0, 0.01, p_old.shape)
+    for t in (0.1, 0.35):
+        got = gp.face_velocity(p, p_old, 0.1, t)
+        ref = O.face_velocity(op, p, p_old, 0.1, t)
+        scale = max(np.max(np.abs(r)) for r in ref)
+        for a, b in zip(got, ref):
+            assert a.shape == b.shape
+            assert np.max(np.abs(a - b)) < 1e-12 * scale
+
+
+def test_face_velocity_golden_default_richards(gpu):
+    """The device velocities from the reference's step-n pressure against
+    its velx/vely/velz outputs of step n (which the oracle reproduces bit for
+    bit); FMA contraction on the device leaves round-off differences."""
+    api = _api()
+    prob, opts, p0, dt, n = ref_decks.default_richards()
+    gp = api.GPUProblem(prob)
+    g = np.load(os.path.join(HERE, "golden", "default_richards.npz"))
+    gv = np.load(os.path.join(HERE, "golden", "default_richards_vel.npz"))
+    steps = [int(x) for x in g["steps"]]
+    for i, s in enumerate(gv["steps"]):
+        p = prob.zeros()
+        prob.interior(p)[...] = g["press"][steps.index(int(s))]
+        got = gp.face_velocity(p, p, dt, s * dt)
+        ref = [gv["velx"][i], gv["vely"][i], gv["velz"][i]]
+        scale = max(np.max(np.abs(r)) for r in ref)
+        for a, b in zip(got, ref):
+            assert np.max(np.abs(a - b)) < 1e-13 * scale, s
+
+
 @pytest.mark.parametrize("symm", [0, 1])
 @pytest.mark.parametrize("overland", [False, True])
 def test_jacobian_and_matvec(gpu, symm, overland):
@@ -379,6 +420,21 @@ def test_pfidb_deck_to_pfb_outputs(gpu, tmp_path):
         _, sat = read_pfb(str(tmp_path / "out" / f"default_richards.out.satur.{s:05d}.pfb"))
         assert np.all((sat > 0.2) & (sat <= 0.99))
     assert log["newton"][0] == 3
+    # Solver.PrintVelocities: zero at step 0, then the reference's own
+    # velocities to the level the pressure agrees at
+    gv = np.load(os.path.join(HERE, "golden", "default_richards_vel.npz"))
+    for c in "xyz":
+        _, v = read_pfb(str(tmp_path / "out" / f"default_richards.out.vel{c}.00000.pfb"))
+        assert np.all(v == 0.0)
+    for i, s in enumerate(gv["steps"]):
+        if s > 10:
+            break
+        ref = [gv["velx"][i], gv["vely"][i], gv["velz"][i]]
+        scale = max(np.max(np.abs(r)) for r in ref)
+        for c, r in zip("xyz", ref):
+            hdr, v = read_pfb(str(tmp_path / "out" / f"default_richards.out.vel{c}.{s:05d}.pfb"))
+            assert v.shape == r.shape
+            assert np.max(np.abs(v - r)) < 1e-6 * scale, (s, c)
 
 
 def test_mixed_precision_close_to_plain(gpu):

@@ -1,6 +1,7 @@
 """CPU tests: the oracle against the reference's own golden outputs and
 against what the mathematics fixes (closed forms, finite differences,
 invariants, brute force on tiny systems)."""
+import dataclasses
 import os
 
 import numpy as np
@@ -137,6 +138,68 @@ def test_residual_mass_balance():
     u = up * prob.dx * prob.dy * prob.g_kz[1, 1:-1, 1:-1] * 2.0 * diff
     bottom = -dt * u.sum()
     assert abs(F_nobot - bottom) < 1e-10 * max(1.0, abs(bottom))
+
+
+def test_default_richards_face_velocity_golden():
+    """Face velocities computed from the reference's own step-n pressure
+    reproduce its velx/vely/velz outputs of step n bit for bit
+    (test/python/default_richards.py: DirEquilRefPatch x faces, FluxConst 0
+    elsewhere).  KINSOL's last residual evaluation of a step is at the
+    accepted iterate, so the velocities written after the step are a
+    function of the step's output pressure (and BC values) alone."""
+    prob, opts, p0, dt, n = ref_decks.default_richards()
+    op = O.OracleProblem(prob)
+    g = np.load(os.path.join(HERE, "golden", "default_richards.npz"))
+    gv = np.load(os.path.join(HERE, "golden", "default_richards_vel.npz"))
+    steps = [int(x) for x in g["steps"]]
+    for i, s in enumerate(gv["steps"]):
+        p = prob.zeros()
+        prob.interior(p)[...] = g["press"][steps.index(int(s))]
+        vx, vy, vz = O.face_velocity(op, p, p, dt, s * dt)
+        assert np.array_equal(vx, gv["velx"][i]), s
+        assert np.array_equal(vy, gv["vely"][i]), s
+        assert np.array_equal(vz, gv["velz"][i]), s
+
+
+@pytest.mark.parametrize("overland,dirs", [(False, False), (False, True), (True, True)])
+def test_face_velocity_divergence_is_residual_flux(overland, dirs):
+    """F = accumulation + dt * (net outflow through the cell's six faces),
+    an outflow being velocity x face area: velocities and residual see the
+    same face fluxes, on interior faces and on Dirichlet, flux and kinematic
+    overland domain faces alike (PL/nl_function_eval.c:846-853, :1225-1230,
+    :1428-1432, :2015-2023).  No reference output holds an overland-top
+    velocity; this identity pins it to F, which the overland goldens pin."""
+    # dx != dy so that a face area taken from the wrong direction shows
+    prob = dataclasses.replace(cases.hetero(7, 6, 5, seed=31, overland=overland,
+                                            dir_sides=dirs), dx=2.0, dy=1.5)
+    op = O.OracleProblem(prob)
+    p_old = cases.random_pressure(prob, seed=32)
+    s_old = O.saturation(op, p_old)
+    p = p_old + np.random.default_rng(33).normal(0, 0.01, p_old.shape)
+    dt, t = 0.1, 0.35
+    F = O.nl_function_eval(op, p, p_old, s_old, dt, t)
+    vx, vy, vz = O.face_velocity(op, p, p_old, dt, t)
+    assert vx.shape == (prob.nz, prob.ny, prob.nx + 1)
+    assert vy.shape == (prob.nz, prob.ny + 1, prob.nx)
+    assert vz.shape == (prob.nz + 1, prob.ny, prob.nx)
+    I = prob.interior
+    s = O.saturation(op, p)
+    rho = prob.rho
+    vol = prob.dx * prob.dy * prob.dz
+    acc = ((I(s) * rho - I(s_old) * rho) * I(prob.g_poro) * vol
+           + I(prob.g_ss) * vol * (I(p) * I(s) * rho - I(p_old) * I(s_old) * rho))
+    out = (prob.dy * prob.dz * (vx[:, :, 1:] - vx[:, :, :-1])
+           + prob.dx * prob.dz * (vy[:, 1:, :] - vy[:, :-1, :])
+           + prob.dx * prob.dy * (vz[1:] - vz[:-1]))
+    lhs = I(F) - acc
+    scale = max(np.max(np.abs(I(F))), np.max(np.abs(acc)), dt * np.max(np.abs(out)))
+    assert np.max(np.abs(lhs - dt * out)) < 1e-12 * scale
+    # the domain faces carry their BC: FluxConst faces the flux value
+    if not dirs:
+        assert np.all(vx[:, :, 0] == 0.0) and np.all(vx[:, :, -1] == 0.0)
+    assert np.all(vy[:, 0, :] == 0.0)
+    if not overland:
+        assert np.all(vz[-1] == -0.01)
 
 
 def _fd_jv(op, p, p_old, s_old, dt, t, v, eps=1e-7):
