@@ -153,6 +153,34 @@ int  pfhip_vec_l1_norm(pfhip_problem *pb, const double *x, double *result,
                        void *stream);
 
 /* ---- water balance ------------------------------------------------------ */
+/* Water-balance terms of one step, as volumes over the step, into host
+ * out[PFHIP_WB_TERMS]: [0] inflow through the domain faces other than an
+ * OverlandKinematic top (-dt sum dir h v, v the BC face velocity of
+ * PL/nl_function_eval.c:1230/:1432); [1] inflow of the overland top's flux
+ * BC (rain, -dt h value); [2] surface runoff leaving through the domain
+ * edges (dt times the kinematic edge fluxes, PL/overlandflow_eval_Kin.c);
+ * [3] sources, sum src vol dt (the increment EvapTransSum adds,
+ * PL/evaptranssum.c:31).  The budget the reference's water-balance tests
+ * close (test/python/water_balance_x.py:546-620): for a converged step
+ * the change of stored water -- the residual's accumulation terms summed
+ * (phi S rho vol + Ss vol p S rho) plus the surface storage dx dy
+ * max(p_top, 0) -- equals [0] + [1] - [2] + [3] to the Newton tolerance.
+ * pftools' subsurface storage (pfhip_water_balance_storage) drops the
+ * compressible term where p < 0, so its change closes the budget only
+ * where that term is small.  PL/overlandsum.c:33 (OverlandSum) accumulates
+ * only for OverlandFlow patches (type 7), never for OverlandKinematic, so
+ * the runoff term here is what the kinematic path has in its place.
+ * p_old: the step's initial pressure (overland top only).  Overwrites the
+ * problem's internal kr*rho buffer. */
+#define PFHIP_WB_TERMS 4
+int  pfhip_water_balance_fluxes(pfhip_problem *pb, const double *p,
+                                const double *p_old, double dt, double *out,
+                                void *stream);
+/* EvapTransSum (PL/evaptranssum.c:31; called at PL/solver_richards.c:3680):
+ * sum += src * dx*dy*dz * dt on every cell of the ghosted device vector
+ * sum.  No-op for a problem without a source field. */
+int  pfhip_evap_trans_sum(pfhip_problem *pb, double dt, double *sum,
+                          void *stream);
 /* calculate_subsurface_storage / calculate_surface_storage
  * (pftools/python/parflow/tools/hydrology.py:136-205) summed over the
  * domain: subsurface = sum(phi S dz dx dy + [p >= 0] p S Ss dz dx dy),

@@ -216,6 +216,15 @@ def face_velocity(op, p, p_old, dt, t):
     return vx, vy, vz
 
 
+def water_balance_fluxes(op, p, p_old, dt, t):
+    """[boundary inflow, overland-top flux inflow, runoff, sources] of one
+    step, volumes over the step."""
+    op.set_time(t)
+    out = np.zeros(4)
+    lib().pfo_water_balance_fluxes(op.ref, _ptr(p), _ptr(p_old), C.c_double(dt), _ptr(out))
+    return out
+
+
 def overland_kin(op, p):
     n = (op.prob.nx + 2) * (op.prob.ny + 2)
     qx = np.zeros(n)

@@ -151,6 +151,8 @@ double pfo_maxnorm(const pfo_problem *pb, const double *x);
 double pfo_l1norm(const pfo_problem *pb, const double *x);
 
 /* ---- water balance (pftools hydrology.py:136-205 restated) -------------- */
+void pfo_water_balance_fluxes(const pfo_problem *pb, const double *p,
+                              const double *p_old, double dt, double *out);
 void pfo_storage(const pfo_problem *pb, const double *p, const double *s,
                  double *subsurface, double *surface);
 

@@ -724,6 +724,72 @@ void pfo_face_velocity(const pfo_problem *pb, const double *p,
   if (qy) free(qy);
 }
 
+/* Water-balance terms of one step (volumes over the step; inflow positive,
+ * runoff positive leaving): [0] domain faces other than an overland top,
+ * from the face velocities above; [1] the overland top's flux BC;
+ * [2] kinematic runoff through the domain edges, from the surface fluxes
+ * of pfo_overland_kin; [3] sum src vol dt (PL/evaptranssum.c:86-92).
+ * The budget of test/python/water_balance_x.py:546-620. */
+void pfo_water_balance_fluxes(const pfo_problem *pb, const double *p,
+                              const double *p_old, double dt, double *out)
+{
+  const int nx = pb->nx, ny = pb->ny, nz = pb->nz;
+  const int sy = nx + 2, sz = (nx + 2) * (ny + 2);
+  const double dx = pb->dx, dy = pb->dy, dz = pb->dz;
+  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
+  const int overland = (pb->bc[5].type == PFO_BC_OVERLAND);
+  double *vx = (double*)malloc(sizeof(double) * (nx + 1) * ny * nz);
+  double *vy = (double*)malloc(sizeof(double) * nx * (ny + 1) * nz);
+  double *vz = (double*)malloc(sizeof(double) * nx * ny * (nz + 1));
+  pfo_face_velocity(pb, p, p_old, dt, vx, vy, vz);
+  double bin = 0.0, rain = 0.0, run = 0.0, src = 0.0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+    {
+      bin += dt * ffx * vx[0 + (nx + 1) * (j + ny * k)];
+      bin -= dt * ffx * vx[nx + (nx + 1) * (j + ny * k)];
+    }
+  for (int k = 0; k < nz; k++)
+    for (int i = 0; i < nx; i++)
+    {
+      bin += dt * ffy * vy[i + nx * (0 + (ny + 1) * k)];
+      bin -= dt * ffy * vy[i + nx * (ny + (ny + 1) * k)];
+    }
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++)
+    {
+      bin += dt * ffz * vz[i + nx * j];
+      if (overland)
+        rain -= dt * ffz * pb->bc_val[5][i + j * nx];
+      else
+        bin -= dt * ffz * vz[i + nx * (j + ny * nz)];
+    }
+  if (overland)
+  {
+    double *qx = (double*)malloc(sizeof(double) * sz);
+    double *qy = (double*)malloc(sizeof(double) * sz);
+    pfo_overland_kin(pb, p, qx, qy);
+    for (int j = 0; j < ny; j++)
+      run += dt * dy * (qx[IO2(nx - 1, j)] - qx[IO2(-1, j)]);
+    for (int i = 0; i < nx; i++)
+      run += dt * dx * (qy[IO2(i, ny - 1)] - qy[IO2(i, -1)]);
+    free(qx);
+    free(qy);
+  }
+  if (pb->src)
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++)
+          src += pb->src[IDX(i, j, k)] * (dx * dy * dz * dt);
+  out[0] = bin;
+  out[1] = rain;
+  out[2] = run;
+  out[3] = src;
+  free(vx);
+  free(vy);
+  free(vz);
+}
+
 /* ---------------------------------------------------------------------- */
 /* Jacobian                                                                */
 /* ---------------------------------------------------------------------- */

@@ -81,7 +81,7 @@ EXPORTS = [
     "pfhip_memcpy_h2d", "pfhip_memcpy_d2h", "pfhip_memset", "pfhip_synchronize",
     "pfhip_problem_create", "pfhip_problem_destroy", "pfhip_bc_pressure",
     "pfhip_saturation", "pfhip_phase_rel_perm", "pfhip_nl_function_eval",
-    "pfhip_face_velocity",
+    "pfhip_face_velocity", "pfhip_water_balance_fluxes", "pfhip_evap_trans_sum",
     "pfhip_richards_jacobian_eval", "pfhip_matvec", "pfhip_overland_flow_eval_kin",
     "pfhip_vec_linear_sum", "pfhip_vec_const", "pfhip_vec_dot", "pfhip_vec_max_norm",
     "pfhip_vec_l1_norm", "pfhip_mg_create", "pfhip_mg_destroy", "pfhip_mg_num_levels",
@@ -113,6 +113,9 @@ def lib():
         L.pfhip_phase_rel_perm.argtypes = [vp, vp, vp, C.c_int, vp]
         L.pfhip_nl_function_eval.argtypes = [vp, vp, vp, vp, vp, C.c_double, vp]
         L.pfhip_face_velocity.argtypes = [vp, vp, vp, C.c_double, vp, vp, vp, vp]
+        L.pfhip_water_balance_fluxes.argtypes = [vp, vp, vp, C.c_double,
+                                                 C.POINTER(C.c_double), vp]
+        L.pfhip_evap_trans_sum.argtypes = [vp, C.c_double, vp, vp]
         L.pfhip_richards_jacobian_eval.argtypes = [vp, vp, C.c_double, C.c_int,
                                                    C.POINTER(vp), vp]
         L.pfhip_matvec.argtypes = [vp, C.POINTER(vp), C.c_int, vp, vp, vp]
@@ -390,6 +393,52 @@ class GPUProblem:
         check(lib().pfhip_water_balance_storage(self.h, dp.ptr, ds.ptr, C.byref(a),
                                                 C.byref(b), None), "storage")
         return a.value, b.value
+
+    def water_balance_fluxes(self, p, p_old, dt, t):
+        """[boundary inflow, overland-top flux inflow, runoff, sources] of a
+        step from p_old to p of length dt ending at time t (volumes)."""
+        self.bc_pressure(t)
+        dp = p if isinstance(p, DeviceArray) else self.vec(p)
+        dpo = p_old if isinstance(p_old, DeviceArray) else self.vec(p_old)
+        out = (C.c_double * 4)()
+        check(lib().pfhip_water_balance_fluxes(self.h, dp.ptr, dpo.ptr, C.c_double(dt), out,
+                                               None), "water_balance_fluxes")
+        return np.array(out[:])
+
+    def evap_trans_sum(self, acc, dt):
+        """acc (DeviceArray, ghosted) += src * dx dy dz * dt (EvapTransSum)."""
+        check(lib().pfhip_evap_trans_sum(self.h, C.c_double(dt), acc.ptr, None),
+              "evap_trans_sum")
+
+
+def water_balance_report(gp, p_hist, s_hist, dts, t0):
+    """Per-step water budget of a run, the bookkeeping of the reference's
+    test/python/water_balance_x.py:520-620 on the device: storages
+    (pftools hydrology.py:136-205) and the step's boundary, overland-top,
+    runoff and source volumes.  p_hist/s_hist: host histories (step 0 = the
+    initial state); dts: the step lengths.  Each row holds the step's terms,
+    the total water (subsurface + surface storage), the expected total
+    (previous total + inflows - runoff) and percent_diff =
+    |total - expected| / |expected| * 100, the quantity the reference test
+    bounds by 0.005 %."""
+    rows = []
+    sub, sur = gp.water_balance_storage(p_hist[0], s_hist[0])
+    prev = sub + sur
+    t = t0
+    for n in range(1, len(p_hist)):
+        dt = float(dts[n - 1])
+        t += dt
+        sub, sur = gp.water_balance_storage(p_hist[n], s_hist[n])
+        f = gp.water_balance_fluxes(p_hist[n], p_hist[n - 1], dt, t)
+        total = sub + sur
+        expected = prev + f[0] + f[1] - f[2] + f[3]
+        rows.append(dict(step=n, time=t, subsurface_storage=sub, surface_storage=sur,
+                         boundary_inflow=f[0], overland_flux_inflow=f[1],
+                         surface_runoff=f[2], sources=f[3], total=total,
+                         expected=expected,
+                         percent_diff=abs(total - expected) / abs(expected) * 100.0))
+        prev = total
+    return rows
 
 
 class GPUMultigrid:

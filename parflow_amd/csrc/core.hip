@@ -2113,6 +2113,136 @@ extern "C" int pfhip_face_velocity(pfhip_problem *pb, const double *p, const dou
   return PFHIP_OK;
 }
 
+// --------------------------------------------------------------------------
+// water-balance terms of one step, volumes over the step (positive = water
+// entering the domain, except the runoff, which leaves it):
+//   [0] inflow through the domain faces other than an overland top:
+//       -dt * sum dir h v over those faces, v the face velocity the BC
+//       patch loops leave (PL/nl_function_eval.c:1230, :1432)
+//   [1] inflow of an OverlandKinematic top's flux BC (rain): -dt h value
+//   [2] surface runoff: dt * the kinematic fluxes leaving through the
+//       domain's edges (the edge sum the q_overlnd divergence of
+//       :1999-2004 telescopes to)
+//   [3] sources: sum src vol dt (EvapTransSum's increment,
+//       PL/evaptranssum.c:86-92)
+// For a converged step, F = 0 summed over the cells gives
+//   dM_sub + dM_surf = [0] + [1] - [2] + [3]
+// with dM_sub the change of ParFlow's own stored mass (the residual's
+// accumulation terms) and dM_surf that of the surface storage
+// dx dy max(p_top, 0).  Row-owning blocks and a fixed-order final pass.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_wb_terms(ResArgs A, int overland,
+                                                  double *__restrict__ partial, int nb)
+{
+  const Geom &g = A.g;
+  const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
+  const double vol_time = dx * dy * dz * dt;
+  double bin = 0.0, rain = 0.0, run = 0.0, src = 0.0;
+  const int rows = g.ny * g.nz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int j = r % g.ny, k = r / g.ny;
+    const int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+      const int ip = base + i;
+      if (A.src) src += A.src[ip] * vol_time;
+      const bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
+                       (k == g.nz - 1);
+      if (!bnd) continue;
+      for (int fc = 0; fc < 6; fc++) {
+        const bool on = (fc == 0 && i == 0) || (fc == 1 && i == g.nx - 1) ||
+                        (fc == 2 && j == 0) || (fc == 3 && j == g.ny - 1) ||
+                        (fc == 4 && k == 0) || (fc == 5 && k == g.nz - 1);
+        if (!on) continue;
+        const double dir = (fc & 1) ? 1.0 : -1.0;
+        const double h = (fc < 2) ? dy * dz : (fc < 4) ? dx * dz : dx * dy;
+        if (fc == 5 && overland) {
+          const int fi = i + j * g.nx;
+          rain += -dt * (h * A.bcv[5][fi]);
+          const int io = (i + 1) + (j + 1) * g.sy;
+          if (i == g.nx - 1) run += dt * dy * A.qx[io];
+          if (i == 0) run -= dt * dy * A.qx[io - 1];
+          if (j == g.ny - 1) run += dt * dx * A.qy[io];
+          if (j == 0) run -= dt * dx * A.qy[io - g.sy];
+        } else {
+          bin += -dt * dir * h * bc_face_velocity(A, fc, i, j, k);
+        }
+      }
+    }
+  }
+  bin = block_reduce<false>(bin);
+  rain = block_reduce<false>(rain);
+  run = block_reduce<false>(run);
+  src = block_reduce<false>(src);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = bin;
+    partial[nb + blockIdx.x] = rain;
+    partial[2 * nb + blockIdx.x] = run;
+    partial[3 * nb + blockIdx.x] = src;
+  }
+}
+
+extern "C" int pfhip_water_balance_fluxes(pfhip_problem *pb, const double *p,
+                                          const double *p_old, double dt, double *out,
+                                          void *stream)
+{
+  if (!pb || !p || !out || !(dt > 0.0)) {
+    pfhip_set_error("water_balance_fluxes: NULL argument or dt <= 0");
+    return PFHIP_ERR_ARG;
+  }
+  if (pb->overland && !p_old) {
+    pfhip_set_error("water_balance_fluxes: an overland top needs p_old");
+    return PFHIP_ERR_ARG;
+  }
+  Geom g = geom_of(pb);
+  hipStream_t st = STREAM(stream);
+  if (pb->overland) {
+    hipLaunchKernelGGL(k_overland_q, grid2(g), BLOCK3, 0, st, g, surf_of(pb), p, pb->q2[0],
+                       pb->q2[1]);
+    LAUNCHCHK();
+  }
+  int rc = launch_closure(pb, 4, p, pb->krho, nullptr, stream);
+  if (rc) return rc;
+  ResArgs A = res_args(pb, p_old, nullptr, dt);
+  A.p = p;
+  int nb = red_blocks(g);
+  hipLaunchKernelGGL(k_wb_terms, dim3(nb), dim3(256), 0, st, A, pb->overland ? 1 : 0,
+                     pb->red_partial, nb);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, st, pb->red_partial, nb,
+                     PFHIP_WB_TERMS, pb->red_dev);
+  LAUNCHCHK();
+  rc = fetch_results(pb, PFHIP_WB_TERMS, stream);
+  if (rc) return rc;
+  for (int q = 0; q < PFHIP_WB_TERMS; q++) out[q] = pb->h_red[q];
+  return PFHIP_OK;
+}
+
+// EvapTransSum (PL/evaptranssum.c:31-92): sum += src vol dt on every cell
+__global__ __launch_bounds__(256) void k_evap_trans_sum(Geom g, double vol_time,
+                                                        const double *__restrict__ src,
+                                                        double *__restrict__ sum)
+{
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  int ip = (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
+  sum[ip] += src[ip] * vol_time;
+}
+
+extern "C" int pfhip_evap_trans_sum(pfhip_problem *pb, double dt, double *sum, void *stream)
+{
+  if (!pb || !sum) {
+    pfhip_set_error("evap_trans_sum: NULL argument");
+    return PFHIP_ERR_ARG;
+  }
+  if (!pb->d.src) return PFHIP_OK;   // no source field: nothing to add
+  Geom g = geom_of(pb);
+  const double vol_time = pb->d.dx * pb->d.dy * pb->d.dz * dt;
+  hipLaunchKernelGGL(k_evap_trans_sum, grid3(g), BLOCK3, 0, STREAM(stream), g, vol_time,
+                     pb->d.src, sum);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
 // Line-search trial point of KINLineSearch (KS/kinsol.c:1228-1405): unew =
 // uu + rl pp formed inside the residual launch (written to unew) and F at it.
 int pfhip_residual_step(pfhip_problem *pb, const double *uu, const double *pp, double rl,

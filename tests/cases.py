@@ -49,3 +49,24 @@ def solver(**kw):
                 precond="MGSemi", eta_choice=2)
     base.update(kw)
     return SolverOptions(**base)
+
+
+def stored_mass(prob, p, s):
+    """Water the residual's accumulation terms book (phi S rho + Ss p S rho)
+    vol over the cells, plus dx dy max(p_top, 0) on an overland top: the
+    mass whose change a converged step's face fluxes account for."""
+    I = prob.interior
+    vol = prob.dx * prob.dy * prob.dz
+    m = np.sum(I(prob.g_poro) * I(s) * prob.rho * vol
+               + I(prob.g_ss) * vol * I(p) * I(s) * prob.rho)
+    if prob.overland:
+        m += np.sum(prob.dx * prob.dy * np.maximum(I(p)[-1], 0.0))
+    return m
+
+
+def hetero_src(prob, seed):
+    """A ghosted source field (1/T) for prob, zero on the ghost layer."""
+    src = np.zeros(prob.shape)
+    src[1:-1, 1:-1, 1:-1] = np.random.default_rng(seed).uniform(
+        -0.01, 0.02, (prob.nz, prob.ny, prob.nx))
+    return src

@@ -233,6 +233,82 @@ def test_water_balance_storage(gpu):
     assert gp.water_balance_storage(p, s) == (sub_g, sur_g)
 
 
+@pytest.mark.parametrize("overland,dirs,with_src", [(False, True, True), (True, True, False),
+                                                    (True, False, True)])
+def test_water_balance_fluxes(gpu, overland, dirs, with_src):
+    """pfhip_water_balance_fluxes against the oracle (which takes the
+    boundary terms from the face velocities instead of the BC formulas)."""
+    api = _api()
+    prob = cases.hetero(37, 21, 9, seed=27, overland=overland, dir_sides=dirs)
+    print(f"water_balance_fluxes overland={overland} dir={dirs} src={with_src}", flush=True)
+    src = cases.hetero_src(prob, 28) if with_src else None
+    gp = api.GPUProblem(prob, src=src)
+    op = O.OracleProblem(prob, src=src)
+    p_old = cases.random_pressure(prob, seed=29)
+    p = p_old + np.random.default_rng(30).normal(0, 0.01, p_old.shape)
+    for t in (0.1, 0.35):
+        got = gp.water_balance_fluxes(p, p_old, 0.1, t)
+        ref = O.water_balance_fluxes(op, p, p_old, 0.1, t)
+        scale = np.sum(np.abs(ref)) + 1e-300
+        assert np.max(np.abs(got - ref)) < 1e-12 * scale, (got, ref)
+        # the overland top rains over [0, 0.2) (cases.hetero's cycle)
+        assert (got[1] != 0.0) == (overland and t < 0.2)
+        assert (got[2] != 0.0) == overland
+        assert (got[3] != 0.0) == with_src
+
+
+def test_evap_trans_sum(gpu):
+    """EvapTransSum: the running sum grows by src dx dy dz dt per call."""
+    api = _api()
+    prob = cases.hetero(13, 7, 5, seed=2)
+    src = cases.hetero_src(prob, 3)
+    gp = api.GPUProblem(prob, src=src)
+    acc = gp.vec(np.zeros(prob.shape))
+    vol = prob.dx * prob.dy * prob.dz
+    want = np.zeros(prob.shape)
+    for dt in (0.1, 0.25, 0.05):
+        gp.evap_trans_sum(acc, dt)
+        want[1:-1, 1:-1, 1:-1] += src[1:-1, 1:-1, 1:-1] * (vol * dt)
+    got = gp.get(acc)
+    # the device fuses the multiply-add: agreement to round-off
+    w = want[1:-1, 1:-1, 1:-1]
+    assert np.max(np.abs(got[1:-1, 1:-1, 1:-1] - w)) <= 1e-15 * np.max(np.abs(w))
+    # a problem without a source field adds nothing
+    gp0 = api.GPUProblem(prob)
+    z = gp0.vec(np.zeros(prob.shape))
+    gp0.evap_trans_sum(z, 0.1)
+    assert np.all(gp0.get(z) == 0.0)
+
+
+@pytest.mark.parametrize("ss", [1.0e-4, 0.0])
+def test_water_balance_budget_on_gpu(gpu, ss):
+    """The device solve of the reference's TiltedV_OverlandKin deck (rain,
+    then recession with runoff): every step's stored-water change equals
+    the device budget terms to the Newton tolerance.  With Ss = 0 pftools'
+    storages are the residual's own, so water_balance_report's percent
+    difference stays far inside the reference test's 0.005 % bar."""
+    import dataclasses
+    api = _api()
+    prob, opts, p0, dt, n = ref_decks.tiltedv_kin()
+    prob = dataclasses.replace(prob, ss=ss)
+    n = 8
+    print(f"water balance tiltedv ss={ss} {n} steps", flush=True)
+    gp = api.GPUProblem(prob)
+    sv = api.GPUSolver(gp, opts)
+    ph, sh, log = sv.advance(p0, 0.0, dt, n)
+    rows = api.water_balance_report(gp, ph, sh, log["dt"], 0.0)
+    assert len(rows) == n
+    for s, row in enumerate(rows, start=1):
+        dm = cases.stored_mass(prob, ph[s], sh[s]) - cases.stored_mass(prob, ph[s - 1], sh[s - 1])
+        terms = [row["boundary_inflow"], row["overland_flux_inflow"], row["surface_runoff"],
+                 row["sources"]]
+        budget = terms[0] + terms[1] - terms[2] + terms[3]
+        assert abs(dm - budget) < 1e-8 * max(1.0, np.sum(np.abs(terms))), s
+        if ss == 0.0:
+            assert row["percent_diff"] < 1e-5, row
+    assert rows[0]["overland_flux_inflow"] > 0.0 and rows[-1]["surface_runoff"] > 0.0
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 def test_multigrid(gpu, precision):
     api = _api()

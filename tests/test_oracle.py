@@ -202,6 +202,36 @@ def test_face_velocity_divergence_is_residual_flux(overland, dirs):
         assert np.all(vz[-1] == -0.01)
 
 
+@pytest.mark.parametrize("case", ["tiltedv", "dir_src"])
+def test_water_balance_budget_closes(case):
+    """After each converged step the change of stored water equals the
+    step's boundary inflow + overland-top flux inflow - runoff + sources
+    (the budget of test/python/water_balance_x.py:546-620), to the Newton
+    tolerance: summing F over the cells cancels every interior face flux."""
+    if case == "tiltedv":
+        prob, opts, p0, dt, n = ref_decks.tiltedv_kin()
+        src, n = None, 8          # two rain steps, then recession with runoff
+    else:
+        prob = cases.hetero(6, 5, 4, seed=41, dir_sides=True)
+        src = cases.hetero_src(prob, 42)
+        opts, p0, dt, n = cases.solver(), cases.random_pressure(prob, seed=43), 0.1, 4
+    op = O.OracleProblem(prob, src=src)
+    r, ph, sh, log = O.advance(op, opts, p0, 0.0, dt, n)
+    assert r == 0
+    t = 0.0
+    seen = np.zeros(4)
+    for s in range(1, n + 1):
+        d = log.dt_used[s - 1]
+        t += d
+        f = O.water_balance_fluxes(op, ph[s], ph[s - 1], d, t)
+        dm = cases.stored_mass(prob, ph[s], sh[s]) - cases.stored_mass(prob, ph[s - 1], sh[s - 1])
+        assert abs(dm - (f[0] + f[1] - f[2] + f[3])) < 1e-8 * max(1.0, np.sum(np.abs(f))), s
+        seen += np.abs(f)
+    # every term the case exercises is non-trivial
+    want = [False, True, True, False] if case == "tiltedv" else [True, False, False, True]
+    assert [bool(x > 1e-3) for x in seen] == want
+
+
 def _fd_jv(op, p, p_old, s_old, dt, t, v, eps=1e-7):
     Fp = O.nl_function_eval(op, p + eps * v, p_old, s_old, dt, t)
     Fm = O.nl_function_eval(op, p - eps * v, p_old, s_old, dt, t)
