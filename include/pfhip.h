@@ -248,6 +248,68 @@ int  pfhip_advance_richards(pfhip_solver *s, double *p, double *sat,
                             int max_conv_failures, int *newton, int *gmres,
                             int *fevals, double *dt_used, int *steps_done);
 
+/* ---- time-step control (SelectTimeStep, the AdvanceRichards loop) ------ */
+#define PFHIP_TS_CONSTANT 0
+#define PFHIP_TS_GROWTH   1
+/* TimeStep.* keys (PL/select_time_step.c:200-245). */
+typedef struct {
+  int type;                          /* PFHIP_TS_CONSTANT / PFHIP_TS_GROWTH  */
+  double value;                      /* Constant: TimeStep.Value             */
+  double initial_step, factor;       /* Growth: TimeStep.InitialStep,        */
+  double min_step, max_step;         /*   GrowthFactor, MinStep, MaxStep     */
+} pfhip_time_step;
+
+/* TimingInfo.* and the loop limits of AdvanceRichards. */
+typedef struct {
+  double start_time, stop_time;      /* TimingInfo.StartTime / StopTime      */
+  double base_unit;                  /* TimingInfo.BaseUnit                  */
+  double dump_interval;              /* TimingInfo.DumpInterval: > 0 a time
+                                        interval, < 0 every -n iterations    */
+  int max_iterations;                /* Solver.MaxIter                       */
+  pfhip_time_step ts;
+} pfhip_time_control;
+
+/* What persists between steps (instance_xtra of PL/solver_richards.c):
+ * zero-initialise, then set t = start_time. */
+typedef struct {
+  double t;                          /* current time                         */
+  double dt;                         /* last step size; 0 before the first   */
+  int iteration_number;              /* completed steps                      */
+  int dump_index;                    /* next dump time index (starts at 1)   */
+} pfhip_time_state;
+
+/* One row of the reference's time log (PL/solver_richards.c:4588-4601). */
+typedef struct {
+  double time, dt;
+  char dt_info;                      /* 'p' policy, 'b' BC cycle, 'f' final  */
+  int dumped;                        /* step ends on a dump time             */
+  int newton, gmres, fevals;         /* summed over the step's attempts      */
+  int conv_failures;
+} pfhip_step_log;
+
+/* TimeCycleDataComputeNextTransition (PL/time_cycle_data.c:206-299) over
+ * the cycles of bc[0..nbc): the time from `time` to the next interval
+ * change of any cycle with more than one interval, or -1 when there is
+ * none (or it is closer than TIME_EPSILON).  Host only. */
+double pfhip_next_transition(const pfhip_bc_spec *bc, int nbc, double base_unit,
+                             double start_time, double stop_time, double time);
+/* SelectTimeStep (PL/select_time_step.c:58-156): *dt in/out (the previous
+ * step, 0 before the first), then clipped to the next BC-cycle transition
+ * (*dt_info 'b') or left as chosen ('p').  No wells in this path, so the
+ * well transition never binds.  Host only. */
+int  pfhip_select_time_step(const pfhip_time_control *tc, const pfhip_bc_spec *bc,
+                            int nbc, double time, double *dt, char *dt_info);
+/* The AdvanceRichards time loop (PL/solver_richards.c:2081-2090,
+ * 2970-3240, 3380-3410, 4603-4612): per step SelectTimeStep (or halving
+ * after a failed solve), the dump-interval and stop-time clips, the Newton
+ * solve at the new time.  Runs until stop_time, max_iterations or
+ * max_steps steps; state carries t, dt and the counters across calls.
+ * log (host, max_steps rows, may be NULL) gets one row per step. */
+int  pfhip_advance_richards_ts(pfhip_solver *s, double *p, double *sat,
+                               const pfhip_time_control *tc, pfhip_time_state *state,
+                               int max_steps, int max_conv_failures,
+                               pfhip_step_log *log, int *steps_done);
+
 /* ---- ensembles of independent problems ------------------------------- */
 /* No single counterpart in the reference, which solves one problem per
  * run: each member keeps the AdvanceRichards contract of

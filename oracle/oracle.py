@@ -296,3 +296,169 @@ def storage(op, p, s):
     b = C.c_double()
     lib().pfo_storage(op.ref, _ptr(p), _ptr(s), C.byref(a), C.byref(b))
     return a.value, b.value
+
+
+# ---------------------------------------------------------------------------
+# Time-step control: SelectTimeStep (PL/select_time_step.c:58-156),
+# TimeCycleDataComputeNextTransition (PL/time_cycle_data.c:206-299) and the
+# AdvanceRichards time loop (PL/solver_richards.c:2081-2090, 2970-3240,
+# 3380-3410, 4603-4612), restated in Python: scalar logic only, the solves
+# go through pfo_kinsol.
+# ---------------------------------------------------------------------------
+TIME_EPSILON = float(np.finfo(np.float32).eps) * 10      # PL/general.h:257
+TIME_CYCLE_SUBDIVISIONS = 1000                            # PL/time_cycle_data.c:150
+
+
+def _c_div(a, b):
+    """C integer division (truncation toward zero)."""
+    q = abs(a) // abs(b)
+    return q if (a >= 0) == (b > 0) else -q
+
+
+def _c_mod(a, b):
+    return a - b * _c_div(a, b)
+
+
+def _pfround(x):
+    return int(x - 0.5) if x < 0.0 else int(x + 0.5)     # PL/general.h:176
+
+
+def next_transition(cycles, base_unit, start_time, stop_time, time):
+    """cycles: list of (lengths, repeat).  Time to the next interval change
+    of any multi-interval cycle, or -1.0."""
+    deltat = -1.0
+    assigned = False
+    for lengths, repeat in cycles:
+        division = len(lengths)
+        cycle_length = int(sum(lengths))
+        time_defined = float(repeat * cycle_length * base_unit) if repeat > 0 else stop_time
+        if not (time < time_defined and division > 1):
+            continue
+        sub = TIME_CYCLE_SUBDIVISIONS
+        dtime = _c_div(_pfround(time / (base_unit / sub)), sub)
+        dstart = _c_div(_pfround(start_time / (base_unit / sub)), sub)
+        n = _c_div(dtime - dstart, cycle_length)
+        completed = _c_mod(dtime - dstart - 1, cycle_length)
+        k, total = 0, int(lengths[0])
+        while completed >= total:
+            k += 1
+            total += int(lengths[k])
+        if (k + 1) % division == 0:
+            n += 1
+            transition = start_time + float(n * cycle_length) * base_unit
+        else:
+            transition = start_time + float(n * cycle_length + total) * base_unit
+        if transition < time_defined:
+            deltat = min(deltat, transition - time) if assigned else transition - time
+            assigned = True
+    if (time + deltat) - time <= TIME_EPSILON:
+        deltat = -1.0
+    return deltat
+
+
+def select_time_step(tc, cycles, time, dt):
+    """(dt, dt_info) of SelectTimeStep; tc a parflow_amd.problem.TimeControl."""
+    if tc.ts_type == "Constant":
+        dt = tc.value
+    else:
+        if dt == 0.0:
+            dt = tc.initial_step
+        else:
+            dt = dt * tc.growth_factor
+            if dt < tc.min_step:
+                dt = tc.min_step
+            if dt > tc.max_step:
+                dt = tc.max_step
+    bc_dt = next_transition(cycles, tc.base_unit, tc.start_time, tc.stop_time, time)
+    if bc_dt <= 0.0:
+        bc_dt = dt
+    if dt > bc_dt:
+        return bc_dt, "b"
+    return dt, "p"
+
+
+def bc_cycles(prob):
+    return [(tuple(b.lengths), b.repeat) for b in prob.bc]
+
+
+def time_loop(tc, cycles, solve, max_steps=10 ** 9, max_conv_failures=3):
+    """The AdvanceRichards loop.  solve(dt, t, retry) -> (ok, (newton, gmres,
+    fevals)) solves the step ending at t (retry: after a failed attempt, the
+    caller restores the step's initial state first).  Returns the step log:
+    rows (time, dt, dt_info, dumped, newton, gmres, fevals, failures)."""
+    return list(time_loop_iter(tc, cycles, solve, max_steps, max_conv_failures))
+
+
+def advance_ts(op, opts, p0, tc, max_steps=10 ** 9):
+    """time_loop with oracle solves; returns (p_hist, s_hist, rows)."""
+    p = np.ascontiguousarray(p0.copy())
+    s = saturation(op, p)
+    ph, sh = [p.copy()], [s.copy()]
+    state = {}
+
+    def solve(dt, t, retry):
+        nonlocal p, s
+        if retry:
+            p[...] = state["p_old"]
+            s[...] = state["s_old"]
+        else:
+            state["p_old"] = p.copy()
+            state["s_old"] = s.copy()
+        r, st = kinsol(op, opts, p, state["p_old"], state["s_old"], dt, t)
+        return r == 0, (st.nni, st.nli, st.nfe)
+
+    rows = []
+    for row in time_loop_iter(tc, bc_cycles(op.prob), solve, max_steps,
+                              opts.max_convergence_failures):
+        s = saturation(op, p)
+        ph.append(p.copy())
+        sh.append(s.copy())
+        rows.append(row)
+    return np.array(ph), np.array(sh), rows
+
+
+def time_loop_iter(tc, cycles, solve, max_steps=10 ** 9, max_conv_failures=3):
+    """Generator form of time_loop: yields each step's row after its solve."""
+    t, dt = tc.start_time, 0.0
+    iteration, dump_index, n = 0, 1, 0
+    while n < max_steps and iteration < tc.max_iterations and t < tc.stop_time:
+        converged, failures = True, 0
+        counts = [0, 0, 0]
+        info = "p"
+        while True:
+            if converged:
+                dt, info = select_time_step(tc, cycles, t, dt)
+            else:
+                t = t - dt
+                new_dt = 0.5 * dt
+                if (t + new_dt) - t > TIME_EPSILON:
+                    dt = new_dt
+                else:
+                    raise RuntimeError("Time increment is too small; solver has failed")
+            dump = False
+            if tc.dump_interval > 0:
+                print_dt = tc.start_time + dump_index * tc.dump_interval - t
+                if (dt + TIME_EPSILON) > print_dt:
+                    if abs(dt - print_dt) > TIME_EPSILON:
+                        dt = print_dt
+                    info = "p"
+                    dump = True
+            elif tc.dump_interval < 0:
+                dump = iteration % (-int(tc.dump_interval)) == 0
+            if (t + dt) >= tc.stop_time:
+                dt = tc.stop_time - t
+                info = "f"
+            t += dt
+            ok, c = solve(dt, t, not converged)
+            counts = [a + b for a, b in zip(counts, c)]
+            if ok:
+                break
+            converged = False
+            failures += 1
+            if failures >= max_conv_failures:
+                raise RuntimeError(f"Time step failed for time {t:12.4e}")
+        iteration += 1
+        if dump:
+            dump_index += 1
+        n += 1
+        yield (t, dt, info, dump, counts[0], counts[1], counts[2], failures)

@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-from .problem import Problem, SolverOptions
+from .problem import Problem, SolverOptions, TimeControl
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpfhip.so")
@@ -68,6 +68,75 @@ class SolverOptsC(C.Structure):
                 ("mg_precision", C.c_int), ("krylov", C.c_int)]
 
 
+def _fill_bc(cb, b):
+    cb.type = b.type
+    cb.n_intervals = len(b.values)
+    cb.repeat = b.repeat
+    for i, (L_, v) in enumerate(zip(b.lengths, b.values)):
+        cb.lengths[i] = int(L_)
+        cb.values[i] = float(v)
+    cb.z_ref = b.z_ref
+
+
+def bc_specs_c(prob: Problem):
+    """The six faces' BC specs as a pfhip_bc_spec[6] array."""
+    arr = (BCSpecC * 6)()
+    for f, b in enumerate(prob.bc):
+        _fill_bc(arr[f], b)
+    return arr
+
+
+class TimeStepC(C.Structure):
+    _fields_ = [("type", C.c_int), ("value", C.c_double), ("initial_step", C.c_double),
+                ("factor", C.c_double), ("min_step", C.c_double), ("max_step", C.c_double)]
+
+
+class TimeControlC(C.Structure):
+    _fields_ = [("start_time", C.c_double), ("stop_time", C.c_double),
+                ("base_unit", C.c_double), ("dump_interval", C.c_double),
+                ("max_iterations", C.c_int), ("ts", TimeStepC)]
+
+
+class TimeStateC(C.Structure):
+    _fields_ = [("t", C.c_double), ("dt", C.c_double), ("iteration_number", C.c_int),
+                ("dump_index", C.c_int)]
+
+
+class StepLogC(C.Structure):
+    _fields_ = [("time", C.c_double), ("dt", C.c_double), ("dt_info", C.c_char),
+                ("dumped", C.c_int), ("newton", C.c_int), ("gmres", C.c_int),
+                ("fevals", C.c_int), ("conv_failures", C.c_int)]
+
+
+def time_control_c(tc: TimeControl):
+    c = TimeControlC()
+    c.start_time, c.stop_time = tc.start_time, tc.stop_time
+    c.base_unit, c.dump_interval = tc.base_unit, tc.dump_interval
+    c.max_iterations = int(tc.max_iterations)
+    c.ts.type = 0 if tc.ts_type == "Constant" else 1
+    c.ts.value = tc.value
+    c.ts.initial_step, c.ts.factor = tc.initial_step, tc.growth_factor
+    c.ts.min_step, c.ts.max_step = tc.min_step, tc.max_step
+    return c
+
+
+def next_transition(prob: Problem, tc: TimeControl, time):
+    """pfhip_next_transition over prob's face cycles (host only)."""
+    return lib().pfhip_next_transition(bc_specs_c(prob), 6, C.c_double(tc.base_unit),
+                                       C.c_double(tc.start_time),
+                                       C.c_double(tc.stop_time), C.c_double(time))
+
+
+def select_time_step(prob: Problem, tc: TimeControl, time, dt):
+    """(dt, dt_info) of pfhip_select_time_step (host only)."""
+    d = C.c_double(dt)
+    info = C.c_char(b"?")
+    check(lib().pfhip_select_time_step(C.byref(time_control_c(tc)), bc_specs_c(prob), 6,
+                                       C.c_double(time), C.byref(d), C.byref(info)),
+          "select_time_step")
+    return d.value, info.value.decode()
+
+
 class KinStatsC(C.Structure):
     _fields_ = [("ret", C.c_int), ("nni", C.c_int), ("nli", C.c_int),
                 ("nfe", C.c_int), ("npe", C.c_int), ("nps", C.c_int),
@@ -82,6 +151,7 @@ EXPORTS = [
     "pfhip_problem_create", "pfhip_problem_destroy", "pfhip_bc_pressure",
     "pfhip_saturation", "pfhip_phase_rel_perm", "pfhip_nl_function_eval",
     "pfhip_face_velocity", "pfhip_water_balance_fluxes", "pfhip_evap_trans_sum",
+    "pfhip_next_transition", "pfhip_select_time_step", "pfhip_advance_richards_ts",
     "pfhip_richards_jacobian_eval", "pfhip_matvec", "pfhip_overland_flow_eval_kin",
     "pfhip_vec_linear_sum", "pfhip_vec_const", "pfhip_vec_dot", "pfhip_vec_max_norm",
     "pfhip_vec_l1_norm", "pfhip_mg_create", "pfhip_mg_destroy", "pfhip_mg_num_levels",
@@ -116,6 +186,13 @@ def lib():
         L.pfhip_water_balance_fluxes.argtypes = [vp, vp, vp, C.c_double,
                                                  C.POINTER(C.c_double), vp]
         L.pfhip_evap_trans_sum.argtypes = [vp, C.c_double, vp, vp]
+        L.pfhip_next_transition.restype = C.c_double
+        L.pfhip_next_transition.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                            C.c_double]
+        L.pfhip_select_time_step.argtypes = [vp, vp, C.c_int, C.c_double,
+                                             C.POINTER(C.c_double), C.POINTER(C.c_char)]
+        L.pfhip_advance_richards_ts.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, vp,
+                                                C.POINTER(C.c_int)]
         L.pfhip_richards_jacobian_eval.argtypes = [vp, vp, C.c_double, C.c_int,
                                                    C.POINTER(vp), vp]
         L.pfhip_matvec.argtypes = [vp, C.POINTER(vp), C.c_int, vp, vp, vp]
@@ -276,14 +353,7 @@ class GPUProblem:
                 setattr(d, name, darr.value)
         d.ov_eps = prob.ov_eps
         for f, b in enumerate(prob.bc):
-            cb = d.bc[f]
-            cb.type = b.type
-            cb.n_intervals = len(b.values)
-            cb.repeat = b.repeat
-            for i, (L_, v) in enumerate(zip(b.lengths, b.values)):
-                cb.lengths[i] = int(L_)
-                cb.values[i] = float(v)
-            cb.z_ref = b.z_ref
+            _fill_bc(d.bc[f], b)
         d.base_unit = prob.base_unit
         d.start_time = prob.start_time
         self.desc = d
@@ -503,6 +573,42 @@ class GPUSolver:
         if rc not in (0, -4):
             check(rc, "kinsol")
         return rc, self.gp.get(dp), st
+
+    def advance_ts_device(self, dp, ds, tc: TimeControl, state: TimeStateC, max_steps):
+        """pfhip_advance_richards_ts on device arrays: up to max_steps steps
+        of the time loop from state (updated in place); returns the rows."""
+        log = (StepLogC * max(int(max_steps), 1))()
+        done = C.c_int()
+        tcc = time_control_c(tc)
+        rc = lib().pfhip_advance_richards_ts(
+            self.h, dp.ptr, ds.ptr, C.byref(tcc), C.byref(state), int(max_steps),
+            int(self.opts.max_convergence_failures), log, C.byref(done))
+        check(rc, "advance_richards_ts")
+        return [dict(time=r.time, dt=r.dt, dt_info=r.dt_info.decode(), dumped=bool(r.dumped),
+                     newton=r.newton, gmres=r.gmres, fevals=r.fevals,
+                     conv_failures=r.conv_failures) for r in log[:done.value]]
+
+    def run(self, p0, tc: TimeControl, max_steps=None, history=True):
+        """The whole time loop from p0 at tc.start_time.  With history the
+        loop returns to the host after every step: (p_hist, s_hist, rows),
+        step 0 the initial state; otherwise (p, s, rows) at the end."""
+        dp, ds = self.gp.vec(p0), self.gp.vec()
+        state = TimeStateC()
+        state.t, state.dt, state.iteration_number, state.dump_index = tc.start_time, 0.0, 0, 1
+        limit = tc.max_iterations if max_steps is None else int(max_steps)
+        if not history:
+            rows = self.advance_ts_device(dp, ds, tc, state, limit)
+            return self.gp.get(dp), self.gp.get(ds), rows
+        check(lib().pfhip_saturation(self.gp.h, dp.ptr, ds.ptr, 0, None), "saturation")
+        ph, sh, rows = [self.gp.get(dp)], [self.gp.get(ds)], []
+        while len(rows) < limit:
+            r = self.advance_ts_device(dp, ds, tc, state, 1)
+            if not r:
+                break
+            rows += r
+            ph.append(self.gp.get(dp))
+            sh.append(self.gp.get(ds))
+        return np.array(ph), np.array(sh), rows
 
     def advance_device(self, dp, ds, t0, dt, nsteps):
         """Runs nsteps on device arrays dp, ds; returns per-step counts."""

@@ -740,3 +740,180 @@ int pfhip_advance_richards(pfhip_solver *s, double *p, double *sat, double t0,
   (void)bytes;
   return s->ops->synchronize();
 }
+
+/* ------------------------------------------------------------------------
+ * Time-step control.  TIME_EPSILON (PL/general.h:257) and the 1000
+ * subdivisions of a base unit (PL/time_cycle_data.c:150).
+ * ---------------------------------------------------------------------- */
+#include <float.h>
+#define TS_TIME_EPSILON (FLT_EPSILON * 10)
+#define TS_SUBDIVISIONS 1000
+
+/* pfround, PL/general.h:176 */
+static int ts_round(double x) { return (x < 0.0) ? (int)(x - 0.5) : (int)(x + 0.5); }
+
+double pfhip_next_transition(const pfhip_bc_spec *bc, int nbc, double base_unit,
+                             double start_time, double stop_time, double time)
+{
+  double deltat = -1.0;
+  int assigned = 0;
+  for (int c = 0; c < nbc; c++) {
+    const pfhip_bc_spec *b = &bc[c];
+    int division = b->n_intervals;
+    if (division < 1 || division > PFHIP_MAX_INTERVALS) continue;
+    int cycle_length = 0;
+    for (int i = 0; i < division; i++) cycle_length += b->lengths[i];
+    if (cycle_length <= 0) continue;
+    double time_defined = (b->repeat > 0)
+                              ? ((double)(b->repeat * cycle_length * base_unit))
+                              : stop_time;
+    if (!(time < time_defined && division > 1)) continue;
+    /* time in whole base units, rounded at 1/1000 of a unit first */
+    int dtime = ts_round(time / (base_unit / TS_SUBDIVISIONS)) / TS_SUBDIVISIONS;
+    int dstart = ts_round(start_time / (base_unit / TS_SUBDIVISIONS)) / TS_SUBDIVISIONS;
+    int n = (dtime - dstart) / cycle_length;
+    int completed = (dtime - dstart - 1) % cycle_length;
+    int in = 0, total = b->lengths[0];
+    while (completed >= total && in + 1 < division) {
+      in++;
+      total += b->lengths[in];
+    }
+    int next = (in + 1) % division;
+    double transition;
+    if (next == 0) {
+      n++;
+      transition = start_time + ((double)(n * cycle_length)) * base_unit;
+    } else {
+      transition = start_time + ((double)(n * cycle_length + total)) * base_unit;
+    }
+    if (transition < time_defined) {
+      if (assigned) {
+        double d = transition - time;
+        deltat = (deltat < d) ? deltat : d;
+      } else {
+        deltat = transition - time;
+        assigned = 1;
+      }
+    }
+  }
+  /* a transition closer than TIME_EPSILON does not count */
+  double test = time + deltat;
+  double diff = test - time;
+  if (diff <= TS_TIME_EPSILON) deltat = -1.0;
+  return deltat;
+}
+
+int pfhip_select_time_step(const pfhip_time_control *tc, const pfhip_bc_spec *bc, int nbc,
+                           double time, double *dt, char *dt_info)
+{
+  const pfhip_time_step *ts = &tc->ts;
+  if (ts->type == PFHIP_TS_CONSTANT) {
+    *dt = ts->value;
+  } else if (ts->type == PFHIP_TS_GROWTH) {
+    if (*dt == 0.0) {
+      *dt = ts->initial_step;
+    } else {
+      *dt = *dt * ts->factor;
+      if (*dt < ts->min_step) *dt = ts->min_step;
+      if (*dt > ts->max_step) *dt = ts->max_step;
+    }
+  } else {
+    pfhip_set_error("select_time_step: unknown TimeStep type %d", ts->type);
+    return PFHIP_ERR_ARG;
+  }
+  double bc_dt = pfhip_next_transition(bc, nbc, tc->base_unit, tc->start_time, tc->stop_time,
+                                       time);
+  if (bc_dt <= 0.0) bc_dt = *dt;
+  if (*dt > bc_dt) {
+    *dt = bc_dt;
+    *dt_info = 'b';
+  } else {
+    *dt_info = 'p';
+  }
+  return PFHIP_OK;
+}
+
+int pfhip_advance_richards_ts(pfhip_solver *s, double *p, double *sat,
+                              const pfhip_time_control *tc, pfhip_time_state *state,
+                              int max_steps, int max_conv_failures, pfhip_step_log *log,
+                              int *steps_done)
+{
+  pfhip_problem *pb = s->pb;
+  const pfhip_bc_spec *bc = pb->d.bc;
+  if (steps_done) *steps_done = 0;
+  if (tc->dump_interval < 0.0 && (int)(-tc->dump_interval) < 1) {
+    pfhip_set_error("advance_richards_ts: DumpInterval %g is not a whole step count",
+                    tc->dump_interval);
+    return PFHIP_ERR_ARG;
+  }
+  if (state->dump_index < 1) state->dump_index = 1;
+  CHK(s->ops->saturation(pb, p, sat, PFHIP_CALCFCN, NULL));
+  int done = 0;
+  while (done < max_steps && state->iteration_number < tc->max_iterations &&
+         state->t < tc->stop_time) {
+    double t = state->t, dt = state->dt;
+    char info = 'p';
+    int converged = 1, failures = 0, dump = 0, tn = 0, tl = 0, tf = 0;
+    do {
+      if (converged) {
+        CHK(pfhip_select_time_step(tc, bc, 6, t, &dt, &info));
+        CHK(s->ops->vec_scale(pb, 1.0, sat, s->s_old, NULL));
+        CHK(s->ops->vec_scale(pb, 1.0, p, s->p_old, NULL));
+      } else {
+        t = t - dt;
+        double new_dt = 0.5 * dt;
+        double test_time = t + new_dt;
+        if (test_time - t > TS_TIME_EPSILON) {
+          dt = new_dt;
+        } else {
+          pfhip_set_error("Time increment is too small; solver has failed (t = %g)", t);
+          return PFHIP_ERR_SOLVER;
+        }
+        CHK(s->ops->vec_scale(pb, 1.0, s->s_old, sat, NULL));
+        CHK(s->ops->vec_scale(pb, 1.0, s->p_old, p, NULL));
+      }
+      /* dump times (PL/solver_richards.c:3141-3176) */
+      dump = 0;
+      if (tc->dump_interval > 0.0) {
+        double print_dt = tc->start_time + state->dump_index * tc->dump_interval - t;
+        if ((dt + TS_TIME_EPSILON) > print_dt) {
+          if (fabs(dt - print_dt) > TS_TIME_EPSILON) dt = print_dt;
+          info = 'p';
+          dump = 1;
+        }
+      } else if (tc->dump_interval < 0.0) {
+        if ((state->iteration_number % (-(int)tc->dump_interval)) == 0) dump = 1;
+      }
+      /* last step (:3210-3232) */
+      if ((t + dt) >= tc->stop_time) {
+        dt = tc->stop_time - t;
+        info = 'f';
+      }
+      t += dt;
+      pfhip_kin_stats st;
+      int rc = pfhip_kinsol_nonlin_solver(s, p, s->p_old, s->s_old, dt, t, &st);
+      if (rc != PFHIP_OK && rc != PFHIP_ERR_SOLVER) return rc;
+      tn += st.nni; tl += st.nli; tf += st.nfe;
+      if (rc == PFHIP_ERR_SOLVER) { converged = 0; failures++; }
+      else converged = 1;
+    } while (!converged && failures < max_conv_failures);
+    if (!converged) {
+      pfhip_set_error("Time step failed for time %12.4e after %d convergence failures", t,
+                      failures);
+      return PFHIP_ERR_SOLVER;
+    }
+    CHK(s->ops->saturation(pb, p, sat, PFHIP_CALCFCN, NULL));
+    state->t = t;
+    state->dt = dt;
+    state->iteration_number++;
+    if (dump) state->dump_index++;
+    if (log) {
+      pfhip_step_log *r = &log[done];
+      r->time = t; r->dt = dt; r->dt_info = info; r->dumped = dump;
+      r->newton = tn; r->gmres = tl; r->fevals = tf; r->conv_failures = failures;
+    }
+    done++;
+    if (steps_done) *steps_done = done;
+  }
+  return s->ops->synchronize();
+}

@@ -42,7 +42,7 @@ import os
 import numpy as np
 
 from .pfb import read_pfb, write_pfb
-from .problem import (BC_DIR_EQUIL, BC_FLUX, BC_OVERLAND, BCSpec, Problem,
+from .problem import (BC_DIR_EQUIL, BC_FLUX, BC_OVERLAND, BCSpec, Problem, TimeControl,
                       SolverOptions)
 
 
@@ -245,6 +245,30 @@ _ETA = {"EtaConstant": 0, "Walker1": 1, "Walker2": 2}
 _GLOB = {"InexactNewton": 0, "LineSearch": 1}
 
 
+def time_control_from_pfidb(db):
+    """TimeControl of a key -> value dict (or a .pfidb path): TimingInfo
+    (problem.c:87-98), TimeStep Constant or Growth
+    (select_time_step.c:200-245) and Solver.MaxIter."""
+    if isinstance(db, str):
+        db = read_pfidb(db)
+    K = _Keys(db)
+    ts = K.s("TimeStep.Type", "Constant")
+    if ts not in ("Constant", "Growth"):
+        raise ValueError(f"pfidb: TimeStep.Type {ts} is not supported")
+    kw = dict(start_time=K.f("TimingInfo.StartTime", 0.0),
+              stop_time=K.f("TimingInfo.StopTime"),
+              base_unit=K.f("TimingInfo.BaseUnit"),
+              dump_interval=K.f("TimingInfo.DumpInterval", -1.0),
+              max_iterations=K.i("Solver.MaxIter", 1000000), ts_type=ts)
+    if ts == "Constant":
+        kw["value"] = K.f("TimeStep.Value")
+    else:
+        kw.update(initial_step=K.f("TimeStep.InitialStep"),
+                  growth_factor=K.f("TimeStep.GrowthFactor"),
+                  min_step=K.f("TimeStep.MinStep"), max_step=K.f("TimeStep.MaxStep"))
+    return TimeControl(**kw)
+
+
 def problem_from_pfidb(db, base_dir=".", name=None):
     """(Problem, SolverOptions, p0, dt, nsteps) of a key -> value dict (or a
     .pfidb path); file names in the keys are relative to base_dir."""
@@ -323,14 +347,14 @@ def problem_from_pfidb(db, base_dir=".", name=None):
     grav = K.f("Gravity", 1.0)
 
     # time
-    if K.s("TimeStep.Type", "Constant") != "Constant":
-        raise ValueError("pfidb: only a constant TimeStep is supported")
-    base_unit = K.f("TimingInfo.BaseUnit")
-    t0 = K.f("TimingInfo.StartTime", 0.0)
-    t1 = K.f("TimingInfo.StopTime")
-    dt = K.f("TimeStep.Value")
-    # Solver.MaxIter caps the number of time steps (solver_richards.c)
-    nsteps = min(int(round((t1 - t0) / dt)), K.i("Solver.MaxIter", 1000000))
+    tc = time_control_from_pfidb(db)
+    base_unit, t0 = tc.base_unit, tc.start_time
+    if tc.ts_type == "Constant":
+        dt = tc.value
+        # Solver.MaxIter caps the number of time steps (solver_richards.c)
+        nsteps = min(int(round((tc.stop_time - t0) / dt)), tc.max_iterations)
+    else:
+        dt, nsteps = tc.initial_step, tc.max_iterations
 
     # cycles and boundary conditions
     cycles = {}
@@ -419,40 +443,45 @@ def problem_from_pfidb(db, base_dir=".", name=None):
 
 
 def advance_deck(path, out_dir, run_name=None, nsteps=None, precision=64, krylov=0):
-    """Runs a .pfidb deck on the device (plain form by default) and writes
-    <run>.out.press.NNNNN.pfb / .satur.NNNNN.pfb after every step, step 0
-    being the initial state, and with ``Solver.PrintVelocities`` True also
-    <run>.out.velx/vely/velz.NNNNN.pfb: the face velocities of the step's
-    last residual evaluation, i.e. at the step's pressure
-    (pfsimulator/parflow_lib/solver_richards.c:1669-1700; zero at step 0,
-    :1034-1044).  Returns the per-step iteration log."""
+    """Runs a .pfidb deck on the device (plain form by default) through the
+    AdvanceRichards time loop (TimeStep Constant or Growth, BC-cycle, dump
+    and stop-time cuts) and writes ParFlow-named PFB outputs: file 00000 the
+    initial state, then one file number per dump step
+    (pfsimulator/parflow_lib/solver_richards.c:3141-3176, 4603-4612):
+    <run>.out.press.NNNNN.pfb and .satur.NNNNN.pfb and, with
+    ``Solver.PrintVelocities`` True, <run>.out.velx/vely/velz.NNNNN.pfb --
+    the face velocities of the step's last residual evaluation, i.e. at the
+    step's pressure (:1669-1700; zero in file 00000, :1034-1044).
+    ``nsteps`` caps the steps taken.  Returns the per-step log rows."""
     from . import api
     db = read_pfidb(path)
     prob, opts, p0, dt, n = problem_from_pfidb(db, os.path.dirname(os.path.abspath(path)))
+    tc = time_control_from_pfidb(db)
     print_vel = _Keys(db).b("Solver.PrintVelocities")
     run_name = run_name or os.path.splitext(os.path.basename(path))[0]
-    n = n if nsteps is None else nsteps
     os.makedirs(out_dir, exist_ok=True)
     gp = api.GPUProblem(prob)
     sv = api.GPUSolver(gp, opts, precision, krylov)
-    ph, sh, log = sv.advance(p0, prob.start_time, dt, n)
+    ph, sh, rows = sv.run(p0, tc, max_steps=nsteps)
     kw = dict(x=prob.x0, y=prob.y0, z=prob.z0, dx=prob.dx, dy=prob.dy, dz=prob.dz)
-    t = prob.start_time
-    for s in range(n + 1):
-        write_pfb(os.path.join(out_dir, f"{run_name}.out.press.{s:05d}.pfb"),
-                  prob.interior(ph[s]), **kw)
-        write_pfb(os.path.join(out_dir, f"{run_name}.out.satur.{s:05d}.pfb"),
-                  prob.interior(sh[s]), **kw)
-        if not print_vel:
+    nx, ny, nz = prob.nx, prob.ny, prob.nz
+    file_number = 0
+    for s in range(len(rows) + 1):
+        if s > 0 and not rows[s - 1]["dumped"]:
             continue
-        if s == 0:
-            nx, ny, nz = prob.nx, prob.ny, prob.nz
-            vel = [np.zeros((nz, ny, nx + 1)), np.zeros((nz, ny + 1, nx)),
-                   np.zeros((nz + 1, ny, nx))]
-        else:
-            dts = float(log["dt"][s - 1])
-            t += dts
-            vel = gp.face_velocity(ph[s], ph[s - 1], dts, t)
-        for c, v in zip("xyz", vel):
-            write_pfb(os.path.join(out_dir, f"{run_name}.out.vel{c}.{s:05d}.pfb"), v, **kw)
-    return log
+        write_pfb(os.path.join(out_dir, f"{run_name}.out.press.{file_number:05d}.pfb"),
+                  prob.interior(ph[s]), **kw)
+        write_pfb(os.path.join(out_dir, f"{run_name}.out.satur.{file_number:05d}.pfb"),
+                  prob.interior(sh[s]), **kw)
+        if print_vel:
+            if s == 0:
+                vel = [np.zeros((nz, ny, nx + 1)), np.zeros((nz, ny + 1, nx)),
+                       np.zeros((nz + 1, ny, nx))]
+            else:
+                r = rows[s - 1]
+                vel = gp.face_velocity(ph[s], ph[s - 1], r["dt"], r["time"])
+            for c, v in zip("xyz", vel):
+                write_pfb(os.path.join(out_dir, f"{run_name}.out.vel{c}.{file_number:05d}.pfb"),
+                          v, **kw)
+        file_number += 1
+    return rows

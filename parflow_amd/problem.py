@@ -96,6 +96,38 @@ class SolverOptions:
         raise ValueError(f"unknown preconditioner {self.precond!r}")
 
 
+@dataclass
+class TimeControl:
+    """``TimingInfo.*``, ``TimeStep.*`` and ``Solver.MaxIter``: what the
+    AdvanceRichards time loop and SelectTimeStep read
+    (pfsimulator/parflow_lib/select_time_step.c:200-245; the TimingInfo
+    keys read at problem.c:87-98).
+
+    ``ts_type`` "Constant" uses ``value``; "Growth" starts at
+    ``initial_step`` and multiplies the previous step by ``growth_factor``,
+    clipped to [``min_step``, ``max_step``].  ``dump_interval`` > 0 is a time
+    interval the steps are clipped to; < 0 dumps every ``-dump_interval``
+    steps."""
+
+    start_time: float = 0.0
+    stop_time: float = 1.0
+    base_unit: float = 1.0
+    dump_interval: float = -1.0
+    max_iterations: int = 1000000
+    ts_type: str = "Constant"
+    value: float = 1.0
+    initial_step: float = 0.0
+    growth_factor: float = 1.0
+    min_step: float = 0.0
+    max_step: float = 0.0
+
+    def __post_init__(self):
+        if self.ts_type not in ("Constant", "Growth"):
+            raise ValueError(f"TimeControl: TimeStep.Type {self.ts_type!r} is not supported")
+        if self.ts_type == "Growth" and not (self.initial_step > 0 and self.max_step > 0):
+            raise ValueError("TimeControl: Growth needs InitialStep and MaxStep > 0")
+
+
 def _ghost3(a, nx, ny, nz, fill_edge=True):
     out = np.zeros((nz + 2, ny + 2, nx + 2), dtype=np.float64)
     a = np.asarray(a, dtype=np.float64)

@@ -12,6 +12,9 @@ Fixtures written:
   default_richards_vel.npz -- the same run's face velocities (velx/vely/velz,
                               (nx+1)ny nz, nx(ny+1)nz, nx ny(nz+1)) at steps 1,
                               10, 50
+  time_logs.npz            -- the time-step tables of LW_var_dz.out.log and
+                              var.dz.1d.out.log (sequence, time, dt, dt_info,
+                              dump file number)
   tiltedv_overlandkin.npz  -- test/python/overland_tiltedV_KWE.py run
                               "TiltedV_OverlandKin" (5x5x1, OverlandKinematic
                               rain/recession cycle): all 21 dumps (one every 2 steps)
@@ -60,5 +63,41 @@ def main(ref):
                         satur=np.array(satur))
 
 
+def parse_time_log(path, max_rows=None):
+    """The "Sequence # / Time / Delta t / Dumpfile #" table of a reference
+    run's <run>.out.log (written by PL/solver_richards.c:4985-5030): rows
+    (seq, time, dt, dt_info, dumpfile or -1), the initial row included."""
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            parts = line.split()
+            if len(parts) < 5 or not parts[0].isdigit() or len(parts[0]) != 6:
+                continue
+            if parts[3] not in ("i", "p", "b", "f", "w"):
+                continue
+            dump = int(parts[4]) if len(parts) == 6 else -1
+            rows.append((int(parts[0]), float(parts[1]), float(parts[2]), parts[3], dump))
+            if max_rows and len(rows) >= max_rows:
+                break
+    return rows
+
+
+def time_logs(ref):
+    """LW_var_dz (TimingInfo.DumpInterval 20 = two steps, StopTime 200) and
+    var.dz.1d (DumpInterval -100, Solver.MaxIter 2500; first 301 rows)."""
+    co = os.path.join(ref, "test", "correct_output")
+    out = {}
+    for name, cap in (("LW_var_dz", None), ("var.dz.1d", 301)):
+        rows = parse_time_log(os.path.join(co, f"{name}.out.log"), cap)
+        key = name.replace(".", "_")
+        out[key + "_seq"] = np.array([r[0] for r in rows])
+        out[key + "_time"] = np.array([r[1] for r in rows])
+        out[key + "_dt"] = np.array([r[2] for r in rows])
+        out[key + "_info"] = np.array([ord(r[3]) for r in rows], dtype=np.int8)
+        out[key + "_dump"] = np.array([r[4] for r in rows])
+    np.savez_compressed(os.path.join(HERE, "time_logs.npz"), **out)
+
+
 if __name__ == "__main__":
     main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    time_logs(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

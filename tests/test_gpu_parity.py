@@ -432,6 +432,65 @@ def _compare_runs(prob, opts, p0, dt, nsteps, precision=64, krylov=0, ptol=1e-9,
     return worst, logg, logo
 
 
+def test_time_loop_growth_matches_oracle(gpu):
+    """pfhip_advance_richards_ts on the device solve of TiltedV_OverlandKin
+    with growing steps cut at the rain's end, at dump times and at StopTime:
+    the same steps as the oracle's loop and the same pressures."""
+    from parflow_amd.problem import TimeControl
+    api = _api()
+    prob, opts, p0, dt, n = ref_decks.tiltedv_kin()
+    tc = TimeControl(start_time=0.0, stop_time=0.5, base_unit=0.05, dump_interval=0.1,
+                     ts_type="Growth", initial_step=0.01, growth_factor=1.5, min_step=0.005,
+                     max_step=0.05)
+    print("time loop growth tiltedv", flush=True)
+    gp = api.GPUProblem(prob)
+    sv = api.GPUSolver(gp, opts)
+    ph, sh, rows = sv.run(p0, tc)
+    op = O.OracleProblem(prob)
+    pho, sho, rowso = O.advance_ts(op, opts, p0, tc)
+    assert len(rows) == len(rowso) == 18
+    for k, (r, ro) in enumerate(zip(rows, rowso)):
+        assert (r["time"], r["dt"], r["dt_info"], r["dumped"]) == ro[:4], k
+        assert abs(r["newton"] - ro[4]) <= 1, k
+        a, b = _I(prob, ph[k + 1]), _I(prob, pho[k + 1])
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-9, k
+    assert [r["dt_info"] for r in rows][-1] == "f" and rows[-1]["time"] == 0.5
+    # history=False runs the whole loop in one call: same end state
+    p_end, s_end, rows2 = sv.run(p0, tc, history=False)
+    assert [(r["time"], r["dt"]) for r in rows2] == [(r["time"], r["dt"]) for r in rows]
+    assert np.max(np.abs(_I(prob, p_end) - _I(prob, ph[-1]))) < 1e-12
+
+
+def test_time_loop_schedule_matches_reference_log(gpu):
+    """The device time loop with LW_var_dz's timing keys (constant step 10,
+    DumpInterval 20, StopTime 200, the rain/recession cycle on the top flux
+    face) steps, dumps and ends exactly as the reference's own run log
+    (test/correct_output/LW_var_dz.out.log)."""
+    import dataclasses
+    from parflow_amd.problem import BC_FLUX, BCSpec, TimeControl
+    api = _api()
+    prob = cases.hetero(6, 5, 4, seed=51)
+    bc = list(prob.bc)
+    bc[5] = BCSpec(BC_FLUX, (-0.01, 0.0), (10, 20), 14)
+    prob = dataclasses.replace(prob, bc=bc, base_unit=10.0)
+    tc = TimeControl(start_time=0.0, stop_time=200.0, base_unit=10.0, dump_interval=20.0,
+                     ts_type="Constant", value=10.0)
+    print("time loop LW_var_dz schedule", flush=True)
+    gp = api.GPUProblem(prob)
+    sv = api.GPUSolver(gp, cases.solver())
+    _, _, rows = sv.run(prob.hydrostatic(1.0, 0.0), tc, history=False)
+    g = np.load(os.path.join(HERE, "golden", "time_logs.npz"))
+    time, dtl, info, dump = (g["LW_var_dz_time"], g["LW_var_dz_dt"], g["LW_var_dz_info"],
+                             g["LW_var_dz_dump"])
+    assert len(rows) == len(time) - 1
+    fileno = 0
+    for k, r in enumerate(rows, start=1):
+        assert abs(r["time"] - time[k]) <= 5e-7 * time[k] and abs(r["dt"] - dtl[k]) <= 5e-7 * dtl[k]
+        assert r["dt_info"] == chr(info[k]) and r["conv_failures"] == 0
+        fileno += r["dumped"]
+        assert (fileno if r["dumped"] else -1) == dump[k], k
+
+
 def test_kinsol_plain_matches_oracle_default_richards(gpu):
     """Reference deck default_richards (UseJacobian True) on the plain path."""
     prob, opts, p0, dt, n = ref_decks.default_richards()
@@ -484,7 +543,7 @@ def test_pfidb_deck_to_pfb_outputs(gpu, tmp_path):
     deck = tmp_path / "default_richards.pfidb"
     pfidb.write_pfidb(keys, str(deck))
     print("advance_deck default_richards.pfidb 10 steps", flush=True)
-    log = pfidb.advance_deck(str(deck), str(tmp_path / "out"), nsteps=10)
+    rows = pfidb.advance_deck(str(deck), str(tmp_path / "out"), nsteps=10)
     g = np.load(os.path.join(HERE, "golden", "default_richards.npz"))
     for i, s in enumerate(g["steps"]):
         if s > 10:
@@ -495,7 +554,7 @@ def test_pfidb_deck_to_pfb_outputs(gpu, tmp_path):
         assert e < 6e-7, (s, e)
         _, sat = read_pfb(str(tmp_path / "out" / f"default_richards.out.satur.{s:05d}.pfb"))
         assert np.all((sat > 0.2) & (sat <= 0.99))
-    assert log["newton"][0] == 3
+    assert rows[0]["newton"] == 3 and len(rows) == 10 and all(r["dumped"] for r in rows)
     # Solver.PrintVelocities: zero at step 0, then the reference's own
     # velocities to the level the pressure agrees at
     gv = np.load(os.path.join(HERE, "golden", "default_richards_vel.npz"))
@@ -511,6 +570,39 @@ def test_pfidb_deck_to_pfb_outputs(gpu, tmp_path):
             hdr, v = read_pfb(str(tmp_path / "out" / f"default_richards.out.vel{c}.{s:05d}.pfb"))
             assert v.shape == r.shape
             assert np.max(np.abs(v - r)) < 1e-6 * scale, (s, c)
+
+
+def test_pfidb_growth_deck_dump_files(gpu, tmp_path):
+    """A Growth deck with a time DumpInterval: files are numbered by dump
+    (00000 the initial state), each holding the state at its dump time,
+    against the oracle's time loop on the same keys."""
+    from parflow_amd import pfidb
+    from parflow_amd.pfb import read_pfb
+    import test_pfidb
+    keys = test_pfidb.default_richards_keys()
+    keys.update({"TimeStep.Type": "Growth", "TimeStep.InitialStep": 0.0005,
+                 "TimeStep.GrowthFactor": 1.5, "TimeStep.MinStep": 0.0001,
+                 "TimeStep.MaxStep": 0.002, "TimingInfo.DumpInterval": 0.003,
+                 "TimingInfo.StopTime": 0.01})
+    deck = tmp_path / "growth.pfidb"
+    pfidb.write_pfidb(keys, str(deck))
+    print("advance_deck growth deck", flush=True)
+    rows = pfidb.advance_deck(str(deck), str(tmp_path / "out"))
+    prob, opts, p0, dt, n = pfidb.problem_from_pfidb(str(deck))
+    tc = pfidb.time_control_from_pfidb(str(deck))
+    op = O.OracleProblem(prob)
+    pho, sho, rowso = O.advance_ts(op, opts, p0, tc)
+    assert [(r["time"], r["dt"], r["dt_info"], r["dumped"]) for r in rows] == \
+        [ro[:4] for ro in rowso]
+    dumps = [0] + [k + 1 for k, r in enumerate(rows) if r["dumped"]]
+    # dumps at t = .003 .006 .009; the final step to StopTime .01 falls
+    # short of the next dump time and writes nothing (no forced last dump)
+    assert len(dumps) == 4 and rows[-1]["dt_info"] == "f" and not rows[-1]["dumped"]
+    for fileno, s in enumerate(dumps):
+        _, a = read_pfb(str(tmp_path / "out" / f"growth.out.press.{fileno:05d}.pfb"))
+        b = prob.interior(pho[s])
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-9, fileno
+    assert not os.path.exists(tmp_path / "out" / f"growth.out.press.{len(dumps):05d}.pfb")
 
 
 def test_mixed_precision_close_to_plain(gpu):

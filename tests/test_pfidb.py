@@ -241,3 +241,23 @@ def test_unsupported_keys_raise(key, value):
     keys[key] = value
     with pytest.raises(ValueError):
         pfidb.problem_from_pfidb({k: pfidb._str(v) for k, v in keys.items()})
+
+
+def test_time_control_keys():
+    """TimingInfo / TimeStep keys to TimeControl: Constant and Growth
+    (select_time_step.c:200-245), DumpInterval and Solver.MaxIter."""
+    keys = {k: pfidb._str(v) for k, v in default_richards_keys().items()}
+    tc = pfidb.time_control_from_pfidb(keys)
+    assert (tc.ts_type, tc.value, tc.stop_time, tc.dump_interval, tc.max_iterations) == \
+        ("Constant", 0.001, 1.0, -1.0, 50)
+    keys.update({"TimeStep.Type": "Growth", "TimeStep.InitialStep": "0.0005",
+                 "TimeStep.GrowthFactor": "1.2", "TimeStep.MinStep": "0.0001",
+                 "TimeStep.MaxStep": "0.01"})
+    tc = pfidb.time_control_from_pfidb(keys)
+    assert (tc.ts_type, tc.initial_step, tc.growth_factor, tc.min_step, tc.max_step) == \
+        ("Growth", 0.0005, 1.2, 0.0001, 0.01)
+    prob, opts, p0, dt, n = pfidb.problem_from_pfidb(keys)
+    assert dt == 0.0005 and n == 50
+    keys["TimeStep.Type"] = "Adaptive"
+    with pytest.raises(ValueError):
+        pfidb.time_control_from_pfidb(keys)
