@@ -248,6 +248,25 @@ int  pfhip_advance_richards(pfhip_solver *s, double *p, double *sat,
                             int max_conv_failures, int *newton, int *gmres,
                             int *fevals, double *dt_used, int *steps_done);
 
+/* The surface predictor of AdvanceRichards (PL/solver_richards.c:3236-3362;
+ * keys Solver.SurfacePredictor, .PressureValue, .LateralFlows at
+ * :6959-6976), applied to the Newton initial guess p before a step's solve:
+ * a top cell with p < 0 whose free pore volume is smaller than the step's
+ * inflow -- src dx dy dz dt, plus dt times the Darcy inflow through its lower
+ * and lateral faces (face velocities vx, vy, vz in the layout of
+ * pfhip_face_velocity), plus with lateral_flows the surface term
+ * dt ((qx_i - qx_(i-1)) dy + (qy_j - qy_(j-1)) dx) as :3312-3321 add it,
+ * from qx, qy (the ke, kn of pfhip_overland_flow_eval_kin; both NULL for
+ * none) -- gets p = excess / (dx dy phi), or pressure_value when that is
+ * > 0.  sat: S at the step's start.  The problem's source field stands for
+ * ParFlow's evap_trans here. */
+int  pfhip_surface_predictor(pfhip_problem *pb, double *p, const double *sat,
+                             const double *vx, const double *vy,
+                             const double *vz, const double *qx,
+                             const double *qy, double dt,
+                             double pressure_value, int lateral_flows,
+                             void *stream);
+
 /* ---- time-step control (SelectTimeStep, the AdvanceRichards loop) ------ */
 #define PFHIP_TS_CONSTANT 0
 #define PFHIP_TS_GROWTH   1
@@ -267,6 +286,9 @@ typedef struct {
                                         interval, < 0 every -n iterations    */
   int max_iterations;                /* Solver.MaxIter                       */
   pfhip_time_step ts;
+  int surface_predictor;             /* Solver.SurfacePredictor              */
+  double surface_predictor_pressure; /*   .PressureValue (default 1e-5)      */
+  int surface_lateral_flows;         /*   .LateralFlows                      */
 } pfhip_time_control;
 
 /* What persists between steps (instance_xtra of PL/solver_richards.c):

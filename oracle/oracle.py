@@ -115,9 +115,11 @@ class OracleProblem:
             a = np.ascontiguousarray(getattr(prob, "g_" + name))
             self.keep.append(a)
             setattr(c, name, _ptr(a))
+        self.src = None
         if src is not None:
             s = np.ascontiguousarray(src, dtype=np.float64)
             self.keep.append(s)
+            self.src = s
             c.src = _ptr(s)
         c.ov_eps = prob.ov_eps
         for f, b in enumerate(prob.bc):
@@ -389,12 +391,57 @@ def time_loop(tc, cycles, solve, max_steps=10 ** 9, max_conv_failures=3):
     return list(time_loop_iter(tc, cycles, solve, max_steps, max_conv_failures))
 
 
+def surface_predictor(op, p, s, vel, q, dt, pressure_value, lateral):
+    """PL/solver_richards.c:3236-3362 on the top layer (z_mult = 1): returns
+    p with unsaturated top cells whose free pore volume is below the step's
+    inflow set to the ponding pressure of the excess.  vel = (vx, vy, vz) as
+    face_velocity returns them; q = (qx, qy) ghosted surface fluxes (ke, kn)
+    or None; the problem's source field stands for evap_trans."""
+    prob = op.prob
+    nx, ny, nz = prob.nx, prob.ny, prob.nz
+    dx, dy, dz = prob.dx, prob.dy, prob.dz
+    k = nz - 1
+    p = p.copy()
+    top = p[nz, 1:-1, 1:-1]
+    po = prob.g_poro[nz, 1:-1, 1:-1]
+    sat = s[nz, 1:-1, 1:-1]
+    et = op.src[nz, 1:-1, 1:-1] if op.src is not None else np.zeros((ny, nx))
+    vx, vy, vz = vel
+    vol = dx * dy * dz * 1.0 * po * sat
+    flux_in = dx * dy * dz * 1.0 * et * dt
+    vol_max = dx * dy * dz * 1.0 * po
+    flux_darcy = (vz[k] * dx * dy * dt + (-vx[k, :, 1:] + vx[k, :, :-1]) * dy * dz * 1.0 * dt
+                  + (-vy[k, 1:, :] + vy[k, :-1, :]) * dx * dz * 1.0 * dt)
+    if lateral:
+        lat = np.zeros((ny, nx))
+        if q is not None:
+            qx, qy = (np.asarray(a).reshape(ny + 2, nx + 2) for a in q)
+            qx_west = qx[1:-1, :-2].copy()
+            qx_west[:, 0] = 0.0                      # (i > 0) ? qx[i-1] : 0
+            qy_south = qy[:-2, 1:-1].copy()
+            qy_south[0, :] = 0.0
+            lat = dt * ((qx[1:-1, 1:-1] - qx_west) * dy + (qy[1:-1, 1:-1] - qy_south) * dx)
+        total = flux_in + flux_darcy + lat
+    else:
+        total = flux_in + flux_darcy
+    pred = (total - (vol_max - vol)) / (dx * dy * po)
+    mask = (total > (vol_max - vol)) & (top < 0.0)
+    if pressure_value > 0.0:
+        pred = np.full_like(pred, pressure_value)
+    top[mask] = pred[mask]
+    return p
+
+
 def advance_ts(op, opts, p0, tc, max_steps=10 ** 9):
-    """time_loop with oracle solves; returns (p_hist, s_hist, rows)."""
+    """time_loop with oracle solves; returns (p_hist, s_hist, rows).  With
+    tc.surface_predictor the predictor adjusts each attempt's initial guess:
+    zero velocities before the first solve, else those of the step's initial
+    pressure with the BC values of the last solve (the previous step's, or
+    the failed attempt's after a failure)."""
     p = np.ascontiguousarray(p0.copy())
     s = saturation(op, p)
     ph, sh = [p.copy()], [s.copy()]
-    state = {}
+    state = {"solved": 0}
 
     def solve(dt, t, retry):
         nonlocal p, s
@@ -404,7 +451,24 @@ def advance_ts(op, opts, p0, tc, max_steps=10 ** 9):
         else:
             state["p_old"] = p.copy()
             state["s_old"] = s.copy()
+        if tc.surface_predictor:
+            prob = op.prob
+            if state["solved"] == 0 and not retry:
+                vel = (np.zeros((prob.nz, prob.ny, prob.nx + 1)),
+                       np.zeros((prob.nz, prob.ny + 1, prob.nx)),
+                       np.zeros((prob.nz + 1, prob.ny, prob.nx)))
+                q = None
+            else:
+                t_bc = t if retry else state["t_prev"]
+                dt_v = state.get("dt_prev", dt)
+                vel = face_velocity(op, p, p, dt_v, t_bc)
+                q = overland_kin(op, p) if prob.overland else None
+            p[...] = surface_predictor(op, p, s, vel, q, dt, tc.surface_predictor_pressure,
+                                       tc.surface_lateral_flows)
         r, st = kinsol(op, opts, p, state["p_old"], state["s_old"], dt, t)
+        if r == 0:
+            state["solved"] += 1
+            state["t_prev"], state["dt_prev"] = t, dt
         return r == 0, (st.nni, st.nli, st.nfe)
 
     rows = []

@@ -94,7 +94,9 @@ class TimeStepC(C.Structure):
 class TimeControlC(C.Structure):
     _fields_ = [("start_time", C.c_double), ("stop_time", C.c_double),
                 ("base_unit", C.c_double), ("dump_interval", C.c_double),
-                ("max_iterations", C.c_int), ("ts", TimeStepC)]
+                ("max_iterations", C.c_int), ("ts", TimeStepC),
+                ("surface_predictor", C.c_int), ("surface_predictor_pressure", C.c_double),
+                ("surface_lateral_flows", C.c_int)]
 
 
 class TimeStateC(C.Structure):
@@ -117,6 +119,9 @@ def time_control_c(tc: TimeControl):
     c.ts.value = tc.value
     c.ts.initial_step, c.ts.factor = tc.initial_step, tc.growth_factor
     c.ts.min_step, c.ts.max_step = tc.min_step, tc.max_step
+    c.surface_predictor = int(bool(tc.surface_predictor))
+    c.surface_predictor_pressure = tc.surface_predictor_pressure
+    c.surface_lateral_flows = int(bool(tc.surface_lateral_flows))
     return c
 
 
@@ -152,6 +157,7 @@ EXPORTS = [
     "pfhip_saturation", "pfhip_phase_rel_perm", "pfhip_nl_function_eval",
     "pfhip_face_velocity", "pfhip_water_balance_fluxes", "pfhip_evap_trans_sum",
     "pfhip_next_transition", "pfhip_select_time_step", "pfhip_advance_richards_ts",
+    "pfhip_surface_predictor",
     "pfhip_richards_jacobian_eval", "pfhip_matvec", "pfhip_overland_flow_eval_kin",
     "pfhip_vec_linear_sum", "pfhip_vec_const", "pfhip_vec_dot", "pfhip_vec_max_norm",
     "pfhip_vec_l1_norm", "pfhip_mg_create", "pfhip_mg_destroy", "pfhip_mg_num_levels",
@@ -193,6 +199,8 @@ def lib():
                                              C.POINTER(C.c_double), C.POINTER(C.c_char)]
         L.pfhip_advance_richards_ts.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, vp,
                                                 C.POINTER(C.c_int)]
+        L.pfhip_surface_predictor.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
+                                              C.c_double, C.c_int, vp]
         L.pfhip_richards_jacobian_eval.argtypes = [vp, vp, C.c_double, C.c_int,
                                                    C.POINTER(vp), vp]
         L.pfhip_matvec.argtypes = [vp, C.POINTER(vp), C.c_int, vp, vp, vp]
@@ -474,6 +482,21 @@ class GPUProblem:
         check(lib().pfhip_water_balance_fluxes(self.h, dp.ptr, dpo.ptr, C.c_double(dt), out,
                                                None), "water_balance_fluxes")
         return np.array(out[:])
+
+    def surface_predictor(self, p, sat, vel, q, dt, pressure_value, lateral):
+        """p after pfhip_surface_predictor; vel = (vx, vy, vz) face arrays,
+        q = (qx, qy) surface fluxes or None."""
+        dp, ds = self.vec(p), self.vec(sat)
+        dv = [DeviceArray.from_host(np.ascontiguousarray(v, dtype=np.float64).ravel())
+              for v in vel]
+        dq = ([DeviceArray.from_host(np.ascontiguousarray(a, dtype=np.float64).ravel())
+               for a in q] if q is not None else None)
+        check(lib().pfhip_surface_predictor(self.h, dp.ptr, ds.ptr, *[v.ptr for v in dv],
+                                            dq[0].ptr if dq else None,
+                                            dq[1].ptr if dq else None, C.c_double(dt),
+                                            C.c_double(pressure_value), int(bool(lateral)),
+                                            None), "surface_predictor")
+        return self.get(dp)
 
     def evap_trans_sum(self, acc, dt):
         """acc (DeviceArray, ghosted) += src * dx dy dz * dt (EvapTransSum)."""

@@ -2243,6 +2243,97 @@ extern "C" int pfhip_evap_trans_sum(pfhip_problem *pb, double dt, double *sum, v
   return PFHIP_OK;
 }
 
+// --------------------------------------------------------------------------
+// surface predictor (PL/solver_richards.c:3236-3362): before a step's solve,
+// each top cell compares the water it must take this step -- the source
+// (evap_trans) volume, the Darcy inflow through its lower and lateral faces
+// from face velocities, optionally the lateral surface term
+// dt ((qx_i - qx_(i-1)) dy + (qy_j - qy_(j-1)) dx), added with the sign
+// :3312-3321 give it -- with its
+// free pore volume; if the inflow is larger and the cell is unsaturated
+// (p < 0), its pressure becomes the ponding depth the excess implies (or
+// the fixed PressureValue when that is > 0).  One thread per top cell.
+// --------------------------------------------------------------------------
+struct PredArgs {
+  Geom g;
+  double dx, dy, dz, dt, pv;
+  int lateral;
+  double *__restrict__ p;
+  const double *__restrict__ sat;
+  const double *__restrict__ poro;
+  const double *__restrict__ src;
+  const double *__restrict__ vx;
+  const double *__restrict__ vy;
+  const double *__restrict__ vz;
+  const double *__restrict__ qx;
+  const double *__restrict__ qy;
+};
+
+__global__ __launch_bounds__(256) void k_surface_predictor(PredArgs A)
+{
+  const Geom &g = A.g;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const int k = g.nz - 1;
+  const int ip = (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
+  const double dx = A.dx, dy = A.dy, dz = A.dz, dt = A.dt;
+  const double po = A.poro[ip];
+  const double vol = dx * dy * dz * 1.0 * po * A.sat[ip];
+  const double flux_in = dx * dy * dz * 1.0 * (A.src ? A.src[ip] : 0.0) * dt;
+  const double vol_max = dx * dy * dz * 1.0 * po;
+  const long vxi = (i + 1) + (long)(g.nx + 1) * (j + (long)g.ny * k);
+  const long vxi_im1 = vxi - 1;
+  const long vyi = i + (long)g.nx * ((j + 1) + (long)(g.ny + 1) * k);
+  const long vyi_jm1 = vyi - g.nx;
+  const long vzi_km1 = i + (long)g.nx * (j + (long)g.ny * k);
+  const double flux_darcy = A.vz[vzi_km1] * dx * dy * dt +
+                            (-A.vx[vxi] + A.vx[vxi_im1]) * dy * dz * 1.0 * dt +
+                            (-A.vy[vyi] + A.vy[vyi_jm1]) * dx * dz * 1.0 * dt;
+  double flux_total;
+  if (A.lateral) {
+    double lat = 0.0;
+    if (A.qx) {
+      const int io = (i + 1) + (j + 1) * g.sy;
+      const double qx_current = A.qx[io];
+      const double qx_west = (i > 0) ? A.qx[io - 1] : 0.0;
+      const double qy_current = A.qy[io];
+      const double qy_south = (j > 0) ? A.qy[io - g.sy] : 0.0;
+      lat = dt * ((qx_current - qx_west) * dy + (qy_current - qy_south) * dx);
+    }
+    flux_total = flux_in + flux_darcy + lat;
+  } else {
+    flux_total = flux_in + flux_darcy;
+  }
+  double press_pred = (flux_total - (vol_max - vol)) / (dx * dy * po);
+  if (flux_total > (vol_max - vol)) {
+    if (A.p[ip] < 0.0) {
+      if (A.pv > 0.0) press_pred = A.pv;
+      A.p[ip] = press_pred;
+    }
+  }
+}
+
+extern "C" int pfhip_surface_predictor(pfhip_problem *pb, double *p, const double *sat,
+                                       const double *vx, const double *vy, const double *vz,
+                                       const double *qx, const double *qy, double dt,
+                                       double pressure_value, int lateral_flows,
+                                       void *stream)
+{
+  if (!pb || !p || !sat || !vx || !vy || !vz || (!qx) != (!qy)) {
+    pfhip_set_error("surface_predictor: NULL argument");
+    return PFHIP_ERR_ARG;
+  }
+  PredArgs A;
+  A.g = geom_of(pb);
+  A.dx = pb->d.dx; A.dy = pb->d.dy; A.dz = pb->d.dz; A.dt = dt;
+  A.pv = pressure_value; A.lateral = lateral_flows ? 1 : 0;
+  A.p = p; A.sat = sat; A.poro = pb->d.poro; A.src = pb->d.src;
+  A.vx = vx; A.vy = vy; A.vz = vz; A.qx = qx; A.qy = qy;
+  hipLaunchKernelGGL(k_surface_predictor, grid2(A.g), BLOCK3, 0, STREAM(stream), A);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
 // Line-search trial point of KINLineSearch (KS/kinsol.c:1228-1405): unew =
 // uu + rl pp formed inside the residual launch (written to unew) and F at it.
 int pfhip_residual_step(pfhip_problem *pb, const double *uu, const double *pp, double rl,

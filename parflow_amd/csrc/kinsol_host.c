@@ -39,6 +39,7 @@ struct pfhip_solver {
   double *J[7];
   pfhip_mg *mg;
   double *p_old, *s_old;        /* advance_richards state                 */
+  double *vel[3];               /* face velocities for the surface predictor */
   /* host Krylov state */
   double **Hes, *givens, *yg, *hbuf;
   int l_max, max_restarts;
@@ -83,6 +84,7 @@ int pfhip_solver_destroy(pfhip_solver *s)
   if (s->V) for (int i = 0; i <= s->l_max; i++) pfhip_free(s->V[i]);
   if (s->Z) for (int i = 0; i < s->l_max; i++) pfhip_free(s->Z[i]);
   for (int q = 0; q < 7; q++) pfhip_free(s->J[q]);
+  for (int q = 0; q < 3; q++) pfhip_free(s->vel[q]);
   if (s->mg) pfhip_mg_destroy(s->mg);
   if (s->Hes) for (int i = 0; i <= s->l_max; i++) free(s->Hes[i]);
   free(s->Hes); free(s->givens); free(s->yg); free(s->hbuf);
@@ -833,6 +835,34 @@ int pfhip_select_time_step(const pfhip_time_control *tc, const pfhip_bc_spec *bc
   return PFHIP_OK;
 }
 
+/* The surface predictor on the step's initial guess.  The reference uses
+ * the face velocities and surface fluxes its last residual evaluation left:
+ * none (zeros) before the run's first solve, else those of the previous
+ * step's accepted pressure, which are recomputed here from p with the BC
+ * values of that solve.  (After a failed attempt the reference's come from
+ * the failed solve's last iterate; here they are those of the step's
+ * initial pressure.) */
+static int surface_predict(pfhip_solver *s, double *p, const double *sat, int have_vel,
+                           double dt_vel, double dt, const pfhip_time_control *tc)
+{
+  pfhip_problem *pb = s->pb;
+  const int nx = pb->d.nx, ny = pb->d.ny, nz = pb->d.nz;
+  const long n[3] = { (long)(nx + 1) * ny * nz, (long)nx * (ny + 1) * nz,
+                      (long)nx * ny * (nz + 1) };
+  for (int q = 0; q < 3; q++)
+    if (!s->vel[q]) CHK(pfhip_dev_alloc(&s->vel[q], n[q]));
+  const double *qx = NULL, *qy = NULL;
+  if (!have_vel) {
+    for (int q = 0; q < 3; q++) CHK(pfhip_zero(s->vel[q], n[q], NULL));
+  } else {
+    CHK(pfhip_face_velocity(pb, p, p, dt_vel, s->vel[0], s->vel[1], s->vel[2], NULL));
+    if (pb->overland) { qx = pb->q2[0]; qy = pb->q2[1]; }
+  }
+  return pfhip_surface_predictor(pb, p, sat, s->vel[0], s->vel[1], s->vel[2], qx, qy, dt,
+                                 tc->surface_predictor_pressure, tc->surface_lateral_flows,
+                                 NULL);
+}
+
 int pfhip_advance_richards_ts(pfhip_solver *s, double *p, double *sat,
                               const pfhip_time_control *tc, pfhip_time_state *state,
                               int max_steps, int max_conv_failures, pfhip_step_log *log,
@@ -890,6 +920,10 @@ int pfhip_advance_richards_ts(pfhip_solver *s, double *p, double *sat,
         info = 'f';
       }
       t += dt;
+      if (tc->surface_predictor) {
+        int have_vel = state->iteration_number > 0 || failures > 0;
+        CHK(surface_predict(s, p, sat, have_vel, state->dt > 0.0 ? state->dt : dt, dt, tc));
+      }
       pfhip_kin_stats st;
       int rc = pfhip_kinsol_nonlin_solver(s, p, s->p_old, s->s_old, dt, t, &st);
       if (rc != PFHIP_OK && rc != PFHIP_ERR_SOLVER) return rc;

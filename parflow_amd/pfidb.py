@@ -248,7 +248,8 @@ _GLOB = {"InexactNewton": 0, "LineSearch": 1}
 def time_control_from_pfidb(db):
     """TimeControl of a key -> value dict (or a .pfidb path): TimingInfo
     (problem.c:87-98), TimeStep Constant or Growth
-    (select_time_step.c:200-245) and Solver.MaxIter."""
+    (select_time_step.c:200-245), Solver.MaxIter and the
+    Solver.SurfacePredictor keys (solver_richards.c:6959-6976)."""
     if isinstance(db, str):
         db = read_pfidb(db)
     K = _Keys(db)
@@ -260,6 +261,9 @@ def time_control_from_pfidb(db):
               base_unit=K.f("TimingInfo.BaseUnit"),
               dump_interval=K.f("TimingInfo.DumpInterval", -1.0),
               max_iterations=K.i("Solver.MaxIter", 1000000), ts_type=ts)
+    kw.update(surface_predictor=K.b("Solver.SurfacePredictor"),
+              surface_predictor_pressure=K.f("Solver.SurfacePredictor.PressureValue", 0.00001),
+              surface_lateral_flows=K.b("Solver.SurfacePredictor.LateralFlows"))
     if ts == "Constant":
         kw["value"] = K.f("TimeStep.Value")
     else:

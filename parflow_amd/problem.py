@@ -107,7 +107,8 @@ class TimeControl:
     ``initial_step`` and multiplies the previous step by ``growth_factor``,
     clipped to [``min_step``, ``max_step``].  ``dump_interval`` > 0 is a time
     interval the steps are clipped to; < 0 dumps every ``-dump_interval``
-    steps."""
+    steps.  ``surface_predictor`` turns on the top-cell ponding predictor
+    applied to each step's initial guess (solver_richards.c:3236-3362)."""
 
     start_time: float = 0.0
     stop_time: float = 1.0
@@ -120,6 +121,11 @@ class TimeControl:
     growth_factor: float = 1.0
     min_step: float = 0.0
     max_step: float = 0.0
+    # Solver.SurfacePredictor, .PressureValue, .LateralFlows
+    # (solver_richards.c:6959-6976)
+    surface_predictor: bool = False
+    surface_predictor_pressure: float = 0.00001
+    surface_lateral_flows: bool = False
 
     def __post_init__(self):
         if self.ts_type not in ("Constant", "Growth"):

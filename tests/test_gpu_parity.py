@@ -491,6 +491,60 @@ def test_time_loop_schedule_matches_reference_log(gpu):
         assert (fileno if r["dumped"] else -1) == dump[k], k
 
 
+@pytest.mark.parametrize("pv,lateral", [(0.0, False), (0.0, True), (1e-5, True)])
+def test_surface_predictor(gpu, pv, lateral):
+    """pfhip_surface_predictor against the oracle on random top-layer
+    states: about half the unsaturated top cells are predicted."""
+    api = _api()
+    prob = cases.hetero(37, 21, 6, seed=61, overland=True)
+    src = cases.hetero_src(prob, 62) * 40.0
+    gp = api.GPUProblem(prob, src=src)
+    op = O.OracleProblem(prob, src=src)
+    rng = np.random.default_rng(63)
+    p = cases.random_pressure(prob, seed=64)
+    s = O.saturation(op, p)
+    nx, ny, nz = prob.nx, prob.ny, prob.nz
+    vel = (rng.normal(0, 0.02, (nz, ny, nx + 1)), rng.normal(0, 0.02, (nz, ny + 1, nx)),
+           rng.normal(0, 0.02, (nz + 1, ny, nx)))
+    q = (rng.normal(0, 0.05, (ny + 2, nx + 2)), rng.normal(0, 0.05, (ny + 2, nx + 2)))
+    print(f"surface_predictor pv={pv} lateral={lateral}", flush=True)
+    got = gp.surface_predictor(p, s, vel, q, 0.1, pv, lateral)
+    ref = O.surface_predictor(op, p, s, vel, q, 0.1, pv, lateral)
+    changed = ref != p
+    assert 0.1 < changed[nz].mean() < 0.9 and not changed[:nz].any()
+    assert np.array_equal(got != p, changed)
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref[changed]))
+
+
+@pytest.mark.parametrize("pv", [0.0, 1e-5])
+def test_time_loop_surface_predictor_matches_oracle(gpu, pv):
+    """The time loop with Solver.SurfacePredictor on TiltedV_OverlandKin with
+    a strong top source: the predictor changes the first step's initial
+    guess (5 Newton iterations without it, 11 or 6 with it, in the oracle),
+    and the device loop follows the oracle's step for step."""
+    from parflow_amd.problem import TimeControl
+    api = _api()
+    prob, opts, p0, dt, n = ref_decks.tiltedv_kin()
+    src = np.zeros(prob.shape)
+    src[1:-1, 1:-1, 1:-1] = 0.3
+    tc = TimeControl(start_time=0.0, stop_time=0.3, base_unit=0.05, dump_interval=0.1,
+                     ts_type="Constant", value=0.05, surface_predictor=True,
+                     surface_predictor_pressure=pv, surface_lateral_flows=True)
+    print(f"time loop surface predictor pv={pv}", flush=True)
+    gp = api.GPUProblem(prob, src=src)
+    sv = api.GPUSolver(gp, opts)
+    ph, sh, rows = sv.run(p0, tc)
+    op = O.OracleProblem(prob, src=src)
+    pho, sho, rowso = O.advance_ts(op, opts, p0, tc)
+    assert rowso[0][4] == (11 if pv == 0.0 else 6)
+    assert len(rows) == len(rowso)
+    for k, (r, ro) in enumerate(zip(rows, rowso)):
+        assert (r["time"], r["dt"], r["dt_info"], r["dumped"]) == ro[:4], k
+        assert abs(r["newton"] - ro[4]) <= 1 and r["conv_failures"] == ro[7], k
+        a, b = _I(prob, ph[k + 1]), _I(prob, pho[k + 1])
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-9, k
+
+
 def test_kinsol_plain_matches_oracle_default_richards(gpu):
     """Reference deck default_richards (UseJacobian True) on the plain path."""
     prob, opts, p0, dt, n = ref_decks.default_richards()

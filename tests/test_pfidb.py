@@ -258,6 +258,12 @@ def test_time_control_keys():
         ("Growth", 0.0005, 1.2, 0.0001, 0.01)
     prob, opts, p0, dt, n = pfidb.problem_from_pfidb(keys)
     assert dt == 0.0005 and n == 50
+    assert not tc.surface_predictor and tc.surface_predictor_pressure == 0.00001
+    keys.update({"Solver.SurfacePredictor": "True", "Solver.SurfacePredictor.PressureValue":
+                 "0.001", "Solver.SurfacePredictor.LateralFlows": "True"})
+    tc = pfidb.time_control_from_pfidb(keys)
+    assert tc.surface_predictor and tc.surface_lateral_flows
+    assert tc.surface_predictor_pressure == 0.001
     keys["TimeStep.Type"] = "Adaptive"
     with pytest.raises(ValueError):
         pfidb.time_control_from_pfidb(keys)

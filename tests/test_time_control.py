@@ -181,3 +181,57 @@ def test_next_transition_hand_worked():
     assert O.next_transition([((2, 3), 2)], 1.0, 0.0, 100.0, 6.0) == 1.0
     # single-interval cycles never bind
     assert O.next_transition([((7,), -1)], 1.0, 0.0, 100.0, 3.0) == -1.0
+
+
+def _column_problem():
+    from parflow_amd.problem import Problem
+    bc = [BCSpec(BC_FLUX, (0.0,)) for _ in range(6)]
+    return Problem(nx=2, ny=1, nz=2, dx=2.0, dy=1.0, dz=0.5, kx=1.0, poro=0.4, ss=1e-4,
+                   alpha=1.0, nvg=2.0, sres=0.2, ssat=1.0, bc=bc, base_unit=1.0)
+
+
+def test_surface_predictor_hand_worked():
+    """PL/solver_richards.c:3296-3345 worked by hand on a 2 x 1 x 2 box
+    (dx 2, dy 1, dz 0.5, porosity 0.4, dt 1):
+      top cell 0: S 0.9 -> free pore volume 0.4 - 0.36 = 0.04; source 0.05
+        gives 0.05, the lower face (vz 0.01) 0.02, the right face (vx 0.01,
+        outflow) -0.005: 0.065 > 0.04, so p = 0.025 / (2 * 1 * 0.4) = 0.03125
+        (or PressureValue when > 0);
+      top cell 1: S 0.5 -> free 0.2 > 0.05 + 0.005: unchanged;
+      a saturated or ponded cell (p >= 0) is never changed."""
+    prob = _column_problem()
+    src = np.zeros(prob.shape)
+    src[2, 1, 1:3] = 0.05
+    op = O.OracleProblem(prob, src=src)
+    p = np.full(prob.shape, -0.5)
+    s = np.zeros(prob.shape)
+    s[2, 1, 1], s[2, 1, 2] = 0.9, 0.5
+    vx = np.zeros((2, 1, 3))
+    vx[1, 0, 1] = 0.01
+    vy = np.zeros((2, 2, 2))
+    vz = np.zeros((3, 1, 2))
+    vz[1, 0, 0] = 0.01
+    out = O.surface_predictor(op, p, s, (vx, vy, vz), None, 1.0, 0.0, False)
+    assert abs(out[2, 1, 1] - 0.03125) < 1e-15
+    assert out[2, 1, 2] == -0.5
+    assert np.array_equal(np.delete(out.ravel(), np.ravel_multi_index((2, 1, 1), out.shape)),
+                          np.delete(p.ravel(), np.ravel_multi_index((2, 1, 1), p.shape)))
+    assert O.surface_predictor(op, p, s, (vx, vy, vz), None, 1.0, 1e-5, False)[2, 1, 1] == 1e-5
+    p2 = p.copy()
+    p2[2, 1, 1] = 0.0
+    assert O.surface_predictor(op, p2, s, (vx, vy, vz), None, 1.0, 0.0, False)[2, 1, 1] == 0.0
+    # lateral term, as :3312-3321 write it: dt ((qx_i - qx_(i-1)) dy +
+    # (qy_j - qy_(j-1)) dx) is ADDED to the inflow, i.e. a cell's own east
+    # flux ke counts toward its total.  ke = 0.3 in cell 0, 0.1 in cell 1:
+    # cell 0 gets 0.065 + 0.3 = 0.365 > 0.04 -> p = 0.325 / 0.8;
+    # cell 1 gets 0.055 + (0.1 - 0.3) < 0.2 -> unchanged
+    qx = np.zeros((3, 4))
+    qx[1, 1], qx[1, 2] = 0.3, 0.1
+    qy = np.zeros((3, 4))
+    out = O.surface_predictor(op, p, s, (vx, vy, vz), (qx, qy), 1.0, 0.0, True)
+    assert abs(out[2, 1, 1] - 0.325 / 0.8) < 1e-15 and out[2, 1, 2] == -0.5
+    # the west flux of the first column is not counted (i > 0 test, :3316)
+    qx2 = np.zeros((3, 4))
+    qx2[1, 0] = 5.0
+    out = O.surface_predictor(op, p, s, (vx, vy, vz), (qx2, qy), 1.0, 0.0, True)
+    assert abs(out[2, 1, 1] - 0.03125) < 1e-15
