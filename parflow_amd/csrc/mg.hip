@@ -597,6 +597,128 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_rbgs_pair2(LGeom g, Mat<T> A,
   }
 }
 
+// ---- level residual fused with the z restriction ---------------------------
+// For a level coarsened in z with an even x extent: r = b - A x on the fine
+// planes of a z-chunk (resid_point's terms and order), restricted to the
+// coarse planes as the chunk marches up (restrict_point's weights and
+// order), so r is never stored and re-read.  Lanes own two x-adjacent points
+// as in k_rbgs_pair2 (lanes 0 and 63 supply the x neighbours at the tile
+// edges); a row of the tile is one wavefront; the z neighbours and the lower
+// coupling are carried from plane to plane.  A chunk of KC (even) fine
+// planes produces the coarse planes centred on its even planes and
+// recomputes the fine plane below it for the first one's lower weight.
+template <typename T> struct RRSet {
+  typedef typename V2T<T>::type T2;
+  T2 c, bb, ae, an, as, au, xs, xn;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_rr(RRSet<T> &S, const Mat<T> &A, const T *__restrict__ x,
+                                        const T *__restrict__ b, unsigned ip, unsigned sy)
+{
+  typedef typename V2T<T>::type T2;
+  S.c = *(const T2 *)(A.c + ip);
+  S.bb = *(const T2 *)(b + ip);
+  S.ae = *(const T2 *)(A.u[0] + ip);
+  S.an = *(const T2 *)(A.u[1] + ip);
+  S.as = *(const T2 *)(A.u[1] + ip - sy);
+  S.au = *(const T2 *)(A.u[2] + ip);
+  S.xs = *(const T2 *)(x + ip - sy);
+  S.xn = *(const T2 *)(x + ip + sy);
+}
+
+template <typename T, int TY, int KC>
+__global__ __launch_bounds__(64 * TY) void k_resid_restrict_z(LGeom gf, LGeom gc, Mat<T> A,
+                                                              const T *__restrict__ x,
+                                                              const T *__restrict__ b,
+                                                              T *__restrict__ rc,
+                                                              const T *__restrict__ p1,
+                                                              const T *__restrict__ p2)
+{
+  typedef typename V2T<T>::type T2;
+  static_assert(KC % 2 == 0, "z-chunks start on coarse points");
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int gx = (gf.n[0] + PX2 - 1) / PX2, gy = (gf.n[1] + TY - 1) / TY;
+  const int gz = (gf.n[2] + KC - 1) / KC;
+  const int nb = gx * gy * gz, nb8 = (nb + 7) & ~7;
+  const int bid = blockIdx.x;
+  const int Lt = (bid & 7) * (nb8 >> 3) + (bid >> 3);
+  if (Lt >= nb) return;                       // uniform over the workgroup
+  const int bx = Lt % gx, rr = Lt / gx;
+  const int by = rr % gy, bz = rr / gy;
+  const int i0 = bx * PX2 - 2 + 2 * tx;       // even; the lane's points are i0, i0+1
+  const int j = by * TY + ty;
+  const bool valid = j < gf.n[1] && i0 >= 0 && i0 < gf.n[0];
+  const bool owned = valid && tx >= 1 && tx <= 62;
+  const int nz = gf.n[2];
+  const int kb = bz * KC, ke = min(kb + KC, nz);
+  const unsigned sy = gf.s[1], sz = gf.s[2];
+  const bool hw0 = i0 > 0, he1 = i0 + 2 < gf.n[0], hs = j > 0, hn = j + 1 < gf.n[1];
+  // loads come from the clamped column and planes in [0, nz-1]
+  const int ic = min(max(i0, 0), gf.n[0] - 2), jc = min(j, gf.n[1] - 1);
+  const unsigned col = (unsigned)LIDX(gf, ic, jc, 0) - sz;   // plane q at col + (q+1) sz
+  auto at = [&](int q) { return col + (unsigned)(min(max(q, 0), nz - 1) + 1) * sz; };
+  // coarse store address (same x-y layout as the fine level), coarse plane kb/2
+  unsigned ipc = valid ? (unsigned)LIDX(gc, i0, j, kb / 2) : 0u;
+  const unsigned szc = gc.s[2];
+
+  const int k0 = (kb > 0) ? kb - 1 : 0;
+  RRSet<T> cur, nxt;
+  load_rr<T>(cur, A, x, b, at(k0), sy);
+  T2 al = *(const T2 *)(A.u[2] + at(k0 - 1));         // lower coupling of plane k0
+  T2 xm = *(const T2 *)(x + at(k0 - 1));
+  T2 x0 = *(const T2 *)(x + at(k0));
+  T2 xp = *(const T2 *)(x + at(k0 + 1));
+  T2 rF = T2{T(0), T(0)}, lo = T2{T(0), T(0)};
+  for (int k = k0; k < ke; k++) {
+    load_rr<T>(nxt, A, x, b, at(k + 1), sy);
+    const T2 xpp = *(const T2 *)(x + at(k + 2));
+    const bool odd = (k & 1) != 0;
+    T2 w1 = T2{T(0), T(0)}, w2 = T2{T(0), T(0)};
+    if (odd) {                                          // uniform over the workgroup
+      w1 = *(const T2 *)(p1 + at(k));
+      w2 = *(const T2 *)(p2 + at(k));
+    }
+    // r at plane k, both points
+    const T xw0 = from_prev(x0.y), xe1 = from_next(x0.x);
+    const T aw0 = from_prev(cur.ae.y);
+    T v0 = cur.c.x * x0.x;
+    if (hw0) v0 += aw0 * xw0;
+    v0 += cur.ae.x * x0.y;
+    if (hs) v0 += cur.as.x * cur.xs.x;
+    if (hn) v0 += cur.an.x * cur.xn.x;
+    if (k > 0) v0 += al.x * xm.x;
+    if (k + 1 < nz) v0 += cur.au.x * xp.x;
+    T v1 = cur.c.y * x0.y;
+    v1 += cur.ae.x * x0.x;
+    if (he1) v1 += cur.ae.y * xe1;
+    if (hs) v1 += cur.as.y * cur.xs.y;
+    if (hn) v1 += cur.an.y * cur.xn.y;
+    if (k > 0) v1 += al.y * xm.y;
+    if (k + 1 < nz) v1 += cur.au.y * xp.y;
+    const T2 r = T2{cur.bb.x - v0, cur.bb.y - v1};
+    if (!odd) {
+      // a coarse centre: its lower weight came from plane k-1 (none at k = 0)
+      rF = r;
+      if (k == 0) lo = T2{T(0), T(0)};
+      if (k + 1 >= nz) {                               // no plane above: hi = 0
+        if (owned) *(T2 *)(rc + ipc) = T2{rF.x + (lo.x + T(0)), rF.y + (lo.y + T(0))};
+        ipc += szc;
+      }
+    } else {
+      if (k > kb) {                                     // finish the centre at k-1
+        const T2 hi = T2{w1.x * r.x, w1.y * r.y};
+        if (owned) *(T2 *)(rc + ipc) = T2{rF.x + (lo.x + hi.x), rF.y + (lo.y + hi.y)};
+        ipc += szc;
+      }
+      lo = T2{w2.x * r.x, w2.y * r.y};                 // lower weight of the centre at k+1
+    }
+    al = cur.au;
+    cur = nxt;
+    xm = x0; x0 = xp; xp = xpp;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_residual_l(LGeom g, Mat<T> A, const T *__restrict__ x,
                                                     const T *__restrict__ b, T *__restrict__ t)
@@ -848,6 +970,7 @@ struct pfhip_mg {
   void *raw[MAXL][10];         // allocations; each array starts one element in, so
                                // the ghosted index of an even x is vector aligned
   int fuse;                    // red -> black pairs as one launch (k_rbgs_pair)
+  int rrz;                     // fused level residual + z restriction
   int pv[MAXL];                // pair tile shape of each grid level (tune_pairs)
   int tuned;                   // pv chosen (at the first V-cycle)
   void *levdev;                // device LevelDev<T>[L]
@@ -930,6 +1053,8 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
   // first level small enough for the single-workgroup bottom (never level 0)
   const char *fe = getenv("PFHIP_MG_FUSE");
   mg->fuse = (fe && *fe) ? atoi(fe) : 1;
+  const char *fr = getenv("PFHIP_MG_RRZ");
+  mg->rrz = (fr && *fr) ? atoi(fr) : 1;
   long small = MG_SMALL_DEFAULT;
   const char *env = getenv("PFHIP_MG_SMALL");
   if (env && *env) small = atol(env);
@@ -1206,6 +1331,19 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
   return PFHIP_OK;
 }
 
+template <typename T>
+static void launch_resid_restrict_z(const LGeom &gf, const LGeom &gc, const Mat<T> &A,
+                                    const T *x, const T *b, T *rc, const T *p1, const T *p2,
+                                    hipStream_t st)
+{
+  constexpr int TY = 4, KC = 16;
+  const int gx = (gf.n[0] + PX2 - 1) / PX2, gy = (gf.n[1] + TY - 1) / TY;
+  const int gz = (gf.n[2] + KC - 1) / KC;
+  const unsigned nb8 = ((unsigned)(gx * gy * gz) + 7u) & ~7u;
+  hipLaunchKernelGGL((k_resid_restrict_z<T, TY, KC>), dim3(nb8), dim3(64, TY, 1), 0, st, gf,
+                     gc, A, x, b, rc, p1, p2);
+}
+
 // V-cycle of PL/mg_semi.c:205-330 (MaxIter 1, zero initial guess)
 template <typename T>
 static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t st)
@@ -1220,6 +1358,13 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
   for (l = 0; l < lc; l++) {
     if ((rc = rbgs<T>(mg, l, 1, mg->pre, st))) return rc;
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
+    if (mg->rrz && mg->dir[l] == 2 && (gf.n[0] & 1) == 0) {
+      launch_resid_restrict_z<T>(gf, gc, matof<T>(mg, l), (const T *)mg->x[l],
+                                 (const T *)mg->b[l], (T *)mg->b[l + 1], (const T *)mg->p1[l],
+                                 (const T *)mg->p2[l], st);
+      LAUNCHCHK();
+      continue;
+    }
     hipLaunchKernelGGL(k_residual_l<T>, dim3(ntiles(gf)), LBLOCK, 0, st, gf, matof<T>(mg, l),
                        (const T *)mg->x[l], (const T *)mg->b[l], (T *)mg->t[l]);
     LAUNCHCHK();

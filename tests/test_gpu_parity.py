@@ -340,10 +340,14 @@ def test_multigrid_grid_levels(gpu, precision, fuse, shape, monkeypatch):
     runs in one workgroup, with the red -> black pairs fused or not: levels
     of even x extent take the two-points-per-lane pair (ragged 124 x 14 x 16
     tiles, one or two tiles across x), odd ones the one-point pair (62 x 14
-    x 16), and the 64 x 4 tiles of the other kernels are ragged too."""
+    x 16), and the 64 x 4 tiles of the other kernels are ragged too.  fuse
+    also selects the level residual fused with the z restriction (z-chunks
+    of 16 fine planes, ragged and odd plane counts on the coarser levels)
+    or the two separate kernels."""
     api = _api()
     monkeypatch.setenv("PFHIP_MG_SMALL", "300")
     monkeypatch.setenv("PFHIP_MG_FUSE", str(fuse))
+    monkeypatch.setenv("PFHIP_MG_RRZ", str(fuse))
     prob = cases.hetero(*shape, seed=17, dir_sides=True)
     print(f"mg grid levels precision={precision} fuse={fuse} shape={shape}", flush=True)
     gp = api.GPUProblem(prob)
