@@ -493,8 +493,12 @@ __device__ __forceinline__ typename V2T<T>::type load_x2(const T *__restrict__ x
   return *(const T2 *)(xin + ip);
 }
 
+// FP32: ask for 8 waves per SIMD (<= 64 VGPRs), so two 1024-thread tiles fit
+// on a CU; FP64 keeps its register budget
+#define PAIR2_MIN_WAVES(T) (sizeof(T) == 4 ? 8 : 1)
+
 template <typename T, int TY, int KC, bool INIT>
-__global__ __launch_bounds__(64 * (TY + 2)) void k_rbgs_pair2(LGeom g, Mat<T> A,
+__global__ __launch_bounds__(64 * (TY + 2), PAIR2_MIN_WAVES(T)) void k_rbgs_pair2(LGeom g, Mat<T> A,
                                                               const T *__restrict__ xin,
                                                               T *__restrict__ xout,
                                                               const T *__restrict__ b)
