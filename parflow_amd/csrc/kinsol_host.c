@@ -35,6 +35,7 @@ struct pfhip_solver {
   pfhip_solver_opts o;
   long ng;
   double *unew, *fval, *pp, *vtemp1, *vtemp2, *xcor, *vtemp;
+  double *unew_own;             /* unew's allocation (unew and uu exchange) */
   double **V, **Z;
   double *J[7];
   pfhip_mg *mg;
@@ -78,7 +79,7 @@ static int alloc_vec(pfhip_solver *s, double **p)
 int pfhip_solver_destroy(pfhip_solver *s)
 {
   if (!s) return PFHIP_OK;
-  double *v[9] = { s->unew, s->fval, s->pp, s->vtemp1, s->vtemp2, s->xcor,
+  double *v[9] = { s->unew_own, s->fval, s->pp, s->vtemp1, s->vtemp2, s->xcor,
                    s->vtemp, s->p_old, s->s_old };
   for (int i = 0; i < 9; i++) pfhip_free(v[i]);
   if (s->V) for (int i = 0; i <= s->l_max; i++) pfhip_free(s->V[i]);
@@ -115,6 +116,7 @@ int pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
   double **vv[9] = { &s->unew, &s->fval, &s->pp, &s->vtemp1, &s->vtemp2,
                      &s->xcor, &s->vtemp, &s->p_old, &s->s_old };
   for (int i = 0; i < 9 && !rc; i++) rc = alloc_vec(s, vv[i]);
+  s->unew_own = s->unew;
   s->V = (double **)calloc(s->l_max + 1, sizeof(double *));
   for (int i = 0; i <= s->l_max && !rc; i++) rc = alloc_vec(s, &s->V[i]);
   if (opts->krylov == 1) {
@@ -416,8 +418,12 @@ static int lsolve(pfhip_solver *s, double *xx, double *bb, int *lret)
   s->nli += nli_inc;
   if (sret != 0) s->ncfl++;
   CHK(atimes(s, xx, bb));
-  CHK(l2norm(s, bb, &s->sJpnorm));
-  CHK(s->ops->vec_dot(pb, s->fval, bb, &s->sfdotJp, NULL));
+  /* ||J p|| and F.(J p) from one pass: the (u.v, v.v) sums of the DQ
+   * reduction are the same per-cell products in the same order as the two
+   * separate dot products */
+  double jp2, jp1;
+  CHK(s->ops->vec_dq_sums(pb, s->fval, bb, &s->sfdotJp, &jp2, &jp1, NULL));
+  s->sJpnorm = sqrt(jp2);
   *lret = (sret == SPGMR_SUCCESS || sret == SPGMR_RES_REDUCED) ? 0 : 8;
   return PFHIP_OK;
 }
@@ -617,6 +623,7 @@ int pfhip_kinsol_nonlin_solver(pfhip_solver *s, double *p, const double *p_old,
   const pfhip_solver_opts *o = &s->o;
   CHK(s->ops->bc_pressure(pb, time, NULL));
   s->uu = p;
+  s->unew = s->unew_own;
   s->cur_p_old = p_old;
   s->cur_s_old = s_old;
   s->dt = dt;
@@ -681,11 +688,18 @@ int pfhip_kinsol_nonlin_solver(pfhip_solver *s, double *p, const double *p_old,
       if (s->callForcingTerm) forcing_term(s, fnormp);
       s->fnorm = fnormp;
       CHK(kin_stop(s, maxStep, gret, &ret));
-      CHK(s->ops->vec_scale(pb, 1.0, s->unew, s->uu, NULL));
+      /* uu = unew by exchanging the two buffers instead of copying */
+      { double *t = s->uu; s->uu = s->unew; s->unew = t; }
       s->f1norm = f1normp;
       if (ret != 0) break;
     }
   }
+  if (s->uu != p) {
+    /* the accepted iterate is in the solver's own buffer: hand it back in p */
+    CHK(s->ops->vec_scale(pb, 1.0, s->uu, p, NULL));
+    s->uu = p;
+  }
+  s->unew = s->unew_own;
   CHK(s->ops->synchronize());
   if (st) {
     st->ret = ret;
