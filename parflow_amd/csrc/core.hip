@@ -812,6 +812,46 @@ __global__ __launch_bounds__(256) void k_multi_axpy_d(Geom g, VecPtrs V,
   }
 }
 
+// CGS2's middle passes in one: w -= V h (h on the device), then the dot
+// products V[q].w of the updated w, per cell in the order of k_multi_axpy_d
+// followed by k_multi_dot -- the same products and sums, one read of V
+template <int M>
+__global__ __launch_bounds__(256) void k_multi_axpy_dot(Geom g, VecPtrs V,
+                                                        const double *__restrict__ h, int m,
+                                                        double *__restrict__ w,
+                                                        double *__restrict__ partial, int nb)
+{
+  double c[M], acc[M];
+#pragma unroll
+  for (int q = 0; q < M; q++) { c[q] = (q < m) ? -1.0 * h[q] : 0.0; acc[q] = 0.0; }
+  int rows = g.ny * g.nz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    int j = r % g.ny, k = r / g.ny;
+    int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+      int ip = base + i;
+      double v[M];
+#pragma unroll
+      for (int q = 0; q < M; q++) v[q] = (q < m) ? V.v[q][ip] : 0.0;
+      double s = w[ip];
+#pragma unroll
+      for (int q = 0; q < M; q++)
+        if (q < m) s += c[q] * v[q];
+      w[ip] = s;
+#pragma unroll
+      for (int q = 0; q < M; q++)
+        if (q < m) acc[q] += v[q] * s;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < M; q++) {
+    if (q < m) {
+      double t = block_reduce<false>(acc[q]);
+      if (threadIdx.x == 0) partial[q * nb + blockIdx.x] = t;
+    }
+  }
+}
+
 // one device pass of h = V^T w (m values into dst)
 static void multi_dot_dev(pfhip_problem *pb, const Geom &g, int nb, const double *const *V,
                           int m, const double *w, double *dst, hipStream_t st)
@@ -866,8 +906,22 @@ int pfhip_vec_cgs2(pfhip_problem *pb, const double *const *V, int k, double *w, 
   double *d1 = pb->red_dev, *d2 = pb->red_dev + PFHIP_SLOT_H2;
   double *dn = pb->red_dev + PFHIP_SLOT_N2;
   multi_dot_dev(pb, g, nb, V, k, w, d1, st);
-  multi_axpy_dev(pb, g, nb, V, k, d1, w, nullptr, st);
-  multi_dot_dev(pb, g, nb, V, k, w, d2, st);
+  if (k <= 8) {
+    // w -= V h1 and h2 = V^T w in one pass over V
+    VecPtrs P;
+    for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < k) ? V[q] : w;
+    if (k <= 4)
+      hipLaunchKernelGGL(k_multi_axpy_dot<4>, dim3(nb), dim3(256), 0, st, g, P, d1, k, w,
+                         pb->red_partial, nb);
+    else
+      hipLaunchKernelGGL(k_multi_axpy_dot<8>, dim3(nb), dim3(256), 0, st, g, P, d1, k, w,
+                         pb->red_partial, nb);
+    hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, st, pb->red_partial, nb,
+                       k, d2);
+  } else {
+    multi_axpy_dev(pb, g, nb, V, k, d1, w, nullptr, st);
+    multi_dot_dev(pb, g, nb, V, k, w, d2, st);
+  }
   multi_axpy_dev(pb, g, nb, V, k, d2, w, dn, st);
   LAUNCHCHK();
   int rc = fetch_results(pb, PFHIP_MAX_MULTI, stream);

@@ -216,6 +216,48 @@ def test_block_gram_schmidt_products(gpu, m):
     assert abs(n2.value - float(np.sum(wref * wref))) <= 1e-12 * float(np.sum(wref * wref))
 
 
+@pytest.mark.parametrize("k", [1, 4, 5, 8, 9, 17])
+def test_cgs2_matches_separate_passes(gpu, k):
+    """pfhip_vec_cgs2 (for k <= 8 its middle passes fused into one) against
+    the same projection done by separate multi-dot / multi-axpy calls: the
+    same products summed in the same order."""
+    import ctypes as C
+    api = _api()
+    L = api.lib()
+    prob = cases.hetero(37, 21, 9, seed=5)
+    gp = api.GPUProblem(prob)
+    rng = np.random.default_rng(200 + k)
+
+    def field():
+        a = np.full(prob.shape, 1e9)
+        a[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+        return a
+    Vh = [field() for _ in range(k)]
+    wh = field() + 0.3 * Vh[0]
+    V = [gp.vec(a) for a in Vh]
+    ptrs = (C.c_void_p * k)(*[v.value for v in V])
+    vp = C.c_void_p
+    L.pfhip_vec_cgs2.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp]
+    L.pfhip_vec_multi_dot.argtypes = [vp, vp, C.c_int, vp, vp, vp]
+    L.pfhip_vec_multi_axpy.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
+    w1 = gp.vec(wh)
+    h1, h2, n2 = (C.c_double * k)(), (C.c_double * k)(), C.c_double()
+    api.check(L.pfhip_vec_cgs2(gp.h, ptrs, k, w1.ptr, h1, h2, C.byref(n2), None), "cgs2")
+    w2 = gp.vec(wh)
+    a1, a2, m2 = (C.c_double * k)(), (C.c_double * k)(), C.c_double()
+    api.check(L.pfhip_vec_multi_dot(gp.h, ptrs, k, w2.ptr, a1, None), "multi_dot")
+    api.check(L.pfhip_vec_multi_axpy(gp.h, ptrs, k, a1, w2.ptr, None, None), "multi_axpy")
+    api.check(L.pfhip_vec_multi_dot(gp.h, ptrs, k, w2.ptr, a2, None), "multi_dot")
+    api.check(L.pfhip_vec_multi_axpy(gp.h, ptrs, k, a2, w2.ptr, C.byref(m2), None), "axpy")
+    assert list(h1) == list(a1)
+    s2 = max(abs(x) for x in a2) + 1e-300
+    assert max(abs(x - y) for x, y in zip(h2, a2)) <= 1e-13 * s2
+    g1, g2 = _I(prob, gp.get(w1)), _I(prob, gp.get(w2))
+    assert np.max(np.abs(g1 - g2)) <= 1e-13 * np.max(np.abs(g2))
+    assert abs(n2.value - m2.value) <= 1e-13 * m2.value
+    print("bit-identical:", list(h2) == list(a2) and np.array_equal(g1, g2))
+
+
 def test_water_balance_storage(gpu):
     api = _api()
     prob = cases.hetero(37, 21, 9, seed=27)
