@@ -43,6 +43,11 @@ struct pfhip_solver {
   double *vel[3];               /* face velocities for the surface predictor */
   /* host Krylov state */
   double **Hes, *givens, *yg, *hbuf;
+  double *Hraw;                 /* flexible variant: the Hessenberg columns
+                                   before the QR update, (l_max+1) x l_max   */
+  int jp_ok;                    /* flexible variant: J p from the Arnoldi
+                                   relation of the last solve is valid      */
+  double jp_norm2, jp_fdot;     /* its ||J p||^2 and F.(J p)                */
   int l_max, max_restarts;
   /* KINSOL scalars */
   double *uu;
@@ -88,7 +93,7 @@ int pfhip_solver_destroy(pfhip_solver *s)
   for (int q = 0; q < 3; q++) pfhip_free(s->vel[q]);
   if (s->mg) pfhip_mg_destroy(s->mg);
   if (s->Hes) for (int i = 0; i <= s->l_max; i++) free(s->Hes[i]);
-  free(s->Hes); free(s->givens); free(s->yg); free(s->hbuf);
+  free(s->Hes); free(s->givens); free(s->yg); free(s->hbuf); free(s->Hraw);
   free(s->V); free(s->Z);
   free(s);
   return PFHIP_OK;
@@ -135,6 +140,7 @@ int pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
   s->givens = (double *)calloc(2 * s->l_max, sizeof(double));
   s->yg = (double *)calloc(s->l_max + 1, sizeof(double));
   s->hbuf = (double *)calloc(2 * (s->l_max + 2), sizeof(double));
+  s->Hraw = (double *)calloc((size_t)(s->l_max + 1) * s->l_max, sizeof(double));
   if (!rc) rc = s->ops->synchronize();
   if (rc) { pfhip_solver_destroy(s); return rc; }
   *out = s;
@@ -353,6 +359,8 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double de
       CHK(atimes(s, av, V[l + 1]));
       if (flex) CHK(cgs2(s, l + 1, &Hes[l + 1][l]));
       else CHK(modified_gs(s, l + 1, &Hes[l + 1][l]));
+      if (flex && ntries == 0)
+        for (int i = 0; i <= l + 1; i++) s->Hraw[(size_t)i * l_max + l] = Hes[i][l];
       if (qrfact(krydim, Hes, s->givens) != 0) { *ret = SPGMR_QRFACT_FAIL; return PFHIP_OK; }
       rotation_product *= s->givens[2 * l + 1];
       *res_norm = rho = fabs(rotation_product * r_norm);
@@ -365,6 +373,23 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double de
     if (flex) {
       /* x += Z y directly (flexible GMRES) */
       CHK(s->ops->vec_combine(pb, (const double *const *)s->Z, krydim, s->yg, x, NULL));
+      if (ntries == 0) {
+        /* Arnoldi relation of this cycle: J Z = V Hbar, so with x = Z y
+         * (x started at 0), J x = V g with g = Hbar y; V has orthonormal
+         * columns and V[0] = -F / r_norm, hence ||J x|| = ||g|| and
+         * F.(J x) = -r_norm g[0] */
+        double g2 = 0.0, g0 = 0.0;
+        for (int i = 0; i <= krydim; i++) {
+          double gi = 0.0;
+          for (int j = (i > 0 ? i - 1 : 0); j < krydim; j++)
+            gi += s->Hraw[(size_t)i * l_max + j] * s->yg[j];
+          if (i == 0) g0 = gi;
+          g2 += gi * gi;
+        }
+        s->jp_ok = 1;
+        s->jp_norm2 = g2;
+        s->jp_fdot = -beta * g0;
+      }
     } else {
       for (int k = 0; k < krydim; k++)
         CHK(s->ops->vec_linear_sum(pb, s->yg[k], V[k], 1.0, s->xcor, s->xcor, NULL));
@@ -414,16 +439,25 @@ static int lsolve(pfhip_solver *s, double *xx, double *bb, int *lret)
   double res_norm;
   CHK(s->ops->vec_const(pb, 0.0, xx, NULL));
   s->new_uu = 1;
+  s->jp_ok = 0;
   CHK(spgmr_solve(s, xx, s->fval, s->eps, &res_norm, &nli_inc, &sret));
   s->nli += nli_inc;
   if (sret != 0) s->ncfl++;
-  CHK(atimes(s, xx, bb));
-  /* ||J p|| and F.(J p) from one pass: the (u.v, v.v) sums of the DQ
-   * reduction are the same per-cell products in the same order as the two
-   * separate dot products */
-  double jp2, jp1;
-  CHK(s->ops->vec_dq_sums(pb, s->fval, bb, &s->sfdotJp, &jp2, &jp1, NULL));
-  s->sJpnorm = sqrt(jp2);
+  if (s->o.krylov == 1 && s->jp_ok) {
+    /* flexible variant: ||J p|| and F.(J p) from the Arnoldi relation of
+     * the solve (no further residual evaluation); the reference evaluates
+     * the difference quotient at p instead (KS/kinspgmr.c:400-412) */
+    s->sJpnorm = sqrt(s->jp_norm2);
+    s->sfdotJp = s->jp_fdot;
+  } else {
+    CHK(atimes(s, xx, bb));
+    /* ||J p|| and F.(J p) from one pass: the (u.v, v.v) sums of the DQ
+     * reduction are the same per-cell products in the same order as the
+     * two separate dot products */
+    double jp2, jp1;
+    CHK(s->ops->vec_dq_sums(pb, s->fval, bb, &s->sfdotJp, &jp2, &jp1, NULL));
+    s->sJpnorm = sqrt(jp2);
+  }
   *lret = (sret == SPGMR_SUCCESS || sret == SPGMR_RES_REDUCED) ? 0 : 8;
   return PFHIP_OK;
 }
