@@ -325,8 +325,10 @@ static int qrsol(int n, double **h, double *q, double *b)
  * with krylov = 1 the flexible variant keeps Z[l] = M^{-1} V[l].  The
  * right-hand side is b = -fneg (the Newton system J p = -F): ||b|| is
  * ||fneg|| and V[0] = b / ||b|| is formed in one pass. */
-static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double delta,
-                       double *res_norm, int *nli, int *ret)
+/* fneg_norm: ||fneg|| when the caller already holds it (computed by the same
+ * reduction), else < 0 */
+static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fneg_norm,
+                       double delta, double *res_norm, int *nli, int *ret)
 {
   pfhip_problem *pb = s->pb;
   double **V = s->V, **Hes = s->Hes;
@@ -335,7 +337,8 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double de
   double beta, rotation_product, r_norm, s_product, rho = 0;
   int krydim = 0, converged = 0;
   *nli = 0;
-  CHK(l2norm(s, fneg, &r_norm));
+  if (fneg_norm >= 0.0) r_norm = fneg_norm;
+  else CHK(l2norm(s, fneg, &r_norm));
   *res_norm = beta = r_norm;
   if (r_norm <= delta) { *ret = SPGMR_SUCCESS; return PFHIP_OK; }
   if (!flex) CHK(s->ops->vec_const(pb, 0.0, s->xcor, NULL));
@@ -440,7 +443,9 @@ static int lsolve(pfhip_solver *s, double *xx, double *bb, int *lret)
   CHK(s->ops->vec_const(pb, 0.0, xx, NULL));
   s->new_uu = 1;
   s->jp_ok = 0;
-  CHK(spgmr_solve(s, xx, s->fval, s->eps, &res_norm, &nli_inc, &sret));
+  /* s->fnorm is l2norm(fval) of the current iterate: the line search (or
+   * the initial check) computed it with the same reduction */
+  CHK(spgmr_solve(s, xx, s->fval, s->fnorm, s->eps, &res_norm, &nli_inc, &sret));
   s->nli += nli_inc;
   if (sret != 0) s->ncfl++;
   if (s->o.krylov == 1 && s->jp_ok) {
