@@ -68,6 +68,13 @@ typedef struct {
   double ov_eps;                     /* Solver.OverlandKinematic.Epsilon    */
   pfhip_bc_spec bc[6];
   double base_unit, start_time;      /* TimingInfo.BaseUnit / StartTime     */
+  /* van Genuchten kr lookup table (Geom.*.RelPerm.NumSamplePoints,
+   * .MinPressureHead, .InterpolationMethod; PL/problem_phase_rel_perm.c
+   * :116-354, :1894-1915): points 0 for the closed form; else the table of
+   * kr and its derivative for the region's alpha and n (which every cell
+   * must then share), method 0 Spline, 1 Linear, min_head < 0. */
+  int vg_table_points, vg_table_method;
+  double vg_table_min_head, vg_table_alpha, vg_table_n;
 } pfhip_problem_desc;
 
 typedef struct pfhip_problem pfhip_problem;

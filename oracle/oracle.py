@@ -39,7 +39,9 @@ class ProblemC(C.Structure):
                 ("ov_eps", C.c_double),
                 ("bc", BCSpecC * 6),
                 ("base_unit", C.c_double), ("start_time", C.c_double),
-                ("bc_val", _dp * 6)]
+                ("bc_val", _dp * 6),
+                ("vgt", _dp), ("vgt_points", C.c_int), ("vgt_method", C.c_int),
+                ("vgt_min_head", C.c_double)]
 
 
 class OptsC(C.Structure):
@@ -78,6 +80,7 @@ def lib():
             build()
         _lib = C.CDLL(_LIB)
         _lib.pfo_kinsol.restype = C.c_int
+        _lib.pfo_vg_table_lookup.restype = C.c_double
         _lib.pfo_advance.restype = C.c_int
         for f in ("pfo_dot", "pfo_l2norm", "pfo_maxnorm", "pfo_l1norm"):
             getattr(_lib, f).restype = C.c_double
@@ -136,6 +139,13 @@ class OracleProblem:
             c.bc_val[f] = _ptr(buf)
         c.base_unit = prob.base_unit
         c.start_time = prob.start_time
+        tp = prob.vg_table_params
+        self.vg_table = None
+        if tp is not None:
+            method, pts, head, alpha, n = tp
+            self.vg_table = vg_table_build(method, pts, head, alpha, n)
+            c.vgt = _ptr(self.vg_table)
+            c.vgt_points, c.vgt_method, c.vgt_min_head = pts, method, head
         self.c = c
         self.bc_bufs = [k for k in self.keep[-6:]]
 
@@ -145,6 +155,20 @@ class OracleProblem:
 
     def set_time(self, t):
         lib().pfo_bc_pressure(self.ref, C.c_double(t))
+
+
+def vg_table_build(method, points, min_head, alpha, n):
+    """pfo_vg_table_build: 7 x (points+1) array x | a | d | a_der | d_der |
+    slope | slope_der."""
+    tab = np.zeros(7 * (points + 1))
+    lib().pfo_vg_table_build(C.c_int(method), C.c_int(points), C.c_double(min_head),
+                             C.c_double(alpha), C.c_double(n), _ptr(tab))
+    return tab
+
+
+def vg_table_lookup(tab, method, points, min_head, head, fcn=0):
+    return lib().pfo_vg_table_lookup(_ptr(tab), C.c_int(method), C.c_int(points),
+                                     C.c_double(min_head), C.c_double(head), C.c_int(fcn))
 
 
 def opts_c(o):

@@ -61,6 +61,12 @@ typedef struct {
   double base_unit, start_time;         /* TimingInfo.BaseUnit/StartTime    */
   /* per-face BC values at the current time, filled by pfo_bc_pressure */
   double *bc_val[6];
+  /* van Genuchten kr lookup table (RelPerm.NumSamplePoints > 0) or NULL:
+   * 7 arrays of vgt_points+1 values, x | a | d | a_der | d_der | slope |
+   * slope_der, built by pfo_vg_table_build */
+  const double *vgt;
+  int vgt_points, vgt_method;           /* method 0 spline, 1 linear        */
+  double vgt_min_head;
 } pfo_problem;
 
 /* CALCFCN / CALCDER flags (PL/problem_saturation.c) */
@@ -89,6 +95,10 @@ typedef struct {
 } pfo_kin_stats;
 
 /* ---- closures ---------------------------------------------------------- */
+void   pfo_vg_table_build(int method, int points, double min_head, double alpha,
+                          double n, double *tab);
+double pfo_vg_table_lookup(const double *tab, int method, int points,
+                           double min_head, double head, int fcn);
 void pfo_saturation(const pfo_problem *pb, const double *p, double *out,
                     int fcn);
 void pfo_relperm(const pfo_problem *pb, const double *p, double *out, int fcn);

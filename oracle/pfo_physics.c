@@ -74,25 +74,157 @@ static inline double vg_dsat(double p, double rho, double g, double alpha,
          / (pow(1.0 + pow(alpha * head, n), m + 1));
 }
 
-/* kr(p): PL/problem_phase_rel_perm.c:767-792 */
-static inline double vg_kr(double p, double rho, double g, double alpha,
-                           double n)
+/* ---------------------------------------------------------------------- */
+/* van Genuchten kr lookup table                                           */
+/* ---------------------------------------------------------------------- */
+
+/* VanGComputeTable (PL/problem_phase_rel_perm.c:116-262): kr and its
+ * derivative formula at points+1 evenly spaced heads x_i = i |min_head| /
+ * (points-1), the Fritsch-Carlson monotone slopes d of both, and the
+ * linear slopes.  tab holds 7 arrays of points+1 values:
+ * x | a | d | a_der | d_der | slope | slope_der. */
+void pfo_vg_table_build(int method, int points, double min_head, double alpha,
+                        double n, double *tab)
+{
+  const int N = points;
+  double *x = tab, *a = tab + (N + 1), *d = tab + 2 * (N + 1);
+  double *a_der = tab + 3 * (N + 1), *d_der = tab + 4 * (N + 1);
+  double *slope = tab + 5 * (N + 1), *slope_der = tab + 6 * (N + 1);
+  double interval = min_head / (double)(N - 1);
+  double ainterval = fabs(interval);
+  double m = 1.0e0 - (1.0e0 / n);
+  for (int i = 0; i <= N; i++)
+  {
+    x[i] = fabs(i * interval);
+    double opahn = 1.0 + pow(alpha * x[i], n);
+    double ahnm1 = pow(alpha * x[i], n - 1);
+    a[i] = pow(1.0 - ahnm1 / (pow(opahn, m)), 2) / pow(opahn, (m / 2));
+    double coeff = 1.0 - ahnm1 * pow(opahn, -m);
+    a_der[i] = 2.0 * (coeff / (pow(opahn, (m / 2))))
+               * ((n - 1) * pow(alpha * x[i], n - 2) * alpha * pow(opahn, -m)
+                  - ahnm1 * m * pow(opahn, -(m + 1)) * n * alpha * ahnm1)
+               + pow(coeff, 2) * (m / 2) * pow(opahn, (-(m + 2) / 2))
+               * n * alpha * ahnm1;
+    /* 1 < n < 2: the derivative is infinite at head 0 (:181-184) */
+    if ((n < 2) && (i == 0)) a_der[i] = 0;
+    slope[i] = 0.0;
+    slope_der[i] = 0.0;
+  }
+  if (method == 1)
+    for (int i = 0; i < N; i++)
+    {
+      slope[i] = (a[i + 1] - a[i]) / ainterval;
+      slope_der[i] = (a_der[i + 1] - a_der[i]) / ainterval;
+    }
+  /* monotone cubic Hermite slopes, Fritsch & Carlson (1980) (:201-259) */
+  double *del = (double*)malloc(sizeof(double) * (N + 1));
+  double *del_der = (double*)malloc(sizeof(double) * (N + 1));
+  for (int i = 0; i < N; i++)
+  {
+    double h = x[i + 1] - x[i];
+    del[i] = (a[i + 1] - a[i]) / h;
+    del_der[i] = (a_der[i + 1] - a_der[i]) / h;
+  }
+  d[0] = del[0];
+  d[N] = del[N - 1];
+  d_der[0] = del_der[0];
+  d_der[N] = del_der[N - 1];
+  for (int i = 1; i < N; i++)
+  {
+    d[i] = (del[i - 1] + del[i]) / 2;
+    d_der[i] = (del_der[i - 1] + del_der[i]) / 2;
+  }
+  for (int i = 0; i < N; i++)
+  {
+    if (del[i] == 0.0)
+    {
+      d[i] = 0;
+      d[i + 1] = 0;
+    }
+    else
+    {
+      double al = d[i] / del[i], be = d[i + 1] / del[i];
+      double magn = pow(al, 2) + pow(be, 2);
+      if (magn > 9.0)
+      {
+        d[i] = 3 * al * del[i] / magn;
+        d[i + 1] = 3 * be * del[i] / magn;
+      }
+    }
+    if (del_der[i] == 0.0)
+    {
+      d_der[i] = 0;
+      d_der[i + 1] = 0;
+    }
+    else
+    {
+      double al = d_der[i] / del_der[i], be = d_der[i + 1] / del_der[i];
+      double magn = pow(al, 2) + pow(be, 2);
+      if (magn > 9.0)
+      {
+        d_der[i] = 3 * al * del_der[i] / magn;
+        d_der[i + 1] = 3 * be * del_der[i] / magn;
+      }
+    }
+  }
+  free(del);
+  free(del_der);
+}
+
+/* VanGLookupSpline (PL/problem_phase_rel_perm.c:265-354) and the linear
+ * lookup of PhaseRelPerm (:718-760, :846-893); head >= 0, fcn 0 kr,
+ * 1 the derivative formula's value.  0 beyond |min_head|. */
+double pfo_vg_table_lookup(const double *tab, int method, int points,
+                           double min_head, double head, int fcn)
+{
+  const int N = points;
+  const double *x = tab, *a = tab + (N + 1), *d = tab + 2 * (N + 1);
+  const double *a_der = tab + 3 * (N + 1), *d_der = tab + 4 * (N + 1);
+  const double *slope = tab + 5 * (N + 1), *slope_der = tab + 6 * (N + 1);
+  double interval = fabs(min_head / (double)(N - 1));
+  if (head >= fabs(min_head)) return 0.0;
+  int pt = (int)floor(head / interval);
+  if (method == 1)
+  {
+    if (fcn == PFO_CALCFCN) return a[pt] + slope[pt] * (head - x[pt]);
+    return a_der[pt] + slope_der[pt] * (head - x[pt]);
+  }
+  if (pt > N + 1) pt = N;
+  double x0 = x[pt], h = x[pt + 1] - x0;
+  double t = (head - x0) / h;
+  const double *A = (fcn == PFO_CALCFCN) ? a : a_der;
+  const double *D = (fcn == PFO_CALCFCN) ? d : d_der;
+  return (2.0 * pow(t, 3) - 3.0 * pow(t, 2) + 1.0) * A[pt]
+         + (pow(t, 3) - 2.0 * pow(t, 2) + t) * h * D[pt]
+         + (-2.0 * pow(t, 3) + 3.0 * pow(t, 2)) * A[pt + 1]
+         + (pow(t, 3) - pow(t, 2)) * h * D[pt + 1];
+}
+
+/* kr(p): PL/problem_phase_rel_perm.c:767-792, or the problem's table */
+static inline double vg_kr(const pfo_problem *pb, double p, double rho, double g,
+                           double alpha, double n)
 {
   if (p >= 0.0) return 1.0;
-  double m = 1.0e0 - (1.0e0 / n);
   double head = fabs(p) / (rho * g);
+  if (pb->vgt)
+    return pfo_vg_table_lookup(pb->vgt, pb->vgt_method, pb->vgt_points,
+                               pb->vgt_min_head, head, PFO_CALCFCN);
+  double m = 1.0e0 - (1.0e0 / n);
   double opahn = 1.0 + pow(alpha * head, n);
   double ahnm1 = pow(alpha * head, n - 1);
   return pow(1.0 - ahnm1 / (pow(opahn, m)), 2) / pow(opahn, (m / 2));
 }
 
-/* dkr/dp: PL/problem_phase_rel_perm.c:895-912 */
-static inline double vg_dkr(double p, double rho, double g, double alpha,
-                            double n)
+/* dkr/dp: PL/problem_phase_rel_perm.c:895-912, or the problem's table */
+static inline double vg_dkr(const pfo_problem *pb, double p, double rho, double g,
+                            double alpha, double n)
 {
   if (p >= 0.0) return 0.0;
-  double m = 1.0e0 - (1.0e0 / n);
   double head = fabs(p) / (rho * g);
+  if (pb->vgt)
+    return pfo_vg_table_lookup(pb->vgt, pb->vgt_method, pb->vgt_points,
+                               pb->vgt_min_head, head, PFO_CALCDER);
+  double m = 1.0e0 - (1.0e0 / n);
   double opahn = 1.0 + pow(alpha * head, n);
   double ahnm1 = pow(alpha * head, n - 1);
   double coeff = 1.0 - ahnm1 * pow(opahn, -m);
@@ -134,9 +266,9 @@ void pfo_relperm(const pfo_problem *pb, const double *p, double *out, int fcn)
       {
         int ip = IDX(i, j, k);
         out[ip] = (fcn == PFO_CALCFCN)
-                  ? vg_kr(p[ip], pb->rho, pb->gravity, pb->alpha[ip],
+                  ? vg_kr(pb, p[ip], pb->rho, pb->gravity, pb->alpha[ip],
                           pb->nvg[ip])
-                  : vg_dkr(p[ip], pb->rho, pb->gravity, pb->alpha[ip],
+                  : vg_dkr(pb, p[ip], pb->rho, pb->gravity, pb->alpha[ip],
                            pb->nvg[ip]);
       }
 }
@@ -391,7 +523,7 @@ static double dir_u_new(const pfo_problem *pb, int fc, int ip, double value,
   const double rho = pb->rho, g = pb->gravity, mu = pb->mu;
   const double z_dir_g = 1.0;
   double h = (fc < 2) ? ffx : (fc < 4) ? ffy : ffz;
-  double kg = vg_kr(value, rho, g, pb->alpha[ip], pb->nvg[ip]) * rho;
+  double kg = vg_kr(pb, value, rho, g, pb->alpha[ip], pb->nvg[ip]) * rho;
   double u_new;
   if (fc == 0)
   {
@@ -471,7 +603,7 @@ void pfo_nl_function_eval(const pfo_problem *pb, const double *p,
       for (int i = 0; i < nx; i++)
       {
         int ip = IDX(i, j, k);
-        krho[ip] = vg_kr(p[ip], rho, g, pb->alpha[ip], pb->nvg[ip]) * rho;
+        krho[ip] = vg_kr(pb, p[ip], rho, g, pb->alpha[ip], pb->nvg[ip]) * rho;
       }
 
   #pragma omp parallel for collapse(2)
@@ -638,7 +770,7 @@ void pfo_face_velocity(const pfo_problem *pb, const double *p,
       for (int i = 0; i < nx; i++)
       {
         int ip = IDX(i, j, k);
-        krho[ip] = vg_kr(p[ip], rho, g, pb->alpha[ip], pb->nvg[ip]) * rho;
+        krho[ip] = vg_kr(pb, p[ip], rho, g, pb->alpha[ip], pb->nvg[ip]) * rho;
       }
 
   /* value the BC patch loop of face fc leaves for boundary cell (i,j,k) */
@@ -830,9 +962,9 @@ void pfo_jacobian_eval(const pfo_problem *pb, const double *p, double dt,
       for (int i = 0; i < nx; i++)
       {
         int ip = IDX(i, j, k);
-        krho[ip] = vg_kr(p[ip], rho, g, pb->alpha[ip], pb->nvg[ip]) * rho;
-        dkrho[ip] = vg_dkr(p[ip], rho, g, pb->alpha[ip], pb->nvg[ip]) * rho
-                    + vg_kr(p[ip], rho, g, pb->alpha[ip], pb->nvg[ip]) * ddp;
+        krho[ip] = vg_kr(pb, p[ip], rho, g, pb->alpha[ip], pb->nvg[ip]) * rho;
+        dkrho[ip] = vg_dkr(pb, p[ip], rho, g, pb->alpha[ip], pb->nvg[ip]) * rho
+                    + vg_kr(pb, p[ip], rho, g, pb->alpha[ip], pb->nvg[ip]) * ddp;
       }
 
   int overland = (pb->bc[5].type == PFO_BC_OVERLAND);
@@ -990,7 +1122,7 @@ void pfo_jacobian_eval(const pfo_problem *pb, const double *p, double dt,
           double den_d = rho;
           double prod = krho[ip];
           double prod_der = dkrho[ip];
-          double kval = vg_kr(value, rho, g, pb->alpha[ip], pb->nvg[ip]);
+          double kval = vg_kr(pb, value, rho, g, pb->alpha[ip], pb->nvg[ip]);
           double prod_val = kval * den_d;
           double coeff, diff, o_temp;
           if (fc == 0 || fc == 2)

@@ -53,7 +53,10 @@ class ProblemDescC(C.Structure):
                 ("slope_x", C.c_void_p), ("slope_y", C.c_void_p), ("mann", C.c_void_p),
                 ("ov_eps", C.c_double),
                 ("bc", BCSpecC * 6),
-                ("base_unit", C.c_double), ("start_time", C.c_double)]
+                ("base_unit", C.c_double), ("start_time", C.c_double),
+                ("vg_table_points", C.c_int), ("vg_table_method", C.c_int),
+                ("vg_table_min_head", C.c_double), ("vg_table_alpha", C.c_double),
+                ("vg_table_n", C.c_double)]
 
 
 class SolverOptsC(C.Structure):
@@ -364,6 +367,10 @@ class GPUProblem:
             _fill_bc(d.bc[f], b)
         d.base_unit = prob.base_unit
         d.start_time = prob.start_time
+        tp = prob.vg_table_params
+        if tp is not None:
+            (d.vg_table_method, d.vg_table_points, d.vg_table_min_head, d.vg_table_alpha,
+             d.vg_table_n) = tp
         self.desc = d
         h = C.c_void_p()
         check(lib().pfhip_problem_create(C.byref(d), C.byref(h)), "pfhip_problem_create")

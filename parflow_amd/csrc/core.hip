@@ -233,6 +233,65 @@ static int face_ncells(const pfhip_problem_desc *d, int f)
   return d->nx * d->ny;
 }
 
+// VanGComputeTable (PL/problem_phase_rel_perm.c:116-262) on the host: kr and
+// its derivative formula at points+1 heads spaced |min_head| / (points-1)
+// from 0, the Fritsch-Carlson monotone Hermite slopes of both (Fritsch &
+// Carlson, SIAM J. Numer. Anal. 17 (1980)) and the linear slopes, stored
+// x | a | d | a_der | d_der | slope | slope_der
+static void build_kr_table(int method, int points, double min_head, double alpha, double n,
+                           double *tab)
+{
+  const int N = points, N1 = points + 1;
+  double *x = tab, *a = tab + N1, *d = tab + 2 * N1, *a_der = tab + 3 * N1;
+  double *d_der = tab + 4 * N1, *slope = tab + 5 * N1, *slope_der = tab + 6 * N1;
+  const double step = min_head / (double)(N - 1), astep = fabs(step);
+  const double m = 1.0e0 - (1.0e0 / n);
+  for (int i = 0; i < N1; i++) {
+    x[i] = fabs(i * step);
+    double opahn = 1.0 + pow(alpha * x[i], n);
+    double ahnm1 = pow(alpha * x[i], n - 1);
+    a[i] = pow(1.0 - ahnm1 / pow(opahn, m), 2) / pow(opahn, m / 2);
+    double coeff = 1.0 - ahnm1 * pow(opahn, -m);
+    a_der[i] = 2.0 * (coeff / pow(opahn, m / 2)) *
+                   ((n - 1) * pow(alpha * x[i], n - 2) * alpha * pow(opahn, -m) -
+                    ahnm1 * m * pow(opahn, -(m + 1)) * n * alpha * ahnm1) +
+               pow(coeff, 2) * (m / 2) * pow(opahn, -(m + 2) / 2) * n * alpha * ahnm1;
+    if (n < 2 && i == 0) a_der[i] = 0;   // unbounded at head 0 for 1 < n < 2
+  }
+  if (method == 1)
+    for (int i = 0; i < N; i++) {
+      slope[i] = (a[i + 1] - a[i]) / astep;
+      slope_der[i] = (a_der[i + 1] - a_der[i]) / astep;
+    }
+  double *del = (double *)malloc(sizeof(double) * 2 * N1), *del_der = del + N1;
+  for (int i = 0; i < N; i++) {
+    double h = x[i + 1] - x[i];
+    del[i] = (a[i + 1] - a[i]) / h;
+    del_der[i] = (a_der[i + 1] - a_der[i]) / h;
+  }
+  d[0] = del[0]; d[N] = del[N - 1];
+  d_der[0] = del_der[0]; d_der[N] = del_der[N - 1];
+  for (int i = 1; i < N; i++) {
+    d[i] = (del[i - 1] + del[i]) / 2;
+    d_der[i] = (del_der[i - 1] + del_der[i]) / 2;
+  }
+  // limit the slopes interval by interval so the interpolant stays monotone
+  auto limit = [](double *dd, const double *dl, int i) {
+    if (dl[i] == 0.0) { dd[i] = 0; dd[i + 1] = 0; return; }
+    double al = dd[i] / dl[i], be = dd[i + 1] / dl[i];
+    double magn = pow(al, 2) + pow(be, 2);
+    if (magn > 9.0) {
+      dd[i] = 3 * al * dl[i] / magn;
+      dd[i + 1] = 3 * be * dl[i] / magn;
+    }
+  };
+  for (int i = 0; i < N; i++) {
+    limit(d, del, i);
+    limit(d_der, del_der, i);
+  }
+  free(del);
+}
+
 extern "C" int pfhip_problem_create(const pfhip_problem_desc *desc,
                                     pfhip_problem **out)
 {
@@ -262,6 +321,13 @@ extern "C" int pfhip_problem_create(const pfhip_problem_desc *desc,
   if (d->bc[5].type == PFHIP_BC_OVERLAND &&
       (!d->slope_x || !d->slope_y || !d->mann)) {
     pfhip_set_error("OverlandKinematic needs slope_x, slope_y, mann");
+    return PFHIP_ERR_ARG;
+  }
+  if (d->vg_table_points != 0 &&
+      (d->vg_table_points < 2 || !(d->vg_table_min_head < 0.0) || d->vg_table_method < 0 ||
+       d->vg_table_method > 1 || !(d->vg_table_n > 1.0) || !(d->vg_table_alpha > 0.0))) {
+    pfhip_set_error("bad van Genuchten table: points %d, min head %g, method %d",
+                    d->vg_table_points, d->vg_table_min_head, d->vg_table_method);
     return PFHIP_ERR_ARG;
   }
   long ng = (long)(d->nx + 2) * (d->ny + 2) * (d->nz + 2);
@@ -297,6 +363,22 @@ extern "C" int pfhip_problem_create(const pfhip_problem_desc *desc,
   }
   if (rc == PFHIP_OK) rc = pfhip_zero(pb->krho, ng, nullptr);
   if (rc == PFHIP_OK) rc = pfhip_zero(pb->dkrho, ng, nullptr);
+  if (rc == PFHIP_OK && d->vg_table_points > 0) {
+    const int N1 = d->vg_table_points + 1;
+    double *host = (double *)calloc(7 * (size_t)N1, sizeof(double));
+    if (!host) rc = PFHIP_ERR_NOMEM;
+    if (rc == PFHIP_OK) {
+      build_kr_table(d->vg_table_method, d->vg_table_points, d->vg_table_min_head,
+                     d->vg_table_alpha, d->vg_table_n, host);
+      rc = pfhip_dev_alloc(&pb->krtab, 7 * (long)N1);
+    }
+    if (rc == PFHIP_OK && hipMemcpy(pb->krtab, host, sizeof(double) * 7 * N1,
+                                    hipMemcpyHostToDevice) != hipSuccess) {
+      pfhip_set_error("hipMemcpy of the kr table");
+      rc = PFHIP_ERR_HIP;
+    }
+    free(host);
+  }
   if (rc != PFHIP_OK) { pfhip_problem_destroy(pb); return rc; }
   HIPCHK(hipDeviceSynchronize());
   *out = pb;
@@ -316,6 +398,7 @@ extern "C" int pfhip_problem_destroy(pfhip_problem *pb)
   if (pb->red_partial) hipFree(pb->red_partial);
   if (pb->red_dev) hipFree(pb->red_dev);
   if (pb->h_red) hipHostFree(pb->h_red);
+  if (pb->krtab) hipFree(pb->krtab);
   free(pb);
   return PFHIP_OK;
 }
@@ -1091,6 +1174,36 @@ __device__ __forceinline__ double vg_dsat(double p, double irg, double alpha, do
   return (v.m * n * alpha * v.ahnm1) * s_dif * pf_div(v.a, 1.0 + v.t);
 }
 
+// kr lookup table of PL/problem_phase_rel_perm.c:116-354 (region-uniform
+// alpha, n): 7 arrays of n+1 values x | a | d | a_der | d_der | slope |
+// slope_der on the device; t == nullptr selects the closed form
+struct KrTab {
+  const double *__restrict__ t;
+  int n, method;                     // method 0 monotone cubic Hermite, 1 linear
+  double interval, maxhead;          // node spacing, |MinPressureHead|
+};
+
+// kr (der = false) or the derivative formula's value (der = true) at head
+// h >= 0: VanGLookupSpline (:265-354) / the linear lookup (:718-760)
+__device__ __forceinline__ double kr_table(const KrTab &tb, double head, bool der)
+{
+  if (head >= tb.maxhead) return 0.0;
+  const int N1 = tb.n + 1;
+  int pt = (int)floor(head / tb.interval);
+  const double *x = tb.t;
+  const double *a = tb.t + N1 * (der ? 3 : 1);
+  if (tb.method == 1) {
+    const double *sl = tb.t + N1 * (der ? 6 : 5);
+    return a[pt] + sl[pt] * (head - x[pt]);
+  }
+  if (pt > tb.n + 1) pt = tb.n;
+  const double *d = tb.t + N1 * (der ? 4 : 2);
+  const double x0 = x[pt], h = x[pt + 1] - x0;
+  const double t = (head - x0) / h, t2 = t * t, t3 = t2 * t;
+  return (2.0 * t3 - 3.0 * t2 + 1.0) * a[pt] + (t3 - 2.0 * t2 + t) * h * d[pt] +
+         (-2.0 * t3 + 3.0 * t2) * a[pt + 1] + (t3 - t2) * h * d[pt + 1];
+}
+
 __device__ __forceinline__ double vg_kr_t(const VGTerms &v)
 {
   double coeff = 1.0 - v.ahnm1 * v.a;
@@ -1107,27 +1220,36 @@ __device__ __forceinline__ double vg_dkr_t(const VGTerms &v, double alpha, doubl
          (coeff * coeff) * (m / 2) * (v.sa * ropahn) * n * alpha * v.ahnm1;
 }
 
-__device__ __forceinline__ double vg_kr(double p, double irg, double alpha, double n)
+template <bool TAB = true>
+__device__ __forceinline__ double vg_kr(double p, double irg, double alpha, double n,
+                                        const KrTab &tb)
 {
   if (p >= 0.0) return 1.0;
+  if (TAB && tb.t) return kr_table(tb, fabs(p) * irg, false);
   return vg_kr_t(vg_terms(p, irg, alpha, n));
 }
 
-__device__ __forceinline__ double vg_dkr(double p, double irg, double alpha, double n)
+template <bool TAB = true>
+__device__ __forceinline__ double vg_dkr(double p, double irg, double alpha, double n,
+                                         const KrTab &tb)
 {
   if (p >= 0.0) return 0.0;
+  if (TAB && tb.t) return kr_table(tb, fabs(p) * irg, true);
   return vg_dkr_t(vg_terms(p, irg, alpha, n), alpha, n);
 }
 
-// S and kr of one cell from one chain
+// S and kr of one cell from one chain; TAB = false compiles the closed form
+// only (kernels specialised for problems without a table)
+template <bool TAB = true>
 __device__ __forceinline__ void vg_s_kr(double p, double irg, double alpha, double n,
-                                        double sres, double ssat, double &S, double &kr)
+                                        double sres, double ssat, double &S, double &kr,
+                                        const KrTab &tb)
 {
   double s_dif = ssat - sres;
   if (p >= 0.0) { S = s_dif + sres; kr = 1.0; return; }
   VGTerms v = vg_terms(p, irg, alpha, n);
   S = s_dif * v.a + sres;
-  kr = vg_kr_t(v);
+  kr = (TAB && tb.t) ? kr_table(tb, fabs(p) * irg, false) : vg_kr_t(v);
 }
 
 struct Phys {
@@ -1138,6 +1260,7 @@ struct Phys {
   const double *__restrict__ ssat;   // NULL: uniform ssat_v
   double sres_v, ssat_v;
   double irg;                        // 1 / (rho g)
+  KrTab tb;                          // kr lookup table or tb.t == nullptr
   __device__ __forceinline__ double Sres(int ip) const { return sres ? sres[ip] : sres_v; }
   __device__ __forceinline__ double Ssat(int ip) const { return ssat ? ssat[ip] : ssat_v; }
 };
@@ -1149,6 +1272,11 @@ static inline Phys phys_of(const pfhip_problem *pb)
   f.alpha = pb->d.alpha; f.nvg = pb->d.nvg; f.sres = pb->d.sres; f.ssat = pb->d.ssat;
   f.sres_v = pb->d.sres_value; f.ssat_v = pb->d.ssat_value;
   f.irg = 1.0 / (f.rho * f.g);
+  f.tb.t = pb->krtab;
+  f.tb.n = pb->d.vg_table_points;
+  f.tb.method = pb->d.vg_table_method;
+  f.tb.interval = fabs(pb->d.vg_table_min_head / (double)(pb->d.vg_table_points - 1));
+  f.tb.maxhead = fabs(pb->d.vg_table_min_head);
   return f;
 }
 
@@ -1166,13 +1294,13 @@ __global__ __launch_bounds__(256) void k_closure(Geom g, Phys f, int what,
   switch (what) {
     case 0: out[ip] = vg_sat(pv, f.irg, a, n, f.Sres(ip), f.Ssat(ip)); break;
     case 1: out[ip] = vg_dsat(pv, f.irg, a, n, f.Sres(ip), f.Ssat(ip)); break;
-    case 2: out[ip] = vg_kr(pv, f.irg, a, n); break;
-    case 3: out[ip] = vg_dkr(pv, f.irg, a, n); break;
-    case 4: out[ip] = vg_kr(pv, f.irg, a, n) * f.rho; break;
+    case 2: out[ip] = vg_kr(pv, f.irg, a, n, f.tb); break;
+    case 3: out[ip] = vg_dkr(pv, f.irg, a, n, f.tb); break;
+    case 4: out[ip] = vg_kr(pv, f.irg, a, n, f.tb) * f.rho; break;
     default: {
-      double kr = vg_kr(pv, f.irg, a, n);
+      double kr = vg_kr(pv, f.irg, a, n, f.tb);
       out[ip] = kr * f.rho;
-      out2[ip] = vg_dkr(pv, f.irg, a, n) * f.rho + kr * 0.0;
+      out2[ip] = vg_dkr(pv, f.irg, a, n, f.tb) * f.rho + kr * 0.0;
     }
   }
 }
@@ -1427,7 +1555,7 @@ __device__ __forceinline__ double dir_u_new(const ResArgs &A, int fc, int ip, do
   const double rho = A.f.rho, grav = A.f.g, mu = A.f.mu;
   const double dx = A.dx, dy = A.dy, dz = A.dz;
   const double h = (fc < 2) ? dy * dz : (fc < 4) ? dx * dz : dx * dy;
-  double kg = vg_kr(value, A.f.irg, A.f.alpha[ip], A.f.nvg[ip]) * rho;
+  double kg = vg_kr(value, A.f.irg, A.f.alpha[ip], A.f.nvg[ip], A.f.tb) * rho;
   double u_new;
   if (fc == 0) {
     double diff = value - pc;
@@ -1563,12 +1691,12 @@ __global__ __launch_bounds__(RTX * TY) void k_residual_fused(ResArgs A)
     if (k0 > 0) {
       int ipm = col + k0 * g.sz;
       p_m = A.p[ipm];
-      kr_m = vg_kr(p_m, f.irg, f.alpha[ipm], f.nvg[ipm]) * rho;
+      kr_m = vg_kr(p_m, f.irg, f.alpha[ipm], f.nvg[ipm], f.tb) * rho;
       kz_m = A.kz[ipm];
     }
     int ip0 = col + (k0 + 1) * g.sz;
     p_c = A.p[ip0];
-    vg_s_kr(p_c, f.irg, f.alpha[ip0], f.nvg[ip0], f.Sres(ip0), f.Ssat(ip0), S_c, kr_c);
+    vg_s_kr(p_c, f.irg, f.alpha[ip0], f.nvg[ip0], f.Sres(ip0), f.Ssat(ip0), S_c, kr_c, f.tb);
     kr_c *= rho;
     kz_c = A.kz[ip0];
   }
@@ -1582,7 +1710,7 @@ __global__ __launch_bounds__(RTX * TY) void k_residual_fused(ResArgs A)
     if (act && k + 1 < g.nz) {
       int ipn = ip + g.sz;
       p_n = A.p[ipn];
-      vg_s_kr(p_n, f.irg, f.alpha[ipn], f.nvg[ipn], f.Sres(ipn), f.Ssat(ipn), S_n, kr_n);
+      vg_s_kr(p_n, f.irg, f.alpha[ipn], f.nvg[ipn], f.Sres(ipn), f.Ssat(ipn), S_n, kr_n, f.tb);
       kr_n *= rho;
       kz_n = A.kz[ipn];
     }
@@ -1590,7 +1718,7 @@ __global__ __launch_bounds__(RTX * TY) void k_residual_fused(ResArgs A)
     if (hact) {
       double ph = A.p[hip];
       s_p[hls] = ph;
-      s_kr[hls] = vg_kr(ph, f.irg, f.alpha[hip], f.nvg[hip]) * rho;
+      s_kr[hls] = vg_kr(ph, f.irg, f.alpha[hip], f.nvg[hip], f.tb) * rho;
     }
     if (act) { s_p[ls] = p_c; s_kr[ls] = kr_c; }
     __syncthreads();
@@ -1740,7 +1868,7 @@ __device__ __forceinline__ void load_hal(HalIn &h, const ResArgs &A, int hip, do
   h.p = p_at<MODE>(A, hip, psig); h.al = ldb(A.f.alpha, hip); h.n = ldb(A.f.nvg, hip);
 }
 
-template <int TX, int TY, int ZC, bool PCS, int MODE, bool ISO>
+template <int TX, int TY, int ZC, bool PCS, int MODE, bool ISO, bool TAB>
 __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
 {
   constexpr int HALO = 2 * TX + 2 * TY;
@@ -1800,11 +1928,11 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
       VGIn<PCS> vm;
       load_vg<PCS, MODE>(vm, A, col + plane(k0 - 1), psig);
       p_m = vm.p;
-      kr_m = vg_kr(p_m, irg, vm.al, vm.n) * rho;
+      kr_m = vg_kr<TAB>(p_m, irg, vm.al, vm.n, f.tb) * rho;
       kz_m = vm.kz;
     }
     p_c = v0.p;
-    vg_s_kr(p_c, irg, v0.al, v0.n, vg_sr<PCS>(v0, f), vg_st<PCS>(v0, f), S_c, kr_c);
+    vg_s_kr<TAB>(p_c, irg, v0.al, v0.n, vg_sr<PCS>(v0, f), vg_st<PCS>(v0, f), S_c, kr_c, f.tb);
     kr_c *= rho;
     kz_c = v0.kz;
   }
@@ -1831,13 +1959,13 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
     // ---- A: closures of the own cell one plane ahead and of the halo cell
     if (k + 1 < nz) {
       p_n = vgU.p;
-      vg_s_kr(p_n, irg, vgU.al, vgU.n, vg_sr<PCS>(vgU, f), vg_st<PCS>(vgU, f), S_n, kr_n);
+      vg_s_kr<TAB>(p_n, irg, vgU.al, vgU.n, vg_sr<PCS>(vgU, f), vg_st<PCS>(vgU, f), S_n, kr_n, f.tb);
       kr_n *= rho;
       kz_n = vgU.kz;
     }
     if (hact) {
       s_p[hls] = h.p;
-      s_kr[hls] = vg_kr(h.p, irg, h.al, h.n) * rho;
+      s_kr[hls] = vg_kr<TAB>(h.p, irg, h.al, h.n, f.tb) * rho;
     }
     s_p[ls] = p_c;
     s_kr[ls] = kr_c;
@@ -1945,7 +2073,7 @@ __global__ __launch_bounds__(256) void k_residual_bc(ResArgs A, long nshell)
   double psig = 0.0, fsc = 0.0;
   if constexpr (MODE != 0) form_scalars(A, psig, fsc);
   double pc = p_at<MODE>(A, ip, psig);
-  double krc = vg_kr(pc, A.f.irg, A.f.alpha[ip], A.f.nvg[ip]) * A.f.rho;
+  double krc = vg_kr(pc, A.f.irg, A.f.alpha[ip], A.f.nvg[ip], A.f.tb) * A.f.rho;
   double fv = residual_bc(A, i, j, k, ip, A.F[ip], pc, krc, A.kx[ip], A.ky[ip], A.kz[ip]);
   if constexpr (MODE == 1) fv = fsc * fv - fsc * A.fsub[ip];
   A.F[ip] = fv;
@@ -1961,12 +2089,18 @@ static void launch_pf(const ResArgs &A, hipStream_t st)
                   : (MODE == 2) ? PFHIP_PROF_RESIDUAL_STEP : PFHIP_PROF_RESIDUAL;
   PROF_BEGIN(pid, st);
   const bool iso = (A.kx == A.ky) && (A.kx == A.kz);
-  if (A.f.sres || A.f.ssat) {
-    if (iso) hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, MODE, true>), grid, block, 0, st, A);
-    else hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, MODE, false>), grid, block, 0, st, A);
+  if (A.f.tb.t) {
+    // kr from a lookup table: the general (anisotropic) form
+    if (A.f.sres || A.f.ssat)
+      hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, MODE, false, true>), grid, block, 0, st, A);
+    else
+      hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, MODE, false, true>), grid, block, 0, st, A);
+  } else if (A.f.sres || A.f.ssat) {
+    if (iso) hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, MODE, true, false>), grid, block, 0, st, A);
+    else hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, true, MODE, false, false>), grid, block, 0, st, A);
   } else {
-    if (iso) hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, MODE, true>), grid, block, 0, st, A);
-    else hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, MODE, false>), grid, block, 0, st, A);
+    if (iso) hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, MODE, true, false>), grid, block, 0, st, A);
+    else hipLaunchKernelGGL((k_residual_pf<TX, TY, ZC, false, MODE, false, false>), grid, block, 0, st, A);
   }
   PROF_END(pid, st);
   long ns = shell_count(g);
@@ -2509,17 +2643,24 @@ struct JacPFArgs {
   double idx, idy, idz, imu;
 };
 
+template <bool TAB = true>
 __device__ __forceinline__ void vg_all(double p, double irg, double alpha, double n,
                                        double sres, double ssat, double &S, double &dS,
-                                       double &kr, double &dkr)
+                                       double &kr, double &dkr, const KrTab &tb)
 {
   double s_dif = ssat - sres;
   if (p >= 0.0) { S = s_dif + sres; dS = 0.0; kr = 1.0; dkr = 0.0; return; }
   VGTerms v = vg_terms(p, irg, alpha, n);
   S = s_dif * v.a + sres;
   dS = (v.m * n * alpha * v.ahnm1) * s_dif * pf_div(v.a, 1.0 + v.t);
-  kr = vg_kr_t(v);
-  dkr = vg_dkr_t(v, alpha, n);
+  if (TAB && tb.t) {
+    const double head = fabs(p) * irg;
+    kr = kr_table(tb, head, false);
+    dkr = kr_table(tb, head, true);
+  } else {
+    kr = vg_kr_t(v);
+    dkr = vg_dkr_t(v, alpha, n);
+  }
 }
 
 template <bool PCS> struct JVG { double p, al, n, kz, sr, st; };
@@ -2576,7 +2717,7 @@ __device__ __forceinline__ void jface_z(double pa, double pb, double ka, double 
   T2 = zc * (diff * upw(lc, uc, 0.0, dkb)) + sym_u;
 }
 
-template <int TX, int TY, int ZC, bool PCS, bool SYMM, typename OT>
+template <int TX, int TY, int ZC, bool PCS, bool SYMM, typename OT, bool TAB>
 // four waves per SIMD: the register budget of 128 VGPRs (without the hint
 // the compiler took 131 and three waves; 401 -> 328 us on 16.8M cells)
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4)))
@@ -2636,7 +2777,7 @@ void k_jacobian_pf(JacPFArgs P, JOut<OT> J)
     JVG<PCS> v0;
     jload_vg<PCS>(v0, A, col + plane(k0));
     p_c = v0.p;
-    vg_all(p_c, irg, v0.al, v0.n, jsr<PCS>(v0, f), jst<PCS>(v0, f), S_c, dS_c, kr_c, dk_c);
+    vg_all<TAB>(p_c, irg, v0.al, v0.n, jsr<PCS>(v0, f), jst<PCS>(v0, f), S_c, dS_c, kr_c, dk_c, f.tb);
     kr_c *= rho;
     dk_c = dk_c * rho + kr_c * 0.0;
     kz_c = v0.kz;
@@ -2644,7 +2785,7 @@ void k_jacobian_pf(JacPFArgs P, JOut<OT> J)
       JVG<PCS> vm;
       jload_vg<PCS>(vm, A, col + plane(k0 - 1));
       double Sm, dSm, krm, dkm;
-      vg_all(vm.p, irg, vm.al, vm.n, jsr<PCS>(vm, f), jst<PCS>(vm, f), Sm, dSm, krm, dkm);
+      vg_all<TAB>(vm.p, irg, vm.al, vm.n, jsr<PCS>(vm, f), jst<PCS>(vm, f), Sm, dSm, krm, dkm, f.tb);
       krm *= rho;
       dkm = dkm * rho + krm * 0.0;
       double symu;
@@ -2677,15 +2818,15 @@ void k_jacobian_pf(JacPFArgs P, JOut<OT> J)
     // ---- A: closures one plane ahead (own) and of the halo cell
     if (k + 1 < nz) {
       p_n = vgU.p;
-      vg_all(p_n, irg, vgU.al, vgU.n, jsr<PCS>(vgU, f), jst<PCS>(vgU, f), S_n, dS_n, kr_n,
-             dk_n);
+      vg_all<TAB>(p_n, irg, vgU.al, vgU.n, jsr<PCS>(vgU, f), jst<PCS>(vgU, f), S_n, dS_n, kr_n,
+             dk_n, f.tb);
       kr_n *= rho;
       dk_n = dk_n * rho + kr_n * 0.0;
       kz_n = vgU.kz;
     }
     if (hact) {
       double Sh, dSh, krh, dkh;
-      vg_all(h.p, irg, h.al, h.n, f.sres_v, f.ssat_v, Sh, dSh, krh, dkh);
+      vg_all<TAB>(h.p, irg, h.al, h.n, f.sres_v, f.ssat_v, Sh, dSh, krh, dkh, f.tb);
       krh *= rho;
       s_p[hls] = h.p;
       s_kr[hls] = krh;
@@ -2774,7 +2915,7 @@ __device__ __forceinline__ double jac_dir_faces(const JacArgs &A, int i, int j, 
   const double pc = A.p[ip];
   const double a = A.f.alpha[ip], nn = A.f.nvg[ip];
   double S, dS, kr, dkr;
-  vg_all(pc, A.f.irg, a, nn, A.f.Sres(ip), A.f.Ssat(ip), S, dS, kr, dkr);
+  vg_all(pc, A.f.irg, a, nn, A.f.Sres(ip), A.f.Ssat(ip), S, dS, kr, dkr, A.f.tb);
   const double krc = kr * rho, dkc = dkr * rho + krc * 0.0;
 #pragma unroll 1
   for (int fc = 0; fc < 6; fc++) {
@@ -2786,7 +2927,7 @@ __device__ __forceinline__ double jac_dir_faces(const JacArgs &A, int i, int j, 
     double value = A.bcv[fc][fi];
     double den_d = rho;
     double prod = krc, prod_der = dkc;
-    double prod_val = vg_kr(value, A.f.irg, a, nn) * den_d;
+    double prod_val = vg_kr(value, A.f.irg, a, nn, A.f.tb) * den_d;
     double coeff, diff, o_temp;
     if (fc == 0 || fc == 2) {
       double ff = (fc == 0) ? ffx : ffy, dd = (fc == 0) ? dx : dy;
@@ -2883,10 +3024,14 @@ static void launch_jacobian_pf(const JacPFArgs &P, const JOut<OT> &J, hipStream_
   dim3 grid((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC);
   dim3 block(TX, TY, 1);
   PROF_BEGIN(PFHIP_PROF_JACOBIAN, st);
-  if (P.a.f.sres || P.a.f.ssat)
-    hipLaunchKernelGGL((k_jacobian_pf<TX, TY, ZC, true, SYMM, OT>), grid, block, 0, st, P, J);
-  else
-    hipLaunchKernelGGL((k_jacobian_pf<TX, TY, ZC, false, SYMM, OT>), grid, block, 0, st, P, J);
+  const bool tab = P.a.f.tb.t != nullptr;
+  if (P.a.f.sres || P.a.f.ssat) {
+    if (tab) hipLaunchKernelGGL((k_jacobian_pf<TX, TY, ZC, true, SYMM, OT, true>), grid, block, 0, st, P, J);
+    else hipLaunchKernelGGL((k_jacobian_pf<TX, TY, ZC, true, SYMM, OT, false>), grid, block, 0, st, P, J);
+  } else {
+    if (tab) hipLaunchKernelGGL((k_jacobian_pf<TX, TY, ZC, false, SYMM, OT, true>), grid, block, 0, st, P, J);
+    else hipLaunchKernelGGL((k_jacobian_pf<TX, TY, ZC, false, SYMM, OT, false>), grid, block, 0, st, P, J);
+  }
   PROF_END(PFHIP_PROF_JACOBIAN, st);
   long ns = shell_count(g);
   hipLaunchKernelGGL((k_jacobian_bc<SYMM, OT>), dim3((unsigned)((ns + 255) / 256)), dim3(256),
@@ -3299,7 +3444,7 @@ __global__ __launch_bounds__(ERB) void k_ens_res_closure(const EnsRes *__restric
   const int ip = cell_ip(A.g, c, i, j, k);
   const double p = (E.mode != 0) ? A.p[ip] + A.psig * A.pv[ip] : A.p[ip];
   double S, kr;
-  vg_s_kr(p, f.irg, f.alpha[ip], f.nvg[ip], f.Sres(ip), f.Ssat(ip), S, kr);
+  vg_s_kr(p, f.irg, f.alpha[ip], f.nvg[ip], f.Sres(ip), f.Ssat(ip), S, kr, f.tb);
   E.pe[ip] = p;
   E.kr[ip] = kr * f.rho;
   E.sat[ip] = S;
@@ -3348,7 +3493,7 @@ __global__ __launch_bounds__(ERB) void k_ens_res_faces(const EnsRes *__restrict_
   const bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
                    (k == g.nz - 1);
   if (bnd) {
-    const double krb = vg_kr(pc, f.irg, f.alpha[ip], f.nvg[ip]) * f.rho;
+    const double krb = vg_kr(pc, f.irg, f.alpha[ip], f.nvg[ip], f.tb) * f.rho;
     fv = residual_bc(A, i, j, k, ip, fv, pc, krb, kxc, kyc, kzc);
   }
   if (E.mode == 1) fv = A.fsc * fv - A.fsc * A.fsub[ip];
@@ -3411,7 +3556,7 @@ __global__ __launch_bounds__(ERB) void k_ens_jac_closure(const EnsJac *__restric
   int i, j, k;
   const int ip = cell_ip(A.g, c, i, j, k);
   double S, dS, kr, dk;
-  vg_all(A.p[ip], f.irg, f.alpha[ip], f.nvg[ip], f.Sres(ip), f.Ssat(ip), S, dS, kr, dk);
+  vg_all(A.p[ip], f.irg, f.alpha[ip], f.nvg[ip], f.Sres(ip), f.Ssat(ip), S, dS, kr, dk, f.tb);
   kr *= f.rho;
   E.kr[ip] = kr;
   E.dk[ip] = dk * f.rho + kr * 0.0;

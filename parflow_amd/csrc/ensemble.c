@@ -351,6 +351,11 @@ int pfhip_ensemble_create(int n, pfhip_solver *const *solvers, pfhip_ensemble **
                       i, PFHIP_ENS_MAXQ - 2);
       return PFHIP_ERR_ARG;
     }
+    if (pb->krtab) {
+      pfhip_set_error("ensemble member %d: a van Genuchten lookup table is not supported in "
+                      "ensembles (members share one closure form per launch)", i);
+      return PFHIP_ERR_ARG;
+    }
   }
   pfhip_ensemble *e = (pfhip_ensemble *)calloc(1, sizeof(pfhip_ensemble));
   if (!e) { pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }

@@ -25,6 +25,7 @@ struct pfhip_problem {
   double *red_partial;        /* reduction partials (RED_MAX_BLOCKS*8)    */
   double *red_dev;            /* device result slots (PFHIP_RED_SLOTS)    */
   double *h_red;              /* pinned host result slots (64)            */
+  double *krtab;              /* device kr lookup table, or NULL          */
 };
 
 #define PFHIP_RED_MAX_BLOCKS 1024

@@ -13,7 +13,9 @@ consumes to ``(Problem, SolverOptions, p0, dt, nsteps)`` for a box domain:
   ``Patches``; ``Domain.GeomName``;
 * ``Perm`` (Constant or PFBFile per geometry, ``TensorByGeom`` factors),
   ``Porosity``, ``SpecificStorage``, van Genuchten ``Phase.Saturation`` /
-  ``Phase.RelPerm`` (region constants or ``VanGenuchten.File`` fields),
+  ``Phase.RelPerm`` (region constants or ``VanGenuchten.File`` fields; the
+  region kr lookup tables ``NumSamplePoints`` / ``MinPressureHead`` /
+  ``InterpolationMethod``),
   constant density, viscosity and gravity;
 * ``TimingInfo`` with a constant ``TimeStep``; ``Cycle.*``;
   ``BCPressure`` FluxConst, DirEquilRefPatch and OverlandKinematic;
@@ -345,6 +347,23 @@ def problem_from_pfidb(db, base_dir=".", name=None):
     if not (np.array_equal(ra, alpha) and np.array_equal(rn, nvg)):
         raise ValueError("pfidb: saturation and relative permeability need the same "
                          "van Genuchten alpha and n")
+    # kr lookup tables (problem_phase_rel_perm.c:1894-1915): every RelPerm
+    # region must ask for the same table, from a uniform alpha and n
+    vg_table = None
+    if not rfile:
+        specs = set()
+        for g in K.names("Phase.RelPerm.GeomNames"):
+            pts = K.i(f"Geom.{g}.RelPerm.NumSamplePoints", 0)
+            if pts:
+                specs.add((pts, K.f(f"Geom.{g}.RelPerm.MinPressureHead"),
+                           K.s(f"Geom.{g}.RelPerm.InterpolationMethod", "Spline")))
+            else:
+                specs.add(None)
+        if specs != {None}:
+            if len(specs) != 1:
+                raise ValueError("pfidb: RelPerm lookup tables must be the same in every "
+                                 "region (and present in all or none)")
+            vg_table = specs.pop()
 
     rho = K.f("Phase.water.Density.Value", 1.0)
     mu = K.f("Phase.water.Viscosity.Value", 1.0)
@@ -402,7 +421,7 @@ def problem_from_pfidb(db, base_dir=".", name=None):
                    slope_x=slope_x, slope_y=slope_y,
                    mann=_unique(mann) if overland else 1.0,
                    ov_eps=K.f("Solver.OverlandKinematic.Epsilon", 1e-5), bc=bc,
-                   base_unit=base_unit, start_time=t0,
+                   base_unit=base_unit, start_time=t0, vg_table=vg_table,
                    name=name or "pfidb")
 
     # initial pressure

@@ -198,12 +198,27 @@ class Problem:
     base_unit: float = 1.0
     start_time: float = 0.0
     name: str = "problem"
+    # van Genuchten kr lookup table (Geom.*.RelPerm.NumSamplePoints,
+    # .MinPressureHead, .InterpolationMethod; problem_phase_rel_perm.c
+    # :1894-1915): (points, min_pressure_head, "Spline" | "Linear"), or None
+    # for the closed form.  The table is built from alpha and n, which must
+    # then be uniform.
+    vg_table: Optional[tuple] = None
 
     def __post_init__(self):
         if self.ky is None:
             self.ky = self.kx
         if self.kz is None:
             self.kz = self.kx
+        if self.vg_table is not None:
+            pts, head, method = self.vg_table
+            if int(pts) < 2 or not float(head) < 0.0 or method not in ("Spline", "Linear"):
+                raise ValueError(f"vg_table {self.vg_table!r}: need points >= 2, "
+                                 "min pressure head < 0, Spline or Linear")
+            for name in ("alpha", "nvg"):
+                a = np.asarray(getattr(self, name), dtype=np.float64)
+                if a.ndim and not np.all(a == a.flat[0]):
+                    raise ValueError(f"vg_table needs a uniform {name}")
         if len(self.bc) != 6:
             raise ValueError("six face boundary conditions required")
         for f, b in enumerate(self.bc):
@@ -234,6 +249,16 @@ class Problem:
     @property
     def shape(self):
         return (self.nz + 2, self.ny + 2, self.nx + 2)
+
+    @property
+    def vg_table_params(self):
+        """(method 0 spline / 1 linear, points, min head, alpha, n) or None."""
+        if self.vg_table is None:
+            return None
+        pts, head, method = self.vg_table
+        alpha = float(np.asarray(self.alpha, dtype=np.float64).flat[0])
+        n = float(np.asarray(self.nvg, dtype=np.float64).flat[0])
+        return (0 if method == "Spline" else 1, int(pts), float(head), alpha, n)
 
     @property
     def ncells(self):

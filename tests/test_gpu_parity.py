@@ -112,6 +112,49 @@ def test_face_velocity_golden_default_richards(gpu):
             assert np.max(np.abs(a - b)) < 1e-13 * scale, s
 
 
+def _table_case(method, overland=False, dirs=True, shape=(11, 9, 6)):
+    import dataclasses
+    base = cases.hetero(*shape, seed=3, overland=overland, dir_sides=dirs)
+    return dataclasses.replace(base, alpha=1.3, nvg=2.3, vg_table=(40, -6.0, method))
+
+
+@pytest.mark.parametrize("method", ["Spline", "Linear"])
+def test_vg_table_closures_residual_jacobian(gpu, method):
+    """A problem with a van Genuchten kr table (region-uniform alpha, n):
+    relperm and its derivative, the residual (Dirichlet sides, and the
+    tiled kernel across several tiles) and the symmetric and full Jacobians
+    against the oracle, which takes kr from its own table."""
+    api = _api()
+    for overland, shape in ((False, (11, 9, 6)), (True, (70, 13, 5))):
+        prob = _table_case(method, overland, shape=shape)
+        print(f"vg table {method} overland={overland} shape={shape}", flush=True)
+        gp = api.GPUProblem(prob)
+        op = O.OracleProblem(prob)
+        p_old = cases.random_pressure(prob, seed=4)
+        p = p_old + np.random.default_rng(5).normal(0, 0.01, p_old.shape)
+        for fcn in (0, 1):
+            a, b = gp.phase_rel_perm(p, fcn), O.relperm(op, p, fcn)
+            assert _relmax(_I(prob, a), _I(prob, b)) < 1e-13, fcn
+        s_old = O.saturation(op, p_old)
+        Fg = gp.nl_function_eval(p, p_old, s_old, 0.1, 0.35)
+        Fo = O.nl_function_eval(op, p, p_old, s_old, 0.1, 0.35)
+        scale = np.max(np.abs(_I(prob, Fo))) + 1e-30
+        assert _relmax(_I(prob, Fg), _I(prob, Fo), scale) < 1e-12
+        for symm in (0, 1):
+            Jg = gp.richards_jacobian_eval(p, 0.1, 0.35, symm)
+            Jo = O.jacobian_eval(op, p, 0.1, 0.35, symm)
+            for q in ((0, 2, 4, 6) if symm else range(7)):
+                sc = np.max(np.abs(_I(prob, Jo[q]))) + 1e-30
+                assert _relmax(_I(prob, Jg[q]), _I(prob, Jo[q]), sc) < 1e-12, (symm, q)
+
+
+def test_vg_table_newton_matches_oracle(gpu):
+    """The plain Newton solve of a problem with a kr table against the oracle
+    (per-cell 1e-9, Newton counts +-1)."""
+    prob = _table_case("Spline")
+    _compare_runs(prob, cases.solver(), cases.random_pressure(prob, seed=9), 0.1, 3)
+
+
 @pytest.mark.parametrize("symm", [0, 1])
 @pytest.mark.parametrize("overland", [False, True])
 def test_jacobian_and_matvec(gpu, symm, overland):

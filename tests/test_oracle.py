@@ -391,3 +391,97 @@ def test_storage_closed_form():
     sub2, sur2 = O.storage(op, np.full(prob.shape, -2.0), np.full(prob.shape, 0.3))
     assert abs(sub2 - 0.4 * vol * 0.3 * n_all) < 1e-12 * sub2
     assert sur2 == 0.0
+
+
+# --- van Genuchten lookup tables ------------------------------------------------
+def _kr_closed(h, alpha, n):
+    """van Genuchten-Mualem kr(h), h >= 0 the pressure head magnitude."""
+    m = 1.0 - 1.0 / n
+    t = (alpha * h) ** n
+    return (1.0 - (alpha * h) ** (n - 1) * (1.0 + t) ** (-m)) ** 2 * (1.0 + t) ** (-m / 2)
+
+
+@pytest.mark.parametrize("n", [2.0, 1.6, 3.5])
+def test_vg_table_nodes_and_accuracy(n):
+    """PL/problem_phase_rel_perm.c:116-354: at its nodes the spline table
+    returns kr exactly as the closed form gives it; between them the
+    monotone cubic Hermite interpolant converges (error ratio >= 3.5 when the
+    spacing halves: the averaged-secant slopes make it second order), stays
+    monotone (Fritsch-Carlson),
+    and the linear table interpolates the nodes; 0 beyond |MinPressureHead|."""
+    alpha, head_min = 1.5, -10.0
+    errs = []
+    for pts in (200, 400):
+        tab = O.vg_table_build(0, pts, head_min, alpha, n)
+        x = tab[:pts + 1]
+        h = abs(head_min) / (pts - 1)
+        assert np.allclose(x, np.arange(pts + 1) * h, rtol=0, atol=1e-12)
+        for i in (0, 1, 7, pts // 2, pts - 2):
+            got = O.vg_table_lookup(tab, 0, pts, head_min, x[i], 0)
+            assert abs(got - _kr_closed(x[i], alpha, n)) <= 1e-14 * max(1.0, got), (i, got)
+        fine = np.linspace(0.0, 9.999, 4001)
+        vals = np.array([O.vg_table_lookup(tab, 0, pts, head_min, v, 0) for v in fine])
+        assert np.all(np.diff(vals) <= 1e-15)             # monotone non-increasing
+        errs.append(np.max(np.abs(vals - _kr_closed(fine, alpha, n))))
+        assert O.vg_table_lookup(tab, 0, pts, head_min, 10.0, 0) == 0.0
+        assert O.vg_table_lookup(tab, 0, pts, head_min, 12.5, 1) == 0.0
+        lin = O.vg_table_build(1, pts, head_min, alpha, n)
+        for i in (3, pts // 3):
+            xa, xb = lin[i], lin[i + 1]
+            ka, kb = _kr_closed(xa, alpha, n), _kr_closed(xb, alpha, n)
+            assert abs(O.vg_table_lookup(lin, 1, pts, head_min, xa, 0) - ka) <= 1e-14
+            mid = O.vg_table_lookup(lin, 1, pts, head_min, 0.5 * (xa + xb), 0)
+            assert abs(mid - 0.5 * (ka + kb)) <= 1e-14
+    if n >= 2.0:
+        assert errs[0] / errs[1] >= 3.5, errs
+    else:
+        # 1 < n < 2: kr has unbounded curvature at h = 0, so only the
+        # interpolation error shrinks, not at a fixed order
+        assert errs[1] < errs[0], errs
+
+
+def test_vg_table_derivative_values():
+    """The derivative table holds the reference's CALCDER formula: it equals
+    -dkr/dh of the closed form (central differences) at interior nodes, and
+    is set to 0 at head 0 when n < 2 (PL/problem_phase_rel_perm.c:181-184)."""
+    alpha, n, pts, head_min = 0.8, 2.4, 300, -20.0
+    tab = O.vg_table_build(0, pts, head_min, alpha, n)
+    x, ader = tab[:pts + 1], tab[3 * (pts + 1):4 * (pts + 1)]
+    for i in (5, 60, 150, 290):
+        e = 1e-6
+        fd = -(_kr_closed(x[i] + e, alpha, n) - _kr_closed(x[i] - e, alpha, n)) / (2 * e)
+        assert abs(ader[i] - fd) <= 1e-6 * max(abs(fd), 1e-3), (i, ader[i], fd)
+        assert O.vg_table_lookup(tab, 0, pts, head_min, x[i], 1) == ader[i]
+    tab15 = O.vg_table_build(0, pts, head_min, alpha, 1.5)
+    assert tab15[3 * (pts + 1)] == 0.0
+
+
+def test_vg_table_in_the_residual():
+    """A problem with a kr table: relperm and the residual take kr from the
+    table (spline and linear differ from the closed form between nodes)."""
+    import dataclasses
+    base = cases.hetero(6, 5, 4, seed=71)
+    kw = dict(alpha=1.2, nvg=2.2)
+    p0 = dataclasses.replace(base, **kw)
+    op0 = O.OracleProblem(p0)
+    p = cases.random_pressure(p0, seed=72)
+    kr0 = O.relperm(op0, p)
+    for method in ("Spline", "Linear"):
+        pt = dataclasses.replace(base, vg_table=(12, -8.0, method), **kw)
+        opt = O.OracleProblem(pt)
+        krt = O.relperm(opt, p)
+        I = pt.interior
+        wet = I(p) >= 0.0
+        assert np.all(I(krt)[wet] == 1.0)
+        d = np.abs(I(krt) - I(kr0))[~wet]
+        assert 0.0 < d.max() < 0.2, (method, d.max())   # a coarse 12-point table
+        # every cell's kr is the table's value at its head
+        heads = np.abs(I(p)[~wet]) / (pt.rho * pt.gravity)
+        tab = opt.vg_table
+        ref = [O.vg_table_lookup(tab, 0 if method == "Spline" else 1, 12, -8.0, h, 0)
+               for h in heads]
+        assert np.array_equal(I(krt)[~wet], np.array(ref))
+        s_old = O.saturation(opt, p)
+        F0 = O.nl_function_eval(op0, p, p, s_old, 0.1, 0.1)
+        Ft = O.nl_function_eval(opt, p, p, s_old, 0.1, 0.1)
+        assert np.max(np.abs(I(F0) - I(Ft))) > 0.0

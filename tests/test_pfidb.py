@@ -267,3 +267,26 @@ def test_time_control_keys():
     keys["TimeStep.Type"] = "Adaptive"
     with pytest.raises(ValueError):
         pfidb.time_control_from_pfidb(keys)
+
+
+def test_vg_table_keys():
+    """Geom.<region>.RelPerm.NumSamplePoints / MinPressureHead /
+    InterpolationMethod (problem_phase_rel_perm.c:1894-1915) become the
+    problem's kr table; regions that disagree raise."""
+    keys = {k: pfidb._str(v) for k, v in default_richards_keys().items()}
+    prob = pfidb.problem_from_pfidb(keys)[0]
+    assert prob.vg_table is None
+    regions = keys["Phase.RelPerm.GeomNames"].split()
+    for g in regions:
+        keys[f"Geom.{g}.RelPerm.NumSamplePoints"] = "2000"
+        keys[f"Geom.{g}.RelPerm.MinPressureHead"] = "-300"
+    prob = pfidb.problem_from_pfidb(keys)[0]
+    assert prob.vg_table == (2000, -300.0, "Spline")
+    assert prob.vg_table_params[:3] == (0, 2000, -300.0)
+    for g in regions:
+        keys[f"Geom.{g}.RelPerm.InterpolationMethod"] = "Linear"
+    assert pfidb.problem_from_pfidb(keys)[0].vg_table == (2000, -300.0, "Linear")
+    if len(regions) > 1:
+        keys[f"Geom.{regions[0]}.RelPerm.NumSamplePoints"] = "100"
+        with pytest.raises(ValueError):
+            pfidb.problem_from_pfidb(keys)
