@@ -634,6 +634,26 @@ def test_time_loop_surface_predictor_matches_oracle(gpu, pv):
         assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-9, k
 
 
+def test_tiltedv_jacobian_true_on_gpu(gpu):
+    """overland_tiltedV_KWE.py's TiltedV_OverlandKin "Jacobian True" variant
+    (analytic J.v with the kinematic surface couplings, PFSymmetric PC) on
+    the plain device path: against the oracle per step and against the
+    reference's correct_output."""
+    import dataclasses
+    prob, opts, p0, dt, n = ref_decks.tiltedv_kin()
+    opts = dataclasses.replace(opts, use_jacobian=True)
+    print("tiltedv UseJacobian True", flush=True)
+    worst, lg, lo = _compare_runs(prob, opts, p0, dt, n)
+    api = _api()
+    gp = api.GPUProblem(prob)
+    ph, sh, log = api.GPUSolver(gp, opts).advance(p0, 0.0, dt, n)
+    g = np.load(os.path.join(HERE, "golden", "tiltedv_overlandkin.npz"))
+    for d in range(21):
+        s = max(2 * d - 1, 0)
+        a, b = prob.interior(ph[s]), g["press"][d]
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-7, d
+
+
 def test_kinsol_plain_matches_oracle_default_richards(gpu):
     """Reference deck default_richards (UseJacobian True) on the plain path."""
     prob, opts, p0, dt, n = ref_decks.default_richards()

@@ -61,6 +61,23 @@ def test_tiltedv_overlandkin_golden():
         assert _rel(prob.interior(sh[s]), g["satur"][d]) < 1e-12, d
 
 
+def test_tiltedv_overlandkin_jacobian_true_golden():
+    """The same deck's "Jacobian True" variant (overland_tiltedV_KWE.py
+    :511-545: UseJacobian True, PFSymmetric preconditioner), checked against
+    the same correct_output: the analytic J.v includes the kinematic surface
+    couplings."""
+    import dataclasses
+    prob, opts, p0, dt, n = ref_decks.tiltedv_kin()
+    opts = dataclasses.replace(opts, use_jacobian=True)
+    op = O.OracleProblem(prob)
+    r, ph, sh, log = O.advance(op, opts, p0, 0.0, dt, n)
+    assert r == 0
+    g = np.load(os.path.join(HERE, "golden", "tiltedv_overlandkin.npz"))
+    for d in range(21):
+        s = max(2 * d - 1, 0)
+        assert _rel(prob.interior(ph[s]), g["press"][d], 1e-3) < 1e-7, d
+
+
 # --- closures --------------------------------------------------------------------
 def test_vg_closed_forms():
     prob = cases.hetero(4, 3, 3, seed=3)
