@@ -81,7 +81,8 @@ PROF_KERNELS = (
     ("residual_step", "PROF_RESIDUAL_STEP",
      "k_residual_pf<64, 8, 16, *, 2> (line-search trial point)"),
     ("jacobian", "PROF_JACOBIAN", "k_jacobian_pf<64, 8, 16, *, true, float> (PC matrix)"),
-    ("rbgs_pair", "PROF_RBGS", "k_rbgs_pair<float, 14, 16, *> (finest-level red/black pair)"),
+    ("rbgs_pair", "PROF_RBGS",
+     "k_rbgs_pair2<float, TY, KC, *> (finest-level red/black pair, autotuned tile)"),
 )
 
 
