@@ -203,10 +203,19 @@ int  pfhip_water_balance_storage(pfhip_problem *pb, const double *p,
  * (PL/rb_GS_point.c:56).  precision = 64 (FP64) or 32 (FP32 levels). */
 int  pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int post,
                      int symmetric, int precision, pfhip_mg **out);
+/* The same hierarchy for PCMatrixType FullJacobian (PL/kinsol_pc.c:215-240,
+ * the non-symmetric hypre matrix of PL/pf_hypre.c:300-350 and 440-520):
+ * every level keeps all seven couplings (C, W, E, S, N, L, U).  Coarse
+ * operators are P^T A P along the coarsening direction with each side of
+ * a row collapsed from its own couplings; on a symmetric matrix they equal
+ * pfhip_mg_create's. */
+int  pfhip_mg_create_full(pfhip_problem *pb, int max_levels, int pre, int post,
+                          int symmetric, int precision, pfhip_mg **out);
 int  pfhip_mg_destroy(pfhip_mg *mg);
 int  pfhip_mg_num_levels(pfhip_mg *mg, int *levels);
 /* PrecondInitInstanceXtraInvoke (PL/parflow_proto.h:419): coarse levels
- * from the symmetric PC matrix (C, E, N, U arrays). */
+ * from the PC matrix: A[7] in stencil order C, W, E, S, N, L, U; a
+ * symmetric hierarchy reads C, E, N, U, a full one all seven. */
 int  pfhip_mg_setup(pfhip_mg *mg, const double *const *Asym, void *stream);
 /* One V-cycle, zero initial guess: x = M^{-1} b. */
 int  pfhip_mg_apply(pfhip_mg *mg, const double *b, double *x, void *stream);
@@ -227,6 +236,9 @@ typedef struct {
   int krylov;                        /* 0 SPGMR + modified Gram-Schmidt (the
                                         reference's), 1 flexible GMRES with
                                         block classical Gram-Schmidt (CGS2)  */
+  int pc_matrix;                     /* Solver.Linear.Preconditioner.
+                                        PCMatrixType: 0 PFSymmetric,
+                                        1 FullJacobian                       */
 } pfhip_solver_opts;
 
 typedef struct {

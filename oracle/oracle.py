@@ -52,7 +52,8 @@ class OptsC(C.Structure):
                 ("eta_gamma", C.c_double), ("use_jacobian", C.c_int),
                 ("deriv_eps", C.c_double), ("globalization", C.c_int),
                 ("precond", C.c_int), ("mg_pre", C.c_int), ("mg_post", C.c_int),
-                ("mg_symmetric", C.c_int), ("mg_max_levels", C.c_int)]
+                ("mg_symmetric", C.c_int), ("mg_max_levels", C.c_int),
+                ("pc_matrix", C.c_int)]
 
 
 class KinStatsC(C.Structure):
@@ -88,6 +89,9 @@ def lib():
         _lib.pfo_mg_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double,
                                        C.c_double, C.c_double, C.c_int, C.c_int,
                                        C.c_int, C.c_int]
+        _lib.pfo_mg_create_full.restype = C.c_void_p
+        _lib.pfo_mg_create_full.argtypes = _lib.pfo_mg_create.argtypes + [C.c_int]
+        _lib.pfo_mg_level_matrix.argtypes = [C.c_void_p, C.c_int, C.POINTER(_dp)]
         _lib.pfo_mg_setup.argtypes = [C.c_void_p, C.POINTER(_dp)]
         _lib.pfo_mg_apply.argtypes = [C.c_void_p, _dp, _dp]
         _lib.pfo_mg_destroy.argtypes = [C.c_void_p]
@@ -189,6 +193,7 @@ def opts_c(o):
     c.precond = 0 if o.precond == "NoPC" else 1
     c.mg_pre, c.mg_post, c.mg_symmetric = pre, post, sym
     c.mg_max_levels = o.max_levels
+    c.pc_matrix = o.pc_matrix_code()
     return c
 
 
@@ -260,11 +265,24 @@ def overland_kin(op, p):
 
 
 class MG:
-    def __init__(self, prob, pre, post, symmetric, max_levels=100):
-        self.h = lib().pfo_mg_create(prob.nx, prob.ny, prob.nz, prob.dx,
-                                     prob.dy, prob.dz, max_levels, pre, post,
-                                     symmetric)
+    """pfo_mg: full=True keeps all seven couplings per row (the FullJacobian
+    PC matrix); setup() then reads J[1], J[3], J[5] (W, S, L) as well."""
+
+    def __init__(self, prob, pre, post, symmetric, max_levels=100, full=False):
+        self.h = lib().pfo_mg_create_full(prob.nx, prob.ny, prob.nz, prob.dx,
+                                          prob.dy, prob.dz, max_levels, pre, post,
+                                          symmetric, int(full))
         self.shape = prob.shape
+        self.full = full
+
+    def level_matrix(self, l):
+        """(C, E, N, U[, W, S, L]) ghosted arrays of level l."""
+        dims, _ = self.levels()[l]
+        shp = (dims[2] + 2, dims[1] + 2, dims[0] + 2)
+        out = [np.zeros(shp) for _ in range(7 if self.full else 4)]
+        arr = (_dp * 7)(*([_ptr(a) for a in out] + [None] * (7 - len(out))))
+        lib().pfo_mg_level_matrix(self.h, l, arr)
+        return out
 
     def levels(self):
         out = []

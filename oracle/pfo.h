@@ -86,6 +86,7 @@ typedef struct {
   int mg_pre, mg_post;                  /* smoother iterations per level    */
   int mg_symmetric;                     /* R-B-R (1) or R-B (0) sweeps      */
   int mg_max_levels;
+  int pc_matrix;                        /* 0 PFSymmetric, 1 FullJacobian    */
 } pfo_solver_opts;
 
 typedef struct {
@@ -131,6 +132,9 @@ void pfo_matvec(const pfo_problem *pb, double **J, int symm, const double *x,
 typedef struct pfo_mg pfo_mg;
 pfo_mg *pfo_mg_create(int nx, int ny, int nz, double dx, double dy, double dz,
                       int max_levels, int pre, int post, int symmetric);
+pfo_mg *pfo_mg_create_full(int nx, int ny, int nz, double dx, double dy, double dz,
+                           int max_levels, int pre, int post, int symmetric, int full);
+void    pfo_mg_level_matrix(const pfo_mg *mg, int l, double **out);
 void    pfo_mg_destroy(pfo_mg *mg);
 int     pfo_mg_num_levels(const pfo_mg *mg);
 void    pfo_mg_level_dims(const pfo_mg *mg, int l, int *dims, int *dir);

@@ -97,7 +97,7 @@ static void func(kin_t *K, const double *u, double *f)
 /* KINSolInitPC: PC matrix at uu + preconditioner setup */
 static void pc_setup(kin_t *K)
 {
-  pfo_jacobian_eval(K->pb, K->uu, K->dt, 1, K->P);
+  pfo_jacobian_eval(K->pb, K->uu, K->dt, !K->o->pc_matrix, K->P);
   pfo_mg_setup(K->mg, K->P);
   K->npe++;
   K->nnilpre = K->nni;
@@ -600,9 +600,9 @@ int pfo_kinsol(pfo_problem *pb, const pfo_solver_opts *opts, double *p,
     K.P[s] = (double*)calloc(ng, sizeof(double));
   }
   if (opts->precond)
-    K.mg = pfo_mg_create(pb->nx, pb->ny, pb->nz, pb->dx, pb->dy, pb->dz,
-                         opts->mg_max_levels, opts->mg_pre, opts->mg_post,
-                         opts->mg_symmetric);
+    K.mg = pfo_mg_create_full(pb->nx, pb->ny, pb->nz, pb->dx, pb->dy, pb->dz,
+                              opts->mg_max_levels, opts->mg_pre, opts->mg_post,
+                              opts->mg_symmetric, opts->pc_matrix);
 
   /* KINSolInit */
   K.uround = unit_roundoff();

@@ -10,8 +10,22 @@
  *                      (PL/comm_pkg.c:325-362)
  *
  * Matrices are symmetric-stored (C, E, N, U; the lower couplings are the
- * neighbour's upper ones, PL/matrix.c:258-290).  All arrays are ghosted
- * (n_x+2)(n_y+2)(n_z+2) and their ghost entries stay zero.
+ * neighbour's upper ones, PL/matrix.c:258-290) or, for the FullJacobian PC
+ * matrix (PL/kinsol_pc.c:215-240, PL/pf_hypre.c:300-350), full-stored
+ * (C, E, N, U, W, S, L: every row keeps its own lower couplings).  All
+ * arrays are ghosted (n_x+2)(n_y+2)(n_z+2) and their ghost entries stay zero.
+ *
+ * Full storage keeps the same cycle: operator-induced prolongation from each
+ * odd point's own row, restriction by its transpose, and the coarse
+ * operator as P^T A P along the coarsening direction with the off-direction
+ * couplings lumped onto the diagonal (the collapse MGSemi uses), each side
+ * of a row collapsed from its own couplings:
+ *   Ac_U(C) = a_U(C) p2(C+1),  Ac_L(C) = a_L(C) p1(C-1),
+ *   Ac_C(C) = a_0 + sum(off) + a_L(C) p2(C-1) + a_U(C) p1(C+1) - sum(Ac off),
+ *   Ac_E(C) = a_E(C) + (a_E(C-1) + a_E(C+1)) / 2  (and W, N, S alike).
+ * On a symmetric matrix these reduce to the MGSemi formulas above.  The
+ * reference's own FullJacobian preconditioner is hypre PFMG (not vendored):
+ * this hierarchy is the build's replacement for it, as for PFSymmetric.
  *
  * Parity status: parity unpinned for the V-cycle output itself.  The
  * reference's default PFMG preconditioner lives in hypre (not vendored) and
@@ -33,18 +47,20 @@ struct pfo_mg {
   int L;
   int n[MAXL][3];
   int dir[MAXL];
-  double *A[MAXL][4];          /* C E N U */
+  double *A[MAXL][7];          /* C E N U, then W S L when full */
   double *x[MAXL], *b[MAXL], *t[MAXL];
   double *p1[MAXL], *p2[MAXL];
   int pre, post, symmetric;
+  int full;                    /* full (non-symmetric) storage */
 };
 
 static long lsize(const int *n) { return (long)(n[0] + 2) * (n[1] + 2) * (n[2] + 2); }
 
-pfo_mg *pfo_mg_create(int nx, int ny, int nz, double dx, double dy, double dz,
-                      int max_levels, int pre, int post, int symmetric)
+pfo_mg *pfo_mg_create_full(int nx, int ny, int nz, double dx, double dy, double dz,
+                           int max_levels, int pre, int post, int symmetric, int full)
 {
   pfo_mg *mg = (pfo_mg*)calloc(1, sizeof(pfo_mg));
+  mg->full = full;
   int N[3] = { nx, ny, nz };
   double D[3] = { dx, dy, dz };
   int min_N = 2;          /* pfmax(2*MaxMinNX-1, 2) with MaxMinNX = 1 */
@@ -74,7 +90,7 @@ pfo_mg *pfo_mg_create(int nx, int ny, int nz, double dx, double dy, double dz,
   for (l = 0; l < mg->L; l++)
   {
     long s = lsize(mg->n[l]);
-    for (int q = 0; q < 4; q++) mg->A[l][q] = (double*)calloc(s, sizeof(double));
+    for (int q = 0; q < (full ? 7 : 4); q++) mg->A[l][q] = (double*)calloc(s, sizeof(double));
     mg->x[l] = (double*)calloc(s, sizeof(double));
     mg->b[l] = (double*)calloc(s, sizeof(double));
     mg->t[l] = (double*)calloc(s, sizeof(double));
@@ -84,11 +100,17 @@ pfo_mg *pfo_mg_create(int nx, int ny, int nz, double dx, double dy, double dz,
   return mg;
 }
 
+pfo_mg *pfo_mg_create(int nx, int ny, int nz, double dx, double dy, double dz,
+                      int max_levels, int pre, int post, int symmetric)
+{
+  return pfo_mg_create_full(nx, ny, nz, dx, dy, dz, max_levels, pre, post, symmetric, 0);
+}
+
 void pfo_mg_destroy(pfo_mg *mg)
 {
   for (int l = 0; l < mg->L; l++)
   {
-    for (int q = 0; q < 4; q++) free(mg->A[l][q]);
+    for (int q = 0; q < 7; q++) free(mg->A[l][q]);
     free(mg->x[l]); free(mg->b[l]); free(mg->t[l]);
     free(mg->p1[l]); free(mg->p2[l]);
   }
@@ -114,11 +136,20 @@ static inline long lidx(const long *s, int i, int j, int k)
 }
 
 /* coupling from point ip in direction q (0,1,2) on side -1/+1; zero when
- * the neighbour lies outside the level (its ghost coefficient is zero) */
+ * the neighbour lies outside the level (its ghost coefficient is zero).
+ * A[4] is non-null exactly for full storage. */
 static inline double cpl(double **A, const long *s, long ip, int q, int side)
 {
-  double *up = A[1 + q];
-  return (side > 0) ? up[ip] : up[ip - s[q]];
+  if (side > 0) return A[1 + q][ip];
+  return A[4] ? A[4 + q][ip] : A[1 + q][ip - s[q]];
+}
+
+/* the level matrix (C, E, N, U[, W, S, L]) of level l, copied out */
+void pfo_mg_level_matrix(const pfo_mg *mg, int l, double **out)
+{
+  long s = lsize(mg->n[l]);
+  for (int q = 0; q < (mg->full ? 7 : 4); q++)
+    if (out[q]) memcpy(out[q], mg->A[l][q], sizeof(double) * s);
 }
 
 void pfo_mg_setup(pfo_mg *mg, double **Asym)
@@ -130,6 +161,12 @@ void pfo_mg_setup(pfo_mg *mg, double **Asym)
     memcpy(mg->A[0][1], Asym[2], sizeof(double) * sz0);
     memcpy(mg->A[0][2], Asym[4], sizeof(double) * sz0);
     memcpy(mg->A[0][3], Asym[6], sizeof(double) * sz0);
+    if (mg->full)
+    {
+      memcpy(mg->A[0][4], Asym[1], sizeof(double) * sz0);
+      memcpy(mg->A[0][5], Asym[3], sizeof(double) * sz0);
+      memcpy(mg->A[0][6], Asym[5], sizeof(double) * sz0);
+    }
   }
   for (int l = 0; l < mg->L - 1; l++)
   {
@@ -144,7 +181,7 @@ void pfo_mg_setup(pfo_mg *mg, double **Asym)
     long lf = lsize(nf), lc = lsize(nc);
     memset(p1, 0, sizeof(double) * lf);
     memset(p2, 0, sizeof(double) * lf);
-    for (int q = 0; q < 4; q++) memset(Ac[q], 0, sizeof(double) * lc);
+    for (int q = 0; q < (mg->full ? 7 : 4); q++) memset(Ac[q], 0, sizeof(double) * lc);
     /* off-direction order of the a3..a6 entries, PL/mg_semi.c:470-492 */
     int o1, o2;
     if (d == 0) { o1 = 1; o2 = 2; }
@@ -191,9 +228,18 @@ void pfo_mg_setup(pfo_mg *mg, double **Asym)
           double ac6 = A[1 + o2][iA] + 0.5 * A[1 + o2][iA1] + 0.5 * A[1 + o2][iA2];
           Ac[1 + o1][iAc] = ac4;
           Ac[1 + o2][iAc] = ac6;
-          /* coupling along d to the next coarse point */
           int cid[3] = { ic, jc, kc };
-          if (cid[d] + 1 < nc[d])
+          if (mg->full)
+          {
+            /* lower off-direction couplings (o1-, o2-) and both couplings
+             * along d, each from the row's own side */
+            Ac[4 + o1][iAc] = A[4 + o1][iA] + 0.5 * A[4 + o1][iA1] + 0.5 * A[4 + o1][iA2];
+            Ac[4 + o2][iAc] = A[4 + o2][iA] + 0.5 * A[4 + o2][iA1] + 0.5 * A[4 + o2][iA2];
+            Ac[1 + d][iAc] = (cid[d] + 1 < nc[d]) ? A[1 + d][iA] * p2[iA2] : 0.0;
+            Ac[4 + d][iAc] = (cid[d] > 0) ? A[4 + d][iA] * p1[iA1] : 0.0;
+          }
+          /* coupling along d to the next coarse point */
+          else if (cid[d] + 1 < nc[d])
           {
             long iAn = iA + 2 * sf[d];
             Ac[1 + d][iAc] = cpl(A, sf, iAn, d, -1) * p1[iA + sf[d]];
@@ -232,7 +278,10 @@ void pfo_mg_setup(pfo_mg *mg, double **Asym)
           long iAc = lidx(sc, ic, jc, kc);
           int cid[3] = { ic, jc, kc };
           for (int q = 0; q < 3; q++)
+          {
             if (cid[q] == nc[q] - 1) Ac[1 + q][iAc] = 0.0;
+            if (mg->full && cid[q] == 0) Ac[4 + q][iAc] = 0.0;
+          }
         }
   }
 }
@@ -277,11 +326,11 @@ static void rbgs(pfo_mg *mg, int l, double *x, const double *b, int zero,
           long ip = lidx(s, i, j, k);
           double sum = 0.0;
           /* a1..a6 order W E S N L U */
-          double w = (i > 0) ? A[1][ip - 1] * x[ip - 1] : 0.0;
+          double w = (i > 0) ? cpl(A, s, ip, 0, -1) * x[ip - 1] : 0.0;
           double e = (i + 1 < n[0]) ? A[1][ip] * x[ip + 1] : 0.0;
-          double so = (j > 0) ? A[2][ip - s[1]] * x[ip - s[1]] : 0.0;
+          double so = (j > 0) ? cpl(A, s, ip, 1, -1) * x[ip - s[1]] : 0.0;
           double no = (j + 1 < n[1]) ? A[2][ip] * x[ip + s[1]] : 0.0;
-          double lo = (k > 0) ? A[3][ip - s[2]] * x[ip - s[2]] : 0.0;
+          double lo = (k > 0) ? cpl(A, s, ip, 2, -1) * x[ip - s[2]] : 0.0;
           double up = (k + 1 < n[2]) ? A[3][ip] * x[ip + s[2]] : 0.0;
           sum = w + e + so + no + lo + up;
           x[ip] = (b[ip] - sum) / A[0][ip];
@@ -304,11 +353,11 @@ static void residual(pfo_mg *mg, int l, const double *x, const double *b,
       {
         long ip = lidx(s, i, j, k);
         double v = A[0][ip] * x[ip];
-        if (i > 0) v += A[1][ip - 1] * x[ip - 1];
+        if (i > 0) v += cpl(A, s, ip, 0, -1) * x[ip - 1];
         if (i + 1 < n[0]) v += A[1][ip] * x[ip + 1];
-        if (j > 0) v += A[2][ip - s[1]] * x[ip - s[1]];
+        if (j > 0) v += cpl(A, s, ip, 1, -1) * x[ip - s[1]];
         if (j + 1 < n[1]) v += A[2][ip] * x[ip + s[1]];
-        if (k > 0) v += A[3][ip - s[2]] * x[ip - s[2]];
+        if (k > 0) v += cpl(A, s, ip, 2, -1) * x[ip - s[2]];
         if (k + 1 < n[2]) v += A[3][ip] * x[ip + s[2]];
         t[ip] = b[ip] - v;
       }

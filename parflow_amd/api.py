@@ -68,7 +68,8 @@ class SolverOptsC(C.Structure):
                 ("deriv_eps", C.c_double), ("globalization", C.c_int),
                 ("precond", C.c_int), ("mg_pre", C.c_int), ("mg_post", C.c_int),
                 ("mg_symmetric", C.c_int), ("mg_max_levels", C.c_int),
-                ("mg_precision", C.c_int), ("krylov", C.c_int)]
+                ("mg_precision", C.c_int), ("krylov", C.c_int),
+                ("pc_matrix", C.c_int)]
 
 
 def _fill_bc(cb, b):
@@ -163,7 +164,8 @@ EXPORTS = [
     "pfhip_surface_predictor",
     "pfhip_richards_jacobian_eval", "pfhip_matvec", "pfhip_overland_flow_eval_kin",
     "pfhip_vec_linear_sum", "pfhip_vec_const", "pfhip_vec_dot", "pfhip_vec_max_norm",
-    "pfhip_vec_l1_norm", "pfhip_mg_create", "pfhip_mg_destroy", "pfhip_mg_num_levels",
+    "pfhip_vec_l1_norm", "pfhip_mg_create", "pfhip_mg_create_full", "pfhip_mg_destroy",
+    "pfhip_mg_num_levels",
     "pfhip_mg_setup", "pfhip_mg_apply", "pfhip_solver_create", "pfhip_solver_destroy",
     "pfhip_kinsol_nonlin_solver", "pfhip_advance_richards", "pfhip_water_balance_storage",
     "pfhip_ensemble_create", "pfhip_ensemble_destroy", "pfhip_ensemble_advance_richards",
@@ -218,6 +220,7 @@ def lib():
                                                   C.POINTER(C.c_double), vp]
         L.pfhip_mg_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.POINTER(vp)]
+        L.pfhip_mg_create_full.argtypes = L.pfhip_mg_create.argtypes
         L.pfhip_mg_destroy.argtypes = [vp]
         L.pfhip_mg_num_levels.argtypes = [vp, C.POINTER(C.c_int)]
         L.pfhip_mg_setup.argtypes = [vp, C.POINTER(vp), vp]
@@ -316,6 +319,7 @@ def solver_opts_c(o: SolverOptions, precision=64, krylov=0):
     c.mg_max_levels = o.max_levels
     c.mg_precision = precision
     c.krylov = krylov
+    c.pc_matrix = o.pc_matrix_code()
     return c
 
 
@@ -542,13 +546,17 @@ def water_balance_report(gp, p_hist, s_hist, dts, t0):
 
 
 class GPUMultigrid:
-    """The multigrid preconditioner module (PrecondInvoke)."""
+    """The multigrid preconditioner module (PrecondInvoke).  full=True keeps
+    all seven couplings per row (PCMatrixType FullJacobian); setup() then
+    reads W, S, L (J[1], J[3], J[5]) as well."""
 
-    def __init__(self, gp: GPUProblem, pre, post, symmetric, precision=64, max_levels=100):
+    def __init__(self, gp: GPUProblem, pre, post, symmetric, precision=64, max_levels=100,
+                 full=False):
         self.gp = gp
         h = C.c_void_p()
-        check(lib().pfhip_mg_create(gp.h, max_levels, pre, post, symmetric, precision,
-                                    C.byref(h)), "mg_create")
+        create = lib().pfhip_mg_create_full if full else lib().pfhip_mg_create
+        check(create(gp.h, max_levels, pre, post, symmetric, precision, C.byref(h)),
+              "mg_create")
         self.h = h
 
     def levels(self):

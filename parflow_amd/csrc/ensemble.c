@@ -356,6 +356,11 @@ int pfhip_ensemble_create(int n, pfhip_solver *const *solvers, pfhip_ensemble **
                       "ensembles (members share one closure form per launch)", i);
       return PFHIP_ERR_ARG;
     }
+    if (o->pc_matrix) {
+      pfhip_set_error("ensemble member %d: ensembles use the PFSymmetric preconditioner "
+                      "matrix", i);
+      return PFHIP_ERR_ARG;
+    }
   }
   pfhip_ensemble *e = (pfhip_ensemble *)calloc(1, sizeof(pfhip_ensemble));
   if (!e) { pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }

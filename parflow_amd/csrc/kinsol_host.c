@@ -132,9 +132,9 @@ int pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
   for (int q = 0; q < 7 && !rc; q++)
     if (opts->use_jacobian) rc = alloc_vec(s, &s->J[q]);
   if (!rc && opts->precond)
-    rc = pfhip_mg_create(pb, opts->mg_max_levels, opts->mg_pre, opts->mg_post,
-                         opts->mg_symmetric, opts->mg_precision ? opts->mg_precision : 64,
-                         &s->mg);
+    rc = (opts->pc_matrix ? pfhip_mg_create_full : pfhip_mg_create)(
+        pb, opts->mg_max_levels, opts->mg_pre, opts->mg_post, opts->mg_symmetric,
+        opts->mg_precision ? opts->mg_precision : 64, &s->mg);
   s->Hes = (double **)calloc(s->l_max + 1, sizeof(double *));
   for (int i = 0; i <= s->l_max; i++) s->Hes[i] = (double *)calloc(s->l_max, sizeof(double));
   s->givens = (double *)calloc(2 * s->l_max, sizeof(double));

@@ -103,7 +103,10 @@ __device__ __forceinline__ void unflat(const LGeom &g, int idx, int &i, int &j, 
   k = r / g.n[1];
 }
 
-template <typename T> struct Mat { T *c, *u[3]; };   // C and upper E, N, U
+// C, upper E, N, U and, for full (non-symmetric) storage, lower W, S, L;
+// with symmetric storage l[] is null and a point's lower coupling is its
+// lower neighbour's upper one
+template <typename T> struct Mat { T *c, *u[3], *l[3]; };
 
 template <typename T> struct LevelDev {
   LGeom g;
@@ -112,11 +115,18 @@ template <typename T> struct LevelDev {
   T *x, *b, *t, *p1, *p2;
 };
 
-// coupling of point ip in direction q on side -1/+1 (symmetric storage)
-template <typename T>
+// coupling of point ip to its lower neighbour in direction q (stride s)
+template <typename T, bool FULL>
+__device__ __forceinline__ T lo_cpl(const Mat<T> &A, long ip, int q, long s)
+{
+  return FULL ? A.l[q][ip] : A.u[q][ip - s];
+}
+
+// coupling of point ip in direction q on side -1/+1
+template <typename T, bool FULL>
 __device__ __forceinline__ T cpl(const Mat<T> &A, const LGeom &g, long ip, int q, int side)
 {
-  return (side > 0) ? A.u[q][ip] : A.u[q][ip - g.s[q]];
+  return (side > 0) ? A.u[q][ip] : lo_cpl<T, FULL>(A, ip, q, g.s[q]);
 }
 
 // ---- per-point operations -------------------------------------------------
@@ -127,26 +137,38 @@ __device__ __forceinline__ T cpl(const Mat<T> &A, const LGeom &g, long ip, int q
 // and the centre completion (:720-750), whose lower-neighbour couplings are
 // recomputed from the fine operator instead of read back, with the
 // reference's operand order.
-template <typename T>
+//
+// With full storage (the FullJacobian PC matrix) each side of a row is
+// collapsed from its own couplings: along d the coarse couplings are the
+// P^T A P ones, a_U(C) p2(C+1) and a_L(C) p1(C-1), and the lower off-
+// direction couplings are collapsed like the upper ones; on a symmetric
+// matrix this gives the symmetric formulas.
+template <typename T, bool FULL>
 __device__ __forceinline__ void pw_calc(const LGeom &g, const Mat<T> &A, int d, int o1, int o2,
                                         long ip, T &w1, T &w2)
 {
   T a0 = A.c[ip];
-  T a1 = cpl(A, g, ip, d, -1), a2 = cpl(A, g, ip, d, +1);
-  T a3 = cpl(A, g, ip, o1, -1), a4 = cpl(A, g, ip, o1, +1);
-  T a5 = cpl(A, g, ip, o2, -1), a6 = cpl(A, g, ip, o2, +1);
+  T a1 = cpl<T, FULL>(A, g, ip, d, -1), a2 = cpl<T, FULL>(A, g, ip, d, +1);
+  T a3 = cpl<T, FULL>(A, g, ip, o1, -1), a4 = cpl<T, FULL>(A, g, ip, o1, +1);
+  T a5 = cpl<T, FULL>(A, g, ip, o2, -1), a6 = cpl<T, FULL>(A, g, ip, o2, +1);
   T ap0 = a0 + a3 + a4 + a5 + a6;
   if (ap0 != T(0)) { w1 = -a1 / ap0; w2 = -a2 / ap0; }
   else { w1 = T(0); w2 = T(0); }
 }
 
 template <typename T>
-__device__ __forceinline__ T up_coupling(const Mat<T> &A, int q, long iA, long sd)
+__device__ __forceinline__ T collapse3(const T *a, long iA, long sd)
 {
-  return A.u[q][iA] + T(0.5) * A.u[q][iA - sd] + T(0.5) * A.u[q][iA + sd];
+  return a[iA] + T(0.5) * a[iA - sd] + T(0.5) * a[iA + sd];
 }
 
 template <typename T>
+__device__ __forceinline__ T up_coupling(const Mat<T> &A, int q, long iA, long sd)
+{
+  return collapse3(A.u[q], iA, sd);
+}
+
+template <typename T, bool FULL>
 __device__ __forceinline__ void coarse_point(const LGeom &gf, const LGeom &gc, const Mat<T> &A,
                                              const Mat<T> &Ac, int d, int o1, int o2, T *p1,
                                              T *p2, int i, int j, int k)
@@ -160,13 +182,17 @@ __device__ __forceinline__ void coarse_point(const LGeom &gf, const LGeom &gc, c
   long iA1 = iA - sd, iA2 = iA + sd;
   // weights of the odd point below (owned by the coarse point below) and
   // above (owned here); absent points have zero weights
-  T p2lo = T(0), p1hi = T(0);
-  if (fi[d] > 0) { T w1, w2; pw_calc(gf, A, d, o1, o2, iA1, w1, w2); p2lo = w2; }
+  T p2lo = T(0), p1lo = T(0), p1hi = T(0), p2hi = T(0);
+  if (fi[d] > 0) {
+    T w1, w2;
+    pw_calc<T, FULL>(gf, A, d, o1, o2, iA1, w1, w2);
+    p2lo = w2; p1lo = w1;
+  }
   if (fi[d] + 1 < gf.n[d]) {
     T w1, w2;
-    pw_calc(gf, A, d, o1, o2, iA2, w1, w2);
+    pw_calc<T, FULL>(gf, A, d, o1, o2, iA2, w1, w2);
     p1[iA2] = w1; p2[iA2] = w2;
-    p1hi = w1;
+    p1hi = w1; p2hi = w2;
   }
   T up1 = up_coupling(A, o1, iA, sd);
   T up2 = up_coupling(A, o2, iA, sd);
@@ -174,18 +200,28 @@ __device__ __forceinline__ void coarse_point(const LGeom &gf, const LGeom &gc, c
   T c6 = (cid[o2] == gc.n[o2] - 1) ? T(0) : up2;
   Ac.u[o1][iAc] = c4;
   Ac.u[o2][iAc] = c6;
-  T upd;
-  if (cid[d] + 1 < gc.n[d]) upd = cpl(A, gf, iA + 2 * sd, d, -1) * p1hi;
-  else upd = T(0);
-  Ac.u[d][iAc] = upd;
+  T c3, c5;
+  if (FULL) {
+    c3 = (cid[o1] > 0) ? collapse3(A.l[o1], iA, sd) : T(0);
+    c5 = (cid[o2] > 0) ? collapse3(A.l[o2], iA, sd) : T(0);
+    Ac.l[o1][iAc] = c3;
+    Ac.l[o2][iAc] = c5;
+    Ac.u[d][iAc] = (cid[d] + 1 < gc.n[d]) ? A.u[d][iA] * p2hi : T(0);
+    Ac.l[d][iAc] = (cid[d] > 0) ? A.l[d][iA] * p1lo : T(0);
+  } else {
+    T upd;
+    if (cid[d] + 1 < gc.n[d]) upd = cpl<T, false>(A, gf, iA + 2 * sd, d, -1) * p1hi;
+    else upd = T(0);
+    Ac.u[d][iAc] = upd;
+    // lower neighbours' o1 / o2 couplings (never the last index, so unzeroed)
+    c3 = (cid[o1] > 0) ? up_coupling(A, o1, iA - gf.s[o1], sd) : T(0);
+    c5 = (cid[o2] > 0) ? up_coupling(A, o2, iA - gf.s[o2], sd) : T(0);
+  }
   T a0 = A.c[iA];
-  T a1 = cpl(A, gf, iA, d, -1), a2 = cpl(A, gf, iA, d, +1);
-  T a3 = cpl(A, gf, iA, o1, -1), a4 = cpl(A, gf, iA, o1, +1);
-  T a5 = cpl(A, gf, iA, o2, -1), a6 = cpl(A, gf, iA, o2, +1);
+  T a1 = cpl<T, FULL>(A, gf, iA, d, -1), a2 = cpl<T, FULL>(A, gf, iA, d, +1);
+  T a3 = cpl<T, FULL>(A, gf, iA, o1, -1), a4 = cpl<T, FULL>(A, gf, iA, o1, +1);
+  T a5 = cpl<T, FULL>(A, gf, iA, o2, -1), a6 = cpl<T, FULL>(A, gf, iA, o2, +1);
   T cc = a0 + a3 + a4 + a5 + a6 + a1 * p2lo + a2 * p1hi;
-  // lower neighbours' o1 / o2 couplings (never the last index, so unzeroed)
-  T c3 = (cid[o1] > 0) ? up_coupling(A, o1, iA - gf.s[o1], sd) : T(0);
-  T c5 = (cid[o2] > 0) ? up_coupling(A, o2, iA - gf.s[o2], sd) : T(0);
   Ac.c[iAc] = cc - (c3 + c4 + c5 + c6);
 }
 
@@ -208,7 +244,7 @@ __device__ __forceinline__ T gs_update(T bb, T c, T aw, T xw, bool hw, T ae, T x
 }
 
 // red/black update of one point of parity `par`; init: x = b / a0
-template <typename T>
+template <typename T, bool FULL>
 __device__ __forceinline__ void rbgs_point(const LGeom &g, const Mat<T> &A, T *x, const T *b,
                                            int par, int init, int i, int j, int k)
 {
@@ -219,26 +255,26 @@ __device__ __forceinline__ void rbgs_point(const LGeom &g, const Mat<T> &A, T *x
   bool hw = i > 0, he = i + 1 < g.n[0], hs = j > 0, hn = j + 1 < g.n[1];
   bool hl = k > 0, hu = k + 1 < g.n[2];
   x[ip] = gs_update(b[ip], A.c[ip],
-                    hw ? A.u[0][ip - 1] : T(0), hw ? x[ip - 1] : T(0), hw,
+                    hw ? lo_cpl<T, FULL>(A, ip, 0, 1) : T(0), hw ? x[ip - 1] : T(0), hw,
                     A.u[0][ip], he ? x[ip + 1] : T(0), he,
-                    hs ? A.u[1][ip - sy] : T(0), hs ? x[ip - sy] : T(0), hs,
+                    hs ? lo_cpl<T, FULL>(A, ip, 1, sy) : T(0), hs ? x[ip - sy] : T(0), hs,
                     A.u[1][ip], hn ? x[ip + sy] : T(0), hn,
-                    hl ? A.u[2][ip - sz] : T(0), hl ? x[ip - sz] : T(0), hl,
+                    hl ? lo_cpl<T, FULL>(A, ip, 2, sz) : T(0), hl ? x[ip - sz] : T(0), hl,
                     A.u[2][ip], hu ? x[ip + sz] : T(0), hu);
 }
 
-template <typename T>
+template <typename T, bool FULL>
 __device__ __forceinline__ void resid_point(const LGeom &g, const Mat<T> &A, const T *x,
                                             const T *b, T *t, int i, int j, int k)
 {
   long ip = LIDX(g, i, j, k);
   const int sy = g.s[1], sz = g.s[2];
   T v = A.c[ip] * x[ip];
-  if (i > 0) v += A.u[0][ip - 1] * x[ip - 1];
+  if (i > 0) v += lo_cpl<T, FULL>(A, ip, 0, 1) * x[ip - 1];
   if (i + 1 < g.n[0]) v += A.u[0][ip] * x[ip + 1];
-  if (j > 0) v += A.u[1][ip - sy] * x[ip - sy];
+  if (j > 0) v += lo_cpl<T, FULL>(A, ip, 1, sy) * x[ip - sy];
   if (j + 1 < g.n[1]) v += A.u[1][ip] * x[ip + sy];
-  if (k > 0) v += A.u[2][ip - sz] * x[ip - sz];
+  if (k > 0) v += lo_cpl<T, FULL>(A, ip, 2, sz) * x[ip - sz];
   if (k + 1 < g.n[2]) v += A.u[2][ip] * x[ip + sz];
   t[ip] = b[ip] - v;
 }
@@ -283,21 +319,21 @@ __device__ __forceinline__ void prolong_point(const LGeom &gf, const LGeom &gc, 
 }
 
 // ---- grid kernels (large levels) --------------------------------------------
-template <typename T>
+template <typename T, bool FULL>
 __global__ __launch_bounds__(256) void k_coarse_fused(LGeom gf, LGeom gc, Mat<T> A, Mat<T> Ac,
                                                       int d, int o1, int o2, T *__restrict__ p1,
                                                       T *__restrict__ p2)
 {
   XCD_IJK(gc);
-  coarse_point(gf, gc, A, Ac, d, o1, o2, p1, p2, i, j, k);
+  coarse_point<T, FULL>(gf, gc, A, Ac, d, o1, o2, p1, p2, i, j, k);
 }
 
-template <typename T>
+template <typename T, bool FULL>
 __global__ __launch_bounds__(256) void k_rbgs(LGeom g, Mat<T> A, T *__restrict__ x,
                                               const T *__restrict__ b, int par, int init)
 {
   XCD_IJK(g);
-  rbgs_point(g, A, x, b, par, init, i, j, k);
+  rbgs_point<T, FULL>(g, A, x, b, par, init, i, j, k);
 }
 
 // ---- fused red -> black sweep pair ------------------------------------------
@@ -723,12 +759,12 @@ __global__ __launch_bounds__(64 * TY) void k_resid_restrict_z(LGeom gf, LGeom gc
   }
 }
 
-template <typename T>
+template <typename T, bool FULL>
 __global__ __launch_bounds__(256) void k_residual_l(LGeom g, Mat<T> A, const T *__restrict__ x,
                                                     const T *__restrict__ b, T *__restrict__ t)
 {
   XCD_IJK(g);
-  resid_point(g, A, x, b, t, i, j, k);
+  resid_point<T, FULL>(g, A, x, b, t, i, j, k);
 }
 
 template <typename T>
@@ -788,7 +824,7 @@ __global__ void k_fill_t(T *p, long n)
 // 2 and 4 raise register use past the 1024-thread budget).  The arithmetic
 // per point is rbgs_point's.
 #define WUNR 1
-template <typename T>
+template <typename T, bool FULL>
 __device__ void rbgs_phase(const LGeom &g, const Mat<T> &A, T *__restrict__ x,
                            const T *__restrict__ b, int par, int init)
 {
@@ -816,15 +852,15 @@ __device__ void rbgs_phase(const LGeom &g, const Mat<T> &A, T *__restrict__ x,
       bb[u] = a ? b[q] : T(0);
       c[u] = a ? A.c[q] : T(1);
       const bool nb = a && !init;
-      aw[u] = (nb && hw[u]) ? A.u[0][q - 1] : T(0);
+      aw[u] = (nb && hw[u]) ? lo_cpl<T, FULL>(A, q, 0, 1) : T(0);
       xw[u] = (nb && hw[u]) ? x[q - 1] : T(0);
       ae[u] = nb ? A.u[0][q] : T(0);
       xe[u] = (nb && he[u]) ? x[q + 1] : T(0);
-      as[u] = (nb && hs[u]) ? A.u[1][q - sy] : T(0);
+      as[u] = (nb && hs[u]) ? lo_cpl<T, FULL>(A, q, 1, sy) : T(0);
       xs[u] = (nb && hs[u]) ? x[q - sy] : T(0);
       an[u] = nb ? A.u[1][q] : T(0);
       xn[u] = (nb && hn[u]) ? x[q + sy] : T(0);
-      al[u] = (nb && hl[u]) ? A.u[2][q - sz] : T(0);
+      al[u] = (nb && hl[u]) ? lo_cpl<T, FULL>(A, q, 2, sz) : T(0);
       xl[u] = (nb && hl[u]) ? x[q - sz] : T(0);
       au[u] = nb ? A.u[2][q] : T(0);
       xu[u] = (nb && hu[u]) ? x[q + sz] : T(0);
@@ -843,7 +879,7 @@ __device__ void rbgs_phase(const LGeom &g, const Mat<T> &A, T *__restrict__ x,
 
 // t = b - A x over a level inside a workgroup, WUNR points per thread per
 // pass with the loads first; resid_point's expressions and order
-template <typename T>
+template <typename T, bool FULL>
 __device__ void resid_phase(const LGeom &g, const Mat<T> &A, const T *__restrict__ x,
                             const T *__restrict__ b, T *__restrict__ tt)
 {
@@ -869,15 +905,15 @@ __device__ void resid_phase(const LGeom &g, const Mat<T> &A, const T *__restrict
       bb[u] = a ? b[q] : T(0);
       c[u] = a ? A.c[q] : T(0);
       x0[u] = a ? x[q] : T(0);
-      aw[u] = hw[u] ? A.u[0][q - 1] : T(0);
+      aw[u] = hw[u] ? lo_cpl<T, FULL>(A, q, 0, 1) : T(0);
       xw[u] = hw[u] ? x[q - 1] : T(0);
       ae[u] = he[u] ? A.u[0][q] : T(0);
       xe[u] = he[u] ? x[q + 1] : T(0);
-      as[u] = hs[u] ? A.u[1][q - sy] : T(0);
+      as[u] = hs[u] ? lo_cpl<T, FULL>(A, q, 1, sy) : T(0);
       xs[u] = hs[u] ? x[q - sy] : T(0);
       an[u] = hn[u] ? A.u[1][q] : T(0);
       xn[u] = hn[u] ? x[q + sy] : T(0);
-      al[u] = hl[u] ? A.u[2][q - sz] : T(0);
+      al[u] = hl[u] ? lo_cpl<T, FULL>(A, q, 2, sz) : T(0);
       xl[u] = hl[u] ? x[q - sz] : T(0);
       au[u] = hu[u] ? A.u[2][q] : T(0);
       xu[u] = hu[u] ? x[q + sz] : T(0);
@@ -899,7 +935,7 @@ __device__ void resid_phase(const LGeom &g, const Mat<T> &A, const T *__restrict
 }
 
 // RedBlackGSPoint call sequence (PL/rb_GS_point.c:100-300) inside a block
-template <typename T>
+template <typename T, bool FULL>
 __device__ void rbgs_block(const LevelDev<T> &L, int zero, int max_iter, int symmetric)
 {
   const LGeom &g = L.g;
@@ -911,54 +947,55 @@ __device__ void rbgs_block(const LevelDev<T> &L, int zero, int max_iter, int sym
       FOR_POINTS(g, L.x[LIDX(g, i, j, k)] = T(0));
       if (!symmetric) return;
     }
-    rbgs_phase(g, L.A, L.x, L.b, 1, 1);
+    rbgs_phase<T, FULL>(g, L.A, L.x, L.b, 1, 1);
     rb = !rb;
   }
   for (; (count / 2) <= max_iter; count++) {
     int par = rb ? 0 : 1;
-    rbgs_phase(g, L.A, L.x, L.b, par, 0);
+    rbgs_phase<T, FULL>(g, L.A, L.x, L.b, par, 0);
     rb = !rb;
   }
 }
 
 // V-cycle over levels lc .. L-1 inside one workgroup (b[lc] given, x[lc]
 // returned)
-template <typename T>
+template <typename T, bool FULL>
 __device__ void vcycle_block(const LevelDev<T> *__restrict__ lev, int lc, int L, int pre,
                              int post, int symmetric)
 {
   int l;
   for (l = lc; l <= L - 2; l++) {
     const LevelDev<T> &F = lev[l], &Cc = lev[l + 1];
-    rbgs_block(F, 1, pre, symmetric);
-    resid_phase(F.g, F.A, F.x, F.b, F.t);
+    rbgs_block<T, FULL>(F, 1, pre, symmetric);
+    resid_phase<T, FULL>(F.g, F.A, F.x, F.b, F.t);
     FOR_POINTS(Cc.g, restrict_point(F.g, Cc.g, F.dir, F.t, Cc.b, F.p1, F.p2, i, j, k));
   }
-  rbgs_block(lev[L - 1], 1, 1, symmetric);
+  rbgs_block<T, FULL>(lev[L - 1], 1, 1, symmetric);
   for (l = L - 2; l >= lc; l--) {
     const LevelDev<T> &F = lev[l], &Cc = lev[l + 1];
     FOR_POINTS(F.g, prolong_point(F.g, Cc.g, F.dir, Cc.x, F.x, F.p1, F.p2, i, j, k));
-    rbgs_block(F, 0, post, symmetric);
+    rbgs_block<T, FULL>(F, 0, post, symmetric);
   }
 }
 
 // bottom of the V-cycle: levels lc .. L-1 (b[lc] given, x[lc] returned)
-template <typename T>
+template <typename T, bool FULL>
 __global__ __launch_bounds__(CWG) void k_coarse_vcycle(const LevelDev<T> *__restrict__ lev,
                                                        int lc, int L, int pre, int post,
                                                        int symmetric)
 {
-  vcycle_block(lev, lc, L, pre, post, symmetric);
+  vcycle_block<T, FULL>(lev, lc, L, pre, post, symmetric);
 }
 
 // coarse operators of levels ls .. L-2 -> ls+1 .. L-1
-template <typename T>
+template <typename T, bool FULL>
 __global__ __launch_bounds__(CWG) void k_coarse_setup(const LevelDev<T> *__restrict__ lev,
                                                       int ls, int L)
 {
   for (int l = ls; l <= L - 2; l++) {
     const LevelDev<T> &F = lev[l], &Cc = lev[l + 1];
-    FOR_POINTS(Cc.g, coarse_point(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1, F.p2, i, j, k));
+    FOR_POINTS(Cc.g, (coarse_point<T, FULL>(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1, F.p2,
+                                            i, j, k)));
   }
 }
 
@@ -968,10 +1005,11 @@ struct pfhip_mg {
   int L, prec, pre, post, symmetric;
   int lc;                      // first level run by the single-workgroup kernels
   int n[MAXL][3], dir[MAXL];
-  void *A[MAXL][4];
+  int full;                    // full storage (FullJacobian PC matrix)
+  void *A[MAXL][7];            // C, E, N, U and (full) W, S, L
   void *x[MAXL], *b[MAXL], *t[MAXL], *p1[MAXL], *p2[MAXL];
   void *xa[MAXL];              // second iterate buffer of the grid levels (l < lc)
-  void *raw[MAXL][10];         // allocations; each array starts one element in, so
+  void *raw[MAXL][13];         // allocations; each array starts one element in, so
                                // the ghosted index of an even x is vector aligned
   int fuse;                    // red -> black pairs as one launch (k_rbgs_pair)
   int rrz;                     // fused level residual + z restriction
@@ -987,7 +1025,10 @@ template <typename T> static Mat<T> matof(pfhip_mg *mg, int l)
 {
   Mat<T> M;
   M.c = (T *)mg->A[l][0];
-  for (int q = 0; q < 3; q++) M.u[q] = (T *)mg->A[l][1 + q];
+  for (int q = 0; q < 3; q++) {
+    M.u[q] = (T *)mg->A[l][1 + q];
+    M.l[q] = mg->full ? (T *)mg->A[l][4 + q] : nullptr;
+  }
   return M;
 }
 
@@ -1017,8 +1058,8 @@ template <typename T> static int upload_levels(pfhip_mg *mg)
   return PFHIP_OK;
 }
 
-extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int post,
-                               int symmetric, int precision, pfhip_mg **out)
+static int mg_create(pfhip_problem *pb, int max_levels, int pre, int post, int symmetric,
+                     int precision, int full, pfhip_mg **out)
 {
   if (precision != 64 && precision != 32) {
     pfhip_set_error("mg precision must be 64 or 32");
@@ -1032,6 +1073,7 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
   mg->pb = pb;
   mg->prec = precision;
   mg->pre = pre; mg->post = post; mg->symmetric = symmetric;
+  mg->full = full;
   int N[3] = { pb->d.nx, pb->d.ny, pb->d.nz };
   double D[3] = { pb->d.dx, pb->d.dy, pb->d.dz };
   if (max_levels <= 0 || max_levels > MAXL) max_levels = MAXL;
@@ -1069,11 +1111,11 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
   int rc = PFHIP_OK;
   for (l = 0; l < mg->L && rc == PFHIP_OK; l++) {
     long s = lsize(mg->n[l]);
-    void **ptrs[10] = { &mg->A[l][0], &mg->A[l][1], &mg->A[l][2], &mg->A[l][3],
+    void **ptrs[13] = { &mg->A[l][0], &mg->A[l][1], &mg->A[l][2], &mg->A[l][3],
                         &mg->x[l], &mg->b[l], &mg->t[l], &mg->p1[l], &mg->p2[l],
-                        &mg->xa[l] };
-    int nalloc = (l < mg->lc) ? 10 : 9;
-    for (int q = 0; q < nalloc && rc == PFHIP_OK; q++) {
+                        &mg->xa[l], &mg->A[l][4], &mg->A[l][5], &mg->A[l][6] };
+    for (int q = 0; q < 13 && rc == PFHIP_OK; q++) {
+      if ((q == 9 && l >= mg->lc) || (q >= 10 && !full)) continue;
       rc = pfhip_malloc(&mg->raw[l][q], (long)((s + 1) * es));
       if (rc == PFHIP_OK) {
         *ptrs[q] = (char *)mg->raw[l][q] + es;
@@ -1087,17 +1129,29 @@ extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int p
   if (rc != PFHIP_OK) { pfhip_mg_destroy(mg); return rc; }
   int forced = rbgs_forced();
   for (l = 0; l < MAXL; l++) mg->pv[l] = (forced >= 0) ? forced : 0;
-  mg->tuned = (forced >= 0) || !mg->fuse;
+  mg->tuned = (forced >= 0) || !mg->fuse || full;
   HIPCHK(hipDeviceSynchronize());
   *out = mg;
   return PFHIP_OK;
+}
+
+extern "C" int pfhip_mg_create(pfhip_problem *pb, int max_levels, int pre, int post,
+                               int symmetric, int precision, pfhip_mg **out)
+{
+  return mg_create(pb, max_levels, pre, post, symmetric, precision, 0, out);
+}
+
+extern "C" int pfhip_mg_create_full(pfhip_problem *pb, int max_levels, int pre, int post,
+                                    int symmetric, int precision, pfhip_mg **out)
+{
+  return mg_create(pb, max_levels, pre, post, symmetric, precision, 1, out);
 }
 
 extern "C" int pfhip_mg_destroy(pfhip_mg *mg)
 {
   if (!mg) return PFHIP_OK;
   for (int l = 0; l < mg->L; l++) {
-    for (int q = 0; q < 10; q++) if (mg->raw[l][q]) hipFree(mg->raw[l][q]);
+    for (int q = 0; q < 13; q++) if (mg->raw[l][q]) hipFree(mg->raw[l][q]);
   }
   if (mg->levdev) hipFree(mg->levdev);
   free(mg);
@@ -1116,8 +1170,8 @@ template <typename T>
 static int mg_setup_t(pfhip_mg *mg, const double *const *Asym, hipStream_t st)
 {
   LGeom g0 = lgeom(mg->n[0]);
-  const double *src[4] = { Asym[0], Asym[2], Asym[4], Asym[6] };
-  for (int q = 0; q < 4; q++) {
+  const double *src[7] = { Asym[0], Asym[2], Asym[4], Asym[6], Asym[1], Asym[3], Asym[5] };
+  for (int q = 0; q < (mg->full ? 7 : 4); q++) {
     hipLaunchKernelGGL((k_convert<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, src[q],
                        (T *)mg->A[0][q]);
     LAUNCHCHK();
@@ -1133,28 +1187,39 @@ template <typename T> static int mg_coarse_setup_t(pfhip_mg *mg, hipStream_t st)
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
     int d = mg->dir[l], o1, o2;
     offdirs(d, &o1, &o2);
-    hipLaunchKernelGGL(k_coarse_fused<T>, dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc,
-                       matof<T>(mg, l), matof<T>(mg, l + 1), d, o1, o2, (T *)mg->p1[l],
-                       (T *)mg->p2[l]);
+    if (mg->full)
+      hipLaunchKernelGGL((k_coarse_fused<T, true>), dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc,
+                         matof<T>(mg, l), matof<T>(mg, l + 1), d, o1, o2, (T *)mg->p1[l],
+                         (T *)mg->p2[l]);
+    else
+      hipLaunchKernelGGL((k_coarse_fused<T, false>), dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc,
+                         matof<T>(mg, l), matof<T>(mg, l + 1), d, o1, o2, (T *)mg->p1[l],
+                         (T *)mg->p2[l]);
     LAUNCHCHK();
   }
   // small levels: one workgroup
   if (mg->lc <= mg->L - 2) {
-    hipLaunchKernelGGL(k_coarse_setup<T>, dim3(1), dim3(CWG), 0, st,
-                       (const LevelDev<T> *)mg->levdev, mg->lc, mg->L);
+    if (mg->full)
+      hipLaunchKernelGGL((k_coarse_setup<T, true>), dim3(1), dim3(CWG), 0, st,
+                         (const LevelDev<T> *)mg->levdev, mg->lc, mg->L);
+    else
+      hipLaunchKernelGGL((k_coarse_setup<T, false>), dim3(1), dim3(CWG), 0, st,
+                         (const LevelDev<T> *)mg->levdev, mg->lc, mg->L);
     LAUNCHCHK();
   }
   return PFHIP_OK;
 }
 
 // PC setup of KinsolPC (PL/kinsol_pc.c:147-154) without an intermediate
-// matrix: the symmetric Jacobian of pressure p is assembled straight into
-// the finest level (in the level precision), then the coarse operators.
+// matrix: the Jacobian of pressure p (its symmetric part, or for a full-
+// storage hierarchy the full Jacobian with the surface couplings) is
+// assembled straight into the finest level (in the level precision), then
+// the coarse operators.
 int pfhip_mg_setup_jacobian(pfhip_mg *mg, const double *p, double dt, void *stream)
 {
-  void *J[7] = { mg->A[0][0], nullptr, mg->A[0][1], nullptr, mg->A[0][2], nullptr,
-                 mg->A[0][3] };
-  int rc = pfhip_jacobian_into(mg->pb, p, dt, 1, J, mg->prec == 32, stream);
+  void *J[7] = { mg->A[0][0], mg->A[0][4], mg->A[0][1], mg->A[0][5], mg->A[0][2],
+                 mg->A[0][6], mg->A[0][3] };
+  int rc = pfhip_jacobian_into(mg->pb, p, dt, !mg->full, J, mg->prec == 32, stream);
   if (rc) return rc;
   return (mg->prec == 64) ? mg_coarse_setup_t<double>(mg, STREAM(stream))
                           : mg_coarse_setup_t<float>(mg, STREAM(stream));
@@ -1164,6 +1229,10 @@ extern "C" int pfhip_mg_setup(pfhip_mg *mg, const double *const *Asym, void *str
 {
   if (!Asym || !Asym[0] || !Asym[2] || !Asym[4] || !Asym[6]) {
     pfhip_set_error("mg_setup: need C, E, N, U arrays");
+    return PFHIP_ERR_ARG;
+  }
+  if (mg->full && (!Asym[1] || !Asym[3] || !Asym[5])) {
+    pfhip_set_error("mg_setup: a full-storage multigrid needs all seven arrays");
     return PFHIP_ERR_ARG;
   }
   return (mg->prec == 64) ? mg_setup_t<double>(mg, Asym, STREAM(stream))
@@ -1320,13 +1389,18 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
   }
   for (int q = 0; q < n;) {
     if (l == 0) pfhip_prof_mark(PFHIP_PROF_RBGS, 0, st);
-    if (mg->fuse && mg->xa[l] && q + 1 < n && seq[q].par == 1 && seq[q + 1].par == 0) {
+    if (mg->fuse && !mg->full && mg->xa[l] && q + 1 < n && seq[q].par == 1 &&
+        seq[q + 1].par == 0) {
       launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, mg->pv[l], st);
       void *tmp = mg->x[l]; mg->x[l] = mg->xa[l]; mg->xa[l] = tmp;
       q += 2;
     } else {
-      hipLaunchKernelGGL(k_rbgs<T>, grid, LBLOCK, 0, st, g, A, (T *)mg->x[l], b, seq[q].par,
-                         seq[q].init);
+      if (mg->full)
+        hipLaunchKernelGGL((k_rbgs<T, true>), grid, LBLOCK, 0, st, g, A, (T *)mg->x[l], b,
+                           seq[q].par, seq[q].init);
+      else
+        hipLaunchKernelGGL((k_rbgs<T, false>), grid, LBLOCK, 0, st, g, A, (T *)mg->x[l], b,
+                           seq[q].par, seq[q].init);
       q += 1;
     }
     if (l == 0) pfhip_prof_mark(PFHIP_PROF_RBGS, 1, st);
@@ -1362,15 +1436,21 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
   for (l = 0; l < lc; l++) {
     if ((rc = rbgs<T>(mg, l, 1, mg->pre, st))) return rc;
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
-    if (mg->rrz && mg->dir[l] == 2 && (gf.n[0] & 1) == 0) {
+    if (mg->rrz && !mg->full && mg->dir[l] == 2 && (gf.n[0] & 1) == 0) {
       launch_resid_restrict_z<T>(gf, gc, matof<T>(mg, l), (const T *)mg->x[l],
                                  (const T *)mg->b[l], (T *)mg->b[l + 1], (const T *)mg->p1[l],
                                  (const T *)mg->p2[l], st);
       LAUNCHCHK();
       continue;
     }
-    hipLaunchKernelGGL(k_residual_l<T>, dim3(ntiles(gf)), LBLOCK, 0, st, gf, matof<T>(mg, l),
-                       (const T *)mg->x[l], (const T *)mg->b[l], (T *)mg->t[l]);
+    if (mg->full)
+      hipLaunchKernelGGL((k_residual_l<T, true>), dim3(ntiles(gf)), LBLOCK, 0, st, gf,
+                         matof<T>(mg, l), (const T *)mg->x[l], (const T *)mg->b[l],
+                         (T *)mg->t[l]);
+    else
+      hipLaunchKernelGGL((k_residual_l<T, false>), dim3(ntiles(gf)), LBLOCK, 0, st, gf,
+                         matof<T>(mg, l), (const T *)mg->x[l], (const T *)mg->b[l],
+                         (T *)mg->t[l]);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_restrict<T>, dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc, mg->dir[l],
                        (const T *)mg->t[l], (T *)mg->b[l + 1], (const T *)mg->p1[l],
@@ -1380,9 +1460,14 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
   // bottom of the cycle: levels lc .. L-1 in one workgroup; the coarsest
   // level's solve is RedBlackGSPoint with MaxIter 1
   if (lc <= L - 2) {
-    hipLaunchKernelGGL(k_coarse_vcycle<T>, dim3(1), dim3(CWG), 0, st,
-                       (const LevelDev<T> *)mg->levdev, lc, L, mg->pre, mg->post,
-                       mg->symmetric);
+    if (mg->full)
+      hipLaunchKernelGGL((k_coarse_vcycle<T, true>), dim3(1), dim3(CWG), 0, st,
+                         (const LevelDev<T> *)mg->levdev, lc, L, mg->pre, mg->post,
+                         mg->symmetric);
+    else
+      hipLaunchKernelGGL((k_coarse_vcycle<T, false>), dim3(1), dim3(CWG), 0, st,
+                         (const LevelDev<T> *)mg->levdev, lc, L, mg->pre, mg->post,
+                         mg->symmetric);
     LAUNCHCHK();
   } else {
     if ((rc = rbgs<T>(mg, L - 1, 1, 1, st))) return rc;
@@ -1424,7 +1509,8 @@ __global__ __launch_bounds__(CWG) void k_ens_mg_coarse(const EnsMG<T> *__restric
   const EnsMG<T> &M = E[blockIdx.x];
   for (int l = 0; l <= M.L - 2; l++) {
     const LevelDev<T> &F = M.lev[l], &Cc = M.lev[l + 1];
-    FOR_POINTS(Cc.g, coarse_point(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1, F.p2, i, j, k));
+    FOR_POINTS(Cc.g, (coarse_point<T, false>(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1,
+                                             F.p2, i, j, k)));
   }
 }
 
@@ -1434,7 +1520,7 @@ __global__ __launch_bounds__(CWG) void k_ens_vcycle(const EnsMG<T> *__restrict__
   const EnsMG<T> &M = E[blockIdx.x];
   const LevelDev<T> &L0 = M.lev[0];
   FOR_POINTS(L0.g, L0.b[LIDX(L0.g, i, j, k)] = (T)M.bin[LIDX(L0.g, i, j, k)]);
-  vcycle_block(M.lev, 0, M.L, M.pre, M.post, M.symmetric);
+  vcycle_block<T, false>(M.lev, 0, M.L, M.pre, M.post, M.symmetric);
   FOR_POINTS(L0.g, M.xout[LIDX(L0.g, i, j, k)] = (double)L0.x[LIDX(L0.g, i, j, k)]);
 }
 
@@ -1474,11 +1560,16 @@ static int ens_mg_exec(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st, int
 {
   if (n <= 0) return PFHIP_OK;
   int prec = rq[0]->mg->prec;
-  for (int r = 1; r < n; r++)
+  for (int r = 0; r < n; r++) {
     if (rq[r]->mg->prec != prec) {
       pfhip_set_error("ensemble members need one multigrid precision");
       return PFHIP_ERR_ARG;
     }
+    if (rq[r]->mg->full) {
+      pfhip_set_error("ensemble members need the PFSymmetric preconditioner matrix");
+      return PFHIP_ERR_ARG;
+    }
+  }
   return (prec == 32) ? ens_mg_launch<float>(rq, n, st, apply, STREAM(stream))
                       : ens_mg_launch<double>(rq, n, st, apply, STREAM(stream));
 }

@@ -291,8 +291,9 @@ def problem_from_pfidb(db, base_dir=".", name=None):
         raise ValueError("pfidb: wells are not supported")
     if K.s("Solver", "Richards") != "Richards":
         raise ValueError("pfidb: only the Richards solver is supported")
-    if K.s("Solver.Linear.Preconditioner.PCMatrixType", "PFSymmetric") != "PFSymmetric":
-        raise ValueError("pfidb: only the PFSymmetric preconditioner matrix is supported")
+    pc_matrix = K.s("Solver.Linear.Preconditioner.PCMatrixType", "PFSymmetric")
+    if pc_matrix not in ("PFSymmetric", "FullJacobian"):
+        raise ValueError(f"pfidb: PCMatrixType {pc_matrix} is not supported")
     G = _Grid(K)
     geoms = _Geoms(K)
     dom = K.s("Domain.GeomName")
@@ -461,7 +462,8 @@ def problem_from_pfidb(db, base_dir=".", name=None):
         num_pre_relax=K.i("Solver.Linear.Preconditioner.PFMG.NumPreRelax", 1),
         num_post_relax=K.i("Solver.Linear.Preconditioner.PFMG.NumPostRelax", 1),
         max_levels=K.i("Solver.Linear.Preconditioner.MGSemi.MaxLevels", 100),
-        max_convergence_failures=K.i("Solver.MaxConvergenceFailures", 3))
+        max_convergence_failures=K.i("Solver.MaxConvergenceFailures", 3),
+        pc_matrix=pc_matrix)
     return prob, opts, p0, dt, nsteps
 
 

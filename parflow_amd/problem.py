@@ -80,6 +80,16 @@ class SolverOptions:
     num_post_relax: int = 1             # PFMG NumPostRelax
     max_levels: int = 100               # MGSemi MaxLevels
     max_convergence_failures: int = 3   # Solver.MaxConvergenceFailures
+    pc_matrix: str = "PFSymmetric"      # Solver.Linear.Preconditioner.PCMatrixType
+
+    def pc_matrix_code(self):
+        """0 PFSymmetric (symmetric part, C E N U), 1 FullJacobian (all seven
+        couplings); kinsol_pc.c:215-240.  SymmetricPart and Picard are not
+        part of this build."""
+        codes = {"PFSymmetric": 0, "FullJacobian": 1}
+        if self.pc_matrix not in codes:
+            raise ValueError(f"unsupported PCMatrixType {self.pc_matrix!r}")
+        return codes[self.pc_matrix]
 
     def mg_params(self):
         """(pre, post, symmetric) sweeps of the V-cycle for ``precond``.

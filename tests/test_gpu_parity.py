@@ -417,6 +417,37 @@ def test_multigrid(gpu, precision):
         assert _relmax(_I(prob, xg), _I(prob, xo)) < tol
 
 
+@pytest.mark.parametrize("shape", [(16, 12, 10), (70, 40, 24)])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_multigrid_full_storage(gpu, precision, shape):
+    """pfhip_mg_create_full (PCMatrixType FullJacobian) on the full, non-
+    symmetric Jacobian of an overland problem against the oracle's full-
+    storage hierarchy: the level-0 grid kernels and the one-workgroup
+    bottom, plus grid-kernel coarse levels at the larger shape."""
+    api = _api()
+    prob = cases.hetero(*shape, seed=17, overland=True)
+    print(f"mg full precision={precision} shape={shape}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p = cases.random_pressure(prob, seed=18)
+    J = O.jacobian_eval(op, p, 0.1, 0.1, 0)
+    # the PC matrix is non-symmetric: W(i) != E(i - 1) somewhere
+    assert np.max(np.abs(J[1][1:-1, 1:-1, 2:-1] - J[2][1:-1, 1:-1, 1:-2])) > \
+        1e-3 * np.max(np.abs(J[2]))
+    for pre, post, sym in ((1, 1, 1), (2, 2, 0)):
+        mgo = O.MG(prob, pre, post, sym, full=True)
+        mgo.setup(J)
+        mgg = api.GPUMultigrid(gp, pre, post, sym, precision, full=True)
+        assert mgg.levels() == len(mgo.levels())
+        mgg.setup(J)
+        b = np.zeros(prob.shape)
+        b[1:-1, 1:-1, 1:-1] = np.random.default_rng(19).normal(size=(prob.nz, prob.ny, prob.nx))
+        xo = mgo.apply(b)
+        xg = mgg.apply(b)
+        tol = 1e-11 if precision == 64 else 2e-5
+        assert _relmax(_I(prob, xg), _I(prob, xo)) < tol, (pre, post, sym)
+
+
 @pytest.mark.parametrize("shape", [(70, 40, 24), (130, 20, 18)])
 @pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("fuse", [0, 1])
@@ -652,6 +683,54 @@ def test_tiltedv_jacobian_true_on_gpu(gpu):
         s = max(2 * d - 1, 0)
         a, b = prob.interior(ph[s]), g["press"][d]
         assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-7, d
+
+
+def test_tiltedv_jacobian_true_nonsymmetric_pc_on_gpu(gpu):
+    """overland_tiltedV_KWE.py's "Jacobian True Nonsymmetric Preconditioner"
+    variant (:417-440: UseJacobian True, PCMatrixType FullJacobian): the
+    full-storage hierarchy built from the full Jacobian on the device,
+    against the oracle per step and against the reference's correct_output
+    (the same files every variant of the deck is checked against)."""
+    import dataclasses
+    prob, opts, p0, dt, n = ref_decks.tiltedv_kin()
+    opts = dataclasses.replace(opts, use_jacobian=True, pc_matrix="FullJacobian")
+    print("tiltedv UseJacobian True FullJacobian", flush=True)
+    worst, lg, lo = _compare_runs(prob, opts, p0, dt, n)
+    # the full PC matrix cuts the linear iterations of the PFSymmetric run
+    sym = dataclasses.replace(opts, pc_matrix="PFSymmetric")
+    r, _, _, los = O.advance(O.OracleProblem(prob), sym, p0, 0.0, dt, n)
+    assert int(np.sum(lg["gmres"])) < 0.5 * int(np.sum(los.gmres[:n]))
+    api = _api()
+    gp = api.GPUProblem(prob)
+    ph, sh, log = api.GPUSolver(gp, opts).advance(p0, 0.0, dt, n)
+    g = np.load(os.path.join(HERE, "golden", "tiltedv_overlandkin.npz"))
+    for d in range(21):
+        s = max(2 * d - 1, 0)
+        a, b = prob.interior(ph[s]), g["press"][d]
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-7, d
+
+
+@pytest.mark.parametrize("use_jac", [False, True])
+def test_kinsol_full_jacobian_pc_hetero_overland(gpu, use_jac):
+    """PCMatrixType FullJacobian on a heterogeneous overland problem: the
+    plain device path against the oracle step for step, and the mixed path
+    (FGMRES + FP32 full-storage V-cycle) within the solver tolerance of
+    the plain one."""
+    api = _api()
+    prob = cases.hetero(20, 16, 8, seed=23, overland=True)
+    opts = cases.solver(use_jacobian=use_jac, residual_tol=1e-8, krylov_dim=25,
+                        precond="PFMG", pc_matrix="FullJacobian")
+    p0 = prob.hydrostatic(0.5, 0.0)
+    print(f"advance hetero overland FullJacobian use_jac={use_jac}", flush=True)
+    _compare_runs(prob, opts, p0, 0.1, 4)
+    gp = api.GPUProblem(prob)
+    tight = cases.solver(use_jacobian=use_jac, residual_tol=1e-12, krylov_dim=25,
+                         precond="PFMG", pc_matrix="FullJacobian")
+    pp, _, _ = api.GPUSolver(gp, tight, 64, 0).advance(p0, 0.0, 0.1, 3)
+    pm, _, _ = api.GPUSolver(gp, tight, 32, 1).advance(p0, 0.0, 0.1, 3)
+    for s in range(4):
+        a, b = _I(prob, pm[s]), _I(prob, pp[s])
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-8, s
 
 
 def test_kinsol_plain_matches_oracle_default_richards(gpu):

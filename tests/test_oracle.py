@@ -78,6 +78,23 @@ def test_tiltedv_overlandkin_jacobian_true_golden():
         assert _rel(prob.interior(ph[s]), g["press"][d], 1e-3) < 1e-7, d
 
 
+def test_tiltedv_overlandkin_jacobian_true_nonsymmetric_pc_golden():
+    """The "Jacobian True Nonsymmetric Preconditioner" variant
+    (overland_tiltedV_KWE.py:417-440: UseJacobian True, PCMatrixType
+    FullJacobian), checked against the same correct_output: the
+    preconditioner matrix is the full Jacobian with the kinematic surface
+    couplings (PL/pf_hypre.c:355-520) in a full-storage hierarchy."""
+    prob, opts, p0, dt, n = ref_decks.tiltedv_kin()
+    opts = dataclasses.replace(opts, use_jacobian=True, pc_matrix="FullJacobian")
+    op = O.OracleProblem(prob)
+    r, ph, sh, log = O.advance(op, opts, p0, 0.0, dt, n)
+    assert r == 0
+    g = np.load(os.path.join(HERE, "golden", "tiltedv_overlandkin.npz"))
+    for d in range(21):
+        s = max(2 * d - 1, 0)
+        assert _rel(prob.interior(ph[s]), g["press"][d], 1e-3) < 1e-7, d
+
+
 # --- closures --------------------------------------------------------------------
 def test_vg_closed_forms():
     prob = cases.hetero(4, 3, 3, seed=3)
@@ -341,6 +358,166 @@ def test_mg_is_convergent_solver_on_poisson():
         rn.append(np.linalg.norm(rr[sl]))
     factors = np.array(rn[1:]) / np.array(rn[:-1])
     assert np.all(factors[1:] < 0.3), factors
+
+
+def _full_operator(prob, seed, symmetric=False, vel=(0.0, 0.0, 0.0)):
+    """Random 7-point operator in full storage (C, W, E, S, N, L, U), ghosted,
+    strictly diagonally dominant, couplings that leave the grid zero.  With
+    ``symmetric`` the lower couplings are the neighbours' upper ones; ``vel``
+    adds upwinded convection (a non-symmetric M-matrix)."""
+    rng = np.random.default_rng(seed)
+    shp = prob.shape
+    sl = (slice(1, -1),) * 3
+    A = [np.zeros(shp) for _ in range(7)]
+    for q in (2, 4, 6):
+        A[q][sl] = -rng.uniform(0.5, 1.5, size=(prob.nz, prob.ny, prob.nx))
+    A[2][:, :, -2] = 0
+    A[4][:, -2, :] = 0
+    A[6][-2, :, :] = 0
+    if symmetric:
+        A[1] = np.roll(A[2], 1, axis=2)
+        A[3] = np.roll(A[4], 1, axis=1)
+        A[5] = np.roll(A[6], 1, axis=0)
+    else:
+        for q in (1, 3, 5):
+            A[q][sl] = -rng.uniform(0.5, 1.5, size=(prob.nz, prob.ny, prob.nx))
+    for ax, v in enumerate(vel):
+        lo, hi = (1, 2) if ax == 0 else ((3, 4) if ax == 1 else (5, 6))
+        if v > 0:
+            A[lo][sl] -= v
+        elif v < 0:
+            A[hi][sl] += v
+    A[1][:, :, 1] = 0
+    A[3][:, 1, :] = 0
+    A[5][1, :, :] = 0
+    A[0][sl] = sum(np.abs(A[q][sl]) for q in range(1, 7)) + 0.1
+    return A
+
+
+def test_full_mg_reduces_to_symmetric_mg():
+    """A symmetric matrix in full storage gives the symmetric hierarchy: the
+    same level operators (lower couplings = the neighbours' upper ones) and
+    the same V-cycle output."""
+    from parflow_amd.problem import Problem
+    prob = Problem(nx=9, ny=7, nz=6, dx=1.0, dy=1.5, dz=0.7)
+    A = _full_operator(prob, 21, symmetric=True)
+    ms = O.MG(prob, 1, 1, 0)
+    mf = O.MG(prob, 1, 1, 0, full=True)
+    ms.setup(A)
+    mf.setup(A)
+    assert len(ms.levels()) > 4
+    for l in range(len(ms.levels())):
+        Ls, Lf = ms.level_matrix(l), mf.level_matrix(l)
+        for q in range(4):
+            assert np.allclose(Lf[q], Ls[q], rtol=1e-13, atol=1e-13), (l, q)
+        assert np.allclose(Lf[4], np.roll(Ls[1], 1, axis=2), rtol=1e-13, atol=1e-13), l
+        assert np.allclose(Lf[5], np.roll(Ls[2], 1, axis=1), rtol=1e-13, atol=1e-13), l
+        assert np.allclose(Lf[6], np.roll(Ls[3], 1, axis=0), rtol=1e-13, atol=1e-13), l
+    rng = np.random.default_rng(22)
+    b = np.zeros(prob.shape)
+    b[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+    assert _rel(mf.apply(b), ms.apply(b), 1e-3) < 1e-12
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+@pytest.mark.parametrize("full", [False, True])
+def test_mg_coarse_operator_is_galerkin_on_a_line(d, full):
+    """On a line (one coarsening direction, no off-direction couplings) the
+    coarse operator is exactly P^T A P with the operator-induced
+    prolongation P (injection at even points, -a_L/a_0 and -a_U/a_0 at odd
+    ones): brute force with dense matrices, a non-symmetric A in full
+    storage, a symmetric one otherwise."""
+    from parflow_amd.problem import Problem
+    n = 9
+    dims = [1, 1, 1]
+    dims[d] = n
+    prob = Problem(nx=dims[0], ny=dims[1], nz=dims[2], dx=1.0, dy=1.0, dz=1.0)
+    A = _full_operator(prob, 30 + d, symmetric=not full)
+    line = lambda a: a[1:-1, 1:-1, 1:-1].reshape(-1)
+    c, lo, up = line(A[0]), line(A[1 + 2 * d]), line(A[2 + 2 * d])
+    Ad = np.diag(c) + np.diag(up[:-1], 1) + np.diag(lo[1:], -1)
+    nc = (n + 1) // 2
+    P = np.zeros((n, nc))
+    for f in range(n):
+        if f % 2 == 0:
+            P[f, f // 2] = 1.0
+        else:
+            P[f, (f - 1) // 2] = -lo[f] / c[f]
+            if f + 1 < n:
+                P[f, (f + 1) // 2] = -up[f] / c[f]
+    Ac = P.T @ Ad @ P
+    mg = O.MG(prob, 1, 1, 0, full=full)
+    mg.setup(A)
+    assert mg.levels()[1][0][d] == nc
+    L1 = [line(a) for a in mg.level_matrix(1)]
+    assert np.allclose(L1[0], np.diag(Ac), rtol=1e-13)
+    assert np.allclose(L1[1 + d][:-1], np.diag(Ac, 1), rtol=1e-13)
+    assert L1[1 + d][-1] == 0.0
+    if full:
+        assert np.allclose(L1[4 + d][1:], np.diag(Ac, -1), rtol=1e-13)
+        assert L1[4 + d][0] == 0.0
+    # everything outside the tridiagonal band of Ac is zero
+    assert np.count_nonzero(np.triu(Ac, 2)) == 0 and np.count_nonzero(np.tril(Ac, -2)) == 0
+
+
+def test_full_mg_off_direction_collapse_orientation():
+    """Off-direction couplings are collapsed from the three fine rows a
+    coarse point stands for (weights 1, 1/2, 1/2), each side from its own
+    side: with couplings that do not vary along the coarsening direction,
+    an interior coarse point's north and south couplings are exactly twice
+    the fine ones, and they stay apart when north != south."""
+    from parflow_amd.problem import Problem
+    prob = Problem(nx=9, ny=5, nz=1, dx=1.0, dy=4.0, dz=1.0)
+    A = _full_operator(prob, 40)
+    rng = np.random.default_rng(41)
+    nrow = rng.uniform(-1.5, -0.5, size=prob.ny)
+    srow = rng.uniform(-3.0, -2.0, size=prob.ny)
+    A[4][1, 1:-1, 1:-1] = nrow[:, None]
+    A[3][1, 1:-1, 1:-1] = srow[:, None]
+    A[4][:, -2, :] = 0
+    A[3][:, 1, :] = 0
+    mg = O.MG(prob, 1, 1, 0, full=True)
+    mg.setup(A)
+    assert mg.levels()[0][1] == 0 and mg.levels()[1][0] == (5, 5, 1)
+    L1 = mg.level_matrix(1)
+    inner = (1, slice(1, -1), slice(2, -2))       # coarse i = 1 .. 3
+    assert np.allclose(L1[2][inner], 2 * A[4][1, 1:-1, 1:4], rtol=1e-14)
+    assert np.allclose(L1[5][inner], 2 * A[3][1, 1:-1, 1:4], rtol=1e-14)
+
+
+def test_full_mg_is_convergent_on_convection_diffusion():
+    """Repeated V-cycles of the full-storage hierarchy on a non-symmetric
+    operator (random couplings plus upwinded convection) contract the
+    residual per cycle, while the hierarchy of its symmetric part (what
+    PFSymmetric storage keeps of it) makes the same iteration diverge."""
+    from parflow_amd.problem import Problem
+    prob = Problem(nx=16, ny=12, nz=10, dx=1.0, dy=1.0, dz=0.5)
+    A = _full_operator(prob, 50, vel=(0.25, -0.125, 0.0625))
+    S = [A[0], None, 0.5 * (A[2] + np.roll(A[1], -1, axis=2)), None,
+         0.5 * (A[4] + np.roll(A[3], -1, axis=1)), None,
+         0.5 * (A[6] + np.roll(A[5], -1, axis=0))]
+    S = [a if a is not None else np.zeros(prob.shape) for a in S]
+    op = O.OracleProblem(prob)
+    sl = (slice(1, -1),) * 3
+    rng = np.random.default_rng(51)
+    b = np.zeros(prob.shape)
+    b[sl] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+
+    def factors(mg):
+        x = np.zeros(prob.shape)
+        rn = [np.linalg.norm(b)]
+        for it in range(6):
+            x = x + mg.apply(b - O.matvec(op, A, 0, x))
+            rn.append(np.linalg.norm((b - O.matvec(op, A, 0, x))[sl]))
+        return np.array(rn[1:]) / np.array(rn[:-1])
+
+    mf = O.MG(prob, 1, 1, 0, full=True)
+    mf.setup(A)
+    f = factors(mf)
+    assert np.all(f < 0.5), f
+    ms = O.MG(prob, 1, 1, 0)
+    ms.setup(S)
+    assert factors(ms)[-1] > 1.0
 
 
 def test_gmres_matches_direct_solve():

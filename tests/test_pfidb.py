@@ -232,7 +232,7 @@ def test_pfb_file_fields(tmp_path):
     ("Wells.Names", "w1"),
     ("Solver.TerrainFollowingGrid", True),
     ("Patch.top.BCPressure.Type", "SeepageFace"),
-    ("Solver.Linear.Preconditioner.PCMatrixType", "FullJacobian"),
+    ("Solver.Linear.Preconditioner.PCMatrixType", "Picard"),
     ("Geom.domain.RelPerm.N", 3.0),
     ("PhaseSources.water.Geom.background.Value", 1.0),
 ])
@@ -290,3 +290,17 @@ def test_vg_table_keys():
         keys[f"Geom.{regions[0]}.RelPerm.NumSamplePoints"] = "100"
         with pytest.raises(ValueError):
             pfidb.problem_from_pfidb(keys)
+
+
+def test_pc_matrix_type_key():
+    """Solver.Linear.Preconditioner.PCMatrixType (kinsol_pc.c:215-240):
+    PFSymmetric by default, FullJacobian selects the full-storage
+    hierarchy (SolverOptions.pc_matrix_code 1 in the C options)."""
+    keys = {k: pfidb._str(v) for k, v in default_richards_keys().items()}
+    prob, opts, p0, dt, n = pfidb.problem_from_pfidb(keys)
+    assert opts.pc_matrix == "PFSymmetric" and opts.pc_matrix_code() == 0
+    keys["Solver.Linear.Preconditioner.PCMatrixType"] = "FullJacobian"
+    prob, opts, p0, dt, n = pfidb.problem_from_pfidb(keys)
+    assert opts.pc_matrix == "FullJacobian" and opts.pc_matrix_code() == 1
+    from parflow_amd import api
+    assert api.solver_opts_c(opts).pc_matrix == 1
