@@ -27,13 +27,17 @@
  * reference's own FullJacobian preconditioner is hypre PFMG (not vendored):
  * this hierarchy is the build's replacement for it, as for PFSymmetric.
  *
- * Parity status: parity unpinned for the V-cycle output itself.  The
- * reference's default PFMG preconditioner lives in hypre (not vendored) and
- * no reference test dumps an MGSemi iterate.  What pins this file: the level
- * hierarchy against the coarsening rule of PL/mg_semi.c:832-900, the cycle
- * as a convergent solver on a 7-point Poisson operator (error contraction
- * per cycle well below 1, exact on a 1-point coarsest level), and the
- * end-to-end Newton solves reproducing the reference's goldens
+ * Parity status.  The reference's default PFMG preconditioner lives in
+ * hypre (not vendored) and no reference test dumps an MGSemi iterate, so
+ * the V-cycle is pinned by what the mathematics fixes: the level hierarchy
+ * against the coarsening rule of PL/mg_semi.c:832-900; on a line, one
+ * V-cycle from a zero guess solves A x = b to round-off (semicoarsening
+ * with operator-induced weights is cyclic reduction there), which fixes
+ * the coarse operators, weights, prolongation and the coarsest solve; with
+ * MGSemi's palindromic red-black-red sweeps the cycle is a symmetric
+ * operator, which fixes the restriction as the prolongation's transpose;
+ * the cycle is a convergent solver on a 7-point Poisson operator; and the
+ * end-to-end Newton solves reproduce the reference's goldens
  * (test/correct_output/default_richards.out.*, overland_tiltedV_KWE).
  */
 #include <stdlib.h>

@@ -70,3 +70,37 @@ def hetero_src(prob, seed):
     src[1:-1, 1:-1, 1:-1] = np.random.default_rng(seed).uniform(
         -0.01, 0.02, (prob.nz, prob.ny, prob.nx))
     return src
+
+
+def full_operator(prob, seed, symmetric=False, vel=(0.0, 0.0, 0.0)):
+    """Random 7-point operator in full storage (C, W, E, S, N, L, U), ghosted,
+    strictly diagonally dominant, couplings that leave the grid zero.  With
+    ``symmetric`` the lower couplings are the neighbours' upper ones; ``vel``
+    adds upwinded convection (a non-symmetric M-matrix)."""
+    rng = np.random.default_rng(seed)
+    shp = prob.shape
+    sl = (slice(1, -1),) * 3
+    A = [np.zeros(shp) for _ in range(7)]
+    for q in (2, 4, 6):
+        A[q][sl] = -rng.uniform(0.5, 1.5, size=(prob.nz, prob.ny, prob.nx))
+    A[2][:, :, -2] = 0
+    A[4][:, -2, :] = 0
+    A[6][-2, :, :] = 0
+    if symmetric:
+        A[1] = np.roll(A[2], 1, axis=2)
+        A[3] = np.roll(A[4], 1, axis=1)
+        A[5] = np.roll(A[6], 1, axis=0)
+    else:
+        for q in (1, 3, 5):
+            A[q][sl] = -rng.uniform(0.5, 1.5, size=(prob.nz, prob.ny, prob.nx))
+    for ax, v in enumerate(vel):
+        lo, hi = (1, 2) if ax == 0 else ((3, 4) if ax == 1 else (5, 6))
+        if v > 0:
+            A[lo][sl] -= v
+        elif v < 0:
+            A[hi][sl] += v
+    A[1][:, :, 1] = 0
+    A[3][:, 1, :] = 0
+    A[5][1, :, :] = 0
+    A[0][sl] = sum(np.abs(A[q][sl]) for q in range(1, 7)) + 0.1
+    return A

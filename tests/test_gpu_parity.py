@@ -417,6 +417,42 @@ def test_multigrid(gpu, precision):
         assert _relmax(_I(prob, xg), _I(prob, xo)) < tol
 
 
+@pytest.mark.parametrize("d,n", [(0, 32), (0, 33), (1, 64), (2, 40)])
+def test_multigrid_exact_on_a_line(gpu, d, n):
+    """The device V-cycle (FP64, both sweep orders) solves a symmetric line
+    operator exactly in one cycle, as cyclic reduction does; the same
+    hierarchy's V-cycle on a 3-D SPD matrix with MGSemi's palindromic
+    sweeps is a symmetric operator."""
+    api = _api()
+    from parflow_amd.problem import Problem
+    dims = [1, 1, 1]
+    dims[d] = n
+    prob = Problem(nx=dims[0], ny=dims[1], nz=dims[2], dx=1.0, dy=1.0, dz=1.0)
+    print(f"mg line d={d} n={n}", flush=True)
+    A = cases.full_operator(prob, 80 + d, symmetric=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    b = np.zeros(prob.shape)
+    b[1:-1, 1:-1, 1:-1] = np.random.default_rng(6).normal(size=(prob.nz, prob.ny, prob.nx))
+    for sym, sweeps in ((1, 1), (0, 2)):
+        mg = api.GPUMultigrid(gp, sweeps, sweeps, sym, 64)
+        mg.setup(A)
+        r = b - O.matvec(op, A, 0, mg.apply(b))
+        assert np.linalg.norm(r[1:-1, 1:-1, 1:-1]) < 1e-13 * np.linalg.norm(b), (sym, sweeps)
+    prob = Problem(nx=40, ny=18, nz=12, dx=1.0, dy=1.5, dz=0.7)
+    A = cases.full_operator(prob, 90, symmetric=True)
+    gp = api.GPUProblem(prob)
+    mg = api.GPUMultigrid(gp, 1, 1, 1, 64)
+    mg.setup(A)
+    rng = np.random.default_rng(7)
+    u = np.zeros(prob.shape)
+    w = np.zeros(prob.shape)
+    u[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+    w[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+    a, c = (u * mg.apply(w)).sum(), (w * mg.apply(u)).sum()
+    assert abs(a - c) < 1e-12 * abs(a)
+
+
 @pytest.mark.parametrize("shape", [(16, 12, 10), (70, 40, 24)])
 @pytest.mark.parametrize("precision", [64, 32])
 def test_multigrid_full_storage(gpu, precision, shape):

@@ -360,38 +360,54 @@ def test_mg_is_convergent_solver_on_poisson():
     assert np.all(factors[1:] < 0.3), factors
 
 
-def _full_operator(prob, seed, symmetric=False, vel=(0.0, 0.0, 0.0)):
-    """Random 7-point operator in full storage (C, W, E, S, N, L, U), ghosted,
-    strictly diagonally dominant, couplings that leave the grid zero.  With
-    ``symmetric`` the lower couplings are the neighbours' upper ones; ``vel``
-    adds upwinded convection (a non-symmetric M-matrix)."""
-    rng = np.random.default_rng(seed)
-    shp = prob.shape
+@pytest.mark.parametrize("d", [0, 1, 2])
+@pytest.mark.parametrize("n", [17, 32, 33])
+@pytest.mark.parametrize("sym,sweeps", [(1, 1), (0, 1), (0, 2)])
+def test_mg_vcycle_is_exact_on_a_line(d, n, sym, sweeps):
+    """On a line the semicoarsening V-cycle is cyclic reduction: red = odd
+    points are the fine points, the operator-induced weights make the
+    coarse operator the Schur complement, and the 1-point coarsest level is
+    solved exactly, so ONE V-cycle from a zero guess solves A x = b to
+    round-off, for both sweep orders.  A dropped coarse-operator term, a
+    wrong weight or a wrong prolongation index leaves a residual of O(1)."""
+    from parflow_amd.problem import Problem
+    dims = [1, 1, 1]
+    dims[d] = n
+    prob = Problem(nx=dims[0], ny=dims[1], nz=dims[2], dx=1.0, dy=1.0, dz=1.0)
+    A = cases.full_operator(prob, 60 + d, symmetric=True)
+    mg = O.MG(prob, sweeps, sweeps, sym)
+    mg.setup(A)
+    b = np.zeros(prob.shape)
+    b[1:-1, 1:-1, 1:-1] = np.random.default_rng(5).normal(size=(prob.nz, prob.ny, prob.nx))
+    r = b - O.matvec(O.OracleProblem(prob), A, 0, mg.apply(b))
+    assert np.linalg.norm(r[1:-1, 1:-1, 1:-1]) < 1e-14 * np.linalg.norm(b)
+
+
+def test_mg_vcycle_with_palindromic_sweeps_is_symmetric():
+    """MGSemi's smoother (Symmetric True) sweeps red-black-red before and
+    after the coarse correction, a palindrome, so with restriction = the
+    transpose of prolongation and symmetric coarse operators the V-cycle
+    is a symmetric operator: u.M(w) = w.M(u) to round-off on a 3-D SPD
+    matrix.  Swapped restriction weights break it (7e-3 in a planted
+    test); the red-black sweeps of PFMG's nonsymmetric smoother are not
+    palindromic, and there it does not hold."""
+    from parflow_amd.problem import Problem
+    prob = Problem(nx=9, ny=7, nz=6, dx=1.0, dy=1.5, dz=0.7)
+    A = cases.full_operator(prob, 70, symmetric=True)
+    rng = np.random.default_rng(5)
     sl = (slice(1, -1),) * 3
-    A = [np.zeros(shp) for _ in range(7)]
-    for q in (2, 4, 6):
-        A[q][sl] = -rng.uniform(0.5, 1.5, size=(prob.nz, prob.ny, prob.nx))
-    A[2][:, :, -2] = 0
-    A[4][:, -2, :] = 0
-    A[6][-2, :, :] = 0
-    if symmetric:
-        A[1] = np.roll(A[2], 1, axis=2)
-        A[3] = np.roll(A[4], 1, axis=1)
-        A[5] = np.roll(A[6], 1, axis=0)
-    else:
-        for q in (1, 3, 5):
-            A[q][sl] = -rng.uniform(0.5, 1.5, size=(prob.nz, prob.ny, prob.nx))
-    for ax, v in enumerate(vel):
-        lo, hi = (1, 2) if ax == 0 else ((3, 4) if ax == 1 else (5, 6))
-        if v > 0:
-            A[lo][sl] -= v
-        elif v < 0:
-            A[hi][sl] += v
-    A[1][:, :, 1] = 0
-    A[3][:, 1, :] = 0
-    A[5][1, :, :] = 0
-    A[0][sl] = sum(np.abs(A[q][sl]) for q in range(1, 7)) + 0.1
-    return A
+    u = np.zeros(prob.shape)
+    w = np.zeros(prob.shape)
+    u[sl] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+    w[sl] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+    mg = O.MG(prob, 1, 1, 1)
+    mg.setup(A)
+    a, b = (u * mg.apply(w)).sum(), (w * mg.apply(u)).sum()
+    assert abs(a - b) < 1e-13 * abs(a)
+    mg = O.MG(prob, 1, 1, 0)
+    mg.setup(A)
+    a, b = (u * mg.apply(w)).sum(), (w * mg.apply(u)).sum()
+    assert abs(a - b) > 1e-3 * abs(a)
 
 
 def test_full_mg_reduces_to_symmetric_mg():
@@ -400,7 +416,7 @@ def test_full_mg_reduces_to_symmetric_mg():
     the same V-cycle output."""
     from parflow_amd.problem import Problem
     prob = Problem(nx=9, ny=7, nz=6, dx=1.0, dy=1.5, dz=0.7)
-    A = _full_operator(prob, 21, symmetric=True)
+    A = cases.full_operator(prob, 21, symmetric=True)
     ms = O.MG(prob, 1, 1, 0)
     mf = O.MG(prob, 1, 1, 0, full=True)
     ms.setup(A)
@@ -432,7 +448,7 @@ def test_mg_coarse_operator_is_galerkin_on_a_line(d, full):
     dims = [1, 1, 1]
     dims[d] = n
     prob = Problem(nx=dims[0], ny=dims[1], nz=dims[2], dx=1.0, dy=1.0, dz=1.0)
-    A = _full_operator(prob, 30 + d, symmetric=not full)
+    A = cases.full_operator(prob, 30 + d, symmetric=not full)
     line = lambda a: a[1:-1, 1:-1, 1:-1].reshape(-1)
     c, lo, up = line(A[0]), line(A[1 + 2 * d]), line(A[2 + 2 * d])
     Ad = np.diag(c) + np.diag(up[:-1], 1) + np.diag(lo[1:], -1)
@@ -468,7 +484,7 @@ def test_full_mg_off_direction_collapse_orientation():
     the fine ones, and they stay apart when north != south."""
     from parflow_amd.problem import Problem
     prob = Problem(nx=9, ny=5, nz=1, dx=1.0, dy=4.0, dz=1.0)
-    A = _full_operator(prob, 40)
+    A = cases.full_operator(prob, 40)
     rng = np.random.default_rng(41)
     nrow = rng.uniform(-1.5, -0.5, size=prob.ny)
     srow = rng.uniform(-3.0, -2.0, size=prob.ny)
@@ -492,7 +508,7 @@ def test_full_mg_is_convergent_on_convection_diffusion():
     PFSymmetric storage keeps of it) makes the same iteration diverge."""
     from parflow_amd.problem import Problem
     prob = Problem(nx=16, ny=12, nz=10, dx=1.0, dy=1.0, dz=0.5)
-    A = _full_operator(prob, 50, vel=(0.25, -0.125, 0.0625))
+    A = cases.full_operator(prob, 50, vel=(0.25, -0.125, 0.0625))
     S = [A[0], None, 0.5 * (A[2] + np.roll(A[1], -1, axis=2)), None,
          0.5 * (A[4] + np.roll(A[3], -1, axis=1)), None,
          0.5 * (A[6] + np.roll(A[5], -1, axis=0))]
