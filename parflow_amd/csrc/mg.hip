@@ -328,6 +328,81 @@ __global__ __launch_bounds__(256) void k_coarse_fused(LGeom gf, LGeom gc, Mat<T>
   coarse_point<T, FULL>(gf, gc, A, Ac, d, o1, o2, p1, p2, i, j, k);
 }
 
+// Coarse operators of a z-semicoarsened level (symmetric storage) in one
+// z-marching pass: a thread owns an (i, j) column of 64 x 4 column tiles
+// and walks the coarse planes upwards.  The odd fine plane between coarse
+// points C and C+2 is read once, as the plane above C (its weights and
+// couplings), and its values are carried as the plane below C+2, so a
+// coarse point loads two fine planes (12 values) instead of coarse_point's
+// four.  The next point's loads are issued before the current point is
+// computed.  The expressions and their order are coarse_point's for d = 2
+// (o1 = y, o2 = x), so the result is the same.
+template <typename T> struct CzPlane { T c, ux, uxw, uy, uys, uz; };
+
+template <typename T>
+__device__ __forceinline__ void cz_load(CzPlane<T> &P, const Mat<T> &A, long ip, long sy)
+{
+  P.c = A.c[ip];
+  P.ux = A.u[0][ip]; P.uxw = A.u[0][ip - 1];
+  P.uy = A.u[1][ip]; P.uys = A.u[1][ip - sy];
+  P.uz = A.u[2][ip];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_z(LGeom gf, LGeom gc, Mat<T> A, Mat<T> Ac,
+                                                  T *__restrict__ p1, T *__restrict__ p2)
+{
+  const int gx = (gc.n[0] + 63) >> 6, gy = (gc.n[1] + 3) >> 2;
+  const int nb = gx * gy, nb8 = (nb + 7) & ~7;
+  const int b = blockIdx.x;
+  const int L = (b & 7) * (nb8 >> 3) + (b >> 3);
+  if (L >= nb) return;
+  const int i = (L % gx) * 64 + threadIdx.x, j = (L / gx) * 4 + threadIdx.y;
+  if (i >= gc.n[0] || j >= gc.n[1]) return;
+  const long sy = gf.s[1], sz = gf.s[2];
+  const int nfz = gf.n[2], ncz = gc.n[2];
+  // values of the odd plane below (the ghost plane below the first point:
+  // every level array's ghost entries are zero)
+  T lo_uy = T(0), lo_uys = T(0), lo_ux = T(0), lo_uxw = T(0), lo_uz = T(0), p2lo = T(0);
+  long iA = LIDX(gf, i, j, 0);
+  long iAc = LIDX(gc, i, j, 0);
+  CzPlane<T> e, o, en, on;
+  cz_load(e, A, iA, sy);
+  cz_load(o, A, iA + sz, sy);            // the ghost plane above when nfz == 1
+  for (int kc = 0; kc < ncz; kc++) {
+    const int fk = 2 * kc;
+    if (kc + 1 < ncz) {
+      cz_load(en, A, iA + 2 * sz, sy);
+      cz_load(on, A, iA + 3 * sz, sy);   // at most the ghost plane above
+    }
+    const bool hi = fk + 1 < nfz;
+    // weights of the odd point above (pw_calc)
+    T w1 = T(0), w2 = T(0);
+    if (hi) {
+      T ap0 = o.c + o.uys + o.uy + o.uxw + o.ux;
+      if (ap0 != T(0)) { w1 = -e.uz / ap0; w2 = -o.uz / ap0; }
+      p1[iA + sz] = w1; p2[iA + sz] = w2;
+    }
+    const T p1hi = w1;
+    T up1 = e.uy + T(0.5) * lo_uy + T(0.5) * o.uy;
+    T up2 = e.ux + T(0.5) * lo_ux + T(0.5) * o.ux;
+    T c4 = (j == gc.n[1] - 1) ? T(0) : up1;
+    T c6 = (i == gc.n[0] - 1) ? T(0) : up2;
+    Ac.u[1][iAc] = c4;
+    Ac.u[0][iAc] = c6;
+    Ac.u[2][iAc] = (kc + 1 < ncz) ? o.uz * p1hi : T(0);
+    T cc = e.c + e.uys + e.uy + e.uxw + e.ux + lo_uz * p2lo + e.uz * p1hi;
+    T c3 = (j > 0) ? e.uys + T(0.5) * lo_uys + T(0.5) * o.uys : T(0);
+    T c5 = (i > 0) ? e.uxw + T(0.5) * lo_uxw + T(0.5) * o.uxw : T(0);
+    Ac.c[iAc] = cc - (c3 + c4 + c5 + c6);
+    lo_uy = o.uy; lo_uys = o.uys; lo_ux = o.ux; lo_uxw = o.uxw; lo_uz = o.uz;
+    p2lo = w2;
+    e = en; o = on;
+    iA += 2 * sz;
+    iAc += gc.s[2];
+  }
+}
+
 template <typename T, bool FULL>
 __global__ __launch_bounds__(256) void k_rbgs(LGeom g, Mat<T> A, T *__restrict__ x,
                                               const T *__restrict__ b, int par, int init)
@@ -1013,6 +1088,7 @@ struct pfhip_mg {
                                // the ghosted index of an even x is vector aligned
   int fuse;                    // red -> black pairs as one launch (k_rbgs_pair)
   int rrz;                     // fused level residual + z restriction
+  int cz;                      // z-marching coarse operators (k_coarse_z)
   int pv[MAXL];                // pair tile shape of each grid level (tune_pairs)
   int tuned;                   // pv chosen (at the first V-cycle)
   void *levdev;                // device LevelDev<T>[L]
@@ -1101,6 +1177,8 @@ static int mg_create(pfhip_problem *pb, int max_levels, int pre, int post, int s
   mg->fuse = (fe && *fe) ? atoi(fe) : 1;
   const char *fr = getenv("PFHIP_MG_RRZ");
   mg->rrz = (fr && *fr) ? atoi(fr) : 1;
+  const char *fc = getenv("PFHIP_MG_CZ");
+  mg->cz = (fc && *fc) ? atoi(fc) : 1;
   long small = MG_SMALL_DEFAULT;
   const char *env = getenv("PFHIP_MG_SMALL");
   if (env && *env) small = atol(env);
@@ -1187,7 +1265,11 @@ template <typename T> static int mg_coarse_setup_t(pfhip_mg *mg, hipStream_t st)
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
     int d = mg->dir[l], o1, o2;
     offdirs(d, &o1, &o2);
-    if (mg->full)
+    if (!mg->full && mg->cz && d == 2) {
+      const unsigned nb = (unsigned)((gc.n[0] + 63) / 64) * ((gc.n[1] + 3) / 4);
+      hipLaunchKernelGGL(k_coarse_z<T>, dim3((nb + 7u) & ~7u), LBLOCK, 0, st, gf, gc,
+                         matof<T>(mg, l), matof<T>(mg, l + 1), (T *)mg->p1[l], (T *)mg->p2[l]);
+    } else if (mg->full)
       hipLaunchKernelGGL((k_coarse_fused<T, true>), dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc,
                          matof<T>(mg, l), matof<T>(mg, l + 1), d, o1, o2, (T *)mg->p1[l],
                          (T *)mg->p2[l]);

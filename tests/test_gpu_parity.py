@@ -495,11 +495,13 @@ def test_multigrid_grid_levels(gpu, precision, fuse, shape, monkeypatch):
     x 16), and the 64 x 4 tiles of the other kernels are ragged too.  fuse
     also selects the level residual fused with the z restriction (z-chunks
     of 16 fine planes, ragged and odd plane counts on the coarser levels)
-    or the two separate kernels."""
+    or the two separate kernels, and the z-marching coarse operators of
+    the z-coarsened levels or the point-wise ones."""
     api = _api()
     monkeypatch.setenv("PFHIP_MG_SMALL", "300")
     monkeypatch.setenv("PFHIP_MG_FUSE", str(fuse))
     monkeypatch.setenv("PFHIP_MG_RRZ", str(fuse))
+    monkeypatch.setenv("PFHIP_MG_CZ", str(fuse))
     prob = cases.hetero(*shape, seed=17, dir_sides=True)
     print(f"mg grid levels precision={precision} fuse={fuse} shape={shape}", flush=True)
     gp = api.GPUProblem(prob)
