@@ -863,6 +863,55 @@ __global__ __launch_bounds__(256) void k_prolong_add(LGeom gf, LGeom gc, int d,
   prolong_point(gf, gc, d, ec, xf, p1, p2, i, j, k);
 }
 
+// x_f += P e_c on a z-coarsened level as a z-march over (i, j) columns (64 x
+// 4 column tiles, XCD-aware): each coarse value is loaded once and used by
+// the even plane it is injected into and the two odd planes beside it; the
+// next planes' loads are issued before the current ones are added.  Same
+// expressions as prolong_point.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong_z(LGeom gf, LGeom gc, const T *__restrict__ ec,
+                                                   T *__restrict__ xf,
+                                                   const T *__restrict__ p1,
+                                                   const T *__restrict__ p2)
+{
+  const int gx = (gf.n[0] + 63) >> 6, gy = (gf.n[1] + 3) >> 2;
+  const int nb = gx * gy, nb8 = (nb + 7) & ~7;
+  const int b = blockIdx.x;
+  const int L = (b & 7) * (nb8 >> 3) + (b >> 3);
+  if (L >= nb) return;
+  const int i = (L % gx) * 64 + threadIdx.x, j = (L / gx) * 4 + threadIdx.y;
+  if (i >= gf.n[0] || j >= gf.n[1]) return;
+  const long sf = gf.s[2], sc = gc.s[2];
+  const int nfz = gf.n[2], ncz = gc.n[2];
+  long ip = LIDX(gf, i, j, 0);
+  long ic = LIDX(gc, i, j, 0);
+  // plane pair (2kc, 2kc+1): coarse values kc and kc+1 (the ghost plane,
+  // zero, above the last), fine x of both planes, weights of the odd one
+  T e0 = ec[ic], e1 = ec[ic + sc];
+  T xe = xf[ip], xo = xf[ip + sf], w1 = p1[ip + sf], w2 = p2[ip + sf];
+  for (int kc = 0; kc < ncz; kc++) {
+    const int fk = 2 * kc;
+    T ne1 = T(0), nxe = T(0), nxo = T(0), nw1 = T(0), nw2 = T(0);
+    if (kc + 1 < ncz) {
+      ne1 = ec[ic + 2 * sc];               // at most the ghost plane above
+      nxe = xf[ip + 2 * sf];
+      nxo = xf[ip + 3 * sf];               // at most the ghost plane above
+      nw1 = p1[ip + 3 * sf];
+      nw2 = p2[ip + 3 * sf];
+    }
+    xf[ip] = xe + e0;
+    if (fk + 1 < nfz) {
+      T lo = w1 * e0;
+      T hi = (fk + 2 < nfz) ? w2 * e1 : T(0);
+      xf[ip + sf] = xo + (lo + hi);
+    }
+    e0 = e1; e1 = ne1;
+    xe = nxe; xo = nxo; w1 = nw1; w2 = nw2;
+    ip += 2 * sf;
+    ic += sc;
+  }
+}
+
 template <typename S, typename D>
 __global__ __launch_bounds__(256) void k_convert(LGeom g, const S *__restrict__ src,
                                                  D *__restrict__ dst)
@@ -1556,9 +1605,16 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
   }
   for (l = lc - 1; l >= 0; l--) {
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
-    hipLaunchKernelGGL(k_prolong_add<T>, dim3(ntiles(gf)), LBLOCK, 0, st, gf, gc, mg->dir[l],
-                       (const T *)mg->x[l + 1], (T *)mg->x[l], (const T *)mg->p1[l],
-                       (const T *)mg->p2[l]);
+    if (mg->cz && mg->dir[l] == 2) {
+      const unsigned nb = (unsigned)((gf.n[0] + 63) / 64) * ((gf.n[1] + 3) / 4);
+      hipLaunchKernelGGL(k_prolong_z<T>, dim3((nb + 7u) & ~7u), LBLOCK, 0, st, gf, gc,
+                         (const T *)mg->x[l + 1], (T *)mg->x[l], (const T *)mg->p1[l],
+                         (const T *)mg->p2[l]);
+    } else {
+      hipLaunchKernelGGL(k_prolong_add<T>, dim3(ntiles(gf)), LBLOCK, 0, st, gf, gc, mg->dir[l],
+                         (const T *)mg->x[l + 1], (T *)mg->x[l], (const T *)mg->p1[l],
+                         (const T *)mg->p2[l]);
+    }
     LAUNCHCHK();
     if ((rc = rbgs<T>(mg, l, 0, mg->post, st))) return rc;
   }
