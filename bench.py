@@ -61,7 +61,8 @@ def kernel_bytes_per_cell(gp):
       Jacobian (PC matrix): p, alpha, n, K, porosity, Ss read; C, E, N, U
                            written in the multigrid's FP32
       smoother pair (finest level, FP32): C, E, N, U, b, x_in read, x_out
-                           written"""
+                           written (the V-cycle's last pair, which writes
+                           the FP64 result instead, is not in the timed set)"""
     f = _field_reads(gp)
     res = 8 * (1 + f + 2 + 2 + 1)
     return {
@@ -82,7 +83,8 @@ PROF_KERNELS = (
      "k_residual_pf<64, 8, 16, *, 2> (line-search trial point)"),
     ("jacobian", "PROF_JACOBIAN", "k_jacobian_pf<64, 8, 16, *, true, float> (PC matrix)"),
     ("rbgs_pair", "PROF_RBGS",
-     "k_rbgs_pair2<float, TY, KC, *> (finest-level red/black pair, autotuned tile)"),
+     "k_rbgs_pair2<float, TY, KC, *, false> (finest-level red/black pair, FP32 result, "
+     "autotuned tile)"),
 )
 
 

@@ -608,11 +608,16 @@ __device__ __forceinline__ typename V2T<T>::type load_x2(const T *__restrict__ x
 // on a CU; FP64 keeps its register budget
 #define PAIR2_MIN_WAVES(T) (sizeof(T) == 4 ? 8 : 1)
 
-template <typename T, int TY, int KC, bool INIT>
+// DOUT: the pair writes its result as FP64 into dout (a vector of the
+// level's ghosted layout, no alignment assumed) instead of xout: the last
+// sweep of a V-cycle hands the preconditioned vector straight to the FP64
+// solver, without a conversion pass.
+template <typename T, int TY, int KC, bool INIT, bool DOUT = false>
 __global__ __launch_bounds__(64 * (TY + 2), PAIR2_MIN_WAVES(T)) void k_rbgs_pair2(LGeom g, Mat<T> A,
                                                               const T *__restrict__ xin,
                                                               T *__restrict__ xout,
-                                                              const T *__restrict__ b)
+                                                              const T *__restrict__ b,
+                                                              double *__restrict__ dout)
 {
   typedef typename V2T<T>::type T2;
   __shared__ T red[3][TY + 2][64];
@@ -691,7 +696,12 @@ __global__ __launch_bounds__(64 * (TY + 2), PAIR2_MIN_WAVES(T)) void k_rbgs_pair
                        sal, rm2, kk > 0, sau, rk, kk + 1 < nz);
       if (owned && kk >= kb && kk < ke) {
         T2 out = p0red ? T2{vb, rm1} : T2{rm1, vb};
-        *(T2 *)(xout + ip - sz) = out;
+        if (DOUT) {
+          dout[ip - sz] = (double)out.x;
+          dout[ip - sz + 1] = (double)out.y;
+        } else {
+          *(T2 *)(xout + ip - sz) = out;
+        }
       }
     }
     // the black point of plane k keeps its coefficients for the next step
@@ -1420,31 +1430,35 @@ static void launch_pair_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout
 
 template <typename T, int TY, int KC>
 static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
-                           int init, hipStream_t st)
+                           int init, hipStream_t st, double *dout)
 {
   unsigned gx = (g.n[0] + PX2 - 1) / PX2, gy = (g.n[1] + TY - 1) / TY, gz = (g.n[2] + KC - 1) / KC;
   unsigned nb = gx * gy * gz, nb8 = (nb + 7u) & ~7u;
   if (init)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true>), dim3(nb8), dim3(64, TY + 2, 1), 0, st,
-                       g, A, xin, xout, b);
+                       g, A, xin, xout, b, nullptr);
+  else if (dout)
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true>), dim3(nb8), dim3(64, TY + 2, 1), 0,
+                       st, g, A, xin, xout, b, dout);
   else
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false>), dim3(nb8), dim3(64, TY + 2, 1), 0, st,
-                       g, A, xin, xout, b);
+                       g, A, xin, xout, b, nullptr);
 }
 
-// shape v: (TY, KC) = (14, 16), (6, 16), (6, 32), (14, 8), (14, 4)
+// shape v: (TY, KC) = (14, 16), (6, 16), (6, 32), (14, 8), (14, 4); dout
+// (even x extents, non-initial pairs only) takes the result in FP64
 template <typename T>
 static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
-                        int init, int v, hipStream_t st)
+                        int init, int v, hipStream_t st, double *dout = nullptr)
 {
   if ((g.n[0] % 2) == 0) {
     // two points per lane (aligned vector loads need an even x extent)
     switch (v) {
-      case 1: launch_pair2_t<T, 6, 16>(g, A, xin, xout, b, init, st); return;
-      case 2: launch_pair2_t<T, 6, 32>(g, A, xin, xout, b, init, st); return;
-      case 3: launch_pair2_t<T, 14, 8>(g, A, xin, xout, b, init, st); return;
-      case 4: launch_pair2_t<T, 14, 4>(g, A, xin, xout, b, init, st); return;
-      default: launch_pair2_t<T, 14, 16>(g, A, xin, xout, b, init, st); return;
+      case 1: launch_pair2_t<T, 6, 16>(g, A, xin, xout, b, init, st, dout); return;
+      case 2: launch_pair2_t<T, 6, 32>(g, A, xin, xout, b, init, st, dout); return;
+      case 3: launch_pair2_t<T, 14, 8>(g, A, xin, xout, b, init, st, dout); return;
+      case 4: launch_pair2_t<T, 14, 4>(g, A, xin, xout, b, init, st, dout); return;
+      default: launch_pair2_t<T, 14, 16>(g, A, xin, xout, b, init, st, dout); return;
     }
   }
   switch (v) {
@@ -1503,9 +1517,12 @@ extern "C" int pfhip_mg_pair_shapes(pfhip_mg *mg, int *shape, int max)
 }
 
 // smoothing of grid level l: red -> black pairs run fused (x and its second
-// buffer swap roles after each pair), any other half-sweep runs alone
+// buffer swap roles after each pair), any other half-sweep runs alone.
+// With dout, a last half-sweep pair on an even x extent writes its result
+// into dout in FP64 (and *wrote is set); x then holds no valid iterate.
 template <typename T>
-static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
+static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
+                double *dout = nullptr, int *wrote = nullptr)
 {
   LGeom g = lgeom(mg->n[l]);
   dim3 grid(ntiles(g));
@@ -1519,10 +1536,18 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
     LAUNCHCHK();
   }
   for (int q = 0; q < n;) {
-    if (l == 0) pfhip_prof_mark(PFHIP_PROF_RBGS, 0, st);
-    if (mg->fuse && !mg->full && mg->xa[l] && q + 1 < n && seq[q].par == 1 &&
-        seq[q + 1].par == 0) {
-      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, mg->pv[l], st);
+    const bool pair = mg->fuse && !mg->full && mg->xa[l] && q + 1 < n && seq[q].par == 1 &&
+                      seq[q + 1].par == 0;
+    double *d = (pair && dout && q + 2 == n && !seq[q].init && (g.n[0] % 2) == 0) ? dout
+                                                                                 : nullptr;
+    // level 0's launches with a level-precision result are the timed ones
+    // (the FP64-result pair stores 4 more bytes per point)
+    const bool mark = l == 0 && !d;
+    if (mark) pfhip_prof_mark(PFHIP_PROF_RBGS, 0, st);
+    if (pair) {
+      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, mg->pv[l], st,
+                     d);
+      if (d && wrote) *wrote = 1;
       void *tmp = mg->x[l]; mg->x[l] = mg->xa[l]; mg->xa[l] = tmp;
       q += 2;
     } else {
@@ -1534,7 +1559,7 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st)
                            seq[q].par, seq[q].init);
       q += 1;
     }
-    if (l == 0) pfhip_prof_mark(PFHIP_PROF_RBGS, 1, st);
+    if (mark) pfhip_prof_mark(PFHIP_PROF_RBGS, 1, st);
     LAUNCHCHK();
   }
   return PFHIP_OK;
@@ -1557,7 +1582,7 @@ static void launch_resid_restrict_z(const LGeom &gf, const LGeom &gc, const Mat<
 template <typename T>
 static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t st)
 {
-  int L = mg->L, lc = mg->lc, rc;
+  int L = mg->L, lc = mg->lc, rc, wrote = 0;
   if (!mg->tuned && (rc = tune_pairs<T>(mg, st))) return rc;
   LGeom g0 = lgeom(mg->n[0]);
   hipLaunchKernelGGL((k_convert<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, bin,
@@ -1616,11 +1641,13 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
                          (const T *)mg->p2[l]);
     }
     LAUNCHCHK();
-    if ((rc = rbgs<T>(mg, l, 0, mg->post, st))) return rc;
+    if ((rc = rbgs<T>(mg, l, 0, mg->post, st, l == 0 ? xout : nullptr, &wrote))) return rc;
   }
-  hipLaunchKernelGGL((k_convert<T, double>), dim3(ntiles(g0)), LBLOCK, 0, st, g0,
-                     (const T *)mg->x[0], xout);
-  LAUNCHCHK();
+  if (!wrote) {
+    hipLaunchKernelGGL((k_convert<T, double>), dim3(ntiles(g0)), LBLOCK, 0, st, g0,
+                       (const T *)mg->x[0], xout);
+    LAUNCHCHK();
+  }
   return PFHIP_OK;
 }
 
