@@ -62,7 +62,9 @@ def kernel_bytes_per_cell(gp):
                            written in the multigrid's FP32
       smoother pair (finest level, FP32): C, E, N, U, b, x_in read, x_out
                            written (the V-cycle's last pair, which writes
-                           the FP64 result instead, is not in the timed set)"""
+                           the FP64 result instead, and its first, which
+                           reads the FP64 right-hand side and stores its
+                           FP32 copy, are not in the timed set)"""
     f = _field_reads(gp)
     res = 8 * (1 + f + 2 + 2 + 1)
     return {

@@ -575,14 +575,16 @@ template <typename T> struct PairSet {
   T2 c, bb, ae, an, as, au, xs, xn;
 };
 
-template <typename T, bool INIT>
+template <typename T, bool INIT, bool DIN = false>
 __device__ __forceinline__ void load_pair(PairSet<T> &S, const Mat<T> &A,
                                           const T *__restrict__ xin, const T *__restrict__ b,
-                                          unsigned ip, unsigned sy)
+                                          unsigned ip, unsigned sy,
+                                          const double *__restrict__ din = nullptr)
 {
   typedef typename V2T<T>::type T2;
   S.c = *(const T2 *)(A.c + ip);
-  S.bb = *(const T2 *)(b + ip);
+  if (DIN) S.bb = T2{(T)din[ip], (T)din[ip + 1]};
+  else S.bb = *(const T2 *)(b + ip);
   S.ae = *(const T2 *)(A.u[0] + ip);
   S.an = *(const T2 *)(A.u[1] + ip);
   S.as = *(const T2 *)(A.u[1] + ip - sy);
@@ -611,13 +613,18 @@ __device__ __forceinline__ typename V2T<T>::type load_x2(const T *__restrict__ x
 // DOUT: the pair writes its result as FP64 into dout (a vector of the
 // level's ghosted layout, no alignment assumed) instead of xout: the last
 // sweep of a V-cycle hands the preconditioned vector straight to the FP64
-// solver, without a conversion pass.
-template <typename T, int TY, int KC, bool INIT, bool DOUT = false>
+// solver, without a conversion pass.  DIN (initial pairs): the right-hand
+// side is read from the FP64 vector din and stored, converted, into bout
+// (the level's b) at the pair's own points, which replaces the conversion
+// pass at the start of the V-cycle.
+template <typename T, int TY, int KC, bool INIT, bool DOUT = false, bool DIN = false>
 __global__ __launch_bounds__(64 * (TY + 2), PAIR2_MIN_WAVES(T)) void k_rbgs_pair2(LGeom g, Mat<T> A,
                                                               const T *__restrict__ xin,
                                                               T *__restrict__ xout,
                                                               const T *__restrict__ b,
-                                                              double *__restrict__ dout)
+                                                              double *__restrict__ dout,
+                                                              const double *__restrict__ din,
+                                                              T *__restrict__ bout)
 {
   typedef typename V2T<T>::type T2;
   __shared__ T red[3][TY + 2][64];
@@ -647,7 +654,7 @@ __global__ __launch_bounds__(64 * (TY + 2), PAIR2_MIN_WAVES(T)) void k_rbgs_pair
   T sc = T(1), sb = T(0), saw = T(0), sae = T(0), sas = T(0), san = T(0), sal = T(0), sau = T(0);
   T rm1 = T(0), rm2 = T(0);
   PairSet<T> cur, nxt;
-  load_pair<T, INIT>(cur, A, xin, b, at(kb - 1), sy);
+  load_pair<T, INIT, DIN>(cur, A, xin, b, at(kb - 1), sy, din);
   T2 al = *(const T2 *)(A.u[2] + at(kb - 2));        // lower coupling of plane kb-1
   T2 xm = load_x2<T, INIT>(xin, at(kb - 2));
   T2 x0 = load_x2<T, INIT>(xin, at(kb - 1));
@@ -655,8 +662,9 @@ __global__ __launch_bounds__(64 * (TY + 2), PAIR2_MIN_WAVES(T)) void k_rbgs_pair
   int it = 0;
   for (int k = kb - 1; k <= ke; k++, it++) {
     // prefetch: coefficients and y-neighbour x of plane k+1, x of plane k+2
-    load_pair<T, INIT>(nxt, A, xin, b, at(k + 1), sy);
+    load_pair<T, INIT, DIN>(nxt, A, xin, b, at(k + 1), sy, din);
     const T2 xpp = load_x2<T, INIT>(xin, at(k + 2));
+    if (DIN && owned && k >= kb && k < ke) *(T2 *)(bout + ip) = cur.bb;
     // ---- red at plane k: the lane's red point is 0 when (i0+j+k) is odd
     const bool p0red = ((i0 + j + k) & 1) == 1;
     const bool here = valid && k >= 0 && k < nz;
@@ -1430,35 +1438,42 @@ static void launch_pair_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout
 
 template <typename T, int TY, int KC>
 static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
-                           int init, hipStream_t st, double *dout)
+                           int init, hipStream_t st, double *dout, const double *din)
 {
   unsigned gx = (g.n[0] + PX2 - 1) / PX2, gy = (g.n[1] + TY - 1) / TY, gz = (g.n[2] + KC - 1) / KC;
   unsigned nb = gx * gy * gz, nb8 = (nb + 7u) & ~7u;
-  if (init)
+  if (init && din)
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true>), dim3(nb8),
+                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, din,
+                       (T *)b);
+  else if (init)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true>), dim3(nb8), dim3(64, TY + 2, 1), 0, st,
-                       g, A, xin, xout, b, nullptr);
+                       g, A, xin, xout, b, nullptr, nullptr, nullptr);
   else if (dout)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true>), dim3(nb8), dim3(64, TY + 2, 1), 0,
-                       st, g, A, xin, xout, b, dout);
+                       st, g, A, xin, xout, b, dout, nullptr, nullptr);
   else
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false>), dim3(nb8), dim3(64, TY + 2, 1), 0, st,
-                       g, A, xin, xout, b, nullptr);
+                       g, A, xin, xout, b, nullptr, nullptr, nullptr);
 }
 
 // shape v: (TY, KC) = (14, 16), (6, 16), (6, 32), (14, 8), (14, 4); dout
-// (even x extents, non-initial pairs only) takes the result in FP64
+// (even x extents, non-initial pairs only) takes the result in FP64, din
+// (even x extents, initial pairs only) gives the right-hand side in FP64
+// and b receives it in the level precision
 template <typename T>
 static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
-                        int init, int v, hipStream_t st, double *dout = nullptr)
+                        int init, int v, hipStream_t st, double *dout = nullptr,
+                        const double *din = nullptr)
 {
   if ((g.n[0] % 2) == 0) {
     // two points per lane (aligned vector loads need an even x extent)
     switch (v) {
-      case 1: launch_pair2_t<T, 6, 16>(g, A, xin, xout, b, init, st, dout); return;
-      case 2: launch_pair2_t<T, 6, 32>(g, A, xin, xout, b, init, st, dout); return;
-      case 3: launch_pair2_t<T, 14, 8>(g, A, xin, xout, b, init, st, dout); return;
-      case 4: launch_pair2_t<T, 14, 4>(g, A, xin, xout, b, init, st, dout); return;
-      default: launch_pair2_t<T, 14, 16>(g, A, xin, xout, b, init, st, dout); return;
+      case 1: launch_pair2_t<T, 6, 16>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 2: launch_pair2_t<T, 6, 32>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 3: launch_pair2_t<T, 14, 8>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 4: launch_pair2_t<T, 14, 4>(g, A, xin, xout, b, init, st, dout, din); return;
+      default: launch_pair2_t<T, 14, 16>(g, A, xin, xout, b, init, st, dout, din); return;
     }
   }
   switch (v) {
@@ -1522,7 +1537,7 @@ extern "C" int pfhip_mg_pair_shapes(pfhip_mg *mg, int *shape, int max)
 // into dout in FP64 (and *wrote is set); x then holds no valid iterate.
 template <typename T>
 static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
-                double *dout = nullptr, int *wrote = nullptr)
+                double *dout = nullptr, int *wrote = nullptr, const double *din = nullptr)
 {
   LGeom g = lgeom(mg->n[l]);
   dim3 grid(ntiles(g));
@@ -1540,13 +1555,15 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
                       seq[q + 1].par == 0;
     double *d = (pair && dout && q + 2 == n && !seq[q].init && (g.n[0] % 2) == 0) ? dout
                                                                                  : nullptr;
-    // level 0's launches with a level-precision result are the timed ones
-    // (the FP64-result pair stores 4 more bytes per point)
-    const bool mark = l == 0 && !d;
+    const double *di = (pair && din && q == 0 && seq[q].init) ? din : nullptr;
+    // level 0's launches with level-precision operands are the timed ones
+    // (the FP64-result pair stores 4 more bytes per point, the pair that
+    // converts the FP64 right-hand side reads 4 and stores 4 more)
+    const bool mark = l == 0 && !d && !di;
     if (mark) pfhip_prof_mark(PFHIP_PROF_RBGS, 0, st);
     if (pair) {
       launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, mg->pv[l], st,
-                     d);
+                     d, di);
       if (d && wrote) *wrote = 1;
       void *tmp = mg->x[l]; mg->x[l] = mg->xa[l]; mg->xa[l] = tmp;
       q += 2;
@@ -1585,12 +1602,22 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
   int L = mg->L, lc = mg->lc, rc, wrote = 0;
   if (!mg->tuned && (rc = tune_pairs<T>(mg, st))) return rc;
   LGeom g0 = lgeom(mg->n[0]);
-  hipLaunchKernelGGL((k_convert<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, bin,
-                     (T *)mg->b[0]);
-  LAUNCHCHK();
+  // the first pre-smoothing pair of level 0 converts b itself when it is an
+  // initial two-points-per-lane pair
+  HalfSweep seq0[2 * MAX_SWEEPS + 4];
+  int fill0;
+  const int n0 = sweep_list(mg->symmetric, 1, mg->pre, seq0, &fill0);
+  const bool din = mg->fuse && !mg->full && mg->xa[0] && (g0.n[0] % 2) == 0 && !fill0 &&
+                   n0 >= 2 && seq0[0].par == 1 && seq0[0].init && seq0[1].par == 0;
+  if (!din) {
+    hipLaunchKernelGGL((k_convert<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, bin,
+                       (T *)mg->b[0]);
+    LAUNCHCHK();
+  }
   int l;
   for (l = 0; l < lc; l++) {
-    if ((rc = rbgs<T>(mg, l, 1, mg->pre, st))) return rc;
+    if ((rc = rbgs<T>(mg, l, 1, mg->pre, st, nullptr, nullptr, (l == 0 && din) ? bin : nullptr)))
+      return rc;
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
     if (mg->rrz && !mg->full && mg->dir[l] == 2 && (gf.n[0] & 1) == 0) {
       launch_resid_restrict_z<T>(gf, gc, matof<T>(mg, l), (const T *)mg->x[l],
