@@ -38,6 +38,10 @@ extern "C" {
 #define PFHIP_BC_DIR_EQUIL  1    /* DirEquilRefPatch                     */
 #define PFHIP_BC_OVERLAND   2    /* OverlandKinematic (z-upper face)     */
 #define PFHIP_MAX_INTERVALS 8
+/* Largest ghosted grid (nx+2)(ny+2)(nz+2) a problem accepts: 2^29 cells, an
+ * FP64 array of 4 GiB (the residual kernels address cells by 32-bit byte
+ * offsets from each array's base). */
+#define PFHIP_MAX_GHOSTED_CELLS 536870912L
 
 /* One box face: type, time cycle (PL/time_cycle_data.c:152-197) and, for
  * DirEquilRefPatch, the reference elevation (PL/problem_bc_pressure.c:143). */

@@ -331,8 +331,11 @@ extern "C" int pfhip_problem_create(const pfhip_problem_desc *desc,
     return PFHIP_ERR_ARG;
   }
   long ng = (long)(d->nx + 2) * (d->ny + 2) * (d->nz + 2);
-  if (ng > 2147483647L) {
-    pfhip_set_error("grid too large for 32-bit cell indexing");
+  // the residual kernels load through 32-bit byte offsets from each array's
+  // base (ldb): an FP64 ghosted array must stay below 4 GiB, 2^29 cells
+  if (ng >= PFHIP_MAX_GHOSTED_CELLS) {
+    pfhip_set_error("ghosted grid of %ld cells exceeds the limit of %ld (4 GiB per FP64 array)",
+                    ng, (long)PFHIP_MAX_GHOSTED_CELLS);
     return PFHIP_ERR_ARG;
   }
   pfhip_problem *pb = (pfhip_problem *)calloc(1, sizeof(pfhip_problem));
@@ -1632,137 +1635,8 @@ __device__ __forceinline__ double residual_bc(const ResArgs &A, int i, int j, in
   return fv;
 }
 
-// Fused residual: one launch per evaluation.  A 64 x TY block owns a tile
-// of columns and marches up a chunk of ZC z-planes.  Per plane:
-//  A. every thread computes S and kr*rho of its own cell one plane ahead
-//     (registers, for the z faces); the 2*(64+TY) edge-halo cells' pressure
-//     and kr*rho go to LDS with the tile's own;
-//  B. every cell computes the fluxes through its own x+, y+, z+ faces once;
-//     the west/south halo cells compute the face into the tile; x/y faces
-//     go to LDS, the z face stays in a register for the plane above;
-//  C. each cell sums outgoing minus incoming face fluxes.
-// So each face flux and each kr is evaluated once (plus the halo share),
-// and p, alpha, n, K stream from HBM once.
+// Residual tiles are RTX columns wide (one wavefront per tile row).
 #define RTX 64
-
-template <int TY, int ZC>
-__global__ __launch_bounds__(RTX * TY) void k_residual_fused(ResArgs A)
-{
-  constexpr int HALO = 2 * RTX + 2 * TY;
-  constexpr int LDSN = (RTX + 2) * (TY + 2);
-  __shared__ double s_p[LDSN];
-  __shared__ double s_kr[LDSN];
-  __shared__ double s_ux[TY * (RTX + 1)];   // x faces: west face of tile column tx at tx
-  __shared__ double s_uy[(TY + 1) * RTX];   // y faces: south face of tile row ty at ty
-  const Geom g = A.g;
-  const Phys &f = A.f;
-  const double rho = f.rho, grav = f.g, mu = f.mu;
-  const double dx = A.dx, dy = A.dy, dz = A.dz;
-  const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
-  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * RTX + tx;
-  const int i = blockIdx.x * RTX + tx, j = blockIdx.y * TY + ty;
-  const bool act = (i < g.nx) && (j < g.ny);
-  const int ls = (ty + 1) * (RTX + 2) + (tx + 1);
-  // edge-halo cell of this thread (corners are not needed by 7 points)
-  int hx = 0, hy = 0;
-  bool hact = false;
-  if (tid < HALO) {
-    if (tid < RTX) { hx = tid + 1; hy = 0; }
-    else if (tid < 2 * RTX) { hx = tid - RTX + 1; hy = TY + 1; }
-    else if (tid < 2 * RTX + TY) { hx = 0; hy = tid - 2 * RTX + 1; }
-    else { hx = RTX + 1; hy = tid - 2 * RTX - TY + 1; }
-    int hi = blockIdx.x * RTX + hx - 1, hj = blockIdx.y * TY + hy - 1;
-    hact = (hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny);
-  }
-  const int hls = hy * (RTX + 2) + hx;
-  const int hcol = (blockIdx.x * RTX + hx) + (blockIdx.y * TY + hy) * g.sy;
-  // west halo (hx == 0) supplies the x face into the tile, south halo
-  // (hy == 0) the y face into the tile
-  const bool hwest = hact && hx == 0;
-  const bool hsouth = hact && hy == 0;
-  const int col = (i + 1) + (j + 1) * g.sy;
-  const int k0 = blockIdx.z * ZC;
-  const int k1 = min(k0 + ZC, g.nz);
-
-  double p_m = 0, kr_m = 0, kz_m = 0;
-  double p_c = 0, kr_c = 0, S_c = 0, kz_c = 0;
-  double p_n = 0, kr_n = 0, S_n = 0, kz_n = 0;
-  if (act) {
-    if (k0 > 0) {
-      int ipm = col + k0 * g.sz;
-      p_m = A.p[ipm];
-      kr_m = vg_kr(p_m, f.irg, f.alpha[ipm], f.nvg[ipm], f.tb) * rho;
-      kz_m = A.kz[ipm];
-    }
-    int ip0 = col + (k0 + 1) * g.sz;
-    p_c = A.p[ip0];
-    vg_s_kr(p_c, f.irg, f.alpha[ip0], f.nvg[ip0], f.Sres(ip0), f.Ssat(ip0), S_c, kr_c, f.tb);
-    kr_c *= rho;
-    kz_c = A.kz[ip0];
-  }
-  // flux through the lower z face of the own cell at plane k0
-  double ub = 0.0;
-  if (act && k0 > 0) ub = flux_z(p_m, p_c, kz_m, kz_c, kr_m, kr_c, ffz, dz, rho, grav, mu);
-
-  for (int k = k0; k < k1; k++) {
-    const int ip = col + (k + 1) * g.sz;
-    // ---- A
-    if (act && k + 1 < g.nz) {
-      int ipn = ip + g.sz;
-      p_n = A.p[ipn];
-      vg_s_kr(p_n, f.irg, f.alpha[ipn], f.nvg[ipn], f.Sres(ipn), f.Ssat(ipn), S_n, kr_n, f.tb);
-      kr_n *= rho;
-      kz_n = A.kz[ipn];
-    }
-    int hip = hcol + (k + 1) * g.sz;
-    if (hact) {
-      double ph = A.p[hip];
-      s_p[hls] = ph;
-      s_kr[hls] = vg_kr(ph, f.irg, f.alpha[hip], f.nvg[hip], f.tb) * rho;
-    }
-    if (act) { s_p[ls] = p_c; s_kr[ls] = kr_c; }
-    __syncthreads();
-    // ---- B
-    double ur = 0.0, uf = 0.0, uu = 0.0, kxc = 0.0, kyc = 0.0;
-    if (act) {
-      kxc = A.kx[ip];
-      kyc = A.ky[ip];
-      if (i + 1 < g.nx)
-        ur = flux_xy(p_c, s_p[ls + 1], kxc, A.kx[ip + 1], kr_c, s_kr[ls + 1], ffx, dx,
-                     grav, mu);
-      if (j + 1 < g.ny)
-        uf = flux_xy(p_c, s_p[ls + RTX + 2], kyc, A.ky[ip + g.sy], kr_c,
-                     s_kr[ls + RTX + 2], ffy, dy, grav, mu);
-      if (k + 1 < g.nz)
-        uu = flux_z(p_c, p_n, kz_c, kz_n, kr_c, kr_n, ffz, dz, rho, grav, mu);
-      s_ux[ty * (RTX + 1) + tx + 1] = ur;
-      s_uy[(ty + 1) * RTX + tx] = uf;
-    }
-    if (hwest) {
-      // west halo cell -> face into tile column 0 (that tile cell is in the
-      // domain: the block's first column always is, and hj < ny)
-      s_ux[(hy - 1) * (RTX + 1)] =
-          flux_xy(s_p[hls], s_p[hls + 1], A.kx[hip], A.kx[hip + 1], s_kr[hls],
-                  s_kr[hls + 1], ffx, dx, grav, mu);
-    }
-    if (hsouth) {
-      s_uy[hx - 1] = flux_xy(s_p[hls], s_p[hls + RTX + 2], A.ky[hip], A.ky[hip + g.sy],
-                             s_kr[hls], s_kr[hls + RTX + 2], ffy, dy, grav, mu);
-    }
-    __syncthreads();
-    // ---- C
-    if (act) {
-      double ul = s_ux[ty * (RTX + 1) + tx];
-      double ud = s_uy[ty * RTX + tx];
-      double fv = residual_cell(A, i, j, k, ip, p_c, kr_c, S_c, kxc, kyc, kz_c, ur, uf, uu,
-                                ul, ud, ub, A.s_old[ip], A.poro[ip], A.ss[ip], A.p_old[ip]);
-      A.F[ip] = residual_bc(A, i, j, k, ip, fv, p_c, kr_c, kxc, kyc, kz_c);
-    }
-    ub = uu;
-    p_m = p_c; kr_m = kr_c; kz_m = kz_c;
-    p_c = p_n; kr_c = kr_n; S_c = S_n; kz_c = kz_n;
-  }
-}
 
 // Face fluxes with reciprocal constants (1/d, 1/mu multiply instead of
 // divide: within an ulp of the quotients) and the fast harmonic mean.  Same
@@ -1789,9 +1663,10 @@ __device__ __forceinline__ double flux_z_i(double pa, double pb, double ka, doub
   return ffz * hmean_f(ka, kb) * diff * upw(lc, uc, kra, krb) * imu;
 }
 
-// Software-pipelined fused residual.  Same tiling and phases as
-// k_residual_fused (TX x TY columns, a z-chunk of ZC planes, own x+/y+/z+
-// faces once, edge halo recomputed), but every global load of plane k+1
+// Software-pipelined fused residual.  A TX x TY block owns a tile of
+// columns and marches up a z-chunk of ZC planes; each cell computes its
+// closures one plane ahead and the fluxes through its own x+, y+, z+ faces
+// once (the tile's west/south edge halo is recomputed); every global load of plane k+1
 // (k+2 for the closure inputs) is issued at the top of plane k from a
 // clamped in-bounds address, and the two per-plane barriers order LDS only,
 // so the loads stay in flight while plane k is computed.  PCS: s_res/s_sat
@@ -2008,17 +1883,6 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
   }
 }
 
-// tile height / z-chunk variants; PFHIP_RES_VARIANT selects one (default 0)
-static int residual_variant()
-{
-  static int v = -1;
-  if (v < 0) {
-    const char *e = getenv("PFHIP_RES_VARIANT");
-    v = e ? atoi(e) : 0;
-  }
-  return v;
-}
-
 // Boundary cells of the domain, each exactly once: both z planes first,
 // then the y-lower/upper rows of the inner planes, then the x-lower/upper
 // columns of the inner planes and rows.
@@ -2110,29 +1974,10 @@ static void launch_pf(const ResArgs &A, hipStream_t st)
 
 static void launch_residual(const ResArgs &A, hipStream_t st)
 {
-  const Geom &g = A.g;
-  int v = residual_variant();
   const int mode = (A.pv == nullptr) ? 0 : (A.fsub != nullptr) ? 1 : 2;
-  if (v != 5 || mode != 0) {
-    if (mode == 1) launch_pf<64, 8, 16, 1>(A, st);
-    else if (mode == 2) launch_pf<64, 8, 16, 2>(A, st);
-    else launch_pf<64, 8, 16, 0>(A, st);
-    return;
-  }
-  v = 0;   // 5: the unpipelined kernel, 64 x 8 x 16
-  PROF_BEGIN(PFHIP_PROF_RESIDUAL, st);
-  int TY = (v == 0 || v == 1 || v == 4) ? 8 : 4;
-  int ZC = (v == 0) ? 16 : (v == 1) ? 64 : (v == 2) ? 16 : (v == 3) ? 8 : 4;
-  dim3 grid((g.nx + RTX - 1) / RTX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC);
-  dim3 block(RTX, TY, 1);
-  switch (v) {
-    case 1: hipLaunchKernelGGL((k_residual_fused<8, 64>), grid, block, 0, st, A); break;
-    case 2: hipLaunchKernelGGL((k_residual_fused<4, 16>), grid, block, 0, st, A); break;
-    case 3: hipLaunchKernelGGL((k_residual_fused<4, 8>), grid, block, 0, st, A); break;
-    case 4: hipLaunchKernelGGL((k_residual_fused<8, 4>), grid, block, 0, st, A); break;
-    default: hipLaunchKernelGGL((k_residual_fused<8, 16>), grid, block, 0, st, A); break;
-  }
-  PROF_END(PFHIP_PROF_RESIDUAL, st);
+  if (mode == 1) launch_pf<64, 8, 16, 1>(A, st);
+  else if (mode == 2) launch_pf<64, 8, 16, 2>(A, st);
+  else launch_pf<64, 8, 16, 0>(A, st);
 }
 
 // the problem-fixed part of the residual arguments; the caller sets the

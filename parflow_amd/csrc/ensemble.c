@@ -70,6 +70,7 @@ typedef struct ens_member {
   void *sp;                             /* saved stack pointer             */
   void *stack;
   int live, rc;
+  char err[256];                        /* message of the member's failure */
   int waiting;                          /* queue awaits the scheduler      */
   int qlen, nexec, rc_round;            /* queued / run this round / error */
   pfhip_ens_req q[ENS_QMAX];
@@ -412,8 +413,14 @@ static void member_main(void)
   m->rc = pfhip_advance_richards(m->s, m->p, m->sat, m->t0, m->dt, m->nsteps, m->max_fail,
                                  m->newton, m->gmres, m->fevals, m->dt_used,
                                  &m->steps_done);
+  /* the error message is per thread and every member shares the thread:
+   * keep this member's own message before another member overwrites it */
+  if (m->rc != PFHIP_OK) snprintf(m->err, sizeof(m->err), "%s", pfhip_last_error());
   int frc = flush();
-  if (m->rc == PFHIP_OK) m->rc = frc;
+  if (m->rc == PFHIP_OK && frc != PFHIP_OK) {
+    m->rc = frc;
+    snprintf(m->err, sizeof(m->err), "%s", pfhip_last_error());
+  }
   m->live = 0;
   pfhip_ens_switch(&m->sp, e->sched_sp);   /* never resumed */
   abort();
@@ -565,6 +572,7 @@ int pfhip_ensemble_advance_richards(pfhip_ensemble *e, double *const *p, double 
     m->dt_used = NULL;
     m->steps_done = 0;
     m->rc = PFHIP_OK;
+    m->err[0] = 0;
     if (!m->stack) m->stack = malloc(ENS_STACK);
     if (!m->stack) { rc = PFHIP_ERR_NOMEM; break; }
     m->sp = coroutine_start(m->stack, ENS_STACK);
@@ -605,8 +613,8 @@ int pfhip_ensemble_advance_richards(pfhip_ensemble *e, double *const *p, double 
   g_ens = NULL;
   int sync = pfhip_synchronize();
   if (first_err != PFHIP_OK) {
-    char msg[400];
-    snprintf(msg, sizeof(msg), "%s", pfhip_last_error());
+    char msg[256];
+    snprintf(msg, sizeof(msg), "%s", e->m[who].err[0] ? e->m[who].err : pfhip_last_error());
     pfhip_set_error("ensemble member %d: %s", who, msg);
     return first_err;
   }

@@ -107,9 +107,6 @@ int pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
     pfhip_set_error("krylov_dim must be 1..%d", PFHIP_MAX_MULTI - 2);
     return PFHIP_ERR_ARG;
   }
-  if (opts->krylov == 1 && !opts->precond) {
-    /* FGMRES without a preconditioner is plain GMRES: allowed */
-  }
   pfhip_solver *s = (pfhip_solver *)calloc(1, sizeof(pfhip_solver));
   s->pb = pb;
   s->ops = &pfhip_direct_ops;
@@ -349,6 +346,9 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fn
     rotation_product = 1.0;
     if (ntries == 0) CHK(s->ops->vec_scale(pb, -1.0 / r_norm, fneg, V[0], NULL));
     else CHK(s->ops->vec_scale(pb, 1.0 / r_norm, V[0], V[0], NULL));
+    /* a restart cycle adds Z_c y_c to x: the first cycle's Arnoldi relation
+     * no longer gives J x, so lsolve falls back to the difference quotient */
+    if (ntries > 0) s->jp_ok = 0;
     int l;
     for (l = 0; l < l_max; l++) {
       (*nli)++;
@@ -374,8 +374,10 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fn
     for (int i = 1; i <= krydim; i++) s->yg[i] = 0.0;
     if (qrsol(krydim, Hes, s->givens, s->yg) != 0) { *ret = SPGMR_QRFACT_FAIL; return PFHIP_OK; }
     if (flex) {
-      /* x += Z y directly (flexible GMRES) */
-      CHK(s->ops->vec_combine(pb, (const double *const *)s->Z, krydim, s->yg, x, NULL));
+      /* x += Z y directly (flexible GMRES); without a preconditioner the
+       * search directions are the basis vectors themselves (Z = V) */
+      double **Zs = preOnRight ? s->Z : V;
+      CHK(s->ops->vec_combine(pb, (const double *const *)Zs, krydim, s->yg, x, NULL));
       if (ntries == 0) {
         /* Arnoldi relation of this cycle: J Z = V Hbar, so with x = Z y
          * (x started at 0), J x = V g with g = Hbar y; V has orthonormal

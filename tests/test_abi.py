@@ -57,3 +57,27 @@ def test_no_cpu_fallback_when_library_missing(monkeypatch, tmp_path):
     monkeypatch.setattr(api, "_lib", None)
     with pytest.raises(api.PFHIPError):
         api.lib()
+
+
+def test_problem_create_rejects_grids_beyond_the_offset_limit(lib):
+    """pfhip_problem_create refuses a ghosted grid of 2^29 cells or more (the
+    residual kernels' 32-bit byte offsets would wrap) with an argument error,
+    before any device call."""
+    import ctypes as C
+    from parflow_amd import api
+    L = api.lib()
+    d = api.ProblemDescC()
+    d.dx = d.dy = d.dz = 1.0
+    cell = C.c_double(0.0)
+    dummy = C.c_void_p(C.addressof(cell))
+    for f in ("kx", "ky", "kz", "poro", "ss", "alpha", "nvg"):
+        setattr(d, f, dummy)
+    for b in d.bc:
+        b.n_intervals = 1
+        b.lengths[0] = 1
+    out = C.c_void_p()
+    d.nx, d.ny, d.nz = 1022, 1022, 510          # (nx+2)(ny+2)(nz+2) = 2^29
+    rc = L.pfhip_problem_create(C.byref(d), C.byref(out))
+    assert rc == -2 and not out.value
+    msg = L.pfhip_last_error().decode()
+    assert "exceeds the limit of 536870912" in msg, msg

@@ -217,9 +217,9 @@ def test_vector_reductions(gpu):
 
 @pytest.mark.parametrize("m", [1, 7, 16, 31])
 def test_block_gram_schmidt_products(gpu, m):
-    """CGS2 block products: h = V^T w on the FP64 matrix cores and
-    w -= V h with the fused ||w||^2, against numpy on the interior cells
-    (ragged nx, ghosts poisoned)."""
+    """CGS2 block products: the FP64 multi-dot h = V^T w and w -= V h with
+    the fused ||w||^2, against numpy on the interior cells (ragged nx,
+    ghosts poisoned)."""
     import ctypes as C
     api = _api()
     L = api.lib()
@@ -944,3 +944,149 @@ def test_ensemble_matches_one_at_a_time(gpu):
     e = np.max(np.abs(pe - b) / np.maximum(np.abs(b), 1e-3))
     print(f"member {small} vs oracle plain: {e:.3e}", flush=True)
     assert e < 1e-8
+
+
+def _phys(prob, rho, gravity, mu):
+    import dataclasses
+    return dataclasses.replace(prob, rho=rho, gravity=gravity, mu=mu)
+
+
+@pytest.mark.parametrize("overland,dirs", [(False, True), (True, False)])
+def test_density_gravity_viscosity_kernels(gpu, overland, dirs):
+    """rho, g, mu != 1 (PL/problem_phase_density.c:119, PL/problem.c:152,
+    the phase viscosity): closures and their derivatives (taken with respect
+    to the head p / (rho g), as the reference does), the residual, the full
+    and PFSymmetric Jacobians, the face velocities and the storage against
+    the oracle, whose placement of rho, g and mu the CPU identities pin.
+    The second shape spans several residual tiles."""
+    api = _api()
+    shape = (11, 9, 6) if not overland else (70, 13, 5)
+    prob = _phys(cases.hetero(*shape, seed=41, overland=overland, dir_sides=dirs), 1.7, 2.3, 0.6)
+    print(f"rho g mu kernels overland={overland} dirs={dirs}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p_old = cases.random_pressure(prob, seed=42)
+    s_old = O.saturation(op, p_old)
+    p = p_old + np.random.default_rng(43).normal(0, 0.01, p_old.shape)
+    for fcn in (0, 1):
+        assert _relmax(_I(prob, gp.saturation(p, fcn)), _I(prob, O.saturation(op, p, fcn))) < 1e-13
+        assert _relmax(_I(prob, gp.phase_rel_perm(p, fcn)), _I(prob, O.relperm(op, p, fcn))) < 1e-12
+    Fg = gp.nl_function_eval(p, p_old, s_old, 0.1, 0.35)
+    Fo = O.nl_function_eval(op, p, p_old, s_old, 0.1, 0.35)
+    assert _relmax(_I(prob, Fg), _I(prob, Fo), np.max(np.abs(_I(prob, Fo)))) < 1e-12
+    for symm in (0, 1):
+        Jg = gp.richards_jacobian_eval(p, 0.1, 0.35, symm)
+        Jo = O.jacobian_eval(op, p, 0.1, 0.35, symm)
+        idx = [0, 2, 4, 6] if symm else range(7)
+        sc = max(np.max(np.abs(Jo[q])) for q in idx)
+        for q in idx:
+            assert _relmax(_I(prob, Jg[q]), _I(prob, Jo[q]), sc) < 1e-12, (symm, q)
+    got = gp.face_velocity(p, p_old, 0.1, 0.35)
+    ref = O.face_velocity(op, p, p_old, 0.1, 0.35)
+    scale = max(np.max(np.abs(r)) for r in ref)
+    for a, b in zip(got, ref):
+        assert np.max(np.abs(a - b)) < 1e-12 * scale
+    a, b = gp.water_balance_storage(p, s_old), O.storage(op, p, s_old)
+    assert abs(a[0] - b[0]) <= 1e-13 * abs(b[0]) and abs(a[1] - b[1]) <= 1e-13 * max(abs(b[1]), 1)
+
+
+@pytest.mark.parametrize("use_jac", [False, True])
+def test_density_gravity_viscosity_newton(gpu, use_jac):
+    """The plain Newton solve with rho, g, mu != 1 against the oracle per
+    cell per step, Newton +-1."""
+    prob = _phys(cases.hetero(14, 12, 10, seed=44, dir_sides=True), 1.7, 2.3, 0.6)
+    opts = cases.solver(use_jacobian=use_jac, residual_tol=1e-8, krylov_dim=25,
+                        precond="PFMG")
+    p0 = prob.hydrostatic(0.5, 0.0)
+    print(f"advance rho g mu use_jac={use_jac}", flush=True)
+    _compare_runs(prob, opts, p0, 0.2, 4)
+
+
+def test_time_loop_halving_matches_oracle(gpu):
+    """The dt-halving branch on the device (PL/solver_richards.c:3001-3027):
+    with Solver.Nonlinear.MaxIter 8 the first Growth step of 1.0 fails
+    (it needs 10 Newton iterations), the state is restored and the step
+    retried at 0.5, and growth continues from the halved step.  The device
+    loop takes the oracle's steps, failures and sizes exactly, its
+    pressures within 1e-9."""
+    from parflow_amd.problem import TimeControl
+    api = _api()
+    prob = cases.hetero(14, 12, 10, seed=21, dir_sides=True)
+    p0 = prob.hydrostatic(0.5, 0.0)
+    opts = cases.solver(residual_tol=1e-8, krylov_dim=25, precond="PFMG", max_iter=8)
+    tc = TimeControl(start_time=0.0, stop_time=4.0, base_unit=1.0, dump_interval=-1,
+                     ts_type="Growth", initial_step=1.0, growth_factor=1.5, min_step=1.0 / 16,
+                     max_step=4.0)
+    print("time loop halving", flush=True)
+    gp = api.GPUProblem(prob)
+    ph, sh, rows = api.GPUSolver(gp, opts).run(p0, tc)
+    pho, sho, rowso = O.advance_ts(O.OracleProblem(prob), opts, p0, tc)
+    print([(r["dt"], r["newton"], r["conv_failures"]) for r in rows], flush=True)
+    assert rowso[0][7] == 1 and rowso[0][1] == 0.5
+    assert len(rows) == len(rowso)
+    for k, (r, ro) in enumerate(zip(rows, rowso)):
+        assert (r["time"], r["dt"], r["dt_info"], r["dumped"]) == ro[:4], k
+        assert r["conv_failures"] == ro[7] and abs(r["newton"] - ro[4]) <= 1, k
+        a, b = _I(prob, ph[k + 1]), _I(prob, pho[k + 1])
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-9, k
+
+
+def test_fgmres_without_preconditioner_matches_oracle(gpu):
+    """The flexible GMRES variant with NoPC (its search directions are the
+    basis vectors themselves) against the oracle's plain NoPC solve."""
+    api = _api()
+    prob = cases.hetero(12, 10, 8, seed=45, dir_sides=True)
+    opts = cases.solver(residual_tol=1e-12, krylov_dim=40, precond="NoPC")
+    p0 = prob.hydrostatic(0.5, 0.0)
+    print("FGMRES NoPC", flush=True)
+    gp = api.GPUProblem(prob)
+    pg, sg, lg = api.GPUSolver(gp, opts, 64, 1).advance(p0, 0.0, 0.1, 3)
+    r, pho, sho, lo = O.advance(O.OracleProblem(prob), opts, p0, 0.0, 0.1, 3)
+    assert r == 0
+    for s in range(4):
+        a, b = _I(prob, pg[s]), _I(prob, pho[s])
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-8, s
+    assert np.all(np.abs(lg["newton"] - np.array(lo.newton[:3])) <= 1)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_fgmres_with_restarts_matches_oracle(gpu, precision):
+    """Solver.Linear.MaxRestarts > 0 with a Krylov dimension small enough
+    that GMRES restarts: the flexible variant (FP64 or FP32 V-cycle) against
+    the oracle's plain solve with the same restarts."""
+    api = _api()
+    prob = cases.hetero(16, 14, 12, seed=46, dir_sides=True)
+    opts = cases.solver(residual_tol=1e-12, krylov_dim=3, max_restarts=6, precond="PFMG")
+    p0 = prob.hydrostatic(0.5, 0.0)
+    print(f"FGMRES restarts precision={precision}", flush=True)
+    gp = api.GPUProblem(prob)
+    pg, sg, lg = api.GPUSolver(gp, opts, precision, 1).advance(p0, 0.0, 0.2, 3)
+    r, pho, sho, lo = O.advance(O.OracleProblem(prob), opts, p0, 0.0, 0.2, 3)
+    assert r == 0
+    # some linear solve needed more than one cycle of 3
+    assert np.max(lg["gmres"] / lg["newton"]) > opts.krylov_dim
+    for s in range(4):
+        a, b = _I(prob, pg[s]), _I(prob, pho[s])
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-8, s
+
+
+def test_mixed_precision_matches_oracle(gpu):
+    """The mixed form (flexible GMRES, FP32 V-cycle, CGS2, Arnoldi ||J p||)
+    against the oracle's plain FP64 solve directly: per cell within 1e-8 at
+    a tight tolerance on a heterogeneous deck with Dirichlet sides, Newton
+    +-1."""
+    api = _api()
+    prob = cases.hetero(16, 14, 12, seed=31, dir_sides=True)
+    opts = cases.solver(residual_tol=1e-12, krylov_dim=30, precond="PFMG",
+                        num_pre_relax=2, num_post_relax=2)
+    p0 = prob.hydrostatic(0.5, 0.0)
+    print("mixed vs oracle", flush=True)
+    gp = api.GPUProblem(prob)
+    pm, sm, lm = api.GPUSolver(gp, opts, 32, 1).advance(p0, 0.0, 0.2, 3)
+    r, pho, sho, lo = O.advance(O.OracleProblem(prob), opts, p0, 0.0, 0.2, 3)
+    assert r == 0
+    for s in range(4):
+        a, b = _I(prob, pm[s]), _I(prob, pho[s])
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < 1e-8, s
+        assert _relmax(_I(prob, sm[s]), _I(prob, sho[s])) < 1e-8, s
+    assert np.all(np.abs(lm["newton"] - np.array(lo.newton[:3])) <= 1)

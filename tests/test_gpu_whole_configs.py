@@ -1,0 +1,323 @@
+"""The benchmark workloads themselves on the GPU, at their full sizes:
+
+c0  32^3 homogeneous box, 10 steps: the plain device path against the
+    oracle per cell per step, and the mixed path against the oracle;
+    water balance at ResidualTol 1e-12.
+c1  256x256x64 heterogeneous, 20 steps: the drop-in contract -- the plain
+    device path against the oracle's plain FP64 run, per cell within 1e-9
+    relative for p and S after every step (PL/solver_richards.c:3368 per
+    step), Newton counts within +-1.  Run as four tests of five steps that
+    continue each other's state, so no single test runs for minutes.
+c2  512x512x64 heterogeneous, 20 steps: the mixed path against the plain
+    path per step (1e-8), and step 1 of the plain path against the oracle.
+c3  512x512x32 tilted-V overland, 100 steps (rain, then recession): steps
+    1-2 of the plain path against the oracle; mixed against plain over all
+    100 steps; the water balance of the plain path at ResidualTol 1e-12.
+c4  the 512-member hillslope ensemble, 2 steps: every member of the batched
+    advance against the same member advanced one at a time.
+
+Bars as the north star states them: plain vs the reference CPU algorithm
+1e-9 per cell and Newton +-1; mixed vs plain 1e-8; the water-balance
+residual below 1e-10 of the storage change
+(test/python/water_balance_x.py:612-620 checks the same budget at 0.005 %).
+"""
+import dataclasses
+import os
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+import cases
+
+pytestmark = pytest.mark.gpu
+
+_STATE = {}
+
+
+def _progress(msg):
+    """A progress line: printed, and appended to the file PFHIP_TEST_PROGRESS
+    names, when set (long tests report every step)."""
+    print(msg, flush=True)
+    path = os.environ.get("PFHIP_TEST_PROGRESS")
+    if path:
+        with open(path, "a") as f:
+            f.write(msg + "\n")
+
+
+def _api():
+    from parflow_amd import api
+    return api
+
+
+def _deck(name):
+    from parflow_amd.decks import deck
+    return deck(name)
+
+
+def _prel(a, b):
+    """Relative per-cell difference with the floor of the plain-path tests
+    (pressures near zero compare absolutely below 1e-3)."""
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)))
+
+
+def _srel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+class _Run:
+    """A device solver advancing its own state one step at a time."""
+
+    def __init__(self, gp, opts, p0, precision=64, krylov=0):
+        api = _api()
+        self.gp = gp
+        self.sv = api.GPUSolver(gp, opts, precision, krylov)
+        self.dp, self.ds = gp.vec(p0), gp.vec()
+        api.check(api.lib().pfhip_saturation(gp.h, self.dp.ptr, self.ds.ptr, 0, None),
+                  "saturation")
+        self.t = 0.0
+
+    def step(self, dt):
+        lg = self.sv.advance_device(self.dp, self.ds, self.t, dt, 1)
+        self.t += float(lg["dt"][0])
+        return int(lg["newton"][0]), int(lg["gmres"][0])
+
+    def p(self):
+        return self.gp.prob.interior(self.gp.get(self.dp))
+
+    def s(self):
+        return self.gp.prob.interior(self.gp.get(self.ds))
+
+
+def _oracle_steps(prob, opts, p, t, dt, n):
+    """n oracle steps from host pressure p at time t: (p_hist, s_hist, newton)."""
+    op = O.OracleProblem(prob)
+    r, ph, sh, log = O.advance(op, opts, p, t, dt, n)
+    assert r == 0
+    return ph, sh, list(log.newton[:n])
+
+
+def _wb_ratio(gp, prob, p1, s1, p0, s0, dt, t):
+    """|change of stored water - step budget| / |change of stored water|.
+    Stored water is what the residual's accumulation terms book
+    (phi S rho + Ss p S rho) vol per cell, plus the ponded depth on an
+    overland top, differenced cell by cell; the budget is the device's
+    boundary inflow + overland-top flux - runoff + sources of the step."""
+    I = prob.interior
+    vol = prob.dx * prob.dy * prob.dz
+    d = (I(prob.g_poro) * (I(s1) - I(s0)) * prob.rho * vol
+         + I(prob.g_ss) * vol * (I(p1) * I(s1) - I(p0) * I(s0)) * prob.rho)
+    dm = float(np.sum(d))
+    if prob.overland:
+        dm += float(np.sum(prob.dx * prob.dy * (np.maximum(I(p1)[-1], 0.0)
+                                                 - np.maximum(I(p0)[-1], 0.0))))
+    f = gp.water_balance_fluxes(p1, p0, dt, t)
+    budget = f[0] + f[1] - f[2] + f[3]
+    return abs(dm - budget) / abs(dm), dm
+
+
+# ---------------------------------------------------------------------- c0
+def test_c0_plain_matches_oracle(gpu):
+    api = _api()
+    prob, opts, p0, dt, n, _ = _deck("c0")
+    _progress(f"c0 {prob.nx}x{prob.ny}x{prob.nz}, {n} steps plain vs oracle")
+    gp = api.GPUProblem(prob)
+    ph, sh, lg = api.GPUSolver(gp, opts).advance(p0, 0.0, dt, n)
+    r, pho, sho, lo = O.advance(O.OracleProblem(prob), opts, p0, 0.0, dt, n)
+    assert r == 0
+    I = prob.interior
+    for s in range(1, n + 1):
+        assert _prel(I(ph[s]), I(pho[s])) < 1e-9, s
+        assert _srel(I(sh[s]), I(sho[s])) < 1e-9, s
+    assert np.all(np.abs(lg["newton"] - np.array(lo.newton[:n])) <= 1)
+
+
+def test_c0_mixed_matches_oracle(gpu):
+    """The mixed form at the deck's own tolerances against the oracle's
+    plain FP64 run: 1e-8 per cell per step."""
+    api = _api()
+    prob, opts, p0, dt, n, _ = _deck("c0")
+    _progress("c0 mixed vs oracle")
+    gp = api.GPUProblem(prob)
+    pm, sm, lm = api.GPUSolver(gp, opts, 32, 1).advance(p0, 0.0, dt, n)
+    r, pho, sho, lo = O.advance(O.OracleProblem(prob), opts, p0, 0.0, dt, n)
+    assert r == 0
+    I = prob.interior
+    for s in range(1, n + 1):
+        assert _prel(I(pm[s]), I(pho[s])) < 1e-8, s
+        assert _srel(I(sm[s]), I(sho[s])) < 1e-8, s
+    assert np.all(np.abs(lm["newton"] - np.array(lo.newton[:n])) <= 1)
+
+
+def test_c0_water_balance(gpu):
+    """Plain path at ResidualTol 1e-12: the water-balance residual of every
+    step below 1e-10 of the step's storage change."""
+    api = _api()
+    prob, opts, p0, dt, n, _ = _deck("c0")
+    opts = dataclasses.replace(opts, residual_tol=1e-12)
+    gp = api.GPUProblem(prob)
+    ph, sh, lg = api.GPUSolver(gp, opts).advance(p0, 0.0, dt, n)
+    worst = 0.0
+    for s in range(1, n + 1):
+        ratio, dm = _wb_ratio(gp, prob, ph[s], sh[s], ph[s - 1], sh[s - 1], dt, s * dt)
+        worst = max(worst, ratio)
+        assert ratio < 1e-10, (s, ratio, dm)
+    _progress(f"c0 water balance: worst residual / storage change {worst:.2e}")
+
+
+# ---------------------------------------------------------------------- c1
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("part", [0, 1, 2, 3])
+def test_c1_plain_matches_oracle(gpu, part):
+    """Steps 5*part+1 .. 5*part+5 of c1: the device continues its own state
+    and the oracle its own from the previous part."""
+    api = _api()
+    if part == 0:
+        prob, opts, p0, dt, n, _ = _deck("c1")
+        assert n == 20
+        gp = api.GPUProblem(prob)
+        _STATE["c1"] = dict(prob=prob, opts=opts, dt=dt, gp=gp, run=_Run(gp, opts, p0),
+                            p_or=p0, t_or=0.0, step=0)
+    st = _STATE["c1"]
+    assert st["step"] == 5 * part, "parts run in order"
+    prob, opts, dt = st["prob"], st["opts"], st["dt"]
+    t0 = time.time()
+    pho, sho, newton_o = _oracle_steps(prob, opts, st["p_or"], st["t_or"], dt, 5)
+    I = prob.interior
+    for k in range(5):
+        ng, lg = st["run"].step(dt)
+        ep = _prel(st["run"].p(), I(pho[k + 1]))
+        es = _srel(st["run"].s(), I(sho[k + 1]))
+        _progress(f"c1 step {st['step'] + k + 1}: newton gpu {ng} oracle {newton_o[k]}, "
+                  f"rel p {ep:.2e} S {es:.2e}")
+        assert ep < 1e-9 and es < 1e-9, (k, ep, es)
+        assert abs(ng - newton_o[k]) <= 1
+    st["p_or"], st["t_or"], st["step"] = pho[5], st["t_or"] + 5 * dt, st["step"] + 5
+    _progress(f"c1 part {part}: {time.time() - t0:.0f} s")
+    if part == 3:
+        _STATE.pop("c1")
+
+
+# ---------------------------------------------------------------------- c2
+@pytest.mark.timeout(600)
+def test_c2_mixed_vs_plain_and_step1_vs_oracle(gpu):
+    """Mixed (FGMRES(30), FP32 V-cycle with 2+2 sweeps) against plain over
+    all 20 steps at 1e-8 per cell, identical Newton counts +-1; step 1 of
+    the plain path against the oracle at 1e-9."""
+    api = _api()
+    prob, opts, p0, dt, n, _ = _deck("c2")
+    gp = api.GPUProblem(prob)
+    plain, mixed = _Run(gp, opts, p0, 64, 0), _Run(gp, opts, p0, 32, 1)
+    for s in range(1, n + 1):
+        npl, _ = plain.step(dt)
+        nmx, _ = mixed.step(dt)
+        pp = plain.p()
+        e = _prel(mixed.p(), pp)
+        _progress(f"c2 step {s}: newton plain {npl} mixed {nmx}, mixed vs plain {e:.2e}")
+        assert e < 1e-8, (s, e)
+        assert abs(npl - nmx) <= 1
+        if s == 1:
+            t0 = time.time()
+            pho, sho, newton_o = _oracle_steps(prob, opts, p0, 0.0, dt, 1)
+            I = prob.interior
+            eo, es = _prel(pp, I(pho[1])), _srel(plain.s(), I(sho[1]))
+            _progress(f"c2 step 1 plain vs oracle: rel p {eo:.2e} S {es:.2e}, newton "
+                      f"{npl} vs {newton_o[0]} ({time.time() - t0:.0f} s)")
+            assert eo < 1e-9 and es < 1e-9
+            assert abs(npl - newton_o[0]) <= 1
+            del pho, sho
+
+
+# ---------------------------------------------------------------------- c3
+@pytest.mark.timeout(600)
+def test_c3_plain_steps_1_2_match_oracle(gpu):
+    api = _api()
+    prob, opts, p0, dt, n, _ = _deck("c3")
+    gp = api.GPUProblem(prob)
+    run = _Run(gp, opts, p0)
+    t0 = time.time()
+    pho, sho, newton_o = _oracle_steps(prob, opts, p0, 0.0, dt, 2)
+    I = prob.interior
+    for k in range(2):
+        ng, _ = run.step(dt)
+        ep, es = _prel(run.p(), I(pho[k + 1])), _srel(run.s(), I(sho[k + 1]))
+        _progress(f"c3 step {k + 1}: newton gpu {ng} oracle {newton_o[k]}, rel p {ep:.2e} "
+                  f"S {es:.2e} ({time.time() - t0:.0f} s)")
+        assert ep < 1e-9 and es < 1e-9
+        assert abs(ng - newton_o[k]) <= 1
+
+
+@pytest.mark.timeout(600)
+def test_c3_mixed_vs_plain_all_steps(gpu):
+    """All 100 steps: 20 of rain, 80 of recession and runoff."""
+    api = _api()
+    prob, opts, p0, dt, n, _ = _deck("c3")
+    assert n == 100
+    gp = api.GPUProblem(prob)
+    plain, mixed = _Run(gp, opts, p0, 64, 0), _Run(gp, opts, p0, 32, 1)
+    worst = 0.0
+    for s in range(1, n + 1):
+        npl, _ = plain.step(dt)
+        nmx, _ = mixed.step(dt)
+        e = _prel(mixed.p(), plain.p())
+        worst = max(worst, e)
+        if s % 10 == 0:
+            _progress(f"c3 step {s}: newton plain {npl} mixed {nmx}, worst so far {worst:.2e}")
+        assert e < 1e-8, (s, e)
+        assert abs(npl - nmx) <= 1, s
+
+
+@pytest.mark.timeout(600)
+def test_c3_water_balance(gpu):
+    """The plain path at ResidualTol 1e-12 over all 100 steps (rain, then
+    recession with runoff): water-balance residual below 1e-10 of the
+    storage change every step."""
+    api = _api()
+    prob, opts, p0, dt, n, _ = _deck("c3")
+    opts = dataclasses.replace(opts, residual_tol=1e-12)
+    gp = api.GPUProblem(prob)
+    run = _Run(gp, opts, p0)
+    p_prev, s_prev = gp.get(run.dp), gp.get(run.ds)
+    worst = 0.0
+    for s in range(1, n + 1):
+        run.step(dt)
+        p1, s1 = gp.get(run.dp), gp.get(run.ds)
+        ratio, dm = _wb_ratio(gp, prob, p1, s1, p_prev, s_prev, dt, run.t)
+        worst = max(worst, ratio)
+        if s % 10 == 0:
+            _progress(f"c3 water balance step {s}: storage change {dm:.4e}, worst ratio "
+                      f"{worst:.2e}")
+        assert ratio < 1e-10, (s, ratio, dm)
+        p_prev, s_prev = p1, s1
+
+
+# ---------------------------------------------------------------------- c4
+@pytest.mark.timeout(600)
+def test_c4_all_members_batched_vs_one_at_a_time(gpu):
+    """All 512 hillslopes advanced 2 steps with batched launches against
+    each member advanced alone on the single-problem mixed path (the same
+    solver objects, ops reset to direct): 1e-10 per cell, Newton +-1."""
+    api = _api()
+    from parflow_amd.decks import c4_members
+    nsteps = 2
+    members = c4_members(512)
+    _progress(f"c4: {len(members)} members, {sum(m[0].ncells for m in members)} cells")
+    ens = api.GPUEnsemble([(p, o) for p, o, _, _ in members])
+    dps = [ens.problems[i].vec(members[i][2]) for i in range(len(members))]
+    dss = [ens.problems[i].vec() for i in range(len(members))]
+    t0 = time.time()
+    lg = ens.advance_device(dps, dss, 0.0, [m[3] for m in members], nsteps)
+    _progress(f"c4 batched: {time.time() - t0:.1f} s, rounds/launches {ens.stats()}")
+    assert np.all(lg["steps"] == nsteps)
+    worst = 0.0
+    t0 = time.time()
+    for i, (prob, opts, p0, dt) in enumerate(members):
+        pg, sg, l1 = ens.solvers[i].advance(p0, 0.0, dt, nsteps, history=False)
+        I = prob.interior
+        pe, se = I(ens.problems[i].get(dps[i])), I(ens.problems[i].get(dss[i]))
+        e = max(_prel(pe, I(pg)), _srel(se, I(sg)))
+        worst = max(worst, e)
+        assert e < 1e-10, (i, e)
+        assert np.all(np.abs(lg["newton"][i] - l1["newton"]) <= 1), i
+    _progress(f"c4 one at a time: {time.time() - t0:.1f} s, worst {worst:.2e}")

@@ -141,6 +141,154 @@ def test_hydrostatic_equilibrium_zero_residual():
     assert np.max(np.abs(prob.interior(F))) < 1e-14
 
 
+def _phys(prob, rho, gravity, mu):
+    return dataclasses.replace(prob, rho=rho, gravity=gravity, mu=mu)
+
+
+def test_hydrostatic_equilibrium_zero_residual_with_density_and_gravity():
+    """With rho g != 1 (PhaseDensity constant, PL/problem_phase_density.c:119;
+    Gravity, PL/problem.c:152) the hydrostatic profile p = -rho g (z - z_ref),
+    with the bottom DirEquilRefPatch value referenced the same way, is a
+    steady state: the z face's lower/upper conditions
+    p/dz -+ rho g / 2 (PL/nl_function_eval.c:817-836) balance exactly and
+    F = 0.  The same profile built with g alone (rho dropped) is not."""
+    from parflow_amd.problem import BCSpec, BC_FLUX
+    base = cases.hetero(6, 5, 7, seed=7)
+    base.bc[5] = BCSpec(BC_FLUX, (0.0,))
+    prob = _phys(base, 1.7, 2.3, 0.6)
+    op = O.OracleProblem(prob)
+    p = prob.hydrostatic(0.0, 0.0)
+    s = O.saturation(op, p)
+    F = O.nl_function_eval(op, p, p, s, 0.5, 0.5)
+    assert np.max(np.abs(prob.interior(F))) < 1e-13
+    wrong = _phys(base, 1.0, 2.3, 0.6).hydrostatic(0.0, 0.0)
+    Fw = O.nl_function_eval(op, wrong, wrong, O.saturation(op, wrong), 0.5, 0.5)
+    assert np.max(np.abs(prob.interior(Fw))) > 1e-3
+
+
+def _zero_flux_tops(prob):
+    from parflow_amd.problem import BCSpec, BC_FLUX
+    bc = [b if b.type != BC_FLUX else BCSpec(BC_FLUX, (0.0,)) for b in prob.bc]
+    return dataclasses.replace(prob, bc=bc)
+
+
+@pytest.mark.parametrize("dirs", [False, True])
+def test_residual_and_jacobian_scale_as_inverse_viscosity(dirs):
+    """Every face flux of the residual, interior and Dirichlet, divides by
+    the viscosity (PL/nl_function_eval.c:767-836, :1018-1232) and the
+    accumulation terms do not, while dt multiplies only the fluxes; with
+    FluxConst values of zero (a flux value is volumetric, not divided by
+    mu) F(p; mu, dt) = F(p; 1, dt / mu) and likewise J."""
+    base = _zero_flux_tops(cases.hetero(6, 5, 4, seed=8, dir_sides=dirs))
+    rng = np.random.default_rng(9)
+    mu, dt = 0.6, 0.1
+    pa, pb = _phys(base, 1.0, 1.0, mu), _phys(base, 1.0, 1.0, 1.0)
+    oa, ob = O.OracleProblem(pa), O.OracleProblem(pb)
+    p_old = cases.random_pressure(base, seed=10)
+    s_old = O.saturation(oa, p_old)
+    p = np.ascontiguousarray(p_old + rng.normal(0, 0.05, p_old.shape))
+    Fa = O.nl_function_eval(oa, p, p_old, s_old, dt, dt)
+    Fb = O.nl_function_eval(ob, p, p_old, s_old, dt / mu, dt)
+    I = base.interior
+    scale = np.max(np.abs(I(Fa)))
+    assert np.max(np.abs(I(Fa) - I(Fb))) < 1e-13 * scale
+    assert np.max(np.abs(I(Fa) - I(O.nl_function_eval(ob, p, p_old, s_old, dt, dt)))) > \
+        1e-3 * scale
+    Ja = O.jacobian_eval(oa, p, dt, dt, 0)
+    Jb = O.jacobian_eval(ob, p, dt / mu, dt, 0)
+    for q in range(7):
+        assert np.max(np.abs(I(Ja[q]) - I(Jb[q]))) <= 1e-13 * np.max(np.abs(I(Ja[0]))), q
+
+
+def test_residual_density_identities():
+    """(rho, g) -> (c rho, g / c) leaves every head p / (rho g) and so S and kr
+    unchanged (PL/problem_saturation.c:265-310).  In the reference the
+    accumulation terms rho phi S vol and Ss vol p S rho and the vertical face
+    flux K (kr rho) (p_a/dz - p_b/dz - rho g) / mu then scale by c, and the
+    lateral face flux K (kr rho) g cos(atan(slope)) (p_a - p_b) / (dx mu)
+    (PL/nl_function_eval.c:767-785, x_dir_g_c) does not.  So on a column
+    (one cell in x and y: accumulation and vertical faces only)
+    F(c rho, g / c, dt) = c F(rho, g, dt), and on a single layer with no-flux
+    faces (accumulation and lateral faces only) F(c rho, g / c, c dt) =
+    c F(rho, g, dt).  A kr that missed its density factor, a gravity term
+    with g but not rho, or a lateral flux without the g factor fails one."""
+    from parflow_amd.problem import BCSpec, BC_FLUX
+    rho, g, c = 1.7, 2.3, 3.0
+    col = _phys(_zero_flux_tops(cases.hetero(1, 1, 9, seed=11)), rho, g, 1.0)
+    lay = cases.hetero(7, 6, 1, seed=12)
+    lay = _phys(dataclasses.replace(lay, bc=[BCSpec(BC_FLUX, (0.0,))] * 6), rho, g, 1.0)
+    for prob, dt_scale in ((col, 1.0), (lay, c)):
+        scaled = _phys(prob, c * rho, g / c, 1.0)
+        oa, ob = O.OracleProblem(prob), O.OracleProblem(scaled)
+        p_old = cases.random_pressure(prob, seed=13)
+        s_old = O.saturation(oa, p_old)
+        assert np.max(np.abs(O.saturation(ob, p_old) - s_old)) < 1e-15
+        p = np.ascontiguousarray(p_old + np.random.default_rng(14).normal(0, 0.2, p_old.shape))
+        dt = 0.3
+        Fa = O.nl_function_eval(oa, p, p_old, s_old, dt, dt)
+        Fb = O.nl_function_eval(ob, p, p_old, s_old, dt * dt_scale, dt)
+        I = prob.interior
+        scale = np.max(np.abs(I(Fa)))
+        assert scale > 0
+        assert np.max(np.abs(c * I(Fa) - I(Fb))) < 1e-13 * c * scale, prob.shape
+        # the fluxes are a sizeable part of F (the identity is not only the
+        # accumulation's)
+        F0 = O.nl_function_eval(oa, p, p_old, s_old, 0.0, dt)
+        assert np.max(np.abs(I(Fa) - I(F0))) > 0.05 * scale
+
+
+def test_jacobian_with_density_gravity_viscosity():
+    """The Jacobian's rho, g and mu factors.
+
+    The reference's closure derivatives are taken with respect to the head
+    h = |p| / (rho g) (PL/problem_saturation.c:306-309,
+    PL/problem_phase_rel_perm.c:900-912: no 1/(rho g) chain factor), so they
+    equal rho g dS/dp.  With rho g = 1 but rho, g, mu != 1 separately the
+    analytic J is therefore the derivative of F (central differences); with
+    rho g != 1, (rho, g) -> (c rho, g / c) scales J by c on a column and
+    J(dt) by c on a no-flux layer with dt -> c dt, as for F."""
+    from parflow_amd.problem import BCSpec, BC_FLUX
+    rng = np.random.default_rng(16)
+    prob = _phys(cases.hetero(5, 4, 4, seed=15, dir_sides=True), 2.0, 0.5, 0.6)
+    op = O.OracleProblem(prob)
+    p = np.ascontiguousarray(prob.hydrostatic(-1.0, 2.0) + rng.normal(0, 0.05, prob.shape))
+    s_old = O.saturation(op, p)
+    J = O.jacobian_eval(op, p, 0.1, 0.1, 0)
+    I = prob.interior
+    for trial in range(3):
+        v = np.zeros_like(p)
+        v[1:-1, 1:-1, 1:-1] = rng.normal(0, 1, (prob.nz, prob.ny, prob.nx))
+        jv = O.matvec(op, J, 0, v)
+        fd = _fd_jv(op, p, p, s_old, 0.1, 0.1, v)
+        assert np.max(np.abs(I(jv) - I(fd))) / np.max(np.abs(I(fd))) < 1e-5
+    # the derivative convention: dS/dp returned = rho g d S / d p
+    pg = _phys(prob, 1.7, 2.3, 0.6)
+    og = O.OracleProblem(pg)
+    q = cases.random_pressure(pg, seed=17)
+    eps = 1e-5
+    fd = (O.saturation(og, q + eps) - O.saturation(og, q - eps)) / (2 * eps)
+    d = O.saturation(og, q, 1)
+    wet = I(q) < -1e-3
+    assert np.allclose(I(d)[wet], 1.7 * 2.3 * I(fd)[wet], rtol=1e-6, atol=1e-12)
+    fd = (O.relperm(og, q + eps) - O.relperm(og, q - eps)) / (2 * eps)
+    d = O.relperm(og, q, 1)
+    assert np.allclose(I(d)[wet], 1.7 * 2.3 * I(fd)[wet], rtol=1e-5, atol=1e-12)
+    # (rho, g) -> (c rho, g / c)
+    rho, g, c = 1.7, 2.3, 3.0
+    col = _phys(_zero_flux_tops(cases.hetero(1, 1, 9, seed=11)), rho, g, 0.6)
+    lay = cases.hetero(7, 6, 1, seed=12)
+    lay = _phys(dataclasses.replace(lay, bc=[BCSpec(BC_FLUX, (0.0,))] * 6), rho, g, 0.6)
+    for pr, dt_scale in ((col, 1.0), (lay, c)):
+        oa, ob = O.OracleProblem(pr), O.OracleProblem(_phys(pr, c * rho, g / c, 0.6))
+        q = np.ascontiguousarray(cases.random_pressure(pr, seed=18))
+        Ja = O.jacobian_eval(oa, q, 0.3, 0.3, 0)
+        Jb = O.jacobian_eval(ob, q, 0.3 * dt_scale, 0.3, 0)
+        Ip = pr.interior
+        for k in range(7):
+            assert np.max(np.abs(c * Ip(Ja[k]) - Ip(Jb[k]))) <= \
+                1e-13 * c * np.max(np.abs(Ip(Ja[0]))), (pr.shape, k)
+
+
 def test_residual_mass_balance():
     """Interior fluxes cancel: sum(F) = storage change - dt * boundary inflow."""
     prob = cases.hetero(6, 5, 4, seed=8)
