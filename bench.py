@@ -61,10 +61,10 @@ def kernel_bytes_per_cell(gp):
       Jacobian (PC matrix): p, alpha, n, K, porosity, Ss read; C, E, N, U
                            written in the multigrid's FP32
       smoother pair (finest level, FP32): C, E, N, U, b, x_in read, x_out
-                           written (the V-cycle's last pair, which writes
-                           the FP64 result instead, and its first, which
-                           reads the FP64 right-hand side and stores its
-                           FP32 copy, are not in the timed set)"""
+                           written; the V-cycle's last pair writes the FP64
+                           result (4 more bytes), its first reads the FP64
+                           right-hand side and stores the level's FP32 copy
+                           (4 more read, 4 more written)"""
     f = _field_reads(gp)
     res = 8 * (1 + f + 2 + 2 + 1)
     return {
@@ -73,6 +73,8 @@ def kernel_bytes_per_cell(gp):
         "residual_step": res + 16,
         "jacobian": 8 * (1 + f + 2) + 4 * 4,
         "rbgs_pair": 4 * 7,
+        "rbgs_pair_dout": 4 * 7 + 4,
+        "rbgs_pair_din": 4 * 7 + 4 + 4,
     }
 
 
@@ -84,9 +86,27 @@ PROF_KERNELS = (
     ("residual_step", "PROF_RESIDUAL_STEP",
      "k_residual_pf<64, 8, 16, *, 2> (line-search trial point)"),
     ("jacobian", "PROF_JACOBIAN", "k_jacobian_pf<64, 8, 16, *, true, float> (PC matrix)"),
-    ("rbgs_pair", "PROF_RBGS",
-     "k_rbgs_pair2<float, TY, KC, *, false> (finest-level red/black pair, FP32 result, "
-     "autotuned tile)"),
+    ("rbgs_pair", "PROF_RBGS", "k_rbgs_pair2<float, ...> FP32 operands"),
+    ("rbgs_pair_dout", "PROF_RBGS_DOUT", "k_rbgs_pair2<float, ...> FP64 result"),
+    ("rbgs_pair_din", "PROF_RBGS_DIN", "k_rbgs_pair2<float, ...> FP64 right-hand side"),
+)
+# kernels reported together as one roofline row: the finest-level smoother
+# pairs in all three operand forms
+FAMILIES = {
+    "rbgs_pair_family": (("rbgs_pair", "rbgs_pair_dout", "rbgs_pair_din"),
+                         "k_rbgs_pair2<float, TY, KC, ...> (finest-level red/black pairs: "
+                         "FP32 operands, FP64 result, FP64 right-hand side; autotuned tile)"),
+}
+
+# solver phases timed in a separate instrumented pass, named after the
+# reference's timers (PL/timing.c, the modules' own timers)
+PHASES = (
+    ("NlFunctionEval", "PROF_PH_NL_F_EVAL"),
+    ("KINSpgmrAtimes (J.v)", "PROF_PH_JV"),
+    ("KinsolPC setup (J + multigrid)", "PROF_PH_PC_SETUP"),
+    ("PFMG (V-cycle)", "PROF_PH_PC_SOLVE"),
+    ("Gram-Schmidt", "PROF_PH_ORTHO"),
+    ("KINSol", "PROF_PH_KINSOL"),
 )
 
 
@@ -107,7 +127,93 @@ def load_traffic():
     return out
 
 
+def _workload_label(name, prob, opts, dt, krylov_label):
+    kind = ("tilted-V catchment with kinematic overland flow" if prob.overland
+            else "heterogeneous Richards")
+    return (f"{name}: {prob.nx}x{prob.ny}x{prob.nz} {kind}, {krylov_label}({opts.krylov_dim}) "
+            f"+ FP32 V-cycle ({opts.num_pre_relax}+{opts.num_post_relax} RB sweeps), "
+            f"dt={dt} h")
+
+
+def _timed_run(api, gp, sv, p0, dt, K, W, snapshot=True):
+    """W untimed warmup steps on a scratch state, then exactly K steps from
+    p0 between two device synchronisations; the pressure after every step is
+    copied into a device snapshot (part of the timed work, one pass)."""
+    L = api.lib()
+    if W > 0:
+        dp, ds = gp.vec(p0), gp.vec()
+        sv.advance_device(dp, ds, 0.0, dt, W)
+        del dp, ds
+    dp, ds = gp.vec(p0), gp.vec()
+    snap = [gp.vec() for _ in range(K)] if snapshot else None
+    newton, gmres, fevals = [], [], []
+    t = 0.0
+    api.check(L.pfhip_synchronize(), "sync")
+    tic = time.perf_counter()
+    for s in range(K):
+        lg = sv.advance_device(dp, ds, t, dt, 1)
+        t += float(lg["dt"][0])
+        newton.append(int(lg["newton"][0]))
+        gmres.append(int(lg["gmres"][0]))
+        fevals.append(int(lg["fevals"][0]))
+        if snapshot:
+            api.check(L.pfhip_vec_linear_sum(gp.h, 1.0, dp.ptr, 0.0, dp.ptr, snap[s].ptr, None),
+                      "snapshot")
+    api.check(L.pfhip_synchronize(), "sync")
+    elapsed = time.perf_counter() - tic
+    return dict(elapsed=elapsed, newton=newton, gmres=gmres, fevals=fevals,
+                value=gp.prob.ncells * K / elapsed), snap
+
+
+def _read_prof(api, pid_names):
+    L = api.lib()
+    out = {}
+    for key, pid in pid_names:
+        ms, n = api.C.c_double(), api.C.c_long()
+        api.check(L.pfhip_prof_read(getattr(api, pid), api.C.byref(ms), api.C.byref(n)),
+                  "prof_read")
+        api.check(L.pfhip_prof_disable(getattr(api, pid)), "prof_off")
+        out[key] = (ms.value, n.value)
+    return out
+
+
+def _phase_pass(api, gp, sv, p0, dt, K):
+    """The same K steps again with the phase timers on (not the timed run):
+    device milliseconds per step of each phase."""
+    for _, pid in PHASES:
+        api.check(api.lib().pfhip_prof_enable(getattr(api, pid), 20000), "prof")
+    _timed_run(api, gp, sv, p0, dt, K, 0, snapshot=False)
+    prof = _read_prof(api, [(name, pid) for name, pid in PHASES])
+    return {name: {"ms_per_step": ms / K, "calls": n} for name, (ms, n) in prof.items()}
+
+
+def _water_balance(gp, prob, snaps, p0, dt):
+    """Per-step |change of stored water - budget| / |change of stored water|
+    of a run's pressures (device snapshots): stored water as the residual's
+    accumulation terms book it, the budget from the device's boundary,
+    overland-top, runoff and source terms."""
+    I = prob.interior
+    vol = prob.dx * prob.dy * prob.dz
+    poro, ss = I(prob.g_poro), I(prob.g_ss)
+    p_prev = p0
+    s_prev = gp.saturation(p0)
+    out = []
+    for k, d in enumerate(snaps):
+        p1 = gp.get(d)
+        s1 = gp.saturation(p1)
+        dm = float(np.sum(poro * (I(s1) - I(s_prev)) * prob.rho * vol
+                          + ss * vol * (I(p1) * I(s1) - I(p_prev) * I(s_prev)) * prob.rho))
+        if prob.overland:
+            dm += float(np.sum(prob.dx * prob.dy * (np.maximum(I(p1)[-1], 0.0)
+                                                     - np.maximum(I(p_prev)[-1], 0.0))))
+        f = gp.water_balance_fluxes(p1, p_prev, dt, (k + 1) * dt)
+        out.append(abs(dm - (f[0] + f[1] - f[2] + f[3])) / abs(dm))
+        p_prev, s_prev = p1, s1
+    return out
+
+
 def run_gpu(args):
+    import dataclasses
     from parflow_amd import api
     from parflow_amd.decks import deck
 
@@ -117,108 +223,98 @@ def run_gpu(args):
     _log(f"deck {args.config}: {prob.nx}x{prob.ny}x{prob.nz} "
          f"({prob.ncells} cells) generated in {time.time() - t0:.1f}s")
     gp = api.GPUProblem(prob)
-    ncell = prob.ncells
     K, W = args.steps, args.warmup
+    res = {"opts": opts, "dt": dt}
 
-    def make(mode):
-        if mode == "mixed":
-            return api.GPUSolver(gp, opts, precision=32, krylov=1)
-        return api.GPUSolver(gp, opts, precision=64, krylov=0)
-
-    results = {}
-    snaps = {}
-    shapes = []
-    for mode in ("mixed", "plain"):
-        if mode == "plain" and args.no_plain:
-            continue
-        sv = make(mode)
-        # warmup on a scratch state
-        if W > 0:
-            dp, ds = gp.vec(p0), gp.vec()
-            sv.advance_device(dp, ds, 0.0, dt, W)
-            del dp, ds
-        dp, ds = gp.vec(p0), gp.vec()
-        snap = [gp.vec() for _ in range(K)]
-        if mode == "mixed":
-            for _, pid, _ in PROF_KERNELS:
-                api.check(L.pfhip_prof_enable(getattr(api, pid), 200000), "prof")
-        newton, gmres, fevals = [], [], []
-        t = 0.0
-        api.check(L.pfhip_synchronize(), "sync")
-        tic = time.perf_counter()
-        for s in range(K):
-            lg = sv.advance_device(dp, ds, t, dt, 1)
-            t += float(lg["dt"][0])
-            newton.append(int(lg["newton"][0]))
-            gmres.append(int(lg["gmres"][0]))
-            fevals.append(int(lg["fevals"][0]))
-            api.check(L.pfhip_vec_linear_sum(gp.h, 1.0, dp.ptr, 0.0, dp.ptr,
-                                             snap[s].ptr, None), "snapshot")
-        api.check(L.pfhip_synchronize(), "sync")
-        elapsed = time.perf_counter() - tic
-        prof = None
-        if mode == "mixed":
-            prof = {}
-            for key, pid, _ in PROF_KERNELS:
-                ms = api.C.c_double()
-                n = api.C.c_long()
-                api.check(L.pfhip_prof_read(getattr(api, pid), api.C.byref(ms),
-                                            api.C.byref(n)), "prof_read")
-                api.check(L.pfhip_prof_disable(getattr(api, pid)), "prof_off")
-                prof[key] = (ms.value, n.value)
-        if mode == "mixed":
-            # smoother tile shape the multigrid's autotuner chose per grid level
-            # (at its first V-cycle)
-            L.pfhip_solver_mg.restype = api.C.c_void_p
-            L.pfhip_solver_mg.argtypes = [api.C.c_void_p]
-            L.pfhip_mg_pair_shapes.argtypes = [api.C.c_void_p, api.C.c_void_p, api.C.c_int]
-            arr = (api.C.c_int * 64)()
-            nl = L.pfhip_mg_pair_shapes(L.pfhip_solver_mg(sv.h), arr, 64)
-            shapes = list(arr[:nl])
-            _log(f"smoother tile shape per grid level (autotuned): {shapes}")
-        results[mode] = dict(elapsed=elapsed, newton=newton, gmres=gmres, fevals=fevals,
-                             value=ncell * K / elapsed, prof=prof)
-        snaps[mode] = snap
-        _log(f"{mode}: {K} steps in {elapsed:.3f}s = {ncell * K / elapsed:.4e} cell-steps/s "
-             f"newton={newton} gmres={gmres}")
+    # mixed: the timed run with the kernel timers on
+    sv = api.GPUSolver(gp, opts, precision=32, krylov=1)
+    for _, pid, _ in PROF_KERNELS:
+        api.check(L.pfhip_prof_enable(getattr(api, pid), 20000), "prof")
+    mixed, snap_m = _timed_run(api, gp, sv, p0, dt, K, W)
+    mixed["prof"] = _read_prof(api, [(k, pid) for k, pid, _ in PROF_KERNELS])
+    L.pfhip_solver_mg.restype = api.C.c_void_p
+    L.pfhip_solver_mg.argtypes = [api.C.c_void_p]
+    L.pfhip_mg_pair_shapes.argtypes = [api.C.c_void_p, api.C.c_void_p, api.C.c_int]
+    arr = (api.C.c_int * 64)()
+    nl = L.pfhip_mg_pair_shapes(L.pfhip_solver_mg(sv.h), arr, 64)
+    mixed["pair_shapes"] = list(arr[:nl])
+    _log(f"smoother tile shape per grid level (autotuned): {mixed['pair_shapes']}")
+    _log(f"mixed: {K} steps in {mixed['elapsed']:.3f}s = {mixed['value']:.4e} cell-steps/s "
+         f"newton={mixed['newton']} gmres={mixed['gmres']}")
+    if not args.no_phases:
+        mixed["phases"] = _phase_pass(api, gp, sv, p0, dt, K)
+    del sv
+    res["mixed"] = mixed
+    if not args.no_plain:
+        # plain: the reference's algorithm in FP64 with the deck's sweeps
+        sv = api.GPUSolver(gp, opts, precision=64, krylov=0)
+        plain, snap_p = _timed_run(api, gp, sv, p0, dt, K, W)
+        _log(f"plain: {K} steps in {plain['elapsed']:.3f}s = {plain['value']:.4e} "
+             f"cell-steps/s newton={plain['newton']} gmres={plain['gmres']}")
+        if not args.no_phases:
+            plain["phases"] = _phase_pass(api, gp, sv, p0, dt, K)
         del sv
-
-    errs = []
-    if "plain" in snaps:
+        plain["err"] = []
         for s in range(K):
-            a = prob.interior(gp.get(snaps["mixed"][s]))
-            b = prob.interior(gp.get(snaps["plain"][s]))
-            errs.append(float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3))))
-    results["mixed"]["pair_shapes"] = shapes
-    return prob, gp, results, errs
+            a = prob.interior(gp.get(snap_m[s]))
+            b = prob.interior(gp.get(snap_p[s]))
+            plain["err"].append(float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3))))
+        res["plain"] = plain
+        # plain at the reference's PFMG settings: 1 pre- and 1 post-sweep
+        # (PL/pf_pfmg.c:294-297 defaults)
+        ref_opts = dataclasses.replace(opts, num_pre_relax=1, num_post_relax=1)
+        sv = api.GPUSolver(gp, ref_opts, precision=64, krylov=0)
+        pref, _ = _timed_run(api, gp, sv, p0, dt, K, W, snapshot=False)
+        _log(f"plain (1+1 sweeps): {K} steps in {pref['elapsed']:.3f}s = "
+             f"{pref['value']:.4e} cell-steps/s newton={pref['newton']} gmres={pref['gmres']}")
+        del sv
+        res["plain_ref"] = pref
+        # the device plain state after step 10: the CPU baseline's start
+        ks = min(10, K - 1)
+        res["cpu_start"] = (ks, gp.get(snap_p[ks - 1]) if ks > 0 else p0,
+                            gp.get(snap_p[ks]), plain["newton"][ks])
+        del snap_p
+    if not args.no_water_balance:
+        res["water_balance"] = _water_balance(gp, prob, snap_m, p0, dt)
+        _log(f"water balance of the mixed run (ResidualTol {opts.residual_tol}): "
+             f"worst {max(res['water_balance']):.2e}")
+    del snap_m
+    return prob, gp, res
 
 
-def run_cpu_baseline(args, cores):
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_baseline(args, cores, prob, opts, dt, start):
     """The oracle (C restatement, OpenMP over cells) on a bounded sample of
-    the same workload: the deck's fields cropped to the first
-    (nx_s x ny_s x nz) columns, one time step."""
+    the same workload: one time step of the full fields, step ks + 1 taken
+    from the plain device path's state after step ks (a step with the
+    run's typical Newton count; step 1 needs the most).  Its result is
+    checked against the device's step ks + 1 as well."""
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
     from oracle import oracle as O
-    from parflow_amd.decks import deck
-    from parflow_amd.problem import Problem
-
-    prob, opts, p0, dt, nsteps_cfg, extras = deck(args.config)
-    n = args.cpu_sample
-    I = lambda a: prob.interior(a)[:, :n, :n]
-    sub = Problem(nx=n, ny=n, nz=prob.nz, dx=prob.dx, dy=prob.dy, dz=prob.dz,
-                  x0=prob.x0, y0=prob.y0, z0=prob.z0, kx=I(prob.g_kx), poro=I(prob.g_poro),
-                  ss=I(prob.g_ss), alpha=I(prob.g_alpha), nvg=I(prob.g_nvg),
-                  sres=I(prob.g_sres), ssat=I(prob.g_ssat), bc=prob.bc)
-    op = O.OracleProblem(sub)
-    ps = sub.hydrostatic(0.0, prob.bc[4].z_ref)
+    ks, p_start, p_dev_next, newton_dev = start
+    op = O.OracleProblem(prob)
     tic = time.perf_counter()
-    r, ph, sh, log = O.advance(op, opts, ps, 0.0, dt, 1, keep_history=False)
+    r, ph, sh, log = O.advance(op, opts, p_start, ks * dt, dt, 1, keep_history=False)
     el = time.perf_counter() - tic
-    return dict(value=sub.ncells / el, unit="cell-timesteps/s", cores=int(cores),
-                kind="port",
-                sample=f"oracle (C restatement, OpenMP) on the {n}x{n}x{prob.nz} "
-                       f"corner of the {args.config} fields, 1 step of dt={dt} h, "
-                       f"{el:.1f}s, newton={log.newton[0]}, status={r}")
+    a, b = prob.interior(ph), prob.interior(p_dev_next)
+    err = float(np.max(np.abs(b - a) / np.maximum(np.abs(a), 1e-3)))
+    return dict(value=prob.ncells / el, unit="cell-timesteps/s", cores=int(cores),
+                kind="port", cpu=_cpu_model(),
+                sample=f"oracle (C restatement, OpenMP) on the full {prob.nx}x{prob.ny}x"
+                       f"{prob.nz} {args.config} fields: step {ks + 1} (dt={dt} h) from the "
+                       f"plain device state after step {ks}, {el:.1f} s, newton="
+                       f"{log.newton[0]} (device {newton_dev}), status={r}, device vs oracle "
+                       f"max rel p diff {err:.1e}",
+                newton=int(log.newton[0]), seconds=el, device_rel_diff=err)
 
 
 def kernel_probe(args):
@@ -263,12 +359,12 @@ def kernel_probe(args):
 def run_ensemble(args):
     """The hillslope ensemble (deck c4): every member advanced together with
     batched launches, then a sample of the same members advanced one at a
-    time on the single-problem path; prints one JSON line."""
+    time on the single-problem path; returns the JSON line's fields."""
     from parflow_amd import api
     from parflow_amd.decks import c4_members
 
     L = api.lib()
-    K = args.steps if args.steps is not None else 24
+    K = args.ens_steps if args.config != "c4" else (args.steps if args.steps is not None else 24)
     W = args.warmup
     tic = time.perf_counter()
     members = c4_members(args.members)
@@ -347,7 +443,51 @@ def run_ensemble(args):
             "speedup": value / seq_value,
         },
     }
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def kernel_rows(prob, gp, prof):
+    """Roofline rows of the timed run: achieved = algorithmic bytes of a
+    launch / its average HIP-event duration; one row per kernel, and one
+    per family (its forms' bytes and times summed)."""
+    bpc_all = kernel_bytes_per_cell(gp)
+    traffic = load_traffic()
+    rows = {}
+    for key, _, kname in PROF_KERNELS:
+        ms_tot, nlaunch = prof[key]
+        if nlaunch == 0:
+            continue
+        avg_s = (ms_tot / 1e3) / nlaunch
+        bpl = bpc_all[key] * prob.ncells
+        ach = bpl / avg_s / 1e9
+        rows[key] = {"kernel": kname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic.get(key),
+                     "bytes_per_cell": bpc_all[key], "avg_launch_us": avg_s * 1e6,
+                     "launches": nlaunch, "total_ms": ms_tot}
+    out = []
+    members = set()
+    for fam, (keys, fname) in FAMILIES.items():
+        parts = [rows[k] for k in keys if k in rows]
+        if not parts:
+            continue
+        members.update(keys)
+        n = sum(r["launches"] for r in parts)
+        ms = sum(r["total_ms"] for r in parts)
+        nbytes = sum(r["bytes_per_cell"] * prob.ncells * r["launches"] for r in parts)
+        avg_s = ms / 1e3 / n
+        ach = nbytes / n / avg_s / 1e9
+        tr = ([r["traffic"] for r in parts] if all(r["traffic"] for r in parts) else None)
+        out.append({"kernel": fname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                    "traffic": (sum(t * r["launches"] for t, r in zip(tr, parts)) / n
+                                if tr else None),
+                    "bytes_per_cell": nbytes / n / prob.ncells, "avg_launch_us": avg_s * 1e6,
+                    "launches": n, "total_ms": ms,
+                    "forms": {k: {f: rows[k][f] for f in ("launches", "avg_launch_us", "frac",
+                                                          "bytes_per_cell", "traffic")}
+                              for k in keys if k in rows}})
+    out += [r for k, r in rows.items() if k not in members]
+    return out
 
 
 def main():
@@ -358,11 +498,15 @@ def main():
     ap.add_argument("--members", type=int, default=512, help="c4 ensemble size")
     ap.add_argument("--ensemble-stride", type=int, default=16,
                     help="c4: every stride-th member runs one at a time for the comparison")
+    ap.add_argument("--ens-steps", type=int, default=24,
+                    help="steps of the ensemble extra of the c1-c3 line")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=384)
+    ap.add_argument("--no-phases", action="store_true")
+    ap.add_argument("--no-water-balance", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="run each measured kernel once (for counter runs) and exit")
     args = ap.parse_args()
@@ -374,29 +518,15 @@ def main():
               "(multi-GPU domain decomposition is design-only).", file=sys.stderr)
         sys.exit(2)
     if args.config == "c4":
-        run_ensemble(args)
+        print(json.dumps(run_ensemble(args)), flush=True)
         return
     if args.steps is None:
         args.steps = 20
 
-    prob, gp, res, errs = run_gpu(args)
-    mixed = res["mixed"]
+    prob, gp, res = run_gpu(args)
+    mixed, opts, dt = res["mixed"], res["opts"], res["dt"]
     K = args.steps
-    bpc_all = kernel_bytes_per_cell(gp)
-    traffic = load_traffic()
-    kernels = []
-    for key, _, kname in PROF_KERNELS:
-        ms_tot, nlaunch = mixed["prof"][key]
-        if nlaunch == 0:
-            continue
-        avg_s = (ms_tot / 1e3) / nlaunch
-        bpl = bpc_all[key] * prob.ncells
-        ach = bpl / avg_s / 1e9
-        kernels.append({"kernel": kname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                        "traffic": traffic.get(key), "bytes_per_cell": bpc_all[key],
-                        "avg_launch_us": avg_s * 1e6, "launches": nlaunch,
-                        "total_ms": ms_tot})
+    kernels = kernel_rows(prob, gp, mixed["prof"])
     # the dominant kernel: most total time in the timed region
     dom = max(kernels, key=lambda r: r["total_ms"])
     line = {
@@ -411,10 +541,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64 (FP32 preconditioner)",
-        "data": "synthetic (seeded FFT Gaussian random field, seeded van Genuchten fields)",
+        "data": "synthetic (seeded FFT Gaussian random field, seeded van Genuchten fields)"
+                if not prob.overland else "synthetic (seeded tilted-V slopes)",
         "config": {
-            "workload": f"{args.config}: {prob.nx}x{prob.ny}x{prob.nz} heterogeneous Richards, "
-                        "FGMRES(30) + FP32 V-cycle (2+2 RB sweeps), dt=0.5 h",
+            "workload": _workload_label(args.config, prob, opts, dt, "FGMRES"),
             "cells": prob.ncells,
             "newton_per_step": mixed["newton"],
             "gmres_per_step": mixed["gmres"],
@@ -426,20 +556,53 @@ def main():
                                           "avg_launch_us", "launches")},
         "roofline_kernels": kernels,
     }
+    if "phases" in mixed:
+        line["phases_ms_per_step"] = mixed["phases"]
+    if "water_balance" in res:
+        line["water_balance"] = {
+            "residual_over_storage_change_per_step": res["water_balance"],
+            "residual_tol": opts.residual_tol,
+            "note": "mixed run at the deck's ResidualTol; the 1e-10 bar is tested at "
+                    "ResidualTol 1e-12 (tests/test_gpu_whole_configs.py)"}
     if "plain" in res:
-        plain = res["plain"]
+        plain, pref = res["plain"], res["plain_ref"]
         line["plain_fp64"] = {
             "value": plain["value"],
+            "workload": _workload_label(args.config, prob, opts, dt, "GMRES"),
             "ms_per_step": 1e3 * plain["elapsed"] / K,
             "newton_per_step": plain["newton"],
             "gmres_per_step": plain["gmres"],
             "mixed_over_plain": mixed["value"] / plain["value"],
-            "max_rel_err_vs_plain_per_step": errs,
+            "max_rel_err_vs_plain_per_step": plain["err"],
         }
-    if not args.no_cpu:
+        if "phases" in plain:
+            line["plain_fp64"]["phases_ms_per_step"] = plain["phases"]
+        line["plain_fp64_reference_pfmg_settings"] = {
+            "value": pref["value"],
+            "workload": _workload_label(args.config, prob, dataclasses_replace(
+                opts, num_pre_relax=1, num_post_relax=1), dt, "GMRES"),
+            "ms_per_step": 1e3 * pref["elapsed"] / K,
+            "newton_per_step": pref["newton"],
+            "gmres_per_step": pref["gmres"],
+            "mixed_over_this": mixed["value"] / pref["value"],
+        }
+    if not args.no_cpu and "cpu_start" in res:
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        line["cpu_baseline"] = run_cpu_baseline(args, cores)
+        line["cpu_baseline"] = run_cpu_baseline(args, cores, prob, opts, dt, res["cpu_start"])
+    if not args.no_ensemble:
+        del gp
+        e = run_ensemble(args)
+        line["ensemble"] = {"metric": e["metric"], "value": e["value"], "steps": e["steps"],
+                            "workload": e["config"]["workload"],
+                            "newton_per_member_step": e["config"]["newton_per_member_step"],
+                            "gmres_per_member_step": e["config"]["gmres_per_member_step"],
+                            **e["ensemble"]}
     print(json.dumps(line), flush=True)
+
+
+def dataclasses_replace(obj, **kw):
+    import dataclasses
+    return dataclasses.replace(obj, **kw)
 
 
 if __name__ == "__main__":

@@ -152,16 +152,59 @@ int  pfhip_overland_flow_eval_kin(pfhip_problem *pb, const double *p,
                                   double *out3, int fcn, void *stream);
 
 /* ---- N_Vector operations (PL/n_vector.h -> PL/vector_utilities.c) ---- */
-/* All act on the interior cells of ghosted vectors of problem pb. */
+/* All act on the interior cells of ghosted vectors of problem pb (ghost
+ * entries are neither read nor written); reductions are deterministic
+ * (fixed-order block partials) and return their result on the host. */
+/* PFVLinearSum (:80): z = a x + b y */
 int  pfhip_vec_linear_sum(pfhip_problem *pb, double a, const double *x,
                           double b, const double *y, double *z, void *stream);
+/* PFVConstInit (:233): z = c */
 int  pfhip_vec_const(pfhip_problem *pb, double c, double *z, void *stream);
+/* PFVProd (:280): z = x y */
+int  pfhip_vec_prod(pfhip_problem *pb, const double *x, const double *y,
+                    double *z, void *stream);
+/* PFVDiv (:352): z = x / y */
+int  pfhip_vec_div(pfhip_problem *pb, const double *x, const double *y,
+                   double *z, void *stream);
+/* PFVScale (:423): z = c x */
+int  pfhip_vec_scale(pfhip_problem *pb, double c, const double *x, double *z,
+                     void *stream);
+/* PFVAbs (:500): z = |x| */
+int  pfhip_vec_abs(pfhip_problem *pb, const double *x, double *z, void *stream);
+/* PFVInv (:558): z = 1 / x */
+int  pfhip_vec_inv(pfhip_problem *pb, const double *x, double *z, void *stream);
+/* PFVAddConst (:617): z = x + b */
+int  pfhip_vec_add_const(pfhip_problem *pb, const double *x, double b,
+                         double *z, void *stream);
+/* PFVDotProd (:677): sum x y */
 int  pfhip_vec_dot(pfhip_problem *pb, const double *x, const double *y,
                    double *result, void *stream);
+/* PFVMaxNorm (:748): max |x| */
 int  pfhip_vec_max_norm(pfhip_problem *pb, const double *x, double *result,
                         void *stream);
+/* PFVWrmsNorm (:805): sqrt(sum (x w)^2 / N), N the interior cell count */
+int  pfhip_vec_wrms_norm(pfhip_problem *pb, const double *x, const double *w,
+                         double *result, void *stream);
+/* PFVWL2Norm (:877): sqrt(sum (x w)^2) */
+int  pfhip_vec_wl2_norm(pfhip_problem *pb, const double *x, const double *w,
+                        double *result, void *stream);
+/* PFVL1Norm (:949): sum |x| */
 int  pfhip_vec_l1_norm(pfhip_problem *pb, const double *x, double *result,
                        void *stream);
+/* PFVMin (:1005): min x */
+int  pfhip_vec_min(pfhip_problem *pb, const double *x, double *result,
+                   void *stream);
+/* PFVConstrProdPos (:1145): *result = 0 (FALSE) if some c != 0 has
+ * x c <= 0, else 1 */
+int  pfhip_vec_constr_prod_pos(pfhip_problem *pb, const double *c,
+                               const double *x, int *result, void *stream);
+/* PFVCompare (:1228): z = (|x| >= c) ? 1 : 0 */
+int  pfhip_vec_compare(pfhip_problem *pb, double c, const double *x, double *z,
+                       void *stream);
+/* PFVInvTest (:1288): z = 1 / x where x != 0 (unchanged where x == 0);
+ * *result = 0 (FALSE) if some x == 0, else 1 */
+int  pfhip_vec_inv_test(pfhip_problem *pb, const double *x, double *z,
+                        int *result, void *stream);
 
 /* ---- water balance ------------------------------------------------------ */
 /* Water-balance terms of one step, as volumes over the step, into host
@@ -381,16 +424,36 @@ int  pfhip_ensemble_advance_richards(pfhip_ensemble *e, double *const *p,
 /* scheduler rounds and batched launches of the last advance */
 int  pfhip_ensemble_stats(pfhip_ensemble *e, long *rounds, long *launches);
 
-/* ---- live kernel timing (HIP events on the launching stream) --------- */
+/* ---- live kernel and phase timing (HIP events on the launching stream) */
 /* No counterpart in the reference (its timers are host wall clocks,
- * PL/timing.c:75).  kernel_id: */
+ * PL/timing.c:75).  kernel_id: a kernel ... */
 #define PFHIP_PROF_RESIDUAL  0    /* the residual kernel of NlFunctionEval  */
 #define PFHIP_PROF_CLOSURE   1    /* van Genuchten closure kernel           */
 #define PFHIP_PROF_JACOBIAN  2    /* RichardsJacobianEval assembly kernel   */
-#define PFHIP_PROF_RBGS      3    /* finest-level red/black smoothing launch */
+#define PFHIP_PROF_RBGS      3    /* finest-level red/black smoothing launch
+                                     with level-precision operands          */
 #define PFHIP_PROF_RESIDUAL_JV 4  /* residual kernel in the J.v quotient form */
 #define PFHIP_PROF_RESIDUAL_STEP 5 /* residual kernel at a line-search point */
-#define PFHIP_PROF_NKERNELS  6
+#define PFHIP_PROF_RBGS_DOUT 6    /* the V-cycle's last finest-level pair,
+                                     writing the FP64 result               */
+#define PFHIP_PROF_RBGS_DIN  7    /* the V-cycle's first finest-level pair,
+                                     reading the FP64 right-hand side      */
+/* ... or a phase of the nonlinear solve, after the reference's timers: */
+#define PFHIP_PROF_PH_NL_F_EVAL 8 /* residual evaluations, F and line-search
+                                     points (NlFunctionEval timer,
+                                     PL/nl_function_eval.c:293)            */
+#define PFHIP_PROF_PH_JV     9    /* J.v products of GMRES (KINSpgmrAtimes:
+                                     difference quotient or Matvec,
+                                     PL/matvec.c:89)                        */
+#define PFHIP_PROF_PH_PC_SETUP 10 /* PC matrix and multigrid setup
+                                     (KinsolPC init, PL/kinsol_pc.c:97)    */
+#define PFHIP_PROF_PH_PC_SOLVE 11 /* V-cycle applications (PFMG timer,
+                                     PL/pf_pfmg.c:98)                      */
+#define PFHIP_PROF_PH_ORTHO  12   /* Gram-Schmidt of the Krylov basis
+                                     (SpgmrSolve's ModifiedGS, or CGS2)    */
+#define PFHIP_PROF_PH_KINSOL 13   /* whole nonlinear solves (KINSol timer,
+                                     PL/kinsol_nonlin_solver.c:368)        */
+#define PFHIP_PROF_NKERNELS  14
 int  pfhip_prof_enable(int kernel_id, int max_launches);
 int  pfhip_prof_reset(int kernel_id);
 int  pfhip_prof_read(int kernel_id, double *total_ms, long *launches);

@@ -24,9 +24,11 @@ _lib = None
 _dp = C.POINTER(C.c_double)
 MAXI = 8
 
-# kernel ids of pfhip_prof_* (include/pfhip.h)
+# kernel and phase timer ids of pfhip_prof_* (include/pfhip.h)
 (PROF_RESIDUAL, PROF_CLOSURE, PROF_JACOBIAN, PROF_RBGS, PROF_RESIDUAL_JV,
- PROF_RESIDUAL_STEP) = 0, 1, 2, 3, 4, 5
+ PROF_RESIDUAL_STEP, PROF_RBGS_DOUT, PROF_RBGS_DIN) = range(8)
+(PROF_PH_NL_F_EVAL, PROF_PH_JV, PROF_PH_PC_SETUP, PROF_PH_PC_SOLVE, PROF_PH_ORTHO,
+ PROF_PH_KINSOL) = range(8, 14)
 
 
 class PFHIPError(RuntimeError):
@@ -164,7 +166,10 @@ EXPORTS = [
     "pfhip_surface_predictor",
     "pfhip_richards_jacobian_eval", "pfhip_matvec", "pfhip_overland_flow_eval_kin",
     "pfhip_vec_linear_sum", "pfhip_vec_const", "pfhip_vec_dot", "pfhip_vec_max_norm",
-    "pfhip_vec_l1_norm", "pfhip_mg_create", "pfhip_mg_create_full", "pfhip_mg_destroy",
+    "pfhip_vec_l1_norm", "pfhip_vec_prod", "pfhip_vec_div", "pfhip_vec_scale", "pfhip_vec_abs",
+    "pfhip_vec_inv", "pfhip_vec_add_const", "pfhip_vec_wrms_norm", "pfhip_vec_wl2_norm",
+    "pfhip_vec_min", "pfhip_vec_constr_prod_pos", "pfhip_vec_compare", "pfhip_vec_inv_test",
+    "pfhip_mg_create", "pfhip_mg_create_full", "pfhip_mg_destroy",
     "pfhip_mg_num_levels",
     "pfhip_mg_setup", "pfhip_mg_apply", "pfhip_solver_create", "pfhip_solver_destroy",
     "pfhip_kinsol_nonlin_solver", "pfhip_advance_richards", "pfhip_water_balance_storage",
@@ -216,6 +221,18 @@ def lib():
             getattr(L, f).argtypes = [vp, vp, vp, C.POINTER(C.c_double), vp]
         for f in ("pfhip_vec_max_norm", "pfhip_vec_l1_norm"):
             getattr(L, f).argtypes = [vp, vp, C.POINTER(C.c_double), vp]
+        for f in ("pfhip_vec_prod", "pfhip_vec_div"):
+            getattr(L, f).argtypes = [vp, vp, vp, vp, vp]
+        for f in ("pfhip_vec_abs", "pfhip_vec_inv"):
+            getattr(L, f).argtypes = [vp, vp, vp, vp]
+        L.pfhip_vec_scale.argtypes = [vp, C.c_double, vp, vp, vp]
+        L.pfhip_vec_compare.argtypes = [vp, C.c_double, vp, vp, vp]
+        L.pfhip_vec_add_const.argtypes = [vp, vp, C.c_double, vp, vp]
+        for f in ("pfhip_vec_wrms_norm", "pfhip_vec_wl2_norm"):
+            getattr(L, f).argtypes = [vp, vp, vp, C.POINTER(C.c_double), vp]
+        L.pfhip_vec_min.argtypes = [vp, vp, C.POINTER(C.c_double), vp]
+        L.pfhip_vec_constr_prod_pos.argtypes = [vp, vp, vp, C.POINTER(C.c_int), vp]
+        L.pfhip_vec_inv_test.argtypes = [vp, vp, vp, C.POINTER(C.c_int), vp]
         L.pfhip_water_balance_storage.argtypes = [vp, vp, vp, C.POINTER(C.c_double),
                                                   C.POINTER(C.c_double), vp]
         L.pfhip_mg_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

@@ -55,13 +55,13 @@ extern "C" const char *pfhip_last_error(void) { return g_err; }
 #define STREAM(s) ((hipStream_t)(s))
 
 // --------------------------------------------------------------------------
-// live per-kernel timing with HIP events (pfhip_prof_*): when a kernel id is
-// enabled, every launch of that kernel is bracketed by an event pair on the
-// stream it is launched on.
+// live timing with HIP events (pfhip_prof_*): when a timer id is enabled,
+// every launch of that kernel (or every call of that solver phase) is
+// bracketed by an event pair on the stream it is launched on.
 // --------------------------------------------------------------------------
 struct ProfState {
   int enabled[PFHIP_PROF_NKERNELS];
-  int cap;
+  int cap[PFHIP_PROF_NKERNELS];
   int count[PFHIP_PROF_NKERNELS];
   hipEvent_t *ev[PFHIP_PROF_NKERNELS];
 };
@@ -73,14 +73,26 @@ extern "C" int pfhip_prof_enable(int kernel_id, int max_launches)
     pfhip_set_error("prof_enable: bad kernel id %d", kernel_id);
     return PFHIP_ERR_ARG;
   }
+  g_prof.enabled[kernel_id] = 0;
   if (g_prof.ev[kernel_id]) {
-    for (int i = 0; i < 2 * g_prof.cap; i++) hipEventDestroy(g_prof.ev[kernel_id][i]);
+    for (int i = 0; i < 2 * g_prof.cap[kernel_id]; i++) hipEventDestroy(g_prof.ev[kernel_id][i]);
     free(g_prof.ev[kernel_id]);
+    g_prof.ev[kernel_id] = nullptr;
+    g_prof.cap[kernel_id] = 0;
   }
-  g_prof.cap = max_launches;
-  g_prof.ev[kernel_id] = (hipEvent_t *)calloc(2 * max_launches, sizeof(hipEvent_t));
-  for (int i = 0; i < 2 * max_launches; i++)
-    HIPCHK(hipEventCreate(&g_prof.ev[kernel_id][i]));
+  hipEvent_t *ev = (hipEvent_t *)calloc(2 * (size_t)max_launches, sizeof(hipEvent_t));
+  if (!ev) { pfhip_set_error("prof_enable: out of host memory"); return PFHIP_ERR_NOMEM; }
+  for (int i = 0; i < 2 * max_launches; i++) {
+    hipError_t e = hipEventCreate(&ev[i]);
+    if (e != hipSuccess) {
+      for (int q = 0; q < i; q++) hipEventDestroy(ev[q]);
+      free(ev);
+      pfhip_set_error("prof_enable: hipEventCreate: %s", hipGetErrorString(e));
+      return PFHIP_ERR_HIP;
+    }
+  }
+  g_prof.ev[kernel_id] = ev;
+  g_prof.cap[kernel_id] = max_launches;
   g_prof.count[kernel_id] = 0;
   g_prof.enabled[kernel_id] = 1;
   return PFHIP_OK;
@@ -98,7 +110,7 @@ extern "C" int pfhip_prof_read(int kernel_id, double *total_ms, long *launches)
   if (kernel_id < 0 || kernel_id >= PFHIP_PROF_NKERNELS) return PFHIP_ERR_ARG;
   double tot = 0.0;
   int n = g_prof.count[kernel_id];
-  if (n > g_prof.cap) n = g_prof.cap;
+  if (n > g_prof.cap[kernel_id]) n = g_prof.cap[kernel_id];
   for (int i = 0; i < n; i++) {
     float ms = 0.f;
     HIPCHK(hipEventSynchronize(g_prof.ev[kernel_id][2 * i + 1]));
@@ -122,7 +134,7 @@ extern "C" void pfhip_prof_mark(int kernel_id, int end, void *stream)
 {
   if (!g_prof.enabled[kernel_id]) return;
   int c = g_prof.count[kernel_id];
-  if (c >= g_prof.cap) return;
+  if (c >= g_prof.cap[kernel_id]) return;
   hipEventRecord(g_prof.ev[kernel_id][2 * c + end], STREAM(stream));
   if (end) g_prof.count[kernel_id] = c + 1;
 }
@@ -514,14 +526,15 @@ __device__ __forceinline__ double block_reduce(double v)
   return r;
 }
 
-enum RedOp { RED_DOT = 0, RED_MAXABS = 1, RED_L1 = 2, RED_SCSTEP = 3, RED_DQ = 4 };
+enum RedOp { RED_DOT = 0, RED_MAXABS = 1, RED_L1 = 2, RED_SCSTEP = 3, RED_DQ = 4,
+             RED_WL2 = 5, RED_NEGMAX = 6, RED_CONSTR = 7 };
 
 template <int OP>
 __global__ __launch_bounds__(256) void k_reduce(Geom g, const double *__restrict__ x,
                                                 const double *__restrict__ y,
                                                 double *__restrict__ partial, int nb)
 {
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  double a0 = (OP == RED_NEGMAX) ? -INFINITY : 0.0, a1 = 0.0, a2 = 0.0;
   int rows = g.ny * g.nz;
   for (int r = blockIdx.x; r < rows; r += gridDim.x) {
     int j = r % g.ny, k = r / g.ny;
@@ -529,6 +542,9 @@ __global__ __launch_bounds__(256) void k_reduce(Geom g, const double *__restrict
     for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
       int ip = base + i;
       if (OP == RED_DOT) a0 += x[ip] * y[ip];
+      else if (OP == RED_WL2) { double pr = x[ip] * y[ip]; a0 += pr * pr; }
+      else if (OP == RED_NEGMAX) a0 = fmax(a0, -x[ip]);
+      else if (OP == RED_CONSTR) a0 += (y[ip] != 0.0 && x[ip] * y[ip] <= 0.0) ? 1.0 : 0.0;
       else if (OP == RED_MAXABS) a0 = fmax(a0, fabs(x[ip]));
       else if (OP == RED_L1) a0 += fabs(x[ip]);
       else if (OP == RED_SCSTEP) a0 = fmax(a0, fabs(y[ip] / (1.0 + fabs(x[ip]))));
@@ -540,7 +556,7 @@ __global__ __launch_bounds__(256) void k_reduce(Geom g, const double *__restrict
       }
     }
   }
-  const bool MX = (OP == RED_MAXABS || OP == RED_SCSTEP);
+  const bool MX = (OP == RED_MAXABS || OP == RED_SCSTEP || OP == RED_NEGMAX);
   a0 = block_reduce<MX>(a0);
   if (OP == RED_DQ) { a1 = block_reduce<false>(a1); a2 = block_reduce<false>(a2); }
   if (threadIdx.x == 0) {
@@ -556,7 +572,7 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double *__restrict__
                                                       double *__restrict__ out)
 {
   for (int v = 0; v < nv; v++) {
-    double a = 0.0;
+    double a = MAX ? -INFINITY : 0.0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
       double t = partial[v * nb + i];
       a = MAX ? fmax(a, t) : a + t;
@@ -583,7 +599,7 @@ static int run_reduce(pfhip_problem *pb, const double *x, const double *y,
   hipLaunchKernelGGL(k_reduce<OP>, dim3(nb), dim3(256), 0, STREAM(stream), g, x,
                      y, pb->red_partial, nb);
   LAUNCHCHK();
-  const bool MX = (OP == RED_MAXABS || OP == RED_SCSTEP);
+  const bool MX = (OP == RED_MAXABS || OP == RED_SCSTEP || OP == RED_NEGMAX);
   hipLaunchKernelGGL(k_reduce_final<MX>, dim3(1), dim3(256), 0, STREAM(stream),
                      pb->red_partial, nb, nv, pb->red_dev);
   LAUNCHCHK();
@@ -635,6 +651,61 @@ extern "C" int pfhip_vec_dq_sums(pfhip_problem *pb, const double *u,
   *sutsv = pb->h_red[0];
   *vtv = pb->h_red[1];
   *l1 = pb->h_red[2];
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_wl2_norm(pfhip_problem *pb, const double *x, const double *w,
+                                  double *result, void *stream)
+{
+  int rc = run_reduce<RED_WL2>(pb, x, w, 1, stream);
+  if (rc) return rc;
+  *result = sqrt(pb->h_red[0]);
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_wrms_norm(pfhip_problem *pb, const double *x, const double *w,
+                                   double *result, void *stream)
+{
+  int rc = run_reduce<RED_WL2>(pb, x, w, 1, stream);
+  if (rc) return rc;
+  *result = sqrt(pb->h_red[0] / (double)pb->ncell);
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_min(pfhip_problem *pb, const double *x, double *result, void *stream)
+{
+  int rc = run_reduce<RED_NEGMAX>(pb, x, x, 1, stream);
+  if (rc) return rc;
+  *result = -pb->h_red[0];
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_constr_prod_pos(pfhip_problem *pb, const double *c, const double *x,
+                                         int *result, void *stream)
+{
+  int rc = run_reduce<RED_CONSTR>(pb, x, c, 1, stream);
+  if (rc) return rc;
+  *result = (pb->h_red[0] == 0.0) ? 1 : 0;
+  return PFHIP_OK;
+}
+
+__global__ void k_inv_test(Geom g, const double *__restrict__ x, double *__restrict__ z,
+                           double *__restrict__ partial);
+
+extern "C" int pfhip_vec_inv_test(pfhip_problem *pb, const double *x, double *z, int *result,
+                                  void *stream)
+{
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  hipLaunchKernelGGL(k_inv_test, dim3(nb), dim3(256), 0, STREAM(stream), g, x, z,
+                     pb->red_partial);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, STREAM(stream),
+                     pb->red_partial, nb, 1, pb->red_dev);
+  LAUNCHCHK();
+  int rc = fetch_results(pb, 1, stream);
+  if (rc) return rc;
+  *result = (pb->h_red[0] == 0.0) ? 1 : 0;
   return PFHIP_OK;
 }
 
@@ -1126,6 +1197,92 @@ extern "C" int pfhip_vec_const(pfhip_problem *pb, double c, double *z, void *str
   hipLaunchKernelGGL(k_const, grid3(g), BLOCK3, 0, STREAM(stream), g, c, z);
   LAUNCHCHK();
   return PFHIP_OK;
+}
+
+// The other N_Vector operations of PL/n_vector.h (PL/vector_utilities.c),
+// elementwise over the interior cells with the reference's operation per
+// cell; ghost entries are left as they are.
+enum EwOp { EW_PROD = 0, EW_DIV = 1, EW_ABS = 2, EW_INV = 3, EW_ADDC = 4, EW_CMP = 5 };
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_vec_ew(Geom g, double c, const double *__restrict__ x,
+                                                const double *__restrict__ y,
+                                                double *__restrict__ z)
+{
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  int ip = (i + 1) + (j + 1) * g.sy + (k + 1) * g.sz;
+  double v = x[ip];
+  if (OP == EW_PROD) z[ip] = v * y[ip];
+  else if (OP == EW_DIV) z[ip] = v / y[ip];
+  else if (OP == EW_ABS) z[ip] = fabs(v);
+  else if (OP == EW_INV) z[ip] = 1.0 / v;
+  else if (OP == EW_ADDC) z[ip] = v + c;
+  else z[ip] = (fabs(v) >= c) ? 1.0 : 0.0;
+}
+
+template <int OP>
+static int vec_ew(pfhip_problem *pb, double c, const double *x, const double *y, double *z,
+                  void *stream)
+{
+  if (!pb || !x || !z || ((OP == EW_PROD || OP == EW_DIV) && !y)) {
+    pfhip_set_error("N_Vector operation: NULL argument");
+    return PFHIP_ERR_ARG;
+  }
+  Geom g = geom_of(pb);
+  hipLaunchKernelGGL(k_vec_ew<OP>, grid3(g), BLOCK3, 0, STREAM(stream), g, c, x, y, z);
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_vec_prod(pfhip_problem *pb, const double *x, const double *y, double *z,
+                              void *stream)
+{
+  return vec_ew<EW_PROD>(pb, 0.0, x, y, z, stream);
+}
+extern "C" int pfhip_vec_div(pfhip_problem *pb, const double *x, const double *y, double *z,
+                             void *stream)
+{
+  return vec_ew<EW_DIV>(pb, 0.0, x, y, z, stream);
+}
+extern "C" int pfhip_vec_abs(pfhip_problem *pb, const double *x, double *z, void *stream)
+{
+  return vec_ew<EW_ABS>(pb, 0.0, x, nullptr, z, stream);
+}
+extern "C" int pfhip_vec_inv(pfhip_problem *pb, const double *x, double *z, void *stream)
+{
+  return vec_ew<EW_INV>(pb, 0.0, x, nullptr, z, stream);
+}
+extern "C" int pfhip_vec_add_const(pfhip_problem *pb, const double *x, double b, double *z,
+                                   void *stream)
+{
+  return vec_ew<EW_ADDC>(pb, b, x, nullptr, z, stream);
+}
+extern "C" int pfhip_vec_compare(pfhip_problem *pb, double c, const double *x, double *z,
+                                 void *stream)
+{
+  return vec_ew<EW_CMP>(pb, c, x, nullptr, z, stream);
+}
+
+// InvTest: z = 1/x where x != 0 (z keeps its value where x == 0), and a
+// count of the zeros; row-owning blocks as the reductions
+__global__ __launch_bounds__(256) void k_inv_test(Geom g, const double *__restrict__ x,
+                                                  double *__restrict__ z,
+                                                  double *__restrict__ partial)
+{
+  double zeros = 0.0;
+  int rows = g.ny * g.nz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    int j = r % g.ny, k = r / g.ny;
+    int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+      double v = x[base + i];
+      if (v == 0.0) zeros += 1.0;
+      else z[base + i] = 1.0 / v;
+    }
+  }
+  zeros = block_reduce<false>(zeros);
+  if (threadIdx.x == 0) partial[blockIdx.x] = zeros;
 }
 
 // --------------------------------------------------------------------------

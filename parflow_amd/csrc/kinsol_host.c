@@ -145,10 +145,23 @@ int pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
 }
 
 /* ---- callbacks -------------------------------------------------------- */
-static int func(pfhip_solver *s, const double *u, double *f)
+/* phase timers (pfhip_prof_*, PFHIP_PROF_PH_*): an event pair around each
+ * call of a solver phase when that timer is enabled */
+#define PH_BEGIN(id) pfhip_prof_mark((id), 0, NULL)
+#define PH_END(id) pfhip_prof_mark((id), 1, NULL)
+
+static int func_eval(pfhip_solver *s, const double *u, double *f)
 {
   s->nfe++;
   return s->ops->nl_function_eval(s->pb, u, f, s->cur_p_old, s->cur_s_old, s->dt, NULL);
+}
+
+static int func(pfhip_solver *s, const double *u, double *f)
+{
+  PH_BEGIN(PFHIP_PROF_PH_NL_F_EVAL);
+  int rc = func_eval(s, u, f);
+  PH_END(PFHIP_PROF_PH_NL_F_EVAL);
+  return rc;
 }
 
 static int l2norm(pfhip_solver *s, const double *x, double *r)
@@ -162,7 +175,10 @@ static int l2norm(pfhip_solver *s, const double *x, double *r)
 /* KINSolInitPC -> KinsolPCInitInstanceXtra: PC matrix + multigrid setup */
 static int pc_setup(pfhip_solver *s)
 {
-  CHK(s->ops->mg_setup_jacobian(s->mg, s->uu, s->dt, NULL));
+  PH_BEGIN(PFHIP_PROF_PH_PC_SETUP);
+  int rc = s->ops->mg_setup_jacobian(s->mg, s->uu, s->dt, NULL);
+  PH_END(PFHIP_PROF_PH_PC_SETUP);
+  CHK(rc);
   s->npe++;
   s->nnilpre = s->nni;
   s->precondcurrent = 1;
@@ -173,15 +189,20 @@ static int pc_setup(pfhip_solver *s)
 static int psolve(pfhip_solver *s, const double *r, double *z)
 {
   s->nps++;
+  PH_BEGIN(PFHIP_PROF_PH_PC_SOLVE);
+  int rc = PFHIP_OK;
   if (r == z) {
-    CHK(s->ops->vec_scale(s->pb, 1.0, r, s->vtemp2, NULL));
-    return s->ops->mg_apply(s->mg, s->vtemp2, z, NULL);
+    rc = s->ops->vec_scale(s->pb, 1.0, r, s->vtemp2, NULL);
+    if (!rc) rc = s->ops->mg_apply(s->mg, s->vtemp2, z, NULL);
+  } else {
+    rc = s->ops->mg_apply(s->mg, r, z, NULL);
   }
-  return s->ops->mg_apply(s->mg, r, z, NULL);
+  PH_END(PFHIP_PROF_PH_PC_SOLVE);
+  return rc;
 }
 
 /* KINSpgmrAtimes: KINSolMatVec or difference quotient KS/kinspgmr.c:481-531 */
-static int atimes(pfhip_solver *s, const double *v, double *z)
+static int atimes_eval(pfhip_solver *s, const double *v, double *z)
 {
   pfhip_problem *pb = s->pb;
   if (s->o.use_jacobian) {
@@ -210,12 +231,20 @@ static int atimes(pfhip_solver *s, const double *v, double *z)
                              s->cur_s_old, s->dt, NULL);
   }
   CHK(s->ops->vec_linear_sum(pb, 1.0, s->uu, sigma, v, s->vtemp1, NULL));
-  CHK(func(s, s->vtemp1, s->vtemp2));
+  CHK(func_eval(s, s->vtemp1, s->vtemp2));
   return s->ops->vec_linear_sum(pb, sigma_inv, s->vtemp2, -sigma_inv, s->fval, z, NULL);
 }
 
+static int atimes(pfhip_solver *s, const double *v, double *z)
+{
+  PH_BEGIN(PFHIP_PROF_PH_JV);
+  int rc = atimes_eval(s, v, z);
+  PH_END(PFHIP_PROF_PH_JV);
+  return rc;
+}
+
 /* ModifiedGS (KS/iterativ.c:30-73), each step one fused axpy+dot pass */
-static int modified_gs(pfhip_solver *s, int k, double *new_vk_norm)
+static int modified_gs_eval(pfhip_solver *s, int k, double *new_vk_norm)
 {
   pfhip_problem *pb = s->pb;
   double **v = s->V, **h = s->Hes;
@@ -253,7 +282,7 @@ static int modified_gs(pfhip_solver *s, int k, double *new_vk_norm)
 }
 
 /* CGS2: two passes of block classical Gram-Schmidt against V[0..k-1] */
-static int cgs2(pfhip_solver *s, int k, double *new_vk_norm)
+static int cgs2_eval(pfhip_solver *s, int k, double *new_vk_norm)
 {
   pfhip_problem *pb = s->pb;
   double *h1 = s->hbuf, *h2 = s->hbuf + (s->l_max + 2);
@@ -270,6 +299,22 @@ static int cgs2(pfhip_solver *s, int k, double *new_vk_norm)
   for (int i = 0; i < k; i++) s->Hes[i][km1] = h1[i] + h2[i];
   *new_vk_norm = sqrt(n2 > 0.0 ? n2 : 0.0);
   return PFHIP_OK;
+}
+
+static int modified_gs(pfhip_solver *s, int k, double *new_vk_norm)
+{
+  PH_BEGIN(PFHIP_PROF_PH_ORTHO);
+  int rc = modified_gs_eval(s, k, new_vk_norm);
+  PH_END(PFHIP_PROF_PH_ORTHO);
+  return rc;
+}
+
+static int cgs2(pfhip_solver *s, int k, double *new_vk_norm)
+{
+  PH_BEGIN(PFHIP_PROF_PH_ORTHO);
+  int rc = cgs2_eval(s, k, new_vk_norm);
+  PH_END(PFHIP_PROF_PH_ORTHO);
+  return rc;
 }
 
 static int qrfact(int n, double **h, double *q)
@@ -490,8 +535,11 @@ static int eval_unew(pfhip_solver *s, double rl, double *fnormp, double *f1normp
   if (!s->pb->overland) {
     /* unew = uu + rl pp formed inside the residual launch */
     s->nfe++;
-    CHK(s->ops->residual_step(s->pb, s->uu, s->pp, rl, s->unew, s->fval, s->cur_p_old,
-                            s->cur_s_old, s->dt, NULL));
+    PH_BEGIN(PFHIP_PROF_PH_NL_F_EVAL);
+    int rc = s->ops->residual_step(s->pb, s->uu, s->pp, rl, s->unew, s->fval, s->cur_p_old,
+                                   s->cur_s_old, s->dt, NULL);
+    PH_END(PFHIP_PROF_PH_NL_F_EVAL);
+    CHK(rc);
   } else {
     CHK(s->ops->vec_linear_sum(s->pb, 1.0, s->uu, rl, s->pp, s->unew, NULL));
     CHK(func(s, s->unew, s->fval));
@@ -656,9 +704,21 @@ static double unit_roundoff(void)
   return u * 2.0;
 }
 
+static int kinsol_solve(pfhip_solver *s, double *p, const double *p_old,
+                        const double *s_old, double dt, double time, pfhip_kin_stats *st);
+
 int pfhip_kinsol_nonlin_solver(pfhip_solver *s, double *p, const double *p_old,
                                const double *s_old, double dt, double time,
                                pfhip_kin_stats *st)
+{
+  PH_BEGIN(PFHIP_PROF_PH_KINSOL);
+  int rc = kinsol_solve(s, p, p_old, s_old, dt, time, st);
+  PH_END(PFHIP_PROF_PH_KINSOL);
+  return rc;
+}
+
+static int kinsol_solve(pfhip_solver *s, double *p, const double *p_old,
+                        const double *s_old, double dt, double time, pfhip_kin_stats *st)
 {
   pfhip_problem *pb = s->pb;
   const pfhip_solver_opts *o = &s->o;

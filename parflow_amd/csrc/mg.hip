@@ -1556,11 +1556,12 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
     double *d = (pair && dout && q + 2 == n && !seq[q].init && (g.n[0] % 2) == 0) ? dout
                                                                                  : nullptr;
     const double *di = (pair && din && q == 0 && seq[q].init) ? din : nullptr;
-    // level 0's launches with level-precision operands are the timed ones
-    // (the FP64-result pair stores 4 more bytes per point, the pair that
-    // converts the FP64 right-hand side reads 4 and stores 4 more)
-    const bool mark = l == 0 && !d && !di;
-    if (mark) pfhip_prof_mark(PFHIP_PROF_RBGS, 0, st);
+    // level 0's launches, timed by form: level-precision operands, the pair
+    // writing the FP64 result (4 more bytes stored per point) and the pair
+    // converting the FP64 right-hand side (4 more read, 4 more stored)
+    const int mark = (l != 0) ? -1 : d ? PFHIP_PROF_RBGS_DOUT : di ? PFHIP_PROF_RBGS_DIN
+                                                                   : PFHIP_PROF_RBGS;
+    if (mark >= 0) pfhip_prof_mark(mark, 0, st);
     if (pair) {
       launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, mg->pv[l], st,
                      d, di);
@@ -1576,7 +1577,7 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
                            seq[q].par, seq[q].init);
       q += 1;
     }
-    if (mark) pfhip_prof_mark(PFHIP_PROF_RBGS, 1, st);
+    if (mark >= 0) pfhip_prof_mark(mark, 1, st);
     LAUNCHCHK();
   }
   return PFHIP_OK;

@@ -81,3 +81,4 @@ def test_problem_create_rejects_grids_beyond_the_offset_limit(lib):
     assert rc == -2 and not out.value
     msg = L.pfhip_last_error().decode()
     assert "exceeds the limit of 536870912" in msg, msg
+
