@@ -127,11 +127,11 @@ def load_traffic():
     return out
 
 
-def _workload_label(name, prob, opts, dt, krylov_label):
+def _workload_label(name, prob, opts, dt, krylov_label, vcycle="FP32"):
     kind = ("tilted-V catchment with kinematic overland flow" if prob.overland
             else "heterogeneous Richards")
     return (f"{name}: {prob.nx}x{prob.ny}x{prob.nz} {kind}, {krylov_label}({opts.krylov_dim}) "
-            f"+ FP32 V-cycle ({opts.num_pre_relax}+{opts.num_post_relax} RB sweeps), "
+            f"+ {vcycle} V-cycle ({opts.num_pre_relax}+{opts.num_post_relax} RB sweeps), "
             f"dt={dt} h")
 
 
@@ -568,7 +568,7 @@ def main():
         plain, pref = res["plain"], res["plain_ref"]
         line["plain_fp64"] = {
             "value": plain["value"],
-            "workload": _workload_label(args.config, prob, opts, dt, "GMRES"),
+            "workload": _workload_label(args.config, prob, opts, dt, "GMRES", "FP64"),
             "ms_per_step": 1e3 * plain["elapsed"] / K,
             "newton_per_step": plain["newton"],
             "gmres_per_step": plain["gmres"],
@@ -580,7 +580,7 @@ def main():
         line["plain_fp64_reference_pfmg_settings"] = {
             "value": pref["value"],
             "workload": _workload_label(args.config, prob, dataclasses_replace(
-                opts, num_pre_relax=1, num_post_relax=1), dt, "GMRES"),
+                opts, num_pre_relax=1, num_post_relax=1), dt, "GMRES", "FP64"),
             "ms_per_step": 1e3 * pref["elapsed"] / K,
             "newton_per_step": pref["newton"],
             "gmres_per_step": pref["gmres"],

@@ -92,6 +92,7 @@ struct pfhip_ensemble {
   pfhip_ens_req **lists[PFHIP_EOP_COUNT];
   int nlist[PFHIP_EOP_COUNT];
   pfhip_ens_req **vec, **outs;
+  pfhip_ens_mgplan *mgplan;
   long rounds, launches;
 };
 
@@ -115,6 +116,7 @@ static pfhip_ens_req *new_req(int op)
   r->z = r->z2 = NULL;
   r->xq[0] = r->xq[1] = NULL;
   r->op = op;
+  r->member = g_ens->cur;
   r->pb = m->pb;
   r->mg = m->mg;
   for (int q = 0; q < 4; q++) r->scratch[q] = m->scratch[q];
@@ -331,6 +333,7 @@ int pfhip_ensemble_destroy(pfhip_ensemble *e)
   pfhip_free(e->d_part);
   pfhip_host_free(e->h_res);
   pfhip_ens_stage_destroy(&e->st);
+  pfhip_ens_mg_plan_destroy(e->mgplan);
   for (int k = 0; k < PFHIP_EOP_COUNT; k++) free(e->lists[k]);
   free(e->vec);
   free(e->outs);
@@ -391,6 +394,13 @@ int pfhip_ensemble_create(int n, pfhip_solver *const *solvers, pfhip_ensemble **
   if (!rc) rc = pfhip_dev_alloc(&e->d_res, (long)n * ENS_QMAX * PFHIP_ENS_NRES);
   if (!rc) rc = pfhip_host_alloc(&e->h_res, (long)n * ENS_QMAX * PFHIP_ENS_NRES);
   if (!rc) rc = pfhip_ens_stage_create(&e->st, ENS_STAGE);
+  if (!rc) {
+    pfhip_mg **mgs = (pfhip_mg **)malloc(sizeof(pfhip_mg *) * (size_t)n);
+    if (!mgs) rc = PFHIP_ERR_NOMEM;
+    for (int i = 0; i < n && !rc; i++) mgs[i] = e->m[i].mg;
+    if (!rc) rc = pfhip_ens_mg_plan_create(mgs, n, &e->mgplan);
+    free(mgs);
+  }
   if (!rc) rc = pfhip_synchronize();
   if (rc) { pfhip_ensemble_destroy(e); return rc; }
   *out = e;
@@ -497,15 +507,18 @@ static int serve_round(pfhip_ensemble *e)
     int ns = e->nlist[PFHIP_EOP_MG_SETUP];
     if (ns) {
       rc = pfhip_ens_exec_jacobian(e->lists[PFHIP_EOP_MG_SETUP], ns, &e->st, NULL);
-      if (!rc) rc = pfhip_ens_exec_mg_coarse(e->lists[PFHIP_EOP_MG_SETUP], ns, &e->st, NULL);
+      long l0 = pfhip_ens_mg_plan_launches(e->mgplan);
+      if (!rc) rc = pfhip_ens_exec_mg_coarse(e->mgplan, e->lists[PFHIP_EOP_MG_SETUP], ns, &e->st,
+                                             NULL);
       set_rc(e->lists[PFHIP_EOP_MG_SETUP], ns, rc);
-      e->launches += 3;
+      e->launches += 2 + (pfhip_ens_mg_plan_launches(e->mgplan) - l0);
     }
     int na = e->nlist[PFHIP_EOP_MG_APPLY];
     if (na) {
-      rc = pfhip_ens_exec_mg_apply(e->lists[PFHIP_EOP_MG_APPLY], na, &e->st, NULL);
+      long l0 = pfhip_ens_mg_plan_launches(e->mgplan);
+      rc = pfhip_ens_exec_mg_apply(e->mgplan, e->lists[PFHIP_EOP_MG_APPLY], na, &e->st, NULL);
       set_rc(e->lists[PFHIP_EOP_MG_APPLY], na, rc);
-      e->launches++;
+      e->launches += pfhip_ens_mg_plan_launches(e->mgplan) - l0;
     }
     int nb = e->nlist[PFHIP_EOP_BC];
     if (nb) {

@@ -1685,36 +1685,128 @@ extern "C" int pfhip_mg_apply(pfhip_mg *mg, const double *b, double *x, void *st
                           : mg_apply_t<float>(mg, b, x, STREAM(stream));
 }
 
-// ---- ensemble: one workgroup per member ---------------------------------------
-// A member's whole hierarchy (every level, not only the small ones) runs in
-// its workgroup: ensemble members are small, and one launch then covers the
-// coarse-operator setup or the V-cycle of every member with a pending request.
-template <typename T> struct EnsMG {
+// ---- ensemble multigrid ---------------------------------------------------------
+// The levels of a member with at most ens_small() points (the bottom of its
+// hierarchy; the whole hierarchy for small members) run inside one
+// workgroup per member, one launch for every member with a request.  The
+// larger levels of the larger members run as grid-level phases: one launch
+// per phase and level covers that level's points of every member that has
+// it (a half-sweep, the level residual, the restriction, the prolongation,
+// the coarse operators), so no single member's one-workgroup V-cycle sets
+// the cost of every heavy round.  A launch's blocks are dealt to members in
+// segments (EnsLSeg: the member and its first block), found by a binary
+// search over the segment starts.  The per-point arithmetic is the single-
+// problem multigrid's (rbgs_point, resid_point, restrict_point,
+// prolong_point, coarse_point), in the same sweep order.
+struct EnsLSeg { int m, blk0; };
+
+template <typename T> struct EnsLv {
   const LevelDev<T> *lev;
-  int L, pre, post, symmetric;
-  const double *bin;
-  double *xout;
+  const double *bin;    // this round's V-cycle right-hand side (FP64)
+  double *xout;         // and result
+  int L, lc, pre, post, symmetric, act;
 };
 
-template <typename T>
-__global__ __launch_bounds__(CWG) void k_ens_mg_coarse(const EnsMG<T> *__restrict__ E)
+// point sets a grid-level launch enumerates
+enum { PS_FINE = 0, PS_HALF = 1, PS_COARSE = 2, PS_N = 3 };
+enum { EL_CONV_IN, EL_FILL, EL_HALF, EL_RESID, EL_RESTRICT, EL_PROLONG, EL_CONV_OUT, EL_COARSE };
+#define ENS_LBLK 256
+
+struct pfhip_ens_mgplan {
+  int n, prec, G;                     // members, level precision, grid levels
+  int *lc;                            // per member: first one-workgroup level
+  int pre, post, symmetric;           // the members' common sweep schedule
+  int nseg[MAXL][PS_N], nblk[MAXL][PS_N];
+  EnsLSeg *dseg[MAXL][PS_N];          // device segment tables
+  void *dM;                           // device member table EnsLv<T>[n], refilled per round
+  int *dlist;                         // 0 .. n-1 (the one-workgroup kernels' members)
+  int use_graph;                      // replay captured launch sequences
+  hipStream_t cs;                     // capture stream
+  hipGraphExec_t gx[2];               // [0] coarse operators, [1] V-cycle
+  long glaunch[2];                    // kernels in each captured sequence
+  long launches;                      // kernel launches made so far
+};
+
+static long ens_small()
 {
-  const EnsMG<T> &M = E[blockIdx.x];
-  for (int l = 0; l <= M.L - 2; l++) {
-    const LevelDev<T> &F = M.lev[l], &Cc = M.lev[l + 1];
-    FOR_POINTS(Cc.g, (coarse_point<T, false>(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1,
-                                             F.p2, i, j, k)));
-  }
+  const char *e = getenv("PFHIP_ENS_MG_SMALL");
+  return (e && *e) ? atol(e) : MG_SMALL_DEFAULT;
 }
 
 template <typename T>
-__global__ __launch_bounds__(CWG) void k_ens_vcycle(const EnsMG<T> *__restrict__ E)
+__global__ __launch_bounds__(ENS_LBLK) void k_ens_lvl(const EnsLSeg *__restrict__ seg, int nseg,
+                                                      const EnsLv<T> *__restrict__ M, int l,
+                                                      int OP, int par, int init)
 {
-  const EnsMG<T> &M = E[blockIdx.x];
-  const LevelDev<T> &L0 = M.lev[0];
-  FOR_POINTS(L0.g, L0.b[LIDX(L0.g, i, j, k)] = (T)M.bin[LIDX(L0.g, i, j, k)]);
-  vcycle_block<T, false>(M.lev, 0, M.L, M.pre, M.post, M.symmetric);
-  FOR_POINTS(L0.g, M.xout[LIDX(L0.g, i, j, k)] = (double)L0.x[LIDX(L0.g, i, j, k)]);
+  int lo = 0, hi = nseg - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seg[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+  }
+  const EnsLv<T> &mm = M[seg[lo].m];
+  if (!mm.act) return;
+  const LevelDev<T> &F = mm.lev[l];
+  const int idx = (b - seg[lo].blk0) * ENS_LBLK + threadIdx.x;
+  int i, j, k;
+  if (OP == EL_HALF) {
+    const int nx = F.g.n[0], ny = F.g.n[1], nz = F.g.n[2];
+    const int ncol = (nx + 1) >> 1;
+    if (idx >= ncol * ny * nz) return;
+    const int t = idx % ncol, r = idx / ncol;
+    j = r % ny; k = r / ny;
+    i = ((par - j - k) & 1) + 2 * t;
+    if (i >= nx) return;
+    rbgs_point<T, false>(F.g, F.A, F.x, F.b, par, init, i, j, k);
+    return;
+  }
+  if (OP == EL_RESTRICT || OP == EL_COARSE) {
+    const LevelDev<T> &Cc = mm.lev[l + 1];
+    if (idx >= Cc.g.n[0] * Cc.g.n[1] * Cc.g.n[2]) return;
+    unflat(Cc.g, idx, i, j, k);
+    if (OP == EL_RESTRICT)
+      restrict_point(F.g, Cc.g, F.dir, F.t, Cc.b, F.p1, F.p2, i, j, k);
+    else
+      coarse_point<T, false>(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1, F.p2, i, j, k);
+    return;
+  }
+  if (idx >= F.g.n[0] * F.g.n[1] * F.g.n[2]) return;
+  unflat(F.g, idx, i, j, k);
+  const long ip = LIDX(F.g, i, j, k);
+  if (OP == EL_CONV_IN) F.b[ip] = (T)mm.bin[ip];
+  else if (OP == EL_FILL) F.x[ip] = T(0);
+  else if (OP == EL_RESID) resid_point<T, false>(F.g, F.A, F.x, F.b, F.t, i, j, k);
+  else if (OP == EL_PROLONG) {
+    const LevelDev<T> &Cc = mm.lev[l + 1];
+    prolong_point(F.g, Cc.g, F.dir, Cc.x, F.x, F.p1, F.p2, i, j, k);
+  } else if (OP == EL_CONV_OUT) mm.xout[ip] = (double)F.x[ip];
+}
+
+// the one-workgroup part of every listed member: levels lc .. L-1, with the
+// conversions of the right-hand side and the result when lc = 0
+template <typename T>
+__global__ __launch_bounds__(CWG) void k_ens_vcycle(const EnsLv<T> *__restrict__ M,
+                                                    const int *__restrict__ list)
+{
+  const EnsLv<T> &mm = M[list[blockIdx.x]];
+  if (!mm.act) return;                          // uniform over the workgroup
+  const LevelDev<T> &L0 = mm.lev[0];
+  if (mm.lc == 0) FOR_POINTS(L0.g, L0.b[LIDX(L0.g, i, j, k)] = (T)mm.bin[LIDX(L0.g, i, j, k)]);
+  vcycle_block<T, false>(mm.lev, mm.lc, mm.L, mm.pre, mm.post, mm.symmetric);
+  if (mm.lc == 0) FOR_POINTS(L0.g, mm.xout[LIDX(L0.g, i, j, k)] = (double)L0.x[LIDX(L0.g, i, j, k)]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(CWG) void k_ens_mg_coarse(const EnsLv<T> *__restrict__ M,
+                                                       const int *__restrict__ list)
+{
+  const EnsLv<T> &mm = M[list[blockIdx.x]];
+  if (!mm.act) return;                          // uniform over the workgroup
+  for (int l = mm.lc; l <= mm.L - 2; l++) {
+    const LevelDev<T> &F = mm.lev[l], &Cc = mm.lev[l + 1];
+    FOR_POINTS(Cc.g, (coarse_point<T, false>(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1,
+                                             F.p2, i, j, k)));
+  }
 }
 
 extern "C" void pfhip_mg_level0(pfhip_mg *mg, void **A, int *prec)
@@ -1723,58 +1815,268 @@ extern "C" void pfhip_mg_level0(pfhip_mg *mg, void **A, int *prec)
   *prec = mg->prec;
 }
 
-template <typename T>
-static int ens_mg_launch(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st, int apply,
-                         hipStream_t s)
+static long ps_points(const pfhip_mg *mg, int l, int ps)
 {
-  EnsMG<T> *T_ = (EnsMG<T> *)calloc(n, sizeof(EnsMG<T>));
-  if (!T_) { pfhip_set_error("out of host memory"); return PFHIP_ERR_NOMEM; }
-  for (int r = 0; r < n; r++) {
-    pfhip_mg *mg = rq[r]->mg;
-    T_[r].lev = (const LevelDev<T> *)mg->levdev;
-    T_[r].L = mg->L;
-    T_[r].pre = mg->pre; T_[r].post = mg->post; T_[r].symmetric = mg->symmetric;
-    T_[r].bin = rq[r]->x;
-    T_[r].xout = rq[r]->z;
+  if (ps == PS_COARSE) return npts(mg->n[l + 1]);
+  if (ps == PS_HALF) return (long)((mg->n[l][0] + 1) / 2) * mg->n[l][1] * mg->n[l][2];
+  return npts(mg->n[l]);
+}
+
+extern "C" long pfhip_ens_mg_plan_launches(const pfhip_ens_mgplan *p)
+{
+  return p ? p->launches : 0;
+}
+
+extern "C" int pfhip_ens_mg_plan_destroy(pfhip_ens_mgplan *p)
+{
+  if (!p) return PFHIP_OK;
+  for (int l = 0; l < MAXL; l++)
+    for (int s = 0; s < PS_N; s++) if (p->dseg[l][s]) pfhip_free(p->dseg[l][s]);
+  for (int q = 0; q < 2; q++) if (p->gx[q]) hipGraphExecDestroy(p->gx[q]);
+  if (p->cs) hipStreamDestroy(p->cs);
+  if (p->dM) pfhip_free(p->dM);
+  if (p->dlist) pfhip_free(p->dlist);
+  free(p->lc);
+  free(p);
+  return PFHIP_OK;
+}
+
+extern "C" int pfhip_ens_mg_plan_create(pfhip_mg *const *mgs, int n, pfhip_ens_mgplan **out)
+{
+  pfhip_ens_mgplan *p = (pfhip_ens_mgplan *)calloc(1, sizeof(pfhip_ens_mgplan));
+  int *order = (int *)malloc(sizeof(int) * (size_t)n);
+  EnsLSeg *h = (EnsLSeg *)malloc(sizeof(EnsLSeg) * (size_t)n);
+  if (p) p->lc = (int *)malloc(sizeof(int) * (size_t)n);
+  if (!p || !order || !h || !p->lc) {
+    free(order); free(h); pfhip_ens_mg_plan_destroy(p);
+    pfhip_set_error("out of host memory");
+    return PFHIP_ERR_NOMEM;
   }
-  void *dT = pfhip_ens_upload(st, T_, (long)n * sizeof(EnsMG<T>), s);
-  free(T_);
-  if (!dT) return PFHIP_ERR_HIP;
-  if (apply)
-    hipLaunchKernelGGL(k_ens_vcycle<T>, dim3(n), dim3(CWG), 0, s, (const EnsMG<T> *)dT);
-  else
-    hipLaunchKernelGGL(k_ens_mg_coarse<T>, dim3(n), dim3(CWG), 0, s, (const EnsMG<T> *)dT);
+  p->n = n;
+  p->prec = mgs[0]->prec;
+  p->pre = mgs[0]->pre; p->post = mgs[0]->post; p->symmetric = mgs[0]->symmetric;
+  const char *ge = getenv("PFHIP_ENS_GRAPH");
+  p->use_graph = (ge && *ge) ? atoi(ge) : 1;
+  int rc = PFHIP_OK;
+  const long small = ens_small();
+  for (int m = 0; m < n; m++) {
+    const pfhip_mg *mg = mgs[m];
+    if (mg->prec != p->prec || mg->full || mg->pre != p->pre || mg->post != p->post ||
+        mg->symmetric != p->symmetric) {
+      pfhip_set_error("ensemble member %d: members need one multigrid precision and sweep "
+                      "schedule and the PFSymmetric matrix", m);
+      rc = PFHIP_ERR_ARG;
+    }
+    int lc = mg->L - 1;
+    for (int l = 0; l < mg->L; l++)
+      if (npts(mg->n[l]) <= small) { lc = l; break; }
+    p->lc[m] = lc;
+    if (lc > p->G) p->G = lc;
+    order[m] = m;
+  }
+  // members in decreasing size, so the largest segments start first
+  for (int a = 1; a < n; a++) {
+    int v = order[a], q = a;
+    while (q > 0 && npts(mgs[order[q - 1]]->n[0]) < npts(mgs[v]->n[0])) {
+      order[q] = order[q - 1];
+      q--;
+    }
+    order[q] = v;
+  }
+  for (int l = 0; l < p->G && rc == PFHIP_OK; l++) {
+    for (int s = 0; s < PS_N && rc == PFHIP_OK; s++) {
+      int ns = 0, nb = 0;
+      for (int q = 0; q < n; q++) {
+        const int m = order[q];
+        if (p->lc[m] <= l) continue;
+        h[ns].m = m;
+        h[ns].blk0 = nb;
+        nb += (int)((ps_points(mgs[m], l, s) + ENS_LBLK - 1) / ENS_LBLK);
+        ns++;
+      }
+      p->nseg[l][s] = ns;
+      p->nblk[l][s] = nb;
+      if (ns == 0) continue;
+      rc = pfhip_malloc((void **)&p->dseg[l][s], (long)sizeof(EnsLSeg) * ns);
+      if (rc == PFHIP_OK &&
+          hipMemcpy(p->dseg[l][s], h, sizeof(EnsLSeg) * ns, hipMemcpyHostToDevice) != hipSuccess) {
+        pfhip_set_error("hipMemcpy of an ensemble segment table");
+        rc = PFHIP_ERR_HIP;
+      }
+    }
+  }
+  const size_t mb = (size_t)n * (p->prec == 32 ? sizeof(EnsLv<float>) : sizeof(EnsLv<double>));
+  if (rc == PFHIP_OK) rc = pfhip_malloc(&p->dM, (long)mb);
+  if (rc == PFHIP_OK) rc = pfhip_malloc((void **)&p->dlist, (long)sizeof(int) * n);
+  if (rc == PFHIP_OK) {
+    for (int m = 0; m < n; m++) order[m] = m;
+    if (hipMemcpy(p->dlist, order, sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
+      pfhip_set_error("hipMemcpy of the ensemble member list");
+      rc = PFHIP_ERR_HIP;
+    }
+  }
+  if (rc == PFHIP_OK && p->use_graph &&
+      hipStreamCreateWithFlags(&p->cs, hipStreamNonBlocking) != hipSuccess) {
+    pfhip_set_error("hipStreamCreate for the ensemble graph capture");
+    rc = PFHIP_ERR_HIP;
+  }
+  free(order);
+  free(h);
+  if (rc) { pfhip_ens_mg_plan_destroy(p); return rc; }
+  *out = p;
+  return PFHIP_OK;
+}
+
+template <typename T>
+static void ens_phase(pfhip_ens_mgplan *p, int l, int ps, int op, int par, int init,
+                      hipStream_t s)
+{
+  if (p->nblk[l][ps] == 0) return;
+  p->launches++;
+  hipLaunchKernelGGL(k_ens_lvl<T>, dim3(p->nblk[l][ps]), dim3(ENS_LBLK), 0, s, p->dseg[l][ps],
+                     p->nseg[l][ps], (const EnsLv<T> *)p->dM, l, op, par, init);
+}
+
+template <typename T>
+static void ens_sweeps(pfhip_ens_mgplan *p, int l, int zero, int max_iter, hipStream_t s)
+{
+  HalfSweep seq[2 * MAX_SWEEPS + 4];
+  int fill;
+  const int ns = sweep_list(p->symmetric, zero, max_iter, seq, &fill);
+  if (fill) ens_phase<T>(p, l, PS_FINE, EL_FILL, 0, 0, s);
+  for (int q = 0; q < ns; q++) ens_phase<T>(p, l, PS_HALF, EL_HALF, seq[q].par, seq[q].init, s);
+}
+
+// The launch sequence of one round: the coarse operators (apply = 0) or one
+// V-cycle (apply = 1) of every member whose entry in the member table is
+// active.  It reads only the plan's fixed tables, so it is the same in
+// every round and can be captured once.
+template <typename T>
+static int ens_sequence(pfhip_ens_mgplan *p, int apply, hipStream_t s)
+{
+  const EnsLv<T> *dM = (const EnsLv<T> *)p->dM;
+  if (!apply) {
+    for (int l = 0; l < p->G; l++) {
+      ens_phase<T>(p, l, PS_COARSE, EL_COARSE, 0, 0, s);
+      LAUNCHCHK();
+    }
+    hipLaunchKernelGGL(k_ens_mg_coarse<T>, dim3(p->n), dim3(CWG), 0, s, dM, p->dlist);
+    LAUNCHCHK();
+    p->launches++;
+    return PFHIP_OK;
+  }
+  if (p->G > 0) ens_phase<T>(p, 0, PS_FINE, EL_CONV_IN, 0, 0, s);
+  for (int l = 0; l < p->G; l++) {
+    ens_sweeps<T>(p, l, 1, p->pre, s);
+    ens_phase<T>(p, l, PS_FINE, EL_RESID, 0, 0, s);
+    ens_phase<T>(p, l, PS_COARSE, EL_RESTRICT, 0, 0, s);
+    LAUNCHCHK();
+  }
+  hipLaunchKernelGGL(k_ens_vcycle<T>, dim3(p->n), dim3(CWG), 0, s, dM, p->dlist);
+  LAUNCHCHK();
+  p->launches++;
+  for (int l = p->G - 1; l >= 0; l--) {
+    ens_phase<T>(p, l, PS_FINE, EL_PROLONG, 0, 0, s);
+    ens_sweeps<T>(p, l, 0, p->post, s);
+    LAUNCHCHK();
+  }
+  if (p->G > 0) ens_phase<T>(p, 0, PS_FINE, EL_CONV_OUT, 0, 0, s);
   LAUNCHCHK();
   return PFHIP_OK;
 }
 
-static int ens_mg_exec(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st, int apply,
-                       void *stream)
+// capture the round's launch sequence on the plan's stream (nothing runs
+// during the capture) and instantiate it; kernel nodes only
+template <typename T>
+static int ens_capture(pfhip_ens_mgplan *p, int apply)
 {
-  if (n <= 0) return PFHIP_OK;
-  int prec = rq[0]->mg->prec;
+  hipGraph_t g = nullptr;
+  const long l0 = p->launches;
+  HIPCHK(hipStreamBeginCapture(p->cs, hipStreamCaptureModeRelaxed));
+  int rc = ens_sequence<T>(p, apply, p->cs);
+  hipError_t e = hipStreamEndCapture(p->cs, &g);
+  p->glaunch[apply] = p->launches - l0;
+  p->launches = l0;
+  if (rc) { if (g) hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess || !g) {
+    pfhip_set_error("ensemble graph capture: %s", hipGetErrorString(e));
+    return PFHIP_ERR_HIP;
+  }
+  e = hipGraphInstantiate(&p->gx[apply], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    p->gx[apply] = nullptr;
+    pfhip_set_error("ensemble graph instantiation: %s", hipGetErrorString(e));
+    return PFHIP_ERR_HIP;
+  }
+  return PFHIP_OK;
+}
+
+template <typename T>
+static int ens_mg_launch(pfhip_ens_mgplan *p, pfhip_ens_req *const *rq, int n, int apply,
+                         pfhip_ens_stage *st, hipStream_t s)
+{
+  // the member table of this round: every member inactive but those with a
+  // request; staged through the ring, then copied to the plan's fixed table
+  const long mbytes = (long)sizeof(EnsLv<T>) * p->n;
+  EnsLv<T> *M = (EnsLv<T> *)pfhip_ens_reserve(st, mbytes, s);
+  if (!M) return PFHIP_ERR_HIP;
+  memset(M, 0, (size_t)mbytes);
   for (int r = 0; r < n; r++) {
-    if (rq[r]->mg->prec != prec) {
-      pfhip_set_error("ensemble members need one multigrid precision");
+    pfhip_mg *mg = rq[r]->mg;
+    const int m = rq[r]->member;
+    if (m < 0 || m >= p->n) {
+      pfhip_set_error("ensemble multigrid: request of member %d outside the plan", m);
       return PFHIP_ERR_ARG;
     }
-    if (rq[r]->mg->full) {
-      pfhip_set_error("ensemble members need the PFSymmetric preconditioner matrix");
+    EnsLv<T> &e = M[m];
+    e.lev = (const LevelDev<T> *)mg->levdev;
+    e.bin = rq[r]->x;
+    e.xout = rq[r]->z;
+    e.L = mg->L; e.lc = p->lc[m];
+    e.pre = mg->pre; e.post = mg->post; e.symmetric = mg->symmetric;
+    e.act = 1;
+  }
+  void *dstage = pfhip_ens_commit(st, mbytes, s);
+  if (!dstage) return PFHIP_ERR_HIP;
+  HIPCHK(hipMemcpyAsync(p->dM, dstage, (size_t)mbytes, hipMemcpyDeviceToDevice, s));
+  if (p->use_graph) {
+    if (!p->gx[apply]) {
+      int rc = ens_capture<T>(p, apply);
+      if (rc) return rc;
+    }
+    HIPCHK(hipGraphLaunch(p->gx[apply], s));
+    p->launches += p->glaunch[apply];
+    return PFHIP_OK;
+  }
+  return ens_sequence<T>(p, apply, s);
+}
+
+static int ens_mg_exec(pfhip_ens_mgplan *p, pfhip_ens_req *const *rq, int n, int apply,
+                       pfhip_ens_stage *st, void *stream)
+{
+  if (n <= 0) return PFHIP_OK;
+  for (int r = 0; r < n; r++) {
+    const pfhip_mg *mg = rq[r]->mg;
+    if (mg->prec != p->prec || mg->full || mg->pre != p->pre || mg->post != p->post ||
+        mg->symmetric != p->symmetric) {
+      pfhip_set_error("ensemble members need one multigrid precision and sweep schedule and "
+                      "the PFSymmetric preconditioner matrix");
       return PFHIP_ERR_ARG;
     }
   }
-  return (prec == 32) ? ens_mg_launch<float>(rq, n, st, apply, STREAM(stream))
-                      : ens_mg_launch<double>(rq, n, st, apply, STREAM(stream));
+  return (p->prec == 32) ? ens_mg_launch<float>(p, rq, n, apply, st, STREAM(stream))
+                         : ens_mg_launch<double>(p, rq, n, apply, st, STREAM(stream));
 }
 
-extern "C" int pfhip_ens_exec_mg_coarse(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
-                                        void *stream)
+extern "C" int pfhip_ens_exec_mg_coarse(pfhip_ens_mgplan *p, pfhip_ens_req *const *rq, int n,
+                                        pfhip_ens_stage *st, void *stream)
 {
-  return ens_mg_exec(rq, n, st, 0, stream);
+  return ens_mg_exec(p, rq, n, 0, st, stream);
 }
 
-extern "C" int pfhip_ens_exec_mg_apply(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
-                                       void *stream)
+extern "C" int pfhip_ens_exec_mg_apply(pfhip_ens_mgplan *p, pfhip_ens_req *const *rq, int n,
+                                       pfhip_ens_stage *st, void *stream)
 {
-  return ens_mg_exec(rq, n, st, 1, stream);
+  return ens_mg_exec(p, rq, n, 1, st, stream);
 }

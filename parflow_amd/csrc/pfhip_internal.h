@@ -193,6 +193,7 @@ typedef struct pfhip_ens_req {
   double *out[PFHIP_ENS_NRES];  /* host destinations of the results      */
   int nout;
   int slot;                     /* result slot within its round          */
+  int member;                   /* index of the requesting member        */
   int rc;
 } pfhip_ens_req;
 
@@ -219,10 +220,18 @@ int pfhip_ens_exec_residual(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st
                             void *stream);
 int pfhip_ens_exec_jacobian(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
                             void *stream);
-int pfhip_ens_exec_mg_coarse(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
-                             void *stream);
-int pfhip_ens_exec_mg_apply(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
-                            void *stream);
+/* the ensemble's multigrid launch plan: which levels of each member run as
+ * grid-level phases (one launch per phase covering every member with that
+ * level) and which inside one workgroup per member; built once from every
+ * member's hierarchy */
+typedef struct pfhip_ens_mgplan pfhip_ens_mgplan;
+int pfhip_ens_mg_plan_create(pfhip_mg *const *mgs, int n, pfhip_ens_mgplan **out);
+int pfhip_ens_mg_plan_destroy(pfhip_ens_mgplan *p);
+long pfhip_ens_mg_plan_launches(const pfhip_ens_mgplan *p);
+int pfhip_ens_exec_mg_coarse(pfhip_ens_mgplan *p, pfhip_ens_req *const *rq, int n,
+                             pfhip_ens_stage *st, void *stream);
+int pfhip_ens_exec_mg_apply(pfhip_ens_mgplan *p, pfhip_ens_req *const *rq, int n,
+                            pfhip_ens_stage *st, void *stream);
 int pfhip_ens_exec_bc(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st, void *stream);
 int pfhip_ens_exec_saturation(pfhip_ens_req *const *rq, int n, pfhip_ens_stage *st,
                               void *stream);
