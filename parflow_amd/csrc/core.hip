@@ -1641,7 +1641,6 @@ struct ResArgs {
   const double *bcv[6];
   int bct[6];
   double *__restrict__ F;
-  int xcd;                           // XCD-aware tile order (xcd_tile3)
 };
 
 __device__ __forceinline__ double hmean(double a, double b)
@@ -1901,37 +1900,6 @@ __device__ __forceinline__ void load_hal(HalIn &h, const ResArgs &A, int hip, do
   h.p = p_at<MODE>(A, hip, psig); h.al = ldb(A.f.alpha, hip); h.n = ldb(A.f.nvg, hip);
 }
 
-// XCD-aware numbering of a gx x gy x gz tile grid launched as a 1-D grid
-// rounded up to a multiple of 8: the hardware deals workgroup b to XCD
-// b % 8, so logical tile (b % 8) * (nb8 / 8) + b / 8 gives each XCD a
-// contiguous range of tiles (x fastest, then y, then z) and the halo rows
-// and columns neighbouring tiles share are read through the same L2.  With
-// xcd = 0 the logical tile is b itself.  False for the padding blocks
-// (uniform over the workgroup).
-__device__ __forceinline__ bool xcd_tile3(int gx, int gy, int gz, int xcd, int &bx, int &by,
-                                          int &bz)
-{
-  const int nb = gx * gy * gz, nb8 = (nb + 7) & ~7;
-  const int b = blockIdx.x;
-  const int L = xcd ? (b & 7) * (nb8 >> 3) + (b >> 3) : b;
-  if (L >= nb) return false;
-  bx = L % gx;
-  const int r = L / gx;
-  by = r % gy;
-  bz = r / gy;
-  return true;
-}
-
-static int tile_order_xcd()
-{
-  static int v = -1;
-  if (v < 0) {
-    const char *e = getenv("PFHIP_TILE_XCD");
-    v = (e && *e) ? atoi(e) : 1;
-  }
-  return v;
-}
-
 template <int TX, int TY, int ZC, bool PCS, int MODE, bool ISO, bool TAB>
 __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
 {
@@ -1952,10 +1920,7 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
   const double idx = A.idx, idy = A.idy, idz = A.idz;
   const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-  int bx, by, bz;
-  if (!xcd_tile3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC, A.xcd, bx,
-                 by, bz))
-    return;
+  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
   const int i = bx * TX + tx, j = by * TY + ty;
   const bool act = (i < g.nx) && (j < g.ny);
   const int nz = g.nz, sy = g.sy, sz = g.sz;
@@ -2137,14 +2102,10 @@ __global__ __launch_bounds__(256) void k_residual_bc(ResArgs A, long nshell)
 }
 
 template <int TX, int TY, int ZC, int MODE>
-static void launch_pf(const ResArgs &A0, hipStream_t st)
+static void launch_pf(const ResArgs &A, hipStream_t st)
 {
-  ResArgs A = A0;
-  A.xcd = tile_order_xcd();
   const Geom &g = A.g;
-  const unsigned nb = (unsigned)((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) *
-                      ((g.nz + ZC - 1) / ZC);
-  dim3 grid((nb + 7u) & ~7u);
+  dim3 grid((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC);
   dim3 block(TX, TY, 1);
   const int pid = (MODE == 1) ? PFHIP_PROF_RESIDUAL_JV
                   : (MODE == 2) ? PFHIP_PROF_RESIDUAL_STEP : PFHIP_PROF_RESIDUAL;
@@ -2683,7 +2644,6 @@ template <typename OT> struct JOut { OT *c[7]; };
 struct JacPFArgs {
   JacArgs a;
   double idx, idy, idz, imu;
-  int xcd;                           // XCD-aware tile order (xcd_tile3)
 };
 
 template <bool TAB = true>
@@ -2788,10 +2748,7 @@ void k_jacobian_pf(JacPFArgs P, JOut<OT> J)
   const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
   const double ddp = 0.0;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-  int bx, by, bz;
-  if (!xcd_tile3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC, P.xcd, bx,
-                 by, bz))
-    return;
+  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
   const int i = bx * TX + tx, j = by * TY + ty;
   const bool act = (i < g.nx) && (j < g.ny);
   const int nz = g.nz, sy = g.sy, sz = g.sz;
@@ -3064,15 +3021,11 @@ __global__ __launch_bounds__(256) void k_jacobian_bc(JacArgs A, JOut<OT> J, long
 }
 
 template <bool SYMM, typename OT>
-static void launch_jacobian_pf(const JacPFArgs &P0, const JOut<OT> &J, hipStream_t st)
+static void launch_jacobian_pf(const JacPFArgs &P, const JOut<OT> &J, hipStream_t st)
 {
-  JacPFArgs P = P0;
-  P.xcd = tile_order_xcd();
   const Geom &g = P.a.g;
   constexpr int TX = 64, TY = 8, ZC = 16;
-  const unsigned nb = (unsigned)((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) *
-                      ((g.nz + ZC - 1) / ZC);
-  dim3 grid((nb + 7u) & ~7u);
+  dim3 grid((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + ZC - 1) / ZC);
   dim3 block(TX, TY, 1);
   PROF_BEGIN(PFHIP_PROF_JACOBIAN, st);
   const bool tab = P.a.f.tb.t != nullptr;

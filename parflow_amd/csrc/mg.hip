@@ -606,9 +606,39 @@ __device__ __forceinline__ typename V2T<T>::type load_x2(const T *__restrict__ x
   return *(const T2 *)(xin + ip);
 }
 
+// the YL form's plane loads: centre, b, east and upper couplings always;
+// the south coupling and x_in of the row below only for the tile's lowest
+// row (own_s), x_in of the row above only for its highest (own_n); the
+// north coupling is loaded two planes ahead by the caller
+template <typename T, bool INIT, bool DIN = false>
+__device__ __forceinline__ void load_pair_yl(PairSet<T> &S, const Mat<T> &A,
+                                             const T *__restrict__ xin,
+                                             const T *__restrict__ b, unsigned ip, unsigned sy,
+                                             bool own_s, bool own_n,
+                                             const double *__restrict__ din = nullptr)
+{
+  typedef typename V2T<T>::type T2;
+  S.c = *(const T2 *)(A.c + ip);
+  if (DIN) S.bb = T2{(T)din[ip], (T)din[ip + 1]};
+  else S.bb = *(const T2 *)(b + ip);
+  S.ae = *(const T2 *)(A.u[0] + ip);
+  S.au = *(const T2 *)(A.u[2] + ip);
+  S.xs = T2{T(0), T(0)};
+  S.xn = T2{T(0), T(0)};
+  S.as = T2{T(0), T(0)};
+  if (own_s) {
+    S.as = *(const T2 *)(A.u[1] + ip - sy);
+    if (!INIT) S.xs = *(const T2 *)(xin + ip - sy);
+  }
+  if (own_n && !INIT) S.xn = *(const T2 *)(xin + ip + sy);
+}
+
 // FP32: ask for 8 waves per SIMD (<= 64 VGPRs), so two 1024-thread tiles fit
 // on a CU; FP64 keeps its register budget
 #define PAIR2_MIN_WAVES(T) (sizeof(T) == 4 ? 8 : 1)
+// the YL form carries the north couplings two planes ahead and its outer
+// rows' neighbour loads: up to 80 VGPRs (6 waves per SIMD) in FP32
+#define PAIR2_YL_MIN_WAVES(T) (sizeof(T) == 4 ? 6 : 1)
 
 // DOUT: the pair writes its result as FP64 into dout (a vector of the
 // level's ghosted layout, no alignment assumed) instead of xout: the last
@@ -617,8 +647,18 @@ __device__ __forceinline__ typename V2T<T>::type load_x2(const T *__restrict__ x
 // side is read from the FP64 vector din and stored, converted, into bout
 // (the level's b) at the pair's own points, which replaces the conversion
 // pass at the start of the V-cycle.
-template <typename T, int TY, int KC, bool INIT, bool DOUT = false, bool DIN = false>
-__global__ __launch_bounds__(64 * (TY + 2), PAIR2_MIN_WAVES(T)) void k_rbgs_pair2(LGeom g, Mat<T> A,
+//
+// YL: the y neighbours' values (the south coupling, which is the row
+// below's north one, and x_in of the rows below and above) come from LDS
+// instead of three more loads per plane: every row publishes its north
+// couplings and x_in of plane k+1 one plane ahead (both loaded two planes
+// ahead), in an LDS buffer per plane parity, before the plane's barrier.
+// Only the tile's outer rows load their outer neighbours.  6 loads per
+// plane instead of 9, with the same single barrier per plane.
+template <typename T, int TY, int KC, bool INIT, bool DOUT = false, bool DIN = false,
+          bool YL = false>
+__global__ __launch_bounds__(64 * (TY + 2), YL ? PAIR2_YL_MIN_WAVES(T) : PAIR2_MIN_WAVES(T))
+void k_rbgs_pair2(LGeom g, Mat<T> A,
                                                               const T *__restrict__ xin,
                                                               T *__restrict__ xout,
                                                               const T *__restrict__ b,
@@ -628,6 +668,8 @@ __global__ __launch_bounds__(64 * (TY + 2), PAIR2_MIN_WAVES(T)) void k_rbgs_pair
 {
   typedef typename V2T<T>::type T2;
   __shared__ T red[3][TY + 2][64];
+  __shared__ T2 syn[YL ? 2 : 1][YL ? TY + 2 : 1][64];   // north couplings by plane parity
+  __shared__ T2 syx[YL && !INIT ? 2 : 1][YL && !INIT ? TY + 2 : 1][64];   // x_in by parity
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int gx = (g.n[0] + PX2 - 1) / PX2, gy = (g.n[1] + TY - 1) / TY;
   const int gz = (g.n[2] + KC - 1) / KC;
@@ -649,22 +691,56 @@ __global__ __launch_bounds__(64 * (TY + 2), PAIR2_MIN_WAVES(T)) void k_rbgs_pair
   const unsigned col = (unsigned)LIDX(g, ic, jc, 0) - sz;   // plane q at col + (q+1) sz
   auto at = [&](int q) { return col + (unsigned)(min(max(q, 0), nz - 1) + 1) * sz; };
   unsigned ip = valid ? (unsigned)LIDX(g, i0, j, kb - 1) : 0u;   // store address, plane k-1
+  // YL: the tile's outer rows still load their outer neighbours (uniform
+  // over each row's wavefront)
+  const bool own_s = !YL || ty == 0, own_n = !YL || ty == TY + 1;
 
   // carried state
   T sc = T(1), sb = T(0), saw = T(0), sae = T(0), sas = T(0), san = T(0), sal = T(0), sau = T(0);
   T rm1 = T(0), rm2 = T(0);
   PairSet<T> cur, nxt;
-  load_pair<T, INIT, DIN>(cur, A, xin, b, at(kb - 1), sy, din);
+  T2 an1 = T2{T(0), T(0)}, an2 = T2{T(0), T(0)};
+  if (YL) {
+    load_pair_yl<T, INIT, DIN>(cur, A, xin, b, at(kb - 1), sy, own_s, own_n, din);
+    cur.an = *(const T2 *)(A.u[1] + at(kb - 1));
+    an1 = *(const T2 *)(A.u[1] + at(kb));
+  } else {
+    load_pair<T, INIT, DIN>(cur, A, xin, b, at(kb - 1), sy, din);
+  }
   T2 al = *(const T2 *)(A.u[2] + at(kb - 2));        // lower coupling of plane kb-1
   T2 xm = load_x2<T, INIT>(xin, at(kb - 2));
   T2 x0 = load_x2<T, INIT>(xin, at(kb - 1));
   T2 xp = load_x2<T, INIT>(xin, at(kb));
+  if (YL) {
+    // plane kb-1's shared values, before the first plane's red update
+    syn[(kb - 1) & 1][ty][tx] = cur.an;
+    if (!INIT) syx[(kb - 1) & 1][ty][tx] = x0;
+    lds_barrier();
+  }
   int it = 0;
   for (int k = kb - 1; k <= ke; k++, it++) {
     // prefetch: coefficients and y-neighbour x of plane k+1, x of plane k+2
-    load_pair<T, INIT, DIN>(nxt, A, xin, b, at(k + 1), sy, din);
+    // (YL: the north couplings of plane k+2 as well, published one plane
+    // later)
+    if (YL) {
+      load_pair_yl<T, INIT, DIN>(nxt, A, xin, b, at(k + 1), sy, own_s, own_n, din);
+      an2 = *(const T2 *)(A.u[1] + at(k + 2));
+    } else {
+      load_pair<T, INIT, DIN>(nxt, A, xin, b, at(k + 1), sy, din);
+    }
     const T2 xpp = load_x2<T, INIT>(xin, at(k + 2));
     if (DIN && owned && k >= kb && k < ke) *(T2 *)(bout + ip) = cur.bb;
+    if (YL) {
+      // plane k+1's north couplings and x_in for the rows beside
+      syn[(k + 1) & 1][ty][tx] = an1;
+      if (!INIT) syx[(k + 1) & 1][ty][tx] = xp;
+      // this plane's y neighbours, published one plane ago
+      if (!own_s) {
+        cur.as = syn[k & 1][ty - 1][tx];
+        if (!INIT) cur.xs = syx[k & 1][ty - 1][tx];
+      }
+      if (!own_n && !INIT) cur.xn = syx[k & 1][ty + 1][tx];
+    }
     // ---- red at plane k: the lane's red point is 0 when (i0+j+k) is odd
     const bool p0red = ((i0 + j + k) & 1) == 1;
     const bool here = valid && k >= 0 && k < nz;
@@ -725,6 +801,7 @@ __global__ __launch_bounds__(64 * (TY + 2), PAIR2_MIN_WAVES(T)) void k_rbgs_pair
     rm1 = rk;
     al = cur.au;
     cur = nxt;
+    if (YL) { cur.an = an1; an1 = an2; }
     xm = x0; x0 = xp; xp = xpp;
     ip += sz;
   }
@@ -1414,7 +1491,7 @@ static int sweep_list(int symmetric, int zero, int max_iter, HalfSweep *seq, int
 // Fused pair tile shapes (rows TY of a tile, z-chunk KC).  Each multigrid
 // times them per grid level when it is created (tune_pairs) and keeps the
 // fastest; PFHIP_RBGS_VARIANT=v forces shape v on every level.
-#define NPAIRV 5
+#define NPAIRV 10
 static int rbgs_forced()
 {
   const char *e = getenv("PFHIP_RBGS_VARIANT");
@@ -1436,31 +1513,32 @@ static void launch_pair_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout
                        A, xin, xout, b);
 }
 
-template <typename T, int TY, int KC>
+template <typename T, int TY, int KC, bool YL>
 static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
                            int init, hipStream_t st, double *dout, const double *din)
 {
   unsigned gx = (g.n[0] + PX2 - 1) / PX2, gy = (g.n[1] + TY - 1) / TY, gz = (g.n[2] + KC - 1) / KC;
   unsigned nb = gx * gy * gz, nb8 = (nb + 7u) & ~7u;
   if (init && din)
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true>), dim3(nb8),
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true, YL>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, din,
                        (T *)b);
   else if (init)
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true>), dim3(nb8), dim3(64, TY + 2, 1), 0, st,
-                       g, A, xin, xout, b, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, false, YL>), dim3(nb8),
+                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr);
   else if (dout)
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true>), dim3(nb8), dim3(64, TY + 2, 1), 0,
-                       st, g, A, xin, xout, b, dout, nullptr, nullptr);
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true, false, YL>), dim3(nb8),
+                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr);
   else
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false>), dim3(nb8), dim3(64, TY + 2, 1), 0, st,
-                       g, A, xin, xout, b, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, false, false, YL>), dim3(nb8),
+                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr);
 }
 
-// shape v: (TY, KC) = (14, 16), (6, 16), (6, 32), (14, 8), (14, 4); dout
-// (even x extents, non-initial pairs only) takes the result in FP64, din
-// (even x extents, initial pairs only) gives the right-hand side in FP64
-// and b receives it in the level precision
+// shape v % 5: (TY, KC) = (14, 16), (6, 16), (6, 32), (14, 8), (14, 4);
+// v >= 5 takes the y neighbours through LDS (the YL form; even x extents);
+// dout (even x extents, non-initial pairs only) takes the result in FP64,
+// din (even x extents, initial pairs only) gives the right-hand side in
+// FP64 and b receives it in the level precision
 template <typename T>
 static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
                         int init, int v, hipStream_t st, double *dout = nullptr,
@@ -1469,14 +1547,19 @@ static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, 
   if ((g.n[0] % 2) == 0) {
     // two points per lane (aligned vector loads need an even x extent)
     switch (v) {
-      case 1: launch_pair2_t<T, 6, 16>(g, A, xin, xout, b, init, st, dout, din); return;
-      case 2: launch_pair2_t<T, 6, 32>(g, A, xin, xout, b, init, st, dout, din); return;
-      case 3: launch_pair2_t<T, 14, 8>(g, A, xin, xout, b, init, st, dout, din); return;
-      case 4: launch_pair2_t<T, 14, 4>(g, A, xin, xout, b, init, st, dout, din); return;
-      default: launch_pair2_t<T, 14, 16>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 1: launch_pair2_t<T, 6, 16, false>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 2: launch_pair2_t<T, 6, 32, false>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 3: launch_pair2_t<T, 14, 8, false>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 4: launch_pair2_t<T, 14, 4, false>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 5: launch_pair2_t<T, 14, 16, true>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 6: launch_pair2_t<T, 6, 16, true>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 7: launch_pair2_t<T, 6, 32, true>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 8: launch_pair2_t<T, 14, 8, true>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 9: launch_pair2_t<T, 14, 4, true>(g, A, xin, xout, b, init, st, dout, din); return;
+      default: launch_pair2_t<T, 14, 16, false>(g, A, xin, xout, b, init, st, dout, din); return;
     }
   }
-  switch (v) {
+  switch (v % 5) {
     case 1: launch_pair_t<T, 6, 16>(g, A, xin, xout, b, init, st); break;
     case 2: launch_pair_t<T, 6, 32>(g, A, xin, xout, b, init, st); break;
     case 3: launch_pair_t<T, 14, 8>(g, A, xin, xout, b, init, st); break;

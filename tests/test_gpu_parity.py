@@ -594,10 +594,12 @@ def test_multigrid_grid_levels(gpu, precision, fuse, shape, monkeypatch):
             assert err < tol, (pre, post, sym, rep, err)
 
 
+@pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("shape", [(70, 40, 24), (131, 20, 18)])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
-def test_multigrid_pair_shapes(gpu, variant, shape, monkeypatch):
-    """Every smoother tile shape the autotuner chooses from, forced on all
+@pytest.mark.parametrize("variant", list(range(10)))
+def test_multigrid_pair_shapes(gpu, variant, shape, precision, monkeypatch):
+    """Every smoother tile shape the autotuner chooses from (5 tile shapes,
+    each with the y neighbours loaded or taken through LDS), forced on all
     grid levels (even and odd x extents), against the oracle V-cycle."""
     api = _api()
     monkeypatch.setenv("PFHIP_MG_SMALL", "300")
@@ -611,12 +613,12 @@ def test_multigrid_pair_shapes(gpu, variant, shape, monkeypatch):
     rng = np.random.default_rng(25)
     mgo = O.MG(prob, 2, 2, 1)
     mgo.setup(J)
-    mgg = api.GPUMultigrid(gp, 2, 2, 1, 64)
+    mgg = api.GPUMultigrid(gp, 2, 2, 1, precision)
     mgg.setup(J)
     b = np.zeros(prob.shape)
     b[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
     err = _relmax(_I(prob, mgg.apply(b)), _I(prob, mgo.apply(b)))
-    assert err < 1e-11, err
+    assert err < (1e-11 if precision == 64 else 2e-5), err
 
 
 @pytest.mark.parametrize("overland,dirs", [(False, True), (True, False)])

@@ -316,11 +316,20 @@ def test_c3_water_balance(gpu):
 def test_c4_all_members_batched_vs_one_at_a_time(gpu):
     """All 512 hillslopes advanced 2 steps with batched launches against
     each member advanced alone on the single-problem mixed path (the same
-    solver objects, ops reset to direct): 1e-10 per cell, Newton +-1."""
+    solver objects, ops reset to direct): 1e-10 per cell, Newton +-1.
+    Both are mixed solves whose FP32 V-cycles sum in different orders (the
+    single-problem path's fused pair and residual-restriction kernels
+    against the ensemble's per-point phases), so they agree to the accuracy
+    of the nonlinear solves, not bit for bit: at the deck's ResidualTol and
+    StepTol the worst member differed by 5.3e-8.  The comparison therefore
+    runs both at ResidualTol 1e-12 and StepTol 1e-30.  The ensemble's own
+    execution forms agree bit for bit
+    (test_gpu_parity.py::test_ensemble_grid_levels_and_graph_replay)."""
     api = _api()
     from parflow_amd.decks import c4_members
     nsteps = 2
-    members = c4_members(512)
+    members = [(p, dataclasses.replace(o, residual_tol=1e-12, step_tol=1e-30), p0, dt)
+               for p, o, p0, dt in c4_members(512)]
     _progress(f"c4: {len(members)} members, {sum(m[0].ncells for m in members)} cells")
     ens = api.GPUEnsemble([(p, o) for p, o, _, _ in members])
     dps = [ens.problems[i].vec(members[i][2]) for i in range(len(members))]
