@@ -242,6 +242,22 @@ int  pfhip_evap_trans_sum(pfhip_problem *pb, double dt, double *sum,
 int  pfhip_water_balance_storage(pfhip_problem *pb, const double *p,
                                  const double *s, double *subsurface,
                                  double *surface, void *stream);
+/* No counterpart in the reference.  pfhip_water_balance_storage for
+ * callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and the same results
+ * bit for bit, with the reduction partials and results kept in a
+ * file-scope device array of PFHIP_NOALLOC_SCRATCH_DOUBLES doubles, one
+ * copy per GPU, held until the process ends; this call allocates nothing.
+ * Its address is looked up per device at pfhip_problem_create (or the
+ * first call on a device).  Calls that overlap on one GPU (several
+ * streams or host threads) share that copy and race: serialise them.
+ * A problem whose reduction needs more scratch is rejected with
+ * PFHIP_ERR_ARG (none does today: each sum has at most 1024 block
+ * partials, so 2 * 1024 + 2 doubles are used at most). */
+#define PFHIP_NOALLOC_SCRATCH_DOUBLES 4096
+int  pfhip_water_balance_storage_noalloc(pfhip_problem *pb, const double *p,
+                                         const double *s, double *subsurface,
+                                         double *surface, void *stream);
 
 /* ---- multigrid preconditioner (PrecondInvoke, PL/parflow_proto.h:418) - */
 /* Semicoarsening V-cycle with the level structure, operator-induced

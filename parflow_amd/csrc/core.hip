@@ -8,6 +8,7 @@
 // every load along x is coalesced) with one z-plane per gridDim.z slice.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -304,6 +305,8 @@ static void build_kr_table(int method, int points, double min_head, double alpha
   free(del);
 }
 
+static double *noalloc_scratch();
+
 extern "C" int pfhip_problem_create(const pfhip_problem_desc *desc,
                                     pfhip_problem **out)
 {
@@ -394,6 +397,8 @@ extern "C" int pfhip_problem_create(const pfhip_problem_desc *desc,
     }
     free(host);
   }
+  // set-up of the no-allocation entry point: this device's scratch address
+  if (rc == PFHIP_OK && !noalloc_scratch()) rc = PFHIP_ERR_HIP;
   if (rc != PFHIP_OK) { pfhip_problem_destroy(pb); return rc; }
   HIPCHK(hipDeviceSynchronize());
   *out = pb;
@@ -806,6 +811,80 @@ extern "C" int pfhip_water_balance_storage(pfhip_problem *pb, const double *p,
   LAUNCHCHK();
   int rc = fetch_results(pb, 2, stream);
   if (rc) return rc;
+  *subsurface = pb->h_red[0];
+  *surface = pb->h_red[1];
+  return PFHIP_OK;
+}
+
+// ---- the same storage sums without allocating ------------------------------
+// For callers that may not allocate device memory: the block partials and
+// the two results live in a file-scope device array, one copy per GPU, held
+// until the process ends.  Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the symbol): the address of
+// each GPU's copy is looked up once per device with hipGetSymbolAddress,
+// with that device current, and kept in noalloc_ptr[device].  Every call on
+// one GPU shares that GPU's copy: calls that overlap on one GPU (concurrent
+// streams or host threads) race.  Nothing in it has to start at zero: every
+// block writes its own partial and the final pass reads exactly those.
+#define PFHIP_NOALLOC_MAX_DEV 64
+static __device__ __align__(256) double scratch[PFHIP_NOALLOC_SCRATCH_DOUBLES];
+static double *noalloc_ptr[PFHIP_NOALLOC_MAX_DEV];
+static pthread_mutex_t noalloc_mu = PTHREAD_MUTEX_INITIALIZER;
+
+// the current device's copy of `scratch` (resolved on the first call per
+// device); NULL with the error set on failure
+static double *noalloc_scratch()
+{
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= PFHIP_NOALLOC_MAX_DEV) {
+    pfhip_set_error("no-allocation scratch: current device %d not usable", dev);
+    return nullptr;
+  }
+  pthread_mutex_lock(&noalloc_mu);
+  double *p = noalloc_ptr[dev];
+  if (!p) {
+    void *q = nullptr;
+    hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !q) {
+      pfhip_set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                      hipGetErrorString(e));
+    } else {
+      p = (double *)q;
+      noalloc_ptr[dev] = p;
+    }
+  }
+  pthread_mutex_unlock(&noalloc_mu);
+  return p;
+}
+
+extern "C" int pfhip_water_balance_storage_noalloc(pfhip_problem *pb, const double *p,
+                                                   const double *s, double *subsurface,
+                                                   double *surface, void *stream)
+{
+  if (!pb || !p || !s || !subsurface || !surface) {
+    pfhip_set_error("water_balance_storage_noalloc: NULL argument");
+    return PFHIP_ERR_ARG;
+  }
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  // layout: nb subsurface partials, nb surface partials, the two results
+  if (2L * nb + 2 > PFHIP_NOALLOC_SCRATCH_DOUBLES) {
+    pfhip_set_error("water_balance_storage_noalloc: %d reduction blocks need %ld doubles of "
+                    "scratch, more than its %d", nb, 2L * nb + 2,
+                    PFHIP_NOALLOC_SCRATCH_DOUBLES);
+    return PFHIP_ERR_ARG;
+  }
+  double *sp = noalloc_scratch();
+  if (!sp) return PFHIP_ERR_HIP;
+  hipLaunchKernelGGL(k_storage, dim3(nb), dim3(256), 0, STREAM(stream), g, pb->d.dx,
+                     pb->d.dy, pb->d.dz, p, s, pb->d.poro, pb->d.ss, sp, nb);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, STREAM(stream), sp, nb, 2,
+                     sp + 2 * nb);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(pb->h_red, sp + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                        STREAM(stream)));
+  HIPCHK(hipStreamSynchronize(STREAM(stream)));
   *subsurface = pb->h_red[0];
   *surface = pb->h_red[1];
   return PFHIP_OK;
