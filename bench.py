@@ -62,9 +62,10 @@ def kernel_bytes_per_cell(gp):
                            written in the multigrid's FP32
       smoother pair (finest level, FP32): C, E, N, U, b, x_in read, x_out
                            written; the V-cycle's last pair writes the FP64
-                           result (4 more bytes), its first reads the FP64
-                           right-hand side and stores the level's FP32 copy
-                           (4 more read, 4 more written)"""
+                           result instead (4 more bytes); its first (zero
+                           initial guess: no x_in) reads C, E, N, U and the
+                           FP64 right-hand side and writes the level's FP32
+                           copy of it and x_out"""
     f = _field_reads(gp)
     res = 8 * (1 + f + 2 + 2 + 1)
     return {
@@ -73,8 +74,8 @@ def kernel_bytes_per_cell(gp):
         "residual_step": res + 16,
         "jacobian": 8 * (1 + f + 2) + 4 * 4,
         "rbgs_pair": 4 * 7,
-        "rbgs_pair_dout": 4 * 7 + 4,
-        "rbgs_pair_din": 4 * 7 + 4 + 4,
+        "rbgs_pair_dout": 4 * 6 + 8,
+        "rbgs_pair_din": 4 * 4 + 8 + 4 + 4,
     }
 
 
@@ -226,11 +227,14 @@ def run_gpu(args):
     K, W = args.steps, args.warmup
     res = {"opts": opts, "dt": dt}
 
-    # mixed: the timed run with the kernel timers on
+    # mixed: the timed run (no timers), then the same steps with the
+    # per-launch kernel timers on (their events bracket each launch, and the
+    # V-cycle then launches directly instead of replaying its graph)
     sv = api.GPUSolver(gp, opts, precision=32, krylov=1)
+    mixed, snap_m = _timed_run(api, gp, sv, p0, dt, K, W)
     for _, pid, _ in PROF_KERNELS:
         api.check(L.pfhip_prof_enable(getattr(api, pid), 20000), "prof")
-    mixed, snap_m = _timed_run(api, gp, sv, p0, dt, K, W)
+    _timed_run(api, gp, sv, p0, dt, K, 0, snapshot=False)
     mixed["prof"] = _read_prof(api, [(k, pid) for k, pid, _ in PROF_KERNELS])
     L.pfhip_solver_mg.restype = api.C.c_void_p
     L.pfhip_solver_mg.argtypes = [api.C.c_void_p]
@@ -320,7 +324,7 @@ def run_cpu_baseline(args, cores, prob, opts, dt, start):
 def kernel_probe(args):
     """Each measured kernel once on the workload's fields, for hardware
     counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the residual in
-    both forms, the PC-matrix assembly into the FP32 multigrid, one V-cycle,
+    its three forms, the PC-matrix assembly into the FP32 multigrid, one V-cycle,
     and a plain copy kernel (z = 1.0 x over the interior cells) that
     calibrates the counters."""
     import ctypes as C
@@ -341,12 +345,17 @@ def kernel_probe(args):
     L.pfhip_residual_jv.argtypes = [C.c_void_p] * 3 + [C.c_double, C.c_void_p, C.c_double] + \
         [C.c_void_p] * 3 + [C.c_double, C.c_void_p]
     L.pfhip_mg_setup_jacobian.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
+    L.pfhip_residual_step.argtypes = [C.c_void_p] * 3 + [C.c_double] + [C.c_void_p] * 4 + \
+        [C.c_double, C.c_void_p]
+    unew = gp.vec()
     L.pfhip_vec_scale.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
     for _ in range(2):   # the first round warms up; a counter run reads both
         api.check(L.pfhip_nl_function_eval(gp.h, p.ptr, F.ptr, pold.ptr, sold.ptr,
                                            C.c_double(dt), None), "F")
         api.check(L.pfhip_residual_jv(gp.h, p.ptr, v.ptr, 1e-6, F.ptr, 1e6, Z.ptr, pold.ptr,
                                       sold.ptr, dt, None), "Jv")
+        api.check(L.pfhip_residual_step(gp.h, p.ptr, v.ptr, 1e-3, unew.ptr, Z.ptr, pold.ptr,
+                                        sold.ptr, dt, None), "line-search point")
         api.check(L.pfhip_mg_setup_jacobian(mg.h, p.ptr, dt, None), "PC")
         api.check(L.pfhip_mg_apply(mg.h, F.ptr, Z.ptr, None), "V-cycle")
         api.check(L.pfhip_vec_scale(gp.h, 1.0, F.ptr, Z.ptr, None), "copy")
@@ -447,9 +456,10 @@ def run_ensemble(args):
 
 
 def kernel_rows(prob, gp, prof):
-    """Roofline rows of the timed run: achieved = algorithmic bytes of a
-    launch / its average HIP-event duration; one row per kernel, and one
-    per family (its forms' bytes and times summed)."""
+    """Roofline rows of the timed steps (repeated with the per-launch timers
+    on): achieved = algorithmic bytes of a launch / its average HIP-event
+    duration; one row per kernel, and one per family (its forms' bytes and
+    times summed)."""
     bpc_all = kernel_bytes_per_cell(gp)
     traffic = load_traffic()
     rows = {}
@@ -527,7 +537,7 @@ def main():
     mixed, opts, dt = res["mixed"], res["opts"], res["dt"]
     K = args.steps
     kernels = kernel_rows(prob, gp, mixed["prof"])
-    # the dominant kernel: most total time in the timed region
+    # the dominant kernel: most total time over the timed steps
     dom = max(kernels, key=lambda r: r["total_ms"])
     line = {
         "metric": "cell-timesteps/s (16.8M cells), mixed-precond vs FP64 plain at 1e-8 rel err",

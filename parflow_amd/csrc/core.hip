@@ -131,6 +131,11 @@ extern "C" int pfhip_prof_disable(int kernel_id)
   return PFHIP_OK;
 }
 
+extern "C" int pfhip_prof_enabled(int kernel_id)
+{
+  return (kernel_id >= 0 && kernel_id < PFHIP_PROF_NKERNELS) ? g_prof.enabled[kernel_id] : 0;
+}
+
 extern "C" void pfhip_prof_mark(int kernel_id, int end, void *stream)
 {
   if (!g_prof.enabled[kernel_id]) return;

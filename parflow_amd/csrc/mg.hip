@@ -43,6 +43,7 @@
 #define STREAM(s) ((hipStream_t)(s))
 #define MAXL 128
 #define MG_SMALL_DEFAULT 4096
+#define MG_LDS_DEFAULT (144L * 1024)   // LDS of the bottom-of-cycle workgroup
 #define CWG 1024
 #define MAX_SWEEPS 64
 
@@ -664,9 +665,14 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
                                                               const T *__restrict__ b,
                                                               double *__restrict__ dout,
                                                               const double *__restrict__ din,
-                                                              T *__restrict__ bout)
+                                                              T *__restrict__ bout,
+                                                              void *const *io)
 {
   typedef typename V2T<T>::type T2;
+  // io: the FP64 interface vectors' addresses are read from device memory
+  // (a launch sequence captured once and replayed with other vectors)
+  if (DIN && io) din = (const double *)io[0];
+  if (DOUT && io) dout = (double *)io[1];
   __shared__ T red[3][TY + 2][64];
   __shared__ T2 syn[YL ? 2 : 1][YL ? TY + 2 : 1][64];   // north couplings by plane parity
   __shared__ T2 syx[YL && !INIT ? 2 : 1][YL && !INIT ? TY + 2 : 1][64];   // x_in by parity
@@ -690,7 +696,11 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
   const int ic = min(max(i0, 0), g.n[0] - 2), jc = min(max(j, 0), g.n[1] - 1);
   const unsigned col = (unsigned)LIDX(g, ic, jc, 0) - sz;   // plane q at col + (q+1) sz
   auto at = [&](int q) { return col + (unsigned)(min(max(q, 0), nz - 1) + 1) * sz; };
-  unsigned ip = valid ? (unsigned)LIDX(g, i0, j, kb - 1) : 0u;   // store address, plane k-1
+  // the march starts one plane below the chunk (its red values feed the
+  // chunk's first black update), except at the bottom of the level, where
+  // that plane does not exist
+  const int k0 = (kb == 0) ? 0 : kb - 1;
+  unsigned ip = valid ? (unsigned)LIDX(g, i0, j, k0) : 0u;   // store address, plane k-1
   // YL: the tile's outer rows still load their outer neighbours (uniform
   // over each row's wavefront)
   const bool own_s = !YL || ty == 0, own_n = !YL || ty == TY + 1;
@@ -701,24 +711,24 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
   PairSet<T> cur, nxt;
   T2 an1 = T2{T(0), T(0)}, an2 = T2{T(0), T(0)};
   if (YL) {
-    load_pair_yl<T, INIT, DIN>(cur, A, xin, b, at(kb - 1), sy, own_s, own_n, din);
-    cur.an = *(const T2 *)(A.u[1] + at(kb - 1));
-    an1 = *(const T2 *)(A.u[1] + at(kb));
+    load_pair_yl<T, INIT, DIN>(cur, A, xin, b, at(k0), sy, own_s, own_n, din);
+    cur.an = *(const T2 *)(A.u[1] + at(k0));
+    an1 = *(const T2 *)(A.u[1] + at(k0 + 1));
   } else {
-    load_pair<T, INIT, DIN>(cur, A, xin, b, at(kb - 1), sy, din);
+    load_pair<T, INIT, DIN>(cur, A, xin, b, at(k0), sy, din);
   }
-  T2 al = *(const T2 *)(A.u[2] + at(kb - 2));        // lower coupling of plane kb-1
-  T2 xm = load_x2<T, INIT>(xin, at(kb - 2));
-  T2 x0 = load_x2<T, INIT>(xin, at(kb - 1));
-  T2 xp = load_x2<T, INIT>(xin, at(kb));
+  T2 al = *(const T2 *)(A.u[2] + at(k0 - 1));        // lower coupling of plane k0
+  T2 xm = load_x2<T, INIT>(xin, at(k0 - 1));
+  T2 x0 = load_x2<T, INIT>(xin, at(k0));
+  T2 xp = load_x2<T, INIT>(xin, at(k0 + 1));
   if (YL) {
-    // plane kb-1's shared values, before the first plane's red update
-    syn[(kb - 1) & 1][ty][tx] = cur.an;
-    if (!INIT) syx[(kb - 1) & 1][ty][tx] = x0;
+    // plane k0's shared values, before the first plane's red update
+    syn[k0 & 1][ty][tx] = cur.an;
+    if (!INIT) syx[k0 & 1][ty][tx] = x0;
     lds_barrier();
   }
   int it = 0;
-  for (int k = kb - 1; k <= ke; k++, it++) {
+  for (int k = k0; k <= ke; k++, it++) {
     // prefetch: coefficients and y-neighbour x of plane k+1, x of plane k+2
     // (YL: the north couplings of plane k+2 as well, published one plane
     // later)
@@ -1007,6 +1017,27 @@ __global__ __launch_bounds__(256) void k_prolong_z(LGeom gf, LGeom gc, const T *
   }
 }
 
+// k_convert with the FP64 interface vector's address read from io[q] (q = 0:
+// the source is the right-hand side, q = 1: the destination is the result),
+// for launch sequences captured once and replayed with other vectors
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void k_convert_io(LGeom g, const S *__restrict__ src,
+                                                    D *__restrict__ dst, void *const *io, int q)
+{
+  XCD_IJK(g);
+  const S *sp = (q == 0) ? (const S *)io[0] : src;
+  D *dp = (q == 1) ? (D *)io[1] : dst;
+  long ip = LIDX(g, i, j, k);
+  dp[ip] = (D)sp[ip];
+}
+
+// the interface addresses of one replay of a captured V-cycle
+__global__ void k_set_io(void **io, const double *b, double *x)
+{
+  io[0] = (void *)b;
+  io[1] = (void *)x;
+}
+
 template <typename S, typename D>
 __global__ __launch_bounds__(256) void k_convert(LGeom g, const S *__restrict__ src,
                                                  D *__restrict__ dst)
@@ -1206,6 +1237,103 @@ __global__ __launch_bounds__(CWG) void k_coarse_vcycle(const LevelDev<T> *__rest
   vcycle_block<T, FULL>(lev, lc, L, pre, post, symmetric);
 }
 
+// The same bottom with every level's arrays in LDS (symmetric storage).
+// Each level is copied into a compact, ghost-free block of LDS (C, E, N,
+// U, x, b, t, p1, p2 of its points); its LDS level record carries compact
+// strides and array pointers shifted so that LIDX addresses the compact
+// block.  Every per-point function reads a neighbour or a lower coupling
+// only when that neighbour exists, so no ghost entry is ever touched.  The
+// phases then wait on LDS instead of L2, and the sweeps, residuals and
+// transfers are the global kernel's (the same functions, in the same
+// order).  b of the top level is copied in, x of the top level out; the
+// coarser levels' x, b, t are produced inside.
+#define LDS_ALIGN 16
+__host__ __device__ static inline long lds_level_table_bytes(int nl, long rec)
+{
+  return ((nl * rec) + LDS_ALIGN - 1) & ~(long)(LDS_ALIGN - 1);
+}
+
+// bin / xout (the ensemble's levels 0 .. L-1 in one workgroup): the top
+// level's b comes from the FP64 vector bin and its x goes to xout, both in
+// the top level's ghosted layout; otherwise b and x are the top level's own
+template <typename T>
+__device__ void vcycle_lds(const LevelDev<T> *__restrict__ lev, int lc, int L, int pre,
+                           int post, int symmetric, const double *__restrict__ bin,
+                           double *__restrict__ xout)
+{
+  extern __shared__ __align__(LDS_ALIGN) unsigned char smem[];
+  const int nl = L - lc;
+  LevelDev<T> *lv = (LevelDev<T> *)smem;
+  T *pool = (T *)(smem + lds_level_table_bytes(nl, (long)sizeof(LevelDev<T>)));
+  if (threadIdx.x == 0) {
+    T *q = pool;
+    for (int m = 0; m < nl; m++) {
+      const LevelDev<T> &G = lev[lc + m];
+      LevelDev<T> D;
+      D.g = G.g;
+      D.g.s[0] = 1;
+      D.g.s[1] = G.g.n[0];
+      D.g.s[2] = G.g.n[0] * G.g.n[1];
+      D.dir = G.dir; D.o1 = G.o1; D.o2 = G.o2;
+      const long np = (long)G.g.n[0] * G.g.n[1] * G.g.n[2];
+      const long sh = 1 + D.g.s[1] + D.g.s[2];   // LIDX of (0, 0, 0)
+      D.A.c = q - sh; q += np;
+      for (int d = 0; d < 3; d++) { D.A.u[d] = q - sh; q += np; D.A.l[d] = nullptr; }
+      D.x = q - sh; q += np;
+      D.b = q - sh; q += np;
+      D.t = q - sh; q += np;
+      D.p1 = q - sh; q += np;
+      D.p2 = q - sh; q += np;
+      lv[m] = D;
+    }
+  }
+  __syncthreads();
+  for (int m = 0; m < nl; m++) {
+    const LevelDev<T> &G = lev[lc + m];
+    const LevelDev<T> &D = lv[m];
+    const bool top = (m == 0);
+    const bool last = (m == nl - 1);
+    const int np = G.g.n[0] * G.g.n[1] * G.g.n[2];
+    for (int idx = threadIdx.x; idx < np; idx += blockDim.x) {
+      int i, j, k;
+      unflat(G.g, idx, i, j, k);
+      const long ig = LIDX(G.g, i, j, k);
+      const long il = LIDX(D.g, i, j, k);
+      D.A.c[il] = G.A.c[ig];
+      D.A.u[0][il] = G.A.u[0][ig];
+      D.A.u[1][il] = G.A.u[1][ig];
+      D.A.u[2][il] = G.A.u[2][ig];
+      if (!last) { D.p1[il] = G.p1[ig]; D.p2[il] = G.p2[ig]; }
+      if (top) D.b[il] = bin ? (T)bin[ig] : G.b[ig];
+    }
+  }
+  __syncthreads();
+  vcycle_block<T, false>(lv, 0, nl, pre, post, symmetric);
+  const LevelDev<T> &G = lev[lc];
+  const LevelDev<T> &D = lv[0];
+  if (xout) {
+    FOR_POINTS(G.g, xout[LIDX(G.g, i, j, k)] = (double)D.x[LIDX(D.g, i, j, k)]);
+  } else {
+    FOR_POINTS(G.g, G.x[LIDX(G.g, i, j, k)] = D.x[LIDX(D.g, i, j, k)]);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(CWG) void k_coarse_vcycle_lds(const LevelDev<T> *__restrict__ lev,
+                                                           int lc, int L, int pre, int post,
+                                                           int symmetric)
+{
+  vcycle_lds<T>(lev, lc, L, pre, post, symmetric, nullptr, nullptr);
+}
+
+// LDS bytes of the bottom levels lc .. L-1 in k_coarse_vcycle_lds
+static long lds_bottom_bytes(const int (*n)[3], int lc, int L, size_t es, size_t rec)
+{
+  long b = lds_level_table_bytes(L - lc, (long)rec);
+  for (int l = lc; l < L; l++) b += 9L * (long)es * n[l][0] * n[l][1] * n[l][2];
+  return b;
+}
+
 // coarse operators of levels ls .. L-2 -> ls+1 .. L-1
 template <typename T, bool FULL>
 __global__ __launch_bounds__(CWG) void k_coarse_setup(const LevelDev<T> *__restrict__ lev,
@@ -1236,6 +1364,11 @@ struct pfhip_mg {
   int pv[MAXL];                // pair tile shape of each grid level (tune_pairs)
   int tuned;                   // pv chosen (at the first V-cycle)
   void *levdev;                // device LevelDev<T>[L]
+  long lds_bytes;              // > 0: the bottom runs in LDS (k_coarse_vcycle_lds)
+  int use_graph;               // replay the captured V-cycle (mg_capture)
+  hipStream_t cs;              // capture stream
+  hipGraphExec_t gx;           // the captured V-cycle
+  void **io;                   // device: the replay's right-hand side and result
 };
 
 template <typename T> static int tune_pairs(pfhip_mg *mg, hipStream_t st);
@@ -1330,6 +1463,44 @@ static int mg_create(pfhip_problem *pb, int max_levels, int pre, int post, int s
   for (l = 1; l < mg->L; l++)
     if (npts(mg->n[l]) <= small) { mg->lc = l; break; }
   size_t es = (precision == 64) ? sizeof(double) : sizeof(float);
+  // bottom in LDS (symmetric storage): the first level from which every
+  // coarser level fits the LDS budget, if that is no larger than the
+  // single-workgroup bound
+  const char *fl = getenv("PFHIP_MG_LDS");
+  const long lds_budget = (fl && *fl) ? atol(fl) : MG_LDS_DEFAULT;
+  mg->lds_bytes = 0;
+  int dev = 0, lds_max = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+          hipSuccess)
+    lds_max = 0;
+  if (!full && lds_budget > 0 && lds_budget <= lds_max && !(env && *env)) {
+    const size_t rec = (precision == 64) ? sizeof(LevelDev<double>) : sizeof(LevelDev<float>);
+    for (l = 1; l <= mg->L - 2; l++) {
+      const long bytes = lds_bottom_bytes(mg->n, l, mg->L, es, rec);
+      if (bytes <= lds_budget && npts(mg->n[l]) <= small) {
+        mg->lc = l;
+        mg->lds_bytes = bytes;
+        break;
+      }
+    }
+    // more than the default dynamic LDS of a launch: raise the kernel's limit
+    if (mg->lds_bytes > 0) {
+      hipError_t e = (precision == 64)
+          ? hipFuncSetAttribute((const void *)k_coarse_vcycle_lds<double>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)mg->lds_bytes)
+          : hipFuncSetAttribute((const void *)k_coarse_vcycle_lds<float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)mg->lds_bytes);
+      if (e != hipSuccess) {
+        // keep the bottom in global memory with the single-workgroup bound
+        (void)hipGetLastError();
+        mg->lds_bytes = 0;
+        mg->lc = mg->L - 1;
+        for (l = 1; l < mg->L; l++)
+          if (npts(mg->n[l]) <= small) { mg->lc = l; break; }
+      }
+    }
+  }
   int rc = PFHIP_OK;
   for (l = 0; l < mg->L && rc == PFHIP_OK; l++) {
     long s = lsize(mg->n[l]);
@@ -1352,6 +1523,8 @@ static int mg_create(pfhip_problem *pb, int max_levels, int pre, int post, int s
   int forced = rbgs_forced();
   for (l = 0; l < MAXL; l++) mg->pv[l] = (forced >= 0) ? forced : 0;
   mg->tuned = (forced >= 0) || !mg->fuse || full;
+  const char *fg = getenv("PFHIP_MG_GRAPH");
+  mg->use_graph = (fg && *fg) ? atoi(fg) : 1;
   HIPCHK(hipDeviceSynchronize());
   *out = mg;
   return PFHIP_OK;
@@ -1372,6 +1545,9 @@ extern "C" int pfhip_mg_create_full(pfhip_problem *pb, int max_levels, int pre, 
 extern "C" int pfhip_mg_destroy(pfhip_mg *mg)
 {
   if (!mg) return PFHIP_OK;
+  if (mg->gx) (void)hipGraphExecDestroy(mg->gx);
+  if (mg->cs) (void)hipStreamDestroy(mg->cs);
+  if (mg->io) pfhip_free(mg->io);
   for (int l = 0; l < mg->L; l++) {
     for (int q = 0; q < 13; q++) if (mg->raw[l][q]) hipFree(mg->raw[l][q]);
   }
@@ -1515,23 +1691,26 @@ static void launch_pair_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout
 
 template <typename T, int TY, int KC, bool YL>
 static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
-                           int init, hipStream_t st, double *dout, const double *din)
+                           int init, hipStream_t st, double *dout, const double *din,
+                           void *const *io)
 {
   unsigned gx = (g.n[0] + PX2 - 1) / PX2, gy = (g.n[1] + TY - 1) / TY, gz = (g.n[2] + KC - 1) / KC;
   unsigned nb = gx * gy * gz, nb8 = (nb + 7u) & ~7u;
   if (init && din)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true, YL>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, din,
-                       (T *)b);
+                       (T *)b, io);
   else if (init)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, false, YL>), dim3(nb8),
-                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr);
+                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr,
+                       io);
   else if (dout)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true, false, YL>), dim3(nb8),
-                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr);
+                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr, io);
   else
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, false, false, YL>), dim3(nb8),
-                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr);
+                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr,
+                       io);
 }
 
 // shape v % 5: (TY, KC) = (14, 16), (6, 16), (6, 32), (14, 8), (14, 4);
@@ -1542,21 +1721,21 @@ static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xou
 template <typename T>
 static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
                         int init, int v, hipStream_t st, double *dout = nullptr,
-                        const double *din = nullptr)
+                        const double *din = nullptr, void *const *io = nullptr)
 {
   if ((g.n[0] % 2) == 0) {
     // two points per lane (aligned vector loads need an even x extent)
     switch (v) {
-      case 1: launch_pair2_t<T, 6, 16, false>(g, A, xin, xout, b, init, st, dout, din); return;
-      case 2: launch_pair2_t<T, 6, 32, false>(g, A, xin, xout, b, init, st, dout, din); return;
-      case 3: launch_pair2_t<T, 14, 8, false>(g, A, xin, xout, b, init, st, dout, din); return;
-      case 4: launch_pair2_t<T, 14, 4, false>(g, A, xin, xout, b, init, st, dout, din); return;
-      case 5: launch_pair2_t<T, 14, 16, true>(g, A, xin, xout, b, init, st, dout, din); return;
-      case 6: launch_pair2_t<T, 6, 16, true>(g, A, xin, xout, b, init, st, dout, din); return;
-      case 7: launch_pair2_t<T, 6, 32, true>(g, A, xin, xout, b, init, st, dout, din); return;
-      case 8: launch_pair2_t<T, 14, 8, true>(g, A, xin, xout, b, init, st, dout, din); return;
-      case 9: launch_pair2_t<T, 14, 4, true>(g, A, xin, xout, b, init, st, dout, din); return;
-      default: launch_pair2_t<T, 14, 16, false>(g, A, xin, xout, b, init, st, dout, din); return;
+      case 1: launch_pair2_t<T, 6, 16, false>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 2: launch_pair2_t<T, 6, 32, false>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 3: launch_pair2_t<T, 14, 8, false>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 4: launch_pair2_t<T, 14, 4, false>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 5: launch_pair2_t<T, 14, 16, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 6: launch_pair2_t<T, 6, 16, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 7: launch_pair2_t<T, 6, 32, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 8: launch_pair2_t<T, 14, 8, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 9: launch_pair2_t<T, 14, 4, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      default: launch_pair2_t<T, 14, 16, false>(g, A, xin, xout, b, init, st, dout, din, io); return;
     }
   }
   switch (v % 5) {
@@ -1620,7 +1799,8 @@ extern "C" int pfhip_mg_pair_shapes(pfhip_mg *mg, int *shape, int max)
 // into dout in FP64 (and *wrote is set); x then holds no valid iterate.
 template <typename T>
 static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
-                double *dout = nullptr, int *wrote = nullptr, const double *din = nullptr)
+                double *dout = nullptr, int *wrote = nullptr, const double *din = nullptr,
+                void *const *io = nullptr)
 {
   LGeom g = lgeom(mg->n[l]);
   dim3 grid(ntiles(g));
@@ -1647,7 +1827,7 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
     if (mark >= 0) pfhip_prof_mark(mark, 0, st);
     if (pair) {
       launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, mg->pv[l], st,
-                     d, di);
+                     d, di, io);
       if (d && wrote) *wrote = 1;
       void *tmp = mg->x[l]; mg->x[l] = mg->xa[l]; mg->xa[l] = tmp;
       q += 2;
@@ -1680,11 +1860,15 @@ static void launch_resid_restrict_z(const LGeom &gf, const LGeom &gc, const Mat<
 }
 
 // V-cycle of PL/mg_semi.c:205-330 (MaxIter 1, zero initial guess)
+// The launch sequence of one V-cycle.  io != nullptr: the FP64 interface
+// vectors are not bin / xout but the addresses in io[0] / io[1] (device
+// memory), read by the kernels that touch them, so the sequence can be
+// captured once and replayed for other vectors.
 template <typename T>
-static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t st)
+static int mg_apply_seq(pfhip_mg *mg, const double *bin, double *xout, hipStream_t st,
+                        void *const *io)
 {
   int L = mg->L, lc = mg->lc, rc, wrote = 0;
-  if (!mg->tuned && (rc = tune_pairs<T>(mg, st))) return rc;
   LGeom g0 = lgeom(mg->n[0]);
   // the first pre-smoothing pair of level 0 converts b itself when it is an
   // initial two-points-per-lane pair
@@ -1694,13 +1878,18 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
   const bool din = mg->fuse && !mg->full && mg->xa[0] && (g0.n[0] % 2) == 0 && !fill0 &&
                    n0 >= 2 && seq0[0].par == 1 && seq0[0].init && seq0[1].par == 0;
   if (!din) {
-    hipLaunchKernelGGL((k_convert<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, bin,
-                       (T *)mg->b[0]);
+    if (io)
+      hipLaunchKernelGGL((k_convert_io<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, bin,
+                         (T *)mg->b[0], io, 0);
+    else
+      hipLaunchKernelGGL((k_convert<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, bin,
+                         (T *)mg->b[0]);
     LAUNCHCHK();
   }
   int l;
   for (l = 0; l < lc; l++) {
-    if ((rc = rbgs<T>(mg, l, 1, mg->pre, st, nullptr, nullptr, (l == 0 && din) ? bin : nullptr)))
+    if ((rc = rbgs<T>(mg, l, 1, mg->pre, st, nullptr, nullptr, (l == 0 && din) ? bin : nullptr,
+                      io)))
       return rc;
     LGeom gf = lgeom(mg->n[l]), gc = lgeom(mg->n[l + 1]);
     if (mg->rrz && !mg->full && mg->dir[l] == 2 && (gf.n[0] & 1) == 0) {
@@ -1726,7 +1915,12 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
   }
   // bottom of the cycle: levels lc .. L-1 in one workgroup; the coarsest
   // level's solve is RedBlackGSPoint with MaxIter 1
-  if (lc <= L - 2) {
+  if (lc <= L - 2 && mg->lds_bytes > 0) {
+    hipLaunchKernelGGL(k_coarse_vcycle_lds<T>, dim3(1), dim3(CWG), (unsigned)mg->lds_bytes, st,
+                       (const LevelDev<T> *)mg->levdev, lc, L, mg->pre, mg->post,
+                       mg->symmetric);
+    LAUNCHCHK();
+  } else if (lc <= L - 2) {
     if (mg->full)
       hipLaunchKernelGGL((k_coarse_vcycle<T, true>), dim3(1), dim3(CWG), 0, st,
                          (const LevelDev<T> *)mg->levdev, lc, L, mg->pre, mg->post,
@@ -1752,14 +1946,95 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
                          (const T *)mg->p2[l]);
     }
     LAUNCHCHK();
-    if ((rc = rbgs<T>(mg, l, 0, mg->post, st, l == 0 ? xout : nullptr, &wrote))) return rc;
+    if ((rc = rbgs<T>(mg, l, 0, mg->post, st, l == 0 ? xout : nullptr, &wrote, nullptr, io)))
+      return rc;
   }
   if (!wrote) {
-    hipLaunchKernelGGL((k_convert<T, double>), dim3(ntiles(g0)), LBLOCK, 0, st, g0,
-                       (const T *)mg->x[0], xout);
+    if (io)
+      hipLaunchKernelGGL((k_convert_io<T, double>), dim3(ntiles(g0)), LBLOCK, 0, st, g0,
+                         (const T *)mg->x[0], xout, io, 1);
+    else
+      hipLaunchKernelGGL((k_convert<T, double>), dim3(ntiles(g0)), LBLOCK, 0, st, g0,
+                         (const T *)mg->x[0], xout);
     LAUNCHCHK();
   }
   return PFHIP_OK;
+}
+
+// Capture the V-cycle's launch sequence once (after the autotuner has
+// chosen its tile shapes) and instantiate it.  Nothing runs during the
+// capture, but the host-side bookkeeping does: the grid levels' iterate
+// buffers swap after every fused pair, so the pointer state is saved and
+// restored, and a sequence that does not return every level's buffers to
+// where they started (an odd number of pairs on a level) is not replayable:
+// the hierarchy then keeps launching directly.
+template <typename T>
+static int mg_capture(pfhip_mg *mg)
+{
+  void *x0[MAXL], *xa0[MAXL];
+  for (int l = 0; l < mg->L; l++) { x0[l] = mg->x[l]; xa0[l] = mg->xa[l]; }
+  if (!mg->cs) HIPCHK(hipStreamCreateWithFlags(&mg->cs, hipStreamNonBlocking));
+  if (!mg->io) {
+    int rc = pfhip_malloc((void **)&mg->io, 2 * (long)sizeof(void *));
+    if (rc) return rc;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(mg->cs, hipStreamCaptureModeRelaxed));
+  // any non-null FP64 vectors select the interface forms; io supplies the
+  // real addresses at each replay
+  int rc = mg_apply_seq<T>(mg, (const double *)mg->io, (double *)mg->io, mg->cs, mg->io);
+  hipError_t e = hipStreamEndCapture(mg->cs, &g);
+  bool same = true;
+  for (int l = 0; l < mg->L; l++) {
+    same = same && mg->x[l] == x0[l] && mg->xa[l] == xa0[l];
+    mg->x[l] = x0[l];
+    mg->xa[l] = xa0[l];
+  }
+  if (rc || e != hipSuccess || !g || !same) {
+    if (g) (void)hipGraphDestroy(g);
+    mg->use_graph = 0;
+    if (rc) return rc;
+    if (e != hipSuccess) {
+      pfhip_set_error("V-cycle graph capture: %s", hipGetErrorString(e));
+      return PFHIP_ERR_HIP;
+    }
+    return PFHIP_OK;                 // not replayable: launch directly
+  }
+  e = hipGraphInstantiate(&mg->gx, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    mg->gx = nullptr;
+    pfhip_set_error("V-cycle graph instantiation: %s", hipGetErrorString(e));
+    return PFHIP_ERR_HIP;
+  }
+  return PFHIP_OK;
+}
+
+// V-cycle of PL/mg_semi.c:205-330 (MaxIter 1, zero initial guess).  The
+// first call tunes the smoother and launches directly; later calls replay
+// the captured sequence (PFHIP_MG_GRAPH=0 launches directly always), except
+// while the smoother's per-launch timers are on (they time each launch).
+template <typename T>
+static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t st)
+{
+  int rc;
+  if (!mg->tuned) {
+    if ((rc = tune_pairs<T>(mg, st))) return rc;
+    return mg_apply_seq<T>(mg, bin, xout, st, nullptr);
+  }
+  const bool timed = pfhip_prof_enabled(PFHIP_PROF_RBGS) ||
+                     pfhip_prof_enabled(PFHIP_PROF_RBGS_DOUT) ||
+                     pfhip_prof_enabled(PFHIP_PROF_RBGS_DIN);
+  if (mg->use_graph && !timed) {
+    if (!mg->gx && (rc = mg_capture<T>(mg))) return rc;
+    if (mg->gx) {
+      hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, st, mg->io, bin, xout);
+      LAUNCHCHK();
+      HIPCHK(hipGraphLaunch(mg->gx, st));
+      return PFHIP_OK;
+    }
+  }
+  return mg_apply_seq<T>(mg, bin, xout, st, nullptr);
 }
 
 extern "C" int pfhip_mg_apply(pfhip_mg *mg, const double *b, double *x, void *stream)
@@ -1807,6 +2082,7 @@ struct pfhip_ens_mgplan {
   hipStream_t cs;                     // capture stream
   hipGraphExec_t gx[2];               // [0] coarse operators, [1] V-cycle
   long glaunch[2];                    // kernels in each captured sequence
+  long lds_bytes;                     // > 0: the one-workgroup part runs in LDS
   long launches;                      // kernel launches made so far
 };
 
@@ -1879,6 +2155,18 @@ __global__ __launch_bounds__(CWG) void k_ens_vcycle(const EnsLv<T> *__restrict__
   if (mm.lc == 0) FOR_POINTS(L0.g, mm.xout[LIDX(L0.g, i, j, k)] = (double)L0.x[LIDX(L0.g, i, j, k)]);
 }
 
+// the same with the member's levels lc .. L-1 in LDS (every member's
+// bottom fits the plan's LDS budget)
+template <typename T>
+__global__ __launch_bounds__(CWG) void k_ens_vcycle_lds(const EnsLv<T> *__restrict__ M,
+                                                        const int *__restrict__ list)
+{
+  const EnsLv<T> &mm = M[list[blockIdx.x]];
+  if (!mm.act) return;                          // uniform over the workgroup
+  vcycle_lds<T>(mm.lev, mm.lc, mm.L, mm.pre, mm.post, mm.symmetric,
+                mm.lc == 0 ? mm.bin : nullptr, mm.lc == 0 ? mm.xout : nullptr);
+}
+
 template <typename T>
 __global__ __launch_bounds__(CWG) void k_ens_mg_coarse(const EnsLv<T> *__restrict__ M,
                                                        const int *__restrict__ list)
@@ -1942,6 +2230,19 @@ extern "C" int pfhip_ens_mg_plan_create(pfhip_mg *const *mgs, int n, pfhip_ens_m
   p->use_graph = (ge && *ge) ? atoi(ge) : 1;
   int rc = PFHIP_OK;
   const long small = ens_small();
+  // the one-workgroup part in LDS: each member's first level from which
+  // every coarser level fits the budget (no PFHIP_ENS_MG_SMALL given)
+  const char *se = getenv("PFHIP_ENS_MG_SMALL");
+  const char *fl = getenv("PFHIP_MG_LDS");
+  const long lds_budget = (fl && *fl) ? atol(fl) : MG_LDS_DEFAULT;
+  int dev = 0, lds_max = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+          hipSuccess)
+    lds_max = 0;
+  const bool lds = !(se && *se) && lds_budget > 0 && lds_budget <= lds_max;
+  const size_t es = (p->prec == 64) ? sizeof(double) : sizeof(float);
+  const size_t rec = (p->prec == 64) ? sizeof(LevelDev<double>) : sizeof(LevelDev<float>);
   for (int m = 0; m < n; m++) {
     const pfhip_mg *mg = mgs[m];
     if (mg->prec != p->prec || mg->full || mg->pre != p->pre || mg->post != p->post ||
@@ -1951,11 +2252,34 @@ extern "C" int pfhip_ens_mg_plan_create(pfhip_mg *const *mgs, int n, pfhip_ens_m
       rc = PFHIP_ERR_ARG;
     }
     int lc = mg->L - 1;
-    for (int l = 0; l < mg->L; l++)
-      if (npts(mg->n[l]) <= small) { lc = l; break; }
+    if (lds) {
+      for (int l = 0; l < mg->L; l++) {
+        const long b = lds_bottom_bytes(mg->n, l, mg->L, es, rec);
+        if (b <= lds_budget) {
+          lc = l;
+          if (b > p->lds_bytes) p->lds_bytes = b;
+          break;
+        }
+      }
+    } else {
+      for (int l = 0; l < mg->L; l++)
+        if (npts(mg->n[l]) <= small) { lc = l; break; }
+    }
     p->lc[m] = lc;
     if (lc > p->G) p->G = lc;
     order[m] = m;
+  }
+  if (rc == PFHIP_OK && p->lds_bytes > 0) {
+    hipError_t e = (p->prec == 64)
+        ? hipFuncSetAttribute((const void *)k_ens_vcycle_lds<double>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_bytes)
+        : hipFuncSetAttribute((const void *)k_ens_vcycle_lds<float>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_bytes);
+    if (e != hipSuccess) {
+      pfhip_set_error("hipFuncSetAttribute for the ensemble's LDS V-cycle: %s",
+                      hipGetErrorString(e));
+      rc = PFHIP_ERR_HIP;
+    }
   }
   // members in decreasing size, so the largest segments start first
   for (int a = 1; a < n; a++) {
@@ -2055,7 +2379,11 @@ static int ens_sequence(pfhip_ens_mgplan *p, int apply, hipStream_t s)
     ens_phase<T>(p, l, PS_COARSE, EL_RESTRICT, 0, 0, s);
     LAUNCHCHK();
   }
-  hipLaunchKernelGGL(k_ens_vcycle<T>, dim3(p->n), dim3(CWG), 0, s, dM, p->dlist);
+  if (p->lds_bytes > 0)
+    hipLaunchKernelGGL(k_ens_vcycle_lds<T>, dim3(p->n), dim3(CWG), (unsigned)p->lds_bytes, s,
+                       dM, p->dlist);
+  else
+    hipLaunchKernelGGL(k_ens_vcycle<T>, dim3(p->n), dim3(CWG), 0, s, dM, p->dlist);
   LAUNCHCHK();
   p->launches++;
   for (int l = p->G - 1; l >= 0; l--) {

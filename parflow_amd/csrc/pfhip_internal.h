@@ -38,6 +38,8 @@ struct pfhip_problem {
 
 void pfhip_set_error(const char *fmt, ...);
 void pfhip_prof_mark(int kernel_id, int end, void *stream);
+/* nonzero while the per-launch timer of kernel_id is on */
+int pfhip_prof_enabled(int kernel_id);
 
 /* ---- extra vector ops used by the nonlinear driver ------------------ */
 int pfhip_vec_scale(pfhip_problem *pb, double c, const double *x, double *z,

@@ -595,6 +595,79 @@ def test_multigrid_grid_levels(gpu, precision, fuse, shape, monkeypatch):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("variant", [0, 7])
+def test_multigrid_graph_replay(gpu, precision, variant, monkeypatch):
+    """The V-cycle replayed from its captured hipGraph against the same
+    hierarchy launching directly (PFHIP_MG_GRAPH=0): six V-cycles on new
+    right-hand sides, the direct hierarchy's V-cycle and a host round trip
+    between the replays, every replay identical to the last bit; and the
+    oracle bar.  The tile shape is forced, so both hierarchies run the same
+    kernels."""
+    api = _api()
+    monkeypatch.setenv("PFHIP_RBGS_VARIANT", str(variant))
+    prob = cases.hetero(70, 40, 24, seed=41, dir_sides=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p = cases.random_pressure(prob, seed=42)
+    J = O.jacobian_eval(op, p, 0.1, 0.1, 1)
+    mgo = O.MG(prob, 2, 2, 0)
+    mgo.setup(J)
+    mgg = api.GPUMultigrid(gp, 2, 2, 0, precision)
+    monkeypatch.setenv("PFHIP_MG_GRAPH", "0")
+    mgd = api.GPUMultigrid(gp, 2, 2, 0, precision)
+    monkeypatch.delenv("PFHIP_MG_GRAPH")
+    mgg.setup(J)
+    mgd.setup(J)
+    rng = np.random.default_rng(43)
+    tol = 1e-11 if precision == 64 else 2e-5
+    for rep in range(6):
+        b = np.zeros(prob.shape)
+        b[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+        xg = mgg.apply(b)
+        xd = mgd.apply(b)
+        assert np.array_equal(_I(prob, xg), _I(prob, xd)), (rep, _relmax(_I(prob, xg),
+                                                                         _I(prob, xd)))
+        assert _relmax(_I(prob, xg), _I(prob, mgo.apply(b))) < tol, rep
+
+
+@pytest.mark.parametrize("shape", [(70, 40, 24), (64, 64, 40), (9, 7, 30)])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_multigrid_bottom_in_lds(gpu, precision, shape, monkeypatch):
+    """The bottom of the V-cycle with every level's arrays in LDS (the
+    default where they fit) against the oracle and against the bottom in
+    global memory (PFHIP_MG_LDS=0): the same per-point arithmetic in the
+    same order, so the two agree to the last bit; three sweep schedules."""
+    api = _api()
+    prob = cases.hetero(*shape, seed=31, dir_sides=True)
+    print(f"mg bottom in LDS precision={precision} shape={shape}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p = cases.random_pressure(prob, seed=32)
+    J = O.jacobian_eval(op, p, 0.1, 0.1, 1)
+    rng = np.random.default_rng(33)
+    for pre, post, sym in ((1, 1, 1), (2, 2, 0), (3, 1, 0)):
+        mgo = O.MG(prob, pre, post, sym)
+        mgo.setup(J)
+        mgl = api.GPUMultigrid(gp, pre, post, sym, precision)
+        monkeypatch.setenv("PFHIP_MG_LDS", "0")
+        mgg = api.GPUMultigrid(gp, pre, post, sym, precision)
+        monkeypatch.delenv("PFHIP_MG_LDS")
+        mgl.setup(J)
+        mgg.setup(J)
+        for rep in range(2):
+            b = np.zeros(prob.shape)
+            b[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+            xo = mgo.apply(b)
+            xl = mgl.apply(b)
+            xg = mgg.apply(b)
+            tol = 1e-11 if precision == 64 else 2e-5
+            err = _relmax(_I(prob, xl), _I(prob, xo))
+            assert err < tol, (pre, post, sym, rep, err)
+            assert np.array_equal(_I(prob, xl), _I(prob, xg)), (
+                pre, post, sym, rep, _relmax(_I(prob, xl), _I(prob, xg)))
+
+
+@pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("shape", [(70, 40, 24), (131, 20, 18)])
 @pytest.mark.parametrize("variant", list(range(10)))
 def test_multigrid_pair_shapes(gpu, variant, shape, precision, monkeypatch):
