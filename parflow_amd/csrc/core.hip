@@ -1766,6 +1766,9 @@ __device__ __forceinline__ double flux_z(double pa, double pb, double ka, double
 // F of one interior cell: accumulation + storage + sources (:394-540), the
 // interior face fluxes (own faces out, incoming faces in, summed in the
 // oracle's order) and the boundary-face terms (:975-2260).
+// XY = false: the caller passes ul = 0 at i = 0 and ud = 0 at j = 0 (adding
+// -dt * 0 leaves fv's bits unchanged), so the x and y tests are left out
+template <bool XY = true>
 __device__ __forceinline__ double residual_cell(const ResArgs &A, int i, int j, int k, int ip,
                                                 double pc, double krc, double S,
                                                 double kxc, double kyc, double kzc,
@@ -1782,8 +1785,8 @@ __device__ __forceinline__ double residual_cell(const ResArgs &A, int i, int j, 
   fv += ssv * vol * (pc * S * rho - pold * so * rho);
   if (A.src) fv -= vol * dt * (A.src[ip]);
   fv += dt * (ur + uf + uu);
-  if (i > 0) fv += -dt * ul;
-  if (j > 0) fv += -dt * ud;
+  if (!XY || i > 0) fv += -dt * ul;
+  if (!XY || j > 0) fv += -dt * ud;
   if (k > 0) fv += -dt * ub;
   return fv;
 }
@@ -1971,8 +1974,8 @@ __device__ __forceinline__ void load_oth(OthIn &o, const ResArgs &A, int ip, int
                                          const double *__restrict__ Kh, int hoff)
 {
   if constexpr (!ISO) { o.kx = ldb(A.kx, ip); o.ky = ldb(A.ky, ip); }
-  o.kxe = ldb(A.kx, ip + 1);
-  o.kyn = ldb(A.ky, ip + sy);
+  o.kxe = ldb(ISO ? A.kz : A.kx, ip + 1);
+  o.kyn = ldb(ISO ? A.kz : A.ky, ip + sy);
   o.hk0 = ldb(Kh, hip); o.hk1 = ldb(Kh, hip + hoff);
   o.so = ldb(A.s_old, ip); o.po = ldb(A.poro, ip); o.ss = ldb(A.ss, ip);
   o.pold = ldb(A.p_old, ip);
@@ -2027,9 +2030,10 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
   const int hcol = hact ? (bx * TX + hx) + (by * TY + hy) * sy : col;
   // west halo (hx == 0) supplies the x face into the tile, south halo
   // (hy == 0) the y face into the tile
-  const bool hwest = hact && hx == 0;
   const bool hsouth = hact && hy == 0;
-  const double *__restrict__ Kh = hsouth ? A.ky : A.kx;
+  // west / south halo threads, also where the halo cell is outside the domain
+  const bool hw0 = tid < HALO && hx == 0, hs0 = tid < HALO && hy == 0;
+  const double *__restrict__ Kh = ISO ? A.kz : hsouth ? A.ky : A.kx;
   const int hoff = hsouth ? sy : 1;
   const int k0 = bz * ZC;
   const int k1 = min(k0 + ZC, nz);
@@ -2097,19 +2101,25 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
     if (k + 1 < nz) uu = flux_z_i(p_c, p_n, kz_c, kz_n, kr_c, kr_n, ffz, idz, hrg, imu);
     s_ux[ty * (TX + 1) + tx + 1] = ur;
     s_uy[(ty + 1) * TX + tx] = uf;
-    if (hwest)
-      s_ux[(hy - 1) * (TX + 1)] = flux_xy_i(s_p[hls], s_p[hls + 1], o.hk0, o.hk1, s_kr[hls],
-                                            s_kr[hls + 1], ffx, idx, grav, imu);
-    if (hsouth)
-      s_uy[hx - 1] = flux_xy_i(s_p[hls], s_p[hls + W2], o.hk0, o.hk1, s_kr[hls], s_kr[hls + W2],
-                               ffy, idy, grav, imu);
+    // the west / south halo slots hold 0 where the tile is on the domain's
+    // west / south side (no face there)
+    if (hw0) {
+      const double u = flux_xy_i(s_p[hls], s_p[hls + 1], o.hk0, o.hk1, s_kr[hls],
+                                 s_kr[hls + 1], ffx, idx, grav, imu);
+      s_ux[(hy - 1) * (TX + 1)] = hact ? u : 0.0;
+    }
+    if (hs0) {
+      const double u = flux_xy_i(s_p[hls], s_p[hls + W2], o.hk0, o.hk1, s_kr[hls],
+                                 s_kr[hls + W2], ffy, idy, grav, imu);
+      s_uy[hx - 1] = hact ? u : 0.0;
+    }
     lds_barrier();
     // ---- C
     if (act) {
       double ul = s_ux[ty * (TX + 1) + tx];
       double ud = s_uy[ty * TX + tx];
-      double fv = residual_cell(A, i, j, k, ip, p_c, kr_c, S_c, o.kx, o.ky, kz_c, ur, uf, uu,
-                                ul, ud, ub, o.so, o.po, o.ss, o.pold);   // + k_residual_bc
+      double fv = residual_cell<false>(A, i, j, k, ip, p_c, kr_c, S_c, o.kx, o.ky, kz_c, ur, uf,
+                                       uu, ul, ud, ub, o.so, o.po, o.ss, o.pold);
       if constexpr (MODE == 1) {
         // boundary cells get the transform after their face terms
         const bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |

@@ -571,6 +571,35 @@ template <> struct V2T<double> { typedef double type __attribute__((ext_vector_t
 template <typename T> __device__ __forceinline__ T from_prev(T v) { return __shfl_up(v, 1, 64); }
 template <typename T> __device__ __forceinline__ T from_next(T v) { return __shfl_down(v, 1, 64); }
 
+// element ip of an array through a 32-bit byte offset from its base (a
+// level array is below 4 GiB: problem creation bounds the finest grid), so
+// the access takes the base from scalar registers and one 32-bit vector
+// offset instead of a 64-bit address per lane
+template <typename V, typename T>
+__device__ __forceinline__ V ldo(const T *__restrict__ base, unsigned ip)
+{
+  return *(const V *)((const char *)base + ip * (unsigned)sizeof(T));
+}
+template <typename V, typename T>
+__device__ __forceinline__ void sto(T *__restrict__ base, unsigned ip, V v)
+{
+  *(V *)((char *)base + ip * (unsigned)sizeof(T)) = v;
+}
+// the same for device-memory arrays whose address the kernel reads from
+// memory (interface vectors of a replayed launch sequence): global-segment
+// accesses instead of flat ones
+template <typename T>
+__device__ __forceinline__ T ldg_o(const T *base, unsigned ip)
+{
+  return *(const __attribute__((address_space(1))) T *)((const char *)base +
+                                                         ip * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void stg_o(T *base, unsigned ip, T v)
+{
+  *(__attribute__((address_space(1))) T *)((char *)base + ip * (unsigned)sizeof(T)) = v;
+}
+
 template <typename T> struct PairSet {
   typedef typename V2T<T>::type T2;
   T2 c, bb, ae, an, as, au, xs, xn;
@@ -583,19 +612,19 @@ __device__ __forceinline__ void load_pair(PairSet<T> &S, const Mat<T> &A,
                                           const double *__restrict__ din = nullptr)
 {
   typedef typename V2T<T>::type T2;
-  S.c = *(const T2 *)(A.c + ip);
-  if (DIN) S.bb = T2{(T)din[ip], (T)din[ip + 1]};
-  else S.bb = *(const T2 *)(b + ip);
-  S.ae = *(const T2 *)(A.u[0] + ip);
-  S.an = *(const T2 *)(A.u[1] + ip);
-  S.as = *(const T2 *)(A.u[1] + ip - sy);
-  S.au = *(const T2 *)(A.u[2] + ip);
+  S.c = ldo<T2>(A.c, ip);
+  if (DIN) S.bb = T2{(T)ldg_o(din, ip), (T)ldg_o(din, ip + 1)};
+  else S.bb = ldo<T2>(b, ip);
+  S.ae = ldo<T2>(A.u[0], ip);
+  S.an = ldo<T2>(A.u[1], ip);
+  S.as = ldo<T2>(A.u[1], ip - sy);
+  S.au = ldo<T2>(A.u[2], ip);
   if (INIT) {
     S.xs = T2{T(0), T(0)};
     S.xn = T2{T(0), T(0)};
   } else {
-    S.xs = *(const T2 *)(xin + ip - sy);
-    S.xn = *(const T2 *)(xin + ip + sy);
+    S.xs = ldo<T2>(xin, ip - sy);
+    S.xn = ldo<T2>(xin, ip + sy);
   }
 }
 
@@ -604,7 +633,7 @@ __device__ __forceinline__ typename V2T<T>::type load_x2(const T *__restrict__ x
 {
   typedef typename V2T<T>::type T2;
   if (INIT) return T2{T(0), T(0)};
-  return *(const T2 *)(xin + ip);
+  return ldo<T2>(xin, ip);
 }
 
 // the YL form's plane loads: centre, b, east and upper couplings always;
@@ -619,19 +648,19 @@ __device__ __forceinline__ void load_pair_yl(PairSet<T> &S, const Mat<T> &A,
                                              const double *__restrict__ din = nullptr)
 {
   typedef typename V2T<T>::type T2;
-  S.c = *(const T2 *)(A.c + ip);
-  if (DIN) S.bb = T2{(T)din[ip], (T)din[ip + 1]};
-  else S.bb = *(const T2 *)(b + ip);
-  S.ae = *(const T2 *)(A.u[0] + ip);
-  S.au = *(const T2 *)(A.u[2] + ip);
+  S.c = ldo<T2>(A.c, ip);
+  if (DIN) S.bb = T2{(T)ldg_o(din, ip), (T)ldg_o(din, ip + 1)};
+  else S.bb = ldo<T2>(b, ip);
+  S.ae = ldo<T2>(A.u[0], ip);
+  S.au = ldo<T2>(A.u[2], ip);
   S.xs = T2{T(0), T(0)};
   S.xn = T2{T(0), T(0)};
   S.as = T2{T(0), T(0)};
   if (own_s) {
-    S.as = *(const T2 *)(A.u[1] + ip - sy);
-    if (!INIT) S.xs = *(const T2 *)(xin + ip - sy);
+    S.as = ldo<T2>(A.u[1], ip - sy);
+    if (!INIT) S.xs = ldo<T2>(xin, ip - sy);
   }
-  if (own_n && !INIT) S.xn = *(const T2 *)(xin + ip + sy);
+  if (own_n && !INIT) S.xn = ldo<T2>(xin, ip + sy);
 }
 
 // FP32: ask for 8 waves per SIMD (<= 64 VGPRs), so two 1024-thread tiles fit
@@ -712,12 +741,12 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
   T2 an1 = T2{T(0), T(0)}, an2 = T2{T(0), T(0)};
   if (YL) {
     load_pair_yl<T, INIT, DIN>(cur, A, xin, b, at(k0), sy, own_s, own_n, din);
-    cur.an = *(const T2 *)(A.u[1] + at(k0));
-    an1 = *(const T2 *)(A.u[1] + at(k0 + 1));
+    cur.an = ldo<T2>(A.u[1], at(k0));
+    an1 = ldo<T2>(A.u[1], at(k0 + 1));
   } else {
     load_pair<T, INIT, DIN>(cur, A, xin, b, at(k0), sy, din);
   }
-  T2 al = *(const T2 *)(A.u[2] + at(k0 - 1));        // lower coupling of plane k0
+  T2 al = ldo<T2>(A.u[2], at(k0 - 1));               // lower coupling of plane k0
   T2 xm = load_x2<T, INIT>(xin, at(k0 - 1));
   T2 x0 = load_x2<T, INIT>(xin, at(k0));
   T2 xp = load_x2<T, INIT>(xin, at(k0 + 1));
@@ -734,12 +763,12 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
     // later)
     if (YL) {
       load_pair_yl<T, INIT, DIN>(nxt, A, xin, b, at(k + 1), sy, own_s, own_n, din);
-      an2 = *(const T2 *)(A.u[1] + at(k + 2));
+      an2 = ldo<T2>(A.u[1], at(k + 2));
     } else {
       load_pair<T, INIT, DIN>(nxt, A, xin, b, at(k + 1), sy, din);
     }
     const T2 xpp = load_x2<T, INIT>(xin, at(k + 2));
-    if (DIN && owned && k >= kb && k < ke) *(T2 *)(bout + ip) = cur.bb;
+    if (DIN && owned && k >= kb && k < ke) sto<T2>(bout, ip, cur.bb);
     if (YL) {
       // plane k+1's north couplings and x_in for the rows beside
       syn[(k + 1) & 1][ty][tx] = an1;
@@ -791,10 +820,10 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
       if (owned && kk >= kb && kk < ke) {
         T2 out = p0red ? T2{vb, rm1} : T2{rm1, vb};
         if (DOUT) {
-          dout[ip - sz] = (double)out.x;
-          dout[ip - sz + 1] = (double)out.y;
+          stg_o(dout, ip - sz, (double)out.x);
+          stg_o(dout, ip - sz + 1, (double)out.y);
         } else {
-          *(T2 *)(xout + ip - sz) = out;
+          sto<T2>(xout, ip - sz, out);
         }
       }
     }
