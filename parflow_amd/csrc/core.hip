@@ -902,14 +902,29 @@ struct VecPtrs {
   const double *v[MD_CHUNK];
 };
 
-// h[j] = V[j].w for j < m (m <= MD_CHUNK), one pass over w
-__global__ __launch_bounds__(256) void k_multi_dot(Geom g, VecPtrs V, int m,
+// The multi-vector kernels take the vector count M as a template argument:
+// with a run-time count every load sat behind its own branch and waited for
+// the one before it (one load in flight per wave); with M fixed the M loads
+// of a cell are issued together.  md_launch picks the instantiation.
+#define MD_CASE(n, ...) case n: __VA_ARGS__(n); break;
+#define MD_SWITCH(m, L)                                                         \
+  switch (m) {                                                                  \
+    MD_CASE(1, L) MD_CASE(2, L) MD_CASE(3, L) MD_CASE(4, L) MD_CASE(5, L)       \
+    MD_CASE(6, L) MD_CASE(7, L) MD_CASE(8, L) MD_CASE(9, L) MD_CASE(10, L)      \
+    MD_CASE(11, L) MD_CASE(12, L) MD_CASE(13, L) MD_CASE(14, L) MD_CASE(15, L)  \
+    MD_CASE(16, L)                                                              \
+    default: break;                                                             \
+  }
+
+// h[j] = V[j].w for j < M (M <= MD_CHUNK), one pass over w
+template <int M>
+__global__ __launch_bounds__(256) void k_multi_dot(Geom g, VecPtrs V,
                                                    const double *__restrict__ w,
                                                    double *__restrict__ partial, int nb)
 {
-  double acc[MD_CHUNK];
+  double acc[M];
 #pragma unroll
-  for (int q = 0; q < MD_CHUNK; q++) acc[q] = 0.0;
+  for (int q = 0; q < M; q++) acc[q] = 0.0;
   int rows = g.ny * g.nz;
   for (int r = blockIdx.x; r < rows; r += gridDim.x) {
     int j = r % g.ny, k = r / g.ny;
@@ -917,18 +932,26 @@ __global__ __launch_bounds__(256) void k_multi_dot(Geom g, VecPtrs V, int m,
     for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
       int ip = base + i;
       double wv = w[ip];
+      double v[M];
 #pragma unroll
-      for (int q = 0; q < MD_CHUNK; q++)
-        if (q < m) acc[q] += V.v[q][ip] * wv;
+      for (int q = 0; q < M; q++) v[q] = V.v[q][ip];
+#pragma unroll
+      for (int q = 0; q < M; q++) acc[q] += v[q] * wv;
     }
   }
 #pragma unroll
-  for (int q = 0; q < MD_CHUNK; q++) {
-    if (q < m) {
-      double s = block_reduce<false>(acc[q]);
-      if (threadIdx.x == 0) partial[q * nb + blockIdx.x] = s;
-    }
+  for (int q = 0; q < M; q++) {
+    double s = block_reduce<false>(acc[q]);
+    if (threadIdx.x == 0) partial[q * nb + blockIdx.x] = s;
   }
+}
+
+static void launch_multi_dot(int mc, int nb, hipStream_t st, const Geom &g, const VecPtrs &P,
+                             const double *w, double *partial)
+{
+#define L_MD(n) hipLaunchKernelGGL(k_multi_dot<n>, dim3(nb), dim3(256), 0, st, g, P, w, partial, nb)
+  MD_SWITCH(mc, L_MD)
+#undef L_MD
 }
 
 extern "C" int pfhip_vec_multi_dot(pfhip_problem *pb, const double *const *V,
@@ -941,8 +964,7 @@ extern "C" int pfhip_vec_multi_dot(pfhip_problem *pb, const double *const *V,
     int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
     VecPtrs P;
     for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < mc) ? V[c0 + q] : w;
-    hipLaunchKernelGGL(k_multi_dot, dim3(nb), dim3(256), 0, STREAM(stream), g, P,
-                       mc, w, pb->red_partial + (long)c0 * nb, nb);
+    launch_multi_dot(mc, nb, STREAM(stream), g, P, w, pb->red_partial + (long)c0 * nb);
     LAUNCHCHK();
   }
   hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, STREAM(stream),
@@ -958,9 +980,9 @@ struct Coefs {
   double c[MD_CHUNK];
 };
 
-// w += sum_q c[q] V[q]; optional ||w||^2 partials
-template <bool NORM>
-__global__ __launch_bounds__(256) void k_multi_axpy(Geom g, VecPtrs V, Coefs C, int m,
+// w += sum_q c[q] V[q] (q < M); optional ||w||^2 partials
+template <int M, bool NORM>
+__global__ __launch_bounds__(256) void k_multi_axpy(Geom g, VecPtrs V, Coefs C,
                                                     double *__restrict__ w,
                                                     double *__restrict__ partial, int nb)
 {
@@ -971,10 +993,12 @@ __global__ __launch_bounds__(256) void k_multi_axpy(Geom g, VecPtrs V, Coefs C, 
     int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
     for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
       int ip = base + i;
+      double v[M];
+#pragma unroll
+      for (int q = 0; q < M; q++) v[q] = V.v[q][ip];
       double s = w[ip];
 #pragma unroll
-      for (int q = 0; q < MD_CHUNK; q++)
-        if (q < m) s += C.c[q] * V.v[q][ip];
+      for (int q = 0; q < M; q++) s += C.c[q] * v[q];
       w[ip] = s;
       if (NORM) acc += s * s;
     }
@@ -1000,12 +1024,15 @@ static int multi_axpy_impl(pfhip_problem *pb, const double *const *V, int m,
       C.c[q] = (q < mc) ? sign * coef[c0 + q] : 0.0;
     }
     bool last = (c0 + MD_CHUNK >= m);
+    hipStream_t st = STREAM(stream);
     if (last && norm2_out) {
-      hipLaunchKernelGGL(k_multi_axpy<true>, dim3(nb), dim3(256), 0, STREAM(stream),
-                         g, P, C, mc, w, pb->red_partial, nb);
+#define L_MA(n) hipLaunchKernelGGL((k_multi_axpy<n, true>), dim3(nb), dim3(256), 0, st, g, P, C, w, pb->red_partial, nb)
+      MD_SWITCH(mc, L_MA)
+#undef L_MA
     } else {
-      hipLaunchKernelGGL(k_multi_axpy<false>, dim3(nb), dim3(256), 0, STREAM(stream),
-                         g, P, C, mc, w, pb->red_partial, nb);
+#define L_MA(n) hipLaunchKernelGGL((k_multi_axpy<n, false>), dim3(nb), dim3(256), 0, st, g, P, C, w, pb->red_partial, nb)
+      MD_SWITCH(mc, L_MA)
+#undef L_MA
     }
     LAUNCHCHK();
   }
@@ -1021,17 +1048,17 @@ static int multi_axpy_impl(pfhip_problem *pb, const double *const *V, int m,
   return PFHIP_OK;
 }
 
-// w -= sum_q h[q] V[q] with h read from device memory (the coefficients the
-// previous multi-dot left there); optional ||w||^2 partials
-template <bool NORM>
+// w -= sum_q h[q] V[q] (q < M) with h read from device memory (the
+// coefficients the previous multi-dot left there); optional ||w||^2 partials
+template <int M, bool NORM>
 __global__ __launch_bounds__(256) void k_multi_axpy_d(Geom g, VecPtrs V,
-                                                      const double *__restrict__ h, int m,
+                                                      const double *__restrict__ h,
                                                       double *__restrict__ w,
                                                       double *__restrict__ partial, int nb)
 {
-  double c[MD_CHUNK];
+  double c[M];
 #pragma unroll
-  for (int q = 0; q < MD_CHUNK; q++) c[q] = (q < m) ? -1.0 * h[q] : 0.0;
+  for (int q = 0; q < M; q++) c[q] = -1.0 * h[q];
   double acc = 0.0;
   int rows = g.ny * g.nz;
   for (int r = blockIdx.x; r < rows; r += gridDim.x) {
@@ -1039,10 +1066,12 @@ __global__ __launch_bounds__(256) void k_multi_axpy_d(Geom g, VecPtrs V,
     int base = 1 + (j + 1) * g.sy + (k + 1) * g.sz;
     for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
       int ip = base + i;
+      double v[M];
+#pragma unroll
+      for (int q = 0; q < M; q++) v[q] = V.v[q][ip];
       double s = w[ip];
 #pragma unroll
-      for (int q = 0; q < MD_CHUNK; q++)
-        if (q < m) s += c[q] * V.v[q][ip];
+      for (int q = 0; q < M; q++) s += c[q] * v[q];
       w[ip] = s;
       if (NORM) acc += s * s;
     }
@@ -1058,13 +1087,13 @@ __global__ __launch_bounds__(256) void k_multi_axpy_d(Geom g, VecPtrs V,
 // followed by k_multi_dot -- the same products and sums, one read of V
 template <int M>
 __global__ __launch_bounds__(256) void k_multi_axpy_dot(Geom g, VecPtrs V,
-                                                        const double *__restrict__ h, int m,
+                                                        const double *__restrict__ h,
                                                         double *__restrict__ w,
                                                         double *__restrict__ partial, int nb)
 {
   double c[M], acc[M];
 #pragma unroll
-  for (int q = 0; q < M; q++) { c[q] = (q < m) ? -1.0 * h[q] : 0.0; acc[q] = 0.0; }
+  for (int q = 0; q < M; q++) { c[q] = -1.0 * h[q]; acc[q] = 0.0; }
   int rows = g.ny * g.nz;
   for (int r = blockIdx.x; r < rows; r += gridDim.x) {
     int j = r % g.ny, k = r / g.ny;
@@ -1073,23 +1102,19 @@ __global__ __launch_bounds__(256) void k_multi_axpy_dot(Geom g, VecPtrs V,
       int ip = base + i;
       double v[M];
 #pragma unroll
-      for (int q = 0; q < M; q++) v[q] = (q < m) ? V.v[q][ip] : 0.0;
+      for (int q = 0; q < M; q++) v[q] = V.v[q][ip];
       double s = w[ip];
 #pragma unroll
-      for (int q = 0; q < M; q++)
-        if (q < m) s += c[q] * v[q];
+      for (int q = 0; q < M; q++) s += c[q] * v[q];
       w[ip] = s;
 #pragma unroll
-      for (int q = 0; q < M; q++)
-        if (q < m) acc[q] += v[q] * s;
+      for (int q = 0; q < M; q++) acc[q] += v[q] * s;
     }
   }
 #pragma unroll
   for (int q = 0; q < M; q++) {
-    if (q < m) {
-      double t = block_reduce<false>(acc[q]);
-      if (threadIdx.x == 0) partial[q * nb + blockIdx.x] = t;
-    }
+    double t = block_reduce<false>(acc[q]);
+    if (threadIdx.x == 0) partial[q * nb + blockIdx.x] = t;
   }
 }
 
@@ -1101,8 +1126,7 @@ static void multi_dot_dev(pfhip_problem *pb, const Geom &g, int nb, const double
     int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
     VecPtrs P;
     for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < mc) ? V[c0 + q] : w;
-    hipLaunchKernelGGL(k_multi_dot, dim3(nb), dim3(256), 0, st, g, P, mc, w,
-                       pb->red_partial + (long)c0 * nb, nb);
+    launch_multi_dot(mc, nb, st, g, P, w, pb->red_partial + (long)c0 * nb);
   }
   hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, st, pb->red_partial, nb, m,
                      dst);
@@ -1118,12 +1142,15 @@ static void multi_axpy_dev(pfhip_problem *pb, const Geom &g, int nb, const doubl
     VecPtrs P;
     for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < mc) ? V[c0 + q] : w;
     bool last = (c0 + MD_CHUNK >= m);
-    if (last && n2dst)
-      hipLaunchKernelGGL(k_multi_axpy_d<true>, dim3(nb), dim3(256), 0, st, g, P, h + c0, mc,
-                         w, pb->red_partial, nb);
-    else
-      hipLaunchKernelGGL(k_multi_axpy_d<false>, dim3(nb), dim3(256), 0, st, g, P, h + c0, mc,
-                         w, pb->red_partial, nb);
+    if (last && n2dst) {
+#define L_MAD(n) hipLaunchKernelGGL((k_multi_axpy_d<n, true>), dim3(nb), dim3(256), 0, st, g, P, h + c0, w, pb->red_partial, nb)
+      MD_SWITCH(mc, L_MAD)
+#undef L_MAD
+    } else {
+#define L_MAD(n) hipLaunchKernelGGL((k_multi_axpy_d<n, false>), dim3(nb), dim3(256), 0, st, g, P, h + c0, w, pb->red_partial, nb)
+      MD_SWITCH(mc, L_MAD)
+#undef L_MAD
+    }
   }
   if (n2dst)
     hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, st, pb->red_partial, nb, 1,
@@ -1151,12 +1178,13 @@ int pfhip_vec_cgs2(pfhip_problem *pb, const double *const *V, int k, double *w, 
     // w -= V h1 and h2 = V^T w in one pass over V
     VecPtrs P;
     for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < k) ? V[q] : w;
-    if (k <= 4)
-      hipLaunchKernelGGL(k_multi_axpy_dot<4>, dim3(nb), dim3(256), 0, st, g, P, d1, k, w,
-                         pb->red_partial, nb);
-    else
-      hipLaunchKernelGGL(k_multi_axpy_dot<8>, dim3(nb), dim3(256), 0, st, g, P, d1, k, w,
-                         pb->red_partial, nb);
+#define L_MADOT(n) hipLaunchKernelGGL(k_multi_axpy_dot<n>, dim3(nb), dim3(256), 0, st, g, P, d1, w, pb->red_partial, nb)
+    switch (k) {
+      MD_CASE(1, L_MADOT) MD_CASE(2, L_MADOT) MD_CASE(3, L_MADOT) MD_CASE(4, L_MADOT)
+      MD_CASE(5, L_MADOT) MD_CASE(6, L_MADOT) MD_CASE(7, L_MADOT) MD_CASE(8, L_MADOT)
+      default: break;
+    }
+#undef L_MADOT
     hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, st, pb->red_partial, nb,
                        k, d2);
   } else {
