@@ -1722,6 +1722,32 @@ extern "C" int pfhip_overland_flow_eval_kin(pfhip_problem *pb, const double *p,
 // (PL/vector.c:321), so a domain face contributes only its BC "u_new" term
 // (:975-2260), which is what the boundary branch computes.
 // --------------------------------------------------------------------------
+// Tile order of the residual and Jacobian launches.  Workgroups are dealt
+// to the 8 XCDs round-robin, so in launch order the x-neighbouring tiles,
+// which share the cache lines at their x edges (a tile row of 64 FP64
+// values straddles five 128-byte lines in the ghosted layout), would land
+// on different XCDs with separate L2s.  Here each run of 8 R gx launch
+// slots gives every XCD R whole tile rows (all gx tiles of a row): the
+// shared lines are then fetched once per XCD.  Tails that do not fill a
+// run keep the launch order.  Measured on 16.8M cells: F -2.5 %, the
+// line-search form -2.7 %, the Jacobian -4 %, but the J.v form +2.5 %, which
+// therefore keeps the launch order (REMAP = false).
+#define TILE_ROWS_PER_XCD 4
+template <bool REMAP = true>
+__device__ __forceinline__ void tile_of_block(int &bx, int &by, int &bz)
+{
+  if (!REMAP) { bx = blockIdx.x; by = blockIdx.y; bz = blockIdx.z; return; }
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int N = gx * gy * gridDim.z, G = 8 * TILE_ROWS_PER_XCD * gx;
+  const int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int q = b / G, r = b - q * G;
+  const int t = ((q + 1) * G <= N) ? q * G + (r & 7) * TILE_ROWS_PER_XCD * gx + (r >> 3) : b;
+  bx = t % gx;
+  const int row = t / gx;
+  by = row % gy;
+  bz = row / gy;
+}
+
 struct ResArgs {
   Geom g;
   Phys f;
@@ -2035,7 +2061,8 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
   const double idx = A.idx, idy = A.idy, idz = A.idz;
   const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  int bx, by, bz;
+  tile_of_block<MODE != 1>(bx, by, bz);
   const int i = bx * TX + tx, j = by * TY + ty;
   const bool act = (i < g.nx) && (j < g.ny);
   const int nz = g.nz, sy = g.sy, sz = g.sz;
@@ -2870,7 +2897,8 @@ void k_jacobian_pf(JacPFArgs P, JOut<OT> J)
   const double ffx = dy * dz, ffy = dx * dz, ffz = dx * dy;
   const double ddp = 0.0;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  int bx, by, bz;
+  tile_of_block(bx, by, bz);
   const int i = bx * TX + tx, j = by * TY + ty;
   const bool act = (i < g.nx) && (j < g.ny);
   const int nz = g.nz, sy = g.sy, sz = g.sz;
