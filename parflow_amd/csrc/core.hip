@@ -244,6 +244,9 @@ static inline dim3 grid3(const Geom &g)
 // --------------------------------------------------------------------------
 // problem
 // --------------------------------------------------------------------------
+static long res_tiles(const Geom &g);
+static long res_bc_blocks(const Geom &g);
+
 static int face_ncells(const pfhip_problem_desc *d, int f)
 {
   if (f < 2) return d->ny * d->nz;
@@ -381,6 +384,11 @@ extern "C" int pfhip_problem_create(const pfhip_problem_desc *desc,
   if (rc == PFHIP_OK) rc = pfhip_dev_alloc(&pb->red_partial, PFHIP_RED_MAX_BLOCKS * PFHIP_MAX_MULTI);
   if (rc == PFHIP_OK) rc = pfhip_dev_alloc(&pb->red_dev, PFHIP_RED_SLOTS);
   if (rc == PFHIP_OK) {
+    Geom g0 = geom_of(pb);
+    pb->stat_nb = res_tiles(g0) + res_bc_blocks(g0);
+    rc = pfhip_dev_alloc(&pb->stat_partial, 2 * pb->stat_nb);
+  }
+  if (rc == PFHIP_OK) {
     hipError_t e = hipHostMalloc((void **)&pb->h_red, sizeof(double) * PFHIP_MAX_MULTI);
     if (e != hipSuccess) { pfhip_set_error("hipHostMalloc"); rc = PFHIP_ERR_NOMEM; }
   }
@@ -422,6 +430,7 @@ extern "C" int pfhip_problem_destroy(pfhip_problem *pb)
   for (int q = 0; q < 4; q++) if (pb->q2[q]) hipFree(pb->q2[q]);
   if (pb->red_partial) hipFree(pb->red_partial);
   if (pb->red_dev) hipFree(pb->red_dev);
+  if (pb->stat_partial) hipFree(pb->stat_partial);
   if (pb->h_red) hipHostFree(pb->h_red);
   if (pb->krtab) hipFree(pb->krtab);
   free(pb);
@@ -537,7 +546,7 @@ __device__ __forceinline__ double block_reduce(double v)
 }
 
 enum RedOp { RED_DOT = 0, RED_MAXABS = 1, RED_L1 = 2, RED_SCSTEP = 3, RED_DQ = 4,
-             RED_WL2 = 5, RED_NEGMAX = 6, RED_CONSTR = 7 };
+             RED_WL2 = 5, RED_NEGMAX = 6, RED_CONSTR = 7, RED_DOTSC = 8 };
 
 template <int OP>
 __global__ __launch_bounds__(256) void k_reduce(Geom g, const double *__restrict__ x,
@@ -558,7 +567,12 @@ __global__ __launch_bounds__(256) void k_reduce(Geom g, const double *__restrict
       else if (OP == RED_MAXABS) a0 = fmax(a0, fabs(x[ip]));
       else if (OP == RED_L1) a0 += fabs(x[ip]);
       else if (OP == RED_SCSTEP) a0 = fmax(a0, fabs(y[ip] / (1.0 + fabs(x[ip]))));
-      else {  // RED_DQ: u.v, v.v, |v|_1
+      else if (OP == RED_DOTSC) {
+        // y.y (RED_DOT's product with x = y) and RED_SCSTEP's maximum
+        const double pv = y[ip];
+        a0 += pv * pv;
+        a1 = fmax(a1, fabs(pv / (1.0 + fabs(x[ip]))));
+      } else {  // RED_DQ: u.v, v.v, |v|_1
         double u = x[ip], v = y[ip];
         a0 += u * v;
         a1 += v * v;
@@ -569,9 +583,11 @@ __global__ __launch_bounds__(256) void k_reduce(Geom g, const double *__restrict
   const bool MX = (OP == RED_MAXABS || OP == RED_SCSTEP || OP == RED_NEGMAX);
   a0 = block_reduce<MX>(a0);
   if (OP == RED_DQ) { a1 = block_reduce<false>(a1); a2 = block_reduce<false>(a2); }
+  if (OP == RED_DOTSC) a1 = block_reduce<true>(a1);
   if (threadIdx.x == 0) {
     partial[blockIdx.x] = a0;
     if (OP == RED_DQ) { partial[nb + blockIdx.x] = a1; partial[2 * nb + blockIdx.x] = a2; }
+    if (OP == RED_DOTSC) partial[nb + blockIdx.x] = a1;
   }
 }
 
@@ -649,6 +665,28 @@ extern "C" int pfhip_vec_scsteplength(pfhip_problem *pb, const double *u,
   int rc = run_reduce<RED_SCSTEP>(pb, u, s, 1, stream);
   if (rc) return rc;
   *result = pb->h_red[0];
+  return PFHIP_OK;
+}
+
+// p.p and max |p| / (1 + |u|) from one pass: the same values as
+// pfhip_vec_dot(p, p) and pfhip_vec_scsteplength(u, p), one synchronisation
+int pfhip_vec_dot_scsteplength(pfhip_problem *pb, const double *u, const double *p,
+                               double *pp, double *rlength, void *stream)
+{
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  hipLaunchKernelGGL(k_reduce<RED_DOTSC>, dim3(nb), dim3(256), 0, STREAM(stream), g, u, p,
+                     pb->red_partial, nb);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, STREAM(stream),
+                     pb->red_partial, nb, 1, pb->red_dev);
+  hipLaunchKernelGGL(k_reduce_final<true>, dim3(1), dim3(256), 0, STREAM(stream),
+                     pb->red_partial + nb, nb, 1, pb->red_dev + 1);
+  LAUNCHCHK();
+  int rc = fetch_results(pb, 2, stream);
+  if (rc) return rc;
+  *pp = pb->h_red[0];
+  *rlength = pb->h_red[1];
   return PFHIP_OK;
 }
 
@@ -1779,7 +1817,27 @@ struct ResArgs {
   const double *bcv[6];
   int bct[6];
   double *__restrict__ F;
+  // MODE 2 with stat set: per-block KINStop maxima (max |F|, max scaled
+  // step) into stat[blk] and stat[stat_nb + blk]; the tiled kernel's blocks
+  // first (interior cells), then the boundary kernel's from stat_main on
+  double *stat;
+  long stat_nb, stat_main;
 };
+
+// max over a workgroup of NT threads of two values; valid in thread 0
+template <int NT>
+__device__ __forceinline__ void block_max2(double &a, double &b)
+{
+  __shared__ double sh[2][NT / 64];
+  a = wave_max(a);
+  b = wave_max(b);
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  if ((t & 63) == 0) { sh[0][t >> 6] = a; sh[1][t >> 6] = b; }
+  __syncthreads();
+  if (t == 0)
+    for (int w = 1; w < NT / 64; w++) { a = fmax(a, sh[0][w]); b = fmax(b, sh[1][w]); }
+  __syncthreads();
+}
 
 __device__ __forceinline__ double hmean(double a, double b)
 {
@@ -2112,6 +2170,7 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
     kr_c *= rho;
     kz_c = v0.kz;
   }
+  double st_f = 0.0, st_s = 0.0;    // MODE 2 KINStop maxima of this thread
   // flux through the lower z face of the own cell at plane k0
   double ub = 0.0;
   if (k0 > 0) ub = flux_z_i(p_m, p_c, kz_m, kz_c, kr_m, kr_c, ffz, idz, hrg, imu);
@@ -2181,12 +2240,35 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
                          (k == nz - 1);
         fv = bnd ? fv : fsc * fv - fsc * o.fs;
       }
-      if constexpr (MODE == 2) A.pout[ip] = p_c;
+      if constexpr (MODE == 2) {
+        A.pout[ip] = p_c;
+        // KINStop's terms (KS/kinsol.c:1507-1540) of the interior cells;
+        // the boundary kernel takes the shell cells after their face terms
+        if (A.stat) {
+          const bool bnd = (i == 0) | (i == g.nx - 1) | (j == 0) | (j == g.ny - 1) | (k == 0) |
+                           (k == nz - 1);
+          if (!bnd) {
+            st_f = fmax(st_f, fabs(fv));
+            const double stp = 1.0 * p_c + (-1.0) * ldb(A.p, ip);
+            st_s = fmax(st_s, fabs(stp / (1.0 + fabs(p_c))));
+          }
+        }
+      }
       A.F[ip] = fv;
     }
     ub = uu;
     p_m = p_c; kr_m = kr_c; kz_m = kz_c;
     p_c = p_n; kr_c = kr_n; S_c = S_n; kz_c = kz_n;
+  }
+  if constexpr (MODE == 2) {
+    if (A.stat) {
+      block_max2<TX * TY>(st_f, st_s);
+      if (tid == 0) {
+        const long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        A.stat[blk] = st_f;
+        A.stat[A.stat_nb + blk] = st_s;
+      }
+    }
   }
 }
 
@@ -2237,18 +2319,43 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_residual_bc(ResArgs A, long nshell)
 {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nshell) return;
   int i, j, k;
-  if (!shell_cell(A.g, idx, i, j, k)) return;
-  const int ip = (i + 1) + (j + 1) * A.g.sy + (k + 1) * A.g.sz;
-  double psig = 0.0, fsc = 0.0;
-  if constexpr (MODE != 0) form_scalars(A, psig, fsc);
-  double pc = p_at<MODE>(A, ip, psig);
-  double krc = vg_kr(pc, A.f.irg, A.f.alpha[ip], A.f.nvg[ip], A.f.tb) * A.f.rho;
-  double fv = residual_bc(A, i, j, k, ip, A.F[ip], pc, krc, A.kx[ip], A.ky[ip], A.kz[ip]);
-  if constexpr (MODE == 1) fv = fsc * fv - fsc * A.fsub[ip];
-  A.F[ip] = fv;
+  double st_f = 0.0, st_s = 0.0;
+  if (idx < nshell && shell_cell(A.g, idx, i, j, k)) {
+    const int ip = (i + 1) + (j + 1) * A.g.sy + (k + 1) * A.g.sz;
+    double psig = 0.0, fsc = 0.0;
+    if constexpr (MODE != 0) form_scalars(A, psig, fsc);
+    double pc = p_at<MODE>(A, ip, psig);
+    double krc = vg_kr(pc, A.f.irg, A.f.alpha[ip], A.f.nvg[ip], A.f.tb) * A.f.rho;
+    double fv = residual_bc(A, i, j, k, ip, A.F[ip], pc, krc, A.kx[ip], A.ky[ip], A.kz[ip]);
+    if constexpr (MODE == 1) fv = fsc * fv - fsc * A.fsub[ip];
+    A.F[ip] = fv;
+    if constexpr (MODE == 2) {
+      if (A.stat) {
+        st_f = fabs(fv);
+        const double stp = 1.0 * pc + (-1.0) * A.p[ip];
+        st_s = fabs(stp / (1.0 + fabs(pc)));
+      }
+    }
+  }
+  if constexpr (MODE == 2) {
+    if (A.stat) {
+      block_max2<256>(st_f, st_s);
+      if (threadIdx.x == 0) {
+        A.stat[A.stat_main + blockIdx.x] = st_f;
+        A.stat[A.stat_nb + A.stat_main + blockIdx.x] = st_s;
+      }
+    }
+  }
 }
+
+// launch geometry of the tiled residual (launch_residual's 64 x 8 x 16
+// tiles) and of the boundary-shell kernel
+static long res_tiles(const Geom &g)
+{
+  return (long)((g.nx + 63) / 64) * ((g.ny + 7) / 8) * ((g.nz + 15) / 16);
+}
+static long res_bc_blocks(const Geom &g) { return (shell_count(g) + 255) / 256; }
 
 template <int TX, int TY, int ZC, int MODE>
 static void launch_pf(const ResArgs &A, hipStream_t st)
@@ -2305,6 +2412,7 @@ static ResArgs res_args(const pfhip_problem *pb, const double *p_old, const doub
   A.p = nullptr; A.F = nullptr;
   A.pv = nullptr; A.fsub = nullptr; A.pout = nullptr; A.psig = 0.0; A.fsc = 0.0;
   A.dq = nullptr; A.sqrf = 0.0;
+  A.stat = nullptr; A.stat_nb = 0; A.stat_main = 0;
   return A;
 }
 
@@ -2690,6 +2798,52 @@ int pfhip_residual_step(pfhip_problem *pb, const double *uu, const double *pp, d
   A.pv = pp; A.pout = unew; A.psig = rl;
   launch_residual(A, STREAM(stream));
   LAUNCHCHK();
+  return PFHIP_OK;
+}
+
+// The line-search point of pfhip_residual_step plus, for KINLineSearch and
+// KINStop (KS/kinsol.c:1228-1405, 1507-1580), F.F (the pfhip_vec_dot pass,
+// same partials and order) and the maxima max |F| and max |unew - u| /
+// (1 + |unew|) that pfhip_kin_stop_norms computes, gathered by the residual
+// launches themselves: one synchronisation instead of three.
+int pfhip_residual_step_stats(pfhip_problem *pb, const double *uu, const double *pp, double rl,
+                              double *unew, double *F, const double *p_old,
+                              const double *s_old, double dt, double *ff, double *fmax,
+                              double *steplength, void *stream)
+{
+  if (pb->overland) {
+    pfhip_set_error("residual_step_stats: overland problems take the two-pass form");
+    return PFHIP_ERR_ARG;
+  }
+  hipStream_t st = STREAM(stream);
+  ResArgs A = res_args(pb, p_old, s_old, dt);
+  A.p = uu;
+  A.F = F;
+  A.pv = pp; A.pout = unew; A.psig = rl;
+  A.stat = pb->stat_partial;
+  A.stat_main = res_tiles(A.g);
+  A.stat_nb = pb->stat_nb;
+  if (A.stat_main + res_bc_blocks(A.g) != A.stat_nb) {
+    pfhip_set_error("residual_step_stats: partial table of %ld, launch needs %ld", A.stat_nb,
+                    A.stat_main + res_bc_blocks(A.g));
+    return PFHIP_ERR_ARG;
+  }
+  launch_residual(A, st);
+  LAUNCHCHK();
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  hipLaunchKernelGGL(k_reduce<RED_DOT>, dim3(nb), dim3(256), 0, st, g, F, F, pb->red_partial,
+                     nb);
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, st, pb->red_partial, nb, 1,
+                     pb->red_dev);
+  hipLaunchKernelGGL(k_reduce_final<true>, dim3(1), dim3(256), 0, st, pb->stat_partial,
+                     (int)pb->stat_nb, 2, pb->red_dev + 1);
+  LAUNCHCHK();
+  int rc = fetch_results(pb, 3, stream);
+  if (rc) return rc;
+  *ff = pb->h_red[0];
+  *fmax = pb->h_red[1];
+  *steplength = pb->h_red[2];
   return PFHIP_OK;
 }
 

@@ -57,6 +57,10 @@ struct pfhip_solver {
   double mxnewtstep, sqrt_relfunc, uround, fnormtol, scsteptol, eps, pthrsh;
   int etaflag, callForcingTerm, nni, nnilpre, nfe, nbcf, nbktrk, ncscmx;
   int precondcurrent, mxiter, nli, nps, npe, ncfl, new_uu;
+  /* KINStop's max |fval| and scaled step length of the current (unew,
+   * fval), when the residual launch that produced them also gathered them */
+  int stop_valid;
+  double stop_fmax, stop_steplength;
 };
 
 #define CHK(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
@@ -67,7 +71,7 @@ const pfhip_ops pfhip_direct_ops = {
   pfhip_vec_multi_axpy, pfhip_vec_combine, pfhip_kin_stop_norms, pfhip_nl_function_eval,
   pfhip_residual_jv, pfhip_residual_step, pfhip_mg_setup_jacobian, pfhip_mg_apply,
   pfhip_bc_pressure, pfhip_saturation, pfhip_synchronize, pfhip_residual_jv_dq,
-  pfhip_vec_cgs2,
+  pfhip_vec_cgs2, pfhip_vec_dot_scsteplength, pfhip_residual_step_stats,
 };
 
 void pfhip_solver_set_ops(pfhip_solver *s, const pfhip_ops *ops) { s->ops = ops; }
@@ -532,6 +536,23 @@ static int linsoldrv(pfhip_solver *s, double *bb, double *xx, int *ret)
 
 static int eval_unew(pfhip_solver *s, double rl, double *fnormp, double *f1normp)
 {
+  s->stop_valid = 0;
+  if (!s->pb->overland && s->ops->residual_step_stats) {
+    /* unew = uu + rl pp formed inside the residual launch, which also
+     * gathers KINStop's maxima; fval.fval from the same synchronisation */
+    double ff;
+    s->nfe++;
+    PH_BEGIN(PFHIP_PROF_PH_NL_F_EVAL);
+    int rc = s->ops->residual_step_stats(s->pb, s->uu, s->pp, rl, s->unew, s->fval,
+                                         s->cur_p_old, s->cur_s_old, s->dt, &ff,
+                                         &s->stop_fmax, &s->stop_steplength, NULL);
+    PH_END(PFHIP_PROF_PH_NL_F_EVAL);
+    CHK(rc);
+    s->stop_valid = 1;
+    *fnormp = sqrt(ff);
+    *f1normp = 0.5 * (*fnormp) * (*fnormp);
+    return PFHIP_OK;
+  }
   if (!s->pb->overland) {
     /* unew = uu + rl pp formed inside the residual launch */
     s->nfe++;
@@ -559,15 +580,26 @@ static int linesearch(pfhip_solver *s, double *fnormp, double *f1normp, int *max
   double pt1trl, rllo, rlincr, alpha_cond, beta_cond = 0.0;
   *maxStep = 0;
   *gret = 0;
-  CHK(l2norm(s, s->pp, &pnorm));
+  int have_rlength = 0;
+  if (s->ops->vec_dot_scsteplength) {
+    /* ||pp|| and the scaled step length from one pass; the step length is
+     * taken again if pp is scaled down */
+    double pp2;
+    CHK(s->ops->vec_dot_scsteplength(pb, s->uu, s->pp, &pp2, &rlength, NULL));
+    pnorm = sqrt(pp2);
+    have_rlength = 1;
+  } else {
+    CHK(l2norm(s, s->pp, &pnorm));
+  }
   if (pnorm > s->mxnewtstep) {
     ratio = s->mxnewtstep / pnorm;
     CHK(s->ops->vec_scale(pb, ratio, s->pp, s->pp, NULL));
     pnorm = s->mxnewtstep;
+    have_rlength = 0;
   }
   s->stepl = pnorm;
   slpi = s->sfdotJp * ratio;
-  CHK(s->ops->vec_scsteplength(pb, s->uu, s->pp, &rlength, NULL));
+  if (!have_rlength) CHK(s->ops->vec_scsteplength(pb, s->uu, s->pp, &rlength, NULL));
   rlmin = s->scsteptol / rlength;
   rl = 1.0;
   int nfesav = s->nfe;
@@ -576,6 +608,7 @@ static int linesearch(pfhip_solver *s, double *fnormp, double *f1normp, int *max
     alpha_cond = s->f1norm + alpha * slpi * rl;
     if ((*f1normp) <= alpha_cond) break;
     if (rl < rlmin) {
+      s->stop_valid = 0;
       CHK(s->ops->vec_scale(pb, 1.0, s->uu, s->unew, NULL));
       CHK(func(s, s->unew, s->fval));
       CHK(l2norm(s, s->fval, fnormp));
@@ -680,8 +713,14 @@ static int kin_stop(pfhip_solver *s, int maxStepTaken, int gret, int *ret)
     return PFHIP_OK;
   }
   double fmax, rlength;
-  /* max|fval| and the scaled step length from one pass */
-  CHK(s->ops->kin_stop_norms(s->pb, s->fval, s->unew, s->uu, &fmax, &rlength, NULL));
+  if (s->stop_valid) {
+    /* gathered by the residual launch of the accepted point */
+    fmax = s->stop_fmax;
+    rlength = s->stop_steplength;
+  } else {
+    /* max|fval| and the scaled step length from one pass */
+    CHK(s->ops->kin_stop_norms(s->pb, s->fval, s->unew, s->uu, &fmax, &rlength, NULL));
+  }
   if (fmax <= s->fnormtol) { *ret = 1; return PFHIP_OK; }
   if (rlength <= s->scsteptol) {
     if (!s->precondcurrent) { s->pthrsh = 2.0; return PFHIP_OK; }
@@ -777,6 +816,7 @@ static int kinsol_solve(pfhip_solver *s, double *p, const double *p_old,
     double epsmin = 0.01 * s->fnormtol;
     for (;;) {
       s->nni++;
+      s->stop_valid = 0;
       s->eps = (s->eta + s->uround) * s->fnorm;
       if (s->eps < epsmin) s->eps = epsmin;
       CHK(linsoldrv(s, s->unew, s->pp, &ret));

@@ -26,6 +26,9 @@ struct pfhip_problem {
   double *red_dev;            /* device result slots (PFHIP_RED_SLOTS)    */
   double *h_red;              /* pinned host result slots (64)            */
   double *krtab;              /* device kr lookup table, or NULL          */
+  double *stat_partial;       /* KINStop partials of the line-search
+                                 residual: 2 x stat_nb                    */
+  long stat_nb;               /* residual tiles + boundary-shell blocks   */
 };
 
 #define PFHIP_RED_MAX_BLOCKS 1024
@@ -60,6 +63,10 @@ int pfhip_vec_multi_axpy(pfhip_problem *pb, const double *const *V, int m,
 /* x += sum_j y[j] V[j] */
 int pfhip_vec_combine(pfhip_problem *pb, const double *const *V, int m,
                       const double *y, double *x, void *stream);
+/* p.p and the scaled step length max |p| / (1 + |u|) in one pass (the
+ * l2norm and KINScSteplength of KINLineSearch's start) */
+int pfhip_vec_dot_scsteplength(pfhip_problem *pb, const double *u, const double *p,
+                               double *pp, double *rlength, void *stream);
 /* MGS step fused: w -= hprev * vprev (if vprev), then returns vnext . w */
 int pfhip_vec_axpy_dot(pfhip_problem *pb, double hprev, const double *vprev,
                        double *w, const double *vnext, double *dot,
@@ -76,6 +83,13 @@ int pfhip_residual_jv(pfhip_problem *pb, const double *u, const double *v, doubl
 int pfhip_residual_jv_dq(pfhip_problem *pb, const double *u, const double *v, double sqrf,
                          const double *fval, double *z, const double *p_old,
                          const double *s_old, double dt, void *stream);
+/* pfhip_residual_step plus F.F and KINStop's max |F| and scaled step
+ * length max |unew - u| / (1 + |unew|) of the same point, with one host
+ * synchronisation (no overland problems) */
+int pfhip_residual_step_stats(pfhip_problem *pb, const double *u, const double *v,
+                              double sigma, double *unew, double *F, const double *p_old,
+                              const double *s_old, double dt, double *ff, double *fmax,
+                              double *steplength, void *stream);
 /* CGS2 of w against V[0..k-1] (k <= PFHIP_SLOT_H2) with one host
  * synchronisation: h1, h2 and ||w||^2 after the second pass */
 int pfhip_vec_cgs2(pfhip_problem *pb, const double *const *V, int k, double *w, double *h1,
@@ -147,6 +161,11 @@ typedef struct pfhip_ops {
                         void *);
   int (*cgs2)(pfhip_problem *, const double *const *, int, double *, double *, double *,
               double *, void *);
+  int (*vec_dot_scsteplength)(pfhip_problem *, const double *, const double *, double *,
+                              double *, void *);
+  int (*residual_step_stats)(pfhip_problem *, const double *, const double *, double,
+                             double *, double *, const double *, const double *, double,
+                             double *, double *, double *, void *);
 } pfhip_ops;
 
 extern const pfhip_ops pfhip_direct_ops;
