@@ -2027,8 +2027,9 @@ __device__ __forceinline__ double flux_z_i(double pa, double pb, double ka, doub
 // clamped in-bounds address, and the two per-plane barriers order LDS only,
 // so the loads stay in flight while plane k is computed.  PCS: s_res/s_sat
 // are per-cell fields.
-template <bool PCS> struct VGIn { double p, al, n, kz, sr, st; };
-template <> struct VGIn<false> { double p, al, n, kz; };
+// pr: the unshifted pressure (the line-search form's u, for KINStop's step)
+template <bool PCS> struct VGIn { double p, pr, al, n, kz, sr, st; };
+template <> struct VGIn<false> { double p, pr, al, n, kz; };
 struct OthIn { double kx, kxe, ky, kyn, so, po, ss, pold, hk0, hk1, fs; };
 struct HalIn { double p, al, n; };
 
@@ -2067,7 +2068,13 @@ __device__ __forceinline__ void form_scalars(const ResArgs &A, double &psig, dou
 template <bool PCS, int MODE>
 __device__ __forceinline__ void load_vg(VGIn<PCS> &v, const ResArgs &A, int ip, double psig)
 {
-  v.p = p_at<MODE>(A, ip, psig); v.al = ldb(A.f.alpha, ip); v.n = ldb(A.f.nvg, ip);
+  if constexpr (MODE == 2) {
+    v.pr = ldb(A.p, ip);
+    v.p = v.pr + psig * ldb(A.pv, ip);        // p_at's expression
+  } else {
+    v.p = p_at<MODE>(A, ip, psig);
+  }
+  v.al = ldb(A.f.alpha, ip); v.n = ldb(A.f.nvg, ip);
   v.kz = ldb(A.kz, ip);
   if constexpr (PCS) { v.sr = ldb(A.f.sres, ip); v.st = ldb(A.f.ssat, ip); }
 }
@@ -2153,8 +2160,8 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
   auto plane = [&](int q) { return (min(q, nz - 1) + 1) * sz; };
 
   double p_m = 0, kr_m = 0, kz_m = 0;
-  double p_c = 0, kr_c = 0, S_c = 0, kz_c = 0;
-  double p_n = 0, kr_n = 0, S_n = 0, kz_n = 0;
+  double p_c = 0, kr_c = 0, S_c = 0, kz_c = 0, pr_c = 0;
+  double p_n = 0, kr_n = 0, S_n = 0, kz_n = 0, pr_n = 0;
   {
     VGIn<PCS> v0;
     load_vg<PCS, MODE>(v0, A, col + plane(k0), psig);
@@ -2166,6 +2173,7 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
       kz_m = vm.kz;
     }
     p_c = v0.p;
+    if constexpr (MODE == 2) pr_c = v0.pr;
     vg_s_kr<TAB>(p_c, irg, v0.al, v0.n, vg_sr<PCS>(v0, f), vg_st<PCS>(v0, f), S_c, kr_c, f.tb);
     kr_c *= rho;
     kz_c = v0.kz;
@@ -2194,6 +2202,7 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
     // ---- A: closures of the own cell one plane ahead and of the halo cell
     if (k + 1 < nz) {
       p_n = vgU.p;
+      if constexpr (MODE == 2) pr_n = vgU.pr;
       vg_s_kr<TAB>(p_n, irg, vgU.al, vgU.n, vg_sr<PCS>(vgU, f), vg_st<PCS>(vgU, f), S_n, kr_n, f.tb);
       kr_n *= rho;
       kz_n = vgU.kz;
@@ -2249,7 +2258,7 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
                            (k == nz - 1);
           if (!bnd) {
             st_f = fmax(st_f, fabs(fv));
-            const double stp = 1.0 * p_c + (-1.0) * ldb(A.p, ip);
+            const double stp = 1.0 * p_c + (-1.0) * pr_c;
             st_s = fmax(st_s, fabs(stp / (1.0 + fabs(p_c))));
           }
         }
@@ -2258,7 +2267,7 @@ __global__ __launch_bounds__(TX * TY) void k_residual_pf(ResArgs A)
     }
     ub = uu;
     p_m = p_c; kr_m = kr_c; kz_m = kz_c;
-    p_c = p_n; kr_c = kr_n; S_c = S_n; kz_c = kz_n;
+    p_c = p_n; kr_c = kr_n; S_c = S_n; kz_c = kz_n; pr_c = pr_n;
   }
   if constexpr (MODE == 2) {
     if (A.stat) {
