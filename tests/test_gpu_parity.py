@@ -694,6 +694,66 @@ def test_multigrid_pair_shapes(gpu, variant, shape, precision, monkeypatch):
     assert err < (1e-11 if precision == 64 else 2e-5), err
 
 
+@pytest.mark.parametrize("shape,dirs", [((11, 9, 6), True), ((150, 37, 40), False),
+                                        ((70, 13, 33), True)])
+def test_line_search_stats_match_separate_passes(gpu, shape, dirs):
+    """The fused line-search entry points against the passes they replace,
+    bit for bit: pfhip_residual_step_stats (the trial point's residual with
+    F.F and KINStop's max |F| and scaled step length gathered by the
+    residual launches) against pfhip_residual_step + pfhip_vec_dot +
+    pfhip_kin_stop_norms, and pfhip_vec_dot_scsteplength against
+    pfhip_vec_dot + pfhip_vec_scsteplength (KS/kinsol.c:1228-1405,
+    1507-1580)."""
+    import ctypes as C
+    api = _api()
+    L = api.lib()
+    prob = cases.hetero(*shape, seed=31, dir_sides=dirs)
+    print(f"line-search stats shape={shape} dirs={dirs}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    rng = np.random.default_rng(32)
+    p_old = cases.random_pressure(prob, seed=33)
+    s_old = O.saturation(op, p_old)
+    uu_h = p_old + rng.normal(0, 0.02, p_old.shape)
+    pp_h = rng.normal(0, 0.05, p_old.shape)
+    gp.bc_pressure(0.3)
+    dpo, dso, uu, pp = gp.vec(p_old), gp.vec(s_old), gp.vec(uu_h), gp.vec(pp_h)
+    vp, dbl = C.c_void_p, C.c_double
+    L.pfhip_residual_step.argtypes = [vp, vp, vp, dbl, vp, vp, vp, vp, dbl, vp]
+    L.pfhip_residual_step_stats.argtypes = [vp, vp, vp, dbl, vp, vp, vp, vp, dbl, vp, vp, vp,
+                                            vp]
+    L.pfhip_kin_stop_norms.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.pfhip_vec_dot_scsteplength.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.pfhip_vec_scsteplength.argtypes = [vp, vp, vp, vp, vp]
+    L.pfhip_vec_dot.argtypes = [vp, vp, vp, vp, vp]
+    for rl in (1.0, 0.37):
+        un_a, F_a, un_b, F_b = gp.vec(), gp.vec(), gp.vec(), gp.vec()
+        ff, fmax, step = dbl(), dbl(), dbl()
+        api.check(L.pfhip_residual_step_stats(gp.h, uu.ptr, pp.ptr, rl, un_a.ptr, F_a.ptr,
+                                              dpo.ptr, dso.ptr, 0.1, C.byref(ff),
+                                              C.byref(fmax), C.byref(step), None), "stats")
+        api.check(L.pfhip_residual_step(gp.h, uu.ptr, pp.ptr, rl, un_b.ptr, F_b.ptr, dpo.ptr,
+                                        dso.ptr, 0.1, None), "step")
+        ff_b, fmax_b, step_b = dbl(), dbl(), dbl()
+        api.check(L.pfhip_vec_dot(gp.h, F_b.ptr, F_b.ptr, C.byref(ff_b), None), "dot")
+        api.check(L.pfhip_kin_stop_norms(gp.h, F_b.ptr, un_b.ptr, uu.ptr, C.byref(fmax_b),
+                                         C.byref(step_b), None), "kin_stop_norms")
+        assert np.array_equal(_I(prob, gp.get(F_a)), _I(prob, gp.get(F_b)))
+        assert np.array_equal(_I(prob, gp.get(un_a)), _I(prob, gp.get(un_b)))
+        assert (ff.value, fmax.value, step.value) == (ff_b.value, fmax_b.value, step_b.value)
+        # the maxima are those of the host arrays
+        Fh, uh = _I(prob, gp.get(F_b)), _I(prob, gp.get(un_b))
+        assert fmax.value == np.max(np.abs(Fh))
+        assert step.value == np.max(np.abs((uh - _I(prob, uu_h)) / (1.0 + np.abs(uh))))
+    pp2, rlen = dbl(), dbl()
+    api.check(L.pfhip_vec_dot_scsteplength(gp.h, uu.ptr, pp.ptr, C.byref(pp2), C.byref(rlen),
+                                           None), "dot_scsteplength")
+    pp2_b, rlen_b = dbl(), dbl()
+    api.check(L.pfhip_vec_dot(gp.h, pp.ptr, pp.ptr, C.byref(pp2_b), None), "dot")
+    api.check(L.pfhip_vec_scsteplength(gp.h, uu.ptr, pp.ptr, C.byref(rlen_b), None), "scstep")
+    assert (pp2.value, rlen.value) == (pp2_b.value, rlen_b.value)
+
+
 @pytest.mark.parametrize("overland,dirs", [(False, True), (True, False)])
 def test_residual_multi_tile(gpu, overland, dirs):
     """Residual on a grid spanning several 64-wide tiles, ragged tile rows
