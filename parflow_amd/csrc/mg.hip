@@ -280,6 +280,23 @@ __device__ __forceinline__ void resid_point(const LGeom &g, const Mat<T> &A, con
   t[ip] = b[ip] - v;
 }
 
+// resid_point's value without storing it
+template <typename T, bool FULL>
+__device__ __forceinline__ T resid_value(const LGeom &g, const Mat<T> &A, const T *x, const T *b,
+                                         int i, int j, int k)
+{
+  long ip = LIDX(g, i, j, k);
+  const int sy = g.s[1], sz = g.s[2];
+  T v = A.c[ip] * x[ip];
+  if (i > 0) v += lo_cpl<T, FULL>(A, ip, 0, 1) * x[ip - 1];
+  if (i + 1 < g.n[0]) v += A.u[0][ip] * x[ip + 1];
+  if (j > 0) v += lo_cpl<T, FULL>(A, ip, 1, sy) * x[ip - sy];
+  if (j + 1 < g.n[1]) v += A.u[1][ip] * x[ip + sy];
+  if (k > 0) v += lo_cpl<T, FULL>(A, ip, 2, sz) * x[ip - sz];
+  if (k + 1 < g.n[2]) v += A.u[2][ip] * x[ip + sz];
+  return b[ip] - v;
+}
+
 // r_c = r_f[C] + p2[C-1] r_f[C-1] + p1[C+1] r_f[C+1]  (mg_semi_restrict.c:120-150)
 template <typename T>
 __device__ __forceinline__ void restrict_point(const LGeom &gf, const LGeom &gc, int d,
@@ -968,22 +985,37 @@ __global__ __launch_bounds__(64 * TY) void k_resid_restrict_z(LGeom gf, LGeom gc
   }
 }
 
+// level residual and restriction in one pass, one thread per coarse point:
+// the fine residuals at C-1, C, C+1 along the coarsening direction d
+// (resid_point's terms and order; a fine point between two coarse ones is
+// evaluated by both) combined with restrict_point's weights and order, so
+// the residual is never stored and each small level costs one launch, not two
 template <typename T, bool FULL>
-__global__ __launch_bounds__(256) void k_residual_l(LGeom g, Mat<T> A, const T *__restrict__ x,
-                                                    const T *__restrict__ b, T *__restrict__ t)
-{
-  XCD_IJK(g);
-  resid_point<T, FULL>(g, A, x, b, t, i, j, k);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_restrict(LGeom gf, LGeom gc, int d,
-                                                  const T *__restrict__ rf, T *__restrict__ rc,
-                                                  const T *__restrict__ p1,
-                                                  const T *__restrict__ p2)
+__global__ __launch_bounds__(256) void k_resid_restrict(LGeom gf, LGeom gc, int d, Mat<T> A,
+                                                        const T *__restrict__ x,
+                                                        const T *__restrict__ b,
+                                                        T *__restrict__ rc,
+                                                        const T *__restrict__ p1,
+                                                        const T *__restrict__ p2)
 {
   XCD_IJK(gc);
-  restrict_point(gf, gc, d, rf, rc, p1, p2, i, j, k);
+  int f[3] = { i, j, k };
+  f[d] *= 2;
+  const long iF = LIDX(gf, f[0], f[1], f[2]);
+  const long sd = gf.s[d];
+  const T rC = resid_value<T, FULL>(gf, A, x, b, f[0], f[1], f[2]);
+  T lo = T(0), hi = T(0);
+  if (f[d] > 0) {
+    int g[3] = { f[0], f[1], f[2] };
+    g[d] -= 1;
+    lo = p2[iF - sd] * resid_value<T, FULL>(gf, A, x, b, g[0], g[1], g[2]);
+  }
+  if (f[d] + 1 < gf.n[d]) {
+    int g[3] = { f[0], f[1], f[2] };
+    g[d] += 1;
+    hi = p1[iF + sd] * resid_value<T, FULL>(gf, A, x, b, g[0], g[1], g[2]);
+  }
+  rc[LIDX(gc, i, j, k)] = rC + (lo + hi);
 }
 
 template <typename T>
@@ -1929,17 +1961,13 @@ static int mg_apply_seq(pfhip_mg *mg, const double *bin, double *xout, hipStream
       continue;
     }
     if (mg->full)
-      hipLaunchKernelGGL((k_residual_l<T, true>), dim3(ntiles(gf)), LBLOCK, 0, st, gf,
-                         matof<T>(mg, l), (const T *)mg->x[l], (const T *)mg->b[l],
-                         (T *)mg->t[l]);
+      hipLaunchKernelGGL((k_resid_restrict<T, true>), dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc,
+                         mg->dir[l], matof<T>(mg, l), (const T *)mg->x[l], (const T *)mg->b[l],
+                         (T *)mg->b[l + 1], (const T *)mg->p1[l], (const T *)mg->p2[l]);
     else
-      hipLaunchKernelGGL((k_residual_l<T, false>), dim3(ntiles(gf)), LBLOCK, 0, st, gf,
-                         matof<T>(mg, l), (const T *)mg->x[l], (const T *)mg->b[l],
-                         (T *)mg->t[l]);
-    LAUNCHCHK();
-    hipLaunchKernelGGL(k_restrict<T>, dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc, mg->dir[l],
-                       (const T *)mg->t[l], (T *)mg->b[l + 1], (const T *)mg->p1[l],
-                       (const T *)mg->p2[l]);
+      hipLaunchKernelGGL((k_resid_restrict<T, false>), dim3(ntiles(gc)), LBLOCK, 0, st, gf, gc,
+                         mg->dir[l], matof<T>(mg, l), (const T *)mg->x[l], (const T *)mg->b[l],
+                         (T *)mg->b[l + 1], (const T *)mg->p1[l], (const T *)mg->p2[l]);
     LAUNCHCHK();
   }
   // bottom of the cycle: levels lc .. L-1 in one workgroup; the coarsest
