@@ -2137,7 +2137,7 @@ template <typename T> struct EnsLv {
 
 // point sets a grid-level launch enumerates
 enum { PS_FINE = 0, PS_HALF = 1, PS_COARSE = 2, PS_N = 3 };
-enum { EL_CONV_IN, EL_FILL, EL_HALF, EL_RESID, EL_RESTRICT, EL_PROLONG, EL_CONV_OUT, EL_COARSE };
+enum { EL_CONV_IN, EL_FILL, EL_HALF, EL_RESTRICT, EL_PROLONG, EL_CONV_OUT, EL_COARSE };
 #define ENS_LBLK 256
 
 struct pfhip_ens_mgplan {
@@ -2193,8 +2193,28 @@ __global__ __launch_bounds__(ENS_LBLK) void k_ens_lvl(const EnsLSeg *__restrict_
     const LevelDev<T> &Cc = mm.lev[l + 1];
     if (idx >= Cc.g.n[0] * Cc.g.n[1] * Cc.g.n[2]) return;
     unflat(Cc.g, idx, i, j, k);
-    if (OP == EL_RESTRICT)
-      restrict_point(F.g, Cc.g, F.dir, F.t, Cc.b, F.p1, F.p2, i, j, k);
+    if (OP == EL_RESTRICT) {
+      // the fine residuals at C-1, C, C+1 in place (k_resid_restrict's
+      // expressions), so no separate residual phase
+      int f[3] = { i, j, k };
+      const int d = F.dir;
+      f[d] *= 2;
+      const long iF = LIDX(F.g, f[0], f[1], f[2]);
+      const long sd = F.g.s[d];
+      const T rC = resid_value<T, false>(F.g, F.A, F.x, F.b, f[0], f[1], f[2]);
+      T rlo = T(0), rhi = T(0);
+      if (f[d] > 0) {
+        int q[3] = { f[0], f[1], f[2] };
+        q[d] -= 1;
+        rlo = F.p2[iF - sd] * resid_value<T, false>(F.g, F.A, F.x, F.b, q[0], q[1], q[2]);
+      }
+      if (f[d] + 1 < F.g.n[d]) {
+        int q[3] = { f[0], f[1], f[2] };
+        q[d] += 1;
+        rhi = F.p1[iF + sd] * resid_value<T, false>(F.g, F.A, F.x, F.b, q[0], q[1], q[2]);
+      }
+      Cc.b[LIDX(Cc.g, i, j, k)] = rC + (rlo + rhi);
+    }
     else
       coarse_point<T, false>(F.g, Cc.g, F.A, Cc.A, F.dir, F.o1, F.o2, F.p1, F.p2, i, j, k);
     return;
@@ -2204,7 +2224,6 @@ __global__ __launch_bounds__(ENS_LBLK) void k_ens_lvl(const EnsLSeg *__restrict_
   const long ip = LIDX(F.g, i, j, k);
   if (OP == EL_CONV_IN) F.b[ip] = (T)mm.bin[ip];
   else if (OP == EL_FILL) F.x[ip] = T(0);
-  else if (OP == EL_RESID) resid_point<T, false>(F.g, F.A, F.x, F.b, F.t, i, j, k);
   else if (OP == EL_PROLONG) {
     const LevelDev<T> &Cc = mm.lev[l + 1];
     prolong_point(F.g, Cc.g, F.dir, Cc.x, F.x, F.p1, F.p2, i, j, k);
@@ -2445,7 +2464,6 @@ static int ens_sequence(pfhip_ens_mgplan *p, int apply, hipStream_t s)
   if (p->G > 0) ens_phase<T>(p, 0, PS_FINE, EL_CONV_IN, 0, 0, s);
   for (int l = 0; l < p->G; l++) {
     ens_sweeps<T>(p, l, 1, p->pre, s);
-    ens_phase<T>(p, l, PS_FINE, EL_RESID, 0, 0, s);
     ens_phase<T>(p, l, PS_COARSE, EL_RESTRICT, 0, 0, s);
     LAUNCHCHK();
   }
