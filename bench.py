@@ -136,10 +136,14 @@ def _workload_label(name, prob, opts, dt, krylov_label, vcycle="FP32"):
             f"dt={dt} h")
 
 
-def _timed_run(api, gp, sv, p0, dt, K, W, snapshot=True):
+def _timed_run(api, gp, sv, p0, dt, K, W, snapshot=False):
     """W untimed warmup steps on a scratch state, then exactly K steps from
-    p0 between two device synchronisations; the pressure after every step is
-    copied into a device snapshot (part of the timed work, one pass)."""
+    p0 between two device synchronisations, in one pfhip_advance_richards
+    call (the time loop as a caller runs it).  With snapshot, the K steps
+    are run one call at a time instead and the pressure after every step is
+    copied into a device snapshot: for the untimed passes that supply the
+    per-step comparisons (the solver is deterministic, so their states are
+    the timed run's)."""
     L = api.lib()
     if W > 0:
         dp, ds = gp.vec(p0), gp.vec()
@@ -148,18 +152,23 @@ def _timed_run(api, gp, sv, p0, dt, K, W, snapshot=True):
     dp, ds = gp.vec(p0), gp.vec()
     snap = [gp.vec() for _ in range(K)] if snapshot else None
     newton, gmres, fevals = [], [], []
-    t = 0.0
     api.check(L.pfhip_synchronize(), "sync")
     tic = time.perf_counter()
-    for s in range(K):
-        lg = sv.advance_device(dp, ds, t, dt, 1)
-        t += float(lg["dt"][0])
-        newton.append(int(lg["newton"][0]))
-        gmres.append(int(lg["gmres"][0]))
-        fevals.append(int(lg["fevals"][0]))
-        if snapshot:
-            api.check(L.pfhip_vec_linear_sum(gp.h, 1.0, dp.ptr, 0.0, dp.ptr, snap[s].ptr, None),
-                      "snapshot")
+    if snapshot:
+        t = 0.0
+        for s in range(K):
+            lg = sv.advance_device(dp, ds, t, dt, 1)
+            t += float(lg["dt"][0])
+            newton.append(int(lg["newton"][0]))
+            gmres.append(int(lg["gmres"][0]))
+            fevals.append(int(lg["fevals"][0]))
+            api.check(L.pfhip_vec_linear_sum(gp.h, 1.0, dp.ptr, 0.0, dp.ptr, snap[s].ptr,
+                                             None), "snapshot")
+    else:
+        lg = sv.advance_device(dp, ds, 0.0, dt, K)
+        newton = [int(x) for x in lg["newton"]]
+        gmres = [int(x) for x in lg["gmres"]]
+        fevals = [int(x) for x in lg["fevals"]]
     api.check(L.pfhip_synchronize(), "sync")
     elapsed = time.perf_counter() - tic
     return dict(elapsed=elapsed, newton=newton, gmres=gmres, fevals=fevals,
@@ -183,7 +192,7 @@ def _phase_pass(api, gp, sv, p0, dt, K):
     device milliseconds per step of each phase."""
     for _, pid in PHASES:
         api.check(api.lib().pfhip_prof_enable(getattr(api, pid), 20000), "prof")
-    _timed_run(api, gp, sv, p0, dt, K, 0, snapshot=False)
+    _timed_run(api, gp, sv, p0, dt, K, 0)
     prof = _read_prof(api, [(name, pid) for name, pid in PHASES])
     return {name: {"ms_per_step": ms / K, "calls": n} for name, (ms, n) in prof.items()}
 
@@ -236,10 +245,12 @@ def run_gpu(args):
     # per-launch kernel timers on (their events bracket each launch, and the
     # V-cycle then launches directly instead of replaying its graph)
     sv = api.GPUSolver(gp, opts, precision=32, krylov=1)
-    mixed, snap_m = _timed_run(api, gp, sv, p0, dt, K, W)
+    mixed, _ = _timed_run(api, gp, sv, p0, dt, K, W)
     for _, pid, _ in PROF_KERNELS:
         api.check(L.pfhip_prof_enable(getattr(api, pid), 20000), "prof")
-    _timed_run(api, gp, sv, p0, dt, K, 0, snapshot=False)
+    # the kernel-timer pass also takes the per-step pressure snapshots
+    timer_pass, snap_m = _timed_run(api, gp, sv, p0, dt, K, 0, snapshot=True)
+    assert timer_pass["newton"] == mixed["newton"], "mixed passes diverged"
     mixed["prof"] = _read_prof(api, [(k, pid) for k, pid, _ in PROF_KERNELS])
     L.pfhip_solver_mg.restype = api.C.c_void_p
     L.pfhip_solver_mg.argtypes = [api.C.c_void_p]
@@ -257,9 +268,12 @@ def run_gpu(args):
     if not args.no_plain:
         # plain: the reference's algorithm in FP64 with the deck's sweeps
         sv = api.GPUSolver(gp, opts, precision=64, krylov=0)
-        plain, snap_p = _timed_run(api, gp, sv, p0, dt, K, W)
+        plain, _ = _timed_run(api, gp, sv, p0, dt, K, W)
         _log(f"plain: {K} steps in {plain['elapsed']:.3f}s = {plain['value']:.4e} "
              f"cell-steps/s newton={plain['newton']} gmres={plain['gmres']}")
+        # an untimed pass for the per-step snapshots
+        again, snap_p = _timed_run(api, gp, sv, p0, dt, K, 0, snapshot=True)
+        assert again["newton"] == plain["newton"], "plain passes diverged"
         if not args.no_phases:
             plain["phases"] = _phase_pass(api, gp, sv, p0, dt, K)
         del sv
@@ -273,7 +287,7 @@ def run_gpu(args):
         # (PL/pf_pfmg.c:294-297 defaults)
         ref_opts = dataclasses.replace(opts, num_pre_relax=1, num_post_relax=1)
         sv = api.GPUSolver(gp, ref_opts, precision=64, krylov=0)
-        pref, _ = _timed_run(api, gp, sv, p0, dt, K, W, snapshot=False)
+        pref, _ = _timed_run(api, gp, sv, p0, dt, K, W)
         _log(f"plain (1+1 sweeps): {K} steps in {pref['elapsed']:.3f}s = "
              f"{pref['value']:.4e} cell-steps/s newton={pref['newton']} gmres={pref['gmres']}")
         del sv
