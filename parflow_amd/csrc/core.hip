@@ -546,7 +546,7 @@ __device__ __forceinline__ double block_reduce(double v)
 }
 
 enum RedOp { RED_DOT = 0, RED_MAXABS = 1, RED_L1 = 2, RED_SCSTEP = 3, RED_DQ = 4,
-             RED_WL2 = 5, RED_NEGMAX = 6, RED_CONSTR = 7, RED_DOTSC = 8 };
+             RED_WL2 = 5, RED_NEGMAX = 6, RED_CONSTR = 7, RED_DOTSC = 8, RED_KINIT = 9 };
 
 template <int OP>
 __global__ __launch_bounds__(256) void k_reduce(Geom g, const double *__restrict__ x,
@@ -567,7 +567,13 @@ __global__ __launch_bounds__(256) void k_reduce(Geom g, const double *__restrict
       else if (OP == RED_MAXABS) a0 = fmax(a0, fabs(x[ip]));
       else if (OP == RED_L1) a0 += fabs(x[ip]);
       else if (OP == RED_SCSTEP) a0 = fmax(a0, fabs(y[ip] / (1.0 + fabs(x[ip]))));
-      else if (OP == RED_DOTSC) {
+      else if (OP == RED_KINIT) {
+        // max |y| (RED_MAXABS of y), y.y and x.x (RED_DOT's products)
+        const double fv = y[ip], pv = x[ip];
+        a0 = fmax(a0, fabs(fv));
+        a1 += fv * fv;
+        a2 += pv * pv;
+      } else if (OP == RED_DOTSC) {
         // y.y (RED_DOT's product with x = y) and RED_SCSTEP's maximum
         const double pv = y[ip];
         a0 += pv * pv;
@@ -580,13 +586,17 @@ __global__ __launch_bounds__(256) void k_reduce(Geom g, const double *__restrict
       }
     }
   }
-  const bool MX = (OP == RED_MAXABS || OP == RED_SCSTEP || OP == RED_NEGMAX);
+  const bool MX = (OP == RED_MAXABS || OP == RED_SCSTEP || OP == RED_NEGMAX || OP == RED_KINIT);
   a0 = block_reduce<MX>(a0);
   if (OP == RED_DQ) { a1 = block_reduce<false>(a1); a2 = block_reduce<false>(a2); }
   if (OP == RED_DOTSC) a1 = block_reduce<true>(a1);
+  if (OP == RED_KINIT) { a1 = block_reduce<false>(a1); a2 = block_reduce<false>(a2); }
   if (threadIdx.x == 0) {
     partial[blockIdx.x] = a0;
-    if (OP == RED_DQ) { partial[nb + blockIdx.x] = a1; partial[2 * nb + blockIdx.x] = a2; }
+    if (OP == RED_DQ || OP == RED_KINIT) {
+      partial[nb + blockIdx.x] = a1;
+      partial[2 * nb + blockIdx.x] = a2;
+    }
     if (OP == RED_DOTSC) partial[nb + blockIdx.x] = a1;
   }
 }
@@ -687,6 +697,30 @@ int pfhip_vec_dot_scsteplength(pfhip_problem *pb, const double *u, const double 
   if (rc) return rc;
   *pp = pb->h_red[0];
   *rlength = pb->h_red[1];
+  return PFHIP_OK;
+}
+
+// KINSOL's start (KS/kinsol.c:690-760): max |F| for the initial stop test,
+// F.F and u.u from one pass, with the values of pfhip_vec_max_norm(F),
+// pfhip_vec_dot(F, F) and pfhip_vec_dot(u, u) and one synchronisation
+int pfhip_vec_kinsol_init_norms(pfhip_problem *pb, const double *u, const double *f,
+                                double *fmax, double *ff, double *uu, void *stream)
+{
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  hipLaunchKernelGGL(k_reduce<RED_KINIT>, dim3(nb), dim3(256), 0, STREAM(stream), g, u, f,
+                     pb->red_partial, nb);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_reduce_final<true>, dim3(1), dim3(256), 0, STREAM(stream),
+                     pb->red_partial, nb, 1, pb->red_dev);
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, STREAM(stream),
+                     pb->red_partial + nb, nb, 2, pb->red_dev + 1);
+  LAUNCHCHK();
+  int rc = fetch_results(pb, 3, stream);
+  if (rc) return rc;
+  *fmax = pb->h_red[0];
+  *ff = pb->h_red[1];
+  *uu = pb->h_red[2];
   return PFHIP_OK;
 }
 

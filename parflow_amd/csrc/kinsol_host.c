@@ -72,6 +72,7 @@ const pfhip_ops pfhip_direct_ops = {
   pfhip_residual_jv, pfhip_residual_step, pfhip_mg_setup_jacobian, pfhip_mg_apply,
   pfhip_bc_pressure, pfhip_saturation, pfhip_synchronize, pfhip_residual_jv_dq,
   pfhip_vec_cgs2, pfhip_vec_dot_scsteplength, pfhip_residual_step_stats,
+  pfhip_vec_kinsol_init_norms,
 };
 
 void pfhip_solver_set_ops(pfhip_solver *s, const pfhip_ops *ops) { s->ops = ops; }
@@ -776,10 +777,6 @@ static int kinsol_solve(pfhip_solver *s, double *p, const double *p_old,
   s->scsteptol = o->step_tol > 0.0 ? o->step_tol : pow(s->uround, 2.0 / 3.0);
   s->mxiter = o->max_iter > 0 ? o->max_iter : 200;
   s->sqrt_relfunc = (o->deriv_eps > 0.0) ? sqrt(o->deriv_eps) : sqrt(s->uround);
-  double un;
-  CHK(l2norm(s, p, &un));
-  s->mxnewtstep = 1000.0 * un;
-  if (s->mxnewtstep < 1.0) s->mxnewtstep = 1.0;
   /* eta setup, KS/kinsol.c:526-575 */
   if (o->eta_choice == 0) {
     s->etaflag = 0;
@@ -805,12 +802,25 @@ static int kinsol_solve(pfhip_solver *s, double *p, const double *p_old,
   /* KINInitialStop */
   CHK(s->ops->vec_scale(pb, 1.0, p, s->unew, NULL));
   CHK(func(s, p, s->fval));
-  double fmax;
-  CHK(s->ops->vec_max_norm(pb, s->fval, &fmax, NULL));
+  /* mxnewtstep from ||u|| (KS/kinsol.c:518-522), then max |fval| and
+   * ||fval|| (KS/kinsol.c:640-644); the three reductions are order-fixed,
+   * so one pass gives the values of three */
+  double fmax, un, ff;
+  if (s->ops->kinsol_init_norms) {
+    CHK(s->ops->kinsol_init_norms(pb, p, s->fval, &fmax, &ff, &un, NULL));
+    un = sqrt(un);
+  } else {
+    CHK(l2norm(s, p, &un));
+    CHK(s->ops->vec_max_norm(pb, s->fval, &fmax, NULL));
+    ff = -1.0;
+  }
+  s->mxnewtstep = 1000.0 * un;
+  if (s->mxnewtstep < 1.0) s->mxnewtstep = 1.0;
   if (fmax <= 0.01 * s->fnormtol) {
     ret = 2;
   } else {
-    CHK(l2norm(s, s->fval, &s->fnorm));
+    if (ff >= 0.0) s->fnorm = sqrt(ff);
+    else CHK(l2norm(s, s->fval, &s->fnorm));
     s->f1norm = 0.5 * s->fnorm * s->fnorm;
     s->pthrsh = 2.0;
     double epsmin = 0.01 * s->fnormtol;

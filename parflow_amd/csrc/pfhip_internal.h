@@ -67,6 +67,9 @@ int pfhip_vec_combine(pfhip_problem *pb, const double *const *V, int m,
  * l2norm and KINScSteplength of KINLineSearch's start) */
 int pfhip_vec_dot_scsteplength(pfhip_problem *pb, const double *u, const double *p,
                                double *pp, double *rlength, void *stream);
+/* KINSOL's start: max |f|, f.f and u.u in one pass */
+int pfhip_vec_kinsol_init_norms(pfhip_problem *pb, const double *u, const double *f,
+                                double *fmax, double *ff, double *uu, void *stream);
 /* MGS step fused: w -= hprev * vprev (if vprev), then returns vnext . w */
 int pfhip_vec_axpy_dot(pfhip_problem *pb, double hprev, const double *vprev,
                        double *w, const double *vnext, double *dot,
@@ -166,6 +169,8 @@ typedef struct pfhip_ops {
   int (*residual_step_stats)(pfhip_problem *, const double *, const double *, double,
                              double *, double *, const double *, const double *, double,
                              double *, double *, double *, void *);
+  int (*kinsol_init_norms)(pfhip_problem *, const double *, const double *, double *,
+                           double *, double *, void *);
 } pfhip_ops;
 
 extern const pfhip_ops pfhip_direct_ops;

@@ -703,7 +703,8 @@ def test_line_search_stats_match_separate_passes(gpu, shape, dirs):
     residual launches) against pfhip_residual_step + pfhip_vec_dot +
     pfhip_kin_stop_norms, and pfhip_vec_dot_scsteplength against
     pfhip_vec_dot + pfhip_vec_scsteplength (KS/kinsol.c:1228-1405,
-    1507-1580)."""
+    1507-1580), and pfhip_vec_kinsol_init_norms against pfhip_vec_max_norm
+    + two pfhip_vec_dot."""
     import ctypes as C
     api = _api()
     L = api.lib()
@@ -752,6 +753,19 @@ def test_line_search_stats_match_separate_passes(gpu, shape, dirs):
     api.check(L.pfhip_vec_dot(gp.h, pp.ptr, pp.ptr, C.byref(pp2_b), None), "dot")
     api.check(L.pfhip_vec_scsteplength(gp.h, uu.ptr, pp.ptr, C.byref(rlen_b), None), "scstep")
     assert (pp2.value, rlen.value) == (pp2_b.value, rlen_b.value)
+    # KINSOL's start: max |F|, F.F and u.u from one pass against the three
+    # separate reductions (KS/kinsol.c:518-522, 640-644)
+    L.pfhip_vec_kinsol_init_norms.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.pfhip_vec_max_norm.argtypes = [vp, vp, vp, vp]
+    fm, f2, u2 = dbl(), dbl(), dbl()
+    api.check(L.pfhip_vec_kinsol_init_norms(gp.h, uu.ptr, F_a.ptr, C.byref(fm), C.byref(f2),
+                                            C.byref(u2), None), "init_norms")
+    fm_b, f2_b, u2_b = dbl(), dbl(), dbl()
+    api.check(L.pfhip_vec_max_norm(gp.h, F_a.ptr, C.byref(fm_b), None), "max_norm")
+    api.check(L.pfhip_vec_dot(gp.h, F_a.ptr, F_a.ptr, C.byref(f2_b), None), "dot F")
+    api.check(L.pfhip_vec_dot(gp.h, uu.ptr, uu.ptr, C.byref(u2_b), None), "dot u")
+    assert (fm.value, f2.value, u2.value) == (fm_b.value, f2_b.value, u2_b.value)
+    assert fm.value == np.max(np.abs(_I(prob, gp.get(F_a))))
 
 
 @pytest.mark.parametrize("overland,dirs", [(False, True), (True, False)])
