@@ -258,6 +258,11 @@ def run_gpu(args):
     arr = (api.C.c_int * 64)()
     nl = L.pfhip_mg_pair_shapes(L.pfhip_solver_mg(sv.h), arr, 64)
     mixed["pair_shapes"] = list(arr[:nl])
+    din, dout = api.C.c_int(), api.C.c_int()
+    L.pfhip_mg_pair_shapes_fp64.argtypes = [api.C.c_void_p, api.C.c_void_p, api.C.c_void_p]
+    api.check(L.pfhip_mg_pair_shapes_fp64(L.pfhip_solver_mg(sv.h), api.C.byref(din),
+                                          api.C.byref(dout)), "pair_shapes_fp64")
+    mixed["pair_shapes_fp64"] = [din.value, dout.value]
     _log(f"smoother tile shape per grid level (autotuned): {mixed['pair_shapes']}")
     _log(f"mixed: {K} steps in {mixed['elapsed']:.3f}s = {mixed['value']:.4e} cell-steps/s "
          f"newton={mixed['newton']} gmres={mixed['gmres']}")
@@ -579,6 +584,7 @@ def main():
             "gmres_per_step": mixed["gmres"],
             "fevals_per_step": mixed["fevals"],
             "mg_pair_shape_per_level": mixed["pair_shapes"],
+            "mg_pair_shape_level0_fp64_in_out": mixed["pair_shapes_fp64"],
         },
         "roofline": {k: dom[k] for k in ("bound", "achieved", "peak", "unit", "frac",
                                           "traffic", "kernel", "bytes_per_cell",

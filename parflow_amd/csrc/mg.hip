@@ -699,6 +699,10 @@ __device__ __forceinline__ void load_pair_yl(PairSet<T> &S, const Mat<T> &A,
 // the YL form carries the north couplings two planes ahead and its outer
 // rows' neighbour loads: up to 80 VGPRs (6 waves per SIMD) in FP32
 #define PAIR2_YL_MIN_WAVES(T) (sizeof(T) == 4 ? 6 : 1)
+// loading PF > 1 planes ahead keeps PF planes of loads in flight per wave
+// across the per-plane barrier, at up to 128 VGPRs (4 waves per SIMD, the
+// occupancy a 14-row tile of 16 waves has anyway)
+#define PAIR2_PF_MIN_WAVES(T) (sizeof(T) == 4 ? 4 : 1)
 
 // DOUT: the pair writes its result as FP64 into dout (a vector of the
 // level's ghosted layout, no alignment assumed) instead of xout: the last
@@ -715,9 +719,15 @@ __device__ __forceinline__ void load_pair_yl(PairSet<T> &S, const Mat<T> &A,
 // ahead), in an LDS buffer per plane parity, before the plane's barrier.
 // Only the tile's outer rows load their outer neighbours.  6 loads per
 // plane instead of 9, with the same single barrier per plane.
+//
+// PF (YL only): the coefficients are loaded PF planes ahead (x_in and the
+// north couplings PF + 1), so PF planes of loads are in flight while a
+// plane is updated; the arithmetic and its order do not depend on PF.
 template <typename T, int TY, int KC, bool INIT, bool DOUT = false, bool DIN = false,
-          bool YL = false>
-__global__ __launch_bounds__(64 * (TY + 2), YL ? PAIR2_YL_MIN_WAVES(T) : PAIR2_MIN_WAVES(T))
+          bool YL = false, int PF = 1>
+__global__ __launch_bounds__(64 * (TY + 2), YL ? (PF > 1 ? PAIR2_PF_MIN_WAVES(T)
+                                                         : PAIR2_YL_MIN_WAVES(T))
+                                               : PAIR2_MIN_WAVES(T))
 void k_rbgs_pair2(LGeom g, Mat<T> A,
                                                               const T *__restrict__ xin,
                                                               T *__restrict__ xout,
@@ -728,6 +738,7 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
                                                               void *const *io)
 {
   typedef typename V2T<T>::type T2;
+  static_assert(PF >= 1 && (YL || PF == 1), "prefetch depth > 1 needs the YL form");
   // io: the FP64 interface vectors' addresses are read from device memory
   // (a launch sequence captured once and replayed with other vectors)
   if (DIN && io) din = (const double *)io[0];
@@ -767,41 +778,52 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
   // carried state
   T sc = T(1), sb = T(0), saw = T(0), sae = T(0), sas = T(0), san = T(0), sal = T(0), sau = T(0);
   T rm1 = T(0), rm2 = T(0);
-  PairSet<T> cur, nxt;
-  T2 an1 = T2{T(0), T(0)}, an2 = T2{T(0), T(0)};
+  // at the top of the step for plane k: cur = plane k, ahead[d] = plane
+  // k+1+d (d < PF-1), anr[d] = north couplings of plane k+1+d (d < PF),
+  // xr[d] = x_in of plane k-1+d (d < PF+2); the step loads ahead[PF-1],
+  // anr[PF] and xr[PF+2]
+  PairSet<T> cur, ahead[PF];
+  T2 anr[PF + 1];
+  T2 xr[PF + 3];
+#pragma unroll
+  for (int d = 0; d <= PF; d++) anr[d] = T2{T(0), T(0)};
   if (YL) {
     load_pair_yl<T, INIT, DIN>(cur, A, xin, b, at(k0), sy, own_s, own_n, din);
     cur.an = ldo<T2>(A.u[1], at(k0));
-    an1 = ldo<T2>(A.u[1], at(k0 + 1));
+#pragma unroll
+    for (int d = 0; d + 1 < PF; d++)
+      load_pair_yl<T, INIT, DIN>(ahead[d], A, xin, b, at(k0 + 1 + d), sy, own_s, own_n, din);
+#pragma unroll
+    for (int d = 0; d < PF; d++) anr[d] = ldo<T2>(A.u[1], at(k0 + 1 + d));
   } else {
     load_pair<T, INIT, DIN>(cur, A, xin, b, at(k0), sy, din);
   }
   T2 al = ldo<T2>(A.u[2], at(k0 - 1));               // lower coupling of plane k0
-  T2 xm = load_x2<T, INIT>(xin, at(k0 - 1));
-  T2 x0 = load_x2<T, INIT>(xin, at(k0));
-  T2 xp = load_x2<T, INIT>(xin, at(k0 + 1));
+#pragma unroll
+  for (int d = 0; d < PF + 2; d++) xr[d] = load_x2<T, INIT>(xin, at(k0 - 1 + d));
   if (YL) {
     // plane k0's shared values, before the first plane's red update
     syn[k0 & 1][ty][tx] = cur.an;
-    if (!INIT) syx[k0 & 1][ty][tx] = x0;
+    if (!INIT) syx[k0 & 1][ty][tx] = xr[1];
     lds_barrier();
   }
   int it = 0;
   for (int k = k0; k <= ke; k++, it++) {
-    // prefetch: coefficients and y-neighbour x of plane k+1, x of plane k+2
-    // (YL: the north couplings of plane k+2 as well, published one plane
-    // later)
+    // prefetch: coefficients and y-neighbour x of plane k+PF, x of plane
+    // k+PF+1 (YL: the north couplings of plane k+PF+1 as well, published
+    // one plane before their own)
     if (YL) {
-      load_pair_yl<T, INIT, DIN>(nxt, A, xin, b, at(k + 1), sy, own_s, own_n, din);
-      an2 = ldo<T2>(A.u[1], at(k + 2));
+      load_pair_yl<T, INIT, DIN>(ahead[PF - 1], A, xin, b, at(k + PF), sy, own_s, own_n, din);
+      anr[PF] = ldo<T2>(A.u[1], at(k + PF + 1));
     } else {
-      load_pair<T, INIT, DIN>(nxt, A, xin, b, at(k + 1), sy, din);
+      load_pair<T, INIT, DIN>(ahead[0], A, xin, b, at(k + 1), sy, din);
     }
-    const T2 xpp = load_x2<T, INIT>(xin, at(k + 2));
+    xr[PF + 2] = load_x2<T, INIT>(xin, at(k + PF + 1));
+    const T2 xm = xr[0], x0 = xr[1], xp = xr[2];
     if (DIN && owned && k >= kb && k < ke) sto<T2>(bout, ip, cur.bb);
     if (YL) {
       // plane k+1's north couplings and x_in for the rows beside
-      syn[(k + 1) & 1][ty][tx] = an1;
+      syn[(k + 1) & 1][ty][tx] = anr[0];
       if (!INIT) syx[(k + 1) & 1][ty][tx] = xp;
       // this plane's y neighbours, published one plane ago
       if (!own_s) {
@@ -869,9 +891,16 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
     rm2 = rm1;
     rm1 = rk;
     al = cur.au;
-    cur = nxt;
-    if (YL) { cur.an = an1; an1 = an2; }
-    xm = x0; x0 = xp; xp = xpp;
+    cur = ahead[0];
+#pragma unroll
+    for (int d = 0; d + 1 < PF; d++) ahead[d] = ahead[d + 1];
+    if (YL) {
+      cur.an = anr[0];
+#pragma unroll
+      for (int d = 0; d < PF; d++) anr[d] = anr[d + 1];
+    }
+#pragma unroll
+    for (int d = 0; d < PF + 2; d++) xr[d] = xr[d + 1];
     ip += sz;
   }
 }
@@ -1436,6 +1465,7 @@ struct pfhip_mg {
   int rrz;                     // fused level residual + z restriction
   int cz;                      // z-marching coarse operators (k_coarse_z)
   int pv[MAXL];                // pair tile shape of each grid level (tune_pairs)
+  int pv_din, pv_dout;         // level 0's shapes for its FP64-input and FP64-result pairs
   int tuned;                   // pv chosen (at the first V-cycle)
   void *levdev;                // device LevelDev<T>[L]
   long lds_bytes;              // > 0: the bottom runs in LDS (k_coarse_vcycle_lds)
@@ -1596,6 +1626,7 @@ static int mg_create(pfhip_problem *pb, int max_levels, int pre, int post, int s
   if (rc != PFHIP_OK) { pfhip_mg_destroy(mg); return rc; }
   int forced = rbgs_forced();
   for (l = 0; l < MAXL; l++) mg->pv[l] = (forced >= 0) ? forced : 0;
+  mg->pv_din = mg->pv_dout = (forced >= 0) ? forced : 0;
   mg->tuned = (forced >= 0) || !mg->fuse || full;
   const char *fg = getenv("PFHIP_MG_GRAPH");
   mg->use_graph = (fg && *fg) ? atoi(fg) : 1;
@@ -1741,7 +1772,10 @@ static int sweep_list(int symmetric, int zero, int max_iter, HalfSweep *seq, int
 // Fused pair tile shapes (rows TY of a tile, z-chunk KC).  Each multigrid
 // times them per grid level when it is created (tune_pairs) and keeps the
 // fastest; PFHIP_RBGS_VARIANT=v forces shape v on every level.
-#define NPAIRV 10
+#define NPAIRV 17
+// the shapes tuned in each precision: the deeper-prefetch forms (v >= 10)
+// are FP32 only (an FP64 plane set is twice the registers)
+template <typename T> static int npairv() { return sizeof(T) == 4 ? NPAIRV : 10; }
 static int rbgs_forced()
 {
   const char *e = getenv("PFHIP_RBGS_VARIANT");
@@ -1763,7 +1797,7 @@ static void launch_pair_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout
                        A, xin, xout, b);
 }
 
-template <typename T, int TY, int KC, bool YL>
+template <typename T, int TY, int KC, bool YL, int PF = 1>
 static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
                            int init, hipStream_t st, double *dout, const double *din,
                            void *const *io)
@@ -1771,24 +1805,26 @@ static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xou
   unsigned gx = (g.n[0] + PX2 - 1) / PX2, gy = (g.n[1] + TY - 1) / TY, gz = (g.n[2] + KC - 1) / KC;
   unsigned nb = gx * gy * gz, nb8 = (nb + 7u) & ~7u;
   if (init && din)
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true, YL>), dim3(nb8),
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true, YL, PF>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, din,
                        (T *)b, io);
   else if (init)
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, false, YL>), dim3(nb8),
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, false, YL, PF>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr,
                        io);
   else if (dout)
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true, false, YL>), dim3(nb8),
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true, false, YL, PF>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr, io);
   else
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, false, false, YL>), dim3(nb8),
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, false, false, YL, PF>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr,
                        io);
 }
 
 // shape v % 5: (TY, KC) = (14, 16), (6, 16), (6, 32), (14, 8), (14, 4);
 // v >= 5 takes the y neighbours through LDS (the YL form; even x extents);
+// v = 10..14: the YL form loading two planes ahead (FP32), 15 and 16:
+// three planes ahead with (14, 16) and (14, 8);
 // dout (even x extents, non-initial pairs only) takes the result in FP64,
 // din (even x extents, initial pairs only) gives the right-hand side in
 // FP64 and b receives it in the level precision
@@ -1809,8 +1845,20 @@ static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, 
       case 7: launch_pair2_t<T, 6, 32, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
       case 8: launch_pair2_t<T, 14, 8, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
       case 9: launch_pair2_t<T, 14, 4, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      default: launch_pair2_t<T, 14, 16, false>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      default: break;
     }
+    if constexpr (sizeof(T) == 4) switch (v) {
+      case 10: launch_pair2_t<T, 14, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 11: launch_pair2_t<T, 6, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 12: launch_pair2_t<T, 6, 32, true, 2>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 13: launch_pair2_t<T, 14, 8, true, 2>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 14: launch_pair2_t<T, 14, 4, true, 2>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 15: launch_pair2_t<T, 14, 16, true, 3>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 16: launch_pair2_t<T, 14, 8, true, 3>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      default: break;
+    }
+    launch_pair2_t<T, 14, 16, false>(g, A, xin, xout, b, init, st, dout, din, io);
+    return;
   }
   switch (v % 5) {
     case 1: launch_pair_t<T, 6, 16>(g, A, xin, xout, b, init, st); break;
@@ -1825,10 +1873,14 @@ static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, 
 // operator, and never for hierarchies that the ensemble drives a workgroup
 // at a time): every pair shape on every grid level, one untimed launch and
 // then TUNE_REPS timed ones (HIP events), keeping the fastest per level.
+// Level 0 also times the two forms that touch the solver's FP64 vectors
+// (the first pair reading the right-hand side bin, the last writing the
+// result into xout: their extra bytes shift the best shape), with the
+// vectors of this first V-cycle, which then overwrites what they wrote.
 // The iterate buffers are re-zeroed afterwards.
 #define TUNE_REPS 3
 template <typename T>
-static int tune_pairs(pfhip_mg *mg, hipStream_t st)
+static int tune_pairs(pfhip_mg *mg, hipStream_t st, const double *bin, double *xout)
 {
   mg->tuned = 1;
   hipEvent_t e0, e1;
@@ -1838,17 +1890,28 @@ static int tune_pairs(pfhip_mg *mg, hipStream_t st)
     if (!mg->xa[l]) continue;
     LGeom g = lgeom(mg->n[l]);
     Mat<T> A = matof<T>(mg, l);
-    float best = 0.f;
-    for (int v = 0; v < NPAIRV; v++) {
-      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], 0, v, st);
-      HIPCHK(hipEventRecord(e0, st));
-      for (int r = 0; r < TUNE_REPS; r++)
-        launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], 0, v, st);
-      HIPCHK(hipEventRecord(e1, st));
-      HIPCHK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-      if (v == 0 || ms < best) { best = ms; mg->pv[l] = v; }
+    const bool fp64_forms = l == 0 && bin && xout && (g.n[0] % 2) == 0;
+    for (int form = 0; form < (fp64_forms ? 3 : 1); form++) {
+      // form 0: level-precision operands; 1: FP64 right-hand side (zero
+      // initial guess); 2: FP64 result
+      const int init = form == 1;
+      double *dout = form == 2 ? xout : nullptr;
+      const double *din = form == 1 ? bin : nullptr;
+      int *pick = form == 0 ? &mg->pv[l] : form == 1 ? &mg->pv_din : &mg->pv_dout;
+      float best = 0.f;
+      for (int v = 0; v < npairv<T>(); v++) {
+        launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], init, v,
+                       st, dout, din);
+        HIPCHK(hipEventRecord(e0, st));
+        for (int r = 0; r < TUNE_REPS; r++)
+          launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], init,
+                         v, st, dout, din);
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (v == 0 || ms < best) { best = ms; *pick = v; }
+      }
     }
     const long sz = lsize(mg->n[l]) + 1;
     hipLaunchKernelGGL(k_fill_t<T>, dim3(256), dim3(256), 0, st, (T *)mg->raw[l][4], sz);
@@ -1865,6 +1928,15 @@ extern "C" int pfhip_mg_pair_shapes(pfhip_mg *mg, int *shape, int max)
   int n = mg->lc < max ? mg->lc : max;
   for (int l = 0; l < n; l++) shape[l] = mg->pv[l];
   return n;
+}
+
+// level 0's shapes for the pair reading the FP64 right-hand side and the
+// pair writing the FP64 result
+extern "C" int pfhip_mg_pair_shapes_fp64(pfhip_mg *mg, int *din, int *dout)
+{
+  *din = mg->pv_din;
+  *dout = mg->pv_dout;
+  return PFHIP_OK;
 }
 
 // smoothing of grid level l: red -> black pairs run fused (x and its second
@@ -1900,8 +1972,9 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
                                                                    : PFHIP_PROF_RBGS;
     if (mark >= 0) pfhip_prof_mark(mark, 0, st);
     if (pair) {
-      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, mg->pv[l], st,
-                     d, di, io);
+      const int v = (l == 0 && d) ? mg->pv_dout : (l == 0 && di) ? mg->pv_din : mg->pv[l];
+      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, v, st, d, di,
+                     io);
       if (d && wrote) *wrote = 1;
       void *tmp = mg->x[l]; mg->x[l] = mg->xa[l]; mg->xa[l] = tmp;
       q += 2;
@@ -2089,7 +2162,7 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
 {
   int rc;
   if (!mg->tuned) {
-    if ((rc = tune_pairs<T>(mg, st))) return rc;
+    if ((rc = tune_pairs<T>(mg, st, bin, xout))) return rc;
     return mg_apply_seq<T>(mg, bin, xout, st, nullptr);
   }
   const bool timed = pfhip_prof_enabled(PFHIP_PROF_RBGS) ||

@@ -669,11 +669,12 @@ def test_multigrid_bottom_in_lds(gpu, precision, shape, monkeypatch):
 
 @pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("shape", [(70, 40, 24), (131, 20, 18)])
-@pytest.mark.parametrize("variant", list(range(10)))
+@pytest.mark.parametrize("variant", list(range(17)))
 def test_multigrid_pair_shapes(gpu, variant, shape, precision, monkeypatch):
     """Every smoother tile shape the autotuner chooses from (5 tile shapes,
-    each with the y neighbours loaded or taken through LDS), forced on all
-    grid levels (even and odd x extents), against the oracle V-cycle."""
+    each with the y neighbours loaded or taken through LDS, and the LDS form
+    loading two or three planes ahead), forced on all grid levels (even and
+    odd x extents), against the oracle V-cycle."""
     api = _api()
     monkeypatch.setenv("PFHIP_MG_SMALL", "300")
     monkeypatch.setenv("PFHIP_RBGS_VARIANT", str(variant))
@@ -692,6 +693,31 @@ def test_multigrid_pair_shapes(gpu, variant, shape, precision, monkeypatch):
     b[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
     err = _relmax(_I(prob, mgg.apply(b)), _I(prob, mgo.apply(b)))
     assert err < (1e-11 if precision == 64 else 2e-5), err
+
+
+@pytest.mark.parametrize("shape", [(70, 40, 24), (128, 30, 37)])
+def test_multigrid_pair_prefetch_depth_bitwise(gpu, shape, monkeypatch):
+    """The FP32 smoother pair loading its planes two or three planes ahead
+    (variants 10-16) performs the arithmetic of the one-plane-ahead form of
+    the same tile shape (variants 5-9) in the same order: the V-cycles agree
+    to the last bit."""
+    api = _api()
+    monkeypatch.setenv("PFHIP_MG_SMALL", "300")
+    prob = cases.hetero(*shape, seed=27, dir_sides=True)
+    print(f"mg pair prefetch depth shape={shape}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    J = O.jacobian_eval(op, cases.random_pressure(prob, seed=28), 0.1, 0.1, 1)
+    b = np.zeros(prob.shape)
+    b[1:-1, 1:-1, 1:-1] = np.random.default_rng(29).normal(size=(prob.nz, prob.ny, prob.nx))
+    out = {}
+    for v in (5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16):
+        monkeypatch.setenv("PFHIP_RBGS_VARIANT", str(v))
+        mg = api.GPUMultigrid(gp, 2, 2, 0, 32)
+        mg.setup(J)
+        out[v] = _I(prob, mg.apply(b))
+    for deep, base in ((10, 5), (11, 6), (12, 7), (13, 8), (14, 9), (15, 5), (16, 8)):
+        assert np.array_equal(out[deep], out[base]), (deep, base)
 
 
 @pytest.mark.parametrize("shape,dirs", [((11, 9, 6), True), ((150, 37, 40), False),
