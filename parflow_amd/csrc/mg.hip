@@ -630,12 +630,35 @@ __device__ __forceinline__ void stg_o(T *base, unsigned ip, T v)
   *(__attribute__((address_space(1))) T *)((char *)base + ip * (unsigned)sizeof(T)) = v;
 }
 
+// 16-byte FP64 pair access for interface vectors whose even-i points sit on
+// a 16-byte boundary (the solver's vectors that meet the multigrid start one
+// element past an aligned address, as the level arrays do)
+typedef V2T<double>::type D2;
+__device__ __forceinline__ D2 ldg_o2(const double *base, unsigned ip)
+{
+  return *(const __attribute__((address_space(1))) D2 *)((const char *)base + ip * 8u);
+}
+__device__ __forceinline__ void stg_o2(double *base, unsigned ip, D2 v)
+{
+  *(__attribute__((address_space(1))) D2 *)((char *)base + ip * 8u) = v;
+}
+template <typename T, bool A16>
+__device__ __forceinline__ typename V2T<T>::type ld_din(const double *din, unsigned ip)
+{
+  typedef typename V2T<T>::type T2;
+  if (A16) {
+    const D2 v = ldg_o2(din, ip);
+    return T2{(T)v.x, (T)v.y};
+  }
+  return T2{(T)ldg_o(din, ip), (T)ldg_o(din, ip + 1)};
+}
+
 template <typename T> struct PairSet {
   typedef typename V2T<T>::type T2;
   T2 c, bb, ae, an, as, au, xs, xn;
 };
 
-template <typename T, bool INIT, bool DIN = false>
+template <typename T, bool INIT, bool DIN = false, bool A16 = false>
 __device__ __forceinline__ void load_pair(PairSet<T> &S, const Mat<T> &A,
                                           const T *__restrict__ xin, const T *__restrict__ b,
                                           unsigned ip, unsigned sy,
@@ -643,7 +666,7 @@ __device__ __forceinline__ void load_pair(PairSet<T> &S, const Mat<T> &A,
 {
   typedef typename V2T<T>::type T2;
   S.c = ldo<T2>(A.c, ip);
-  if (DIN) S.bb = T2{(T)ldg_o(din, ip), (T)ldg_o(din, ip + 1)};
+  if (DIN) S.bb = ld_din<T, A16>(din, ip);
   else S.bb = ldo<T2>(b, ip);
   S.ae = ldo<T2>(A.u[0], ip);
   S.an = ldo<T2>(A.u[1], ip);
@@ -670,7 +693,7 @@ __device__ __forceinline__ typename V2T<T>::type load_x2(const T *__restrict__ x
 // the south coupling and x_in of the row below only for the tile's lowest
 // row (own_s), x_in of the row above only for its highest (own_n); the
 // north coupling is loaded two planes ahead by the caller
-template <typename T, bool INIT, bool DIN = false>
+template <typename T, bool INIT, bool DIN = false, bool A16 = false>
 __device__ __forceinline__ void load_pair_yl(PairSet<T> &S, const Mat<T> &A,
                                              const T *__restrict__ xin,
                                              const T *__restrict__ b, unsigned ip, unsigned sy,
@@ -679,7 +702,7 @@ __device__ __forceinline__ void load_pair_yl(PairSet<T> &S, const Mat<T> &A,
 {
   typedef typename V2T<T>::type T2;
   S.c = ldo<T2>(A.c, ip);
-  if (DIN) S.bb = T2{(T)ldg_o(din, ip), (T)ldg_o(din, ip + 1)};
+  if (DIN) S.bb = ld_din<T, A16>(din, ip);
   else S.bb = ldo<T2>(b, ip);
   S.ae = ldo<T2>(A.u[0], ip);
   S.au = ldo<T2>(A.u[2], ip);
@@ -723,8 +746,11 @@ __device__ __forceinline__ void load_pair_yl(PairSet<T> &S, const Mat<T> &A,
 // PF (YL only): the coefficients are loaded PF planes ahead (x_in and the
 // north couplings PF + 1), so PF planes of loads are in flight while a
 // plane is updated; the arithmetic and its order do not depend on PF.
+//
+// A16 (DIN or DOUT): the FP64 vector's even-i points are 16-byte aligned,
+// so a lane reads or writes its two values with one 16-byte access.
 template <typename T, int TY, int KC, bool INIT, bool DOUT = false, bool DIN = false,
-          bool YL = false, int PF = 1>
+          bool YL = false, int PF = 1, bool A16 = false>
 __global__ __launch_bounds__(64 * (TY + 2), YL ? (PF > 1 ? PAIR2_PF_MIN_WAVES(T)
                                                          : PAIR2_YL_MIN_WAVES(T))
                                                : PAIR2_MIN_WAVES(T))
@@ -788,15 +814,16 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
 #pragma unroll
   for (int d = 0; d <= PF; d++) anr[d] = T2{T(0), T(0)};
   if (YL) {
-    load_pair_yl<T, INIT, DIN>(cur, A, xin, b, at(k0), sy, own_s, own_n, din);
+    load_pair_yl<T, INIT, DIN, A16>(cur, A, xin, b, at(k0), sy, own_s, own_n, din);
     cur.an = ldo<T2>(A.u[1], at(k0));
 #pragma unroll
     for (int d = 0; d + 1 < PF; d++)
-      load_pair_yl<T, INIT, DIN>(ahead[d], A, xin, b, at(k0 + 1 + d), sy, own_s, own_n, din);
+      load_pair_yl<T, INIT, DIN, A16>(ahead[d], A, xin, b, at(k0 + 1 + d), sy, own_s, own_n,
+                                      din);
 #pragma unroll
     for (int d = 0; d < PF; d++) anr[d] = ldo<T2>(A.u[1], at(k0 + 1 + d));
   } else {
-    load_pair<T, INIT, DIN>(cur, A, xin, b, at(k0), sy, din);
+    load_pair<T, INIT, DIN, A16>(cur, A, xin, b, at(k0), sy, din);
   }
   T2 al = ldo<T2>(A.u[2], at(k0 - 1));               // lower coupling of plane k0
 #pragma unroll
@@ -813,10 +840,11 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
     // k+PF+1 (YL: the north couplings of plane k+PF+1 as well, published
     // one plane before their own)
     if (YL) {
-      load_pair_yl<T, INIT, DIN>(ahead[PF - 1], A, xin, b, at(k + PF), sy, own_s, own_n, din);
+      load_pair_yl<T, INIT, DIN, A16>(ahead[PF - 1], A, xin, b, at(k + PF), sy, own_s, own_n,
+                                      din);
       anr[PF] = ldo<T2>(A.u[1], at(k + PF + 1));
     } else {
-      load_pair<T, INIT, DIN>(ahead[0], A, xin, b, at(k + 1), sy, din);
+      load_pair<T, INIT, DIN, A16>(ahead[0], A, xin, b, at(k + 1), sy, din);
     }
     xr[PF + 2] = load_x2<T, INIT>(xin, at(k + PF + 1));
     const T2 xm = xr[0], x0 = xr[1], xp = xr[2];
@@ -871,7 +899,9 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
                        sal, rm2, kk > 0, sau, rk, kk + 1 < nz);
       if (owned && kk >= kb && kk < ke) {
         T2 out = p0red ? T2{vb, rm1} : T2{rm1, vb};
-        if (DOUT) {
+        if (DOUT && A16) {
+          stg_o2(dout, ip - sz, D2{(double)out.x, (double)out.y});
+        } else if (DOUT) {
           stg_o(dout, ip - sz, (double)out.x);
           stg_o(dout, ip - sz + 1, (double)out.y);
         } else {
@@ -1466,6 +1496,9 @@ struct pfhip_mg {
   int cz;                      // z-marching coarse operators (k_coarse_z)
   int pv[MAXL];                // pair tile shape of each grid level (tune_pairs)
   int pv_din, pv_dout;         // level 0's shapes for its FP64-input and FP64-result pairs
+  int a16;                     // bit 0 / 1: the V-cycle's FP64 input / result vector has
+                               // its even-i points 16-byte aligned (set per call)
+  int gx_a16;                  // a16 of the vectors the graph gx was captured for
   int tuned;                   // pv chosen (at the first V-cycle)
   void *levdev;                // device LevelDev<T>[L]
   long lds_bytes;              // > 0: the bottom runs in LDS (k_coarse_vcycle_lds)
@@ -1800,11 +1833,15 @@ static void launch_pair_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout
 template <typename T, int TY, int KC, bool YL, int PF = 1>
 static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
                            int init, hipStream_t st, double *dout, const double *din,
-                           void *const *io)
+                           void *const *io, bool a16)
 {
   unsigned gx = (g.n[0] + PX2 - 1) / PX2, gy = (g.n[1] + TY - 1) / TY, gz = (g.n[2] + KC - 1) / KC;
   unsigned nb = gx * gy * gz, nb8 = (nb + 7u) & ~7u;
-  if (init && din)
+  if (init && din && a16)
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true, YL, PF, true>), dim3(nb8),
+                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, din,
+                       (T *)b, io);
+  else if (init && din)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true, YL, PF>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, din,
                        (T *)b, io);
@@ -1812,6 +1849,9 @@ static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xou
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, false, YL, PF>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr,
                        io);
+  else if (dout && a16)
+    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true, false, YL, PF, true>), dim3(nb8),
+                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr, io);
   else if (dout)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true, false, YL, PF>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr, io);
@@ -1827,37 +1867,39 @@ static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xou
 // three planes ahead with (14, 16) and (14, 8);
 // dout (even x extents, non-initial pairs only) takes the result in FP64,
 // din (even x extents, initial pairs only) gives the right-hand side in
-// FP64 and b receives it in the level precision
+// FP64 and b receives it in the level precision; a16: that FP64 vector's
+// even-i points are 16-byte aligned
 template <typename T>
 static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
                         int init, int v, hipStream_t st, double *dout = nullptr,
-                        const double *din = nullptr, void *const *io = nullptr)
+                        const double *din = nullptr, void *const *io = nullptr,
+                        bool a16 = false)
 {
   if ((g.n[0] % 2) == 0) {
     // two points per lane (aligned vector loads need an even x extent)
     switch (v) {
-      case 1: launch_pair2_t<T, 6, 16, false>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 2: launch_pair2_t<T, 6, 32, false>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 3: launch_pair2_t<T, 14, 8, false>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 4: launch_pair2_t<T, 14, 4, false>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 5: launch_pair2_t<T, 14, 16, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 6: launch_pair2_t<T, 6, 16, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 7: launch_pair2_t<T, 6, 32, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 8: launch_pair2_t<T, 14, 8, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 9: launch_pair2_t<T, 14, 4, true>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 1: launch_pair2_t<T, 6, 16, false>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 2: launch_pair2_t<T, 6, 32, false>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 3: launch_pair2_t<T, 14, 8, false>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 4: launch_pair2_t<T, 14, 4, false>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 5: launch_pair2_t<T, 14, 16, true>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 6: launch_pair2_t<T, 6, 16, true>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 7: launch_pair2_t<T, 6, 32, true>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 8: launch_pair2_t<T, 14, 8, true>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 9: launch_pair2_t<T, 14, 4, true>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
       default: break;
     }
     if constexpr (sizeof(T) == 4) switch (v) {
-      case 10: launch_pair2_t<T, 14, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 11: launch_pair2_t<T, 6, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 12: launch_pair2_t<T, 6, 32, true, 2>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 13: launch_pair2_t<T, 14, 8, true, 2>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 14: launch_pair2_t<T, 14, 4, true, 2>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 15: launch_pair2_t<T, 14, 16, true, 3>(g, A, xin, xout, b, init, st, dout, din, io); return;
-      case 16: launch_pair2_t<T, 14, 8, true, 3>(g, A, xin, xout, b, init, st, dout, din, io); return;
+      case 10: launch_pair2_t<T, 14, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 11: launch_pair2_t<T, 6, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 12: launch_pair2_t<T, 6, 32, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 13: launch_pair2_t<T, 14, 8, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 14: launch_pair2_t<T, 14, 4, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 15: launch_pair2_t<T, 14, 16, true, 3>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 16: launch_pair2_t<T, 14, 8, true, 3>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
       default: break;
     }
-    launch_pair2_t<T, 14, 16, false>(g, A, xin, xout, b, init, st, dout, din, io);
+    launch_pair2_t<T, 14, 16, false>(g, A, xin, xout, b, init, st, dout, din, io, a16);
     return;
   }
   switch (v % 5) {
@@ -1898,14 +1940,15 @@ static int tune_pairs(pfhip_mg *mg, hipStream_t st, const double *bin, double *x
       double *dout = form == 2 ? xout : nullptr;
       const double *din = form == 1 ? bin : nullptr;
       int *pick = form == 0 ? &mg->pv[l] : form == 1 ? &mg->pv_din : &mg->pv_dout;
+      const bool a16 = form == 1 ? (mg->a16 & 1) : form == 2 ? (mg->a16 & 2) : false;
       float best = 0.f;
       for (int v = 0; v < npairv<T>(); v++) {
         launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], init, v,
-                       st, dout, din);
+                       st, dout, din, nullptr, a16);
         HIPCHK(hipEventRecord(e0, st));
         for (int r = 0; r < TUNE_REPS; r++)
           launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], init,
-                         v, st, dout, din);
+                         v, st, dout, din, nullptr, a16);
         HIPCHK(hipEventRecord(e1, st));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -1973,8 +2016,9 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
     if (mark >= 0) pfhip_prof_mark(mark, 0, st);
     if (pair) {
       const int v = (l == 0 && d) ? mg->pv_dout : (l == 0 && di) ? mg->pv_din : mg->pv[l];
+      const bool a16 = d ? (mg->a16 & 2) : di ? (mg->a16 & 1) : false;
       launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, v, st, d, di,
-                     io);
+                     io, a16);
       if (d && wrote) *wrote = 1;
       void *tmp = mg->x[l]; mg->x[l] = mg->xa[l]; mg->xa[l] = tmp;
       q += 2;
@@ -2145,6 +2189,7 @@ static int mg_capture(pfhip_mg *mg)
   }
   e = hipGraphInstantiate(&mg->gx, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
+  mg->gx_a16 = mg->a16;
   if (e != hipSuccess) {
     mg->gx = nullptr;
     pfhip_set_error("V-cycle graph instantiation: %s", hipGetErrorString(e));
@@ -2161,6 +2206,9 @@ template <typename T>
 static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t st)
 {
   int rc;
+  // an even i has an odd ghosted index, so its FP64 value pair is 16-byte
+  // aligned when the vector starts 8 bytes past a 16-byte boundary
+  mg->a16 = ((((uintptr_t)bin) & 15) == 8 ? 1 : 0) | ((((uintptr_t)xout) & 15) == 8 ? 2 : 0);
   if (!mg->tuned) {
     if ((rc = tune_pairs<T>(mg, st, bin, xout))) return rc;
     return mg_apply_seq<T>(mg, bin, xout, st, nullptr);
@@ -2169,6 +2217,9 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
                      pfhip_prof_enabled(PFHIP_PROF_RBGS_DOUT) ||
                      pfhip_prof_enabled(PFHIP_PROF_RBGS_DIN);
   if (mg->use_graph && !timed) {
+    // the captured launches read the vectors with the access width of the
+    // vectors they were captured for: others are launched directly
+    if (mg->gx && mg->gx_a16 != mg->a16) return mg_apply_seq<T>(mg, bin, xout, st, nullptr);
     if (!mg->gx && (rc = mg_capture<T>(mg))) return rc;
     if (mg->gx) {
       hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, st, mg->io, bin, xout);
