@@ -1053,12 +1053,16 @@ struct Coefs {
 };
 
 // w += sum_q c[q] V[q] (q < M); optional ||w||^2 partials
-template <int M, bool NORM>
+// w += sum_q C[q] V[q] (q < M); NORM: ||w||^2 partials; STEP (with NORM):
+// also max |w| / (1 + |u|) partials (RED_SCSTEP's term with x = u, y = w) at
+// partial[nb + block]; ZERO: w starts at 0 and is not read
+template <int M, bool NORM, bool STEP = false, bool ZERO = false>
 __global__ __launch_bounds__(256) void k_multi_axpy(Geom g, VecPtrs V, Coefs C,
                                                     double *__restrict__ w,
-                                                    double *__restrict__ partial, int nb)
+                                                    double *__restrict__ partial, int nb,
+                                                    const double *__restrict__ u = nullptr)
 {
-  double acc = 0.0;
+  double acc = 0.0, mx = 0.0;
   int rows = g.ny * g.nz;
   for (int r = blockIdx.x; r < rows; r += gridDim.x) {
     int j = r % g.ny, k = r / g.ny;
@@ -1068,16 +1072,21 @@ __global__ __launch_bounds__(256) void k_multi_axpy(Geom g, VecPtrs V, Coefs C,
       double v[M];
 #pragma unroll
       for (int q = 0; q < M; q++) v[q] = V.v[q][ip];
-      double s = w[ip];
+      double s = ZERO ? 0.0 : w[ip];
 #pragma unroll
       for (int q = 0; q < M; q++) s += C.c[q] * v[q];
       w[ip] = s;
       if (NORM) acc += s * s;
+      if (STEP) mx = fmax(mx, fabs(s / (1.0 + fabs(u[ip]))));
     }
   }
   if (NORM) {
     double s = block_reduce<false>(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+  if (STEP) {
+    double m = block_reduce<true>(mx);
+    if (threadIdx.x == 0) partial[nb + blockIdx.x] = m;
   }
 }
 
@@ -1283,6 +1292,61 @@ extern "C" int pfhip_vec_combine(pfhip_problem *pb, const double *const *V, int 
                                  const double *y, double *x, void *stream)
 {
   return multi_axpy_impl(pb, V, m, y, 1.0, x, nullptr, stream);
+}
+
+// x = sum_k y[k] V[k] (zero != 0: x's contents are not read) or x +=, and
+// the line search's two terms of the result from the same pass: x.x (the
+// products and order of pfhip_vec_dot(x, x)) and max |x| / (1 + |u|) (those
+// of pfhip_vec_scsteplength(u, x)); one synchronisation
+extern "C" int pfhip_vec_combine_step(pfhip_problem *pb, const double *const *V, int m,
+                                      const double *y, double *x, const double *u, int zero,
+                                      double *xx, double *steplength, void *stream)
+{
+  if (m < 1) {
+    pfhip_set_error("vec_combine_step: no vectors");
+    return PFHIP_ERR_ARG;
+  }
+  Geom g = geom_of(pb);
+  int nb = red_blocks(g);
+  hipStream_t st = STREAM(stream);
+  for (int c0 = 0; c0 < m; c0 += MD_CHUNK) {
+    int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
+    VecPtrs P;
+    Coefs C;
+    for (int q = 0; q < MD_CHUNK; q++) {
+      P.v[q] = (q < mc) ? V[c0 + q] : x;
+      C.c[q] = (q < mc) ? y[c0 + q] : 0.0;
+    }
+    const bool last = (c0 + MD_CHUNK >= m), z = zero && c0 == 0;
+    if (last && z) {
+#define L_MA(n) hipLaunchKernelGGL((k_multi_axpy<n, true, true, true>), dim3(nb), dim3(256), 0, st, g, P, C, x, pb->red_partial, nb, u)
+      MD_SWITCH(mc, L_MA)
+#undef L_MA
+    } else if (last) {
+#define L_MA(n) hipLaunchKernelGGL((k_multi_axpy<n, true, true, false>), dim3(nb), dim3(256), 0, st, g, P, C, x, pb->red_partial, nb, u)
+      MD_SWITCH(mc, L_MA)
+#undef L_MA
+    } else if (z) {
+#define L_MA(n) hipLaunchKernelGGL((k_multi_axpy<n, false, false, true>), dim3(nb), dim3(256), 0, st, g, P, C, x, pb->red_partial, nb, nullptr)
+      MD_SWITCH(mc, L_MA)
+#undef L_MA
+    } else {
+#define L_MA(n) hipLaunchKernelGGL((k_multi_axpy<n, false>), dim3(nb), dim3(256), 0, st, g, P, C, x, pb->red_partial, nb, nullptr)
+      MD_SWITCH(mc, L_MA)
+#undef L_MA
+    }
+    LAUNCHCHK();
+  }
+  hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, st, pb->red_partial, nb, 1,
+                     pb->red_dev);
+  hipLaunchKernelGGL(k_reduce_final<true>, dim3(1), dim3(256), 0, st, pb->red_partial + nb,
+                     nb, 1, pb->red_dev + 1);
+  LAUNCHCHK();
+  int rc = fetch_results(pb, 2, stream);
+  if (rc) return rc;
+  *xx = pb->h_red[0];
+  *steplength = pb->h_red[1];
+  return PFHIP_OK;
 }
 
 // MGS step: w -= hprev*vprev (if vprev) and dot = vnext.w (if vnext)

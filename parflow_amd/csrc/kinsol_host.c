@@ -61,6 +61,10 @@ struct pfhip_solver {
    * fval), when the residual launch that produced them also gathered them */
   int stop_valid;
   double stop_fmax, stop_steplength;
+  /* pp.pp and pp's scaled step length from uu, gathered by the pass that
+   * formed pp (the flexible GMRES's solution update) */
+  int pp_stats_valid;
+  double pp_dot, pp_steplength;
 };
 
 #define CHK(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
@@ -72,7 +76,7 @@ const pfhip_ops pfhip_direct_ops = {
   pfhip_residual_jv, pfhip_residual_step, pfhip_mg_setup_jacobian, pfhip_mg_apply,
   pfhip_bc_pressure, pfhip_saturation, pfhip_synchronize, pfhip_residual_jv_dq,
   pfhip_vec_cgs2, pfhip_vec_dot_scsteplength, pfhip_residual_step_stats,
-  pfhip_vec_kinsol_init_norms,
+  pfhip_vec_kinsol_init_norms, pfhip_vec_combine_step,
 };
 
 void pfhip_solver_set_ops(pfhip_solver *s, const pfhip_ops *ops) { s->ops = ops; }
@@ -403,11 +407,16 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fn
   int preOnRight = s->o.precond != 0;
   double beta, rotation_product, r_norm, s_product, rho = 0;
   int krydim = 0, converged = 0;
+  /* the flexible variant with the fused update leaves x unwritten until
+   * its first update overwrites it; returns before that zero it */
+  int x_unset = flex && s->ops->vec_combine_step;
+#define X_ZERO_IF_UNSET() \
+  do { if (x_unset) { CHK(s->ops->vec_const(pb, 0.0, x, NULL)); x_unset = 0; } } while (0)
   *nli = 0;
   if (fneg_norm >= 0.0) r_norm = fneg_norm;
   else CHK(l2norm(s, fneg, &r_norm));
   *res_norm = beta = r_norm;
-  if (r_norm <= delta) { *ret = SPGMR_SUCCESS; return PFHIP_OK; }
+  if (r_norm <= delta) { X_ZERO_IF_UNSET(); *ret = SPGMR_SUCCESS; return PFHIP_OK; }
   if (!flex) CHK(s->ops->vec_const(pb, 0.0, s->xcor, NULL));
 
   for (int ntries = 0; ntries <= s->max_restarts; ntries++) {
@@ -434,7 +443,11 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fn
       else CHK(modified_gs(s, l + 1, &Hes[l + 1][l]));
       if (flex && ntries == 0)
         for (int i = 0; i <= l + 1; i++) s->Hraw[(size_t)i * l_max + l] = Hes[i][l];
-      if (qrfact(krydim, Hes, s->givens) != 0) { *ret = SPGMR_QRFACT_FAIL; return PFHIP_OK; }
+      if (qrfact(krydim, Hes, s->givens) != 0) {
+        X_ZERO_IF_UNSET();
+        *ret = SPGMR_QRFACT_FAIL;
+        return PFHIP_OK;
+      }
       rotation_product *= s->givens[2 * l + 1];
       *res_norm = rho = fabs(rotation_product * r_norm);
       if (rho <= delta) { converged = 1; break; }
@@ -442,12 +455,25 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fn
     }
     s->yg[0] = r_norm;
     for (int i = 1; i <= krydim; i++) s->yg[i] = 0.0;
-    if (qrsol(krydim, Hes, s->givens, s->yg) != 0) { *ret = SPGMR_QRFACT_FAIL; return PFHIP_OK; }
+    if (qrsol(krydim, Hes, s->givens, s->yg) != 0) {
+      X_ZERO_IF_UNSET();
+      *ret = SPGMR_QRFACT_FAIL;
+      return PFHIP_OK;
+    }
     if (flex) {
       /* x += Z y directly (flexible GMRES); without a preconditioner the
        * search directions are the basis vectors themselves (Z = V) */
       double **Zs = preOnRight ? s->Z : V;
-      CHK(s->ops->vec_combine(pb, (const double *const *)Zs, krydim, s->yg, x, NULL));
+      if (s->ops->vec_combine_step) {
+        /* x = Z y in the first cycle (x not read), with the line search's
+         * x.x and scaled step length from the same pass */
+        CHK(s->ops->vec_combine_step(pb, (const double *const *)Zs, krydim, s->yg, x, s->uu,
+                                     x_unset, &s->pp_dot, &s->pp_steplength, NULL));
+        x_unset = 0;
+        s->pp_stats_valid = 1;
+      } else {
+        CHK(s->ops->vec_combine(pb, (const double *const *)Zs, krydim, s->yg, x, NULL));
+      }
       if (ntries == 0) {
         /* Arnoldi relation of this cycle: J Z = V Hbar, so with x = Z y
          * (x started at 0), J x = V g with g = Hbar y; V has orthonormal
@@ -503,6 +529,7 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fn
   }
   *ret = SPGMR_CONV_FAIL;
   return PFHIP_OK;
+#undef X_ZERO_IF_UNSET
 }
 
 /* KINSpgmrSolve KS/kinspgmr.c:344-420 for J xx = -fval; bb receives J xx;
@@ -512,7 +539,9 @@ static int lsolve(pfhip_solver *s, double *xx, double *bb, int *lret)
   pfhip_problem *pb = s->pb;
   int nli_inc, sret;
   double res_norm;
-  CHK(s->ops->vec_const(pb, 0.0, xx, NULL));
+  s->pp_stats_valid = 0;
+  /* the flexible variant's fused update writes xx without reading it */
+  if (!(s->o.krylov == 1 && s->ops->vec_combine_step)) CHK(s->ops->vec_const(pb, 0.0, xx, NULL));
   s->new_uu = 1;
   s->jp_ok = 0;
   /* s->fnorm is l2norm(fval) of the current iterate: the line search (or
@@ -602,7 +631,12 @@ static int linesearch(pfhip_solver *s, double *fnormp, double *f1normp, int *max
   *maxStep = 0;
   *gret = 0;
   int have_rlength = 0;
-  if (s->ops->vec_dot_scsteplength) {
+  if (s->pp_stats_valid) {
+    /* gathered by the pass that formed pp, from the same uu */
+    pnorm = sqrt(s->pp_dot);
+    rlength = s->pp_steplength;
+    have_rlength = 1;
+  } else if (s->ops->vec_dot_scsteplength) {
     /* ||pp|| and the scaled step length from one pass; the step length is
      * taken again if pp is scaled down */
     double pp2;
@@ -689,7 +723,8 @@ static int inexact_newton(pfhip_solver *s, double *fnormp, double *f1normp, int 
 {
   double pnorm, ratio = 1.0;
   *maxStep = 0;
-  CHK(l2norm(s, s->pp, &pnorm));
+  if (s->pp_stats_valid) pnorm = sqrt(s->pp_dot);
+  else CHK(l2norm(s, s->pp, &pnorm));
   if (pnorm > s->mxnewtstep) {
     ratio = s->mxnewtstep / pnorm;
     CHK(s->ops->vec_scale(s->pb, ratio, s->pp, s->pp, NULL));

@@ -67,6 +67,11 @@ int pfhip_vec_combine(pfhip_problem *pb, const double *const *V, int m,
  * l2norm and KINScSteplength of KINLineSearch's start) */
 int pfhip_vec_dot_scsteplength(pfhip_problem *pb, const double *u, const double *p,
                                double *pp, double *rlength, void *stream);
+/* x = sum y[k] V[k] (zero: x not read) or x +=, with x.x and
+ * max |x| / (1 + |u|) from the same pass */
+int pfhip_vec_combine_step(pfhip_problem *pb, const double *const *V, int m,
+                           const double *y, double *x, const double *u, int zero,
+                           double *xx, double *steplength, void *stream);
 /* KINSOL's start: max |f|, f.f and u.u in one pass */
 int pfhip_vec_kinsol_init_norms(pfhip_problem *pb, const double *u, const double *f,
                                 double *fmax, double *ff, double *uu, void *stream);
@@ -171,6 +176,8 @@ typedef struct pfhip_ops {
                              double *, double *, double *, void *);
   int (*kinsol_init_norms)(pfhip_problem *, const double *, const double *, double *,
                            double *, double *, void *);
+  int (*vec_combine_step)(pfhip_problem *, const double *const *, int, const double *,
+                          double *, const double *, int, double *, double *, void *);
 } pfhip_ops;
 
 extern const pfhip_ops pfhip_direct_ops;

@@ -844,6 +844,50 @@ def test_line_search_stats_match_separate_passes(gpu, shape, dirs):
     assert fm.value == np.max(np.abs(_I(prob, gp.get(F_a))))
 
 
+@pytest.mark.parametrize("m", [1, 5, 16, 23])
+def test_combine_step_matches_separate_passes(gpu, m):
+    """The flexible GMRES's solution update with the line search's terms:
+    pfhip_vec_combine_step writing x = sum y_k Z_k (x not read) or adding
+    to x, with x.x and max |x| / (1 + |u|), against pfhip_vec_const +
+    pfhip_vec_combine + pfhip_vec_dot + pfhip_vec_scsteplength, bit for bit
+    (KS/spgmr.c:420-450, KS/kinsol.c:1240-1262); m > 16 takes two chunks."""
+    import ctypes as C
+    api = _api()
+    L = api.lib()
+    prob = cases.hetero(37, 21, 11, seed=51, dir_sides=True)
+    print(f"combine_step m={m}", flush=True)
+    gp = api.GPUProblem(prob)
+    rng = np.random.default_rng(52 + m)
+    Z = [gp.vec(rng.normal(size=prob.shape)) for _ in range(m)]
+    u = gp.vec(rng.normal(0, 3, size=prob.shape))
+    y = (C.c_double * m)(*rng.normal(size=m))
+    zp = (C.c_void_p * m)(*[z.ptr.value for z in Z])
+    vp, dbl = C.c_void_p, C.c_double
+    L.pfhip_vec_combine_step.argtypes = [vp, vp, C.c_int, vp, vp, vp, C.c_int, vp, vp, vp]
+    L.pfhip_vec_combine.argtypes = [vp, vp, C.c_int, vp, vp, vp]
+    L.pfhip_vec_const.argtypes = [vp, dbl, vp, vp]
+    L.pfhip_vec_dot.argtypes = [vp, vp, vp, vp, vp]
+    L.pfhip_vec_scsteplength.argtypes = [vp, vp, vp, vp, vp]
+    x0 = rng.normal(size=prob.shape)
+    for zero in (1, 0):
+        # x holds garbage (zero = 1, never read) or a start value (zero = 0)
+        xa = gp.vec(x0 if not zero else rng.normal(size=prob.shape) * 1e300)
+        xb = gp.vec(x0)
+        if zero:
+            api.check(L.pfhip_vec_const(gp.h, 0.0, xb.ptr, None), "const")
+        d, st = dbl(), dbl()
+        api.check(L.pfhip_vec_combine_step(gp.h, zp, m, y, xa.ptr, u.ptr, zero, C.byref(d),
+                                           C.byref(st), None), "combine_step")
+        api.check(L.pfhip_vec_combine(gp.h, zp, m, y, xb.ptr, None), "combine")
+        d2, st2 = dbl(), dbl()
+        api.check(L.pfhip_vec_dot(gp.h, xb.ptr, xb.ptr, C.byref(d2), None), "dot")
+        api.check(L.pfhip_vec_scsteplength(gp.h, u.ptr, xb.ptr, C.byref(st2), None), "scstep")
+        ia, ib = _I(prob, gp.get(xa)), _I(prob, gp.get(xb))
+        assert np.array_equal(ia, ib), (zero, _relmax(ia, ib))
+        assert (d.value, st.value) == (d2.value, st2.value), zero
+        assert st.value == np.max(np.abs(ib / (1.0 + np.abs(_I(prob, gp.get(u))))))
+
+
 @pytest.mark.parametrize("overland,dirs", [(False, True), (True, False)])
 def test_residual_multi_tile(gpu, overland, dirs):
     """Residual on a grid spanning several 64-wide tiles, ragged tile rows
