@@ -970,9 +970,24 @@ extern "C" int pfhip_water_balance_storage_noalloc(pfhip_problem *pb, const doub
 // ---- block Gram-Schmidt kernels ------------------------------------------
 #define MD_CHUNK 16
 
+// the vectors of a multi-vector pass and a factor per vector: a pass reads
+// sc[q] * v[q][i], the value a separate scaling pass (k_scale) would have
+// stored, so a basis vector can be kept unnormalised with its 1/norm beside
+// it (1.0 where a vector is used as stored: x * 1.0 is x)
 struct VecPtrs {
   const double *v[MD_CHUNK];
+  double sc[MD_CHUNK];
 };
+static VecPtrs vec_ptrs(const double *const *V, const double *scale, int c0, int mc,
+                        const double *fill)
+{
+  VecPtrs P;
+  for (int q = 0; q < MD_CHUNK; q++) {
+    P.v[q] = (q < mc) ? V[c0 + q] : fill;
+    P.sc[q] = (q < mc && scale) ? scale[c0 + q] : 1.0;
+  }
+  return P;
+}
 
 // The multi-vector kernels take the vector count M as a template argument:
 // with a run-time count every load sat behind its own branch and waited for
@@ -1006,7 +1021,7 @@ __global__ __launch_bounds__(256) void k_multi_dot(Geom g, VecPtrs V,
       double wv = w[ip];
       double v[M];
 #pragma unroll
-      for (int q = 0; q < M; q++) v[q] = V.v[q][ip];
+      for (int q = 0; q < M; q++) v[q] = V.sc[q] * V.v[q][ip];
 #pragma unroll
       for (int q = 0; q < M; q++) acc[q] += v[q] * wv;
     }
@@ -1034,8 +1049,7 @@ extern "C" int pfhip_vec_multi_dot(pfhip_problem *pb, const double *const *V,
   int nb = red_blocks(g);
   for (int c0 = 0; c0 < m; c0 += MD_CHUNK) {
     int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
-    VecPtrs P;
-    for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < mc) ? V[c0 + q] : w;
+    VecPtrs P = vec_ptrs(V, nullptr, c0, mc, w);
     launch_multi_dot(mc, nb, STREAM(stream), g, P, w, pb->red_partial + (long)c0 * nb);
     LAUNCHCHK();
   }
@@ -1071,7 +1085,7 @@ __global__ __launch_bounds__(256) void k_multi_axpy(Geom g, VecPtrs V, Coefs C,
       int ip = base + i;
       double v[M];
 #pragma unroll
-      for (int q = 0; q < M; q++) v[q] = V.v[q][ip];
+      for (int q = 0; q < M; q++) v[q] = V.sc[q] * V.v[q][ip];
       double s = ZERO ? 0.0 : w[ip];
 #pragma unroll
       for (int q = 0; q < M; q++) s += C.c[q] * v[q];
@@ -1098,12 +1112,9 @@ static int multi_axpy_impl(pfhip_problem *pb, const double *const *V, int m,
   int nb = red_blocks(g);
   for (int c0 = 0; c0 < m; c0 += MD_CHUNK) {
     int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
-    VecPtrs P;
+    VecPtrs P = vec_ptrs(V, nullptr, c0, mc, w);
     Coefs C;
-    for (int q = 0; q < MD_CHUNK; q++) {
-      P.v[q] = (q < mc) ? V[c0 + q] : w;
-      C.c[q] = (q < mc) ? sign * coef[c0 + q] : 0.0;
-    }
+    for (int q = 0; q < MD_CHUNK; q++) C.c[q] = (q < mc) ? sign * coef[c0 + q] : 0.0;
     bool last = (c0 + MD_CHUNK >= m);
     hipStream_t st = STREAM(stream);
     if (last && norm2_out) {
@@ -1149,7 +1160,7 @@ __global__ __launch_bounds__(256) void k_multi_axpy_d(Geom g, VecPtrs V,
       int ip = base + i;
       double v[M];
 #pragma unroll
-      for (int q = 0; q < M; q++) v[q] = V.v[q][ip];
+      for (int q = 0; q < M; q++) v[q] = V.sc[q] * V.v[q][ip];
       double s = w[ip];
 #pragma unroll
       for (int q = 0; q < M; q++) s += c[q] * v[q];
@@ -1183,7 +1194,7 @@ __global__ __launch_bounds__(256) void k_multi_axpy_dot(Geom g, VecPtrs V,
       int ip = base + i;
       double v[M];
 #pragma unroll
-      for (int q = 0; q < M; q++) v[q] = V.v[q][ip];
+      for (int q = 0; q < M; q++) v[q] = V.sc[q] * V.v[q][ip];
       double s = w[ip];
 #pragma unroll
       for (int q = 0; q < M; q++) s += c[q] * v[q];
@@ -1201,12 +1212,12 @@ __global__ __launch_bounds__(256) void k_multi_axpy_dot(Geom g, VecPtrs V,
 
 // one device pass of h = V^T w (m values into dst)
 static void multi_dot_dev(pfhip_problem *pb, const Geom &g, int nb, const double *const *V,
-                          int m, const double *w, double *dst, hipStream_t st)
+                          const double *sc, int m, const double *w, double *dst,
+                          hipStream_t st)
 {
   for (int c0 = 0; c0 < m; c0 += MD_CHUNK) {
     int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
-    VecPtrs P;
-    for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < mc) ? V[c0 + q] : w;
+    VecPtrs P = vec_ptrs(V, sc, c0, mc, w);
     launch_multi_dot(mc, nb, st, g, P, w, pb->red_partial + (long)c0 * nb);
   }
   hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, st, pb->red_partial, nb, m,
@@ -1216,12 +1227,12 @@ static void multi_dot_dev(pfhip_problem *pb, const Geom &g, int nb, const double
 // w -= V h with h on the device; the last chunk optionally adds ||w||^2
 // partials, reduced into *n2dst
 static void multi_axpy_dev(pfhip_problem *pb, const Geom &g, int nb, const double *const *V,
-                           int m, const double *h, double *w, double *n2dst, hipStream_t st)
+                           const double *sc, int m, const double *h, double *w,
+                           double *n2dst, hipStream_t st)
 {
   for (int c0 = 0; c0 < m; c0 += MD_CHUNK) {
     int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
-    VecPtrs P;
-    for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < mc) ? V[c0 + q] : w;
+    VecPtrs P = vec_ptrs(V, sc, c0, mc, w);
     bool last = (c0 + MD_CHUNK >= m);
     if (last && n2dst) {
 #define L_MAD(n) hipLaunchKernelGGL((k_multi_axpy_d<n, true>), dim3(nb), dim3(256), 0, st, g, P, h + c0, w, pb->red_partial, nb)
@@ -1241,9 +1252,11 @@ static void multi_axpy_dev(pfhip_problem *pb, const Geom &g, int nb, const doubl
 // CGS2 against V[0..k-1] with one host synchronisation: h1 = V^T w,
 // w -= V h1, h2 = V^T w, w -= V h2 and ||w||^2, the coefficients passing
 // from each reduction to the next update on the device.  The arithmetic is
-// that of the separate multi-dot / multi-axpy calls.
-int pfhip_vec_cgs2(pfhip_problem *pb, const double *const *V, int k, double *w, double *h1,
-                   double *h2, double *norm2, void *stream)
+// that of the separate multi-dot / multi-axpy calls.  sc (or null): basis
+// vector q is read as sc[q] * V[q], the vector a k_scale pass by sc[q]
+// would have stored.
+int pfhip_vec_cgs2_scaled(pfhip_problem *pb, const double *const *V, const double *sc, int k,
+                          double *w, double *h1, double *h2, double *norm2, void *stream)
 {
   if (k < 1 || k > PFHIP_SLOT_H2) {
     pfhip_set_error("cgs2: k=%d outside 1..%d", k, PFHIP_SLOT_H2);
@@ -1254,11 +1267,10 @@ int pfhip_vec_cgs2(pfhip_problem *pb, const double *const *V, int k, double *w, 
   hipStream_t st = STREAM(stream);
   double *d1 = pb->red_dev, *d2 = pb->red_dev + PFHIP_SLOT_H2;
   double *dn = pb->red_dev + PFHIP_SLOT_N2;
-  multi_dot_dev(pb, g, nb, V, k, w, d1, st);
+  multi_dot_dev(pb, g, nb, V, sc, k, w, d1, st);
   if (k <= 8) {
     // w -= V h1 and h2 = V^T w in one pass over V
-    VecPtrs P;
-    for (int q = 0; q < MD_CHUNK; q++) P.v[q] = (q < k) ? V[q] : w;
+    VecPtrs P = vec_ptrs(V, sc, 0, k, w);
 #define L_MADOT(n) hipLaunchKernelGGL(k_multi_axpy_dot<n>, dim3(nb), dim3(256), 0, st, g, P, d1, w, pb->red_partial, nb)
     switch (k) {
       MD_CASE(1, L_MADOT) MD_CASE(2, L_MADOT) MD_CASE(3, L_MADOT) MD_CASE(4, L_MADOT)
@@ -1269,16 +1281,22 @@ int pfhip_vec_cgs2(pfhip_problem *pb, const double *const *V, int k, double *w, 
     hipLaunchKernelGGL(k_reduce_final<false>, dim3(1), dim3(256), 0, st, pb->red_partial, nb,
                        k, d2);
   } else {
-    multi_axpy_dev(pb, g, nb, V, k, d1, w, nullptr, st);
-    multi_dot_dev(pb, g, nb, V, k, w, d2, st);
+    multi_axpy_dev(pb, g, nb, V, sc, k, d1, w, nullptr, st);
+    multi_dot_dev(pb, g, nb, V, sc, k, w, d2, st);
   }
-  multi_axpy_dev(pb, g, nb, V, k, d2, w, dn, st);
+  multi_axpy_dev(pb, g, nb, V, sc, k, d2, w, dn, st);
   LAUNCHCHK();
   int rc = fetch_results(pb, PFHIP_MAX_MULTI, stream);
   if (rc) return rc;
   for (int q = 0; q < k; q++) { h1[q] = pb->h_red[q]; h2[q] = pb->h_red[PFHIP_SLOT_H2 + q]; }
   *norm2 = pb->h_red[PFHIP_SLOT_N2];
   return PFHIP_OK;
+}
+
+int pfhip_vec_cgs2(pfhip_problem *pb, const double *const *V, int k, double *w, double *h1,
+                   double *h2, double *norm2, void *stream)
+{
+  return pfhip_vec_cgs2_scaled(pb, V, nullptr, k, w, h1, h2, norm2, stream);
 }
 
 extern "C" int pfhip_vec_multi_axpy(pfhip_problem *pb, const double *const *V,
@@ -1311,12 +1329,9 @@ extern "C" int pfhip_vec_combine_step(pfhip_problem *pb, const double *const *V,
   hipStream_t st = STREAM(stream);
   for (int c0 = 0; c0 < m; c0 += MD_CHUNK) {
     int mc = (m - c0 < MD_CHUNK) ? m - c0 : MD_CHUNK;
-    VecPtrs P;
+    VecPtrs P = vec_ptrs(V, nullptr, c0, mc, x);
     Coefs C;
-    for (int q = 0; q < MD_CHUNK; q++) {
-      P.v[q] = (q < mc) ? V[c0 + q] : x;
-      C.c[q] = (q < mc) ? y[c0 + q] : 0.0;
-    }
+    for (int q = 0; q < MD_CHUNK; q++) C.c[q] = (q < mc) ? y[c0 + q] : 0.0;
     const bool last = (c0 + MD_CHUNK >= m), z = zero && c0 == 0;
     if (last && z) {
 #define L_MA(n) hipLaunchKernelGGL((k_multi_axpy<n, true, true, true>), dim3(nb), dim3(256), 0, st, g, P, C, x, pb->red_partial, nb, u)

@@ -37,6 +37,9 @@ struct pfhip_solver {
   double *unew, *fval, *pp, *vtemp1, *vtemp2, *xcor, *vtemp;
   double *unew_own;             /* unew's allocation (unew and uu exchange) */
   double **V, **Z;
+  double *V0_own;               /* V[0]'s buffer (V[0] may point at fval during a solve) */
+  double *vsc;                  /* flexible variant: V[k] holds w_k unnormalised and
+                                   vsc[k] its factor (1/norm), applied where it is read */
   double *J[7];
   pfhip_mg *mg;
   double *p_old, *s_old;        /* advance_richards state                 */
@@ -76,7 +79,8 @@ const pfhip_ops pfhip_direct_ops = {
   pfhip_residual_jv, pfhip_residual_step, pfhip_mg_setup_jacobian, pfhip_mg_apply,
   pfhip_bc_pressure, pfhip_saturation, pfhip_synchronize, pfhip_residual_jv_dq,
   pfhip_vec_cgs2, pfhip_vec_dot_scsteplength, pfhip_residual_step_stats,
-  pfhip_vec_kinsol_init_norms, pfhip_vec_combine_step,
+  pfhip_vec_kinsol_init_norms, pfhip_vec_combine_step, pfhip_vec_cgs2_scaled,
+  pfhip_mg_apply_scaled,
 };
 
 void pfhip_solver_set_ops(pfhip_solver *s, const pfhip_ops *ops) { s->ops = ops; }
@@ -90,8 +94,9 @@ static int alloc_vec(pfhip_solver *s, double **p)
   return pfhip_zero(*p, s->ng, NULL);
 }
 
-/* the vectors the preconditioner reads and writes (the Krylov basis, the
- * flexible basis Z, xcor and vtemp) start one element past the allocation,
+/* the vectors the preconditioner reads and writes (the Krylov basis, fval,
+ * which stands in for the first basis vector, the flexible basis Z, xcor
+ * and vtemp) start one element past the allocation,
  * so the value pairs of even-i cells (odd ghosted index) are 16-byte aligned
  * and the multigrid moves them with one 16-byte access per lane */
 static int alloc_vec_pc(pfhip_solver *s, double **p)
@@ -110,8 +115,9 @@ static void free_vec_pc(double *p)
 int pfhip_solver_destroy(pfhip_solver *s)
 {
   if (!s) return PFHIP_OK;
-  double *v[7] = { s->unew_own, s->fval, s->pp, s->vtemp1, s->vtemp2, s->p_old, s->s_old };
-  for (int i = 0; i < 7; i++) pfhip_free(v[i]);
+  double *v[6] = { s->unew_own, s->pp, s->vtemp1, s->vtemp2, s->p_old, s->s_old };
+  for (int i = 0; i < 6; i++) pfhip_free(v[i]);
+  free_vec_pc(s->fval);
   free_vec_pc(s->xcor);
   free_vec_pc(s->vtemp);
   if (s->V) for (int i = 0; i <= s->l_max; i++) free_vec_pc(s->V[i]);
@@ -120,7 +126,7 @@ int pfhip_solver_destroy(pfhip_solver *s)
   for (int q = 0; q < 3; q++) pfhip_free(s->vel[q]);
   if (s->mg) pfhip_mg_destroy(s->mg);
   if (s->Hes) for (int i = 0; i <= s->l_max; i++) free(s->Hes[i]);
-  free(s->Hes); free(s->givens); free(s->yg); free(s->hbuf); free(s->Hraw);
+  free(s->Hes); free(s->givens); free(s->yg); free(s->hbuf); free(s->Hraw); free(s->vsc);
   free(s->V); free(s->Z);
   free(s);
   return PFHIP_OK;
@@ -142,14 +148,16 @@ int pfhip_solver_create(pfhip_problem *pb, const pfhip_solver_opts *opts,
   s->l_max = opts->krylov_dim;
   s->max_restarts = opts->max_restarts < 0 ? 0 : opts->max_restarts;
   int rc = PFHIP_OK;
-  double **vv[7] = { &s->unew, &s->fval, &s->pp, &s->vtemp1, &s->vtemp2,
-                     &s->p_old, &s->s_old };
-  for (int i = 0; i < 7 && !rc; i++) rc = alloc_vec(s, vv[i]);
+  double **vv[6] = { &s->unew, &s->pp, &s->vtemp1, &s->vtemp2, &s->p_old, &s->s_old };
+  for (int i = 0; i < 6 && !rc; i++) rc = alloc_vec(s, vv[i]);
+  if (!rc) rc = alloc_vec_pc(s, &s->fval);
   if (!rc) rc = alloc_vec_pc(s, &s->xcor);
   if (!rc) rc = alloc_vec_pc(s, &s->vtemp);
   s->unew_own = s->unew;
   s->V = (double **)calloc(s->l_max + 1, sizeof(double *));
   for (int i = 0; i <= s->l_max && !rc; i++) rc = alloc_vec_pc(s, &s->V[i]);
+  s->V0_own = s->V[0];
+  s->vsc = (double *)calloc(s->l_max + 1, sizeof(double));
   if (opts->krylov == 1) {
     s->Z = (double **)calloc(s->l_max, sizeof(double *));
     for (int i = 0; i < s->l_max && !rc; i++) rc = alloc_vec_pc(s, &s->Z[i]);
@@ -226,6 +234,17 @@ static int psolve(pfhip_solver *s, const double *r, double *z)
   } else {
     rc = s->ops->mg_apply(s->mg, r, z, NULL);
   }
+  PH_END(PFHIP_PROF_PH_PC_SOLVE);
+  return rc;
+}
+
+/* z = M^{-1} (sc * r), r != z: the V-cycle applies the factor where it
+ * reads r, as if r had been scaled first */
+static int psolve_scaled(pfhip_solver *s, const double *r, double sc, double *z)
+{
+  s->nps++;
+  PH_BEGIN(PFHIP_PROF_PH_PC_SOLVE);
+  int rc = s->ops->mg_apply_scaled(s->mg, r, sc, z, NULL);
   PH_END(PFHIP_PROF_PH_PC_SOLVE);
   return rc;
 }
@@ -310,14 +329,18 @@ static int modified_gs_eval(pfhip_solver *s, int k, double *new_vk_norm)
   return PFHIP_OK;
 }
 
-/* CGS2: two passes of block classical Gram-Schmidt against V[0..k-1] */
-static int cgs2_eval(pfhip_solver *s, int k, double *new_vk_norm)
+/* CGS2: two passes of block classical Gram-Schmidt against V[0..k-1]
+ * (scaled: V[q] read as vsc[q] * V[q]) */
+static int cgs2_eval(pfhip_solver *s, int k, double *new_vk_norm, int scaled)
 {
   pfhip_problem *pb = s->pb;
   double *h1 = s->hbuf, *h2 = s->hbuf + (s->l_max + 2);
   int km1 = k - 1;
   double n2;
-  if (s->ops->cgs2 && k <= PFHIP_SLOT_H2) {
+  if (scaled) {
+    CHK(s->ops->cgs2_scaled(pb, (const double *const *)s->V, s->vsc, k, s->V[k], h1, h2, &n2,
+                            NULL));
+  } else if (s->ops->cgs2 && k <= PFHIP_SLOT_H2) {
     CHK(s->ops->cgs2(pb, (const double *const *)s->V, k, s->V[k], h1, h2, &n2, NULL));
   } else {
     CHK(s->ops->vec_multi_dot(pb, (const double *const *)s->V, k, s->V[k], h1, NULL));
@@ -338,10 +361,10 @@ static int modified_gs(pfhip_solver *s, int k, double *new_vk_norm)
   return rc;
 }
 
-static int cgs2(pfhip_solver *s, int k, double *new_vk_norm)
+static int cgs2(pfhip_solver *s, int k, double *new_vk_norm, int scaled)
 {
   PH_BEGIN(PFHIP_PROF_PH_ORTHO);
-  int rc = cgs2_eval(s, k, new_vk_norm);
+  int rc = cgs2_eval(s, k, new_vk_norm, scaled);
   PH_END(PFHIP_PROF_PH_ORTHO);
   return rc;
 }
@@ -397,14 +420,22 @@ static int qrsol(int n, double **h, double *q, double *b)
  * right-hand side is b = -fneg (the Newton system J p = -F): ||b|| is
  * ||fneg|| and V[0] = b / ||b|| is formed in one pass. */
 /* fneg_norm: ||fneg|| when the caller already holds it (computed by the same
- * reduction), else < 0 */
-static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fneg_norm,
-                       double delta, double *res_norm, int *nli, int *ret)
+ * reduction), else < 0.
+ * Flexible variant with a preconditioner: the basis is kept unnormalised,
+ * V[k] = w_k with vsc[k] = 1/||w_k|| beside it (V[0] is fneg itself with
+ * vsc[0] = -1/||fneg||), and every pass that reads V[k] (the V-cycle's
+ * right-hand side, the Gram-Schmidt products) takes vsc[k] * V[k], the
+ * value the scaling pass would have stored: the same bits without that
+ * pass.  A restart first stores the scaled vectors. */
+static int spgmr_solve_body(pfhip_solver *s, double *x, const double *fneg, double fneg_norm,
+                            double delta, double *res_norm, int *nli, int *ret)
 {
   pfhip_problem *pb = s->pb;
-  double **V = s->V, **Hes = s->Hes;
+  double **V = s->V, **Hes = s->Hes, *vsc = s->vsc;
   int l_max = s->l_max, flex = (s->o.krylov == 1);
   int preOnRight = s->o.precond != 0;
+  const int defer = flex && preOnRight && s->ops->cgs2_scaled && s->ops->mg_apply_scaled &&
+                    l_max < PFHIP_SLOT_H2;
   double beta, rotation_product, r_norm, s_product, rho = 0;
   int krydim = 0, converged = 0;
   /* the flexible variant with the fused update leaves x unwritten until
@@ -423,8 +454,15 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fn
     for (int i = 0; i <= l_max; i++)
       for (int j = 0; j < l_max; j++) Hes[i][j] = 0.0;
     rotation_product = 1.0;
-    if (ntries == 0) CHK(s->ops->vec_scale(pb, -1.0 / r_norm, fneg, V[0], NULL));
-    else CHK(s->ops->vec_scale(pb, 1.0 / r_norm, V[0], V[0], NULL));
+    for (int i = 0; i <= l_max; i++) vsc[i] = 1.0;
+    if (ntries == 0 && defer) {
+      V[0] = (double *)fneg;
+      vsc[0] = -1.0 / r_norm;
+    } else if (ntries == 0) {
+      CHK(s->ops->vec_scale(pb, -1.0 / r_norm, fneg, V[0], NULL));
+    } else {
+      CHK(s->ops->vec_scale(pb, 1.0 / r_norm, V[0], V[0], NULL));
+    }
     /* a restart cycle adds Z_c y_c to x: the first cycle's Arnoldi relation
      * no longer gives J x, so lsolve falls back to the difference quotient */
     if (ntries > 0) s->jp_ok = 0;
@@ -435,11 +473,12 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fn
       const double *av = V[l];
       if (preOnRight) {
         double *z = flex ? s->Z[l] : s->vtemp;
-        CHK(psolve(s, V[l], z));
+        if (defer) CHK(psolve_scaled(s, V[l], vsc[l], z));
+        else CHK(psolve(s, V[l], z));
         av = z;
       }
       CHK(atimes(s, av, V[l + 1]));
-      if (flex) CHK(cgs2(s, l + 1, &Hes[l + 1][l]));
+      if (flex) CHK(cgs2(s, l + 1, &Hes[l + 1][l], defer));
       else CHK(modified_gs(s, l + 1, &Hes[l + 1][l]));
       if (flex && ntries == 0)
         for (int i = 0; i <= l + 1; i++) s->Hraw[(size_t)i * l_max + l] = Hes[i][l];
@@ -451,7 +490,8 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fn
       rotation_product *= s->givens[2 * l + 1];
       *res_norm = rho = fabs(rotation_product * r_norm);
       if (rho <= delta) { converged = 1; break; }
-      CHK(s->ops->vec_scale(pb, 1.0 / Hes[l + 1][l], V[l + 1], V[l + 1], NULL));
+      if (defer) vsc[l + 1] = 1.0 / Hes[l + 1][l];
+      else CHK(s->ops->vec_scale(pb, 1.0 / Hes[l + 1][l], V[l + 1], V[l + 1], NULL));
     }
     s->yg[0] = r_norm;
     for (int i = 1; i <= krydim; i++) s->yg[i] = 0.0;
@@ -514,6 +554,18 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fn
     r_norm *= s_product;
     for (int i = 0; i <= krydim; i++) s->yg[i] *= r_norm;
     r_norm = fabs(r_norm);
+    if (defer) {
+      /* store the scaled basis (V[0] back in its own buffer) */
+      for (int k = 0; k <= krydim; k++) {
+        if (k == 0 && V[0] != s->V0_own) {
+          CHK(s->ops->vec_scale(pb, vsc[0], V[0], s->V0_own, NULL));
+          V[0] = s->V0_own;
+        } else if (vsc[k] != 1.0) {
+          CHK(s->ops->vec_scale(pb, vsc[k], V[k], V[k], NULL));
+        }
+        vsc[k] = 1.0;
+      }
+    }
     CHK(s->ops->vec_scale(pb, s->yg[0], V[0], V[0], NULL));
     for (int k = 1; k <= krydim; k++)
       CHK(s->ops->vec_linear_sum(pb, s->yg[k], V[k], 1.0, V[0], V[0], NULL));
@@ -530,6 +582,14 @@ static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fn
   *ret = SPGMR_CONV_FAIL;
   return PFHIP_OK;
 #undef X_ZERO_IF_UNSET
+}
+
+static int spgmr_solve(pfhip_solver *s, double *x, const double *fneg, double fneg_norm,
+                       double delta, double *res_norm, int *nli, int *ret)
+{
+  int rc = spgmr_solve_body(s, x, fneg, fneg_norm, delta, res_norm, nli, ret);
+  s->V[0] = s->V0_own;
+  return rc;
 }
 
 /* KINSpgmrSolve KS/kinspgmr.c:344-420 for J xx = -fval; bb receives J xx;

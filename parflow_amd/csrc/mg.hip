@@ -642,15 +642,27 @@ __device__ __forceinline__ void stg_o2(double *base, unsigned ip, D2 v)
 {
   *(__attribute__((address_space(1))) D2 *)((char *)base + ip * 8u) = v;
 }
+// the right-hand side pair of a lane from the FP64 vector, multiplied by
+// dsc first (the value a scaling pass of the vector by dsc would have
+// stored; dsc = 1 reads it as it is)
 template <typename T, bool A16>
-__device__ __forceinline__ typename V2T<T>::type ld_din(const double *din, unsigned ip)
+__device__ __forceinline__ typename V2T<T>::type ld_din(const double *din, unsigned ip,
+                                                        double dsc)
 {
   typedef typename V2T<T>::type T2;
   if (A16) {
     const D2 v = ldg_o2(din, ip);
-    return T2{(T)v.x, (T)v.y};
+    return T2{(T)(dsc * v.x), (T)(dsc * v.y)};
   }
-  return T2{(T)ldg_o(din, ip), (T)ldg_o(din, ip + 1)};
+  return T2{(T)(dsc * ldg_o(din, ip)), (T)(dsc * ldg_o(din, ip + 1))};
+}
+
+// a double kept in a slot of the interface address table
+__device__ __forceinline__ double io_double(void *const *io, int q)
+{
+  double v;
+  __builtin_memcpy(&v, &io[q], sizeof(double));
+  return v;
 }
 
 template <typename T> struct PairSet {
@@ -662,11 +674,12 @@ template <typename T, bool INIT, bool DIN = false, bool A16 = false>
 __device__ __forceinline__ void load_pair(PairSet<T> &S, const Mat<T> &A,
                                           const T *__restrict__ xin, const T *__restrict__ b,
                                           unsigned ip, unsigned sy,
-                                          const double *__restrict__ din = nullptr)
+                                          const double *__restrict__ din = nullptr,
+                                          double dsc = 1.0)
 {
   typedef typename V2T<T>::type T2;
   S.c = ldo<T2>(A.c, ip);
-  if (DIN) S.bb = ld_din<T, A16>(din, ip);
+  if (DIN) S.bb = ld_din<T, A16>(din, ip, dsc);
   else S.bb = ldo<T2>(b, ip);
   S.ae = ldo<T2>(A.u[0], ip);
   S.an = ldo<T2>(A.u[1], ip);
@@ -698,11 +711,12 @@ __device__ __forceinline__ void load_pair_yl(PairSet<T> &S, const Mat<T> &A,
                                              const T *__restrict__ xin,
                                              const T *__restrict__ b, unsigned ip, unsigned sy,
                                              bool own_s, bool own_n,
-                                             const double *__restrict__ din = nullptr)
+                                             const double *__restrict__ din = nullptr,
+                                             double dsc = 1.0)
 {
   typedef typename V2T<T>::type T2;
   S.c = ldo<T2>(A.c, ip);
-  if (DIN) S.bb = ld_din<T, A16>(din, ip);
+  if (DIN) S.bb = ld_din<T, A16>(din, ip, dsc);
   else S.bb = ldo<T2>(b, ip);
   S.ae = ldo<T2>(A.u[0], ip);
   S.au = ldo<T2>(A.u[2], ip);
@@ -761,13 +775,17 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
                                                               double *__restrict__ dout,
                                                               const double *__restrict__ din,
                                                               T *__restrict__ bout,
-                                                              void *const *io)
+                                                              void *const *io, double dsc)
 {
   typedef typename V2T<T>::type T2;
   static_assert(PF >= 1 && (YL || PF == 1), "prefetch depth > 1 needs the YL form");
-  // io: the FP64 interface vectors' addresses are read from device memory
-  // (a launch sequence captured once and replayed with other vectors)
-  if (DIN && io) din = (const double *)io[0];
+  // io: the FP64 interface vectors' addresses (and the right-hand side's
+  // factor dsc) are read from device memory (a launch sequence captured
+  // once and replayed with other vectors)
+  if (DIN && io) {
+    din = (const double *)io[0];
+    dsc = io_double(io, 2);
+  }
   if (DOUT && io) dout = (double *)io[1];
   __shared__ T red[3][TY + 2][64];
   __shared__ T2 syn[YL ? 2 : 1][YL ? TY + 2 : 1][64];   // north couplings by plane parity
@@ -814,16 +832,16 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
 #pragma unroll
   for (int d = 0; d <= PF; d++) anr[d] = T2{T(0), T(0)};
   if (YL) {
-    load_pair_yl<T, INIT, DIN, A16>(cur, A, xin, b, at(k0), sy, own_s, own_n, din);
+    load_pair_yl<T, INIT, DIN, A16>(cur, A, xin, b, at(k0), sy, own_s, own_n, din, dsc);
     cur.an = ldo<T2>(A.u[1], at(k0));
 #pragma unroll
     for (int d = 0; d + 1 < PF; d++)
       load_pair_yl<T, INIT, DIN, A16>(ahead[d], A, xin, b, at(k0 + 1 + d), sy, own_s, own_n,
-                                      din);
+                                      din, dsc);
 #pragma unroll
     for (int d = 0; d < PF; d++) anr[d] = ldo<T2>(A.u[1], at(k0 + 1 + d));
   } else {
-    load_pair<T, INIT, DIN, A16>(cur, A, xin, b, at(k0), sy, din);
+    load_pair<T, INIT, DIN, A16>(cur, A, xin, b, at(k0), sy, din, dsc);
   }
   T2 al = ldo<T2>(A.u[2], at(k0 - 1));               // lower coupling of plane k0
 #pragma unroll
@@ -841,10 +859,10 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
     // one plane before their own)
     if (YL) {
       load_pair_yl<T, INIT, DIN, A16>(ahead[PF - 1], A, xin, b, at(k + PF), sy, own_s, own_n,
-                                      din);
+                                      din, dsc);
       anr[PF] = ldo<T2>(A.u[1], at(k + PF + 1));
     } else {
-      load_pair<T, INIT, DIN, A16>(ahead[0], A, xin, b, at(k + 1), sy, din);
+      load_pair<T, INIT, DIN, A16>(ahead[0], A, xin, b, at(k + 1), sy, din, dsc);
     }
     xr[PF + 2] = load_x2<T, INIT>(xin, at(k + PF + 1));
     const T2 xm = xr[0], x0 = xr[1], xp = xr[2];
@@ -1153,6 +1171,8 @@ __global__ __launch_bounds__(256) void k_prolong_z(LGeom gf, LGeom gc, const T *
 // k_convert with the FP64 interface vector's address read from io[q] (q = 0:
 // the source is the right-hand side, q = 1: the destination is the result),
 // for launch sequences captured once and replayed with other vectors
+// (q = 0 multiplies the source by the right-hand side's factor in io[2]
+// first, as k_convert_in does)
 template <typename S, typename D>
 __global__ __launch_bounds__(256) void k_convert_io(LGeom g, const S *__restrict__ src,
                                                     D *__restrict__ dst, void *const *io, int q)
@@ -1161,14 +1181,27 @@ __global__ __launch_bounds__(256) void k_convert_io(LGeom g, const S *__restrict
   const S *sp = (q == 0) ? (const S *)io[0] : src;
   D *dp = (q == 1) ? (D *)io[1] : dst;
   long ip = LIDX(g, i, j, k);
-  dp[ip] = (D)sp[ip];
+  dp[ip] = (q == 0) ? (D)(io_double(io, 2) * (double)sp[ip]) : (D)sp[ip];
 }
 
-// the interface addresses of one replay of a captured V-cycle
-__global__ void k_set_io(void **io, const double *b, double *x)
+// the interface addresses of one replay of a captured V-cycle, and the
+// right-hand side's factor
+__global__ void k_set_io(void **io, const double *b, double *x, double bscale)
 {
   io[0] = (void *)b;
   io[1] = (void *)x;
+  __builtin_memcpy(&io[2], &bscale, sizeof(double));
+}
+
+// the V-cycle's right-hand side in the level precision: sc * src, the
+// value a scaling pass of src by sc would have stored, converted
+template <typename T>
+__global__ __launch_bounds__(256) void k_convert_in(LGeom g, const double *__restrict__ src,
+                                                    T *__restrict__ dst, double sc)
+{
+  XCD_IJK(g);
+  long ip = LIDX(g, i, j, k);
+  dst[ip] = (T)(sc * src[ip]);
 }
 
 template <typename S, typename D>
@@ -1499,6 +1532,7 @@ struct pfhip_mg {
   int a16;                     // bit 0 / 1: the V-cycle's FP64 input / result vector has
                                // its even-i points 16-byte aligned (set per call)
   int gx_a16;                  // a16 of the vectors the graph gx was captured for
+  double bscale;               // the V-cycle's right-hand side is bscale * b (set per call)
   int tuned;                   // pv chosen (at the first V-cycle)
   void *levdev;                // device LevelDev<T>[L]
   long lds_bytes;              // > 0: the bottom runs in LDS (k_coarse_vcycle_lds)
@@ -1833,32 +1867,34 @@ static void launch_pair_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout
 template <typename T, int TY, int KC, bool YL, int PF = 1>
 static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
                            int init, hipStream_t st, double *dout, const double *din,
-                           void *const *io, bool a16)
+                           void *const *io, bool a16, double dsc)
 {
   unsigned gx = (g.n[0] + PX2 - 1) / PX2, gy = (g.n[1] + TY - 1) / TY, gz = (g.n[2] + KC - 1) / KC;
   unsigned nb = gx * gy * gz, nb8 = (nb + 7u) & ~7u;
   if (init && din && a16)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true, YL, PF, true>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, din,
-                       (T *)b, io);
+                       (T *)b, io, dsc);
   else if (init && din)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true, YL, PF>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, din,
-                       (T *)b, io);
+                       (T *)b, io, dsc);
   else if (init)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, false, YL, PF>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr,
-                       io);
+                       io, 1.0);
   else if (dout && a16)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true, false, YL, PF, true>), dim3(nb8),
-                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr, io);
+                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr, io,
+                       1.0);
   else if (dout)
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true, false, YL, PF>), dim3(nb8),
-                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr, io);
+                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr, io,
+                       1.0);
   else
     hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, false, false, YL, PF>), dim3(nb8),
                        dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr,
-                       io);
+                       io, 1.0);
 }
 
 // shape v % 5: (TY, KC) = (14, 16), (6, 16), (6, 32), (14, 8), (14, 4);
@@ -1873,33 +1909,33 @@ template <typename T>
 static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
                         int init, int v, hipStream_t st, double *dout = nullptr,
                         const double *din = nullptr, void *const *io = nullptr,
-                        bool a16 = false)
+                        bool a16 = false, double dsc = 1.0)
 {
   if ((g.n[0] % 2) == 0) {
     // two points per lane (aligned vector loads need an even x extent)
     switch (v) {
-      case 1: launch_pair2_t<T, 6, 16, false>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 2: launch_pair2_t<T, 6, 32, false>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 3: launch_pair2_t<T, 14, 8, false>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 4: launch_pair2_t<T, 14, 4, false>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 5: launch_pair2_t<T, 14, 16, true>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 6: launch_pair2_t<T, 6, 16, true>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 7: launch_pair2_t<T, 6, 32, true>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 8: launch_pair2_t<T, 14, 8, true>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 9: launch_pair2_t<T, 14, 4, true>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 1: launch_pair2_t<T, 6, 16, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 2: launch_pair2_t<T, 6, 32, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 3: launch_pair2_t<T, 14, 8, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 4: launch_pair2_t<T, 14, 4, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 5: launch_pair2_t<T, 14, 16, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 6: launch_pair2_t<T, 6, 16, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 7: launch_pair2_t<T, 6, 32, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 8: launch_pair2_t<T, 14, 8, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 9: launch_pair2_t<T, 14, 4, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
       default: break;
     }
     if constexpr (sizeof(T) == 4) switch (v) {
-      case 10: launch_pair2_t<T, 14, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 11: launch_pair2_t<T, 6, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 12: launch_pair2_t<T, 6, 32, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 13: launch_pair2_t<T, 14, 8, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 14: launch_pair2_t<T, 14, 4, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 15: launch_pair2_t<T, 14, 16, true, 3>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
-      case 16: launch_pair2_t<T, 14, 8, true, 3>(g, A, xin, xout, b, init, st, dout, din, io, a16); return;
+      case 10: launch_pair2_t<T, 14, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 11: launch_pair2_t<T, 6, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 12: launch_pair2_t<T, 6, 32, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 13: launch_pair2_t<T, 14, 8, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 14: launch_pair2_t<T, 14, 4, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 15: launch_pair2_t<T, 14, 16, true, 3>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 16: launch_pair2_t<T, 14, 8, true, 3>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
       default: break;
     }
-    launch_pair2_t<T, 14, 16, false>(g, A, xin, xout, b, init, st, dout, din, io, a16);
+    launch_pair2_t<T, 14, 16, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc);
     return;
   }
   switch (v % 5) {
@@ -1944,11 +1980,11 @@ static int tune_pairs(pfhip_mg *mg, hipStream_t st, const double *bin, double *x
       float best = 0.f;
       for (int v = 0; v < npairv<T>(); v++) {
         launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], init, v,
-                       st, dout, din, nullptr, a16);
+                       st, dout, din, nullptr, a16, mg->bscale);
         HIPCHK(hipEventRecord(e0, st));
         for (int r = 0; r < TUNE_REPS; r++)
           launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], init,
-                         v, st, dout, din, nullptr, a16);
+                         v, st, dout, din, nullptr, a16, mg->bscale);
         HIPCHK(hipEventRecord(e1, st));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -2018,7 +2054,7 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
       const int v = (l == 0 && d) ? mg->pv_dout : (l == 0 && di) ? mg->pv_din : mg->pv[l];
       const bool a16 = d ? (mg->a16 & 2) : di ? (mg->a16 & 1) : false;
       launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, v, st, d, di,
-                     io, a16);
+                     io, a16, di ? mg->bscale : 1.0);
       if (d && wrote) *wrote = 1;
       void *tmp = mg->x[l]; mg->x[l] = mg->xa[l]; mg->xa[l] = tmp;
       q += 2;
@@ -2073,8 +2109,8 @@ static int mg_apply_seq(pfhip_mg *mg, const double *bin, double *xout, hipStream
       hipLaunchKernelGGL((k_convert_io<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, bin,
                          (T *)mg->b[0], io, 0);
     else
-      hipLaunchKernelGGL((k_convert<double, T>), dim3(ntiles(g0)), LBLOCK, 0, st, g0, bin,
-                         (T *)mg->b[0]);
+      hipLaunchKernelGGL(k_convert_in<T>, dim3(ntiles(g0)), LBLOCK, 0, st, g0, bin,
+                         (T *)mg->b[0], mg->bscale);
     LAUNCHCHK();
   }
   int l;
@@ -2162,7 +2198,8 @@ static int mg_capture(pfhip_mg *mg)
   for (int l = 0; l < mg->L; l++) { x0[l] = mg->x[l]; xa0[l] = mg->xa[l]; }
   if (!mg->cs) HIPCHK(hipStreamCreateWithFlags(&mg->cs, hipStreamNonBlocking));
   if (!mg->io) {
-    int rc = pfhip_malloc((void **)&mg->io, 2 * (long)sizeof(void *));
+    // the two vector addresses and the right-hand side's factor
+    int rc = pfhip_malloc((void **)&mg->io, 3 * (long)sizeof(void *));
     if (rc) return rc;
   }
   hipGraph_t g = nullptr;
@@ -2203,9 +2240,11 @@ static int mg_capture(pfhip_mg *mg)
 // the captured sequence (PFHIP_MG_GRAPH=0 launches directly always), except
 // while the smoother's per-launch timers are on (they time each launch).
 template <typename T>
-static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t st)
+static int mg_apply_t(pfhip_mg *mg, const double *bin, double bscale, double *xout,
+                      hipStream_t st)
 {
   int rc;
+  mg->bscale = bscale;
   // an even i has an odd ghosted index, so its FP64 value pair is 16-byte
   // aligned when the vector starts 8 bytes past a 16-byte boundary
   mg->a16 = ((((uintptr_t)bin) & 15) == 8 ? 1 : 0) | ((((uintptr_t)xout) & 15) == 8 ? 2 : 0);
@@ -2222,7 +2261,7 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
     if (mg->gx && mg->gx_a16 != mg->a16) return mg_apply_seq<T>(mg, bin, xout, st, nullptr);
     if (!mg->gx && (rc = mg_capture<T>(mg))) return rc;
     if (mg->gx) {
-      hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, st, mg->io, bin, xout);
+      hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, st, mg->io, bin, xout, bscale);
       LAUNCHCHK();
       HIPCHK(hipGraphLaunch(mg->gx, st));
       return PFHIP_OK;
@@ -2233,8 +2272,18 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double *xout, hipStream_t
 
 extern "C" int pfhip_mg_apply(pfhip_mg *mg, const double *b, double *x, void *stream)
 {
-  return (mg->prec == 64) ? mg_apply_t<double>(mg, b, x, STREAM(stream))
-                          : mg_apply_t<float>(mg, b, x, STREAM(stream));
+  return (mg->prec == 64) ? mg_apply_t<double>(mg, b, 1.0, x, STREAM(stream))
+                          : mg_apply_t<float>(mg, b, 1.0, x, STREAM(stream));
+}
+
+// x = M^{-1} (bscale * b): the V-cycle of the vector a scaling pass of b by
+// bscale would have stored, with the factor applied where the V-cycle reads
+// b (a Krylov basis vector kept unnormalised beside its 1/norm)
+extern "C" int pfhip_mg_apply_scaled(pfhip_mg *mg, const double *b, double bscale, double *x,
+                                     void *stream)
+{
+  return (mg->prec == 64) ? mg_apply_t<double>(mg, b, bscale, x, STREAM(stream))
+                          : mg_apply_t<float>(mg, b, bscale, x, STREAM(stream));
 }
 
 // ---- ensemble multigrid ---------------------------------------------------------

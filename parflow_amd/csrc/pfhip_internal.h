@@ -102,6 +102,9 @@ int pfhip_residual_step_stats(pfhip_problem *pb, const double *u, const double *
  * synchronisation: h1, h2 and ||w||^2 after the second pass */
 int pfhip_vec_cgs2(pfhip_problem *pb, const double *const *V, int k, double *w, double *h1,
                    double *h2, double *norm2, void *stream);
+/* the same with basis vector q read as sc[q] * V[q] (sc null: as stored) */
+int pfhip_vec_cgs2_scaled(pfhip_problem *pb, const double *const *V, const double *sc, int k,
+                          double *w, double *h1, double *h2, double *norm2, void *stream);
 
 /* RichardsJacobianEval into double (fp32 = 0) or float (fp32 = 1)
  * coefficient arrays J[7] (C, W, E, S, N, L, U; symm: C, E, N, U) */
@@ -110,6 +113,9 @@ int pfhip_jacobian_into(pfhip_problem *pb, const double *p, double dt, int symm,
 /* multigrid PC setup from the pressure: symmetric Jacobian straight into the
  * finest level, then the coarse operators */
 int pfhip_mg_setup_jacobian(pfhip_mg *mg, const double *p, double dt, void *stream);
+/* x = M^{-1} (bscale * b), the factor applied where the V-cycle reads b */
+int pfhip_mg_apply_scaled(pfhip_mg *mg, const double *b, double bscale, double *x,
+                          void *stream);
 
 /* line-search trial point in one residual launch (no overland problems):
  * unew = uu + rl pp, F = F(unew) */
@@ -178,6 +184,9 @@ typedef struct pfhip_ops {
                            double *, double *, void *);
   int (*vec_combine_step)(pfhip_problem *, const double *const *, int, const double *,
                           double *, const double *, int, double *, double *, void *);
+  int (*cgs2_scaled)(pfhip_problem *, const double *const *, const double *, int, double *,
+                     double *, double *, double *, void *);
+  int (*mg_apply_scaled)(pfhip_mg *, const double *, double, double *, void *);
 } pfhip_ops;
 
 extern const pfhip_ops pfhip_direct_ops;

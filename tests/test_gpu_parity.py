@@ -888,6 +888,79 @@ def test_combine_step_matches_separate_passes(gpu, m):
         assert st.value == np.max(np.abs(ib / (1.0 + np.abs(_I(prob, gp.get(u))))))
 
 
+@pytest.mark.parametrize("k", [1, 5, 9, 20])
+def test_cgs2_scaled_basis_matches_scaled_vectors(gpu, k):
+    """CGS2 with the basis kept unnormalised (vector q read as sc[q] * V[q],
+    pfhip_vec_cgs2_scaled) against CGS2 on the vectors scaled first by
+    pfhip_vec_scale (the flexible GMRES's former normalisation pass): h1,
+    h2, ||w||^2 and w agree bit for bit (k <= 8 takes the fused middle
+    pass, k > 16 two chunks)."""
+    import ctypes as C
+    api = _api()
+    L = api.lib()
+    prob = cases.hetero(37, 21, 11, seed=61, dir_sides=True)
+    print(f"cgs2 scaled k={k}", flush=True)
+    gp = api.GPUProblem(prob)
+    rng = np.random.default_rng(62 + k)
+    raw = [rng.normal(size=prob.shape) * rng.uniform(0.1, 10.0) for _ in range(k)]
+    sc = rng.uniform(-3.0, 3.0, size=k)
+    Vr = [gp.vec(a) for a in raw]
+    Vs = [gp.vec(a) for a in raw]
+    vp, dbl = C.c_void_p, C.c_double
+    L.pfhip_vec_scale.argtypes = [vp, dbl, vp, vp, vp]
+    for q in range(k):
+        api.check(L.pfhip_vec_scale(gp.h, sc[q], Vs[q].ptr, Vs[q].ptr, None), "scale")
+    w0 = rng.normal(size=prob.shape)
+    wa, wb = gp.vec(w0), gp.vec(w0)
+    L.pfhip_vec_cgs2_scaled.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp]
+    L.pfhip_vec_cgs2.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp]
+    pa = (vp * k)(*[v.ptr.value for v in Vr])
+    pb_ = (vp * k)(*[v.ptr.value for v in Vs])
+    sca = (dbl * k)(*sc)
+    h1a, h2a, h1b, h2b = (dbl * k)(), (dbl * k)(), (dbl * k)(), (dbl * k)()
+    na, nb = dbl(), dbl()
+    api.check(L.pfhip_vec_cgs2_scaled(gp.h, pa, sca, k, wa.ptr, h1a, h2a, C.byref(na), None),
+              "cgs2_scaled")
+    api.check(L.pfhip_vec_cgs2(gp.h, pb_, k, wb.ptr, h1b, h2b, C.byref(nb), None), "cgs2")
+    assert list(h1a) == list(h1b) and list(h2a) == list(h2b) and na.value == nb.value
+    assert np.array_equal(_I(prob, gp.get(wa)), _I(prob, gp.get(wb)))
+
+
+@pytest.mark.parametrize("precision,shape", [(32, (70, 40, 24)), (64, (70, 40, 24)),
+                                             (32, (71, 30, 20))])
+def test_mg_apply_scaled_matches_scaled_vector(gpu, precision, shape, monkeypatch):
+    """pfhip_mg_apply_scaled(b, c) applies the factor where the V-cycle reads
+    b (the first pair's FP64 right-hand side, or the conversion pass for odd
+    x extents): the result equals pfhip_mg_apply of the vector scaled by
+    pfhip_vec_scale, bit for bit, from the captured graph (several
+    factors in a row, with 1.0 between) and launching directly."""
+    import ctypes as C
+    api = _api()
+    L = api.lib()
+    prob = cases.hetero(*shape, seed=63, dir_sides=True)
+    print(f"mg apply scaled precision={precision} shape={shape}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    J = O.jacobian_eval(op, cases.random_pressure(prob, seed=64), 0.1, 0.1, 1)
+    vp, dbl = C.c_void_p, C.c_double
+    L.pfhip_mg_apply_scaled.argtypes = [vp, vp, dbl, vp, vp]
+    L.pfhip_vec_scale.argtypes = [vp, dbl, vp, vp, vp]
+    rng = np.random.default_rng(65)
+    for graph in ("1", "0"):
+        monkeypatch.setenv("PFHIP_MG_GRAPH", graph)
+        mg = api.GPUMultigrid(gp, 2, 2, 0, precision)
+        mg.setup(J)
+        for c in (0.37, 1.0, -2.5e-3, 7.0):
+            bh = np.zeros(prob.shape)
+            bh[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+            b, b2, xa, xb = gp.vec(bh), gp.vec(bh), gp.vec(), gp.vec()
+            api.check(L.pfhip_mg_apply_scaled(mg.h, b.ptr, c, xa.ptr, None), "apply_scaled")
+            api.check(L.pfhip_vec_scale(gp.h, c, b2.ptr, b2.ptr, None), "scale")
+            api.check(L.pfhip_mg_apply(mg.h, b2.ptr, xb.ptr, None), "apply")
+            ia, ib = _I(prob, gp.get(xa)), _I(prob, gp.get(xb))
+            assert np.array_equal(ia, ib), (graph, c, _relmax(ia, ib))
+
+
 @pytest.mark.parametrize("overland,dirs", [(False, True), (True, False)])
 def test_residual_multi_tile(gpu, overland, dirs):
     """Residual on a grid spanning several 64-wide tiles, ragged tile rows
