@@ -2994,6 +2994,37 @@ int pfhip_residual_jv(pfhip_problem *pb, const double *u, const double *v, doubl
 // |v|_1) stay in the problem's result slots and the residual kernel forms
 // sigma and 1/sigma from them, so no host synchronisation separates the
 // reduction from the residual.
+// the difference-quotient sums (u.v, v.v, |v|_1) the problem's DQ slots
+// hold, to the host (after the work queued on the stream)
+int pfhip_dq_slots(pfhip_problem *pb, double *out, void *stream)
+{
+  HIPCHK(hipMemcpyAsync(out, pb->red_dev + PFHIP_SLOT_DQ, 3 * sizeof(double),
+                        hipMemcpyDeviceToHost, STREAM(stream)));
+  HIPCHK(hipStreamSynchronize(STREAM(stream)));
+  return PFHIP_OK;
+}
+
+// the difference-quotient J.v with its sums u.v, v.v and |v|_1 already in
+// the problem's DQ slots (gathered by the V-cycle that produced v,
+// pfhip_mg_apply_dq): the residual launch alone
+int pfhip_residual_jv_ready(pfhip_problem *pb, const double *u, const double *v, double sqrf,
+                            const double *fval, double *z, const double *p_old,
+                            const double *s_old, double dt, void *stream)
+{
+  if (pb->overland) {
+    pfhip_set_error("residual_jv: overland problems take the two-pass form");
+    return PFHIP_ERR_ARG;
+  }
+  ResArgs A = res_args(pb, p_old, s_old, dt);
+  A.p = u;
+  A.F = z;
+  A.pv = v; A.fsub = fval;
+  A.dq = pb->red_dev + PFHIP_SLOT_DQ; A.sqrf = sqrf;
+  launch_residual(A, STREAM(stream));
+  LAUNCHCHK();
+  return PFHIP_OK;
+}
+
 int pfhip_residual_jv_dq(pfhip_problem *pb, const double *u, const double *v, double sqrf,
                          const double *fval, double *z, const double *p_old,
                          const double *s_old, double dt, void *stream)

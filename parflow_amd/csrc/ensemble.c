@@ -316,7 +316,7 @@ static const pfhip_ops ens_ops = {
   e_vec_scsteplength, e_vec_dq_sums, e_vec_multi_dot, e_vec_multi_axpy, e_vec_combine,
   e_kin_stop_norms, e_nl_function_eval, e_residual_jv, e_residual_step,
   e_mg_setup_jacobian, e_mg_apply, e_bc_pressure, e_saturation, e_synchronize,
-  NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL,
+  NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL,
 };
 
 /* ---- create / destroy --------------------------------------------------- */

@@ -64,6 +64,9 @@ struct pfhip_solver {
    * fval), when the residual launch that produced them also gathered them */
   int stop_valid;
   double stop_fmax, stop_steplength;
+  /* the vector whose difference-quotient sums (with uu) the preconditioner
+   * left in the problem's DQ slots, for the J.v that follows it */
+  const double *dq_ready;
   /* pp.pp and pp's scaled step length from uu, gathered by the pass that
    * formed pp (the flexible GMRES's solution update) */
   int pp_stats_valid;
@@ -80,7 +83,7 @@ const pfhip_ops pfhip_direct_ops = {
   pfhip_bc_pressure, pfhip_saturation, pfhip_synchronize, pfhip_residual_jv_dq,
   pfhip_vec_cgs2, pfhip_vec_dot_scsteplength, pfhip_residual_step_stats,
   pfhip_vec_kinsol_init_norms, pfhip_vec_combine_step, pfhip_vec_cgs2_scaled,
-  pfhip_mg_apply_scaled,
+  pfhip_mg_apply_scaled, pfhip_mg_apply_dq, pfhip_residual_jv_ready,
 };
 
 void pfhip_solver_set_ops(pfhip_solver *s, const pfhip_ops *ops) { s->ops = ops; }
@@ -239,12 +242,22 @@ static int psolve(pfhip_solver *s, const double *r, double *z)
 }
 
 /* z = M^{-1} (sc * r), r != z: the V-cycle applies the factor where it
- * reads r, as if r had been scaled first */
+ * reads r, as if r had been scaled first.  When z is the next J.v's
+ * direction (the difference quotient, no overland), the V-cycle also
+ * leaves that quotient's sums over uu and z on the device. */
 static int psolve_scaled(pfhip_solver *s, const double *r, double sc, double *z)
 {
   s->nps++;
   PH_BEGIN(PFHIP_PROF_PH_PC_SOLVE);
-  int rc = s->ops->mg_apply_scaled(s->mg, r, sc, z, NULL);
+  int rc;
+  s->dq_ready = NULL;
+  if (s->ops->mg_apply_dq && s->ops->residual_jv_ready && !s->o.use_jacobian &&
+      !s->pb->overland) {
+    rc = s->ops->mg_apply_dq(s->mg, r, sc, z, s->uu, NULL);
+    if (!rc) s->dq_ready = z;
+  } else {
+    rc = s->ops->mg_apply_scaled(s->mg, r, sc, z, NULL);
+  }
   PH_END(PFHIP_PROF_PH_PC_SOLVE);
   return rc;
 }
@@ -259,6 +272,13 @@ static int atimes_eval(pfhip_solver *s, const double *v, double *z)
       s->new_uu = 0;
     }
     return pfhip_matvec(pb, (const double *const *)s->J, 0, v, z, NULL);
+  }
+  if (!pb->overland && s->dq_ready == v && s->ops->residual_jv_ready) {
+    /* the DQ sums came with v from the V-cycle */
+    s->dq_ready = NULL;
+    s->nfe++;
+    return s->ops->residual_jv_ready(pb, s->uu, v, s->sqrt_relfunc, s->fval, z, s->cur_p_old,
+                                     s->cur_s_old, s->dt, NULL);
   }
   if (!pb->overland && s->ops->residual_jv_dq) {
     /* the DQ sums, sigma and the quotient without leaving the device */
@@ -600,6 +620,7 @@ static int lsolve(pfhip_solver *s, double *xx, double *bb, int *lret)
   int nli_inc, sret;
   double res_norm;
   s->pp_stats_valid = 0;
+  s->dq_ready = NULL;
   /* the flexible variant's fused update writes xx without reading it */
   if (!(s->o.krylov == 1 && s->ops->vec_combine_step)) CHK(s->ops->vec_const(pb, 0.0, xx, NULL));
   s->new_uu = 1;

@@ -600,6 +600,12 @@ template <> struct V2T<double> { typedef double type __attribute__((ext_vector_t
 // its own): bpermute lane exchanges within the wavefront
 template <typename T> __device__ __forceinline__ T from_prev(T v) { return __shfl_up(v, 1, 64); }
 template <typename T> __device__ __forceinline__ T from_next(T v) { return __shfl_down(v, 1, 64); }
+// sum over the wavefront's 64 lanes (butterfly; lane 0's value is used)
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 
 // element ip of an array through a 32-bit byte offset from its base (a
 // level array is below 4 GiB: problem creation bounds the finest grid), so
@@ -763,11 +769,16 @@ __device__ __forceinline__ void load_pair_yl(PairSet<T> &S, const Mat<T> &A,
 //
 // A16 (DIN or DOUT): the FP64 vector's even-i points are 16-byte aligned,
 // so a lane reads or writes its two values with one 16-byte access.
+//
+// DQ (with DOUT): the sums the difference-quotient J.v of the result z
+// needs, u.z, z.z and sum |z| over the level's points (u: dqu), as per-
+// workgroup partials dqp[q * nblocks + block] (k_dq_final adds them up in
+// block order); u of plane k is loaded a plane before z of plane k is
+// stored.
 template <typename T, int TY, int KC, bool INIT, bool DOUT = false, bool DIN = false,
-          bool YL = false, int PF = 1, bool A16 = false>
-__global__ __launch_bounds__(64 * (TY + 2), YL ? (PF > 1 ? PAIR2_PF_MIN_WAVES(T)
-                                                         : PAIR2_YL_MIN_WAVES(T))
-                                               : PAIR2_MIN_WAVES(T))
+          bool YL = false, int PF = 1, bool A16 = false, bool DQ = false>
+__global__ __launch_bounds__(64 * (TY + 2), (YL && PF > 1) || DQ ? PAIR2_PF_MIN_WAVES(T)
+                                           : YL ? PAIR2_YL_MIN_WAVES(T) : PAIR2_MIN_WAVES(T))
 void k_rbgs_pair2(LGeom g, Mat<T> A,
                                                               const T *__restrict__ xin,
                                                               T *__restrict__ xout,
@@ -775,10 +786,13 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
                                                               double *__restrict__ dout,
                                                               const double *__restrict__ din,
                                                               T *__restrict__ bout,
-                                                              void *const *io, double dsc)
+                                                              void *const *io, double dsc,
+                                                              const double *__restrict__ dqu,
+                                                              double *__restrict__ dqp)
 {
   typedef typename V2T<T>::type T2;
   static_assert(PF >= 1 && (YL || PF == 1), "prefetch depth > 1 needs the YL form");
+  static_assert(!DQ || DOUT, "the J.v sums come with the FP64 result");
   // io: the FP64 interface vectors' addresses (and the right-hand side's
   // factor dsc) are read from device memory (a launch sequence captured
   // once and replayed with other vectors)
@@ -787,7 +801,9 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
     dsc = io_double(io, 2);
   }
   if (DOUT && io) dout = (double *)io[1];
+  if (DQ && io) dqu = (const double *)io[3];
   __shared__ T red[3][TY + 2][64];
+  __shared__ double dqs[DQ ? 3 * (TY + 2) : 1];
   __shared__ T2 syn[YL ? 2 : 1][YL ? TY + 2 : 1][64];   // north couplings by plane parity
   __shared__ T2 syx[YL && !INIT ? 2 : 1][YL && !INIT ? TY + 2 : 1][64];   // x_in by parity
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -796,7 +812,10 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
   const int nb = gx * gy * gz, nb8 = (nb + 7) & ~7;
   const int bid = blockIdx.x;
   const int Lt = (bid & 7) * (nb8 >> 3) + (bid >> 3);
-  if (Lt >= nb) return;                       // uniform over the workgroup
+  if (Lt >= nb) {                             // uniform over the workgroup
+    if (DQ && tx == 0 && ty == 0) dqp[bid] = dqp[nb8 + bid] = dqp[2 * nb8 + bid] = 0.0;
+    return;
+  }
   const int bx = Lt % gx, rr = Lt / gx;
   const int by = rr % gy, bz = rr / gy;
   const int i0 = bx * PX2 - 2 + 2 * tx;       // even; the lane's points are i0, i0+1
@@ -822,6 +841,9 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
   // carried state
   T sc = T(1), sb = T(0), saw = T(0), sae = T(0), sas = T(0), san = T(0), sal = T(0), sau = T(0);
   T rm1 = T(0), rm2 = T(0);
+  // DQ: u of planes k (uq) and k-1 (up), the lane's running sums
+  double uq0 = 0.0, uq1 = 0.0, up0 = 0.0, up1 = 0.0;
+  double q_uz = 0.0, q_zz = 0.0, q_z1 = 0.0;
   // at the top of the step for plane k: cur = plane k, ahead[d] = plane
   // k+1+d (d < PF-1), anr[d] = north couplings of plane k+1+d (d < PF),
   // xr[d] = x_in of plane k-1+d (d < PF+2); the step loads ahead[PF-1],
@@ -865,6 +887,11 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
       load_pair<T, INIT, DIN, A16>(ahead[0], A, xin, b, at(k + 1), sy, din, dsc);
     }
     xr[PF + 2] = load_x2<T, INIT>(xin, at(k + PF + 1));
+    if (DQ) {
+      // u of plane k, for the sums when z of plane k is stored (next step)
+      uq0 = ldg_o(dqu, ip);
+      uq1 = ldg_o(dqu, ip + 1);
+    }
     const T2 xm = xr[0], x0 = xr[1], xp = xr[2];
     if (DIN && owned && k >= kb && k < ke) sto<T2>(bout, ip, cur.bb);
     if (YL) {
@@ -925,6 +952,15 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
         } else {
           sto<T2>(xout, ip - sz, out);
         }
+        if (DQ) {
+          const double z0 = (double)out.x, z1 = (double)out.y;
+          q_uz += up0 * z0;
+          q_uz += up1 * z1;
+          q_zz += z0 * z0;
+          q_zz += z1 * z1;
+          q_z1 += fabs(z0);
+          q_z1 += fabs(z1);
+        }
       }
     }
     // the black point of plane k keeps its coefficients for the next step
@@ -949,9 +985,91 @@ void k_rbgs_pair2(LGeom g, Mat<T> A,
     }
 #pragma unroll
     for (int d = 0; d < PF + 2; d++) xr[d] = xr[d + 1];
+    if (DQ) { up0 = uq0; up1 = uq1; }
     ip += sz;
   }
+  if (DQ) {
+    // per wavefront (one tile row), then the rows in order
+    const double w_uz = wave_sum_d(q_uz), w_zz = wave_sum_d(q_zz), w_z1 = wave_sum_d(q_z1);
+    if (tx == 0) {
+      dqs[3 * ty] = w_uz;
+      dqs[3 * ty + 1] = w_zz;
+      dqs[3 * ty + 2] = w_z1;
+    }
+    __syncthreads();
+    if (tx == 0 && ty == 0) {
+      double a = 0.0, bq = 0.0, c = 0.0;
+      for (int r = 0; r < TY + 2; r++) {
+        a += dqs[3 * r];
+        bq += dqs[3 * r + 1];
+        c += dqs[3 * r + 2];
+      }
+      dqp[bid] = a;
+      dqp[nb8 + bid] = bq;
+      dqp[2 * nb8 + bid] = c;
+    }
+  }
 }
+
+#define DQ_ROW_BLOCKS 1024
+// the J.v sums u.z, z.z and sum |z| over the level's points as per-block
+// partials dqp[q * gridDim.x + block], for V-cycles whose last pair does
+// not write the FP64 result (io: z and u from the interface table)
+__global__ __launch_bounds__(256) void k_dq_rows(LGeom g, const double *__restrict__ u,
+                                                 const double *__restrict__ z,
+                                                 double *__restrict__ dqp, void *const *io)
+{
+  if (io) {
+    z = (const double *)io[1];
+    u = (const double *)io[3];
+  }
+  __shared__ double part[3][256];
+  double a = 0.0, b = 0.0, c = 0.0;
+  const int rows = g.n[1] * g.n[2];
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int j = r % g.n[1], k = r / g.n[1];
+    for (int i = threadIdx.x; i < g.n[0]; i += blockDim.x) {
+      const long ip = LIDX(g, i, j, k);
+      const double zv = z[ip], uv = u[ip];
+      a += uv * zv;
+      b += zv * zv;
+      c += fabs(zv);
+    }
+  }
+  part[0][threadIdx.x] = a;
+  part[1][threadIdx.x] = b;
+  part[2][threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int q = 0; q < 3; q++) part[q][threadIdx.x] += part[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; q++) dqp[q * gridDim.x + blockIdx.x] = part[q][0];
+}
+
+// the J.v sums of k_rbgs_pair2<..., DQ> (or k_dq_rows): dst[q] = sum over
+// the nb8 block partials dqp[q * nb8 + block], in block order (one
+// workgroup)
+__global__ __launch_bounds__(256) void k_dq_final(const double *__restrict__ dqp, int nb8,
+                                                  double *__restrict__ dst)
+{
+  __shared__ double part[256];
+  for (int q = 0; q < 3; q++) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nb8; i += 256) a += dqp[q * nb8 + i];
+    part[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) dst[q] = part[0];
+    __syncthreads();
+  }
+}
+
 
 // ---- level residual fused with the z restriction ---------------------------
 // For a level coarsened in z with an even x extent: r = b - A x on the fine
@@ -1186,11 +1304,13 @@ __global__ __launch_bounds__(256) void k_convert_io(LGeom g, const S *__restrict
 
 // the interface addresses of one replay of a captured V-cycle, and the
 // right-hand side's factor
-__global__ void k_set_io(void **io, const double *b, double *x, double bscale)
+__global__ void k_set_io(void **io, const double *b, double *x, double bscale,
+                         const double *u)
 {
   io[0] = (void *)b;
   io[1] = (void *)x;
   __builtin_memcpy(&io[2], &bscale, sizeof(double));
+  io[3] = (void *)u;
 }
 
 // the V-cycle's right-hand side in the level precision: sc * src, the
@@ -1533,6 +1653,12 @@ struct pfhip_mg {
                                // its even-i points 16-byte aligned (set per call)
   int gx_a16;                  // a16 of the vectors the graph gx was captured for
   double bscale;               // the V-cycle's right-hand side is bscale * b (set per call)
+  const double *dq_u;          // non-null: also the J.v sums u.x, x.x, |x|_1 of the result
+                               // (into the problem's DQ slots), u = dq_u (set per call)
+  int dq_done;                 // the sequence's last pair gathered them
+  int gx_dq;                   // the graph gx gathers them
+  double *dqp;                 // their per-workgroup partials
+  long dqp_n;                  // partials per sum that dqp holds
   int tuned;                   // pv chosen (at the first V-cycle)
   void *levdev;                // device LevelDev<T>[L]
   long lds_bytes;              // > 0: the bottom runs in LDS (k_coarse_vcycle_lds)
@@ -1720,6 +1846,7 @@ extern "C" int pfhip_mg_destroy(pfhip_mg *mg)
   if (mg->gx) (void)hipGraphExecDestroy(mg->gx);
   if (mg->cs) (void)hipStreamDestroy(mg->cs);
   if (mg->io) pfhip_free(mg->io);
+  if (mg->dqp) pfhip_free(mg->dqp);
   for (int l = 0; l < mg->L; l++) {
     for (int q = 0; q < 13; q++) if (mg->raw[l][q]) hipFree(mg->raw[l][q]);
   }
@@ -1864,37 +1991,29 @@ static void launch_pair_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout
                        A, xin, xout, b);
 }
 
+// returns the launch's workgroup count (the DQ partials it wrote)
 template <typename T, int TY, int KC, bool YL, int PF = 1>
-static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
-                           int init, hipStream_t st, double *dout, const double *din,
-                           void *const *io, bool a16, double dsc)
+static int launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
+                          int init, hipStream_t st, double *dout, const double *din,
+                          void *const *io, bool a16, double dsc, const double *dqu,
+                          double *dqp)
 {
   unsigned gx = (g.n[0] + PX2 - 1) / PX2, gy = (g.n[1] + TY - 1) / TY, gz = (g.n[2] + KC - 1) / KC;
   unsigned nb = gx * gy * gz, nb8 = (nb + 7u) & ~7u;
-  if (init && din && a16)
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true, YL, PF, true>), dim3(nb8),
-                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, din,
-                       (T *)b, io, dsc);
-  else if (init && din)
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, true, YL, PF>), dim3(nb8),
-                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, din,
-                       (T *)b, io, dsc);
-  else if (init)
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, true, false, false, YL, PF>), dim3(nb8),
-                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr,
-                       io, 1.0);
-  else if (dout && a16)
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true, false, YL, PF, true>), dim3(nb8),
-                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr, io,
-                       1.0);
-  else if (dout)
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, true, false, YL, PF>), dim3(nb8),
-                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, dout, nullptr, nullptr, io,
-                       1.0);
-  else
-    hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, false, false, false, YL, PF>), dim3(nb8),
-                       dim3(64, TY + 2, 1), 0, st, g, A, xin, xout, b, nullptr, nullptr, nullptr,
-                       io, 1.0);
+  const dim3 grid(nb8), blk(64, TY + 2, 1);
+#define P2(INIT, DOUT, DIN, A16, DQ, DO, DI, BO, DS)                                          \
+  hipLaunchKernelGGL((k_rbgs_pair2<T, TY, KC, INIT, DOUT, DIN, YL, PF, A16, DQ>), grid, blk, 0, \
+                     st, g, A, xin, xout, b, DO, DI, BO, io, DS, dqu, dqp)
+  if (init && din && a16) P2(true, false, true, true, false, nullptr, din, (T *)b, dsc);
+  else if (init && din) P2(true, false, true, false, false, nullptr, din, (T *)b, dsc);
+  else if (init) P2(true, false, false, false, false, nullptr, nullptr, nullptr, 1.0);
+  else if (dout && dqp && a16) P2(false, true, false, true, true, dout, nullptr, nullptr, 1.0);
+  else if (dout && dqp) P2(false, true, false, false, true, dout, nullptr, nullptr, 1.0);
+  else if (dout && a16) P2(false, true, false, true, false, dout, nullptr, nullptr, 1.0);
+  else if (dout) P2(false, true, false, false, false, dout, nullptr, nullptr, 1.0);
+  else P2(false, false, false, false, false, nullptr, nullptr, nullptr, 1.0);
+#undef P2
+  return (int)nb8;
 }
 
 // shape v % 5: (TY, KC) = (14, 16), (6, 16), (6, 32), (14, 8), (14, 4);
@@ -1904,39 +2023,42 @@ static void launch_pair2_t(const LGeom &g, const Mat<T> &A, const T *xin, T *xou
 // dout (even x extents, non-initial pairs only) takes the result in FP64,
 // din (even x extents, initial pairs only) gives the right-hand side in
 // FP64 and b receives it in the level precision; a16: that FP64 vector's
-// even-i points are 16-byte aligned
+// even-i points are 16-byte aligned; dqp (with dout): the result's J.v
+// partials (k_rbgs_pair2's DQ).  Returns the workgroup count of a
+// two-points-per-lane launch, 0 for the one-point form.
 template <typename T>
-static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
+static int launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, const T *b,
                         int init, int v, hipStream_t st, double *dout = nullptr,
                         const double *din = nullptr, void *const *io = nullptr,
-                        bool a16 = false, double dsc = 1.0)
+                        bool a16 = false, double dsc = 1.0, const double *dqu = nullptr,
+                        double *dqp = nullptr)
 {
   if ((g.n[0] % 2) == 0) {
     // two points per lane (aligned vector loads need an even x extent)
     switch (v) {
-      case 1: launch_pair2_t<T, 6, 16, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 2: launch_pair2_t<T, 6, 32, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 3: launch_pair2_t<T, 14, 8, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 4: launch_pair2_t<T, 14, 4, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 5: launch_pair2_t<T, 14, 16, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 6: launch_pair2_t<T, 6, 16, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 7: launch_pair2_t<T, 6, 32, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 8: launch_pair2_t<T, 14, 8, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 9: launch_pair2_t<T, 14, 4, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 1: return launch_pair2_t<T, 6, 16, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 2: return launch_pair2_t<T, 6, 32, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 3: return launch_pair2_t<T, 14, 8, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 4: return launch_pair2_t<T, 14, 4, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 5: return launch_pair2_t<T, 14, 16, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 6: return launch_pair2_t<T, 6, 16, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 7: return launch_pair2_t<T, 6, 32, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 8: return launch_pair2_t<T, 14, 8, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 9: return launch_pair2_t<T, 14, 4, true>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
       default: break;
     }
     if constexpr (sizeof(T) == 4) switch (v) {
-      case 10: launch_pair2_t<T, 14, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 11: launch_pair2_t<T, 6, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 12: launch_pair2_t<T, 6, 32, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 13: launch_pair2_t<T, 14, 8, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 14: launch_pair2_t<T, 14, 4, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 15: launch_pair2_t<T, 14, 16, true, 3>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
-      case 16: launch_pair2_t<T, 14, 8, true, 3>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc); return;
+      case 10: return launch_pair2_t<T, 14, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 11: return launch_pair2_t<T, 6, 16, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 12: return launch_pair2_t<T, 6, 32, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 13: return launch_pair2_t<T, 14, 8, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 14: return launch_pair2_t<T, 14, 4, true, 2>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 15: return launch_pair2_t<T, 14, 16, true, 3>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
+      case 16: return launch_pair2_t<T, 14, 8, true, 3>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc, dqu, dqp);
       default: break;
     }
-    launch_pair2_t<T, 14, 16, false>(g, A, xin, xout, b, init, st, dout, din, io, a16, dsc);
-    return;
+    return launch_pair2_t<T, 14, 16, false>(g, A, xin, xout, b, init, st, dout, din, io, a16,
+                                            dsc, dqu, dqp);
   }
   switch (v % 5) {
     case 1: launch_pair_t<T, 6, 16>(g, A, xin, xout, b, init, st); break;
@@ -1945,6 +2067,7 @@ static void launch_pair(const LGeom &g, const Mat<T> &A, const T *xin, T *xout, 
     case 4: launch_pair_t<T, 14, 4>(g, A, xin, xout, b, init, st); break;
     default: launch_pair_t<T, 14, 16>(g, A, xin, xout, b, init, st); break;
   }
+  return 0;
 }
 
 // The autotuner, run at the first V-cycle of a hierarchy (so on its real
@@ -1977,14 +2100,18 @@ static int tune_pairs(pfhip_mg *mg, hipStream_t st, const double *bin, double *x
       const double *din = form == 1 ? bin : nullptr;
       int *pick = form == 0 ? &mg->pv[l] : form == 1 ? &mg->pv_din : &mg->pv_dout;
       const bool a16 = form == 1 ? (mg->a16 & 1) : form == 2 ? (mg->a16 & 2) : false;
+      // the result form as the V-cycles will run it: with the J.v sums
+      // when this hierarchy gathers them
+      const double *dqu = form == 2 ? mg->dq_u : nullptr;
+      double *dqp = dqu ? mg->dqp : nullptr;
       float best = 0.f;
       for (int v = 0; v < npairv<T>(); v++) {
         launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], init, v,
-                       st, dout, din, nullptr, a16, mg->bscale);
+                       st, dout, din, nullptr, a16, mg->bscale, dqu, dqp);
         HIPCHK(hipEventRecord(e0, st));
         for (int r = 0; r < TUNE_REPS; r++)
           launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], (const T *)mg->b[l], init,
-                         v, st, dout, din, nullptr, a16, mg->bscale);
+                         v, st, dout, din, nullptr, a16, mg->bscale, dqu, dqp);
         HIPCHK(hipEventRecord(e1, st));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -2039,6 +2166,7 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
     LAUNCHCHK();
   }
   for (int q = 0; q < n;) {
+    int dq_nb = 0;
     const bool pair = mg->fuse && !mg->full && mg->xa[l] && q + 1 < n && seq[q].par == 1 &&
                       seq[q + 1].par == 0;
     double *d = (pair && dout && q + 2 == n && !seq[q].init && (g.n[0] % 2) == 0) ? dout
@@ -2053,8 +2181,12 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
     if (pair) {
       const int v = (l == 0 && d) ? mg->pv_dout : (l == 0 && di) ? mg->pv_din : mg->pv[l];
       const bool a16 = d ? (mg->a16 & 2) : di ? (mg->a16 & 1) : false;
-      launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, v, st, d, di,
-                     io, a16, di ? mg->bscale : 1.0);
+      // the result's J.v sums from the pair that writes it
+      const bool dq = d && mg->dq_u;
+      dq_nb = launch_pair<T>(g, A, (const T *)mg->x[l], (T *)mg->xa[l], b, seq[q].init, v, st,
+                             d, di, io, a16, di ? mg->bscale : 1.0, dq ? mg->dq_u : nullptr,
+                             dq ? mg->dqp : nullptr);
+      if (dq) mg->dq_done = 1;
       if (d && wrote) *wrote = 1;
       void *tmp = mg->x[l]; mg->x[l] = mg->xa[l]; mg->xa[l] = tmp;
       q += 2;
@@ -2069,6 +2201,11 @@ static int rbgs(pfhip_mg *mg, int l, int zero, int max_iter, hipStream_t st,
     }
     if (mark >= 0) pfhip_prof_mark(mark, 1, st);
     LAUNCHCHK();
+    if (dq_nb > 0 && mg->dq_done && d) {
+      hipLaunchKernelGGL(k_dq_final, dim3(1), dim3(256), 0, st, mg->dqp, dq_nb,
+                         mg->pb->red_dev + PFHIP_SLOT_DQ);
+      LAUNCHCHK();
+    }
   }
   return PFHIP_OK;
 }
@@ -2097,6 +2234,7 @@ static int mg_apply_seq(pfhip_mg *mg, const double *bin, double *xout, hipStream
 {
   int L = mg->L, lc = mg->lc, rc, wrote = 0;
   LGeom g0 = lgeom(mg->n[0]);
+  mg->dq_done = 0;
   // the first pre-smoothing pair of level 0 converts b itself when it is an
   // initial two-points-per-lane pair
   HalfSweep seq0[2 * MAX_SWEEPS + 4];
@@ -2181,6 +2319,15 @@ static int mg_apply_seq(pfhip_mg *mg, const double *bin, double *xout, hipStream
                          (const T *)mg->x[0], xout);
     LAUNCHCHK();
   }
+  if (mg->dq_u && !mg->dq_done) {
+    // the J.v sums of the result in a pass of their own
+    const long rows = (long)g0.n[1] * g0.n[2];
+    const int nb = (int)(rows < DQ_ROW_BLOCKS ? rows : DQ_ROW_BLOCKS);
+    hipLaunchKernelGGL(k_dq_rows, dim3(nb), dim3(256), 0, st, g0, mg->dq_u, xout, mg->dqp, io);
+    hipLaunchKernelGGL(k_dq_final, dim3(1), dim3(256), 0, st, mg->dqp, nb,
+                       mg->pb->red_dev + PFHIP_SLOT_DQ);
+    LAUNCHCHK();
+  }
   return PFHIP_OK;
 }
 
@@ -2198,8 +2345,8 @@ static int mg_capture(pfhip_mg *mg)
   for (int l = 0; l < mg->L; l++) { x0[l] = mg->x[l]; xa0[l] = mg->xa[l]; }
   if (!mg->cs) HIPCHK(hipStreamCreateWithFlags(&mg->cs, hipStreamNonBlocking));
   if (!mg->io) {
-    // the two vector addresses and the right-hand side's factor
-    int rc = pfhip_malloc((void **)&mg->io, 3 * (long)sizeof(void *));
+    // the two vector addresses, the right-hand side's factor and u
+    int rc = pfhip_malloc((void **)&mg->io, 4 * (long)sizeof(void *));
     if (rc) return rc;
   }
   hipGraph_t g = nullptr;
@@ -2227,6 +2374,7 @@ static int mg_capture(pfhip_mg *mg)
   e = hipGraphInstantiate(&mg->gx, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   mg->gx_a16 = mg->a16;
+  mg->gx_dq = mg->dq_u != nullptr;
   if (e != hipSuccess) {
     mg->gx = nullptr;
     pfhip_set_error("V-cycle graph instantiation: %s", hipGetErrorString(e));
@@ -2239,12 +2387,33 @@ static int mg_capture(pfhip_mg *mg)
 // first call tunes the smoother and launches directly; later calls replay
 // the captured sequence (PFHIP_MG_GRAPH=0 launches directly always), except
 // while the smoother's per-launch timers are on (they time each launch).
+// J.v partials the level-0 pair launches or k_dq_rows can write: the most
+// workgroups of any tile shape, and DQ_ROW_BLOCKS
+static long dq_partials(const pfhip_mg *mg)
+{
+  const int *n = mg->n[0];
+  const int shapes[5][2] = { { 14, 16 }, { 6, 16 }, { 6, 32 }, { 14, 8 }, { 14, 4 } };
+  long most = DQ_ROW_BLOCKS;
+  for (int v = 0; v < 5; v++) {
+    long nb = (long)((n[0] + PX2 - 1) / PX2) * ((n[1] + shapes[v][0] - 1) / shapes[v][0]) *
+              ((n[2] + shapes[v][1] - 1) / shapes[v][1]);
+    nb = (nb + 7) & ~7L;
+    if (nb > most) most = nb;
+  }
+  return most;
+}
+
 template <typename T>
 static int mg_apply_t(pfhip_mg *mg, const double *bin, double bscale, double *xout,
-                      hipStream_t st)
+                      const double *dq_u, hipStream_t st)
 {
   int rc;
   mg->bscale = bscale;
+  mg->dq_u = dq_u;
+  if (dq_u && !mg->dqp) {
+    mg->dqp_n = dq_partials(mg);
+    if ((rc = pfhip_malloc((void **)&mg->dqp, 3 * mg->dqp_n * (long)sizeof(double)))) return rc;
+  }
   // an even i has an odd ghosted index, so its FP64 value pair is 16-byte
   // aligned when the vector starts 8 bytes past a 16-byte boundary
   mg->a16 = ((((uintptr_t)bin) & 15) == 8 ? 1 : 0) | ((((uintptr_t)xout) & 15) == 8 ? 2 : 0);
@@ -2258,10 +2427,11 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double bscale, double *xo
   if (mg->use_graph && !timed) {
     // the captured launches read the vectors with the access width of the
     // vectors they were captured for: others are launched directly
-    if (mg->gx && mg->gx_a16 != mg->a16) return mg_apply_seq<T>(mg, bin, xout, st, nullptr);
+    if (mg->gx && (mg->gx_a16 != mg->a16 || mg->gx_dq != (dq_u != nullptr)))
+      return mg_apply_seq<T>(mg, bin, xout, st, nullptr);
     if (!mg->gx && (rc = mg_capture<T>(mg))) return rc;
     if (mg->gx) {
-      hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, st, mg->io, bin, xout, bscale);
+      hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, st, mg->io, bin, xout, bscale, dq_u);
       LAUNCHCHK();
       HIPCHK(hipGraphLaunch(mg->gx, st));
       return PFHIP_OK;
@@ -2272,8 +2442,8 @@ static int mg_apply_t(pfhip_mg *mg, const double *bin, double bscale, double *xo
 
 extern "C" int pfhip_mg_apply(pfhip_mg *mg, const double *b, double *x, void *stream)
 {
-  return (mg->prec == 64) ? mg_apply_t<double>(mg, b, 1.0, x, STREAM(stream))
-                          : mg_apply_t<float>(mg, b, 1.0, x, STREAM(stream));
+  return (mg->prec == 64) ? mg_apply_t<double>(mg, b, 1.0, x, nullptr, STREAM(stream))
+                          : mg_apply_t<float>(mg, b, 1.0, x, nullptr, STREAM(stream));
 }
 
 // x = M^{-1} (bscale * b): the V-cycle of the vector a scaling pass of b by
@@ -2282,8 +2452,24 @@ extern "C" int pfhip_mg_apply(pfhip_mg *mg, const double *b, double *x, void *st
 extern "C" int pfhip_mg_apply_scaled(pfhip_mg *mg, const double *b, double bscale, double *x,
                                      void *stream)
 {
-  return (mg->prec == 64) ? mg_apply_t<double>(mg, b, bscale, x, STREAM(stream))
-                          : mg_apply_t<float>(mg, b, bscale, x, STREAM(stream));
+  return (mg->prec == 64) ? mg_apply_t<double>(mg, b, bscale, x, nullptr, STREAM(stream))
+                          : mg_apply_t<float>(mg, b, bscale, x, nullptr, STREAM(stream));
+}
+
+// the same, and the sums the difference-quotient J.v of x needs (u.x, x.x
+// and sum |x| over the grid's cells, u the Newton iterate) into the
+// problem's device slots that pfhip_residual_jv_ready reads; no host
+// synchronisation.  The V-cycle's last pair gathers them as it writes x
+// where it writes x in FP64, a pass of their own otherwise.
+extern "C" int pfhip_mg_apply_dq(pfhip_mg *mg, const double *b, double bscale, double *x,
+                                 const double *u, void *stream)
+{
+  if (!u) {
+    pfhip_set_error("mg_apply_dq: no iterate");
+    return PFHIP_ERR_ARG;
+  }
+  return (mg->prec == 64) ? mg_apply_t<double>(mg, b, bscale, x, u, STREAM(stream))
+                          : mg_apply_t<float>(mg, b, bscale, x, u, STREAM(stream));
 }
 
 // ---- ensemble multigrid ---------------------------------------------------------

@@ -116,6 +116,15 @@ int pfhip_mg_setup_jacobian(pfhip_mg *mg, const double *p, double dt, void *stre
 /* x = M^{-1} (bscale * b), the factor applied where the V-cycle reads b */
 int pfhip_mg_apply_scaled(pfhip_mg *mg, const double *b, double bscale, double *x,
                           void *stream);
+/* the same, and the J.v sums u.x, x.x, |x|_1 into the problem's DQ slots */
+int pfhip_mg_apply_dq(pfhip_mg *mg, const double *b, double bscale, double *x,
+                      const double *u, void *stream);
+/* the DQ slots' three sums, to the host */
+int pfhip_dq_slots(pfhip_problem *pb, double *out, void *stream);
+/* difference-quotient J.v with the sums already in the DQ slots */
+int pfhip_residual_jv_ready(pfhip_problem *pb, const double *u, const double *v, double sqrf,
+                            const double *fval, double *z, const double *p_old,
+                            const double *s_old, double dt, void *stream);
 
 /* line-search trial point in one residual launch (no overland problems):
  * unew = uu + rl pp, F = F(unew) */
@@ -187,6 +196,10 @@ typedef struct pfhip_ops {
   int (*cgs2_scaled)(pfhip_problem *, const double *const *, const double *, int, double *,
                      double *, double *, double *, void *);
   int (*mg_apply_scaled)(pfhip_mg *, const double *, double, double *, void *);
+  int (*mg_apply_dq)(pfhip_mg *, const double *, double, double *, const double *, void *);
+  int (*residual_jv_ready)(pfhip_problem *, const double *, const double *, double,
+                           const double *, double *, const double *, const double *, double,
+                           void *);
 } pfhip_ops;
 
 extern const pfhip_ops pfhip_direct_ops;

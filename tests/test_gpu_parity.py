@@ -961,6 +961,70 @@ def test_mg_apply_scaled_matches_scaled_vector(gpu, precision, shape, monkeypatc
             assert np.array_equal(ia, ib), (graph, c, _relmax(ia, ib))
 
 
+@pytest.mark.parametrize("precision,shape", [(32, (70, 40, 24)), (64, (70, 40, 24)),
+                                             (32, (71, 30, 20)), (32, (128, 30, 37))])
+def test_mg_apply_dq_sums(gpu, precision, shape, monkeypatch):
+    """pfhip_mg_apply_dq: the V-cycle's result is that of
+    pfhip_mg_apply_scaled bit for bit, and the sums it leaves for the
+    difference-quotient J.v (u.x, x.x, sum |x| over the grid's cells,
+    gathered by the last pair where it writes x in FP64, by a pass of their
+    own for odd x extents) agree with numpy's to 1e-13 (summation order
+    differs), from the captured graph and launching directly; the J.v of
+    pfhip_residual_jv_ready with them agrees with pfhip_residual_jv_dq
+    (KS/kinspgmr.c:481-531) to 1e-12."""
+    import ctypes as C
+    api = _api()
+    L = api.lib()
+    prob = cases.hetero(*shape, seed=71, dir_sides=True)
+    print(f"mg apply dq precision={precision} shape={shape}", flush=True)
+    gp = api.GPUProblem(prob)
+    op = O.OracleProblem(prob)
+    p_old = cases.random_pressure(prob, seed=72)
+    s_old = O.saturation(op, p_old)
+    J = O.jacobian_eval(op, p_old, 0.1, 0.1, 1)
+    vp, dbl = C.c_void_p, C.c_double
+    L.pfhip_mg_apply_scaled.argtypes = [vp, vp, dbl, vp, vp]
+    L.pfhip_mg_apply_dq.argtypes = [vp, vp, dbl, vp, vp, vp]
+    L.pfhip_dq_slots.argtypes = [vp, vp, vp]
+    L.pfhip_residual_jv_ready.argtypes = [vp, vp, vp, dbl, vp, vp, vp, vp, dbl, vp]
+    L.pfhip_residual_jv_dq.argtypes = [vp, vp, vp, dbl, vp, vp, vp, vp, dbl, vp]
+    rng = np.random.default_rng(73)
+    gp.bc_pressure(0.3)
+    dpo, dso = gp.vec(p_old), gp.vec(s_old)
+    uh = p_old + rng.normal(0, 0.02, p_old.shape)
+    u = gp.vec(uh)
+    F = gp.vec(rng.normal(size=prob.shape))
+    for graph in ("1", "0"):
+        monkeypatch.setenv("PFHIP_MG_GRAPH", graph)
+        mga = api.GPUMultigrid(gp, 2, 2, 0, precision)
+        mgb = api.GPUMultigrid(gp, 2, 2, 0, precision)
+        mga.setup(J)
+        mgb.setup(J)
+        for rep, c in enumerate((1.0, 0.3, -4.0)):
+            bh = np.zeros(prob.shape)
+            bh[1:-1, 1:-1, 1:-1] = rng.normal(size=(prob.nz, prob.ny, prob.nx))
+            b, xa, xb = gp.vec(bh), gp.vec(), gp.vec()
+            api.check(L.pfhip_mg_apply_dq(mga.h, b.ptr, c, xa.ptr, u.ptr, None), "apply_dq")
+            sums = (dbl * 3)()
+            api.check(L.pfhip_dq_slots(gp.h, sums, None), "dq_slots")
+            api.check(L.pfhip_mg_apply_scaled(mgb.h, b.ptr, c, xb.ptr, None), "apply_scaled")
+            ia, ib = _I(prob, gp.get(xa)), _I(prob, gp.get(xb))
+            assert np.array_equal(ia, ib), (graph, rep, _relmax(ia, ib))
+            ui = _I(prob, uh)
+            ref = (np.sum(ui * ia), np.sum(ia * ia), np.sum(np.abs(ia)))
+            for q in range(3):
+                assert abs(sums[q] - ref[q]) <= 1e-13 * max(abs(ref[q]), 1e-300) + \
+                    1e-13 * np.sum(np.abs(ui * ia)), (graph, rep, q, sums[q], ref[q])
+            # the J.v from the sums left in the slots against the separate pass
+            za, zb = gp.vec(), gp.vec()
+            api.check(L.pfhip_residual_jv_ready(gp.h, u.ptr, xa.ptr, 1.49e-8, F.ptr, za.ptr,
+                                                dpo.ptr, dso.ptr, 0.1, None), "jv_ready")
+            api.check(L.pfhip_residual_jv_dq(gp.h, u.ptr, xa.ptr, 1.49e-8, F.ptr, zb.ptr,
+                                             dpo.ptr, dso.ptr, 0.1, None), "jv_dq")
+            za_, zb_ = _I(prob, gp.get(za)), _I(prob, gp.get(zb))
+            assert _relmax(za_, zb_, np.max(np.abs(zb_)) + 1e-300) < 1e-12, (graph, rep)
+
+
 @pytest.mark.parametrize("overland,dirs", [(False, True), (True, False)])
 def test_residual_multi_tile(gpu, overland, dirs):
     """Residual on a grid spanning several 64-wide tiles, ragged tile rows
