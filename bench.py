@@ -62,10 +62,12 @@ def kernel_bytes_per_cell(gp):
                            written in the multigrid's FP32
       smoother pair (finest level, FP32): C, E, N, U, b, x_in read, x_out
                            written; the V-cycle's last pair writes the FP64
-                           result instead (4 more bytes); its first (zero
-                           initial guess: no x_in) reads C, E, N, U and the
-                           FP64 right-hand side and writes the level's FP32
-                           copy of it and x_out"""
+                           result instead (4 more bytes) and, where the
+                           difference-quotient J.v follows (no overland),
+                           reads the Newton iterate u for the J.v's sums (8
+                           more); its first (zero initial guess: no x_in)
+                           reads C, E, N, U and the FP64 right-hand side and
+                           writes the level's FP32 copy of it and x_out"""
     f = _field_reads(gp)
     res = 8 * (1 + f + 2 + 2 + 1)
     return {
@@ -74,7 +76,7 @@ def kernel_bytes_per_cell(gp):
         "residual_step": res + 16,
         "jacobian": 8 * (1 + f + 2) + 4 * 4,
         "rbgs_pair": 4 * 7,
-        "rbgs_pair_dout": 4 * 6 + 8,
+        "rbgs_pair_dout": 4 * 6 + 8 + (0 if gp.prob.overland else 8),
         "rbgs_pair_din": 4 * 4 + 8 + 4 + 4,
     }
 
